@@ -1,0 +1,129 @@
+/* SPDX-License-Identifier: BSD-3-Clause */
+/*
+ * libtg_hip.so — the C ABI of the MI355X (gfx950) hip backend.
+ *
+ * The reference reaches a GPU through one interface, `ComputeDevice` (src/gpu/device.h:49-64),
+ * opened lazily per backend (src/gpu/device.h:66-124) and driven by the strip dispatcher
+ * `RunKernel` (src/gpu/kernel.h:20-47, src/gpu/kernel.cpp:95-153). This header is that boundary
+ * as plain C: no exceptions and no C++ or torch types cross it. Every call returns 0 on success,
+ * or a nonzero TG_ERR_* code with a message written into `err` (at most `err_len` bytes,
+ * NUL-terminated) when `err` is not null.
+ *
+ * Binary layouts of the kernel structs are in tg_layouts.h.
+ */
+#ifndef TG_HIP_H
+#define TG_HIP_H
+
+#include <stddef.h>
+#include <stdint.h>
+
+#include "tg_layouts.h"
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+enum {
+    TG_OK = 0,
+    TG_ERR_ARGUMENT = 1, /* a null pointer, a bad size or an unknown kernel id */
+    TG_ERR_DEVICE = 2,   /* the HIP runtime reported an error; the message is the runtime's */
+    TG_ERR_NO_DEVICE = 3 /* no GPU at the ordinal */
+};
+
+/* The kernels of the reference the backend runs; one per kernel struct, as the reference keeps one
+ * `KernelProgram` per struct (include/terraingen/gpu/program.h:14-23). */
+enum {
+    TG_KERNEL_NOISE_LAYER = 1, /* NoiseLayerKernel, src/layers/kernels/noise_layer.slang */
+    TG_KERNEL_CRATERS = 2      /* CraterShapeKernel, src/layers/kernels/craters.slang */
+};
+
+typedef struct tg_dev tg_dev;
+
+/* A buffer bound after the kernel struct, as `KernelBuffer` (src/gpu/kernel.h:18-33): host bytes,
+ * the element count, and `per_cell` elements per grid cell for a buffer sliced per strip like the
+ * heights (0 for a buffer bound whole). */
+typedef struct tg_buf {
+    const void* data;
+    size_t bytes;
+    size_t count;
+    size_t per_cell;
+} tg_buf;
+
+/* Opens GPU `ordinal` once per process; later calls return the same device. Replaces
+ * `Device::Open` behind `LazyDevice::Get` (src/gpu/device.h:79-90; CUDA: src/gpu/cuda.cpp).
+ * The device is never destroyed, as the reference's never is. */
+int tg_hip_open(int ordinal, tg_dev** out, char* err, size_t err_len);
+
+/* The device name the runtime reports, as `ComputeDevice::DeviceName` (src/gpu/device.h:54). */
+const char* tg_hip_device_name(const tg_dev* dev);
+
+/* Whether this library has a form of kernel `kernel_id`, as `ComputeDevice::CanRun`
+ * (src/gpu/device.h:57). */
+int tg_hip_can_run(const tg_dev* dev, int kernel_id);
+
+/* A no-op kept for symmetry: the device lives until the process ends (src/gpu/device.h:66-68). */
+int tg_hip_close(tg_dev* dev);
+
+/*
+ * Runs kernel `kernel_id` over one strip of grid rows, synchronously, with host memory in and out:
+ * `ComputeDevice::Run` for the binding `RunKernel` makes (src/gpu/kernel.cpp:95-126):
+ *   heights  params->rows x params->width floats, updated in place (binding 0),
+ *   xs       params->width column offsets from the origin (binding 1),
+ *   ys       params->rows row offsets from the origin (binding 2),
+ *   kernel   the kernel struct, `kernel_bytes` long (binding 3),
+ *   extra    the further buffers in binding order (bindings 4...), for the crater kernel:
+ *            records, data, tiles, entries, frames, points (per cell), lows
+ *            (src/layers/craters/shaping/apply.cpp:181-188).
+ * Device memory is allocated and freed per call, as the reference's devices do
+ * (src/gpu/cuda.cpp:123-135, 297-314): this is the plain per-strip path. Runs are serialized
+ * per device.
+ */
+int tg_hip_run(tg_dev* dev, int kernel_id, const void* kernel, size_t kernel_bytes, float* heights,
+               const float* xs, const float* ys, const tg_kernel_params* params,
+               const tg_buf* extra, size_t n_extra, char* err, size_t err_len);
+
+/* ---- Device-resident form ------------------------------------------------------------------
+ * The same kernels with every buffer already in device memory and launched on a caller's HIP
+ * stream (0 for the device's default stream), without copies or synchronisation: the form the
+ * batched path and the measurements use. A `tg_noise_plan` is a NoiseLayerKernel with its
+ * cell-invariant octave quantities worked out (seeds, offsets, frequencies, turns, weights),
+ * resident in device memory until freed. No counterpart in the reference, which only has the
+ * synchronous form above. */
+typedef struct tg_noise_plan tg_noise_plan;
+
+int tg_hip_noise_plan_create(tg_dev* dev, const tg_noise_layer_kernel* kernel,
+                             tg_noise_plan** out, char* err, size_t err_len);
+int tg_hip_noise_plan_free(tg_noise_plan* plan);
+
+/* heights[r * width + c] += stack(xs[c], ys[r]) for r < params->rows, skipping no-data cells:
+ * noise_layer.slang's main() with device pointers. */
+int tg_hip_noise_launch(tg_dev* dev, const tg_noise_plan* plan, float* heights_dev,
+                        const float* xs_dev, const float* ys_dev, const tg_kernel_params* params,
+                        void* stream, char* err, size_t err_len);
+
+/* ---- Device memory, streams and timing --------------------------------------------------
+ * No counterpart in the reference. For callers without a HIP runtime of their own (the tests,
+ * the measurements, hosts written in other languages): one process should hold one HIP runtime,
+ * so these go through this library's. Streams are created at the default priority. */
+int tg_hip_malloc(tg_dev* dev, size_t bytes, void** out, char* err, size_t err_len);
+int tg_hip_free(tg_dev* dev, void* ptr);
+int tg_hip_memcpy_h2d(tg_dev* dev, void* dst, const void* src, size_t bytes, char* err,
+                      size_t err_len);
+int tg_hip_memcpy_d2h(tg_dev* dev, void* dst, const void* src, size_t bytes, char* err,
+                      size_t err_len);
+int tg_hip_memset(tg_dev* dev, void* dst, int value, size_t bytes, char* err, size_t err_len);
+int tg_hip_stream_create(tg_dev* dev, void** out, char* err, size_t err_len);
+int tg_hip_stream_destroy(tg_dev* dev, void* stream);
+int tg_hip_stream_synchronize(tg_dev* dev, void* stream, char* err, size_t err_len);
+int tg_hip_event_create(tg_dev* dev, void** out, char* err, size_t err_len);
+int tg_hip_event_destroy(tg_dev* dev, void* event);
+int tg_hip_event_record(tg_dev* dev, void* event, void* stream, char* err, size_t err_len);
+/* Milliseconds between two recorded events, waiting for the second. */
+int tg_hip_event_elapsed(tg_dev* dev, void* start, void* stop, float* ms, char* err,
+                         size_t err_len);
+
+#ifdef __cplusplus
+}
+#endif
+
+#endif /* TG_HIP_H */

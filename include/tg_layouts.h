@@ -1,0 +1,195 @@
+/* SPDX-License-Identifier: BSD-3-Clause */
+/*
+ * Binary layouts shared by the host and the gfx950 kernels of the hip backend.
+ *
+ * Each struct is the std430 block the reference's kernel headers define, member for member, so a
+ * host that builds the reference's kernel structs can hand their bytes straight to this library:
+ *   tg_fractal            include/terraingen/noise/fractals.h:29-67       (52 B)
+ *   tg_noise_layer_kernel include/terraingen/layers/kernels/noise_layer.h:19-36 (420 B)
+ *   tg_crater_profile     include/terraingen/layers/kernels/craters.h:29-49  (76 B)
+ *   tg_crater_shape       include/terraingen/layers/kernels/craters.h:52-82  (188 B)
+ *   tg_crater_record      include/terraingen/layers/kernels/craters.h:85-122 (144 B)
+ *   tg_crater_frame       include/terraingen/layers/kernels/craters.h:132-148 (52 B)
+ *   tg_split_point        include/terraingen/layers/kernels/craters.h:188-195 (24 B)
+ *   tg_kernel_params      src/gpu/kernel.cpp:34-40, src/gpu/kernel.slang:9-19 (20 B)
+ * A C++ `bool` member of the reference occupies one 32-bit word here; its padding bytes must be
+ * zero, as the reference's GPU conformance tests require (tests/unit/gpu/noise_test.cpp:65-90).
+ */
+#ifndef TG_LAYOUTS_H
+#define TG_LAYOUTS_H
+
+#include <stdint.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+enum { TG_MAX_FRACTALS = 8, TG_MAX_OCTAVES = 32 };
+
+/* NoiseBasis, FractalType and FractalBlend (include/terraingen/noise/fractal.h:20-40,
+ * include/terraingen/noise/fractals.h:12-18). */
+enum {
+    TG_BASIS_PERLIN = 0,
+    TG_BASIS_SIMPLEX = 1,
+    TG_BASIS_VALUE = 2,
+    TG_BASIS_WORLEY_F1 = 3,
+    TG_BASIS_WORLEY_F2 = 4,
+    TG_BASIS_WORLEY_F2_MINUS_F1 = 5,
+    TG_BASIS_VORONOI = 6
+};
+enum { TG_FRACTAL_FBM = 0, TG_FRACTAL_RIDGED = 1, TG_FRACTAL_BILLOW = 2, TG_FRACTAL_HYBRID = 3 };
+enum { TG_BLEND_ADD = 0, TG_BLEND_MAX = 1, TG_BLEND_MIN = 2, TG_BLEND_MASK = 3 };
+
+typedef struct tg_fractal {
+    uint32_t enabled; /* the reference's bool, zero-padded to a word */
+    uint32_t seed;
+    float wavelength;
+    float amplitude;
+    int32_t octaves;
+    float lacunarity;
+    float gain;
+    uint32_t basis;
+    uint32_t type;
+    float offset;
+    uint32_t blend;
+    float warp_amplitude;
+    float warp_wavelength;
+} tg_fractal;
+
+typedef struct tg_noise_layer_kernel {
+    tg_fractal fractals[TG_MAX_FRACTALS];
+    uint32_t count;
+} tg_noise_layer_kernel;
+
+/* CraterProfileKind (craters.h:16-23). */
+enum { TG_PROFILE_FASSETT_THOMSON = 0, TG_PROFILE_MPD = 1, TG_PROFILE_TABULATED = 2 };
+
+typedef struct tg_crater_profile {
+    uint32_t simple;
+    uint32_t rim;
+    uint32_t complex_;
+    uint32_t diffused;
+    float transition_diameter;
+    float floor_coefficient;
+    float floor_exponent;
+    float peak_height_coefficient;
+    float peak_height_exponent;
+    float peak_width_coefficient;
+    float peak_width_exponent;
+    uint32_t peak_from_volume;
+    float peak_fit_max_diameter;
+    float ejecta_exponent;
+    float complex_reach_radii;
+    float table_extent;
+    uint32_t table_cells;
+    uint32_t ages_offset;
+    uint32_t age_count;
+} tg_crater_profile;
+
+typedef struct tg_crater_shape {
+    tg_crater_profile profile;
+    float reach_radii;
+    float profile_reach_radii;
+    float blend_start;
+    float blend_end;
+    uint32_t blend_shape;
+    float interior_replacement;
+    float replace_start;
+    float replace_end;
+    float mantling;
+    float ejecta_reach_radii;
+    float rim_sigma;
+    float rim_inner;
+    float rim_outer;
+    float rim_strength;
+    uint32_t rim_shape;
+    float noise_amplitude;
+    float noise_wavelength;
+    int32_t noise_octaves;
+    float noise_gain;
+    float noise_rim_amplitude;
+    float noise_rim_wavelength;
+    float noise_rim_inner;
+    float noise_rim_outer;
+    float wall_streaks;
+    float wall_streak_width;
+    float ejecta;
+    float ejecta_lineation;
+    float min_freshness;
+} tg_crater_shape;
+
+typedef struct tg_crater_record {
+    float x;
+    float y;
+    float diameter;
+    float resolution;
+    float fresh_depth;
+    float fresh_rim;
+    float current_depth;
+    float current_rim;
+    float diffusion;
+    float ellipticity;
+    float heading;
+    uint32_t first_harmonic;
+    uint32_t harmonic_count;
+    uint32_t harmonic_offset;
+    uint32_t table_count;
+    uint32_t table_offset;
+    uint32_t profile_offset;
+    uint32_t has_datum;
+    float datum_a;
+    float datum_b;
+    float datum_c;
+    uint32_t noise_key;
+    uint32_t texture_key;
+    uint32_t mark_count;
+    uint32_t mark_offset;
+    float shift_x;
+    float shift_y;
+    float upslope_x;
+    float upslope_y;
+    uint32_t terrace_count;
+    uint32_t terrace_offset;
+    uint32_t peak_count;
+    uint32_t peak_offset;
+    uint32_t replace;
+    float x_low;
+    float y_low;
+} tg_crater_record;
+
+typedef struct tg_crater_frame {
+    float x, y, z;
+    float east_x, east_y, east_z;
+    float north_x, north_y, north_z;
+    float sphere_radius;
+    float x_low, y_low, z_low;
+} tg_crater_frame;
+
+typedef struct tg_split_point {
+    float x, x_low, y, y_low, z, z_low;
+} tg_split_point;
+
+/* No-data modes of tg_kernel_params.no_data (src/gpu/kernel.cpp:22-27). */
+enum { TG_NODATA_NONE = 0, TG_NODATA_NAN = 1, TG_NODATA_VALUE = 2 };
+
+typedef struct tg_kernel_params {
+    uint32_t width;
+    uint32_t rows;
+    uint32_t no_data;
+    float no_data_value;
+    uint32_t first_row;
+} tg_kernel_params;
+
+#ifdef __cplusplus
+}
+static_assert(sizeof(tg_fractal) == 52, "tg_fractal must match Fractal's std430 layout");
+static_assert(sizeof(tg_noise_layer_kernel) == 420, "tg_noise_layer_kernel must be 420 bytes");
+static_assert(sizeof(tg_crater_profile) == 76, "tg_crater_profile must be 76 bytes");
+static_assert(sizeof(tg_crater_shape) == 188, "tg_crater_shape must be 188 bytes");
+static_assert(sizeof(tg_crater_record) == 144, "tg_crater_record must be 144 bytes");
+static_assert(sizeof(tg_crater_frame) == 52, "tg_crater_frame must be 52 bytes");
+static_assert(sizeof(tg_split_point) == 24, "tg_split_point must be 24 bytes");
+static_assert(sizeof(tg_kernel_params) == 20, "tg_kernel_params must be 20 bytes");
+#endif
+
+#endif /* TG_LAYOUTS_H */

@@ -1,0 +1,69 @@
+# SPDX-License-Identifier: BSD-3-Clause
+"""ORACLE - test infrastructure only.
+
+ctypes loaders for ``liboracle.so`` (this repository's CPU restatement of the reference, in the
+reference-equivalent FP32 form and as an FP64 oracle) and ``_ref/libtgref.so`` (the reference's
+own code compiled from its sources). Only ``tests/``, ``__graft_entry__.smoke()`` and
+``bench.py``'s CPU-baseline leg may import this package; the hip backend never does.
+"""
+from __future__ import annotations
+
+import ctypes as C
+from pathlib import Path
+
+HERE = Path(__file__).resolve().parent
+ORACLE_SO = HERE / "liboracle.so"
+REF_SO = HERE / "_ref" / "libtgref.so"
+
+_oracle = None
+_ref = None
+
+
+def oracle() -> C.CDLL:
+    global _oracle
+    if _oracle is None:
+        if not ORACLE_SO.exists():
+            raise RuntimeError(f"{ORACLE_SO} is missing: run `make -C oracle`")
+        _oracle = C.CDLL(str(ORACLE_SO))
+        _declare(_oracle)
+    return _oracle
+
+
+def reference() -> C.CDLL | None:
+    """The reference's own code, or None where it was not built (the GPU hosts have only what was
+    built here)."""
+    global _ref
+    if _ref is None and REF_SO.exists():
+        _ref = C.CDLL(str(REF_SO))
+        _declare(_ref)
+    return _ref
+
+
+def _declare(lib: C.CDLL) -> None:
+    from terraingen_amd import native as N
+
+    f, d, u, sz, vp = C.c_float, C.c_double, C.c_uint32, C.c_size_t, C.c_void_p
+    fp = C.POINTER(N.Fractal)
+    sigs = {
+        "or_pcg_hash": (u, [u]), "ref_pcg_hash": (u, [u]),
+        "or_pcg3d": (None, [vp, vp]), "ref_pcg3d": (None, [vp, vp]),
+        "or_hash_to_unit_float": (f, [u]),
+        "or_perlin2": (f, [f, f, u]), "or_simplex2": (f, [f, f, u]), "or_value2": (f, [f, f, u]),
+        "or_worley2": (None, [f, f, u, vp]),
+        "or_basis2": (f, [u, f, f, u]), "ref_basis2": (f, [u, f, f, u]),
+        "or_fractal2": (f, [f, f, fp]), "ref_fractal2": (f, [f, f, fp]),
+        "or_fractal2_f64": (d, [d, d, fp]),
+        "or_eval_fractals": (f, [f, f, fp, sz]), "ref_eval_fractals": (f, [f, f, fp, sz]),
+        "or_eval_fractals_f64": (d, [d, d, fp, sz]),
+        "ref_make_noise_layer_kernel": (None, [fp, sz, vp]),
+        "or_noise_grid": (None, [C.POINTER(N.NoiseLayerKernel), vp, vp, sz, sz,
+                                 C.POINTER(N.KernelParams), vp, C.c_uint]),
+        "or_noise_grid_f64": (None, [C.POINTER(N.NoiseLayerKernel), vp, vp, sz, sz, vp,
+                                     C.c_uint]),
+        "ref_noise_grid": (None, [vp, vp, vp, sz, sz, vp]),
+    }
+    for name, (res, args) in sigs.items():
+        fn = getattr(lib, name, None)
+        if fn is not None:
+            fn.restype = res
+            fn.argtypes = args

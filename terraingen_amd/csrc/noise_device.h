@@ -1,0 +1,335 @@
+// SPDX-License-Identifier: BSD-3-Clause
+//
+// 2D noise for gfx950, one lane per cell, in VALU. Every floating-point operation is the
+// reference's, in its order, and the library is built with -ffp-contract=off, so a cell's noise
+// has the same bits as the reference's CPU backend (include/terraingen/noise/{hash,lattice,
+// perlin,simplex,value,worley,fractal,fractals}.h). Cell-invariant octave quantities come from a
+// NoisePlan (noise_plan.h).
+//
+// The Perlin corner hash is the x word of pcg3d(x, y, seed) (perlin.h:31-33, hash.h:40-57). Its
+// first LCG step is linear in the corner, so the four corners of a cell share work exactly in
+// 32-bit wrap-around arithmetic: lcg(x + 1) = lcg(x) + 1664525, the seed's step is per octave
+// (NoisePlan), and the first cross product y * z depends only on the corner's row.
+#pragma once
+
+#include "noise_plan.h"
+
+#include <hip/hip_runtime.h>
+
+namespace tg {
+
+constexpr uint32_t kLcgMul = 1664525u;
+constexpr uint32_t kLcgAdd = 1013904223u;
+
+struct U3 {
+    uint32_t x, y, z;
+};
+
+__device__ __forceinline__ uint32_t Pcg(uint32_t v) {
+    const uint32_t state = v * 747796405u + 2891336453u;
+    const uint32_t word = ((state >> ((state >> 28u) + 4u)) ^ state) * 277803737u;
+    return (word >> 22u) ^ word;
+}
+
+__device__ __forceinline__ U3 Pcg3(U3 v) {
+    v.x = v.x * kLcgMul + kLcgAdd;
+    v.y = v.y * kLcgMul + kLcgAdd;
+    v.z = v.z * kLcgMul + kLcgAdd;
+    v.x += v.y * v.z;
+    v.y += v.z * v.x;
+    v.z += v.x * v.y;
+    v.x ^= v.x >> 16u;
+    v.y ^= v.y >> 16u;
+    v.z ^= v.z >> 16u;
+    v.x += v.y * v.z;
+    v.y += v.z * v.x;
+    v.z += v.x * v.y;
+    return v;
+}
+
+__device__ __forceinline__ float UnitFloat(uint32_t h) {
+    return float(h >> 8u) * 0x1p-24f;
+}
+
+// std::max, std::min and std::clamp as the reference's C++ evaluates them, so NaN and signed zeros
+// come out as on the CPU (fmaxf/fminf may pick either zero).
+__device__ __forceinline__ float StdMax(float a, float b) {
+    return a < b ? b : a;
+}
+__device__ __forceinline__ float StdMin(float a, float b) {
+    return b < a ? b : a;
+}
+__device__ __forceinline__ float StdClamp(float v, float lo, float hi) {
+    return v < lo ? lo : (hi < v ? hi : v);
+}
+__device__ __forceinline__ float Clamp01(float v) {
+    return StdClamp(v, 0.0f, 1.0f);
+}
+
+// The x word of pcg3d from the corner's LCG'd x and y, the seed's LCG'd word and yz = y' * z'.
+__device__ __forceinline__ uint32_t CornerX(uint32_t ax, uint32_t ay, uint32_t az, uint32_t yz) {
+    uint32_t vx = ax + yz;
+    uint32_t vy = ay + az * vx;
+    uint32_t vz = az + vx * vy;
+    vx ^= vx >> 16u;
+    vy ^= vy >> 16u;
+    vz ^= vz >> 16u;
+    return vx + vy * vz;
+}
+
+// PerlinGradient2 (perlin.h:17-26).
+__device__ __forceinline__ float Grad2(uint32_t hash, float x, float y) {
+    const float kInvSqrt2 = 0.707106781186547524f;
+    const uint32_t h = hash & 7u;
+    const float axis = (h & 2u) != 0u ? y : x;
+    const float sx = h == 5u ? -x : x;
+    const float sy = h == 6u ? -y : y;
+    const float sum = sx + sy;
+    const float diagonal = (h == 7u ? -sum : sum) * kInvSqrt2;
+    return h < 4u ? ((h & 1u) != 0u ? -axis : axis) : diagonal;
+}
+
+__device__ __forceinline__ float Fade(float t) {
+    return t * t * t * (t * (t * 6.0f - 15.0f) + 10.0f);
+}
+
+__device__ __forceinline__ float Mix(float t, float a, float b) {
+    return a + t * (b - a);
+}
+
+// PerlinNoise2 at lattice point (x, y) (perlin.h:72-98) with the octave's seed and LCG'd seed.
+__device__ __forceinline__ float Perlin2(float x, float y, uint32_t seed, uint32_t seedLcg) {
+    (void)seed;
+    const float cx = floorf(x);
+    const float cy = floorf(y);
+    const float fx = x - cx;
+    const float fy = y - cy;
+    const uint32_t ix = uint32_t(int32_t(cx));
+    const uint32_t iy = uint32_t(int32_t(cy));
+    const uint32_t ax0 = ix * kLcgMul + kLcgAdd;
+    const uint32_t ax1 = ax0 + kLcgMul;
+    const uint32_t ay0 = iy * kLcgMul + kLcgAdd;
+    const uint32_t ay1 = ay0 + kLcgMul;
+    const uint32_t yz0 = ay0 * seedLcg;
+    const uint32_t yz1 = ay1 * seedLcg;
+    const float n00 = Grad2(CornerX(ax0, ay0, seedLcg, yz0), fx, fy);
+    const float n10 = Grad2(CornerX(ax1, ay0, seedLcg, yz0), fx - 1.0f, fy);
+    const float n01 = Grad2(CornerX(ax0, ay1, seedLcg, yz1), fx, fy - 1.0f);
+    const float n11 = Grad2(CornerX(ax1, ay1, seedLcg, yz1), fx - 1.0f, fy - 1.0f);
+    const float u = Fade(fx);
+    const float v = Fade(fy);
+    return 1.41421356237309505f * Mix(v, Mix(u, n00, n10), Mix(u, n01, n11));
+}
+
+// SimplexNoise2 (simplex.h:18-59).
+__device__ __forceinline__ float SimplexCorner(uint32_t ci, uint32_t cj, float dx, float dy,
+                                               uint32_t seed) {
+    const float t = 0.5f - dx * dx - dy * dy;
+    const float t2 = t * t;
+    const float corner = t2 * t2 * Grad2(Pcg3({ci, cj, seed}).x, dx, dy);
+    return !(t <= 0.0f) ? corner : 0.0f;
+}
+
+__device__ __forceinline__ float Simplex2(float x, float y, uint32_t seed) {
+    const float kSkew = 0.366025403784438647f;
+    const float kUnskew = 0.211324865405187118f;
+    const float skew = (x + y) * kSkew;
+    const float cellX = floorf(x + skew);
+    const float cellY = floorf(y + skew);
+    const float unskew = (cellX + cellY) * kUnskew;
+    const float x0 = x - (cellX - unskew);
+    const float y0 = y - (cellY - unskew);
+    const uint32_t stepX = x0 > y0 ? 1u : 0u;
+    const uint32_t stepY = 1u - stepX;
+    const float x1 = x0 - float(stepX) + kUnskew;
+    const float y1 = y0 - float(stepY) + kUnskew;
+    const float x2 = x0 - 1.0f + 2.0f * kUnskew;
+    const float y2 = y0 - 1.0f + 2.0f * kUnskew;
+    const uint32_t i = uint32_t(int32_t(cellX));
+    const uint32_t j = uint32_t(int32_t(cellY));
+    return 99.0f * (SimplexCorner(i, j, x0, y0, seed) +
+                    SimplexCorner(i + stepX, j + stepY, x1, y1, seed) +
+                    SimplexCorner(i + 1u, j + 1u, x2, y2, seed));
+}
+
+// ValueNoise2 (value.h:19-45).
+__device__ __forceinline__ float LatticeValue(uint32_t x, uint32_t y, uint32_t seed) {
+    return 2.0f * UnitFloat(Pcg3({x, y, seed}).y) - 1.0f;
+}
+
+__device__ __forceinline__ float Value2(float x, float y, uint32_t seed) {
+    const float cx = floorf(x);
+    const float cy = floorf(y);
+    const float fx = x - cx;
+    const float fy = y - cy;
+    const uint32_t x0 = uint32_t(int32_t(cx));
+    const uint32_t y0 = uint32_t(int32_t(cy));
+    const float v00 = LatticeValue(x0, y0, seed);
+    const float v10 = LatticeValue(x0 + 1u, y0, seed);
+    const float v01 = LatticeValue(x0, y0 + 1u, seed);
+    const float v11 = LatticeValue(x0 + 1u, y0 + 1u, seed);
+    const float u = Fade(fx);
+    const float v = Fade(fy);
+    return Mix(v, Mix(u, v00, v10), Mix(u, v01, v11));
+}
+
+struct Worley {
+    float f1, f2, value;
+};
+
+// Worley2 (worley.h:38-63).
+__device__ __forceinline__ Worley Worley2(float x, float y, uint32_t seed) {
+    const float cx = floorf(x);
+    const float cy = floorf(y);
+    const float fx = x - cx;
+    const float fy = y - cy;
+    const int32_t ix = int32_t(cx);
+    const int32_t iy = int32_t(cy);
+    float d1 = 1e30f;
+    float d2 = 1e30f;
+    uint32_t nearest = 0u;
+    for (int32_t dy = -2; dy <= 2; ++dy) {
+        for (int32_t dx = -2; dx <= 2; ++dx) {
+            const U3 h = Pcg3({uint32_t(ix + dx), uint32_t(iy + dy), seed});
+            const float px = float(dx) + UnitFloat(h.y) - fx;
+            const float py = float(dy) + UnitFloat(h.z) - fy;
+            const float d = px * px + py * py;
+            if (d < d1) {
+                d2 = d1;
+                d1 = d;
+                nearest = h.x;
+            } else if (d < d2) {
+                d2 = d;
+            }
+        }
+    }
+    return {sqrtf(d1), sqrtf(d2), 2.0f * UnitFloat(nearest) - 1.0f};
+}
+
+// BasisNoise2 (fractal.h:89-111).
+__device__ __forceinline__ float Basis2(uint32_t basis, float x, float y, const OctavePlan& q) {
+    const float kInvSqrt2 = 0.707106781186547524f;
+    const float kInvSqrt5 = 0.447213595499957939f;
+    switch (basis) {
+    case TG_BASIS_SIMPLEX:
+        return Simplex2(x, y, q.seed);
+    case TG_BASIS_VALUE:
+        return Value2(x, y, q.seed);
+    case TG_BASIS_WORLEY_F1:
+        return 2.0f * StdMin(Worley2(x, y, q.seed).f1 * kInvSqrt2, 1.0f) - 1.0f;
+    case TG_BASIS_WORLEY_F2:
+        return 2.0f * StdMin(Worley2(x, y, q.seed).f2 * kInvSqrt5, 1.0f) - 1.0f;
+    case TG_BASIS_WORLEY_F2_MINUS_F1: {
+        const Worley w = Worley2(x, y, q.seed);
+        return 2.0f * StdMin((w.f2 - w.f1) * kInvSqrt5, 1.0f) - 1.0f;
+    }
+    case TG_BASIS_VORONOI:
+        return Worley2(x, y, q.seed).value;
+    default:
+        return Perlin2(x, y, q.seed, q.seedLcg);
+    }
+}
+
+// Fractal2 (fractal.h:219-239) over a planned octave loop; AddFractalOctave and FinishFractal
+// (fractal.h:164-211) with the weights read from the plan.
+__device__ __forceinline__ float FractalLoop(const FractalLoopPlan& p, float x, float y) {
+    const uint32_t type = p.type;
+    const uint32_t basis = p.basis;
+    const uint32_t octaves = p.octaves;
+    float sum = 0.0f;
+    float feedback = 1.0f;
+    if (type == TG_FRACTAL_FBM && basis == TG_BASIS_PERLIN) {
+        for (uint32_t o = 0; o < octaves; ++o) {
+            const OctavePlan& q = p.octave[o];
+            const float rx = x * q.turnCos - y * q.turnSin;
+            const float ry = x * q.turnSin + y * q.turnCos;
+            const float n = Perlin2(rx * q.frequency + q.offsetX, ry * q.frequency + q.offsetY,
+                                    q.seed, q.seedLcg);
+            sum += q.weight * n;
+        }
+        return p.amplitude * sum / p.totalWeight;
+    }
+    for (uint32_t o = 0; o < octaves; ++o) {
+        const OctavePlan& q = p.octave[o];
+        const float rx = x * q.turnCos - y * q.turnSin;
+        const float ry = x * q.turnSin + y * q.turnCos;
+        const float n = Basis2(basis, rx * q.frequency + q.offsetX, ry * q.frequency + q.offsetY, q);
+        const float weight = q.weight;
+        switch (type) {
+        case TG_FRACTAL_BILLOW:
+            sum += weight * (2.0f * fabsf(n) - 1.0f);
+            break;
+        case TG_FRACTAL_RIDGED: {
+            float signal = p.offset - fabsf(n);
+            signal *= signal * feedback;
+            feedback = Clamp01(2.0f * signal);
+            sum += weight * signal;
+            break;
+        }
+        case TG_FRACTAL_HYBRID: {
+            const float signal = (n + p.offset) * weight;
+            if (o == 0) {
+                sum = signal;
+                feedback = signal;
+            } else {
+                feedback = Clamp01(feedback);
+                sum += feedback * signal;
+                feedback *= signal;
+            }
+            break;
+        }
+        default:
+            sum += weight * n;
+            break;
+        }
+    }
+    switch (type) {
+    case TG_FRACTAL_RIDGED: {
+        const float ridge = StdMax(p.offset, 1.0f - p.offset);
+        return p.amplitude * (2.0f * sum / (ridge * ridge * p.totalWeight) - 1.0f);
+    }
+    case TG_FRACTAL_HYBRID:
+        return p.amplitude * sum / ((1.0f + p.offset) * p.totalWeight);
+    default:
+        return p.amplitude * sum / p.totalWeight;
+    }
+}
+
+// EvaluateFractals (fractals.h:106-135,151-196): the stack's height at (x, y).
+__device__ __forceinline__ float EvaluateStack(const NoisePlan& plan, float x, float y) {
+    float height = 0.0f;
+    const uint32_t count = plan.count;
+    for (uint32_t i = 0; i < count; ++i) {
+        const FractalPlan& f = plan.fractal[i];
+        float value;
+        if (f.warped) {
+            const float dx = FractalLoop(f.warpX, x, y);
+            const float dy = FractalLoop(f.warpY, x, y);
+            value = FractalLoop(f.unit, x + dx, y + dy);
+        } else {
+            value = FractalLoop(f.unit, x, y);
+        }
+        const float scaled = f.amplitude * value;
+        if (i == 0) {
+            height = scaled;
+            continue;
+        }
+        switch (f.blend) {
+        case TG_BLEND_MAX:
+            height = StdMax(height, scaled);
+            break;
+        case TG_BLEND_MIN:
+            height = StdMin(height, scaled);
+            break;
+        case TG_BLEND_MASK:
+            height *= 0.5f * (value + 1.0f);
+            break;
+        default:
+            height += scaled;
+            break;
+        }
+    }
+    return height;
+}
+
+}  // namespace tg

@@ -1,0 +1,167 @@
+// SPDX-License-Identifier: BSD-3-Clause
+//
+// The noise plan: a fractal stack with every cell-invariant quantity of its octave loops worked
+// out once on the host, in FP32 and in the reference's order of operations, so kernels read them
+// instead of recomputing them per cell. For each octave the reference derives, per cell
+// (include/terraingen/noise/fractal.h:219-239):
+//   seed, offsetX, offsetY  from Pcg3dHash(OctaveKey(seed, octave, 0))          (hash.h:40-62)
+//   frequency               1 / wavelength times lacunarity per octave (FP32)
+//   turnCos, turnSin        the golden-angle recurrence in FP32 (fractal.h:142-144,234-236)
+//   weight, totalWeight     gain^i and their running sum in FP32 (fractal.h:194-196)
+// Each is a pure function of the fractal's parameters, so the plan holds the same bits the
+// reference computes in every cell. Warp fields (fractals.h:98-104) get their own octave tables.
+#pragma once
+
+#include "tg_layouts.h"
+
+#include <cstdint>
+
+namespace tg {
+
+// One octave: 8 words.
+struct OctavePlan {
+    uint32_t seed;      // the octave's lattice seed, Pcg3dHash(key).x
+    uint32_t seedLcg;   // seed * 1664525 + 1013904223, the first step of the corner hash
+    float offsetX;      // HashToUnitFloat(Pcg3dHash(key).y)
+    float offsetY;      // HashToUnitFloat(Pcg3dHash(key).z)
+    float frequency;
+    float turnCos;
+    float turnSin;
+    float weight;       // gain^octave as the FP32 running product
+};
+static_assert(sizeof(OctavePlan) == 32, "OctavePlan is 8 words");
+
+// The octave loop of one fractal (its own or a warp field's).
+struct FractalLoopPlan {
+    uint32_t octaves;
+    uint32_t basis;
+    uint32_t type;
+    float amplitude;    // FinishFractal's amplitude (1 for a stack member's unit value)
+    float offset;       // ridge offset
+    float totalWeight;  // sum of the weights, accumulated in octave order in FP32
+    float gain;
+    uint32_t pad;
+    OctavePlan octave[TG_MAX_OCTAVES];
+};
+
+// One enabled fractal of a stack.
+struct FractalPlan {
+    uint32_t blend;
+    float amplitude;      // scales the unit value into meters
+    uint32_t warped;      // 1 when warpAmplitude > 0
+    uint32_t pad;
+    FractalLoopPlan unit;   // the fractal at amplitude 1
+    FractalLoopPlan warpX;  // first warp field
+    FractalLoopPlan warpY;  // second warp field
+};
+
+struct NoisePlan {
+    uint32_t count;  // enabled fractals, at most TG_MAX_FRACTALS
+    uint32_t maxOctaves;
+    uint32_t allPerlinFbm;  // 1 when every loop is Perlin fBm without warp (the fast path)
+    uint32_t pad;
+    FractalPlan fractal[TG_MAX_FRACTALS];
+};
+
+// Pcg3dHash of (seed, octave, 0): include/terraingen/noise/hash.h:40-57, lattice.h:126-129.
+inline void HostPcg3(uint32_t v[3]) {
+    for (int i = 0; i < 3; ++i) {
+        v[i] = v[i] * 1664525u + 1013904223u;
+    }
+    v[0] += v[1] * v[2];
+    v[1] += v[2] * v[0];
+    v[2] += v[0] * v[1];
+    for (int i = 0; i < 3; ++i) {
+        v[i] ^= v[i] >> 16u;
+    }
+    v[0] += v[1] * v[2];
+    v[1] += v[2] * v[0];
+    v[2] += v[0] * v[1];
+}
+
+inline uint32_t HostPcg(uint32_t v) {
+    const uint32_t state = v * 747796405u + 2891336453u;
+    const uint32_t word = ((state >> ((state >> 28u) + 4u)) ^ state) * 277803737u;
+    return (word >> 22u) ^ word;
+}
+
+// Fills `loop` for a fractal; must be built with FP contraction off, as the reference is.
+inline void PlanLoop(FractalLoopPlan& loop, uint32_t seed, float wavelength, float amplitude,
+                     int octaves, float lacunarity, float gain, uint32_t basis, uint32_t type,
+                     float offset) {
+    const float kTurnCos = -0.7373688780783197f;
+    const float kTurnSin = 0.6754902942615238f;
+    loop.octaves = uint32_t(octaves < 0 ? 0 : (octaves > TG_MAX_OCTAVES ? TG_MAX_OCTAVES : octaves));
+    loop.basis = basis;
+    loop.type = type;
+    loop.amplitude = amplitude;
+    loop.offset = offset;
+    loop.gain = gain;
+    loop.pad = 0;
+    float frequency = 1.0f / wavelength;
+    float c = 1.0f, s = 0.0f, weight = 1.0f, total = 0.0f;
+    for (uint32_t o = 0; o < loop.octaves; ++o) {
+        uint32_t h[3] = {seed, o, 0u};
+        HostPcg3(h);
+        OctavePlan& q = loop.octave[o];
+        q.seed = h[0];
+        q.seedLcg = h[0] * 1664525u + 1013904223u;
+        q.offsetX = float(h[1] >> 8u) * 0x1p-24f;
+        q.offsetY = float(h[2] >> 8u) * 0x1p-24f;
+        q.frequency = frequency;
+        q.turnCos = c;
+        q.turnSin = s;
+        q.weight = weight;
+        total += weight;
+        weight *= gain;
+        frequency *= lacunarity;
+        const float nc = c * kTurnCos - s * kTurnSin;
+        s = s * kTurnCos + c * kTurnSin;
+        c = nc;
+    }
+    loop.totalWeight = total;
+}
+
+// The plan of a noise-layer kernel: its enabled fractals in order (noise_layer.h:41-50 keeps only
+// enabled ones; a disabled one leaves the stack unchanged, so skipping it is exact).
+inline void MakeNoisePlan(const tg_noise_layer_kernel& k, NoisePlan& plan) {
+    plan.count = 0;
+    plan.maxOctaves = 0;
+    plan.allPerlinFbm = 1;
+    plan.pad = 0;
+    const uint32_t n = k.count < TG_MAX_FRACTALS ? k.count : TG_MAX_FRACTALS;
+    for (uint32_t i = 0; i < n; ++i) {
+        const tg_fractal& f = k.fractals[i];
+        if (!f.enabled) {
+            continue;
+        }
+        FractalPlan& p = plan.fractal[plan.count++];
+        p.blend = f.blend;
+        p.amplitude = f.amplitude;
+        p.warped = f.warp_amplitude > 0.0f ? 1u : 0u;
+        p.pad = 0;
+        PlanLoop(p.unit, f.seed, f.wavelength, 1.0f, f.octaves, f.lacunarity, f.gain, f.basis,
+                 f.type, f.offset);
+        if (p.warped) {
+            // WarpParams: 4-octave Perlin fBm, lacunarity 2, gain 0.5 (fractals.h:98-104).
+            const uint32_t seedX = HostPcg(f.seed ^ 0x5bd1e995u);
+            const uint32_t seedY = HostPcg(seedX);
+            PlanLoop(p.warpX, seedX, f.warp_wavelength, f.warp_amplitude, 4, 2.0f, 0.5f,
+                     TG_BASIS_PERLIN, TG_FRACTAL_FBM, 1.0f);
+            PlanLoop(p.warpY, seedY, f.warp_wavelength, f.warp_amplitude, 4, 2.0f, 0.5f,
+                     TG_BASIS_PERLIN, TG_FRACTAL_FBM, 1.0f);
+            plan.allPerlinFbm = 0;
+        } else {
+            p.warpX.octaves = 0;
+            p.warpY.octaves = 0;
+        }
+        if (p.unit.basis != TG_BASIS_PERLIN || p.unit.type != TG_FRACTAL_FBM) {
+            plan.allPerlinFbm = 0;
+        }
+        if (p.unit.octaves > plan.maxOctaves) {
+            plan.maxOctaves = p.unit.octaves;
+        }
+    }
+}
+
+}  // namespace tg

@@ -1,0 +1,523 @@
+// SPDX-License-Identifier: BSD-3-Clause
+//
+// libtg_hip.so: the gfx950 kernels of the hip backend and their C ABI (include/tg_hip.h).
+// Built with -ffp-contract=off, as the reference is (its CMakeLists.txt:32-38), so FP32 results
+// depend only on the order of operations written here.
+
+#include "tg_hip.h"
+
+#include "crater_device.h"
+#include "noise_device.h"
+
+#include <hip/hip_runtime.h>
+
+#include <cmath>
+#include <cstdio>
+#include <cstring>
+#include <memory>
+#include <mutex>
+#include <string>
+#include <vector>
+
+struct tg_dev {
+    int ordinal = 0;
+    std::string name;
+    std::mutex mutex;  // runs are serialized, as ComputeDevice requires (src/gpu/device.h:48)
+};
+
+struct tg_noise_plan {
+    tg_dev* dev = nullptr;
+    tg::NoisePlan* device = nullptr;
+    tg::NoisePlan host{};
+};
+
+namespace {
+
+void SetErr(char* err, size_t len, const std::string& message) {
+    if (err != nullptr && len > 0) {
+        std::snprintf(err, len, "%s", message.c_str());
+    }
+}
+
+struct HipError {
+    std::string message;
+};
+
+void Check(hipError_t status, const char* what) {
+    if (status != hipSuccess) {
+        throw HipError{std::string(what) + ": " + hipGetErrorString(status)};
+    }
+}
+
+// Device memory freed on scope exit.
+template <class T> struct DeviceArray {
+    T* ptr = nullptr;
+    DeviceArray() = default;
+    explicit DeviceArray(size_t count) {
+        if (count > 0) {
+            Check(hipMalloc(reinterpret_cast<void**>(&ptr), count * sizeof(T)), "hipMalloc");
+        }
+    }
+    DeviceArray(const DeviceArray&) = delete;
+    DeviceArray& operator=(const DeviceArray&) = delete;
+    ~DeviceArray() {
+        if (ptr != nullptr) {
+            (void)hipFree(ptr);
+        }
+    }
+    void Upload(const void* src, size_t bytes) {
+        if (bytes > 0) {
+            Check(hipMemcpy(ptr, src, bytes, hipMemcpyHostToDevice), "hipMemcpy to device");
+        }
+    }
+};
+
+__device__ __forceinline__ bool IsNoData(const tg_kernel_params& p, float height) {
+    if (p.no_data == TG_NODATA_NAN) {
+        return isnan(height);
+    }
+    return p.no_data != TG_NODATA_NONE && height == p.no_data_value;
+}
+
+// Cells per block: 64 columns (one wave's lanes along a row, so stores coalesce) by 4 rows.
+constexpr uint32_t kNoiseBlockX = 64;
+constexpr uint32_t kNoiseBlockY = 4;
+
+// noise_layer.slang's main(): heights[i] = height + EvaluateFractals(xs[col], ys[row], stack).
+__global__ void __launch_bounds__(kNoiseBlockX * kNoiseBlockY)
+    NoiseLayerStrip(float* __restrict__ heights, const float* __restrict__ xs,
+                    const float* __restrict__ ys, const tg::NoisePlan* __restrict__ plan,
+                    tg_kernel_params params) {
+    const uint32_t col = blockIdx.x * kNoiseBlockX + threadIdx.x;
+    const uint32_t row = blockIdx.y * kNoiseBlockY + threadIdx.y;
+    if (col >= params.width || row >= params.rows) {
+        return;
+    }
+    const size_t i = size_t(row) * params.width + col;
+    const float height = heights[i];
+    if (IsNoData(params, height)) {
+        return;
+    }
+    heights[i] = height + tg::EvaluateStack(*plan, xs[col], ys[row]);
+}
+
+void LaunchNoise(const tg::NoisePlan* plan, float* heights, const float* xs, const float* ys,
+                 const tg_kernel_params& params, hipStream_t stream) {
+    if (params.width == 0 || params.rows == 0) {
+        return;
+    }
+    const dim3 block(kNoiseBlockX, kNoiseBlockY, 1);
+    const dim3 grid((params.width + kNoiseBlockX - 1) / kNoiseBlockX,
+                    (params.rows + kNoiseBlockY - 1) / kNoiseBlockY, 1);
+    if (grid.y > 65535u * 64u) {
+        throw HipError{"strip has too many rows for one launch"};
+    }
+    hipLaunchKernelGGL(NoiseLayerStrip, grid, block, 0, stream, heights, xs, ys, plan, params);
+    Check(hipGetLastError(), "launching the noise kernel");
+}
+
+// craters.slang's main(): each cell runs the crater list of its tile.
+constexpr uint32_t kCraterTile = 16;
+
+__global__ void __launch_bounds__(kCraterTile * kCraterTile)
+    CraterStrip(float* __restrict__ heights, const float* __restrict__ xs,
+                const float* __restrict__ ys, const tg_crater_shape* __restrict__ kernel,
+                const tg_crater_record* __restrict__ records, const float* __restrict__ data,
+                const uint32_t* __restrict__ tiles, const uint32_t* __restrict__ entries,
+                const tg_crater_frame* __restrict__ frames,
+                const tg_split_point* __restrict__ points, const float* __restrict__ lows,
+                tg_kernel_params params) {
+    const uint32_t col = blockIdx.x * kCraterTile + threadIdx.x;
+    const uint32_t localRow = blockIdx.y * kCraterTile + threadIdx.y;
+    if (col >= params.width || localRow >= params.rows) {
+        return;
+    }
+    const size_t i = size_t(localRow) * params.width + col;
+    const float height = heights[i];
+    if (IsNoData(params, height)) {
+        return;
+    }
+    const uint32_t row = params.first_row + localRow;
+    const uint32_t side = tiles[1];
+    const uint32_t head = 3u + 2u * ((row / side) * tiles[0] + col / side);
+    const uint32_t first = tiles[head];
+    const uint32_t count = tiles[head + 1];
+    const tg::CraterShape shape(*kernel);
+    float h = height;
+    if (tiles[2] == 0u) {
+        tg_split_point p;
+        p.x = xs[col];
+        p.x_low = lows[col];
+        p.y = ys[localRow];
+        p.y_low = lows[params.width + row];
+        p.z = 0.0f;
+        p.z_low = 0.0f;
+        for (uint32_t e = first; e < first + count; ++e) {
+            const uint32_t n = entries[e];
+            const tg_crater_record& c = records[n];
+            const float dx = tg::SplitDifference(p.x, p.x_low, c.x, c.x_low);
+            const float dy = tg::SplitDifference(p.y, p.y_low, c.y, c.y_low);
+            h = tg::ApplyCrater(shape, data, c, h, dx, dy);
+        }
+    } else {
+        const tg_split_point p = points[i];
+        for (uint32_t e = first; e < first + count; ++e) {
+            const uint32_t n = entries[e];
+            const tg::Offset at = tg::BodySplitOffset(frames[n], p);
+            h = tg::ApplyCrater(shape, data, records[n], h, at.dx, at.dy);
+        }
+    }
+    heights[i] = h;
+}
+
+template <class F> int Guard(char* err, size_t errLen, F body) {
+    try {
+        body();
+        return TG_OK;
+    } catch (const HipError& e) {
+        SetErr(err, errLen, e.message);
+        return TG_ERR_DEVICE;
+    } catch (const std::bad_alloc&) {
+        SetErr(err, errLen, "host allocation failed");
+        return TG_ERR_DEVICE;
+    }
+}
+
+}  // namespace
+
+extern "C" {
+
+int tg_hip_open(int ordinal, tg_dev** out, char* err, size_t errLen) {
+    static std::mutex openMutex;
+    static std::vector<std::unique_ptr<tg_dev>> devices;
+    if (out == nullptr) {
+        SetErr(err, errLen, "tg_hip_open: null output");
+        return TG_ERR_ARGUMENT;
+    }
+    *out = nullptr;
+    int count = 0;
+    const hipError_t status = hipGetDeviceCount(&count);
+    if (status != hipSuccess || count == 0) {
+        SetErr(err, errLen,
+               status != hipSuccess ? std::string("no HIP device: ") + hipGetErrorString(status)
+                                    : std::string("no HIP device"));
+        return TG_ERR_NO_DEVICE;
+    }
+    if (ordinal < 0 || ordinal >= count) {
+        SetErr(err, errLen, "HIP device ordinal " + std::to_string(ordinal) + " out of range");
+        return TG_ERR_NO_DEVICE;
+    }
+    std::lock_guard<std::mutex> lock(openMutex);
+    for (const auto& d : devices) {
+        if (d->ordinal == ordinal) {
+            *out = d.get();
+            return TG_OK;
+        }
+    }
+    hipDeviceProp_t prop;
+    if (hipGetDeviceProperties(&prop, ordinal) != hipSuccess) {
+        SetErr(err, errLen, "hipGetDeviceProperties failed");
+        return TG_ERR_DEVICE;
+    }
+    auto dev = std::make_unique<tg_dev>();
+    dev->ordinal = ordinal;
+    dev->name = std::string(prop.name) + " (" + prop.gcnArchName + ")";
+    *out = dev.get();
+    devices.push_back(std::move(dev));
+    return TG_OK;
+}
+
+const char* tg_hip_device_name(const tg_dev* dev) {
+    return dev != nullptr ? dev->name.c_str() : "";
+}
+
+int tg_hip_can_run(const tg_dev* dev, int kernelId) {
+    return dev != nullptr && (kernelId == TG_KERNEL_NOISE_LAYER || kernelId == TG_KERNEL_CRATERS);
+}
+
+int tg_hip_close(tg_dev* dev) {
+    (void)dev;
+    return TG_OK;
+}
+
+int tg_hip_run(tg_dev* dev, int kernelId, const void* kernel, size_t kernelBytes, float* heights,
+               const float* xs, const float* ys, const tg_kernel_params* params,
+               const tg_buf* extra, size_t nExtra, char* err, size_t errLen) {
+    if (dev == nullptr || kernel == nullptr || params == nullptr) {
+        SetErr(err, errLen, "tg_hip_run: null argument");
+        return TG_ERR_ARGUMENT;
+    }
+    const size_t width = params->width;
+    const size_t rows = params->rows;
+    const size_t cells = width * rows;
+    if (cells > 0 && (heights == nullptr || xs == nullptr || ys == nullptr)) {
+        SetErr(err, errLen, "tg_hip_run: null grid buffer");
+        return TG_ERR_ARGUMENT;
+    }
+    if (kernelId == TG_KERNEL_NOISE_LAYER) {
+        if (kernelBytes != sizeof(tg_noise_layer_kernel)) {
+            SetErr(err, errLen, "tg_hip_run: a NoiseLayerKernel is 420 bytes");
+            return TG_ERR_ARGUMENT;
+        }
+    } else if (kernelId == TG_KERNEL_CRATERS) {
+        if (kernelBytes != sizeof(tg_crater_shape)) {
+            SetErr(err, errLen, "tg_hip_run: a CraterShapeKernel is 188 bytes");
+            return TG_ERR_ARGUMENT;
+        }
+        if (nExtra != 7 || extra == nullptr) {
+            SetErr(err, errLen,
+                   "tg_hip_run: the crater kernel binds records, data, tiles, entries, frames, "
+                   "points and lows");
+            return TG_ERR_ARGUMENT;
+        }
+    } else {
+        SetErr(err, errLen, "tg_hip_run: unknown kernel id " + std::to_string(kernelId));
+        return TG_ERR_ARGUMENT;
+    }
+    if (cells == 0) {
+        return TG_OK;
+    }
+    std::lock_guard<std::mutex> lock(dev->mutex);
+    return Guard(err, errLen, [&] {
+        Check(hipSetDevice(dev->ordinal), "hipSetDevice");
+        DeviceArray<float> dHeights(cells);
+        DeviceArray<float> dXs(width);
+        DeviceArray<float> dYs(rows);
+        dHeights.Upload(heights, cells * sizeof(float));
+        dXs.Upload(xs, width * sizeof(float));
+        dYs.Upload(ys, rows * sizeof(float));
+        if (kernelId == TG_KERNEL_NOISE_LAYER) {
+            tg_noise_layer_kernel k;
+            std::memcpy(&k, kernel, sizeof k);
+            auto plan = std::make_unique<tg::NoisePlan>();
+            tg::MakeNoisePlan(k, *plan);
+            DeviceArray<tg::NoisePlan> dPlan(1);
+            dPlan.Upload(plan.get(), sizeof(tg::NoisePlan));
+            LaunchNoise(dPlan.ptr, dHeights.ptr, dXs.ptr, dYs.ptr, *params, nullptr);
+        } else {
+            // Binding order of craters.slang: records, data, tiles, entries, frames, points, lows.
+            std::vector<std::unique_ptr<DeviceArray<unsigned char>>> bound;
+            for (size_t b = 0; b < nExtra; ++b) {
+                const tg_buf& buf = extra[b];
+                size_t bytes = buf.bytes;
+                const unsigned char* src = static_cast<const unsigned char*>(buf.data);
+                if (buf.per_cell != 0 && buf.count > 0) {
+                    // A per-cell buffer is bound one strip at a time by the caller already.
+                    bytes = buf.bytes;
+                }
+                auto d = std::make_unique<DeviceArray<unsigned char>>(bytes > 0 ? bytes : 4);
+                if (bytes > 0 && src != nullptr) {
+                    d->Upload(src, bytes);
+                }
+                bound.push_back(std::move(d));
+            }
+            DeviceArray<tg_crater_shape> dKernel(1);
+            dKernel.Upload(kernel, sizeof(tg_crater_shape));
+            const dim3 block(kCraterTile, kCraterTile, 1);
+            const dim3 grid(uint32_t((width + kCraterTile - 1) / kCraterTile),
+                            uint32_t((rows + kCraterTile - 1) / kCraterTile), 1);
+            hipLaunchKernelGGL(CraterStrip, grid, block, 0, nullptr, dHeights.ptr, dXs.ptr,
+                               dYs.ptr, dKernel.ptr,
+                               reinterpret_cast<const tg_crater_record*>(bound[0]->ptr),
+                               reinterpret_cast<const float*>(bound[1]->ptr),
+                               reinterpret_cast<const uint32_t*>(bound[2]->ptr),
+                               reinterpret_cast<const uint32_t*>(bound[3]->ptr),
+                               reinterpret_cast<const tg_crater_frame*>(bound[4]->ptr),
+                               reinterpret_cast<const tg_split_point*>(bound[5]->ptr),
+                               reinterpret_cast<const float*>(bound[6]->ptr), *params);
+            Check(hipGetLastError(), "launching the crater kernel");
+        }
+        Check(hipDeviceSynchronize(), "running the kernel");
+        Check(hipMemcpy(heights, dHeights.ptr, cells * sizeof(float), hipMemcpyDeviceToHost),
+              "hipMemcpy to host");
+    });
+}
+
+int tg_hip_noise_plan_create(tg_dev* dev, const tg_noise_layer_kernel* kernel, tg_noise_plan** out,
+                             char* err, size_t errLen) {
+    if (dev == nullptr || kernel == nullptr || out == nullptr) {
+        SetErr(err, errLen, "tg_hip_noise_plan_create: null argument");
+        return TG_ERR_ARGUMENT;
+    }
+    *out = nullptr;
+    return Guard(err, errLen, [&] {
+        Check(hipSetDevice(dev->ordinal), "hipSetDevice");
+        auto plan = std::make_unique<tg_noise_plan>();
+        plan->dev = dev;
+        tg::MakeNoisePlan(*kernel, plan->host);
+        Check(hipMalloc(reinterpret_cast<void**>(&plan->device), sizeof(tg::NoisePlan)),
+              "hipMalloc");
+        const hipError_t copied =
+            hipMemcpy(plan->device, &plan->host, sizeof(tg::NoisePlan), hipMemcpyHostToDevice);
+        if (copied != hipSuccess) {
+            (void)hipFree(plan->device);
+            Check(copied, "hipMemcpy to device");
+        }
+        *out = plan.release();
+    });
+}
+
+int tg_hip_noise_plan_free(tg_noise_plan* plan) {
+    if (plan == nullptr) {
+        return TG_OK;
+    }
+    (void)hipSetDevice(plan->dev->ordinal);
+    (void)hipFree(plan->device);
+    delete plan;
+    return TG_OK;
+}
+
+int tg_hip_noise_launch(tg_dev* dev, const tg_noise_plan* plan, float* heights, const float* xs,
+                        const float* ys, const tg_kernel_params* params, void* stream, char* err,
+                        size_t errLen) {
+    if (dev == nullptr || plan == nullptr || params == nullptr || heights == nullptr ||
+        xs == nullptr || ys == nullptr) {
+        SetErr(err, errLen, "tg_hip_noise_launch: null argument");
+        return TG_ERR_ARGUMENT;
+    }
+    return Guard(err, errLen, [&] {
+        LaunchNoise(plan->device, heights, xs, ys, *params, static_cast<hipStream_t>(stream));
+    });
+}
+
+int tg_hip_malloc(tg_dev* dev, size_t bytes, void** out, char* err, size_t errLen) {
+    if (dev == nullptr || out == nullptr) {
+        SetErr(err, errLen, "tg_hip_malloc: null argument");
+        return TG_ERR_ARGUMENT;
+    }
+    *out = nullptr;
+    return Guard(err, errLen, [&] {
+        Check(hipSetDevice(dev->ordinal), "hipSetDevice");
+        Check(hipMalloc(out, bytes > 0 ? bytes : 1), "hipMalloc");
+    });
+}
+
+int tg_hip_free(tg_dev* dev, void* ptr) {
+    if (dev == nullptr || ptr == nullptr) {
+        return TG_OK;
+    }
+    (void)hipSetDevice(dev->ordinal);
+    return hipFree(ptr) == hipSuccess ? TG_OK : TG_ERR_DEVICE;
+}
+
+int tg_hip_memcpy_h2d(tg_dev* dev, void* dst, const void* src, size_t bytes, char* err,
+                      size_t errLen) {
+    if (dev == nullptr || (bytes > 0 && (dst == nullptr || src == nullptr))) {
+        SetErr(err, errLen, "tg_hip_memcpy_h2d: null argument");
+        return TG_ERR_ARGUMENT;
+    }
+    return Guard(err, errLen, [&] {
+        Check(hipSetDevice(dev->ordinal), "hipSetDevice");
+        if (bytes > 0) {
+            Check(hipMemcpy(dst, src, bytes, hipMemcpyHostToDevice), "hipMemcpy to device");
+        }
+    });
+}
+
+int tg_hip_memcpy_d2h(tg_dev* dev, void* dst, const void* src, size_t bytes, char* err,
+                      size_t errLen) {
+    if (dev == nullptr || (bytes > 0 && (dst == nullptr || src == nullptr))) {
+        SetErr(err, errLen, "tg_hip_memcpy_d2h: null argument");
+        return TG_ERR_ARGUMENT;
+    }
+    return Guard(err, errLen, [&] {
+        Check(hipSetDevice(dev->ordinal), "hipSetDevice");
+        if (bytes > 0) {
+            Check(hipMemcpy(dst, src, bytes, hipMemcpyDeviceToHost), "hipMemcpy to host");
+        }
+    });
+}
+
+int tg_hip_memset(tg_dev* dev, void* dst, int value, size_t bytes, char* err, size_t errLen) {
+    if (dev == nullptr || (bytes > 0 && dst == nullptr)) {
+        SetErr(err, errLen, "tg_hip_memset: null argument");
+        return TG_ERR_ARGUMENT;
+    }
+    return Guard(err, errLen, [&] {
+        Check(hipSetDevice(dev->ordinal), "hipSetDevice");
+        if (bytes > 0) {
+            Check(hipMemset(dst, value, bytes), "hipMemset");
+        }
+    });
+}
+
+int tg_hip_stream_create(tg_dev* dev, void** out, char* err, size_t errLen) {
+    if (dev == nullptr || out == nullptr) {
+        SetErr(err, errLen, "tg_hip_stream_create: null argument");
+        return TG_ERR_ARGUMENT;
+    }
+    return Guard(err, errLen, [&] {
+        Check(hipSetDevice(dev->ordinal), "hipSetDevice");
+        hipStream_t s = nullptr;
+        Check(hipStreamCreateWithFlags(&s, hipStreamNonBlocking), "hipStreamCreate");
+        *out = s;
+    });
+}
+
+int tg_hip_stream_destroy(tg_dev* dev, void* stream) {
+    if (dev == nullptr || stream == nullptr) {
+        return TG_OK;
+    }
+    (void)hipSetDevice(dev->ordinal);
+    return hipStreamDestroy(static_cast<hipStream_t>(stream)) == hipSuccess ? TG_OK
+                                                                             : TG_ERR_DEVICE;
+}
+
+int tg_hip_stream_synchronize(tg_dev* dev, void* stream, char* err, size_t errLen) {
+    if (dev == nullptr) {
+        SetErr(err, errLen, "tg_hip_stream_synchronize: null device");
+        return TG_ERR_ARGUMENT;
+    }
+    return Guard(err, errLen, [&] {
+        Check(hipSetDevice(dev->ordinal), "hipSetDevice");
+        Check(hipStreamSynchronize(static_cast<hipStream_t>(stream)), "hipStreamSynchronize");
+    });
+}
+
+int tg_hip_event_create(tg_dev* dev, void** out, char* err, size_t errLen) {
+    if (dev == nullptr || out == nullptr) {
+        SetErr(err, errLen, "tg_hip_event_create: null argument");
+        return TG_ERR_ARGUMENT;
+    }
+    return Guard(err, errLen, [&] {
+        Check(hipSetDevice(dev->ordinal), "hipSetDevice");
+        hipEvent_t e = nullptr;
+        Check(hipEventCreate(&e), "hipEventCreate");
+        *out = e;
+    });
+}
+
+int tg_hip_event_destroy(tg_dev* dev, void* event) {
+    if (dev == nullptr || event == nullptr) {
+        return TG_OK;
+    }
+    (void)hipSetDevice(dev->ordinal);
+    return hipEventDestroy(static_cast<hipEvent_t>(event)) == hipSuccess ? TG_OK : TG_ERR_DEVICE;
+}
+
+int tg_hip_event_record(tg_dev* dev, void* event, void* stream, char* err, size_t errLen) {
+    if (dev == nullptr || event == nullptr) {
+        SetErr(err, errLen, "tg_hip_event_record: null argument");
+        return TG_ERR_ARGUMENT;
+    }
+    return Guard(err, errLen, [&] {
+        Check(hipEventRecord(static_cast<hipEvent_t>(event), static_cast<hipStream_t>(stream)),
+              "hipEventRecord");
+    });
+}
+
+int tg_hip_event_elapsed(tg_dev* dev, void* start, void* stop, float* ms, char* err,
+                         size_t errLen) {
+    if (dev == nullptr || start == nullptr || stop == nullptr || ms == nullptr) {
+        SetErr(err, errLen, "tg_hip_event_elapsed: null argument");
+        return TG_ERR_ARGUMENT;
+    }
+    return Guard(err, errLen, [&] {
+        Check(hipEventSynchronize(static_cast<hipEvent_t>(stop)), "hipEventSynchronize");
+        Check(hipEventElapsedTime(ms, static_cast<hipEvent_t>(start),
+                                  static_cast<hipEvent_t>(stop)),
+              "hipEventElapsedTime");
+    });
+}
+
+}  // extern "C"

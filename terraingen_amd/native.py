@@ -1,0 +1,271 @@
+# SPDX-License-Identifier: BSD-3-Clause
+"""ctypes view of the native libraries: the std430 structs of include/tg_layouts.h and the C ABI of
+include/tg_hip.h.
+
+Loading is strict: a missing or unloadable ``libtg_hip.so`` raises, there is no fallback path.
+"""
+from __future__ import annotations
+
+import ctypes as C
+from pathlib import Path
+
+import numpy as np
+
+PKG = Path(__file__).resolve().parent
+HIP_SO = PKG / "libtg_hip.so"
+HOST_SO = PKG / "libterraingen_amd.so"
+
+
+def _struct(name: str, fields: list[tuple[str, object]]) -> type:
+    return type(name, (C.Structure,), {"_fields_": fields})
+
+
+u32, i32, f32 = C.c_uint32, C.c_int32, C.c_float
+
+Fractal = _struct("Fractal", [
+    ("enabled", u32), ("seed", u32), ("wavelength", f32), ("amplitude", f32), ("octaves", i32),
+    ("lacunarity", f32), ("gain", f32), ("basis", u32), ("type", u32), ("offset", f32),
+    ("blend", u32), ("warp_amplitude", f32), ("warp_wavelength", f32)])
+
+NoiseLayerKernel = _struct("NoiseLayerKernel", [("fractals", Fractal * 8), ("count", u32)])
+
+KernelParams = _struct("KernelParams", [
+    ("width", u32), ("rows", u32), ("no_data", u32), ("no_data_value", f32), ("first_row", u32)])
+
+CraterProfile = _struct("CraterProfile", [
+    ("simple", u32), ("rim", u32), ("complex_", u32), ("diffused", u32),
+    ("transition_diameter", f32), ("floor_coefficient", f32), ("floor_exponent", f32),
+    ("peak_height_coefficient", f32), ("peak_height_exponent", f32),
+    ("peak_width_coefficient", f32), ("peak_width_exponent", f32), ("peak_from_volume", u32),
+    ("peak_fit_max_diameter", f32), ("ejecta_exponent", f32), ("complex_reach_radii", f32),
+    ("table_extent", f32), ("table_cells", u32), ("ages_offset", u32), ("age_count", u32)])
+
+CraterShape = _struct("CraterShape", [("profile", CraterProfile)] + [
+    (n, t) for n, t in [
+        ("reach_radii", f32), ("profile_reach_radii", f32), ("blend_start", f32),
+        ("blend_end", f32), ("blend_shape", u32), ("interior_replacement", f32),
+        ("replace_start", f32), ("replace_end", f32), ("mantling", f32),
+        ("ejecta_reach_radii", f32), ("rim_sigma", f32), ("rim_inner", f32), ("rim_outer", f32),
+        ("rim_strength", f32), ("rim_shape", u32), ("noise_amplitude", f32),
+        ("noise_wavelength", f32), ("noise_octaves", i32), ("noise_gain", f32),
+        ("noise_rim_amplitude", f32), ("noise_rim_wavelength", f32), ("noise_rim_inner", f32),
+        ("noise_rim_outer", f32), ("wall_streaks", f32), ("wall_streak_width", f32),
+        ("ejecta", f32), ("ejecta_lineation", f32), ("min_freshness", f32)]])
+
+RECORD_FIELDS = [
+    ("x", f32), ("y", f32), ("diameter", f32), ("resolution", f32), ("fresh_depth", f32),
+    ("fresh_rim", f32), ("current_depth", f32), ("current_rim", f32), ("diffusion", f32),
+    ("ellipticity", f32), ("heading", f32), ("first_harmonic", u32), ("harmonic_count", u32),
+    ("harmonic_offset", u32), ("table_count", u32), ("table_offset", u32),
+    ("profile_offset", u32), ("has_datum", u32), ("datum_a", f32), ("datum_b", f32),
+    ("datum_c", f32), ("noise_key", u32), ("texture_key", u32), ("mark_count", u32),
+    ("mark_offset", u32), ("shift_x", f32), ("shift_y", f32), ("upslope_x", f32),
+    ("upslope_y", f32), ("terrace_count", u32), ("terrace_offset", u32), ("peak_count", u32),
+    ("peak_offset", u32), ("replace", u32), ("x_low", f32), ("y_low", f32)]
+CraterRecord = _struct("CraterRecord", RECORD_FIELDS)
+RECORD_DTYPE = np.dtype([(n, np.float32 if t is f32 else np.uint32) for n, t in RECORD_FIELDS])
+
+CraterFrame = _struct("CraterFrame", [(n, f32) for n in (
+    "x", "y", "z", "east_x", "east_y", "east_z", "north_x", "north_y", "north_z",
+    "sphere_radius", "x_low", "y_low", "z_low")])
+FRAME_DTYPE = np.dtype([(n, np.float32) for n, _ in CraterFrame._fields_])
+
+SplitPoint = _struct("SplitPoint", [(n, f32) for n in ("x", "x_low", "y", "y_low", "z", "z_low")])
+POINT_DTYPE = np.dtype([(n, np.float32) for n, _ in SplitPoint._fields_])
+
+TgBuf = _struct("TgBuf", [("data", C.c_void_p), ("bytes", C.c_size_t), ("count", C.c_size_t),
+                          ("per_cell", C.c_size_t)])
+
+SIZES = {"Fractal": 52, "NoiseLayerKernel": 420, "CraterProfile": 76, "CraterShape": 188,
+         "CraterRecord": 144, "CraterFrame": 52, "SplitPoint": 24, "KernelParams": 20}
+for _name, _size in SIZES.items():
+    assert C.sizeof(globals()[_name]) == _size, (_name, C.sizeof(globals()[_name]))
+assert RECORD_DTYPE.itemsize == 144 and FRAME_DTYPE.itemsize == 52 and POINT_DTYPE.itemsize == 24
+
+TG_KERNEL_NOISE_LAYER = 1
+TG_KERNEL_CRATERS = 2
+NODATA_NONE, NODATA_NAN, NODATA_VALUE = 0, 1, 2
+
+# Every entry point include/tg_hip.h declares, with its ctypes signature.
+VP, SZ, CP = C.c_void_p, C.c_size_t, C.c_char_p
+HIP_ABI = {
+    "tg_hip_open": (C.c_int, [C.c_int, C.POINTER(VP), CP, SZ]),
+    "tg_hip_device_name": (CP, [VP]),
+    "tg_hip_can_run": (C.c_int, [VP, C.c_int]),
+    "tg_hip_close": (C.c_int, [VP]),
+    "tg_hip_run": (C.c_int, [VP, C.c_int, VP, SZ, VP, VP, VP, C.POINTER(KernelParams),
+                             C.POINTER(TgBuf), SZ, CP, SZ]),
+    "tg_hip_noise_plan_create": (C.c_int, [VP, C.POINTER(NoiseLayerKernel), C.POINTER(VP), CP,
+                                           SZ]),
+    "tg_hip_noise_plan_free": (C.c_int, [VP]),
+    "tg_hip_noise_launch": (C.c_int, [VP, VP, VP, VP, VP, C.POINTER(KernelParams), VP, CP, SZ]),
+    "tg_hip_malloc": (C.c_int, [VP, SZ, C.POINTER(VP), CP, SZ]),
+    "tg_hip_free": (C.c_int, [VP, VP]),
+    "tg_hip_memcpy_h2d": (C.c_int, [VP, VP, VP, SZ, CP, SZ]),
+    "tg_hip_memcpy_d2h": (C.c_int, [VP, VP, VP, SZ, CP, SZ]),
+    "tg_hip_memset": (C.c_int, [VP, VP, C.c_int, SZ, CP, SZ]),
+    "tg_hip_stream_create": (C.c_int, [VP, C.POINTER(VP), CP, SZ]),
+    "tg_hip_stream_destroy": (C.c_int, [VP, VP]),
+    "tg_hip_stream_synchronize": (C.c_int, [VP, VP, CP, SZ]),
+    "tg_hip_event_create": (C.c_int, [VP, C.POINTER(VP), CP, SZ]),
+    "tg_hip_event_destroy": (C.c_int, [VP, VP]),
+    "tg_hip_event_record": (C.c_int, [VP, VP, VP, CP, SZ]),
+    "tg_hip_event_elapsed": (C.c_int, [VP, VP, VP, C.POINTER(C.c_float), CP, SZ]),
+}
+
+_hip = None
+
+
+def hip_library() -> C.CDLL:
+    """The kernels' library; raises if it is missing (build it with ``python -m
+    terraingen_amd.build`` or ``__graft_entry__.build()``)."""
+    global _hip
+    if _hip is None:
+        if not HIP_SO.exists():
+            raise RuntimeError(f"{HIP_SO} is missing: build the hip backend first")
+        lib = C.CDLL(str(HIP_SO))
+        for name, (res, args) in HIP_ABI.items():
+            fn = getattr(lib, name)
+            fn.restype = res
+            fn.argtypes = args
+        _hip = lib
+    return _hip
+
+
+class HipError(RuntimeError):
+    """An error the hip backend reported, with the runtime's message."""
+
+
+def _check(code: int, err: C.Array) -> None:
+    if code != 0:
+        raise HipError(err.value.decode(errors="replace") or f"error {code}")
+
+
+def _ptr(a: np.ndarray) -> int:
+    return a.ctypes.data
+
+
+class HipDevice:
+    """One GPU opened through ``tg_hip_open``; the same object per ordinal for the process."""
+
+    def __init__(self, ordinal: int = 0):
+        lib = hip_library()
+        err = C.create_string_buffer(512)
+        handle = VP()
+        _check(lib.tg_hip_open(ordinal, C.byref(handle), err, 512), err)
+        self.lib = lib
+        self.handle = handle
+        self.ordinal = ordinal
+
+    def call(self, name: str, *args) -> None:
+        """Calls entry point ``name`` with ``args`` plus the error buffer; raises on failure."""
+        err = C.create_string_buffer(512)
+        _check(getattr(self.lib, name)(*args, err, 512), err)
+
+    def synchronize(self, stream=None) -> None:
+        self.call("tg_hip_stream_synchronize", self.handle, stream)
+
+    @property
+    def name(self) -> str:
+        return self.lib.tg_hip_device_name(self.handle).decode()
+
+    def run(self, kernel_id: int, kernel: C.Structure, heights: np.ndarray, xs: np.ndarray,
+            ys: np.ndarray, params: KernelParams, extra: list[tuple[np.ndarray, int]] = ()) -> None:
+        """``tg_hip_run`` on host arrays; ``heights`` (float32, C order) is updated in place."""
+        assert heights.dtype == np.float32 and heights.flags.c_contiguous
+        assert xs.dtype == np.float32 and ys.dtype == np.float32
+        bufs = (TgBuf * max(1, len(extra)))()
+        keep = []
+        for i, (arr, per_cell) in enumerate(extra):
+            arr = np.ascontiguousarray(arr)
+            keep.append(arr)
+            count = arr.shape[0] if arr.ndim > 0 else 1
+            bufs[i] = TgBuf(arr.ctypes.data, arr.nbytes, count, per_cell)
+        err = C.create_string_buffer(512)
+        code = self.lib.tg_hip_run(self.handle, kernel_id, C.byref(kernel), C.sizeof(kernel),
+                                   _ptr(heights), _ptr(xs), _ptr(ys), C.byref(params), bufs,
+                                   len(extra), err, 512)
+        _check(code, err)
+
+
+def noise_kernel(fractals: list[dict]) -> NoiseLayerKernel:
+    """A NoiseLayerKernel of the enabled ``fractals`` (MakeNoiseLayerKernel, noise_layer.h:41-50),
+    each a dict of Fractal fields with the reference's defaults."""
+    k = NoiseLayerKernel()
+    for i in range(8):
+        # Unused slots hold default-constructed fractals, as the reference's FractalStack does.
+        k.fractals[i] = make_fractal()
+    n = 0
+    for f in fractals:
+        fr = make_fractal(**f)
+        if fr.enabled and n < 8:
+            k.fractals[n] = fr
+            n += 1
+    k.count = n
+    return k
+
+
+def make_fractal(enabled=True, seed=0, wavelength=1000.0, amplitude=100.0, octaves=8,
+                 lacunarity=2.0, gain=0.5, basis=0, type=0, offset=1.0, blend=0,
+                 warp_amplitude=0.0, warp_wavelength=500.0) -> Fractal:
+    """A Fractal with the reference's member defaults (fractals.h:29-44)."""
+    return Fractal(int(bool(enabled)), seed, wavelength, amplitude, octaves, lacunarity, gain,
+                   basis, type, offset, blend, warp_amplitude, warp_wavelength)
+
+
+class DeviceArray:
+    """Device memory from ``tg_hip_malloc``, freed with the object."""
+
+    def __init__(self, dev: HipDevice, nbytes: int):
+        self.dev = dev
+        self.nbytes = int(nbytes)
+        ptr = VP()
+        dev.call("tg_hip_malloc", dev.handle, max(1, self.nbytes), C.byref(ptr))
+        self.ptr = ptr.value
+
+    @classmethod
+    def from_host(cls, dev: HipDevice, arr: np.ndarray) -> "DeviceArray":
+        arr = np.ascontiguousarray(arr)
+        d = cls(dev, arr.nbytes)
+        d.upload(arr)
+        return d
+
+    def upload(self, arr: np.ndarray) -> None:
+        arr = np.ascontiguousarray(arr)
+        assert arr.nbytes <= self.nbytes
+        self.dev.call("tg_hip_memcpy_h2d", self.dev.handle, self.ptr, arr.ctypes.data, arr.nbytes)
+
+    def download(self, out: np.ndarray) -> np.ndarray:
+        assert out.flags.c_contiguous and out.nbytes <= self.nbytes
+        self.dev.call("tg_hip_memcpy_d2h", self.dev.handle, out.ctypes.data, self.ptr, out.nbytes)
+        return out
+
+    def zero(self) -> None:
+        self.dev.call("tg_hip_memset", self.dev.handle, self.ptr, 0, self.nbytes)
+
+    def __del__(self):
+        if getattr(self, "ptr", None):
+            self.dev.lib.tg_hip_free(self.dev.handle, self.ptr)
+            self.ptr = None
+
+
+class Event:
+    def __init__(self, dev: HipDevice):
+        self.dev = dev
+        e = VP()
+        dev.call("tg_hip_event_create", dev.handle, C.byref(e))
+        self.handle = e.value
+
+    def record(self, stream=None) -> None:
+        self.dev.call("tg_hip_event_record", self.dev.handle, self.handle, stream)
+
+    def elapsed_ms(self, stop: "Event") -> float:
+        ms = C.c_float()
+        self.dev.call("tg_hip_event_elapsed", self.dev.handle, self.handle, stop.handle,
+                      C.byref(ms))
+        return ms.value
+
+    def __del__(self):
+        if getattr(self, "handle", None):
+            self.dev.lib.tg_hip_event_destroy(self.dev.handle, self.handle)
+            self.handle = None
