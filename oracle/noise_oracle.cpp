@@ -16,6 +16,7 @@
 // Pinned by: tests/golden/noise/kernels_test.cpp:21-103 (golden values), oracle/_ref (the
 // reference's own headers compiled beside it) and brute-force invariants in tests/.
 
+#include "noise_core.h"
 #include "tg_layouts.h"
 
 #include <algorithm>
@@ -25,295 +26,9 @@
 #include <thread>
 #include <vector>
 
+using namespace orc;
+
 namespace {
-
-using u32 = std::uint32_t;
-
-// include/terraingen/noise/hash.h:33-37
-u32 Pcg(u32 v) {
-    const u32 state = v * 747796405u + 2891336453u;
-    const u32 word = ((state >> ((state >> 28u) + 4u)) ^ state) * 277803737u;
-    return (word >> 22u) ^ word;
-}
-
-struct U3 {
-    u32 x, y, z;
-};
-
-// include/terraingen/noise/hash.h:40-57
-U3 Pcg3(U3 v) {
-    v.x = v.x * 1664525u + 1013904223u;
-    v.y = v.y * 1664525u + 1013904223u;
-    v.z = v.z * 1664525u + 1013904223u;
-    v.x += v.y * v.z;
-    v.y += v.z * v.x;
-    v.z += v.x * v.y;
-    v.x ^= v.x >> 16u;
-    v.y ^= v.y >> 16u;
-    v.z ^= v.z >> 16u;
-    v.x += v.y * v.z;
-    v.y += v.z * v.x;
-    v.z += v.x * v.y;
-    return v;
-}
-
-// include/terraingen/noise/hash.h:60-62
-float UnitFloat(u32 h) {
-    return float(h >> 8u) * 0x1p-24f;
-}
-
-template <class R> struct Lattice2 {
-    std::int32_t ix, iy;
-    R fx, fy;
-};
-
-// include/terraingen/noise/lattice.h:31-36 (floor, then the fraction in the same precision).
-template <class R> Lattice2<R> Split2(R x, R y) {
-    const R cx = std::floor(x);
-    const R cy = std::floor(y);
-    return {std::int32_t(std::int64_t(cx)), std::int32_t(std::int64_t(cy)), x - cx, y - cy};
-}
-
-// include/terraingen/noise/lattice.h:76-83
-template <class R> R Fade(R t) {
-    return t * t * t * (t * (t * R(6) - R(15)) + R(10));
-}
-template <class R> R Mix(R t, R a, R b) {
-    return a + t * (b - a);
-}
-
-// include/terraingen/noise/perlin.h:17-26; the constant is the reference's float constant.
-template <class R> R Grad2(u32 hash, R x, R y) {
-    const R invSqrt2 = R(0.707106781186547524f);
-    const u32 h = hash & 7u;
-    const R axis = (h & 2u) != 0u ? y : x;
-    const R sx = h == 5u ? -x : x;
-    const R sy = h == 6u ? -y : y;
-    const R sum = sx + sy;
-    const R diag = (h == 7u ? -sum : sum) * invSqrt2;
-    return h < 4u ? ((h & 1u) != 0u ? -axis : axis) : diag;
-}
-
-// include/terraingen/noise/perlin.h:31-33, lattice.h:89-92
-u32 CornerHash(u32 ix, u32 iy, u32 seed) {
-    return Pcg3({ix, iy, seed}).x;
-}
-
-// include/terraingen/noise/perlin.h:72-90
-template <class R> R Perlin2(std::int32_t ix, std::int32_t iy, R fx, R fy, u32 seed) {
-    const u32 x0 = u32(ix), y0 = u32(iy), x1 = x0 + 1u, y1 = y0 + 1u;
-    const R n00 = Grad2<R>(CornerHash(x0, y0, seed), fx, fy);
-    const R n10 = Grad2<R>(CornerHash(x1, y0, seed), fx - R(1), fy);
-    const R n01 = Grad2<R>(CornerHash(x0, y1, seed), fx, fy - R(1));
-    const R n11 = Grad2<R>(CornerHash(x1, y1, seed), fx - R(1), fy - R(1));
-    const R u = Fade(fx), v = Fade(fy);
-    return R(1.41421356237309505f) * Mix(v, Mix(u, n00, n10), Mix(u, n01, n11));
-}
-
-template <class R> R Perlin2At(R x, R y, u32 seed) {
-    const Lattice2<R> p = Split2(x, y);
-    return Perlin2(p.ix, p.iy, p.fx, p.fy, seed);
-}
-
-// include/terraingen/noise/simplex.h:18-24 (corner) and :34-59 (noise).
-template <class R> R SimplexCorner(u32 ci, u32 cj, R dx, R dy, u32 seed) {
-    const R t = R(0.5) - dx * dx - dy * dy;
-    const R t2 = t * t;
-    const R c = t2 * t2 * Grad2<R>(CornerHash(ci, cj, seed), dx, dy);
-    return !(t <= R(0)) ? c : R(0);
-}
-
-template <class R> R Simplex2(R x, R y, u32 seed) {
-    const R skewK = R(0.366025403784438647f);
-    const R unskewK = R(0.211324865405187118f);
-    const R skew = (x + y) * skewK;
-    const R cx = std::floor(x + skew);
-    const R cy = std::floor(y + skew);
-    const R unskew = (cx + cy) * unskewK;
-    const R x0 = x - (cx - unskew);
-    const R y0 = y - (cy - unskew);
-    const u32 stepX = x0 > y0 ? 1u : 0u;
-    const u32 stepY = 1u - stepX;
-    const R x1 = x0 - R(stepX) + unskewK;
-    const R y1 = y0 - R(stepY) + unskewK;
-    const R x2 = x0 - R(1) + R(2) * unskewK;
-    const R y2 = y0 - R(1) + R(2) * unskewK;
-    const u32 i = u32(std::int32_t(std::int64_t(cx)));
-    const u32 j = u32(std::int32_t(std::int64_t(cy)));
-    return R(99) * (SimplexCorner(i, j, x0, y0, seed) +
-                    SimplexCorner(i + stepX, j + stepY, x1, y1, seed) +
-                    SimplexCorner(i + 1u, j + 1u, x2, y2, seed));
-}
-
-// include/terraingen/noise/value.h:19-21 and :34-45
-template <class R> R LatticeValue(U3 key) {
-    return R(2) * R(UnitFloat(Pcg3(key).y)) - R(1);
-}
-
-template <class R> R Value2(R x, R y, u32 seed) {
-    const Lattice2<R> p = Split2(x, y);
-    const u32 x0 = u32(p.ix), y0 = u32(p.iy);
-    const R v00 = LatticeValue<R>({x0, y0, seed});
-    const R v10 = LatticeValue<R>({x0 + 1u, y0, seed});
-    const R v01 = LatticeValue<R>({x0, y0 + 1u, seed});
-    const R v11 = LatticeValue<R>({x0 + 1u, y0 + 1u, seed});
-    const R u = Fade(p.fx), v = Fade(p.fy);
-    return Mix(v, Mix(u, v00, v10), Mix(u, v01, v11));
-}
-
-template <class R> struct Worley {
-    R f1, f2, value;
-};
-
-// include/terraingen/noise/worley.h:38-63: 5x5 neighbourhood, F1, F2 and the nearest cell's value.
-template <class R> Worley<R> Worley2(R x, R y, u32 seed) {
-    const Lattice2<R> p = Split2(x, y);
-    R d1 = R(1e30f), d2 = R(1e30f);
-    u32 nearest = 0u;
-    for (std::int32_t dy = -2; dy <= 2; ++dy) {
-        for (std::int32_t dx = -2; dx <= 2; ++dx) {
-            const U3 h = Pcg3({u32(p.ix + dx), u32(p.iy + dy), seed});
-            const R px = R(dx) + R(UnitFloat(h.y)) - p.fx;
-            const R py = R(dy) + R(UnitFloat(h.z)) - p.fy;
-            const R d = px * px + py * py;
-            if (d < d1) {
-                d2 = d1;
-                d1 = d;
-                nearest = h.x;
-            } else if (d < d2) {
-                d2 = d;
-            }
-        }
-    }
-    return {std::sqrt(d1), std::sqrt(d2), R(2) * R(UnitFloat(nearest)) - R(1)};
-}
-
-// include/terraingen/noise/fractal.h:89-111
-template <class R> R Basis2(u32 basis, R x, R y, u32 seed) {
-    const R invSqrt2 = R(0.707106781186547524f);
-    const R invSqrt5 = R(0.447213595499957939f);
-    switch (basis) {
-    case TG_BASIS_SIMPLEX:
-        return Simplex2(x, y, seed);
-    case TG_BASIS_VALUE:
-        return Value2(x, y, seed);
-    case TG_BASIS_WORLEY_F1:
-        return R(2) * std::min(Worley2(x, y, seed).f1 * invSqrt2, R(1)) - R(1);
-    case TG_BASIS_WORLEY_F2:
-        return R(2) * std::min(Worley2(x, y, seed).f2 * invSqrt5, R(1)) - R(1);
-    case TG_BASIS_WORLEY_F2_MINUS_F1: {
-        const Worley<R> w = Worley2(x, y, seed);
-        return R(2) * std::min((w.f2 - w.f1) * invSqrt5, R(1)) - R(1);
-    }
-    case TG_BASIS_VORONOI:
-        return Worley2(x, y, seed).value;
-    default:
-        return Perlin2At(x, y, seed);
-    }
-}
-
-// The per-octave parameters of a fractal, derived in FP32 exactly as the reference's octave loop
-// does (fractal.h:219-239): the seed and lattice offsets from Pcg3(OctaveKey(seed, octave, 0)),
-// the frequency by repeated multiplication, the golden-angle turn by its FP32 recurrence
-// (fractal.h:142-144).
-struct Octave {
-    u32 seed;
-    float offsetX, offsetY, frequency, turnCos, turnSin;
-};
-
-struct Params {
-    u32 seed;
-    float wavelength, amplitude;
-    int octaves;
-    float lacunarity, gain;
-    u32 basis, type;
-    float offset;
-};
-
-std::vector<Octave> Octaves(const Params& p) {
-    std::vector<Octave> out;
-    const float kCos = -0.7373688780783197f;
-    const float kSin = 0.6754902942615238f;
-    float frequency = 1.0f / p.wavelength;
-    float c = 1.0f, s = 0.0f;
-    for (int o = 0; o < p.octaves; ++o) {
-        const U3 h = Pcg3({p.seed, u32(o), 0u});
-        out.push_back({h.x, UnitFloat(h.y), UnitFloat(h.z), frequency, c, s});
-        frequency *= p.lacunarity;
-        const float nc = c * kCos - s * kSin;
-        s = s * kCos + c * kSin;
-        c = nc;
-    }
-    return out;
-}
-
-template <class R> struct Sums {
-    R weight = R(1), sum = R(0), total = R(0), feedback = R(1);
-};
-
-// include/terraingen/noise/fractal.h:164-197; the weights follow the gain in precision R.
-template <class R> void AddOctave(Sums<R>& s, const Params& p, int octave, R n) {
-    const R offset = R(p.offset);
-    switch (p.type) {
-    case TG_FRACTAL_BILLOW:
-        s.sum += s.weight * (R(2) * std::abs(n) - R(1));
-        break;
-    case TG_FRACTAL_RIDGED: {
-        R signal = offset - std::abs(n);
-        signal *= signal * s.feedback;
-        s.feedback = std::clamp(R(2) * signal, R(0), R(1));
-        s.sum += s.weight * signal;
-        break;
-    }
-    case TG_FRACTAL_HYBRID: {
-        const R signal = (n + offset) * s.weight;
-        if (octave == 0) {
-            s.sum = signal;
-            s.feedback = signal;
-        } else {
-            s.feedback = std::clamp(s.feedback, R(0), R(1));
-            s.sum += s.feedback * signal;
-            s.feedback *= signal;
-        }
-        break;
-    }
-    default:
-        s.sum += s.weight * n;
-        break;
-    }
-    s.total += s.weight;
-    s.weight *= R(p.gain);
-}
-
-// include/terraingen/noise/fractal.h:201-211
-template <class R> R Finish(const Sums<R>& s, const Params& p) {
-    const R amp = R(p.amplitude), offset = R(p.offset);
-    switch (p.type) {
-    case TG_FRACTAL_RIDGED: {
-        const R ridge = std::max(offset, R(1) - offset);
-        return amp * (R(2) * s.sum / (ridge * ridge * s.total) - R(1));
-    }
-    case TG_FRACTAL_HYBRID:
-        return amp * s.sum / ((R(1) + offset) * s.total);
-    default:
-        return amp * s.sum / s.total;
-    }
-}
-
-// include/terraingen/noise/fractal.h:219-239
-template <class R> R Fractal2(R x, R y, const Params& p) {
-    Sums<R> sums;
-    const std::vector<Octave> octaves = Octaves(p);
-    for (int o = 0; o < p.octaves; ++o) {
-        const Octave& q = octaves[std::size_t(o)];
-        const R c = R(q.turnCos), s = R(q.turnSin), f = R(q.frequency);
-        const R rx = x * c - y * s;
-        const R ry = x * s + y * c;
-        const R n = Basis2<R>(p.basis, rx * f + R(q.offsetX), ry * f + R(q.offsetY), q.seed);
-        AddOctave(sums, p, o, n);
-    }
-    return Finish(sums, p);
-}
 
 Params ParamsOf(const tg_fractal& f) {
     return {f.seed, f.wavelength, f.amplitude, f.octaves, f.lacunarity,

@@ -1,0 +1,116 @@
+// SPDX-License-Identifier: BSD-3-Clause
+//
+// Shared pieces of the host library: grids and pixel windows in the reference's conventions
+// (include/terraingen/raster/{geo_transform,cell_loops}.h), error reporting across the C API.
+#pragma once
+
+#include "terraingen_amd.h"
+
+#include <algorithm>
+#include <cmath>
+#include <cstddef>
+#include <cstdint>
+#include <cstdio>
+#include <cstring>
+#include <stdexcept>
+#include <string>
+#include <vector>
+
+namespace tgh {
+
+// A north-up grid (GeoTransform::NorthUp: pixelWidth = pixel, pixelHeight = -pixel).
+struct Grid {
+    std::size_t width = 0;
+    std::size_t height = 0;
+    double originX = 0.0;
+    double originY = 0.0;
+    double pixel = 1.0;
+
+    static Grid From(const tgx_grid& g) {
+        return {g.width, g.height, g.origin_x, g.origin_y, g.pixel};
+    }
+
+    // GeoTransform::PixelToWorld of the cell's center (raster.h:187-190).
+    double CenterX(std::size_t col) const {
+        return originX + (static_cast<double>(col) + 0.5) * pixel + 0.0;
+    }
+    double CenterY(std::size_t row) const {
+        return originY + (static_cast<double>(row) + 0.5) * -pixel;
+    }
+
+    // GeoTransform::WorldToPixel (geo_transform.h:72-81) without rotation terms changing bits:
+    // det = w * h - 0 * 0, col = (dx * h - dy * 0) / det, row = (dy * w - dx * 0) / det.
+    void WorldToPixel(double x, double y, double& col, double& row) const {
+        const double w = pixel, h = -pixel;
+        const double det = w * h - 0.0 * 0.0;
+        const double dx = x - originX;
+        const double dy = y - originY;
+        col = (dx * h - dy * 0.0) / det;
+        row = (dy * w - dx * 0.0) / det;
+    }
+};
+
+struct PixelWindow {
+    std::size_t colBegin = 0, colEnd = 0, rowBegin = 0, rowEnd = 0;
+    bool Empty() const {
+        return colBegin >= colEnd || rowBegin >= rowEnd;
+    }
+};
+
+// ClampIndex(value, limit + 1) of cell_loops.cpp:14-16 with bits.h:42-44.
+inline std::size_t ClampTo(double value, std::size_t limit) {
+    return static_cast<std::size_t>(std::clamp(value, 0.0, static_cast<double>(limit)));
+}
+
+// PixelWindowIn (src/raster/cell_loops.cpp:20-27): the box's corners mapped to pixel
+// coordinates, floored and ceiled, clamped to the grid.
+inline PixelWindow PixelWindowIn(const Grid& g, double minX, double minY, double maxX,
+                                 double maxY) {
+    double lo[2] = {INFINITY, INFINITY}, hi[2] = {-INFINITY, -INFINITY};
+    for (const double x : {minX, maxX}) {
+        for (const double y : {minY, maxY}) {
+            double c, r;
+            g.WorldToPixel(x, y, c, r);
+            lo[0] = std::fmin(lo[0], c);
+            lo[1] = std::fmin(lo[1], r);
+            hi[0] = std::fmax(hi[0], c);
+            hi[1] = std::fmax(hi[1], r);
+        }
+    }
+    return {ClampTo(std::floor(lo[0]), g.width), ClampTo(std::ceil(hi[0]), g.width),
+            ClampTo(std::floor(lo[1]), g.height), ClampTo(std::ceil(hi[1]), g.height)};
+}
+
+// Copies `v` into a malloc'd array the caller frees with tgx_free.
+template <class T> T* Export(const std::vector<T>& v) {
+    T* out = static_cast<T*>(std::malloc(std::max<std::size_t>(1, v.size()) * sizeof(T)));
+    if (out == nullptr) {
+        throw std::bad_alloc();
+    }
+    if (!v.empty()) {
+        std::memcpy(out, v.data(), v.size() * sizeof(T));
+    }
+    return out;
+}
+
+inline void SetErr(char* err, std::size_t len, const std::string& message) {
+    if (err != nullptr && len > 0) {
+        std::snprintf(err, len, "%s", message.c_str());
+    }
+}
+
+// Runs `body`, turning exceptions into a nonzero return and a message.
+template <class F> int Guard(char* err, std::size_t len, F body) {
+    try {
+        body();
+        return 0;
+    } catch (const std::invalid_argument& e) {
+        SetErr(err, len, e.what());
+        return 1;
+    } catch (const std::exception& e) {
+        SetErr(err, len, e.what());
+        return 2;
+    }
+}
+
+}  // namespace tgh

@@ -269,3 +269,85 @@ class Event:
         if getattr(self, "handle", None):
             self.dev.lib.tg_hip_event_destroy(self.dev.handle, self.handle)
             self.handle = None
+
+
+# ---- The host library (include/terraingen_amd.h) ----------------------------------------------
+
+Grid = _struct("Grid", [("width", u32), ("height", u32), ("origin_x", C.c_double),
+                        ("origin_y", C.c_double), ("pixel", C.c_double)])
+
+HOST_ABI = {
+    "tgx_free": (None, [VP]),
+    "tgx_outline_max_scale": (C.c_double, [C.POINTER(CraterRecord), VP]),
+    "tgx_bin_plane": (C.c_int, [C.POINTER(Grid), VP, VP, VP, SZ, u32, C.POINTER(C.POINTER(u32)),
+                                C.POINTER(SZ), C.POINTER(C.POINTER(u32)), C.POINTER(SZ), CP, SZ]),
+}
+
+_host = None
+
+
+def host_library() -> C.CDLL:
+    """The host library; raises if it is missing."""
+    global _host
+    if _host is None:
+        if not HOST_SO.exists():
+            raise RuntimeError(f"{HOST_SO} is missing: build it first")
+        lib = C.CDLL(str(HOST_SO))
+        for name, (res, args) in HOST_ABI.items():
+            fn = getattr(lib, name)
+            fn.restype = res
+            fn.argtypes = args
+        _host = lib
+    return _host
+
+
+def north_up_grid(width: int, height: int, pixel: float, center=(0.0, 0.0)) -> Grid:
+    """The grid FlatSource builds (src/sources/flat_source.cpp:23-25)."""
+    return Grid(width, height, center[0] - 0.5 * width * pixel, center[1] + 0.5 * height * pixel,
+                pixel)
+
+
+def grid_centers(g: Grid) -> tuple[np.ndarray, np.ndarray]:
+    """FP64 cell-centre x per column and y per row (raster.h:187-190)."""
+    xs = g.origin_x + (np.arange(g.width) + 0.5) * g.pixel + 0.0
+    ys = g.origin_y + (np.arange(g.height) + 0.5) * -g.pixel
+    return xs, ys
+
+
+def split(v: np.ndarray) -> tuple[np.ndarray, np.ndarray]:
+    """SplitCoordinate (crater_kernel.h:35-38): the float and the float of what it leaves."""
+    hi = v.astype(np.float32)
+    lo = (v - hi.astype(np.float64)).astype(np.float32)
+    return hi, lo
+
+
+def bin_plane(g: Grid, cx: np.ndarray, cy: np.ndarray, reach: np.ndarray, side: int = 16):
+    """``tgx_bin_plane``: (tiles, entries) as uint32 arrays."""
+    lib = host_library()
+    cx, cy, reach = (np.ascontiguousarray(a, np.float64) for a in (cx, cy, reach))
+    tp, ep = C.POINTER(u32)(), C.POINTER(u32)()
+    tn, en = SZ(), SZ()
+    err = C.create_string_buffer(512)
+    code = lib.tgx_bin_plane(C.byref(g), cx.ctypes.data, cy.ctypes.data, reach.ctypes.data,
+                             cx.size, side, C.byref(tp), C.byref(tn), C.byref(ep), C.byref(en),
+                             err, 512)
+    _check(code, err)
+    try:
+        tiles = np.ctypeslib.as_array(tp, (tn.value,)).copy()
+        entries = np.ctypeslib.as_array(ep, (en.value,)).copy()
+    finally:
+        lib.tgx_free(tp)
+        lib.tgx_free(ep)
+    return tiles, entries
+
+
+def record_reach(records: np.ndarray, data: np.ndarray, reach_radii: float) -> np.ndarray:
+    """Each record's footprint half-width, MaxScale x reach radii x radius (prepared.h:31-33)."""
+    lib = host_library()
+    out = np.empty(records.size)
+    rp = records.ctypes.data
+    for i in range(records.size):
+        rec = CraterRecord.from_address(rp + i * 144)
+        out[i] = lib.tgx_outline_max_scale(C.byref(rec), data.ctypes.data) * reach_radii * (
+            0.5 * float(records["diameter"][i]))
+    return out
