@@ -49,6 +49,77 @@ int tgx_bin_plane(const tgx_grid* grid, const double* cx, const double* cy, cons
                   size_t n, uint32_t side, uint32_t** tiles, size_t* tile_words,
                   uint32_t** entries, size_t* entry_count, char* err, size_t err_len);
 
+/* A placed crater (the columns of CraterList, include/terraingen/layers/craters/crater.h). */
+typedef struct tgx_crater {
+    double x;
+    double y;
+    double diameter;
+    double age_gyr;
+    uint64_t id;
+} tgx_crater;
+
+/* Generation of craters from a power-law density N(>=D) = sfd_coefficient D^-sfd_exponent per m^2
+ * over the region [min_x, max_x) x [min_y, max_y) meters: GenerateCraters
+ * (src/layers/craters/generation/population.cpp:414-424) with CraterGenerationOptions'
+ * minDiameter, maxDiameter, marginRadii, cellSizeFactor (0 = 16) and minResolvedDiameter, ages
+ * uniform over lifetime_gyr (0 = fresh). */
+typedef struct tgx_generation {
+    double min_x, min_y, max_x, max_y;
+    double sfd_coefficient, sfd_exponent;
+    double min_diameter, max_diameter;
+    double margin_radii;
+    double cell_size_factor;
+    double min_resolved_diameter;
+    double lifetime_gyr;
+    uint32_t seed;
+} tgx_generation;
+
+/* Philox4x32-10 of a 4-word counter under a 2-word key (include/terraingen/random/philox.h:79-91),
+ * the generator every placement draw comes from. */
+void tgx_philox4x32(const uint32_t* counter, const uint32_t* key, uint32_t* out);
+
+/* The craters in band, cell and index order, bit-identical to the reference's. */
+int tgx_generate_craters(const tgx_generation* spec, tgx_crater** craters, size_t* count,
+                         char* err, size_t err_len);
+
+/* The crater models a layer shapes with: `profile` TG_PROFILE_MPD or TG_PROFILE_FASSETT_THOMSON,
+ * complex craters (ComplexCraterProfile with the lunar shape) from `transition_diameter` when
+ * `complex_` is 1, and FreshnessDegradation with the lunar fresh morphometry and ages uniform over
+ * `lifetime_gyr` (include/terraingen/layers/craters/crater_layer.h:29-46). */
+typedef struct tgx_crater_models {
+    uint32_t profile;
+    uint32_t complex_;
+    double transition_diameter;
+    double lifetime_gyr;
+} tgx_crater_models;
+
+/* The CraterApplyOptions switches of the hot path (apply_options.h:296-318): irregular outlines,
+ * central peaks, terraces, and the optional rim smoothing, noise and texture amplitudes. */
+typedef struct tgx_apply {
+    uint32_t irregular;
+    uint32_t peaks;
+    uint32_t terraces;
+    double rim_sigma;
+    double noise_amplitude;
+    double noise_rim_amplitude;
+    double wall_streaks;
+    double ejecta;
+} tgx_apply;
+
+/* The crater layer's margin in radii: CraterReachRadii x MaxOutlineScale
+ * (src/layers/craters/crater_layer.cpp:77-78). */
+double tgx_margin_radii(const tgx_crater_models* models, const tgx_apply* apply);
+
+/* PrepareCraters + MakeCraterKernelBatch over `craters` at sample `spacing` meters, the layer's
+ * apply seeds derived from `layer_seed` (SeededCraterApply): the kernel struct, the records in
+ * apply order (craters too small to resolve dropped), the data, and when `footprint` is not null
+ * three doubles per record (centre x, centre y, footprint half-width). */
+int tgx_prepare_craters(const tgx_crater* craters, size_t n, const tgx_crater_models* models,
+                        const tgx_apply* apply, uint32_t layer_seed, double spacing,
+                        tg_crater_shape* kernel, tg_crater_record** records, size_t* record_count,
+                        float** data, size_t* data_count, double** footprint, char* err,
+                        size_t err_len);
+
 #ifdef __cplusplus
 }
 #endif

@@ -1,0 +1,268 @@
+// SPDX-License-Identifier: BSD-3-Clause
+//
+// Crater (and boulder-candidate) placement on a plane, drawn cell by cell in power-of-two size
+// bands exactly as the reference draws them (src/layers/craters/generation/population.cpp), so
+// the same seed gives the same list, in the same order, on any grid or tiling:
+//   bands      [2^k, 2^(k+1)) clipped to [minDiameter, maxDiameter), drawn in square cells
+//              cellSizeFactor x 2^(k+1) meters across (population.cpp:348-356);
+//   per cell   a Philox stream keyed (seed, kCraterStream, i, j, k) gives a Poisson count of the
+//              band's expected craters, then per crater a diameter by inverse CDF and a uniform
+//              centre; a second stream keyed by kCraterAgeStream gives ages; IDs are
+//              Philox((i, j, k, index), (seed, kCraterIdStream)) (population.cpp:381-408,24-31);
+//   keeping    craters centred in the region grown by marginRadii of their radius, in band, cell
+//              and index order (population.cpp:174-259).
+// Densities are power laws N(>=D) = c D^-e per m^2 (include/terraingen/distribution/power_law.h).
+
+#include "common.h"
+#include "placement.h"
+#include "random.h"
+
+#include <cstdlib>
+#include <memory>
+
+namespace tgh {
+
+double PowerLawDensity::ExpectedCount(double area, double minD, double maxD) const {
+    if (!(minD > 0.0 && std::isfinite(minD) && maxD > minD) || std::isnan(maxD)) {
+        throw std::invalid_argument("Diameter interval must satisfy 0 < minDiameter < maxDiameter");
+    }
+    const double upper = std::isinf(maxD) ? 0.0 : Cumulative(maxD);
+    return area * (Cumulative(minD) - upper);
+}
+
+// SizeFrequencyDistribution::SampleDiameter (src/distribution/size_frequency.cpp:131-140) with
+// the power law's closed-form inverse.
+double PowerLawDensity::SampleDiameter(Stream& random, double minD, double maxD) const {
+    const double count = ExpectedCount(1.0, minD, maxD);
+    if (!(count > 0.0)) {
+        throw std::invalid_argument("The distribution has no mass in the diameter interval");
+    }
+    const double top = Cumulative(minD);
+    const double target = top - random.NextDouble() * count;
+    return std::clamp(Invert(target), minD, maxD);
+}
+
+double Ages::Sample(Stream& random) const {
+    if (lifetimeGyr == 0.0) {
+        return 0.0;
+    }
+    return std::clamp(random.NextDouble() * lifetimeGyr, 0.0, lifetimeGyr);
+}
+
+std::vector<Band> CraterBands(double minD, double maxD, double cellSizeFactor) {
+    std::vector<Band> bands;
+    for (int k = std::ilogb(minD); std::ldexp(1.0, k) < maxD; ++k) {
+        const double low = std::fmax(minD, std::ldexp(1.0, k));
+        const double high = std::fmin(maxD, std::ldexp(1.0, k + 1));
+        bands.push_back({k, low, high, cellSizeFactor * std::ldexp(1.0, k + 1)});
+    }
+    return bands;
+}
+
+namespace {
+
+struct Box {
+    double minX, minY, maxX, maxY;
+    Box Expanded(double m) const {
+        return {minX - m, minY - m, maxX + m, maxY + m};
+    }
+    // Half-open, as Bounds2d::Contains (geometry.h:146-148).
+    bool Contains(double x, double y) const {
+        return x >= minX && x < maxX && y >= minY && y < maxY;
+    }
+};
+
+std::int32_t CellCoordinate(double value) {
+    if (!(std::fabs(value) < 2147483647.0)) {
+        throw std::invalid_argument("Region is too far from the origin for its cell size");
+    }
+    return static_cast<std::int32_t>(value);
+}
+
+struct CellIndex {
+    std::int32_t i, j;
+};
+
+// PlanarCells::CellsCovering (src/geometry/planar_cells.cpp): row-major, increasing j then i.
+std::vector<CellIndex> CellsCovering(const Box& r, double cellSize) {
+    const std::int32_t iMin = CellCoordinate(std::floor(r.minX / cellSize));
+    const std::int32_t jMin = CellCoordinate(std::floor(r.minY / cellSize));
+    const std::int32_t iMax = CellCoordinate(std::ceil(r.maxX / cellSize) - 1.0);
+    const std::int32_t jMax = CellCoordinate(std::ceil(r.maxY / cellSize) - 1.0);
+    std::vector<CellIndex> cells;
+    for (std::int32_t j = jMin; j <= jMax; ++j) {
+        for (std::int32_t i = iMin; i <= iMax; ++i) {
+            cells.push_back({i, j});
+        }
+    }
+    return cells;
+}
+
+class Generator {
+  public:
+    Generator(const Box& region, const PowerLawDensity& density, const Ages& ages,
+              std::uint32_t seed, const GenerationOptions& o)
+        : m_region(region), m_density(density), m_ages(ages), m_seed(seed), m_o(o) {
+        if (!(o.maxDiameter > o.minDiameter) || !std::isfinite(o.maxDiameter)) {
+            throw std::invalid_argument("maxDiameter must be finite and exceed minDiameter");
+        }
+        if (!(o.minDiameter > 0.0)) {
+            throw std::invalid_argument("minDiameter must be positive");
+        }
+    }
+
+    std::vector<Crater> Run() const {
+        struct Planned {
+            Band band;
+            std::vector<CellIndex> cells;
+            Box inner, outer;
+        };
+        std::vector<Planned> plan;
+        double expected = 0.0;
+        for (const Band& band : CraterBands(m_o.minDiameter, m_o.maxDiameter, m_o.cellSizeFactor)) {
+            if (band.high <= m_o.minResolvedDiameter) {
+                continue;
+            }
+            Planned p{band, CellsCovering(m_region.Expanded(Margin(band.high)), band.cellSize),
+                      m_region.Expanded(Margin(band.low)), m_region.Expanded(Margin(band.high))};
+            const double perArea = m_density.ExpectedCount(1.0, band.low, band.high);
+            expected += perArea * band.cellSize * band.cellSize * double(p.cells.size());
+            plan.push_back(std::move(p));
+        }
+        if (expected > m_o.maxExpectedCraters) {
+            throw std::invalid_argument("Expected crater count " + std::to_string(expected) +
+                                        " exceeds maxExpectedCraters");
+        }
+        std::vector<Crater> kept;
+        kept.reserve(static_cast<std::size_t>(std::min(expected, 1e8)));
+        for (const Planned& p : plan) {
+            for (const CellIndex& cell : p.cells) {
+                DrawKept(p.band, p.inner, p.outer, cell, kept);
+            }
+        }
+        return kept;
+    }
+
+  private:
+    double Margin(double diameter) const {
+        return 0.5 * m_o.marginRadii * diameter;
+    }
+
+    bool InRegion(const Crater& c, double margin) const {
+        return m_region.Expanded(margin).Contains(c.x, c.y);
+    }
+
+    // GenerateCellCraters (population.cpp:381-408) for planar cells.
+    std::vector<Crater> DrawCell(const Band& band, const CellIndex& cell) const {
+        const auto k = static_cast<std::uint32_t>(band.exponent);
+        const auto i = static_cast<std::uint32_t>(cell.i);
+        const auto j = static_cast<std::uint32_t>(cell.j);
+        Stream geometry(m_seed, kCraterStream, i, j, k);
+        Stream ageStream(m_seed, kCraterAgeStream, i, j, k);
+        const double perArea = m_density.ExpectedCount(1.0, band.low, band.high);
+        const double area = band.cellSize * band.cellSize;
+        const std::uint64_t count = SamplePoisson(geometry, perArea * area);
+        std::vector<Crater> out;
+        out.reserve(count);
+        for (std::uint64_t index = 0; index < count; ++index) {
+            const double d = m_density.SampleDiameter(geometry, band.low, band.high);
+            const double u = geometry.NextDouble();
+            const double v = geometry.NextDouble();
+            const double x = (cell.i + u) * band.cellSize;
+            const double y = (cell.j + v) * band.cellSize;
+            const double age = m_ages.Sample(ageStream);
+            const Words4 id = Philox({i, j, k, static_cast<std::uint32_t>(index)}, m_seed,
+                                     kCraterIdStream);
+            out.push_back({x, y, d, age, JoinWords(id.x, id.y)});
+        }
+        return out;
+    }
+
+    // Generation::DrawKept, OutsideBand and InBandRegion (population.cpp:202-238): the cell's
+    // craters in drawn order restricted to the rows its band can reach, then kept by region.
+    void DrawKept(const Band& band, const Box& inner, const Box& outer, const CellIndex& cell,
+                  std::vector<Crater>& kept) const {
+        const std::vector<Crater> drawn = DrawCell(band, cell);
+        double largest = 0.0;
+        for (const Crater& c : drawn) {
+            largest = std::fmax(largest, c.diameter);
+        }
+        const Box reach = largest > band.high ? m_region.Expanded(Margin(largest)) : outer;
+        for (const Crater& c : drawn) {
+            if (!(c.y >= reach.minY && c.y <= reach.maxY)) {
+                continue;  // CellCraters::Rows keeps y in [minY, maxY]
+            }
+            const bool inBand = c.diameter >= band.low && c.diameter <= band.high;
+            if (inBand && !outer.Contains(c.x, c.y)) {
+                continue;
+            }
+            bool in;
+            if (!inBand) {
+                in = InRegion(c, Margin(c.diameter));
+            } else {
+                in = inner.Contains(c.x, c.y) || InRegion(c, Margin(c.diameter));
+            }
+            if (in) {
+                kept.push_back(c);
+            }
+        }
+    }
+
+    Box m_region;
+    const PowerLawDensity& m_density;
+    const Ages& m_ages;
+    std::uint32_t m_seed;
+    GenerationOptions m_o;
+};
+
+}  // namespace
+
+std::vector<Crater> GenerateCraters(double minX, double minY, double maxX, double maxY,
+                                    const PowerLawDensity& density, const Ages& ages,
+                                    std::uint32_t seed, const GenerationOptions& options) {
+    if (!(maxX > minX && maxY > minY) || !std::isfinite(minX + minY + maxX + maxY)) {
+        throw std::invalid_argument("Region must be finite and not empty");
+    }
+    return Generator({minX, minY, maxX, maxY}, density, ages, seed, options).Run();
+}
+
+}  // namespace tgh
+
+extern "C" {
+
+void tgx_philox4x32(const uint32_t* counter, const uint32_t* key, uint32_t* out) {
+    const tgh::Words4 r = tgh::Philox({counter[0], counter[1], counter[2], counter[3]}, key[0],
+                                      key[1]);
+    out[0] = r.x;
+    out[1] = r.y;
+    out[2] = r.z;
+    out[3] = r.w;
+}
+
+int tgx_generate_craters(const tgx_generation* g, tgx_crater** out, size_t* count, char* err,
+                         size_t errLen) {
+    return tgh::Guard(err, errLen, [&] {
+        if (g == nullptr || out == nullptr || count == nullptr) {
+            throw std::invalid_argument("tgx_generate_craters: null argument");
+        }
+        const tgh::PowerLawDensity density{g->sfd_coefficient, g->sfd_exponent};
+        const tgh::Ages ages{g->lifetime_gyr};
+        tgh::GenerationOptions o;
+        o.minDiameter = g->min_diameter;
+        o.maxDiameter = g->max_diameter;
+        o.marginRadii = g->margin_radii;
+        o.cellSizeFactor = g->cell_size_factor > 0.0 ? g->cell_size_factor : 16.0;
+        o.minResolvedDiameter = g->min_resolved_diameter;
+        const std::vector<tgh::Crater> craters = tgh::GenerateCraters(
+            g->min_x, g->min_y, g->max_x, g->max_y, density, ages, g->seed, o);
+        std::vector<tgx_crater> flat(craters.size());
+        for (std::size_t n = 0; n < craters.size(); ++n) {
+            flat[n] = {craters[n].x, craters[n].y, craters[n].diameter, craters[n].ageGyr,
+                       craters[n].id};
+        }
+        *out = tgh::Export(flat);
+        *count = flat.size();
+    });
+}
+
+}  // extern "C"

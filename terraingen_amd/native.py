@@ -351,3 +351,68 @@ def record_reach(records: np.ndarray, data: np.ndarray, reach_radii: float) -> n
         out[i] = lib.tgx_outline_max_scale(C.byref(rec), data.ctypes.data) * reach_radii * (
             0.5 * float(records["diameter"][i]))
     return out
+
+
+CRATER_DTYPE = np.dtype([("x", np.float64), ("y", np.float64), ("diameter", np.float64),
+                         ("age_gyr", np.float64), ("id", np.uint64)])
+
+Generation = _struct("Generation", [(n, C.c_double) for n in (
+    "min_x", "min_y", "max_x", "max_y", "sfd_coefficient", "sfd_exponent", "min_diameter",
+    "max_diameter", "margin_radii", "cell_size_factor", "min_resolved_diameter",
+    "lifetime_gyr")] + [("seed", u32)])
+
+HOST_ABI["tgx_philox4x32"] = (None, [VP, VP, VP])
+HOST_ABI["tgx_generate_craters"] = (C.c_int, [C.POINTER(Generation), C.POINTER(VP),
+                                              C.POINTER(SZ), CP, SZ])
+
+
+def generate_craters(spec: Generation) -> np.ndarray:
+    """``tgx_generate_craters``: a structured array of x, y, diameter, age_gyr, id."""
+    lib = host_library()
+    ptr, n = VP(), SZ()
+    err = C.create_string_buffer(512)
+    _check(lib.tgx_generate_craters(C.byref(spec), C.byref(ptr), C.byref(n), err, 512), err)
+    try:
+        raw = C.string_at(ptr.value, n.value * CRATER_DTYPE.itemsize)
+    finally:
+        lib.tgx_free(ptr)
+    return np.frombuffer(raw, CRATER_DTYPE).copy()
+
+
+CraterModels = _struct("CraterModels", [("profile", u32), ("complex_", u32),
+                                        ("transition_diameter", C.c_double),
+                                        ("lifetime_gyr", C.c_double)])
+Apply = _struct("Apply", [("irregular", u32), ("peaks", u32), ("terraces", u32)] + [
+    (n, C.c_double) for n in ("rim_sigma", "noise_amplitude", "noise_rim_amplitude",
+                              "wall_streaks", "ejecta")])
+
+HOST_ABI["tgx_margin_radii"] = (C.c_double, [C.POINTER(CraterModels), C.POINTER(Apply)])
+HOST_ABI["tgx_prepare_craters"] = (C.c_int, [
+    VP, SZ, C.POINTER(CraterModels), C.POINTER(Apply), u32, C.c_double, C.POINTER(CraterShape),
+    C.POINTER(VP), C.POINTER(SZ), C.POINTER(VP), C.POINTER(SZ), C.POINTER(VP), CP, SZ])
+
+
+def prepare_craters(craters: np.ndarray, models, apply, layer_seed: int, spacing: float):
+    """``tgx_prepare_craters``: (kernel, records, data, footprint[n, 3])."""
+    lib = host_library()
+    craters = np.ascontiguousarray(craters, CRATER_DTYPE)
+    kernel = CraterShape()
+    rp, dp, fp = VP(), VP(), VP()
+    rn, dn = SZ(), SZ()
+    err = C.create_string_buffer(512)
+    _check(lib.tgx_prepare_craters(craters.ctypes.data, craters.size, C.byref(models),
+                                   C.byref(apply), layer_seed, spacing, C.byref(kernel),
+                                   C.byref(rp), C.byref(rn), C.byref(dp), C.byref(dn),
+                                   C.byref(fp), err, 512), err)
+    try:
+        records = np.frombuffer(C.string_at(rp.value, rn.value * 144), RECORD_DTYPE).copy()
+        data = np.frombuffer(C.string_at(dp.value, dn.value * 4), np.float32).copy()
+        foot = np.frombuffer(C.string_at(fp.value, rn.value * 24), np.float64).reshape(-1, 3).copy()
+    finally:
+        for p in (rp, dp, fp):
+            lib.tgx_free(p)
+    return kernel, records, data, foot
+
+
+def margin_radii(models, apply) -> float:
+    return host_library().tgx_margin_radii(C.byref(models), C.byref(apply))
