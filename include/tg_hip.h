@@ -101,6 +101,33 @@ int tg_hip_noise_launch(tg_dev* dev, const tg_noise_plan* plan, float* heights_d
                         const float* xs_dev, const float* ys_dev, const tg_kernel_params* params,
                         void* stream, char* err, size_t err_len);
 
+/* SyntheticSource::NoiseAt (src/sources/synthetic_source.cpp:76-80) on the device:
+ * heights[r * width + c] = elevation + stack(xs[c], ys[r]), writing every cell of the strip. The
+ * reference evaluates this source on the CPU only. */
+int tg_hip_noise_fill(tg_dev* dev, const tg_noise_plan* plan, float elevation, float* heights_dev,
+                      const float* xs_dev, const float* ys_dev, const tg_kernel_params* params,
+                      void* stream, char* err, size_t err_len);
+
+/* A crater batch resident in device memory: the kernel struct, records, data, tile heads and
+ * entries, frames and per-column/per-row remainders, bound as craters.slang binds them
+ * (src/layers/craters/shaping/apply.cpp:181-188). `lows` holds `width` column remainders then one
+ * per grid row. Frames may be null on a plane. */
+typedef struct tg_crater_set tg_crater_set;
+
+int tg_hip_crater_set_create(tg_dev* dev, const tg_crater_shape* kernel,
+                             const tg_crater_record* records, size_t record_count,
+                             const float* data, size_t data_count, const uint32_t* tiles,
+                             size_t tile_words, const uint32_t* entries, size_t entry_count,
+                             const tg_crater_frame* frames, size_t frame_count, const float* lows,
+                             size_t low_count, tg_crater_set** out, char* err, size_t err_len);
+int tg_hip_crater_set_free(tg_crater_set* set);
+
+/* craters.slang's main() on a plane strip with device pointers: heights_dev (params->rows x
+ * params->width) updated in place, xs_dev per column, ys_dev per strip row. */
+int tg_hip_crater_launch(tg_dev* dev, const tg_crater_set* set, float* heights_dev,
+                         const float* xs_dev, const float* ys_dev, const tg_kernel_params* params,
+                         void* stream, char* err, size_t err_len);
+
 /* ---- Device memory, streams and timing --------------------------------------------------
  * No counterpart in the reference. For callers without a HIP runtime of their own (the tests,
  * the measurements, hosts written in other languages): one process should hold one HIP runtime,

@@ -25,6 +25,20 @@ struct tg_dev {
     std::mutex mutex;  // runs are serialized, as ComputeDevice requires (src/gpu/device.h:48)
 };
 
+struct tg_crater_set {
+    tg_dev* dev = nullptr;
+    tg_crater_shape* kernel = nullptr;
+    tg_crater_record* records = nullptr;
+    float* data = nullptr;
+    uint32_t* tiles = nullptr;
+    uint32_t* entries = nullptr;
+    tg_crater_frame* frames = nullptr;
+    tg_split_point* points = nullptr;
+    float* lows = nullptr;
+    uint32_t tileSide = 16;
+    bool onBody = false;
+};
+
 struct tg_noise_plan {
     tg_dev* dev = nullptr;
     tg::NoisePlan* device = nullptr;
@@ -101,8 +115,23 @@ __global__ void __launch_bounds__(kNoiseBlockX * kNoiseBlockY)
     heights[i] = height + tg::EvaluateStack(*plan, xs[col], ys[row]);
 }
 
+// SyntheticSource::NoiseAt: heights[i] = elevation + EvaluateFractals(xs[col], ys[row], stack).
+__global__ void __launch_bounds__(kNoiseBlockX * kNoiseBlockY)
+    NoiseFillStrip(float* __restrict__ heights, const float* __restrict__ xs,
+                   const float* __restrict__ ys, const tg::NoisePlan* __restrict__ plan,
+                   float elevation, tg_kernel_params params) {
+    const uint32_t col = blockIdx.x * kNoiseBlockX + threadIdx.x;
+    const uint32_t row = blockIdx.y * kNoiseBlockY + threadIdx.y;
+    if (col >= params.width || row >= params.rows) {
+        return;
+    }
+    heights[size_t(row) * params.width + col] =
+        elevation + tg::EvaluateStack(*plan, xs[col], ys[row]);
+}
+
 void LaunchNoise(const tg::NoisePlan* plan, float* heights, const float* xs, const float* ys,
-                 const tg_kernel_params& params, hipStream_t stream) {
+                 const tg_kernel_params& params, hipStream_t stream, bool fill = false,
+                 float elevation = 0.0f) {
     if (params.width == 0 || params.rows == 0) {
         return;
     }
@@ -112,7 +141,12 @@ void LaunchNoise(const tg::NoisePlan* plan, float* heights, const float* xs, con
     if (grid.y > 65535u * 64u) {
         throw HipError{"strip has too many rows for one launch"};
     }
-    hipLaunchKernelGGL(NoiseLayerStrip, grid, block, 0, stream, heights, xs, ys, plan, params);
+    if (fill) {
+        hipLaunchKernelGGL(NoiseFillStrip, grid, block, 0, stream, heights, xs, ys, plan,
+                           elevation, params);
+    } else {
+        hipLaunchKernelGGL(NoiseLayerStrip, grid, block, 0, stream, heights, xs, ys, plan, params);
+    }
     Check(hipGetLastError(), "launching the noise kernel");
 }
 
@@ -168,6 +202,20 @@ __global__ void __launch_bounds__(kCraterTile * kCraterTile)
         }
     }
     heights[i] = h;
+}
+
+void LaunchCraters(const tg_crater_set& set, float* heights, const float* xs, const float* ys,
+                   const tg_kernel_params& params, hipStream_t stream) {
+    if (params.width == 0 || params.rows == 0) {
+        return;
+    }
+    const dim3 block(kCraterTile, kCraterTile, 1);
+    const dim3 grid((params.width + kCraterTile - 1) / kCraterTile,
+                    (params.rows + kCraterTile - 1) / kCraterTile, 1);
+    hipLaunchKernelGGL(CraterStrip, grid, block, 0, stream, heights, xs, ys, set.kernel,
+                       set.records, set.data, set.tiles, set.entries, set.frames, set.points,
+                       set.lows, params);
+    Check(hipGetLastError(), "launching the crater kernel");
 }
 
 template <class F> int Guard(char* err, size_t errLen, F body) {
@@ -377,6 +425,113 @@ int tg_hip_noise_launch(tg_dev* dev, const tg_noise_plan* plan, float* heights, 
     }
     return Guard(err, errLen, [&] {
         LaunchNoise(plan->device, heights, xs, ys, *params, static_cast<hipStream_t>(stream));
+    });
+}
+
+}  // extern "C"
+
+namespace {
+
+template <class T> T* Upload(const T* src, size_t count, std::vector<void*>& owned) {
+    const size_t bytes = (count > 0 ? count : 1) * sizeof(T);
+    void* p = nullptr;
+    Check(hipMalloc(&p, bytes), "hipMalloc");
+    owned.push_back(p);
+    if (src != nullptr && count > 0) {
+        Check(hipMemcpy(p, src, count * sizeof(T), hipMemcpyHostToDevice), "hipMemcpy to device");
+    } else {
+        Check(hipMemset(p, 0, bytes), "hipMemset");
+    }
+    return static_cast<T*>(p);
+}
+
+}  // namespace
+
+extern "C" {
+
+int tg_hip_crater_set_create(tg_dev* dev, const tg_crater_shape* kernel,
+                             const tg_crater_record* records, size_t recordCount,
+                             const float* data, size_t dataCount, const uint32_t* tiles,
+                             size_t tileWords, const uint32_t* entries, size_t entryCount,
+                             const tg_crater_frame* frames, size_t frameCount, const float* lows,
+                             size_t lowCount, tg_crater_set** out, char* err, size_t errLen) {
+    if (dev == nullptr || kernel == nullptr || out == nullptr || tiles == nullptr ||
+        tileWords < 3 || (recordCount > 0 && records == nullptr)) {
+        SetErr(err, errLen, "tg_hip_crater_set_create: null argument or short tile block");
+        return TG_ERR_ARGUMENT;
+    }
+    if (tiles[1] == 0) {
+        SetErr(err, errLen, "tg_hip_crater_set_create: tile side must be positive");
+        return TG_ERR_ARGUMENT;
+    }
+    *out = nullptr;
+    std::vector<void*> owned;
+    const int code = Guard(err, errLen, [&] {
+        Check(hipSetDevice(dev->ordinal), "hipSetDevice");
+        auto set = std::make_unique<tg_crater_set>();
+        set->dev = dev;
+        set->kernel = Upload(kernel, 1, owned);
+        set->records = Upload(records, recordCount, owned);
+        set->data = Upload(data, dataCount, owned);
+        set->tiles = Upload(tiles, tileWords, owned);
+        set->entries = Upload(entries, entryCount, owned);
+        set->frames = Upload(frames, frameCount, owned);
+        set->points = Upload<tg_split_point>(nullptr, 1, owned);
+        set->lows = Upload(lows, lowCount, owned);
+        set->tileSide = tiles[1];
+        set->onBody = tiles[2] != 0;
+        owned.clear();
+        *out = set.release();
+    });
+    for (void* p : owned) {
+        (void)hipFree(p);
+    }
+    return code;
+}
+
+int tg_hip_crater_set_free(tg_crater_set* set) {
+    if (set == nullptr) {
+        return TG_OK;
+    }
+    (void)hipSetDevice(set->dev->ordinal);
+    for (void* p : {static_cast<void*>(set->kernel), static_cast<void*>(set->records),
+                    static_cast<void*>(set->data), static_cast<void*>(set->tiles),
+                    static_cast<void*>(set->entries), static_cast<void*>(set->frames),
+                    static_cast<void*>(set->points), static_cast<void*>(set->lows)}) {
+        (void)hipFree(p);
+    }
+    delete set;
+    return TG_OK;
+}
+
+int tg_hip_crater_launch(tg_dev* dev, const tg_crater_set* set, float* heights, const float* xs,
+                         const float* ys, const tg_kernel_params* params, void* stream, char* err,
+                         size_t errLen) {
+    if (dev == nullptr || set == nullptr || params == nullptr || heights == nullptr ||
+        xs == nullptr || ys == nullptr) {
+        SetErr(err, errLen, "tg_hip_crater_launch: null argument");
+        return TG_ERR_ARGUMENT;
+    }
+    if (set->onBody) {
+        SetErr(err, errLen, "tg_hip_crater_launch: the set is binned on a body; use the batched path");
+        return TG_ERR_ARGUMENT;
+    }
+    return Guard(err, errLen, [&] {
+        LaunchCraters(*set, heights, xs, ys, *params, static_cast<hipStream_t>(stream));
+    });
+}
+
+int tg_hip_noise_fill(tg_dev* dev, const tg_noise_plan* plan, float elevation, float* heights,
+                      const float* xs, const float* ys, const tg_kernel_params* params,
+                      void* stream, char* err, size_t errLen) {
+    if (dev == nullptr || plan == nullptr || params == nullptr || heights == nullptr ||
+        xs == nullptr || ys == nullptr) {
+        SetErr(err, errLen, "tg_hip_noise_fill: null argument");
+        return TG_ERR_ARGUMENT;
+    }
+    return Guard(err, errLen, [&] {
+        LaunchNoise(plan->device, heights, xs, ys, *params, static_cast<hipStream_t>(stream),
+                    true, elevation);
     });
 }
 

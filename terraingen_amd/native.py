@@ -99,6 +99,12 @@ HIP_ABI = {
                                            SZ]),
     "tg_hip_noise_plan_free": (C.c_int, [VP]),
     "tg_hip_noise_launch": (C.c_int, [VP, VP, VP, VP, VP, C.POINTER(KernelParams), VP, CP, SZ]),
+    "tg_hip_noise_fill": (C.c_int, [VP, VP, C.c_float, VP, VP, VP, C.POINTER(KernelParams), VP,
+                                    CP, SZ]),
+    "tg_hip_crater_set_create": (C.c_int, [VP, C.POINTER(CraterShape), VP, SZ, VP, SZ, VP, SZ,
+                                           VP, SZ, VP, SZ, VP, SZ, C.POINTER(VP), CP, SZ]),
+    "tg_hip_crater_set_free": (C.c_int, [VP]),
+    "tg_hip_crater_launch": (C.c_int, [VP, VP, VP, VP, VP, C.POINTER(KernelParams), VP, CP, SZ]),
     "tg_hip_malloc": (C.c_int, [VP, SZ, C.POINTER(VP), CP, SZ]),
     "tg_hip_free": (C.c_int, [VP, VP]),
     "tg_hip_memcpy_h2d": (C.c_int, [VP, VP, VP, SZ, CP, SZ]),

@@ -41,13 +41,23 @@ CASES = {
                         lifetime=0.2, irregular=1, seed=7, rim_sigma=0.08,
                         noise_amplitude=0.01, noise_rim_amplitude=0.005, wall_streaks=0.004,
                         ejecta=0.3),
+    # The C2 recipe at 257 x 257: synthetic source of 8-octave Perlin fBm (512 m, lacunarity 2,
+    # gain 0.5, 20 m, seed 1) plus power-law craters, MPD with freshness degradation, seed 1.
+    "c2_synthetic": dict(width=257, height=257, pixel=0.5, center_x=300.0, center_y=-150.0,
+                         source_kind=1, fractal_count=1, craters=1, models=1,
+                         sfd_coefficient=0.05, sfd_exponent=2.0, min_d=1.0, max_d=200.0,
+                         lifetime=1.0, irregular=1, seed=1),
 }
+
+C2_FRACTAL = dict(seed=1, wavelength=512.0, amplitude=20.0, octaves=8, lacunarity=2.0, gain=0.5)
 
 
 def run_case(name: str, spec: dict) -> dict:
     ref = reference()
     assert ref is not None, "oracle/_ref is not built"
     case = RefCase(threads=8, **spec)
+    if spec.get("source_kind") == 1:
+        case.fractals[0] = N.make_fractal(**C2_FRACTAL)
     h = ref.ref_case_open(C.byref(case))
     if not h:
         raise RuntimeError(ref.ref_last_error().decode())
