@@ -855,14 +855,67 @@ __device__ __forceinline__ float TerracedRadii(const float* data, const tg_crate
     return inner + (from + shaped * (to - from)) * (1.0f - inner);
 }
 
+// KernelOutlineScaleAt for an outline of harmonics and an ellipse only (no rim table), without
+// inverse or forward trigonometry per harmonic: theta's cosine and sine are dx / d and dy / d, the
+// harmonics' cos(k theta + phi_k) follow from rotating (cos k theta, sin k theta) by theta and the
+// crater's precomputed (A_k cos phi_k, A_k sin phi_k) in `harm` (same slots as its data), and the
+// kept shares share^(k / 2) are powers of sqrt(share) by repeated multiplication. Equal to the
+// reference's form to within FP32 rounding of a dozen products. `ra` holds the crater's
+// cos(heading), sin(heading), sqrt(ellipticity) and its reciprocal.
+__device__ __forceinline__ float FastOutlineScale(const float* harm, float4 ra,
+                                                  const tg_crater_record& c, float dx, float dy,
+                                                  float d, float share) {
+    float ct = 1.0f, st = 0.0f;
+    if (d > 0.0f) {
+        const float inv = 1.0f / d;
+        ct = dx * inv;
+        st = dy * inv;
+    }
+    float scale = 1.0f;
+    if (c.ellipticity != 1.0f) {
+        const float cd = ct * ra.x + st * ra.y;  // cos(theta - heading)
+        const float sd = st * ra.x - ct * ra.y;  // sin(theta - heading)
+        const float u = cd / ra.z;
+        const float v = sd / ra.w;
+        scale = 1.0f / sqrtf(u * u + v * v);
+    }
+    float wobble = 1.0f;
+    if (share > 0.0f && c.harmonic_count > 0u) {
+        const bool partial = share < 1.0f;
+        const float q = partial ? sqrtf(share) : 1.0f;
+        float ck = 1.0f, sk = 0.0f, kept = 1.0f;
+        for (uint32_t k = 0; k < c.first_harmonic; ++k) {
+            const float next = ck * ct - sk * st;
+            sk = sk * ct + ck * st;
+            ck = next;
+            kept *= q;
+        }
+        const float* h = harm + c.harmonic_offset;
+        for (uint32_t n = 0u; n < c.harmonic_count; ++n) {
+            const float wave = h[2u * n] * ck - h[2u * n + 1u] * sk;
+            wobble += (partial ? kept : 1.0f) * wave;
+            const float next = ck * ct - sk * st;
+            sk = sk * ct + ck * st;
+            ck = next;
+            kept *= q;
+        }
+    }
+    return scale * wobble;
+}
+
 // KernelProfileDistance (craters.h:1034-1054).
 __device__ __forceinline__ float ProfileDistance(const float* data, const tg_crater_record& c,
-                                                 float dx, float dy) {
+                                                 float dx, float dy, const float* harm = nullptr,
+                                                 float4 ra = float4{1.0f, 0.0f, 1.0f, 1.0f}) {
     float distance = sqrtf(dx * dx + dy * dy);
     if (c.ellipticity != 1.0f || c.harmonic_count > 0u || c.table_count > 0u ||
         c.mark_count > 0u) {
         const float share = Smoothstep(distance / (0.5f * c.diameter)) * WornOutlineShare(c);
-        distance /= OutlineScaleAt(data, c, dx, dy, share);
+        if (harm != nullptr && c.table_count == 0u) {
+            distance /= FastOutlineScale(harm, ra, c, dx, dy, distance, share);
+        } else {
+            distance /= OutlineScaleAt(data, c, dx, dy, share);
+        }
     }
     if (c.mark_count > 0u) {
         const float radius = 0.5f * c.diameter;
@@ -898,10 +951,11 @@ __device__ __forceinline__ float SlopeRaisedShare(const tg_crater_record& c, flo
 
 // KernelShiftedDistance (craters.h:1077-1081).
 __device__ __forceinline__ float ShiftedDistance(const float* data, const tg_crater_record& c,
-                                                 float dx, float dy) {
+                                                 float dx, float dy, const float* harm = nullptr,
+                                                 float4 ra = float4{1.0f, 0.0f, 1.0f, 1.0f}) {
     const float shift =
         c.shift_x != 0.0f || c.shift_y != 0.0f ? SlopeShiftWeight(c, dx, dy) : 0.0f;
-    return ProfileDistance(data, c, dx - shift * c.shift_x, dy - shift * c.shift_y);
+    return ProfileDistance(data, c, dx - shift * c.shift_x, dy - shift * c.shift_y, harm, ra);
 }
 
 // KernelUpslopeHeight (craters.h:1085-1091).
@@ -995,11 +1049,12 @@ __device__ __forceinline__ float PeakRelief(const tg_crater_shape& k, const floa
 // ApplyCraterKernel (craters.h:1186-1212): `height` after crater `c` at (dx, dy) from its center.
 __device__ __forceinline__ float ApplyCrater(const CraterShape& shape, const float* data,
                                              const tg_crater_record& c, float height, float dx,
-                                             float dy) {
+                                             float dy, const float* harm = nullptr,
+                                             float4 ra = float4{1.0f, 0.0f, 1.0f, 1.0f}) {
     const tg_crater_shape& k = shape.k;
     const float diameter = c.diameter;
     const float radius = 0.5f * diameter;
-    const float distance = ShiftedDistance(data, c, dx, dy);
+    const float distance = ShiftedDistance(data, c, dx, dy, harm, ra);
     if (distance >= k.reach_radii * radius) {
         return height;
     }

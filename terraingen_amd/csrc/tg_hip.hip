@@ -35,6 +35,13 @@ struct tg_crater_set {
     tg_crater_frame* frames = nullptr;
     tg_split_point* points = nullptr;
     float* lows = nullptr;
+    float* cull = nullptr;  // per record: squared round distance (m^2) beyond which it is inert
+    float* harm = nullptr;  // A cos(phi), A sin(phi) in the data's harmonic slots
+    float4* outline = nullptr;  // per record: cos and sin of the heading, sqrt(ellipticity), 1/that
+    size_t pointCount = 0;
+    size_t lowCount = 0;
+    uint32_t tilesAcross = 0;
+    uint32_t tileRows = 0;
     uint32_t tileSide = 16;
     bool onBody = false;
 };
@@ -160,9 +167,15 @@ __global__ void __launch_bounds__(kCraterTile * kCraterTile)
                 const uint32_t* __restrict__ tiles, const uint32_t* __restrict__ entries,
                 const tg_crater_frame* __restrict__ frames,
                 const tg_split_point* __restrict__ points, const float* __restrict__ lows,
-                tg_kernel_params params) {
-    const uint32_t col = blockIdx.x * kCraterTile + threadIdx.x;
-    const uint32_t localRow = blockIdx.y * kCraterTile + threadIdx.y;
+                const float* __restrict__ cull, const float* __restrict__ harm,
+                const float4* __restrict__ outline, tg_kernel_params params) {
+    // Each 64-lane wave covers an 8 x 8 square of the 16 x 16 tile, so a disc-shaped reach
+    // leaves fewer waves partly active than rows of the tile would.
+    const uint32_t t = threadIdx.x;
+    const uint32_t wave = t >> 6u;
+    const uint32_t lane = t & 63u;
+    const uint32_t col = blockIdx.x * kCraterTile + (wave & 1u) * 8u + (lane & 7u);
+    const uint32_t localRow = blockIdx.y * kCraterTile + (wave >> 1u) * 8u + (lane >> 3u);
     if (col >= params.width || localRow >= params.rows) {
         return;
     }
@@ -191,7 +204,10 @@ __global__ void __launch_bounds__(kCraterTile * kCraterTile)
             const tg_crater_record& c = records[n];
             const float dx = tg::SplitDifference(p.x, p.x_low, c.x, c.x_low);
             const float dy = tg::SplitDifference(p.y, p.y_low, c.y, c.y_low);
-            h = tg::ApplyCrater(shape, data, c, h, dx, dy);
+            if (dx * dx + dy * dy >= cull[n]) {
+                continue;  // beyond any reach its outline allows: ApplyCrater returns h as is
+            }
+            h = tg::ApplyCrater(shape, data, c, h, dx, dy, harm, outline[n]);
         }
     } else {
         const tg_split_point p = points[i];
@@ -204,19 +220,86 @@ __global__ void __launch_bounds__(kCraterTile * kCraterTile)
     heights[i] = h;
 }
 
+// The squared round distance from a crater's centre beyond which ApplyCrater leaves every height
+// unchanged (craters.h:1186-1193): reach x radius x an upper bound of its outline scale
+// (KernelOutlineScaleAt, craters.h:772-808: the ellipse stretches by at most sqrt(ellipticity),
+// the wobble by at most 1 + sum|A_k| + max|table| + |table - broad|), with relative slack far
+// above FP32 rounding. Infinite, so never culled, for craters whose distance other terms can
+// shorten: terraces, marks and floor shifts.
+std::vector<float> CullRadii(const tg_crater_shape& k, const tg_crater_record* records, size_t n,
+                             const float* data, size_t dataCount) {
+    std::vector<float> cull(n > 0 ? n : 1, INFINITY);
+    for (size_t i = 0; i < n; ++i) {
+        const tg_crater_record& c = records[i];
+        if (c.terrace_count > 0 || c.mark_count > 0 || c.shift_x != 0.0f || c.shift_y != 0.0f) {
+            continue;
+        }
+        double amplitudes = 0.0;
+        for (uint32_t h = 0; h < c.harmonic_count; ++h) {
+            const size_t at = size_t(c.harmonic_offset) + 2 * h;
+            if (at >= dataCount) {
+                amplitudes = INFINITY;
+                break;
+            }
+            amplitudes += std::fabs(double(data[at]));
+        }
+        double table = 0.0;
+        for (uint32_t t = 0; t < c.table_count; ++t) {
+            const size_t at = size_t(c.table_offset) + t;
+            table = at < dataCount ? std::fmax(table, std::fabs(double(data[at]))) : INFINITY;
+        }
+        const double wobble =
+            c.table_count > 0 ? 1.0 + 2.0 * amplitudes + 2.0 * table : 1.0 + amplitudes;
+        const double scale = std::sqrt(std::fmax(double(c.ellipticity), 1.0)) * wobble;
+        const double r = double(k.reach_radii) * 0.5 * double(c.diameter) * scale * (1.0 + 1e-4);
+        if (std::isfinite(r)) {
+            cull[i] = float(r * r);
+        }
+    }
+    return cull;
+}
+
+// Per-crater outline terms the fast path reads: (A cos phi, A sin phi) in each harmonic's data slot
+// and (cos h, sin h, sqrt(ellipticity), 1 / sqrt(ellipticity)) per record, in FP64 then rounded.
+void OutlineTerms(const tg_crater_record* records, size_t n, const float* data, size_t dataCount,
+                  std::vector<float>& harm, std::vector<float4>& outline) {
+    harm.assign(dataCount > 0 ? dataCount : 1, 0.0f);
+    outline.assign(n > 0 ? n : 1, float4{1.0f, 0.0f, 1.0f, 1.0f});
+    for (size_t i = 0; i < n; ++i) {
+        const tg_crater_record& c = records[i];
+        for (uint32_t k = 0; k < c.harmonic_count; ++k) {
+            const size_t at = size_t(c.harmonic_offset) + 2 * k;
+            const double a = data[at], phi = data[at + 1];
+            harm[at] = float(a * std::cos(phi));
+            harm[at + 1] = float(a * std::sin(phi));
+        }
+        const double along = std::sqrt(double(c.ellipticity));
+        outline[i] = float4{float(std::cos(double(c.heading))), float(std::sin(double(c.heading))),
+                            float(along), float(1.0 / along)};
+    }
+}
+
 void LaunchCraters(const tg_crater_set& set, float* heights, const float* xs, const float* ys,
                    const tg_kernel_params& params, hipStream_t stream) {
     if (params.width == 0 || params.rows == 0) {
         return;
     }
-    const dim3 block(kCraterTile, kCraterTile, 1);
+    const dim3 block(kCraterTile * kCraterTile, 1, 1);
     const dim3 grid((params.width + kCraterTile - 1) / kCraterTile,
                     (params.rows + kCraterTile - 1) / kCraterTile, 1);
     hipLaunchKernelGGL(CraterStrip, grid, block, 0, stream, heights, xs, ys, set.kernel,
                        set.records, set.data, set.tiles, set.entries, set.frames, set.points,
-                       set.lows, params);
+                       set.lows, set.cull, set.harm, set.outline, params);
     Check(hipGetLastError(), "launching the crater kernel");
 }
+
+int CreateCraterSet(tg_dev* dev, const tg_crater_shape* kernel, const tg_crater_record* records,
+                    size_t recordCount, const float* data, size_t dataCount,
+                    const uint32_t* tiles, size_t tileWords, const uint32_t* entries,
+                    size_t entryCount, const tg_crater_frame* frames, size_t frameCount,
+                    const tg_split_point* points, size_t pointCount, const float* lows,
+                    size_t lowCount, tg_crater_set** out, char* err, size_t errLen);
+void CheckLaunch(const tg_crater_set& set, const tg_kernel_params& params);
 
 template <class F> int Guard(char* err, size_t errLen, F body) {
     try {
@@ -344,36 +427,28 @@ int tg_hip_run(tg_dev* dev, int kernelId, const void* kernel, size_t kernelBytes
             LaunchNoise(dPlan.ptr, dHeights.ptr, dXs.ptr, dYs.ptr, *params, nullptr);
         } else {
             // Binding order of craters.slang: records, data, tiles, entries, frames, points, lows.
-            std::vector<std::unique_ptr<DeviceArray<unsigned char>>> bound;
-            for (size_t b = 0; b < nExtra; ++b) {
-                const tg_buf& buf = extra[b];
-                size_t bytes = buf.bytes;
-                const unsigned char* src = static_cast<const unsigned char*>(buf.data);
-                if (buf.per_cell != 0 && buf.count > 0) {
-                    // A per-cell buffer is bound one strip at a time by the caller already.
-                    bytes = buf.bytes;
-                }
-                auto d = std::make_unique<DeviceArray<unsigned char>>(bytes > 0 ? bytes : 4);
-                if (bytes > 0 && src != nullptr) {
-                    d->Upload(src, bytes);
-                }
-                bound.push_back(std::move(d));
+            const tg_buf* b = extra;
+            const size_t frameCount = b[4].bytes / sizeof(tg_crater_frame);
+            const size_t pointCount = b[5].bytes / sizeof(tg_split_point);
+            tg_crater_set* set = nullptr;
+            char inner[512] = {0};
+            const int code = CreateCraterSet(
+                dev, static_cast<const tg_crater_shape*>(kernel),
+                static_cast<const tg_crater_record*>(b[0].data), b[0].bytes / sizeof(tg_crater_record),
+                static_cast<const float*>(b[1].data), b[1].bytes / sizeof(float),
+                static_cast<const uint32_t*>(b[2].data), b[2].bytes / sizeof(uint32_t),
+                static_cast<const uint32_t*>(b[3].data), b[3].bytes / sizeof(uint32_t),
+                static_cast<const tg_crater_frame*>(b[4].data), frameCount,
+                static_cast<const tg_split_point*>(b[5].data), pointCount,
+                static_cast<const float*>(b[6].data), b[6].bytes / sizeof(float), &set, inner,
+                sizeof inner);
+            if (code != TG_OK) {
+                throw HipError{inner};
             }
-            DeviceArray<tg_crater_shape> dKernel(1);
-            dKernel.Upload(kernel, sizeof(tg_crater_shape));
-            const dim3 block(kCraterTile, kCraterTile, 1);
-            const dim3 grid(uint32_t((width + kCraterTile - 1) / kCraterTile),
-                            uint32_t((rows + kCraterTile - 1) / kCraterTile), 1);
-            hipLaunchKernelGGL(CraterStrip, grid, block, 0, nullptr, dHeights.ptr, dXs.ptr,
-                               dYs.ptr, dKernel.ptr,
-                               reinterpret_cast<const tg_crater_record*>(bound[0]->ptr),
-                               reinterpret_cast<const float*>(bound[1]->ptr),
-                               reinterpret_cast<const uint32_t*>(bound[2]->ptr),
-                               reinterpret_cast<const uint32_t*>(bound[3]->ptr),
-                               reinterpret_cast<const tg_crater_frame*>(bound[4]->ptr),
-                               reinterpret_cast<const tg_split_point*>(bound[5]->ptr),
-                               reinterpret_cast<const float*>(bound[6]->ptr), *params);
-            Check(hipGetLastError(), "launching the crater kernel");
+            std::unique_ptr<tg_crater_set, int (*)(tg_crater_set*)> owner(set,
+                                                                          tg_hip_crater_set_free);
+            CheckLaunch(*set, *params);
+            LaunchCraters(*set, dHeights.ptr, dXs.ptr, dYs.ptr, *params, nullptr);
         }
         Check(hipDeviceSynchronize(), "running the kernel");
         Check(hipMemcpy(heights, dHeights.ptr, cells * sizeof(float), hipMemcpyDeviceToHost),
@@ -445,28 +520,86 @@ template <class T> T* Upload(const T* src, size_t count, std::vector<void*>& own
     return static_cast<T*>(p);
 }
 
-}  // namespace
-
-extern "C" {
-
-int tg_hip_crater_set_create(tg_dev* dev, const tg_crater_shape* kernel,
-                             const tg_crater_record* records, size_t recordCount,
-                             const float* data, size_t dataCount, const uint32_t* tiles,
-                             size_t tileWords, const uint32_t* entries, size_t entryCount,
-                             const tg_crater_frame* frames, size_t frameCount, const float* lows,
-                             size_t lowCount, tg_crater_set** out, char* err, size_t errLen) {
-    if (dev == nullptr || kernel == nullptr || out == nullptr || tiles == nullptr ||
-        tileWords < 3 || (recordCount > 0 && records == nullptr)) {
-        SetErr(err, errLen, "tg_hip_crater_set_create: null argument or short tile block");
-        return TG_ERR_ARGUMENT;
+// Every index the crater kernel follows, checked on the host so a malformed batch is an error,
+// never an out-of-bounds read: tile heads against the entries, entries against the records,
+// each record's data ranges against the data (craters.h:772-1182 read them), frames on a body.
+void ValidateCraterBatch(const tg_crater_shape& k, const tg_crater_record* records, size_t n,
+                         size_t dataCount, const uint32_t* tiles, size_t tileWords,
+                         const uint32_t* entries, size_t entryCount, size_t frameCount) {
+    const auto fail = [](const std::string& what) { throw HipError{"invalid crater batch: " + what}; };
+    if (tileWords < 3 || tiles[0] == 0 || tiles[1] == 0 || (tileWords - 3) % 2 != 0 ||
+        ((tileWords - 3) / 2) % tiles[0] != 0) {
+        fail("tile block must be [tiles across, side, body flag, (first, count) per tile]");
     }
-    if (tiles[1] == 0) {
-        SetErr(err, errLen, "tg_hip_crater_set_create: tile side must be positive");
+    for (size_t t = 3; t < tileWords; t += 2) {
+        if (uint64_t(tiles[t]) + tiles[t + 1] > entryCount) {
+            fail("a tile's entries run past the entry list");
+        }
+    }
+    for (size_t e = 0; e < entryCount; ++e) {
+        if (entries[e] >= n && !(n == 0 && entryCount == 1)) {
+            fail("an entry names a record past the end");
+        }
+    }
+    if (tiles[2] != 0 && frameCount < n) {
+        fail("a body batch needs one frame per record");
+    }
+    const tg_crater_profile& p = k.profile;
+    const auto span = [&](uint64_t offset, uint64_t count, const char* what) {
+        if (count > 0 && offset + count > dataCount) {
+            fail(std::string(what) + " run past the data");
+        }
+    };
+    if (p.diffused == 1u || p.simple == TG_PROFILE_TABULATED) {
+        if (p.table_cells == 0) {
+            fail("tabulated profiles need table cells");
+        }
+    }
+    if (p.diffused == 1u) {
+        if (p.age_count < 2) {
+            fail("diffused profiles need at least two ages");
+        }
+        span(p.ages_offset, p.age_count, "diffusion ages");
+    }
+    for (size_t i = 0; i < n; ++i) {
+        const tg_crater_record& c = records[i];
+        span(c.harmonic_offset, 2ull * c.harmonic_count, "outline harmonics");
+        span(c.table_offset, c.table_count, "outline tables");
+        if (c.mark_count > 0) {
+            span(c.mark_offset, 3ull + c.mark_count, "outline marks");
+        }
+        if (c.terrace_count > 0) {
+            span(c.terrace_offset,
+                 20ull + 8ull * (c.terrace_count - 1) + 3ull * c.terrace_count, "terraces");
+        }
+        if (c.peak_count > 0) {
+            span(c.peak_offset, 6ull + 11ull * c.peak_count, "peak massifs");
+        }
+        if (p.simple == TG_PROFILE_TABULATED) {
+            span(c.profile_offset, uint64_t(p.table_cells) + 1, "a tabulated profile row");
+        } else if (p.diffused == 1u && c.diffusion > 0.0f) {
+            span(c.profile_offset, uint64_t(p.age_count) * (uint64_t(p.table_cells) + 1),
+                 "a diffusion table");
+        }
+    }
+}
+
+int CreateCraterSet(tg_dev* dev, const tg_crater_shape* kernel, const tg_crater_record* records,
+                    size_t recordCount, const float* data, size_t dataCount,
+                    const uint32_t* tiles, size_t tileWords, const uint32_t* entries,
+                    size_t entryCount, const tg_crater_frame* frames, size_t frameCount,
+                    const tg_split_point* points, size_t pointCount, const float* lows,
+                    size_t lowCount, tg_crater_set** out, char* err, size_t errLen) {
+    if (dev == nullptr || kernel == nullptr || out == nullptr || tiles == nullptr ||
+        (recordCount > 0 && records == nullptr) || (entryCount > 0 && entries == nullptr)) {
+        SetErr(err, errLen, "crater set: null argument");
         return TG_ERR_ARGUMENT;
     }
     *out = nullptr;
     std::vector<void*> owned;
     const int code = Guard(err, errLen, [&] {
+        ValidateCraterBatch(*kernel, records, recordCount, dataCount, tiles, tileWords, entries,
+                            entryCount, frames != nullptr ? frameCount : 0);
         Check(hipSetDevice(dev->ordinal), "hipSetDevice");
         auto set = std::make_unique<tg_crater_set>();
         set->dev = dev;
@@ -475,10 +608,21 @@ int tg_hip_crater_set_create(tg_dev* dev, const tg_crater_shape* kernel,
         set->data = Upload(data, dataCount, owned);
         set->tiles = Upload(tiles, tileWords, owned);
         set->entries = Upload(entries, entryCount, owned);
-        set->frames = Upload(frames, frameCount, owned);
-        set->points = Upload<tg_split_point>(nullptr, 1, owned);
+        set->frames = Upload(frames, frames != nullptr ? frameCount : 0, owned);
+        set->points = Upload(points, points != nullptr ? pointCount : 0, owned);
+        set->pointCount = points != nullptr ? pointCount : 0;
         set->lows = Upload(lows, lowCount, owned);
+        set->lowCount = lowCount;
+        const std::vector<float> cull = CullRadii(*kernel, records, recordCount, data, dataCount);
+        set->cull = Upload(cull.data(), cull.size(), owned);
+        std::vector<float> harm;
+        std::vector<float4> outline;
+        OutlineTerms(records, recordCount, data, dataCount, harm, outline);
+        set->harm = Upload(harm.data(), harm.size(), owned);
+        set->outline = Upload(outline.data(), outline.size(), owned);
         set->tileSide = tiles[1];
+        set->tilesAcross = tiles[0];
+        set->tileRows = uint32_t((tileWords - 3) / 2 / tiles[0]);
         set->onBody = tiles[2] != 0;
         owned.clear();
         *out = set.release();
@@ -489,6 +633,40 @@ int tg_hip_crater_set_create(tg_dev* dev, const tg_crater_shape* kernel,
     return code;
 }
 
+// A launch's grid against the set's tiles and per-column/row buffers.
+void CheckLaunch(const tg_crater_set& set, const tg_kernel_params& params) {
+    if (params.rows == 0 || params.width == 0) {
+        return;
+    }
+    const uint64_t lastRow = uint64_t(params.first_row) + params.rows - 1;
+    if ((params.width - 1) / set.tileSide >= set.tilesAcross ||
+        lastRow / set.tileSide >= set.tileRows) {
+        throw HipError{"the strip reaches past the crater tiles"};
+    }
+    if (set.onBody) {
+        if (set.pointCount < size_t(params.rows) * params.width) {
+            throw HipError{"a body strip needs one split point per cell"};
+        }
+    } else if (set.lowCount < size_t(params.width) + lastRow + 1) {
+        throw HipError{"lows must hold every column, then every row up to the strip's last"};
+    }
+}
+
+}  // namespace
+
+extern "C" {
+
+int tg_hip_crater_set_create(tg_dev* dev, const tg_crater_shape* kernel,
+                             const tg_crater_record* records, size_t recordCount,
+                             const float* data, size_t dataCount, const uint32_t* tiles,
+                             size_t tileWords, const uint32_t* entries, size_t entryCount,
+                             const tg_crater_frame* frames, size_t frameCount, const float* lows,
+                             size_t lowCount, tg_crater_set** out, char* err, size_t errLen) {
+    return CreateCraterSet(dev, kernel, records, recordCount, data, dataCount, tiles, tileWords,
+                           entries, entryCount, frames, frameCount, nullptr, 0, lows, lowCount,
+                           out, err, errLen);
+}
+
 int tg_hip_crater_set_free(tg_crater_set* set) {
     if (set == nullptr) {
         return TG_OK;
@@ -497,7 +675,9 @@ int tg_hip_crater_set_free(tg_crater_set* set) {
     for (void* p : {static_cast<void*>(set->kernel), static_cast<void*>(set->records),
                     static_cast<void*>(set->data), static_cast<void*>(set->tiles),
                     static_cast<void*>(set->entries), static_cast<void*>(set->frames),
-                    static_cast<void*>(set->points), static_cast<void*>(set->lows)}) {
+                    static_cast<void*>(set->points), static_cast<void*>(set->lows),
+                    static_cast<void*>(set->cull), static_cast<void*>(set->harm),
+                    static_cast<void*>(set->outline)}) {
         (void)hipFree(p);
     }
     delete set;
@@ -517,6 +697,7 @@ int tg_hip_crater_launch(tg_dev* dev, const tg_crater_set* set, float* heights, 
         return TG_ERR_ARGUMENT;
     }
     return Guard(err, errLen, [&] {
+        CheckLaunch(*set, *params);
         LaunchCraters(*set, heights, xs, ys, *params, static_cast<hipStream_t>(stream));
     });
 }
