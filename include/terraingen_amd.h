@@ -15,6 +15,7 @@
 #include <stddef.h>
 #include <stdint.h>
 
+#include "tg_hip.h"
 #include "tg_layouts.h"
 
 #ifdef __cplusplus
@@ -113,12 +114,32 @@ double tgx_margin_radii(const tgx_crater_models* models, const tgx_apply* apply)
 /* PrepareCraters + MakeCraterKernelBatch over `craters` at sample `spacing` meters, the layer's
  * apply seeds derived from `layer_seed` (SeededCraterApply): the kernel struct, the records in
  * apply order (craters too small to resolve dropped), the data, and when `footprint` is not null
- * three doubles per record (centre x, centre y, footprint half-width). */
+ * four doubles per record (centre x, centre y, footprint half-width, diameter). */
 int tgx_prepare_craters(const tgx_crater* craters, size_t n, const tgx_crater_models* models,
                         const tgx_apply* apply, uint32_t layer_seed, double spacing,
                         tg_crater_shape* kernel, tg_crater_record** records, size_t* record_count,
                         float** data, size_t* data_count, double** footprint, char* err,
                         size_t err_len);
+
+/* A tile of a batch: a north-up grid `width` x `height` of `pixel` meters, west and north edges
+ * in FP64. */
+typedef struct tgx_tile {
+    double origin_x;
+    double origin_y;
+    double pixel;
+    uint32_t width;
+    uint32_t height;
+} tgx_tile;
+
+/* Bins a catalogue (crater centres, footprint half-widths and FP64 diameters, in apply order) into
+ * the 16 x 16 bins of every tile as ApplyCraters bins one grid (apply.cpp:119-140): craters the
+ * tile's pixel size cannot resolve are left out, the rest join the bins their footprint's pixel
+ * window meets, in catalogue order. Outputs the tile descriptors (heights packed one tile after
+ * another), the bins and the entries for tg_hip_batch_stage, over `threads` threads (0: all). */
+int tgx_batch_bin(const double* cx, const double* cy, const double* reach, const double* diameter,
+                  size_t n, const tgx_tile* tiles, size_t tile_count, unsigned threads,
+                  tg_tile_desc** descs, tg_bin** bins, size_t* bin_count, uint32_t** entries,
+                  size_t* entry_count, char* err, size_t err_len);
 
 #ifdef __cplusplus
 }

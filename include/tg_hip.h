@@ -128,6 +128,73 @@ int tg_hip_crater_launch(tg_dev* dev, const tg_crater_set* set, float* heights_d
                          const float* xs_dev, const float* ys_dev, const tg_kernel_params* params,
                          void* stream, char* err, size_t err_len);
 
+/* ---- Batched tiles ---------------------------------------------------------------------------
+ * Many tiles of uneven size and pixel size evaluated in one persistent launch: the source fill
+ * (SyntheticSource noise, or a constant) and crater shaping fused per 16 x 16 bin of cells, bins
+ * taken from a work queue by whichever workgroup is free. No counterpart in the reference, which
+ * evaluates one grid at a time through RunKernel. */
+
+/* A crater catalogue resident on the device: the kernel struct, one record per crater in apply
+ * order, the data, and each crater's FP64 diameter (its ResolutionWeight is worked out per tile
+ * from the tile's pixel size, src/layers/fade.cpp:48-52). */
+typedef struct tg_catalogue tg_catalogue;
+
+int tg_hip_catalogue_create(tg_dev* dev, const tg_crater_shape* kernel,
+                            const tg_crater_record* records, size_t record_count,
+                            const float* data, size_t data_count, const double* diameters,
+                            tg_catalogue** out, char* err, size_t err_len);
+int tg_hip_catalogue_free(tg_catalogue* catalogue);
+
+/* One tile: a north-up grid `width` x `height` of `pixel` meters, west and north edges in FP64
+ * (cell centres are worked out in FP64 on the device and narrowed as RunKernel narrows them,
+ * src/gpu/kernel.cpp:55-70), its heights at `out_offset` floats into the output, row by row, its
+ * bins from `bin_first`, `bins_across` per row of bins. */
+typedef struct tg_tile_desc {
+    double origin_x;
+    double origin_y;
+    double pixel;
+    uint32_t width;
+    uint32_t height;
+    uint64_t out_offset;
+    uint32_t bin_first;
+    uint32_t bins_across;
+} tg_tile_desc;
+
+/* A bin of 16 x 16 cells of tile `tile`: the catalogue craters entries[first .. first+count), in
+ * catalogue (apply) order. */
+typedef struct tg_bin {
+    uint32_t tile;
+    uint32_t first;
+    uint32_t count;
+    uint32_t pad;
+} tg_bin;
+
+typedef struct tg_batch tg_batch;
+
+/* Device buffers for batches of at most `max_tiles` tiles, `max_bins` bins, `max_entries`
+ * entries, with their own stream. */
+int tg_hip_batch_create(tg_dev* dev, size_t max_tiles, size_t max_bins, size_t max_entries,
+                        tg_batch** out, char* err, size_t err_len);
+int tg_hip_batch_free(tg_batch* batch);
+
+/* Copies a batch's tiles, bins and entries into its buffers, checking every index against the
+ * others and the capacities. */
+int tg_hip_batch_stage(tg_batch* batch, const tg_tile_desc* tiles, size_t tile_count,
+                       const tg_bin* bins, size_t bin_count, const uint32_t* entries,
+                       size_t entry_count, char* err, size_t err_len);
+
+/* Evaluates the staged batch into `out_dev` (at least `out_capacity` floats): each cell's height
+ * is `elevation` plus the plan's noise (none when `plan` is null) at its centre, then the
+ * catalogue's craters of its bin (none when `catalogue` is null). With `use_graph` the launch is
+ * captured once as a HIP graph for these arguments and replayed. Enqueued on the batch's stream;
+ * tg_hip_batch_wait waits for it. */
+int tg_hip_batch_run(tg_batch* batch, const tg_noise_plan* plan, float elevation,
+                     const tg_catalogue* catalogue, float* out_dev, size_t out_capacity,
+                     int use_graph, char* err, size_t err_len);
+int tg_hip_batch_wait(tg_batch* batch, char* err, size_t err_len);
+/* The batch's stream, for events and copies ordered with its runs. */
+void* tg_hip_batch_stream(tg_batch* batch);
+
 /* ---- Device memory, streams and timing --------------------------------------------------
  * No counterpart in the reference. For callers without a HIP runtime of their own (the tests,
  * the measurements, hosts written in other languages): one process should hold one HIP runtime,

@@ -1,0 +1,450 @@
+// SPDX-License-Identifier: BSD-3-Clause
+//
+// Batched tiles: one persistent launch evaluates many tiles of uneven size and pixel size. The
+// work is a queue of bins (16 x 16 cells of one tile); each workgroup takes the next bin with one
+// atomic add, evaluates its cells in registers (FP64 cell centres narrowed to FP32 as RunKernel
+// narrows them, the source's noise, then the bin's craters in catalogue order) and writes the
+// heights once. A workgroup keeps taking bins until the queue is empty, so tiles of any mix of sizes
+// fill every CU without a launch per tile, and nothing is read back or allocated between tiles.
+
+#include "crater_device.h"
+#include "internal.h"
+#include "noise_device.h"
+
+#include <hip/hip_runtime.h>
+
+#include <cmath>
+#include <cstring>
+
+using namespace tgi;
+
+struct tg_catalogue {
+    tg_dev* dev = nullptr;
+    tg_crater_shape* kernel = nullptr;
+    tg_crater_record* records = nullptr;
+    float* data = nullptr;
+    float* cull = nullptr;
+    float* harm = nullptr;
+    float4* outline = nullptr;
+    double* diameters = nullptr;
+    size_t count = 0;
+};
+
+struct tg_batch {
+    tg_dev* dev = nullptr;
+    hipStream_t stream = nullptr;
+    tg_tile_desc* tiles = nullptr;
+    tg_bin* bins = nullptr;
+    uint32_t* entries = nullptr;
+    uint32_t* counters = nullptr;  // [0] the queue's next bin, [1] the bin count
+    size_t maxTiles = 0, maxBins = 0, maxEntries = 0;
+    // What the staged batch needs of a run: the largest entry and output end.
+    size_t tileCount = 0, binCount = 0;
+    uint64_t maxEntry = 0, outEnd = 0;
+    bool anyEntries = false;
+    // The captured graph and the arguments it was captured for.
+    hipGraphExec_t graph = nullptr;
+    const void* graphPlan = nullptr;
+    const void* graphCatalogue = nullptr;
+    float* graphOut = nullptr;
+    float graphElevation = 0.0f;
+};
+
+namespace {
+
+constexpr uint32_t kBinSide = 16;
+constexpr uint32_t kGroupThreads = 256;  // four independent waves
+
+__global__ void ResetQueue(uint32_t* counters) {
+    counters[0] = 0u;
+}
+
+// ResolutionWeight (src/layers/fade.cpp:48-52) with the smoothstep fade from 2 to 4 samples
+// across, in FP64 as the reference computes it.
+__device__ __forceinline__ float ResolutionAt(double diameter, double pixel) {
+    const double x = diameter / pixel;
+    double falloff;
+    if (x <= 2.0) {
+        falloff = 1.0;
+    } else if (x >= 4.0) {
+        falloff = 0.0;
+    } else {
+        double t = (x - 2.0) / (4.0 - 2.0);
+        t = t < 0.0 ? 0.0 : (t > 1.0 ? 1.0 : t);
+        falloff = 1.0 - t * t * (3.0 - 2.0 * t);
+    }
+    return float(1.0 - falloff);
+}
+
+// One bin per wave: lane 0 takes the next bin with one atomic add and the index is broadcast
+// with readfirstlane, so every branch below is uniform across the wave and no wave waits for
+// another (the kernel has no block barrier). The wave covers the bin as four 8 x 8 squares, the
+// shape the strip kernel uses, so its crater-reach tests stay coherent across lanes.
+__global__ void __launch_bounds__(kGroupThreads)
+    BatchTiles(const tg_tile_desc* __restrict__ tiles, const tg_bin* __restrict__ bins,
+               const uint32_t* __restrict__ entries, uint32_t* __restrict__ counters,
+               const tg::NoisePlan* __restrict__ plan, float elevation,
+               const tg_crater_shape* __restrict__ kernel,
+               const tg_crater_record* __restrict__ records, const float* __restrict__ data,
+               const float* __restrict__ cull, const float* __restrict__ harm,
+               const float4* __restrict__ outline, const double* __restrict__ diameters,
+               float* __restrict__ out) {
+    const uint32_t binCount = counters[1];
+    const uint32_t lane = threadIdx.x & 63u;
+    for (;;) {
+        uint32_t taken = 0u;
+        if (lane == 0u) {
+            taken = atomicAdd(&counters[0], 1u);
+        }
+        const uint32_t b = __builtin_amdgcn_readfirstlane(taken);
+        if (b >= binCount) {
+            return;  // the queue only grows past binCount, so every wave ends here
+        }
+        const tg_bin bin = bins[b];
+        const tg_tile_desc tile = tiles[bin.tile];
+        const uint32_t local = b - tile.bin_first;
+        const uint32_t col0 = (local % tile.bins_across) * kBinSide;
+        const uint32_t row0 = (local / tile.bins_across) * kBinSide;
+        for (uint32_t square = 0; square < 4u; ++square) {
+            const uint32_t col = col0 + (square & 1u) * 8u + (lane & 7u);
+            const uint32_t row = row0 + (square >> 1u) * 8u + (lane >> 3u);
+            if (col < tile.width && row < tile.height) {
+                // GeoTransform::PixelToWorld of the cell centre in FP64, narrowed and split as
+                // ColumnOffsets/RowOffsets and MakeSplitPoint do (src/gpu/kernel.cpp:55-70,
+                // crater_kernel.cpp:229-236): the source origin is {0, 0}.
+                const double cx = tile.origin_x + (double(col) + 0.5) * tile.pixel + 0.0;
+                const double cy = tile.origin_y + (double(row) + 0.5) * -tile.pixel;
+                const float x = float(cx);
+                const float y = float(cy);
+                float h = plan != nullptr ? elevation + tg::EvaluateStack(*plan, x, y) : elevation;
+                if (kernel != nullptr && bin.count > 0u) {
+                    const float xLow = float(cx - double(x));
+                    const float yLow = float(cy - double(y));
+                    const tg::CraterShape shape(*kernel);
+                    for (uint32_t e = bin.first; e < bin.first + bin.count; ++e) {
+                        const uint32_t n = entries[e];
+                        const tg_crater_record& c = records[n];
+                        const float dx = tg::SplitDifference(x, xLow, c.x, c.x_low);
+                        const float dy = tg::SplitDifference(y, yLow, c.y, c.y_low);
+                        if (dx * dx + dy * dy < cull[n]) {
+                            const float weight = ResolutionAt(diameters[n], tile.pixel);
+                            h = tg::ApplyCrater(shape, data, c, h, dx, dy, harm, outline[n],
+                                                weight);
+                        }
+                    }
+                }
+                out[tile.out_offset + size_t(row) * tile.width + col] = h;
+            }
+        }
+    }
+}
+
+template <class T> T* Alloc(size_t count) {
+    void* p = nullptr;
+    Check(hipMalloc(&p, (count > 0 ? count : 1) * sizeof(T)), "hipMalloc");
+    return static_cast<T*>(p);
+}
+
+template <class T> T* UploadArray(const T* src, size_t count) {
+    T* p = Alloc<T>(count);
+    if (src != nullptr && count > 0) {
+        const hipError_t status = hipMemcpy(p, src, count * sizeof(T), hipMemcpyHostToDevice);
+        if (status != hipSuccess) {
+            (void)hipFree(p);
+            Check(status, "hipMemcpy to device");
+        }
+    } else {
+        const hipError_t status = hipMemset(p, 0, (count > 0 ? count : 1) * sizeof(T));
+        if (status != hipSuccess) {
+            (void)hipFree(p);
+            Check(status, "hipMemset");
+        }
+    }
+    return p;
+}
+
+void FreeCatalogue(tg_catalogue* c) {
+    for (void* p : {static_cast<void*>(c->kernel), static_cast<void*>(c->records),
+                    static_cast<void*>(c->data), static_cast<void*>(c->cull),
+                    static_cast<void*>(c->harm), static_cast<void*>(c->outline),
+                    static_cast<void*>(c->diameters)}) {
+        if (p != nullptr) {
+            (void)hipFree(p);
+        }
+    }
+}
+
+void Launch(tg_batch& b, const tg_noise_plan* plan, float elevation, const tg_catalogue* cat,
+            float* out) {
+    // One wave per bin up to a few workgroups per CU: every CU stays busy and each wave keeps
+    // taking bins until the queue is empty.
+    int cus = 256;
+    (void)hipDeviceGetAttribute(&cus, hipDeviceAttributeMultiprocessorCount, b.dev->ordinal);
+    const size_t wavesPerGroup = kGroupThreads / 64;
+    const uint32_t groups = uint32_t(std::max<size_t>(
+        1, std::min<size_t>((b.binCount + wavesPerGroup - 1) / wavesPerGroup, size_t(cus) * 8)));
+    hipLaunchKernelGGL(ResetQueue, dim3(1), dim3(1), 0, b.stream, b.counters);
+    hipLaunchKernelGGL(BatchTiles, dim3(groups), dim3(kGroupThreads), 0, b.stream, b.tiles, b.bins,
+                       b.entries, b.counters, plan != nullptr ? plan->device : nullptr,
+                       elevation, cat != nullptr ? cat->kernel : nullptr,
+                       cat != nullptr ? cat->records : nullptr,
+                       cat != nullptr ? cat->data : nullptr, cat != nullptr ? cat->cull : nullptr,
+                       cat != nullptr ? cat->harm : nullptr,
+                       cat != nullptr ? cat->outline : nullptr,
+                       cat != nullptr ? cat->diameters : nullptr, out);
+    Check(hipGetLastError(), "launching the batched tile kernel");
+}
+
+void DropGraph(tg_batch& b) {
+    if (b.graph != nullptr) {
+        (void)hipGraphExecDestroy(b.graph);
+        b.graph = nullptr;
+    }
+}
+
+}  // namespace
+
+extern "C" {
+
+int tg_hip_catalogue_create(tg_dev* dev, const tg_crater_shape* kernel,
+                            const tg_crater_record* records, size_t count, const float* data,
+                            size_t dataCount, const double* diameters, tg_catalogue** out,
+                            char* err, size_t errLen) {
+    if (dev == nullptr || kernel == nullptr || out == nullptr ||
+        (count > 0 && (records == nullptr || diameters == nullptr))) {
+        SetErr(err, errLen, "tg_hip_catalogue_create: null argument");
+        return TG_ERR_ARGUMENT;
+    }
+    *out = nullptr;
+    auto cat = std::make_unique<tg_catalogue>();
+    const int code = Guard(err, errLen, [&] {
+        // The same data checks as a crater set, with one bin holding every crater.
+        std::vector<uint32_t> tiles{1u, kBinSide, 0u, 0u, uint32_t(count)};
+        std::vector<uint32_t> all(count > 0 ? count : 1, 0u);
+        for (size_t i = 0; i < count; ++i) {
+            all[i] = uint32_t(i);
+        }
+        ValidateCraterBatch(*kernel, records, count, dataCount, tiles.data(), tiles.size(),
+                            all.data(), count > 0 ? count : 1, 0);
+        Check(hipSetDevice(dev->ordinal), "hipSetDevice");
+        cat->dev = dev;
+        cat->count = count;
+        cat->kernel = UploadArray(kernel, 1);
+        cat->records = UploadArray(records, count);
+        cat->data = UploadArray(data, dataCount);
+        const std::vector<float> cull = CullRadii(*kernel, records, count, data, dataCount);
+        cat->cull = UploadArray(cull.data(), cull.size());
+        std::vector<float> harm;
+        std::vector<float4> outline;
+        OutlineTerms(records, count, data, dataCount, harm, outline);
+        cat->harm = UploadArray(harm.data(), harm.size());
+        cat->outline = UploadArray(outline.data(), outline.size());
+        cat->diameters = UploadArray(diameters, count);
+    });
+    if (code != TG_OK) {
+        FreeCatalogue(cat.get());
+        return code;
+    }
+    *out = cat.release();
+    return TG_OK;
+}
+
+int tg_hip_catalogue_free(tg_catalogue* catalogue) {
+    if (catalogue == nullptr) {
+        return TG_OK;
+    }
+    (void)hipSetDevice(catalogue->dev->ordinal);
+    FreeCatalogue(catalogue);
+    delete catalogue;
+    return TG_OK;
+}
+
+int tg_hip_batch_create(tg_dev* dev, size_t maxTiles, size_t maxBins, size_t maxEntries,
+                        tg_batch** out, char* err, size_t errLen) {
+    if (dev == nullptr || out == nullptr) {
+        SetErr(err, errLen, "tg_hip_batch_create: null argument");
+        return TG_ERR_ARGUMENT;
+    }
+    if (maxBins > 0xFFFFFFFFull || maxEntries > 0xFFFFFFFFull || maxTiles > 0xFFFFFFFFull) {
+        SetErr(err, errLen, "tg_hip_batch_create: capacities must fit 32-bit indices");
+        return TG_ERR_ARGUMENT;
+    }
+    *out = nullptr;
+    auto b = std::make_unique<tg_batch>();
+    const int code = Guard(err, errLen, [&] {
+        Check(hipSetDevice(dev->ordinal), "hipSetDevice");
+        b->dev = dev;
+        Check(hipStreamCreateWithFlags(&b->stream, hipStreamNonBlocking), "hipStreamCreate");
+        b->tiles = Alloc<tg_tile_desc>(maxTiles);
+        b->bins = Alloc<tg_bin>(maxBins);
+        b->entries = Alloc<uint32_t>(maxEntries);
+        b->counters = UploadArray<uint32_t>(nullptr, 2);
+        b->maxTiles = maxTiles;
+        b->maxBins = maxBins;
+        b->maxEntries = maxEntries;
+    });
+    if (code != TG_OK) {
+        tg_hip_batch_free(b.release());
+        return code;
+    }
+    *out = b.release();
+    return TG_OK;
+}
+
+int tg_hip_batch_free(tg_batch* b) {
+    if (b == nullptr) {
+        return TG_OK;
+    }
+    if (b->dev != nullptr) {
+        (void)hipSetDevice(b->dev->ordinal);
+    }
+    if (b->stream != nullptr) {
+        (void)hipStreamSynchronize(b->stream);
+    }
+    DropGraph(*b);
+    for (void* p : {static_cast<void*>(b->tiles), static_cast<void*>(b->bins),
+                    static_cast<void*>(b->entries), static_cast<void*>(b->counters)}) {
+        if (p != nullptr) {
+            (void)hipFree(p);
+        }
+    }
+    if (b->stream != nullptr) {
+        (void)hipStreamDestroy(b->stream);
+    }
+    delete b;
+    return TG_OK;
+}
+
+int tg_hip_batch_stage(tg_batch* b, const tg_tile_desc* tiles, size_t tileCount,
+                       const tg_bin* bins, size_t binCount, const uint32_t* entries,
+                       size_t entryCount, char* err, size_t errLen) {
+    if (b == nullptr || (tileCount > 0 && tiles == nullptr) || (binCount > 0 && bins == nullptr) ||
+        (entryCount > 0 && entries == nullptr)) {
+        SetErr(err, errLen, "tg_hip_batch_stage: null argument");
+        return TG_ERR_ARGUMENT;
+    }
+    return Guard(err, errLen, [&] {
+        if (tileCount > b->maxTiles || binCount > b->maxBins || entryCount > b->maxEntries) {
+            throw HipError{"batch exceeds the capacities it was created with"};
+        }
+        uint64_t outEnd = 0;
+        for (size_t i = 0; i < tileCount; ++i) {
+            const tg_tile_desc& t = tiles[i];
+            if (t.width == 0 || t.height == 0 || t.bins_across != (t.width + kBinSide - 1) / kBinSide ||
+                !(t.pixel > 0.0) || !std::isfinite(t.origin_x) || !std::isfinite(t.origin_y)) {
+                throw HipError{"tile " + std::to_string(i) + " is malformed"};
+            }
+            const uint64_t binsDown = (t.height + kBinSide - 1) / kBinSide;
+            if (uint64_t(t.bin_first) + uint64_t(t.bins_across) * binsDown > binCount) {
+                throw HipError{"tile " + std::to_string(i) + "'s bins run past the bin list"};
+            }
+            outEnd = std::max<uint64_t>(outEnd, t.out_offset + uint64_t(t.width) * t.height);
+        }
+        uint64_t maxEntry = 0;
+        bool any = false;
+        for (size_t i = 0; i < binCount; ++i) {
+            const tg_bin& bin = bins[i];
+            if (bin.tile >= tileCount || uint64_t(bin.first) + bin.count > entryCount) {
+                throw HipError{"bin " + std::to_string(i) + " is malformed"};
+            }
+            const tg_tile_desc& t = tiles[bin.tile];
+            const uint64_t binsDown = (t.height + kBinSide - 1) / kBinSide;
+            if (i < t.bin_first || i >= t.bin_first + uint64_t(t.bins_across) * binsDown) {
+                throw HipError{"bin " + std::to_string(i) + " lies outside its tile's bins"};
+            }
+            any = any || bin.count > 0;
+        }
+        for (size_t i = 0; i < entryCount; ++i) {
+            maxEntry = std::max<uint64_t>(maxEntry, entries[i]);
+        }
+        Check(hipSetDevice(b->dev->ordinal), "hipSetDevice");
+        Check(hipStreamSynchronize(b->stream), "waiting for the batch's previous run");
+        if (tileCount > 0) {
+            Check(hipMemcpy(b->tiles, tiles, tileCount * sizeof(tg_tile_desc),
+                            hipMemcpyHostToDevice), "staging tiles");
+        }
+        if (binCount > 0) {
+            Check(hipMemcpy(b->bins, bins, binCount * sizeof(tg_bin), hipMemcpyHostToDevice),
+                  "staging bins");
+        }
+        if (entryCount > 0) {
+            Check(hipMemcpy(b->entries, entries, entryCount * sizeof(uint32_t),
+                            hipMemcpyHostToDevice), "staging entries");
+        }
+        const uint32_t counts[2] = {0u, uint32_t(binCount)};
+        Check(hipMemcpy(b->counters, counts, sizeof counts, hipMemcpyHostToDevice),
+              "staging the bin count");
+        b->tileCount = tileCount;
+        b->binCount = binCount;
+        b->maxEntry = maxEntry;
+        b->anyEntries = entryCount > 0 && any;
+        b->outEnd = outEnd;
+    });
+}
+
+int tg_hip_batch_run(tg_batch* b, const tg_noise_plan* plan, float elevation,
+                     const tg_catalogue* cat, float* out, size_t outCapacity, int useGraph,
+                     char* err, size_t errLen) {
+    if (b == nullptr || out == nullptr) {
+        SetErr(err, errLen, "tg_hip_batch_run: null argument");
+        return TG_ERR_ARGUMENT;
+    }
+    return Guard(err, errLen, [&] {
+        if (b->outEnd > outCapacity) {
+            throw HipError{"the batch's tiles run past the output buffer"};
+        }
+        if (b->anyEntries && (cat == nullptr || b->maxEntry >= cat->count)) {
+            throw HipError{"the batch's bins name craters the catalogue does not hold"};
+        }
+        if (b->binCount == 0) {
+            return;
+        }
+        Check(hipSetDevice(b->dev->ordinal), "hipSetDevice");
+        if (!useGraph) {
+            Launch(*b, plan, elevation, cat, out);
+            return;
+        }
+        if (b->graph == nullptr || b->graphPlan != plan || b->graphCatalogue != cat ||
+            b->graphOut != out || b->graphElevation != elevation) {
+            DropGraph(*b);
+            hipGraph_t graph = nullptr;
+            Check(hipStreamBeginCapture(b->stream, hipStreamCaptureModeThreadLocal),
+                  "hipStreamBeginCapture");
+            try {
+                Launch(*b, plan, elevation, cat, out);
+            } catch (...) {
+                (void)hipStreamEndCapture(b->stream, &graph);
+                if (graph != nullptr) {
+                    (void)hipGraphDestroy(graph);
+                }
+                throw;
+            }
+            Check(hipStreamEndCapture(b->stream, &graph), "hipStreamEndCapture");
+            const hipError_t made = hipGraphInstantiate(&b->graph, graph, nullptr, nullptr, 0);
+            (void)hipGraphDestroy(graph);
+            Check(made, "hipGraphInstantiate");
+            b->graphPlan = plan;
+            b->graphCatalogue = cat;
+            b->graphOut = out;
+            b->graphElevation = elevation;
+        }
+        Check(hipGraphLaunch(b->graph, b->stream), "hipGraphLaunch");
+    });
+}
+
+int tg_hip_batch_wait(tg_batch* b, char* err, size_t errLen) {
+    if (b == nullptr) {
+        SetErr(err, errLen, "tg_hip_batch_wait: null batch");
+        return TG_ERR_ARGUMENT;
+    }
+    return Guard(err, errLen, [&] {
+        Check(hipSetDevice(b->dev->ordinal), "hipSetDevice");
+        Check(hipStreamSynchronize(b->stream), "running the batch");
+    });
+}
+
+void* tg_hip_batch_stream(tg_batch* b) {
+    return b != nullptr ? b->stream : nullptr;
+}
+
+}  // extern "C"

@@ -1050,7 +1050,8 @@ __device__ __forceinline__ float PeakRelief(const tg_crater_shape& k, const floa
 __device__ __forceinline__ float ApplyCrater(const CraterShape& shape, const float* data,
                                              const tg_crater_record& c, float height, float dx,
                                              float dy, const float* harm = nullptr,
-                                             float4 ra = float4{1.0f, 0.0f, 1.0f, 1.0f}) {
+                                             float4 ra = float4{1.0f, 0.0f, 1.0f, 1.0f},
+                                             float resolution = -1.0f) {
     const tg_crater_shape& k = shape.k;
     const float diameter = c.diameter;
     const float radius = 0.5f * diameter;
@@ -1077,7 +1078,10 @@ __device__ __forceinline__ float ApplyCrater(const CraterShape& shape, const flo
                        PeakRelief(k, data, c, distance, dx, dy);
     const float shaped =
         bare + NoiseHeight(k, c, dx, dy, radii) + TextureHeight(k, c, dx, dy, radii);
-    return height + c.resolution * shaped + c.resolution * replaced;
+    // The record's ResolutionWeight, or the caller's for the grid it shapes (a catalogue record
+    // serves grids of every spacing).
+    const float weight = resolution >= 0.0f ? resolution : c.resolution;
+    return height + weight * shaped + weight * replaced;
 }
 
 }  // namespace tg

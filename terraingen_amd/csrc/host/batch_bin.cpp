@@ -1,0 +1,239 @@
+// SPDX-License-Identifier: BSD-3-Clause
+//
+// Binning a crater catalogue into the bins of many tiles at once, for the batched kernel. For each
+// tile this is what ApplyCraters does for one grid (src/layers/craters/shaping/apply.cpp:225-240,
+// 119-140, prepared.cpp:60-87): craters too small for the tile's pixel size are dropped
+// (ResolutionWeight 0), each remaining crater's footprint box becomes the tile's pixel window, and
+// the crater joins every 16 x 16 bin that window meets, in catalogue (apply) order.
+//
+// Candidates come from a spatial index: craters grouped by diameter level (powers of two, as the
+// generator's bands) and, within a level, by square cells a few diameters across, so a tile only
+// visits the cells its box (grown by the level's largest footprint) meets, and skips the levels
+// whose craters its pixel size cannot resolve.
+
+#include "common.h"
+#include "tg_hip.h"
+
+#include <atomic>
+#include <mutex>
+#include <thread>
+#include <unordered_map>
+
+namespace tgh {
+
+namespace {
+
+struct Level {
+    double cell = 0.0;       // index cell side (m)
+    double maxReach = 0.0;   // largest footprint half-width in the level (m)
+    double maxDiameter = 0.0;
+    std::unordered_map<std::uint64_t, std::vector<std::uint32_t>> cells;
+};
+
+std::uint64_t CellKey(std::int64_t i, std::int64_t j) {
+    return (std::uint64_t(std::uint32_t(std::int32_t(i))) << 32) |
+           std::uint32_t(std::int32_t(j));
+}
+
+// ResolutionWeight(D, spacing) != 0 (src/layers/fade.cpp:48-52 with the default fade 2-4).
+bool Resolved(double diameter, double pixel) {
+    const double x = diameter / pixel;
+    if (x <= 2.0) {
+        return false;
+    }
+    if (x >= 4.0) {
+        return true;
+    }
+    double t = std::clamp((x - 2.0) / (4.0 - 2.0), 0.0, 1.0);
+    return 1.0 - (1.0 - t * t * (3.0 - 2.0 * t)) != 0.0;
+}
+
+struct TileBins {
+    std::vector<std::uint32_t> counts;  // per bin
+    std::vector<std::uint32_t> entries;
+};
+
+}  // namespace
+
+void BinBatch(const double* cx, const double* cy, const double* reach, const double* diameter,
+              std::size_t n, const tgx_tile* tiles, std::size_t tileCount, unsigned threads,
+              std::vector<tg_tile_desc>& descs, std::vector<tg_bin>& bins,
+              std::vector<std::uint32_t>& entries) {
+    constexpr std::uint32_t kSide = 16;
+    // The index.
+    std::vector<Level> levels;
+    int base = 0;
+    if (n > 0) {
+        double smallest = INFINITY;
+        for (std::size_t k = 0; k < n; ++k) {
+            smallest = std::fmin(smallest, diameter[k]);
+        }
+        base = std::ilogb(smallest);
+    }
+    for (std::size_t k = 0; k < n; ++k) {
+        if (!(diameter[k] > 0.0) || !std::isfinite(cx[k]) || !std::isfinite(cy[k]) ||
+            !(reach[k] >= 0.0)) {
+            throw std::invalid_argument("catalogue crater " + std::to_string(k) + " is malformed");
+        }
+        const std::size_t l = std::size_t(std::max(0, std::ilogb(diameter[k]) - base));
+        if (l >= levels.size()) {
+            levels.resize(l + 1);
+        }
+        Level& level = levels[l];
+        if (level.cell == 0.0) {
+            level.cell = 8.0 * std::ldexp(1.0, base + int(l) + 1);
+        }
+        level.maxReach = std::fmax(level.maxReach, reach[k]);
+        level.maxDiameter = std::fmax(level.maxDiameter, diameter[k]);
+        const auto i = std::int64_t(std::floor(cx[k] / level.cell));
+        const auto j = std::int64_t(std::floor(cy[k] / level.cell));
+        level.cells[CellKey(i, j)].push_back(std::uint32_t(k));
+    }
+    // Each tile's bins, in parallel.
+    std::vector<TileBins> per(tileCount);
+    std::atomic<std::size_t> next{0};
+    std::atomic<bool> failed{false};
+    std::mutex failureMutex;
+    std::string failure;
+    auto work = [&] {
+        std::vector<std::uint32_t> candidates;
+        for (;;) {
+            const std::size_t t = next.fetch_add(1);
+            if (t >= tileCount || failed.load()) {
+                return;
+            }
+            const tgx_tile& tile = tiles[t];
+            if (tile.width == 0 || tile.height == 0 || !(tile.pixel > 0.0) ||
+                !std::isfinite(tile.origin_x) || !std::isfinite(tile.origin_y)) {
+                std::lock_guard<std::mutex> lock(failureMutex);
+                failed = true;
+                failure = "tile " + std::to_string(t) + " is malformed";
+                return;
+            }
+            const Grid g{tile.width, tile.height, tile.origin_x, tile.origin_y, tile.pixel};
+            const double minX = tile.origin_x, maxX = tile.origin_x + tile.width * tile.pixel;
+            const double maxY = tile.origin_y, minY = tile.origin_y - tile.height * tile.pixel;
+            candidates.clear();
+            for (const Level& level : levels) {
+                if (level.cell == 0.0 || !Resolved(level.maxDiameter, tile.pixel)) {
+                    continue;
+                }
+                const double grow = level.maxReach;
+                const auto i0 = std::int64_t(std::floor((minX - grow) / level.cell));
+                const auto i1 = std::int64_t(std::floor((maxX + grow) / level.cell));
+                const auto j0 = std::int64_t(std::floor((minY - grow) / level.cell));
+                const auto j1 = std::int64_t(std::floor((maxY + grow) / level.cell));
+                if ((i1 - i0 + 1) * (j1 - j0 + 1) > std::int64_t(level.cells.size())) {
+                    for (const auto& [key, list] : level.cells) {
+                        candidates.insert(candidates.end(), list.begin(), list.end());
+                    }
+                    continue;
+                }
+                for (std::int64_t j = j0; j <= j1; ++j) {
+                    for (std::int64_t i = i0; i <= i1; ++i) {
+                        const auto it = level.cells.find(CellKey(i, j));
+                        if (it != level.cells.end()) {
+                            candidates.insert(candidates.end(), it->second.begin(),
+                                              it->second.end());
+                        }
+                    }
+                }
+            }
+            std::sort(candidates.begin(), candidates.end());
+            const std::size_t binsX = (tile.width + kSide - 1) / kSide;
+            const std::size_t binsY = (tile.height + kSide - 1) / kSide;
+            std::vector<std::vector<std::uint32_t>> lists(binsX * binsY);
+            for (const std::uint32_t k : candidates) {
+                if (!Resolved(diameter[k], tile.pixel)) {
+                    continue;
+                }
+                const PixelWindow w = PixelWindowIn(g, cx[k] - reach[k], cy[k] - reach[k],
+                                                    cx[k] + reach[k], cy[k] + reach[k]);
+                if (w.Empty()) {
+                    continue;
+                }
+                for (std::size_t by = w.rowBegin / kSide; by <= (w.rowEnd - 1) / kSide; ++by) {
+                    for (std::size_t bx = w.colBegin / kSide; bx <= (w.colEnd - 1) / kSide; ++bx) {
+                        lists[by * binsX + bx].push_back(k);
+                    }
+                }
+            }
+            TileBins& out = per[t];
+            out.counts.resize(lists.size());
+            for (std::size_t b = 0; b < lists.size(); ++b) {
+                out.counts[b] = std::uint32_t(lists[b].size());
+                out.entries.insert(out.entries.end(), lists[b].begin(), lists[b].end());
+            }
+        }
+    };
+    if (threads == 0) {
+        threads = std::max(1u, std::thread::hardware_concurrency());
+    }
+    std::vector<std::thread> pool;
+    for (unsigned i = 1; i < threads; ++i) {
+        pool.emplace_back(work);
+    }
+    work();
+    for (std::thread& th : pool) {
+        th.join();
+    }
+    if (failed) {
+        throw std::invalid_argument(failure);
+    }
+    // Concatenated in tile order.
+    descs.clear();
+    bins.clear();
+    entries.clear();
+    std::uint64_t out = 0;
+    for (std::size_t t = 0; t < tileCount; ++t) {
+        const tgx_tile& tile = tiles[t];
+        tg_tile_desc d{};
+        d.origin_x = tile.origin_x;
+        d.origin_y = tile.origin_y;
+        d.pixel = tile.pixel;
+        d.width = tile.width;
+        d.height = tile.height;
+        d.out_offset = out;
+        d.bin_first = std::uint32_t(bins.size());
+        d.bins_across = (tile.width + kSide - 1) / kSide;
+        descs.push_back(d);
+        out += std::uint64_t(tile.width) * tile.height;
+        std::uint32_t first = std::uint32_t(entries.size());
+        for (const std::uint32_t count : per[t].counts) {
+            bins.push_back({std::uint32_t(t), first, count, 0u});
+            first += count;
+        }
+        entries.insert(entries.end(), per[t].entries.begin(), per[t].entries.end());
+        if (entries.size() > 0xFFFFFFFFull || bins.size() > 0xFFFFFFFFull) {
+            throw std::invalid_argument("batch too large for 32-bit bin and entry indices");
+        }
+    }
+}
+
+}  // namespace tgh
+
+extern "C" {
+
+int tgx_batch_bin(const double* cx, const double* cy, const double* reach,
+                  const double* diameter, size_t n, const tgx_tile* tiles, size_t tileCount,
+                  unsigned threads, tg_tile_desc** descs, tg_bin** bins, size_t* binCount,
+                  uint32_t** entries, size_t* entryCount, char* err, size_t errLen) {
+    return tgh::Guard(err, errLen, [&] {
+        if (descs == nullptr || bins == nullptr || entries == nullptr || binCount == nullptr ||
+            entryCount == nullptr || (tileCount > 0 && tiles == nullptr) ||
+            (n > 0 && (cx == nullptr || cy == nullptr || reach == nullptr || diameter == nullptr))) {
+            throw std::invalid_argument("tgx_batch_bin: null argument");
+        }
+        std::vector<tg_tile_desc> d;
+        std::vector<tg_bin> b;
+        std::vector<std::uint32_t> e;
+        tgh::BinBatch(cx, cy, reach, diameter, n, tiles, tileCount, threads, d, b, e);
+        *descs = tgh::Export(d);
+        *bins = tgh::Export(b);
+        *binCount = b.size();
+        *entries = tgh::Export(e);
+        *entryCount = e.size();
+    });
+}
+
+}  // extern "C"

@@ -376,6 +376,7 @@ Batch PrepareBatch(const std::vector<tgh::Crater>& craters, const CraterModels& 
         b.cx.push_back(c.x);
         b.cy.push_back(c.y);
         b.reach.push_back(outline.MaxScale() * reachRadii * (0.5 * c.diameter));
+        b.diameter.push_back(c.diameter);
     }
     if (b.data.empty()) {
         b.data.push_back(0.0f);
@@ -416,6 +417,7 @@ int tgx_prepare_craters(const tgx_crater* craters, size_t n, const tgx_crater_mo
                 f.push_back(b.cx[i]);
                 f.push_back(b.cy[i]);
                 f.push_back(b.reach[i]);
+                f.push_back(b.diameter[i]);
             }
             *footprint = tgh::Export(f);
         }

@@ -54,7 +54,7 @@ struct Batch {
     tg_crater_shape kernel{};
     std::vector<tg_crater_record> records;
     std::vector<float> data;
-    std::vector<double> cx, cy, reach;  // footprint: centre and half-width (m)
+    std::vector<double> cx, cy, reach, diameter;  // footprint: centre, half-width; D (m)
 };
 
 inline CraterModels ModelsFrom(const tgx_crater_models& m) {

@@ -1,0 +1,130 @@
+// SPDX-License-Identifier: BSD-3-Clause
+//
+// Internals shared by the translation units of libtg_hip.so: the handles behind the C ABI and the
+// error plumbing (every HIP status checked, every failure a message, never an exception across the
+// ABI).
+#pragma once
+
+#include "tg_hip.h"
+
+#include "noise_plan.h"
+
+#include <hip/hip_runtime.h>
+
+#include <cstdio>
+#include <memory>
+#include <mutex>
+#include <string>
+#include <vector>
+
+struct tg_dev {
+    int ordinal = 0;
+    std::string name;
+    std::mutex mutex;  // runs are serialized, as ComputeDevice requires (src/gpu/device.h:48)
+};
+
+struct tg_crater_set {
+    tg_dev* dev = nullptr;
+    tg_crater_shape* kernel = nullptr;
+    tg_crater_record* records = nullptr;
+    float* data = nullptr;
+    uint32_t* tiles = nullptr;
+    uint32_t* entries = nullptr;
+    tg_crater_frame* frames = nullptr;
+    tg_split_point* points = nullptr;
+    float* lows = nullptr;
+    float* cull = nullptr;  // per record: squared round distance (m^2) beyond which it is inert
+    float* harm = nullptr;  // A cos(phi), A sin(phi) in the data's harmonic slots
+    float4* outline = nullptr;  // per record: cos and sin of the heading, sqrt(ellipticity), 1/that
+    size_t pointCount = 0;
+    size_t lowCount = 0;
+    uint32_t tilesAcross = 0;
+    uint32_t tileRows = 0;
+    uint32_t tileSide = 16;
+    bool onBody = false;
+};
+
+struct tg_noise_plan {
+    tg_dev* dev = nullptr;
+    tg::NoisePlan* device = nullptr;
+    tg::NoisePlan host{};
+};
+
+namespace tgi {
+
+inline void SetErr(char* err, size_t len, const std::string& message) {
+    if (err != nullptr && len > 0) {
+        std::snprintf(err, len, "%s", message.c_str());
+    }
+}
+
+struct HipError {
+    std::string message;
+};
+
+inline void Check(hipError_t status, const char* what) {
+    if (status != hipSuccess) {
+        throw HipError{std::string(what) + ": " + hipGetErrorString(status)};
+    }
+}
+
+// Device memory freed on scope exit.
+template <class T> struct DeviceArray {
+    T* ptr = nullptr;
+    DeviceArray() = default;
+    explicit DeviceArray(size_t count) {
+        if (count > 0) {
+            Check(hipMalloc(reinterpret_cast<void**>(&ptr), count * sizeof(T)), "hipMalloc");
+        }
+    }
+    DeviceArray(const DeviceArray&) = delete;
+    DeviceArray& operator=(const DeviceArray&) = delete;
+    ~DeviceArray() {
+        if (ptr != nullptr) {
+            (void)hipFree(ptr);
+        }
+    }
+    void Upload(const void* src, size_t bytes) {
+        if (bytes > 0) {
+            Check(hipMemcpy(ptr, src, bytes, hipMemcpyHostToDevice), "hipMemcpy to device");
+        }
+    }
+};
+
+}  // namespace tgi
+
+namespace tgi {
+
+template <class F> int Guard(char* err, size_t errLen, F body) {
+    try {
+        body();
+        return TG_OK;
+    } catch (const HipError& e) {
+        SetErr(err, errLen, e.message);
+        return TG_ERR_DEVICE;
+    } catch (const std::bad_alloc&) {
+        SetErr(err, errLen, "host allocation failed");
+        return TG_ERR_DEVICE;
+    }
+}
+
+// Host-side derived per-crater terms (tg_hip.hip): the squared cull distance per record, and the
+// fast outline path's (A cos phi, A sin phi) per harmonic slot and heading/ellipse terms per record.
+std::vector<float> CullRadii(const tg_crater_shape& k, const tg_crater_record* records, size_t n,
+                             const float* data, size_t dataCount);
+void OutlineTerms(const tg_crater_record* records, size_t n, const float* data, size_t dataCount,
+                  std::vector<float>& harm, std::vector<float4>& outline);
+
+// Throws HipError unless every index a crater batch holds stays inside its arrays (tg_hip.hip).
+void ValidateCraterBatch(const tg_crater_shape& k, const tg_crater_record* records, size_t n,
+                         size_t dataCount, const uint32_t* tiles, size_t tileWords,
+                         const uint32_t* entries, size_t entryCount, size_t frameCount);
+
+__device__ __forceinline__ bool IsNoData(const tg_kernel_params& p, float height) {
+    if (p.no_data == TG_NODATA_NAN) {
+        return isnan(height);
+    }
+    return p.no_data != TG_NODATA_NONE && height == p.no_data_value;
+}
+
+}  // namespace tgi

@@ -19,86 +19,12 @@
 #include <string>
 #include <vector>
 
-struct tg_dev {
-    int ordinal = 0;
-    std::string name;
-    std::mutex mutex;  // runs are serialized, as ComputeDevice requires (src/gpu/device.h:48)
-};
+#include "internal.h"
 
-struct tg_crater_set {
-    tg_dev* dev = nullptr;
-    tg_crater_shape* kernel = nullptr;
-    tg_crater_record* records = nullptr;
-    float* data = nullptr;
-    uint32_t* tiles = nullptr;
-    uint32_t* entries = nullptr;
-    tg_crater_frame* frames = nullptr;
-    tg_split_point* points = nullptr;
-    float* lows = nullptr;
-    float* cull = nullptr;  // per record: squared round distance (m^2) beyond which it is inert
-    float* harm = nullptr;  // A cos(phi), A sin(phi) in the data's harmonic slots
-    float4* outline = nullptr;  // per record: cos and sin of the heading, sqrt(ellipticity), 1/that
-    size_t pointCount = 0;
-    size_t lowCount = 0;
-    uint32_t tilesAcross = 0;
-    uint32_t tileRows = 0;
-    uint32_t tileSide = 16;
-    bool onBody = false;
-};
-
-struct tg_noise_plan {
-    tg_dev* dev = nullptr;
-    tg::NoisePlan* device = nullptr;
-    tg::NoisePlan host{};
-};
+using namespace tgi;
 
 namespace {
 
-void SetErr(char* err, size_t len, const std::string& message) {
-    if (err != nullptr && len > 0) {
-        std::snprintf(err, len, "%s", message.c_str());
-    }
-}
-
-struct HipError {
-    std::string message;
-};
-
-void Check(hipError_t status, const char* what) {
-    if (status != hipSuccess) {
-        throw HipError{std::string(what) + ": " + hipGetErrorString(status)};
-    }
-}
-
-// Device memory freed on scope exit.
-template <class T> struct DeviceArray {
-    T* ptr = nullptr;
-    DeviceArray() = default;
-    explicit DeviceArray(size_t count) {
-        if (count > 0) {
-            Check(hipMalloc(reinterpret_cast<void**>(&ptr), count * sizeof(T)), "hipMalloc");
-        }
-    }
-    DeviceArray(const DeviceArray&) = delete;
-    DeviceArray& operator=(const DeviceArray&) = delete;
-    ~DeviceArray() {
-        if (ptr != nullptr) {
-            (void)hipFree(ptr);
-        }
-    }
-    void Upload(const void* src, size_t bytes) {
-        if (bytes > 0) {
-            Check(hipMemcpy(ptr, src, bytes, hipMemcpyHostToDevice), "hipMemcpy to device");
-        }
-    }
-};
-
-__device__ __forceinline__ bool IsNoData(const tg_kernel_params& p, float height) {
-    if (p.no_data == TG_NODATA_NAN) {
-        return isnan(height);
-    }
-    return p.no_data != TG_NODATA_NONE && height == p.no_data_value;
-}
 
 // Cells per block: 64 columns (one wave's lanes along a row, so stores coalesce) by 4 rows.
 constexpr uint32_t kNoiseBlockX = 64;
@@ -220,64 +146,7 @@ __global__ void __launch_bounds__(kCraterTile * kCraterTile)
     heights[i] = h;
 }
 
-// The squared round distance from a crater's centre beyond which ApplyCrater leaves every height
-// unchanged (craters.h:1186-1193): reach x radius x an upper bound of its outline scale
-// (KernelOutlineScaleAt, craters.h:772-808: the ellipse stretches by at most sqrt(ellipticity),
-// the wobble by at most 1 + sum|A_k| + max|table| + |table - broad|), with relative slack far
-// above FP32 rounding. Infinite, so never culled, for craters whose distance other terms can
-// shorten: terraces, marks and floor shifts.
-std::vector<float> CullRadii(const tg_crater_shape& k, const tg_crater_record* records, size_t n,
-                             const float* data, size_t dataCount) {
-    std::vector<float> cull(n > 0 ? n : 1, INFINITY);
-    for (size_t i = 0; i < n; ++i) {
-        const tg_crater_record& c = records[i];
-        if (c.terrace_count > 0 || c.mark_count > 0 || c.shift_x != 0.0f || c.shift_y != 0.0f) {
-            continue;
-        }
-        double amplitudes = 0.0;
-        for (uint32_t h = 0; h < c.harmonic_count; ++h) {
-            const size_t at = size_t(c.harmonic_offset) + 2 * h;
-            if (at >= dataCount) {
-                amplitudes = INFINITY;
-                break;
-            }
-            amplitudes += std::fabs(double(data[at]));
-        }
-        double table = 0.0;
-        for (uint32_t t = 0; t < c.table_count; ++t) {
-            const size_t at = size_t(c.table_offset) + t;
-            table = at < dataCount ? std::fmax(table, std::fabs(double(data[at]))) : INFINITY;
-        }
-        const double wobble =
-            c.table_count > 0 ? 1.0 + 2.0 * amplitudes + 2.0 * table : 1.0 + amplitudes;
-        const double scale = std::sqrt(std::fmax(double(c.ellipticity), 1.0)) * wobble;
-        const double r = double(k.reach_radii) * 0.5 * double(c.diameter) * scale * (1.0 + 1e-4);
-        if (std::isfinite(r)) {
-            cull[i] = float(r * r);
-        }
-    }
-    return cull;
-}
 
-// Per-crater outline terms the fast path reads: (A cos phi, A sin phi) in each harmonic's data slot
-// and (cos h, sin h, sqrt(ellipticity), 1 / sqrt(ellipticity)) per record, in FP64 then rounded.
-void OutlineTerms(const tg_crater_record* records, size_t n, const float* data, size_t dataCount,
-                  std::vector<float>& harm, std::vector<float4>& outline) {
-    harm.assign(dataCount > 0 ? dataCount : 1, 0.0f);
-    outline.assign(n > 0 ? n : 1, float4{1.0f, 0.0f, 1.0f, 1.0f});
-    for (size_t i = 0; i < n; ++i) {
-        const tg_crater_record& c = records[i];
-        for (uint32_t k = 0; k < c.harmonic_count; ++k) {
-            const size_t at = size_t(c.harmonic_offset) + 2 * k;
-            const double a = data[at], phi = data[at + 1];
-            harm[at] = float(a * std::cos(phi));
-            harm[at + 1] = float(a * std::sin(phi));
-        }
-        const double along = std::sqrt(double(c.ellipticity));
-        outline[i] = float4{float(std::cos(double(c.heading))), float(std::sin(double(c.heading))),
-                            float(along), float(1.0 / along)};
-    }
-}
 
 void LaunchCraters(const tg_crater_set& set, float* heights, const float* xs, const float* ys,
                    const tg_kernel_params& params, hipStream_t stream) {
@@ -301,18 +170,6 @@ int CreateCraterSet(tg_dev* dev, const tg_crater_shape* kernel, const tg_crater_
                     size_t lowCount, tg_crater_set** out, char* err, size_t errLen);
 void CheckLaunch(const tg_crater_set& set, const tg_kernel_params& params);
 
-template <class F> int Guard(char* err, size_t errLen, F body) {
-    try {
-        body();
-        return TG_OK;
-    } catch (const HipError& e) {
-        SetErr(err, errLen, e.message);
-        return TG_ERR_DEVICE;
-    } catch (const std::bad_alloc&) {
-        SetErr(err, errLen, "host allocation failed");
-        return TG_ERR_DEVICE;
-    }
-}
 
 }  // namespace
 
@@ -518,70 +375,6 @@ template <class T> T* Upload(const T* src, size_t count, std::vector<void*>& own
         Check(hipMemset(p, 0, bytes), "hipMemset");
     }
     return static_cast<T*>(p);
-}
-
-// Every index the crater kernel follows, checked on the host so a malformed batch is an error,
-// never an out-of-bounds read: tile heads against the entries, entries against the records,
-// each record's data ranges against the data (craters.h:772-1182 read them), frames on a body.
-void ValidateCraterBatch(const tg_crater_shape& k, const tg_crater_record* records, size_t n,
-                         size_t dataCount, const uint32_t* tiles, size_t tileWords,
-                         const uint32_t* entries, size_t entryCount, size_t frameCount) {
-    const auto fail = [](const std::string& what) { throw HipError{"invalid crater batch: " + what}; };
-    if (tileWords < 3 || tiles[0] == 0 || tiles[1] == 0 || (tileWords - 3) % 2 != 0 ||
-        ((tileWords - 3) / 2) % tiles[0] != 0) {
-        fail("tile block must be [tiles across, side, body flag, (first, count) per tile]");
-    }
-    for (size_t t = 3; t < tileWords; t += 2) {
-        if (uint64_t(tiles[t]) + tiles[t + 1] > entryCount) {
-            fail("a tile's entries run past the entry list");
-        }
-    }
-    for (size_t e = 0; e < entryCount; ++e) {
-        if (entries[e] >= n && !(n == 0 && entryCount == 1)) {
-            fail("an entry names a record past the end");
-        }
-    }
-    if (tiles[2] != 0 && frameCount < n) {
-        fail("a body batch needs one frame per record");
-    }
-    const tg_crater_profile& p = k.profile;
-    const auto span = [&](uint64_t offset, uint64_t count, const char* what) {
-        if (count > 0 && offset + count > dataCount) {
-            fail(std::string(what) + " run past the data");
-        }
-    };
-    if (p.diffused == 1u || p.simple == TG_PROFILE_TABULATED) {
-        if (p.table_cells == 0) {
-            fail("tabulated profiles need table cells");
-        }
-    }
-    if (p.diffused == 1u) {
-        if (p.age_count < 2) {
-            fail("diffused profiles need at least two ages");
-        }
-        span(p.ages_offset, p.age_count, "diffusion ages");
-    }
-    for (size_t i = 0; i < n; ++i) {
-        const tg_crater_record& c = records[i];
-        span(c.harmonic_offset, 2ull * c.harmonic_count, "outline harmonics");
-        span(c.table_offset, c.table_count, "outline tables");
-        if (c.mark_count > 0) {
-            span(c.mark_offset, 3ull + c.mark_count, "outline marks");
-        }
-        if (c.terrace_count > 0) {
-            span(c.terrace_offset,
-                 20ull + 8ull * (c.terrace_count - 1) + 3ull * c.terrace_count, "terraces");
-        }
-        if (c.peak_count > 0) {
-            span(c.peak_offset, 6ull + 11ull * c.peak_count, "peak massifs");
-        }
-        if (p.simple == TG_PROFILE_TABULATED) {
-            span(c.profile_offset, uint64_t(p.table_cells) + 1, "a tabulated profile row");
-        } else if (p.diffused == 1u && c.diffusion > 0.0f) {
-            span(c.profile_offset, uint64_t(p.age_count) * (uint64_t(p.table_cells) + 1),
-                 "a diffusion table");
-        }
-    }
 }
 
 int CreateCraterSet(tg_dev* dev, const tg_crater_shape* kernel, const tg_crater_record* records,
@@ -857,3 +650,138 @@ int tg_hip_event_elapsed(tg_dev* dev, void* start, void* stop, float* ms, char* 
 }
 
 }  // extern "C"
+
+namespace tgi {
+
+// The squared round distance from a crater's centre beyond which ApplyCrater leaves every height
+// unchanged (craters.h:1186-1193): reach x radius x an upper bound of its outline scale
+// (KernelOutlineScaleAt, craters.h:772-808: the ellipse stretches by at most sqrt(ellipticity),
+// the wobble by at most 1 + sum|A_k| + max|table| + |table - broad|), with relative slack far
+// above FP32 rounding. Infinite, so never culled, for craters whose distance other terms can
+// shorten: terraces, marks and floor shifts.
+std::vector<float> CullRadii(const tg_crater_shape& k, const tg_crater_record* records, size_t n,
+                             const float* data, size_t dataCount) {
+    std::vector<float> cull(n > 0 ? n : 1, INFINITY);
+    for (size_t i = 0; i < n; ++i) {
+        const tg_crater_record& c = records[i];
+        if (c.terrace_count > 0 || c.mark_count > 0 || c.shift_x != 0.0f || c.shift_y != 0.0f) {
+            continue;
+        }
+        double amplitudes = 0.0;
+        for (uint32_t h = 0; h < c.harmonic_count; ++h) {
+            const size_t at = size_t(c.harmonic_offset) + 2 * h;
+            if (at >= dataCount) {
+                amplitudes = INFINITY;
+                break;
+            }
+            amplitudes += std::fabs(double(data[at]));
+        }
+        double table = 0.0;
+        for (uint32_t t = 0; t < c.table_count; ++t) {
+            const size_t at = size_t(c.table_offset) + t;
+            table = at < dataCount ? std::fmax(table, std::fabs(double(data[at]))) : INFINITY;
+        }
+        const double wobble =
+            c.table_count > 0 ? 1.0 + 2.0 * amplitudes + 2.0 * table : 1.0 + amplitudes;
+        const double scale = std::sqrt(std::fmax(double(c.ellipticity), 1.0)) * wobble;
+        const double r = double(k.reach_radii) * 0.5 * double(c.diameter) * scale * (1.0 + 1e-4);
+        if (std::isfinite(r)) {
+            cull[i] = float(r * r);
+        }
+    }
+    return cull;
+}
+
+}  // namespace tgi
+
+namespace tgi {
+
+// Per-crater outline terms the fast path reads: (A cos phi, A sin phi) in each harmonic's data slot
+// and (cos h, sin h, sqrt(ellipticity), 1 / sqrt(ellipticity)) per record, in FP64 then rounded.
+void OutlineTerms(const tg_crater_record* records, size_t n, const float* data, size_t dataCount,
+                  std::vector<float>& harm, std::vector<float4>& outline) {
+    harm.assign(dataCount > 0 ? dataCount : 1, 0.0f);
+    outline.assign(n > 0 ? n : 1, float4{1.0f, 0.0f, 1.0f, 1.0f});
+    for (size_t i = 0; i < n; ++i) {
+        const tg_crater_record& c = records[i];
+        for (uint32_t k = 0; k < c.harmonic_count; ++k) {
+            const size_t at = size_t(c.harmonic_offset) + 2 * k;
+            const double a = data[at], phi = data[at + 1];
+            harm[at] = float(a * std::cos(phi));
+            harm[at + 1] = float(a * std::sin(phi));
+        }
+        const double along = std::sqrt(double(c.ellipticity));
+        outline[i] = float4{float(std::cos(double(c.heading))), float(std::sin(double(c.heading))),
+                            float(along), float(1.0 / along)};
+    }
+}
+
+}  // namespace tgi
+
+namespace tgi {
+
+// Every index the crater kernel follows, checked on the host so a malformed batch is an error,
+// never an out-of-bounds read: tile heads against the entries, entries against the records,
+// each record's data ranges against the data (craters.h:772-1182 read them), frames on a body.
+void ValidateCraterBatch(const tg_crater_shape& k, const tg_crater_record* records, size_t n,
+                         size_t dataCount, const uint32_t* tiles, size_t tileWords,
+                         const uint32_t* entries, size_t entryCount, size_t frameCount) {
+    const auto fail = [](const std::string& what) { throw HipError{"invalid crater batch: " + what}; };
+    if (tileWords < 3 || tiles[0] == 0 || tiles[1] == 0 || (tileWords - 3) % 2 != 0 ||
+        ((tileWords - 3) / 2) % tiles[0] != 0) {
+        fail("tile block must be [tiles across, side, body flag, (first, count) per tile]");
+    }
+    for (size_t t = 3; t < tileWords; t += 2) {
+        if (uint64_t(tiles[t]) + tiles[t + 1] > entryCount) {
+            fail("a tile's entries run past the entry list");
+        }
+    }
+    for (size_t e = 0; e < entryCount; ++e) {
+        if (entries[e] >= n && !(n == 0 && entryCount == 1)) {
+            fail("an entry names a record past the end");
+        }
+    }
+    if (tiles[2] != 0 && frameCount < n) {
+        fail("a body batch needs one frame per record");
+    }
+    const tg_crater_profile& p = k.profile;
+    const auto span = [&](uint64_t offset, uint64_t count, const char* what) {
+        if (count > 0 && offset + count > dataCount) {
+            fail(std::string(what) + " run past the data");
+        }
+    };
+    if (p.diffused == 1u || p.simple == TG_PROFILE_TABULATED) {
+        if (p.table_cells == 0) {
+            fail("tabulated profiles need table cells");
+        }
+    }
+    if (p.diffused == 1u) {
+        if (p.age_count < 2) {
+            fail("diffused profiles need at least two ages");
+        }
+        span(p.ages_offset, p.age_count, "diffusion ages");
+    }
+    for (size_t i = 0; i < n; ++i) {
+        const tg_crater_record& c = records[i];
+        span(c.harmonic_offset, 2ull * c.harmonic_count, "outline harmonics");
+        span(c.table_offset, c.table_count, "outline tables");
+        if (c.mark_count > 0) {
+            span(c.mark_offset, 3ull + c.mark_count, "outline marks");
+        }
+        if (c.terrace_count > 0) {
+            span(c.terrace_offset,
+                 20ull + 8ull * (c.terrace_count - 1) + 3ull * c.terrace_count, "terraces");
+        }
+        if (c.peak_count > 0) {
+            span(c.peak_offset, 6ull + 11ull * c.peak_count, "peak massifs");
+        }
+        if (p.simple == TG_PROFILE_TABULATED) {
+            span(c.profile_offset, uint64_t(p.table_cells) + 1, "a tabulated profile row");
+        } else if (p.diffused == 1u && c.diffusion > 0.0f) {
+            span(c.profile_offset, uint64_t(p.age_count) * (uint64_t(p.table_cells) + 1),
+                 "a diffusion table");
+        }
+    }
+}
+
+}  // namespace tgi

@@ -105,6 +105,15 @@ HIP_ABI = {
                                            VP, SZ, VP, SZ, VP, SZ, C.POINTER(VP), CP, SZ]),
     "tg_hip_crater_set_free": (C.c_int, [VP]),
     "tg_hip_crater_launch": (C.c_int, [VP, VP, VP, VP, VP, C.POINTER(KernelParams), VP, CP, SZ]),
+    "tg_hip_catalogue_create": (C.c_int, [VP, C.POINTER(CraterShape), VP, SZ, VP, SZ, VP,
+                                          C.POINTER(VP), CP, SZ]),
+    "tg_hip_catalogue_free": (C.c_int, [VP]),
+    "tg_hip_batch_create": (C.c_int, [VP, SZ, SZ, SZ, C.POINTER(VP), CP, SZ]),
+    "tg_hip_batch_free": (C.c_int, [VP]),
+    "tg_hip_batch_stage": (C.c_int, [VP, VP, SZ, VP, SZ, VP, SZ, CP, SZ]),
+    "tg_hip_batch_run": (C.c_int, [VP, VP, C.c_float, VP, VP, SZ, C.c_int, CP, SZ]),
+    "tg_hip_batch_wait": (C.c_int, [VP, CP, SZ]),
+    "tg_hip_batch_stream": (VP, [VP]),
     "tg_hip_malloc": (C.c_int, [VP, SZ, C.POINTER(VP), CP, SZ]),
     "tg_hip_free": (C.c_int, [VP, VP]),
     "tg_hip_memcpy_h2d": (C.c_int, [VP, VP, VP, SZ, CP, SZ]),
@@ -399,7 +408,7 @@ HOST_ABI["tgx_prepare_craters"] = (C.c_int, [
 
 
 def prepare_craters(craters: np.ndarray, models, apply, layer_seed: int, spacing: float):
-    """``tgx_prepare_craters``: (kernel, records, data, footprint[n, 3])."""
+    """``tgx_prepare_craters``: (kernel, records, data, footprint[n, 4]: x, y, reach, D)."""
     lib = host_library()
     craters = np.ascontiguousarray(craters, CRATER_DTYPE)
     kernel = CraterShape()
@@ -413,7 +422,7 @@ def prepare_craters(craters: np.ndarray, models, apply, layer_seed: int, spacing
     try:
         records = np.frombuffer(C.string_at(rp.value, rn.value * 144), RECORD_DTYPE).copy()
         data = np.frombuffer(C.string_at(dp.value, dn.value * 4), np.float32).copy()
-        foot = np.frombuffer(C.string_at(fp.value, rn.value * 24), np.float64).reshape(-1, 3).copy()
+        foot = np.frombuffer(C.string_at(fp.value, rn.value * 32), np.float64).reshape(-1, 4).copy()
     finally:
         for p in (rp, dp, fp):
             lib.tgx_free(p)
@@ -422,3 +431,42 @@ def prepare_craters(craters: np.ndarray, models, apply, layer_seed: int, spacing
 
 def margin_radii(models, apply) -> float:
     return host_library().tgx_margin_radii(C.byref(models), C.byref(apply))
+
+
+TILE_DESC_DTYPE = np.dtype([("origin_x", np.float64), ("origin_y", np.float64),
+                            ("pixel", np.float64), ("width", np.uint32), ("height", np.uint32),
+                            ("out_offset", np.uint64), ("bin_first", np.uint32),
+                            ("bins_across", np.uint32)])
+BIN_DTYPE = np.dtype([("tile", np.uint32), ("first", np.uint32), ("count", np.uint32),
+                      ("pad", np.uint32)])
+TILE_DTYPE = np.dtype([("origin_x", np.float64), ("origin_y", np.float64), ("pixel", np.float64),
+                       ("width", np.uint32), ("height", np.uint32)])
+assert TILE_DESC_DTYPE.itemsize == 48 and BIN_DTYPE.itemsize == 16 and TILE_DTYPE.itemsize == 32
+
+HOST_ABI["tgx_batch_bin"] = (C.c_int, [VP, VP, VP, VP, SZ, VP, SZ, C.c_uint, C.POINTER(VP),
+                                       C.POINTER(VP), C.POINTER(SZ), C.POINTER(VP), C.POINTER(SZ),
+                                       CP, SZ])
+
+
+def batch_bin(foot: np.ndarray, tiles: np.ndarray, threads: int = 0):
+    """``tgx_batch_bin``: (tile descriptors, bins, entries) for a catalogue footprint
+    (x, y, reach, D per crater, apply order) and a structured array of tiles."""
+    lib = host_library()
+    foot = np.ascontiguousarray(foot, np.float64)
+    cx, cy, reach, d = (np.ascontiguousarray(foot[:, i]) for i in range(4))
+    tiles = np.ascontiguousarray(tiles, TILE_DTYPE)
+    dp, bp, ep = VP(), VP(), VP()
+    bn, en = SZ(), SZ()
+    err = C.create_string_buffer(512)
+    _check(lib.tgx_batch_bin(cx.ctypes.data, cy.ctypes.data, reach.ctypes.data, d.ctypes.data,
+                             cx.size, tiles.ctypes.data, tiles.size, threads, C.byref(dp),
+                             C.byref(bp), C.byref(bn), C.byref(ep), C.byref(en), err, 512), err)
+    try:
+        descs = np.frombuffer(C.string_at(dp.value, tiles.size * 48), TILE_DESC_DTYPE).copy()
+        bins = np.frombuffer(C.string_at(bp.value, bn.value * 16), BIN_DTYPE).copy()
+        entries = np.frombuffer(C.string_at(ep.value, max(en.value, 1) * 4), np.uint32)[
+            :en.value].copy()
+    finally:
+        for p in (dp, bp, ep):
+            lib.tgx_free(p)
+    return descs, bins, entries

@@ -159,3 +159,168 @@ class PlaneTerrain:
         if getattr(self, "crater_set", None) is not None:
             self.dev.lib.tg_hip_crater_set_free(self.crater_set)
             self.crater_set = None
+
+
+def apply_order(craters: np.ndarray) -> np.ndarray:
+    """CraterApplyOrder, oldest first (src/layers/craters/shaping/apply_options.cpp:96-128):
+    decreasing age, then decreasing diameter, then increasing ID, then list index."""
+    idx = np.arange(craters.size)
+    return np.lexsort((idx, craters["id"], -craters["diameter"], -craters["age_gyr"]))
+
+
+@dataclass
+class TileBatch:
+    """Many plane tiles evaluated in one persistent launch against a resident crater catalogue
+    (tg_hip_catalogue_*, tg_hip_batch_*), and, for comparison, one at a time through the plain
+    synchronous path (tg_hip_run per layer, host memory in and out)."""
+    dev: N.HipDevice
+    source_fractals: list
+    craters: CraterLayerSpec
+    region: tuple  # (min_x, min_y, max_x, max_y) of the catalogue, meters
+    finest_pixel: float = 0.25
+    elevation: float = 0.0
+    timings: dict = field(default_factory=dict)
+
+    def build(self) -> None:
+        spec = self.craters
+        x0, y0, x1, y1 = self.region
+        t0 = time.perf_counter()
+        gen = N.Generation(min_x=x0, min_y=y0, max_x=x1, max_y=y1,
+                           sfd_coefficient=spec.sfd_coefficient, sfd_exponent=spec.sfd_exponent,
+                           min_diameter=spec.min_diameter, max_diameter=spec.max_diameter,
+                           margin_radii=N.margin_radii(spec.models(), spec.apply()),
+                           cell_size_factor=16.0, min_resolved_diameter=2.0 * self.finest_pixel,
+                           lifetime_gyr=spec.lifetime_gyr, seed=spec.seed)
+        self.crater_list = N.generate_craters(gen)
+        t1 = time.perf_counter()
+        self.kernel, self.records, self.data, self.foot = N.prepare_craters(
+            self.crater_list, spec.models(), spec.apply(), spec.seed, self.finest_pixel)
+        # The catalogue rows in apply order, for per-tile preparation on the plain path; no crater
+        # is dropped at the finest pixel size, so record i is row i.
+        if self.records.size != self.crater_list.size:
+            raise ValueError("the catalogue's finest pixel size drops craters: lower min_diameter "
+                             "or finest_pixel")
+        self.ordered = self.crater_list[apply_order(self.crater_list)]
+        t2 = time.perf_counter()
+        diam = np.ascontiguousarray(self.foot[:, 3])
+        cat = C.c_void_p()
+        self.dev.call("tg_hip_catalogue_create", self.dev.handle, C.byref(self.kernel),
+                      self.records.ctypes.data, self.records.size, self.data.ctypes.data,
+                      self.data.size, diam.ctypes.data, C.byref(cat))
+        self.catalogue = cat
+        t3 = time.perf_counter()
+        self.plan = None
+        if self.source_fractals:
+            kernel = N.noise_kernel(self.source_fractals)
+            plan = C.c_void_p()
+            self.dev.call("tg_hip_noise_plan_create", self.dev.handle, C.byref(kernel),
+                          C.byref(plan))
+            self.plan = plan
+            self.noise_kernel = kernel
+        self.batch = None
+        self.timings.update(generate_ms=(t1 - t0) * 1e3, prepare_ms=(t2 - t1) * 1e3,
+                            catalogue_upload_ms=(t3 - t2) * 1e3)
+
+    def stage(self, tiles: np.ndarray, threads: int = 0) -> None:
+        t0 = time.perf_counter()
+        self.tiles = np.ascontiguousarray(tiles, N.TILE_DTYPE)
+        self.descs, self.bins, self.entries = N.batch_bin(self.foot, self.tiles, threads)
+        t1 = time.perf_counter()
+        if self.batch is None or self._capacity < (self.tiles.size, self.bins.size,
+                                                   self.entries.size):
+            if self.batch is not None:
+                self.dev.lib.tg_hip_batch_free(self.batch)
+            cap = (int(self.tiles.size * 1.25) + 1, int(self.bins.size * 1.25) + 1,
+                   int(self.entries.size * 1.25) + 1)
+            b = C.c_void_p()
+            self.dev.call("tg_hip_batch_create", self.dev.handle, *cap, C.byref(b))
+            self.batch, self._capacity = b, cap
+        self.dev.call("tg_hip_batch_stage", self.batch, self.descs.ctypes.data, self.descs.size,
+                      self.bins.ctypes.data, self.bins.size, self.entries.ctypes.data,
+                      self.entries.size)
+        t2 = time.perf_counter()
+        self.cells = int((self.tiles["width"].astype(np.int64) *
+                          self.tiles["height"].astype(np.int64)).sum())
+        self.timings.update(bin_ms=(t1 - t0) * 1e3, stage_ms=(t2 - t1) * 1e3)
+
+    @property
+    def stream(self):
+        return self.dev.lib.tg_hip_batch_stream(self.batch)
+
+    def run(self, out: N.DeviceArray, use_graph: bool = True) -> None:
+        self.dev.call("tg_hip_batch_run", self.batch, self.plan, self.elevation, self.catalogue,
+                      out.ptr, out.nbytes // 4, 1 if use_graph else 0)
+
+    def wait(self) -> None:
+        self.dev.call("tg_hip_batch_wait", self.batch)
+
+    def tile_heights(self, out_host: np.ndarray, i: int) -> np.ndarray:
+        d = self.descs[i]
+        n = int(d["width"]) * int(d["height"])
+        o = int(d["out_offset"])
+        return out_host[o:o + n].reshape(int(d["height"]), int(d["width"]))
+
+    # ---- the plain per-tile path ---------------------------------------------------------
+    def plain_inputs(self, i: int) -> dict:
+        """Tile i's inputs as the reference's per-grid path builds them: its craters prepared at
+        its pixel size (resolution baked into the records), binned into its 16 x 16 tiles."""
+        t = self.tiles[i]
+        grid = N.Grid(int(t["width"]), int(t["height"]), float(t["origin_x"]),
+                      float(t["origin_y"]), float(t["pixel"]))
+        d = self.descs[i]
+        nb = int(d["bins_across"]) * ((int(d["height"]) + 15) // 16)
+        bins = self.bins[int(d["bin_first"]):int(d["bin_first"]) + nb]
+        idx = np.unique(np.concatenate([self.entries[b["first"]:b["first"] + b["count"]]
+                                        for b in bins] + [np.zeros(0, np.uint32)]))
+        cx64, cy64 = N.grid_centers(grid)
+        xs, xlo = N.split(cx64)
+        ys, ylo = N.split(cy64)
+        inputs = dict(grid=grid, xs=xs, ys=ys, lows=np.concatenate([xlo, ylo]), craters=None)
+        if idx.size > 0:
+            spec = self.craters
+            kernel, records, data, foot = N.prepare_craters(
+                self.ordered[idx], spec.models(), spec.apply(), spec.seed, grid.pixel)
+            tiles, entries = N.bin_plane(grid, foot[:, 0], foot[:, 1], foot[:, 2], 16)
+            inputs["craters"] = (kernel, records, data, tiles, entries)
+        return inputs
+
+    def plain_run(self, inputs: dict) -> np.ndarray:
+        """The tile through tg_hip_run: the noise layer kernel then the crater kernel, each a
+        synchronous call with host buffers, as RunKernel drives a device per layer."""
+        g = inputs["grid"]
+        h = np.full((g.height, g.width), self.elevation, np.float32)
+        params = N.KernelParams(g.width, g.height, N.NODATA_NONE, 0.0, 0)
+        if self.plan is not None:
+            self.dev.run(N.TG_KERNEL_NOISE_LAYER, self.noise_kernel, h, inputs["xs"],
+                         inputs["ys"], params)
+        if inputs["craters"] is not None:
+            kernel, records, data, tiles, entries = inputs["craters"]
+            extra = [(records, 0), (data, 0), (tiles, 0), (entries, 0),
+                     (np.zeros(1, N.FRAME_DTYPE), 0), (np.zeros(1, N.POINT_DTYPE), 0),
+                     (inputs["lows"], 0)]
+            self.dev.run(N.TG_KERNEL_CRATERS, kernel, h, inputs["xs"], inputs["ys"], params,
+                         extra)
+        return h
+
+    def close(self) -> None:
+        for name, fn in (("batch", "tg_hip_batch_free"), ("catalogue", "tg_hip_catalogue_free"),
+                         ("plan", "tg_hip_noise_plan_free")):
+            h = getattr(self, name, None)
+            if h is not None:
+                getattr(self.dev.lib, fn)(h)
+                setattr(self, name, None)
+
+
+def c4_tiles(count: int, side: float, seed: int = 4, min_k: int = 6, max_k: int = 11,
+             pixel_range=(0.25, 8.0)) -> np.ndarray:
+    """Mixed tiles: edges 2^k + 1 with k uniform in [min_k, max_k], pixel sizes log-uniform over
+    `pixel_range`, west/north edges uniform over a square of `side` meters."""
+    rng = np.random.default_rng(seed)
+    tiles = np.zeros(count, N.TILE_DTYPE)
+    edge = 2 ** rng.integers(min_k, max_k + 1, count) + 1
+    tiles["width"] = edge
+    tiles["height"] = edge
+    tiles["pixel"] = np.exp(rng.uniform(np.log(pixel_range[0]), np.log(pixel_range[1]), count))
+    tiles["origin_x"] = rng.uniform(0.0, side, count)
+    tiles["origin_y"] = rng.uniform(0.0, side, count)
+    return tiles

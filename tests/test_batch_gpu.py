@@ -1,0 +1,93 @@
+# SPDX-License-Identifier: BSD-3-Clause
+"""Batched tiles: one persistent launch over tiles of mixed size and pixel size gives, tile for
+tile, the same heights as evaluating each tile alone through the plain path, and the oracle's."""
+import ctypes as C
+
+import numpy as np
+import pytest
+
+from terraingen_amd import native as N
+from terraingen_amd.pipeline import CraterLayerSpec, TileBatch, c4_tiles
+
+pytestmark = pytest.mark.gpu
+
+C2_FRACTAL = dict(seed=1, wavelength=512.0, amplitude=20.0, octaves=8, lacunarity=2.0, gain=0.5)
+
+
+@pytest.fixture(scope="module")
+def batch(hip):
+    b = TileBatch(hip, [C2_FRACTAL], CraterLayerSpec(seed=4), region=(0.0, 0.0, 300.0, 300.0))
+    b.build()
+    tiles = c4_tiles(24, 300.0, seed=9, min_k=4, max_k=8, pixel_range=(0.25, 2.0))
+    b.stage(tiles)
+    yield b
+    b.close()
+
+
+def run_batch(hip, b, use_graph):
+    out = N.DeviceArray(hip, b.cells * 4)
+    b.run(out, use_graph)
+    b.wait()
+    return out.download(np.empty(b.cells, np.float32))
+
+
+def test_batched_equals_each_tile_alone(hip, batch):
+    got = run_batch(hip, batch, use_graph=False)
+    worst = 0.0
+    for i in range(batch.tiles.size):
+        alone = batch.plain_run(batch.plain_inputs(i))
+        mine = batch.tile_heights(got, i)
+        worst = max(worst, float(np.abs(mine - alone).max()))
+        assert np.array_equal(mine.view(np.uint32), alone.view(np.uint32)), i
+    print(f"[batch] {batch.tiles.size} tiles, {batch.cells} cells: bit-equal to the plain path",
+          flush=True)
+
+
+def test_graph_replays_match_eager(hip, batch):
+    """The captured graph, replayed several times with eager work in between, gives the eager
+    result every time."""
+    eager = run_batch(hip, batch, use_graph=False)
+    for _ in range(3):
+        again = run_batch(hip, batch, use_graph=True)
+        assert np.array_equal(again.view(np.uint32), eager.view(np.uint32))
+        hip.synchronize()
+
+
+def test_batched_tile_matches_oracle(hip, orc, batch):
+    got = run_batch(hip, batch, use_graph=True)
+    from crater_cases import oracle_heights_f64  # noqa: F401  (FP64 oracle on the plane)
+
+    for i in (0, 5, 11):
+        inp = batch.plain_inputs(i)
+        g = inp["grid"]
+        want = np.zeros((g.height, g.width), np.float32)
+        params = N.KernelParams(g.width, g.height, 0, 0.0, 0)
+        orc.or_noise_grid(C.byref(batch.noise_kernel), inp["xs"].ctypes.data,
+                          inp["ys"].ctypes.data, g.width, g.height, C.byref(params),
+                          want.ctypes.data, 1)
+        if inp["craters"] is not None:
+            kernel, records, data, tiles, entries = inp["craters"]
+            orc.or_shape_grid(C.byref(kernel), records.ctypes.data, data.ctypes.data,
+                              tiles.ctypes.data, entries.ctypes.data, inp["xs"].ctypes.data,
+                              inp["ys"].ctypes.data, inp["lows"].ctypes.data, g.width, g.height,
+                              C.byref(params), want.ctypes.data, 1)
+        d = np.abs(batch.tile_heights(got, i) - want)
+        assert d.max() <= 1e-4, (i, d.max())
+
+
+def test_stage_rejects_malformed_batches(hip, batch):
+    bins = batch.bins.copy()
+    bins["count"][0] = 10 ** 8  # runs past the entries
+    with pytest.raises(N.HipError):
+        hip.call("tg_hip_batch_stage", batch.batch, batch.descs.ctypes.data, batch.descs.size,
+                 bins.ctypes.data, bins.size, batch.entries.ctypes.data, batch.entries.size)
+    descs = batch.descs.copy()
+    descs["bins_across"][1] += 1
+    with pytest.raises(N.HipError):
+        hip.call("tg_hip_batch_stage", batch.batch, descs.ctypes.data, descs.size,
+                 batch.bins.ctypes.data, batch.bins.size, batch.entries.ctypes.data,
+                 batch.entries.size)
+    # Restage the good batch so later tests see it.
+    hip.call("tg_hip_batch_stage", batch.batch, batch.descs.ctypes.data, batch.descs.size,
+             batch.bins.ctypes.data, batch.bins.size, batch.entries.ctypes.data,
+             batch.entries.size)

@@ -59,7 +59,8 @@ def test_prepared_batch_is_the_references(name):
     assert bytes(kernel) == z["kernel"].tobytes()
     assert records.view(np.uint8).tobytes() == z["records"].tobytes()
     assert data.view(np.uint32).tobytes() == z["data"].view(np.uint32).tobytes()
-    assert foot.shape == (records.size, 3) and np.all(foot[:, 2] > 0)
+    assert foot.shape == (records.size, 4) and np.all(foot[:, 2] > 0)
+    assert np.array_equal(foot[:, 3].astype(np.float32), records["diameter"])
 
 
 def test_prepared_peaks_are_the_references():
