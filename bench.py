@@ -1,13 +1,27 @@
 # SPDX-License-Identifier: BSD-3-Clause
 """Throughput of the hip backend's noise + crater shaping hot path on one MI355X.
 
-Workload: the synthetic source 4097 x 4097 at 0.5 m/px (north-up, centred on 0, source origin
-{0, 0}), 8-octave Perlin fBm (base wavelength 512 m, lacunarity 2, gain 0.5, amplitude 20 m,
-seed 1), plus craters from the power-law cumulative SFD N(>D) = 0.05 D^-2 per m^2, D in [1, 200] m,
-uniform centres, ages uniform over 1 Gyr, seed 1 (Martin-Parkes-Dunstan profile, freshness
-degradation, irregular outlines), FP32 heights. Host placement, preparation and binning run once
-(reported as host_ms); one step is the device evaluation of the whole grid with every input
-resident in HBM: the source fill with noise, then the crater kernel. Prints one JSON line.
+Default workload (``--workload tiles``): 1024 tiles of mixed size and pixel size, evaluated in one
+persistent launch. Edges are 2^k + 1 with k uniform in 6..11, pixel sizes log-uniform in
+0.25-8 m, and west/north edges uniform over a 4472 m square. The synthetic-source noise is an
+8-octave Perlin fBm (base wavelength 512 m, lacunarity 2, gain 0.5, amplitude 20 m, seed 1, origin
+{0, 0}). A shared catalogue of about 1e6 craters covers the square. It follows the power-law
+cumulative SFD N(>D) = 0.05 D^-2 per m^2 with D in [1, 200] m and ages uniform over 1 Gyr, seed 4,
+using the Martin-Parkes-Dunstan profile with freshness degradation and irregular outlines. Heights
+are FP32, with FP64 cell centres.
+
+* ``value``: Gcells/s of the batched path. One step is one graph launch of the persistent kernel
+  over every tile, with the catalogue, bins and outputs resident in HBM.
+* ``plain``: Gcells/s of the plain per-tile path over the same tiles. Each tile goes through
+  ``tg_hip_run``, once for the noise layer and once for the craters, synchronous with host memory
+  in and out, as RunKernel drives a device. Host preparation of its inputs is excluded.
+* ``tile_latency_513_ms``: p50/p99 request-to-host latency of single 513 x 513 tiles through the
+  batched path: binning, staging, the launch and the copy back, one request at a time.
+
+``--workload grid`` is one 4097 x 4097 grid at 0.5 m, with the same noise and craters (seed 1),
+evaluated by the strip kernels.
+
+Prints one JSON line.
 """
 from __future__ import annotations
 
@@ -30,132 +44,267 @@ sys.path.insert(0, str(ROOT))
 #   a Perlin fBm octave per cell: rotation 6, scale and offset 4, lattice split 6, 4 corner hashes
 #   x 18, 4 gradients x 21, offsets 2, fades 14, lerps 9, sqrt(2) 1, weighted sum 2
 #   (include/terraingen/noise/fractal.h:228-232, perlin.h:17-33,72-90, hash.h:40-57);
-#   a (cell, crater) pair inside the reach: 110 (split offset, distance, reach test, profile,
-#   blend; craters.h:1186-1211) + 12 per outline harmonic (craters.h:787-803);
-#   a pair outside the reach: the same up to the reach test, 40 + 12 per harmonic.
+#   a (cell, crater) pair whose cell lies inside the crater's footprint (reach radii x outline
+#   bound x radius): 110 (split offset, distance, reach test, profile, blend;
+#   craters.h:1186-1211) + 12 per outline harmonic (craters.h:787-803) + 12 for an elongated
+#   outline. Pairs outside the footprint count nothing.
 # Frozen; never tuned to a measurement.
 OPS_PER_PERLIN_OCTAVE = 200
-OPS_PAIR_IN = 110
-OPS_PAIR_OUT = 40
+OPS_PAIR = 110
 OPS_PER_HARMONIC = 12
+OPS_ELLIPSE = 12
 VALU_PEAK = 256 * 64 * 2.4e9  # lane-ops/s: 256 CUs x 64 lanes x 2.4 GHz
 HBM_PEAK_GBS = 8000.0
+METRIC = "Gcells/s batched vs plain HIP at <=1e-3 m err; p50/p99 tile latency"
 
 C2_FRACTAL = dict(seed=1, wavelength=512.0, amplitude=20.0, octaves=8, lacunarity=2.0, gain=0.5)
+TILES_SIDE = 4472.0  # m: 0.05 x 4472^2 = 1.0e6 craters of D >= 1 m
 
 
-def pair_ops(terrain) -> tuple[float, int, int]:
-    """Algorithmic lane ops of the crater kernel over the grid: every (cell, listed crater) pair,
-    split by whether the cell lies within the crater's round reach (counted on the host from the
-    tile lists and the records, not by the kernel)."""
-    g = terrain.grid
-    side = int(terrain.tiles[1])
-    tiles_x = int(terrain.tiles[0])
-    tiles_y = (g.height + side - 1) // side
-    rec = terrain.records
-    reach_r = float(terrain.kernel.reach_radii) * 0.5 * rec["diameter"].astype(np.float64)
-    cx = rec["x"].astype(np.float64) + rec["x_low"]
-    cy = rec["y"].astype(np.float64) + rec["y_low"]
-    harm = rec["harmonic_count"].astype(np.float64)
-    xs, ys = N_grid_centers(g)
-    total_in = total_out = 0
-    ops = 0.0
-    for ty in range(tiles_y):
-        r0, r1 = ty * side, min(g.height, (ty + 1) * side)
-        for tx in range(tiles_x):
-            t = ty * tiles_x + tx
-            first, count = int(terrain.tiles[3 + 2 * t]), int(terrain.tiles[4 + 2 * t])
-            if count == 0:
-                continue
-            c0, c1 = tx * side, min(g.width, (tx + 1) * side)
-            idx = terrain.entries[first:first + count]
-            dx = xs[c0:c1][None, None, :] - cx[idx][:, None, None]
-            dy = ys[r0:r1][None, :, None] - cy[idx][:, None, None]
-            inside = (dx * dx + dy * dy) < (reach_r[idx] ** 2)[:, None, None]
-            n_in = inside.sum(axis=(1, 2))
-            cells = (r1 - r0) * (c1 - c0)
-            n_out = cells - n_in
-            ops += float((n_in * (OPS_PAIR_IN + OPS_PER_HARMONIC * harm[idx])).sum())
-            ops += float((n_out * (OPS_PAIR_OUT + OPS_PER_HARMONIC * harm[idx])).sum())
-            total_in += int(n_in.sum())
-            total_out += int(n_out.sum())
-    return ops, total_in, total_out
+def roofline(ops: float, kernel_ms: float, bytes_written: float) -> dict:
+    achieved = ops / (kernel_ms * 1e-3)
+    return {"bound": "valu", "achieved": round(achieved / 1e12, 3),
+            "peak": round(VALU_PEAK / 1e12, 3), "unit": "Tops/s",
+            "frac": round(achieved / VALU_PEAK, 4), "traffic": None,
+            "hbm_write_GBs": round(bytes_written / (kernel_ms * 1e-3) / 1e9, 1),
+            "hbm_write_frac": round(bytes_written / (kernel_ms * 1e-3) / 1e9 / HBM_PEAK_GBS, 5)}
 
 
-def N_grid_centers(g):
+def percentile(values, q) -> float:
+    return float(np.percentile(np.asarray(values), q))
+
+
+# ---- the CPU baseline leg (oracle restatement of the reference's CPU path) -------------------
+
+def cpu_tiles(batch, sample: list[int], cores: int) -> dict:
+    """The reference's CPU path over sampled tiles, timed by the oracle restatement: noise rows
+    then crater tiles, FP32 kernel math on all host threads. Also the FP64 oracle of the same
+    tiles, for the error of the hip result."""
+    from oracle import oracle
     from terraingen_amd import native as N
 
-    return N.grid_centers(g)
+    orc = oracle()
+    elapsed = 0.0
+    cells = 0
+    fp32, fp64 = {}, {}
+    for i in sample:
+        inp = batch.plain_inputs(i)
+        g = inp["grid"]
+        params = N.KernelParams(g.width, g.height, 0, 0.0, 0)
+        out = np.zeros((g.height, g.width), np.float32)
+        t0 = time.perf_counter()
+        orc.or_noise_grid(C.byref(batch.noise_kernel), inp["xs"].ctypes.data,
+                          inp["ys"].ctypes.data, g.width, g.height, C.byref(params),
+                          out.ctypes.data, cores)
+        if inp["craters"] is not None:
+            kernel, records, data, tiles, entries = inp["craters"]
+            orc.or_shape_grid(C.byref(kernel), records.ctypes.data, data.ctypes.data,
+                              tiles.ctypes.data, entries.ctypes.data, inp["xs"].ctypes.data,
+                              inp["ys"].ctypes.data, inp["lows"].ctypes.data, g.width,
+                              g.height, C.byref(params), out.ctypes.data, cores)
+        elapsed += time.perf_counter() - t0
+        cells += g.width * g.height
+        fp32[i] = out
+        cx64, cy64 = N.grid_centers(g)
+        cx64, cy64 = np.ascontiguousarray(cx64), np.ascontiguousarray(cy64)
+        ref = np.zeros((g.height, g.width), np.float64)
+        orc.or_noise_grid_f64(C.byref(batch.noise_kernel), cx64.ctypes.data, cy64.ctypes.data,
+                              g.width, g.height, ref.ctypes.data, cores)
+        if inp["craters"] is not None:
+            orc.or_shape_grid_f64(C.byref(kernel), records.ctypes.data, data.ctypes.data,
+                                  tiles.ctypes.data, entries.ctypes.data, cx64.ctypes.data,
+                                  cy64.ctypes.data, g.width, g.height, 0, ref.ctypes.data, cores)
+        fp64[i] = ref
+    return {"value": round(cells / elapsed / 1e9, 6), "unit": "Gcells/s", "cores": cores,
+            "kind": "port",
+            "sample": f"{len(sample)} of the tiles ({cells} cells), noise then craters, once "
+                      f"each, {platform.processor() or platform.machine()}",
+            "_fp32": fp32, "_fp64": fp64}
 
 
-def cpu_baseline(terrain, rows: int) -> dict:
-    """The reference's CPU path (FP32 kernel math over all host threads: noise rows, then crater
-    tiles), timed by the oracle restatement on a bounded sample of the grid's rows; also the
-    hip result's error on those rows against the FP64 oracle."""
+def cpu_grid(terrain, rows: int, cores: int) -> dict:
+    """As cpu_tiles for the single grid, on a band of its rows."""
     from oracle import oracle
     from terraingen_amd import native as N
 
     orc = oracle()
     g = terrain.grid
-    cores = os.cpu_count() or 1
     r0 = (g.height - rows) // 2
     sub_ys = np.ascontiguousarray(terrain.ys[r0:r0 + rows])
     kernel = N.noise_kernel([C2_FRACTAL])
     params = N.KernelParams(g.width, rows, 0, 0.0, r0)
     out = np.zeros((rows, g.width), np.float32)
-
-    def run_once():
-        out[:] = 0.0
-        orc.or_noise_grid(C.byref(kernel), terrain.xs.ctypes.data, sub_ys.ctypes.data, g.width,
-                          rows, C.byref(params), out.ctypes.data, cores)
-        orc.or_shape_grid(C.byref(terrain.kernel), terrain.records.ctypes.data,
-                          terrain.data.ctypes.data, terrain.tiles.ctypes.data,
-                          terrain.entries.ctypes.data, terrain.xs.ctypes.data,
-                          sub_ys.ctypes.data, terrain.lows.ctypes.data, g.width, rows,
-                          C.byref(params), out.ctypes.data, cores)
-
-    run_once()
-    best = float("inf")
-    for _ in range(3):
-        t0 = time.perf_counter()
-        run_once()
-        best = min(best, time.perf_counter() - t0)
-    # FP64 oracle on the same rows.
+    t0 = time.perf_counter()
+    orc.or_noise_grid(C.byref(kernel), terrain.xs.ctypes.data, sub_ys.ctypes.data, g.width,
+                      rows, C.byref(params), out.ctypes.data, cores)
+    orc.or_shape_grid(C.byref(terrain.kernel), terrain.records.ctypes.data,
+                      terrain.data.ctypes.data, terrain.tiles.ctypes.data,
+                      terrain.entries.ctypes.data, terrain.xs.ctypes.data, sub_ys.ctypes.data,
+                      terrain.lows.ctypes.data, g.width, rows, C.byref(params), out.ctypes.data,
+                      cores)
+    elapsed = time.perf_counter() - t0
     cx64, cy64 = N.grid_centers(g)
     xs64 = np.ascontiguousarray(cx64)
     ys64 = np.ascontiguousarray(cy64[r0:r0 + rows])
-    ref64 = np.zeros((rows, g.width), np.float64)
+    ref = np.zeros((rows, g.width), np.float64)
     orc.or_noise_grid_f64(C.byref(kernel), xs64.ctypes.data, ys64.ctypes.data, g.width, rows,
-                          ref64.ctypes.data, cores)
+                          ref.ctypes.data, cores)
     orc.or_shape_grid_f64(C.byref(terrain.kernel), terrain.records.ctypes.data,
                           terrain.data.ctypes.data, terrain.tiles.ctypes.data,
                           terrain.entries.ctypes.data, xs64.ctypes.data, ys64.ctypes.data,
-                          g.width, rows, r0, ref64.ctypes.data, cores)
+                          g.width, rows, r0, ref.ctypes.data, cores)
     cells = rows * g.width
-    return {"value": round(cells / best / 1e9, 5), "unit": "Gcells/s", "cores": cores,
+    return {"value": round(cells / elapsed / 1e9, 6), "unit": "Gcells/s", "cores": cores,
             "kind": "port",
-            "sample": f"rows {r0}..{r0 + rows - 1} (x {g.width} cells) of the 4097^2 grid: noise "
-                      f"then craters, best of 3, {platform.processor() or platform.machine()}",
-            "_ref64": ref64, "_r0": r0, "_fp32": out.copy()}
+            "sample": f"rows {r0}..{r0 + rows - 1} (x {g.width} cells) of the grid: noise then "
+                      f"craters, once, {platform.processor() or platform.machine()}",
+            "_r0": r0, "_fp32": out, "_fp64": ref}
 
 
-def main() -> int:
-    ap = argparse.ArgumentParser()
-    ap.add_argument("--gpus", type=int, default=1)
-    ap.add_argument("--steps", type=int, default=30)
-    ap.add_argument("--warmup", type=int, default=3)
-    ap.add_argument("--size", type=int, default=4097)
-    ap.add_argument("--no-cpu-baseline", action="store_true")
-    args = ap.parse_args()
-    if args.gpus != 1:
-        print(json.dumps({"error": "this build measures one GPU; tiles shard across GPUs without "
-                                   "exchange, but multi-GPU runs are not part of it"}))
-        return 2
+def error_summary(pairs) -> tuple[dict, float]:
+    """Max/RMS |hip - FP64 oracle| and max |hip - FP32 oracle| over (hip, fp32, fp64) triples."""
+    worst = worst32 = 0.0
+    sq = 0.0
+    n = 0
+    for got, f32, f64 in pairs:
+        d = np.abs(got.astype(np.float64) - f64)
+        worst = max(worst, float(d.max()))
+        sq += float((d ** 2).sum())
+        n += d.size
+        worst32 = max(worst32, float(np.abs(got - f32).max()))
+    return ({"max": float(f"{worst:.3g}"), "rms": float(f"{np.sqrt(sq / max(n, 1)):.3g}"),
+             "cells": n}, float(f"{worst32:.3g}"))
 
-    from terraingen_amd import native as N
-    from terraingen_amd.pipeline import CraterLayerSpec, PlaneTerrain
 
-    dev = N.HipDevice(0)
+# ---- workloads --------------------------------------------------------------------------------
+
+def bench_tiles(args, N, dev) -> dict:
+    from terraingen_amd.pipeline import CraterLayerSpec, TileBatch, c4_tiles, crater_pair_cost
+
+    t0 = time.perf_counter()
+    batch = TileBatch(dev, [C2_FRACTAL], CraterLayerSpec(seed=4),
+                      region=(0.0, 0.0, TILES_SIDE, TILES_SIDE))
+    batch.build()
+    tiles = c4_tiles(args.tiles, TILES_SIDE, seed=4)
+    batch.stage(tiles)
+    host_ms = (time.perf_counter() - t0) * 1e3
+    print(f"[bench] staged {tiles.size} tiles, {batch.cells} cells, {batch.records.size} craters,"
+          f" {batch.bins.size} bins, {batch.entries.size} entries", file=sys.stderr, flush=True)
+    out = N.DeviceArray(dev, batch.cells * 4)
+    stream = batch.stream
+    for _ in range(args.warmup):
+        batch.run(out, use_graph=True)
+    batch.wait()
+    # Timed: K graph launches on the batch's stream, an event pair around each.
+    events = [(N.Event(dev), N.Event(dev)) for _ in range(args.steps)]
+    start, stop = N.Event(dev), N.Event(dev)
+    t0 = time.perf_counter()
+    start.record(stream)
+    for a, b in events:
+        a.record(stream)
+        batch.run(out, use_graph=True)
+        b.record(stream)
+    stop.record(stream)
+    batch.wait()
+    wall_ms = (time.perf_counter() - t0) * 1e3 / args.steps
+    step_ms = start.elapsed_ms(stop) / args.steps
+    launch_ms = [a.elapsed_ms(b) for a, b in events]
+    kernel_ms = float(np.mean(launch_ms))
+    print(f"[bench] batched {step_ms:.3f} ms/step", file=sys.stderr, flush=True)
+    got = out.download(np.empty(batch.cells, np.float32))
+
+    # The plain path over every tile: tg_hip_run per layer with host buffers.
+    plain_s = 0.0
+    plain_cells = 0
+    worst_plain = 0.0
+    for i in range(tiles.size):
+        inp = batch.plain_inputs(i)
+        t1 = time.perf_counter()
+        h = batch.plain_run(inp)
+        plain_s += time.perf_counter() - t1
+        plain_cells += h.size
+        worst_plain = max(worst_plain, float(np.abs(h - batch.tile_heights(got, i)).max()))
+    plain = plain_cells / plain_s / 1e9
+    print(f"[bench] plain {plain:.3f} Gcells/s", file=sys.stderr, flush=True)
+
+    # Algorithmic work of one launch, counted on the host from the bins and footprints.
+    cells = batch.cells
+    ops_noise = cells * C2_FRACTAL["octaves"] * OPS_PER_PERLIN_OCTAVE
+    cost = crater_pair_cost(batch.records, OPS_PAIR, OPS_PER_HARMONIC, OPS_ELLIPSE)
+    listed, inside, ops_craters = N.batch_pair_ops(batch.descs, batch.bins, batch.entries,
+                                                   batch.foot, cost)
+    n_bins, n_entries = int(batch.bins.size), int(batch.entries.size)
+    value = cells / (step_ms * 1e-3) / 1e9
+
+    extra = {}
+    if not args.no_cpu_baseline:
+        cores = os.cpu_count() or 1
+        budget = args.cpu_cells
+        sample, total = [], 0
+        for i in range(0, tiles.size, 7):
+            n = int(tiles["width"][i]) * int(tiles["height"][i])
+            if total + n > budget:
+                continue
+            sample.append(i)
+            total += n
+        base = cpu_tiles(batch, sample, cores)
+        fp32, fp64 = base.pop("_fp32"), base.pop("_fp64")
+        err, diff32 = error_summary((batch.tile_heights(got, i), fp32[i], fp64[i])
+                                    for i in sample)
+        extra["cpu_baseline"] = base
+        extra["error_vs_fp64_oracle_m"] = err
+        extra["max_diff_vs_fp32_oracle_m"] = diff32
+
+    # Unloaded request latency of single 513 x 513 tiles through the batched path.
+    rng = np.random.default_rng(5)
+    lat = []
+    for r in range(args.latency_requests):
+        one = np.zeros(1, N.TILE_DTYPE)
+        one["width"] = one["height"] = 513
+        one["pixel"] = np.exp(rng.uniform(np.log(0.25), np.log(8.0)))
+        one["origin_x"], one["origin_y"] = rng.uniform(0.0, TILES_SIDE, 2)
+        host = np.empty(513 * 513, np.float32)
+        t1 = time.perf_counter()
+        batch.stage(one)
+        batch.run(out, use_graph=True)
+        out.download(host)  # synchronous: waits for the stream's work
+        lat.append((time.perf_counter() - t1) * 1e3)
+    lat = lat[min(3, len(lat) // 4):]  # the first requests capture a graph for the smaller grid
+
+    result = {
+        "metric": METRIC, "value": round(value, 3), "unit": "Gcells/s",
+        "ms_per_step": round(step_ms, 4), "wall_ms_per_step": round(wall_ms, 4),
+        "dtype": "f32+u32",
+        "config": {"workload": f"{tiles.size} mixed tiles, edges 2^k+1 (k 6-11), pixel "
+                               f"0.25-8 m, over a {TILES_SIDE:.0f} m square; 8-octave Perlin fBm "
+                               "(512 m, lac 2, gain 0.5, 20 m, seed 1) + shared catalogue of "
+                               "power-law craters N(>D)=0.05 D^-2, D 1-200 m, seed 4",
+                   "tiles": int(tiles.size), "cells": cells,
+                   "craters": int(batch.records.size), "bins": n_bins,
+                   "entries": n_entries, "pairs_listed": listed,
+                   "pairs_in_reach": inside},
+        "plain": {"value": round(plain, 4), "unit": "Gcells/s",
+                  "path": "tg_hip_run per tile and layer, host buffers, synchronous",
+                  "max_diff_vs_batched_m": worst_plain},
+        "batched_over_plain": round(value / plain, 2),
+        "tile_latency_513_ms": {"p50": round(percentile(lat, 50), 3),
+                                "p99": round(percentile(lat, 99), 3), "requests": len(lat),
+                                "mode": "one request at a time, to heights in host memory"},
+        "kernels_ms": {"batch_tiles": round(kernel_ms, 4)},
+        "host_ms": {k: round(v, 1) for k, v in batch.timings.items()} | {
+            "total_setup": round(host_ms, 1)},
+        "roofline": roofline(ops_noise + ops_craters, kernel_ms, cells * 4.0) | {
+            "kernel": "BatchTiles", "ops_noise": ops_noise, "ops_craters": ops_craters},
+    }
+    result |= extra
+    batch.close()
+    return result
+
+
+def bench_grid(args, N, dev) -> dict:
+    from terraingen_amd.pipeline import (CraterLayerSpec, PlaneTerrain, crater_pair_cost,
+                                         plane_batch_view)
+
     n = args.size
     grid = N.north_up_grid(n, n, 0.5)
     terrain = PlaneTerrain(dev, grid, [C2_FRACTAL], 0.0, CraterLayerSpec(seed=1))
@@ -166,7 +315,6 @@ def main() -> int:
     sv = C.c_void_p()
     dev.call("tg_hip_stream_create", dev.handle, C.byref(sv))
     stream = sv.value
-
     for _ in range(args.warmup):
         terrain.launch(heights, stream)
     dev.synchronize(stream)
@@ -180,7 +328,6 @@ def main() -> int:
     wall = time.perf_counter() - t0
     step_ms = start.elapsed_ms(stop) / args.steps
 
-    # Each kernel alone, on the same stream, for the roofline of the dominant one.
     def time_kernel(fn):
         a, b = N.Event(dev), N.Event(dev)
         a.record(stream)
@@ -195,55 +342,70 @@ def main() -> int:
     terrain.launch(heights, stream)
     dev.synchronize(stream)
     got = heights.download(np.empty((n, n), np.float32))
-
     cells = n * n
-    gcells = cells / (step_ms * 1e-3) / 1e9
     noise_ops = cells * C2_FRACTAL["octaves"] * OPS_PER_PERLIN_OCTAVE
-    crater_ops, pairs_in, pairs_out = pair_ops(terrain)
-    dominant = "craters" if crater_ms >= noise_ms else "noise"
+    descs, bins, entries = plane_batch_view(grid, terrain.tiles, terrain.entries)
+    cost = crater_pair_cost(terrain.records, OPS_PAIR, OPS_PER_HARMONIC, OPS_ELLIPSE)
+    listed, inside, crater_ops = N.batch_pair_ops(descs, bins, entries, terrain.foot, cost)
+    dominant = "CraterStrip" if crater_ms >= noise_ms else "NoiseFillStrip"
     dom_ms = max(crater_ms, noise_ms)
-    dom_ops = crater_ops if dominant == "craters" else noise_ops
-    achieved = dom_ops / (dom_ms * 1e-3)
+    dom_ops = crater_ops if crater_ms >= noise_ms else noise_ops
     result = {
-        "metric": "Gcells/s batched vs plain HIP at <=1e-3 m err; p50/p99 tile latency",
-        "value": round(gcells, 3),
-        "unit": "Gcells/s",
-        "n_gpus": 1,
-        "steps": args.steps,
-        "warmup": args.warmup,
-        "ms_per_step": round(step_ms, 4),
-        "wall_ms_per_step": round(wall * 1e3 / args.steps, 4),
-        "higher_is_better": True,
-        "scaling": "weak",
-        "vs_baseline": None,
+        "metric": METRIC, "value": round(cells / (step_ms * 1e-3) / 1e9, 3), "unit": "Gcells/s",
+        "ms_per_step": round(step_ms, 4), "wall_ms_per_step": round(wall * 1e3 / args.steps, 4),
         "dtype": "f32+u32",
-        "data": "synthetic",
         "config": {"workload": f"synthetic source {n}x{n} @0.5 m: 8-octave Perlin fBm (512 m, "
                                "lac 2, gain 0.5, 20 m, seed 1) + power-law craters "
                                "N(>D)=0.05 D^-2, D 1-200 m, seed 1",
                    "cells": cells, "craters": int(terrain.crater_count),
-                   "pairs_in_reach": pairs_in, "pairs_listed": pairs_in + pairs_out},
+                   "pairs_listed": listed, "pairs_in_reach": inside},
         "kernels_ms": {"noise_fill": round(noise_ms, 4), "craters": round(crater_ms, 4)},
         "host_ms": {k: round(v, 1) for k, v in terrain.timings.items()} | {
             "total_prepare": round(host_ms, 1)},
-        "roofline": {"bound": "valu", "kernel": dominant, "achieved": round(achieved / 1e12, 3),
-                     "peak": round(VALU_PEAK / 1e12, 3), "unit": "Tops/s",
-                     "frac": round(achieved / VALU_PEAK, 4), "traffic": None,
-                     "hbm_write_GBs": round(cells * 4 / (step_ms * 1e-3) / 1e9, 1)},
+        "roofline": roofline(dom_ops, dom_ms, cells * 4.0) | {"kernel": dominant},
     }
     if not args.no_cpu_baseline:
-        base = cpu_baseline(terrain, 16)
-        r0 = base.pop("_r0")
-        ref64 = base.pop("_ref64")
-        fp32 = base.pop("_fp32")
-        d = np.abs(got[r0:r0 + ref64.shape[0]] - ref64)
+        base = cpu_grid(terrain, 16, os.cpu_count() or 1)
+        r0, fp32, fp64 = base.pop("_r0"), base.pop("_fp32"), base.pop("_fp64")
+        err, diff32 = error_summary([(got[r0:r0 + fp32.shape[0]], fp32, fp64)])
         result["cpu_baseline"] = base
-        result["error_vs_fp64_oracle_m"] = {"max": float(f"{d.max():.3g}"),
-                                            "rms": float(f"{np.sqrt((d ** 2).mean()):.3g}"),
-                                            "rows": int(ref64.shape[0])}
-        result["max_diff_vs_fp32_oracle_m"] = float(
-            f"{np.abs(got[r0:r0 + fp32.shape[0]] - fp32).max():.3g}")
+        result["error_vs_fp64_oracle_m"] = err
+        result["max_diff_vs_fp32_oracle_m"] = diff32
     terrain.close()
+    return result
+
+
+def main() -> int:
+    ap = argparse.ArgumentParser()
+    ap.add_argument("--gpus", type=int, default=1)
+    ap.add_argument("--steps", type=int, default=10)
+    ap.add_argument("--warmup", type=int, default=2)
+    ap.add_argument("--workload", choices=("tiles", "grid"), default="tiles")
+    ap.add_argument("--tiles", type=int, default=1024)
+    ap.add_argument("--size", type=int, default=4097, help="grid workload edge")
+    ap.add_argument("--latency-requests", type=int, default=200)
+    ap.add_argument("--cpu-cells", type=int, default=6_000_000,
+                    help="cells of the tiles workload timed on the host (CPU baseline)")
+    ap.add_argument("--no-cpu-baseline", action="store_true")
+    args = ap.parse_args()
+    if args.gpus != 1:
+        print(json.dumps({"error": "this build measures one GPU: tiles would shard across GPUs "
+                                   "with no exchange between them, but runs on more than one "
+                                   "GPU are not part of it"}), flush=True)
+        return 2
+
+    from terraingen_amd import native as N
+
+    dev = N.HipDevice(0)
+    if args.workload == "tiles":
+        body = bench_tiles(args, N, dev)
+    else:
+        body = bench_grid(args, N, dev)
+    result = {"metric": body.pop("metric"), "value": body.pop("value"), "unit": body.pop("unit"),
+              "n_gpus": 1, "steps": args.steps, "warmup": args.warmup,
+              "ms_per_step": body.pop("ms_per_step"), "higher_is_better": True,
+              "scaling": "weak", "vs_baseline": None, "dtype": body.pop("dtype"),
+              "data": "synthetic", "config": body.pop("config")} | body
     print(json.dumps(result), flush=True)
     return 0
 

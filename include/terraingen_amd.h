@@ -141,6 +141,17 @@ int tgx_batch_bin(const double* cx, const double* cy, const double* reach, const
                   tg_tile_desc** descs, tg_bin** bins, size_t* bin_count, uint32_t** entries,
                   size_t* entry_count, char* err, size_t err_len);
 
+/* Counts the (cell, crater) pairs of a binned batch (the output of tgx_batch_bin) whose FP64 cell
+ * centre lies strictly inside the crater's footprint half-width `reach`: the pairs a crater kernel
+ * evaluates in full. `listed` receives every (cell, listed crater) pair, `inside` those within
+ * reach and `ops` the sum of `cost[k]` over them. A measurement aid with no counterpart in the
+ * reference; it never touches a device. */
+int tgx_batch_pair_ops(const tg_tile_desc* descs, size_t tile_count, const tg_bin* bins,
+                       size_t bin_count, const uint32_t* entries, size_t entry_count,
+                       const double* cx, const double* cy, const double* reach,
+                       const double* cost, size_t n, unsigned threads, uint64_t* listed,
+                       uint64_t* inside, double* ops, char* err, size_t err_len);
+
 #ifdef __cplusplus
 }
 #endif

@@ -1,10 +1,10 @@
 // SPDX-License-Identifier: BSD-3-Clause
 //
 // Batched tiles: one persistent launch evaluates many tiles of uneven size and pixel size. The
-// work is a queue of bins (16 x 16 cells of one tile); each workgroup takes the next bin with one
+// work is a queue of bins (16 x 16 cells of one tile); each wave takes the next bin with one
 // atomic add, evaluates its cells in registers (FP64 cell centres narrowed to FP32 as RunKernel
 // narrows them, the source's noise, then the bin's craters in catalogue order) and writes the
-// heights once. A workgroup keeps taking bins until the queue is empty, so tiles of any mix of sizes
+// heights once. A wave keeps taking bins until the queue is empty, so tiles of any mix of sizes
 // fill every CU without a launch per tile, and nothing is read back or allocated between tiles.
 
 #include "crater_device.h"
@@ -48,6 +48,7 @@ struct tg_batch {
     const void* graphCatalogue = nullptr;
     float* graphOut = nullptr;
     float graphElevation = 0.0f;
+    uint32_t graphGroups = 0;
 };
 
 namespace {
@@ -174,15 +175,19 @@ void FreeCatalogue(tg_catalogue* c) {
     }
 }
 
-void Launch(tg_batch& b, const tg_noise_plan* plan, float elevation, const tg_catalogue* cat,
-            float* out) {
-    // One wave per bin up to a few workgroups per CU: every CU stays busy and each wave keeps
-    // taking bins until the queue is empty.
+// One wave per bin up to a few workgroups per CU: every CU stays busy and each wave keeps
+// taking bins until the queue is empty.
+uint32_t GroupsFor(const tg_batch& b) {
     int cus = 256;
     (void)hipDeviceGetAttribute(&cus, hipDeviceAttributeMultiprocessorCount, b.dev->ordinal);
     const size_t wavesPerGroup = kGroupThreads / 64;
-    const uint32_t groups = uint32_t(std::max<size_t>(
+    return uint32_t(std::max<size_t>(
         1, std::min<size_t>((b.binCount + wavesPerGroup - 1) / wavesPerGroup, size_t(cus) * 8)));
+}
+
+void Launch(tg_batch& b, const tg_noise_plan* plan, float elevation, const tg_catalogue* cat,
+            float* out) {
+    const uint32_t groups = GroupsFor(b);
     hipLaunchKernelGGL(ResetQueue, dim3(1), dim3(1), 0, b.stream, b.counters);
     hipLaunchKernelGGL(BatchTiles, dim3(groups), dim3(kGroupThreads), 0, b.stream, b.tiles, b.bins,
                        b.entries, b.counters, plan != nullptr ? plan->device : nullptr,
@@ -404,8 +409,11 @@ int tg_hip_batch_run(tg_batch* b, const tg_noise_plan* plan, float elevation,
             Launch(*b, plan, elevation, cat, out);
             return;
         }
+        // The graph holds the launch's arguments and grid, not the staged batch (the kernel reads
+        // the bin count from device memory), so a restage of the same grid replays it.
+        const uint32_t groups = GroupsFor(*b);
         if (b->graph == nullptr || b->graphPlan != plan || b->graphCatalogue != cat ||
-            b->graphOut != out || b->graphElevation != elevation) {
+            b->graphOut != out || b->graphElevation != elevation || b->graphGroups != groups) {
             DropGraph(*b);
             hipGraph_t graph = nullptr;
             Check(hipStreamBeginCapture(b->stream, hipStreamCaptureModeThreadLocal),
@@ -427,6 +435,7 @@ int tg_hip_batch_run(tg_batch* b, const tg_noise_plan* plan, float elevation,
             b->graphCatalogue = cat;
             b->graphOut = out;
             b->graphElevation = elevation;
+            b->graphGroups = groups;
         }
         Check(hipGraphLaunch(b->graph, b->stream), "hipGraphLaunch");
     });

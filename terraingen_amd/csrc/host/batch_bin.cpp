@@ -210,9 +210,104 @@ void BinBatch(const double* cx, const double* cy, const double* reach, const dou
     }
 }
 
+// The (cell, crater) pairs of a binned batch whose FP64 cell centre lies strictly inside the
+// crater's footprint radius (reach radii x outline bound x R), the pairs a crater kernel must
+// evaluate in full, with the summed per-crater cost of those pairs. Independent of any kernel: the
+// cell centres are the FP64 PixelToWorld centres.
+void BatchPairOps(const tg_tile_desc* descs, std::size_t tileCount, const tg_bin* bins,
+                  std::size_t binCount, const std::uint32_t* entries, std::size_t entryCount,
+                  const double* cx, const double* cy, const double* reach, const double* cost,
+                  std::size_t n, unsigned threads, std::uint64_t& listed, std::uint64_t& inside,
+                  double& ops) {
+    constexpr std::uint32_t kSide = 16;
+    for (std::size_t b = 0; b < binCount; ++b) {
+        const tg_bin& bin = bins[b];
+        if (bin.tile >= tileCount || std::uint64_t(bin.first) + bin.count > entryCount) {
+            throw std::invalid_argument("bin " + std::to_string(b) + " is out of range");
+        }
+    }
+    for (std::size_t e = 0; e < entryCount; ++e) {
+        if (entries[e] >= n) {
+            throw std::invalid_argument("entry " + std::to_string(e) + " is out of range");
+        }
+    }
+    if (threads == 0) {
+        threads = std::max(1u, std::thread::hardware_concurrency());
+    }
+    std::atomic<std::size_t> next{0};
+    std::mutex sumMutex;
+    listed = inside = 0;
+    ops = 0.0;
+    auto work = [&] {
+        std::uint64_t myListed = 0, myInside = 0;
+        double myOps = 0.0;
+        constexpr std::size_t kChunk = 256;
+        for (;;) {
+            const std::size_t begin = next.fetch_add(kChunk);
+            if (begin >= binCount) {
+                break;
+            }
+            const std::size_t end = std::min(binCount, begin + kChunk);
+            for (std::size_t b = begin; b < end; ++b) {
+                const tg_bin& bin = bins[b];
+                const tg_tile_desc& t = descs[bin.tile];
+                const std::uint32_t local = std::uint32_t(b) - t.bin_first;
+                const std::uint32_t c0 = (local % t.bins_across) * kSide;
+                const std::uint32_t r0 = (local / t.bins_across) * kSide;
+                const std::uint32_t c1 = std::min(t.width, c0 + kSide);
+                const std::uint32_t r1 = std::min(t.height, r0 + kSide);
+                for (std::uint32_t e = bin.first; e < bin.first + bin.count; ++e) {
+                    const std::uint32_t k = entries[e];
+                    const double r2 = reach[k] * reach[k];
+                    std::uint64_t count = 0;
+                    for (std::uint32_t row = r0; row < r1; ++row) {
+                        const double dy = t.origin_y - (double(row) + 0.5) * t.pixel - cy[k];
+                        for (std::uint32_t col = c0; col < c1; ++col) {
+                            const double dx = t.origin_x + (double(col) + 0.5) * t.pixel - cx[k];
+                            count += (dx * dx + dy * dy < r2) ? 1u : 0u;
+                        }
+                    }
+                    myListed += std::uint64_t(r1 - r0) * (c1 - c0);
+                    myInside += count;
+                    myOps += double(count) * cost[k];
+                }
+            }
+        }
+        std::lock_guard<std::mutex> lock(sumMutex);
+        listed += myListed;
+        inside += myInside;
+        ops += myOps;
+    };
+    std::vector<std::thread> pool;
+    for (unsigned i = 1; i < threads; ++i) {
+        pool.emplace_back(work);
+    }
+    work();
+    for (std::thread& th : pool) {
+        th.join();
+    }
+}
+
 }  // namespace tgh
 
 extern "C" {
+
+int tgx_batch_pair_ops(const tg_tile_desc* descs, size_t tileCount, const tg_bin* bins,
+                       size_t binCount, const uint32_t* entries, size_t entryCount,
+                       const double* cx, const double* cy, const double* reach,
+                       const double* cost, size_t n, unsigned threads, uint64_t* listed,
+                       uint64_t* inside, double* ops, char* err, size_t errLen) {
+    return tgh::Guard(err, errLen, [&] {
+        if (listed == nullptr || inside == nullptr || ops == nullptr ||
+            (tileCount > 0 && descs == nullptr) || (binCount > 0 && bins == nullptr) ||
+            (entryCount > 0 && entries == nullptr) ||
+            (n > 0 && (cx == nullptr || cy == nullptr || reach == nullptr || cost == nullptr))) {
+            throw std::invalid_argument("tgx_batch_pair_ops: null argument");
+        }
+        tgh::BatchPairOps(descs, tileCount, bins, binCount, entries, entryCount, cx, cy, reach,
+                          cost, n, threads, *listed, *inside, *ops);
+    });
+}
 
 int tgx_batch_bin(const double* cx, const double* cy, const double* reach,
                   const double* diameter, size_t n, const tgx_tile* tiles, size_t tileCount,

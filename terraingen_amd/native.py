@@ -250,9 +250,11 @@ class DeviceArray:
         assert arr.nbytes <= self.nbytes
         self.dev.call("tg_hip_memcpy_h2d", self.dev.handle, self.ptr, arr.ctypes.data, arr.nbytes)
 
-    def download(self, out: np.ndarray) -> np.ndarray:
-        assert out.flags.c_contiguous and out.nbytes <= self.nbytes
-        self.dev.call("tg_hip_memcpy_d2h", self.dev.handle, out.ctypes.data, self.ptr, out.nbytes)
+    def download(self, out: np.ndarray, offset: int = 0) -> np.ndarray:
+        """Copies `out.nbytes` bytes starting `offset` bytes into the array into `out`."""
+        assert out.flags.c_contiguous and 0 <= offset and offset + out.nbytes <= self.nbytes
+        self.dev.call("tg_hip_memcpy_d2h", self.dev.handle, out.ctypes.data, self.ptr + offset,
+                      out.nbytes)
         return out
 
     def zero(self) -> None:
@@ -470,3 +472,30 @@ def batch_bin(foot: np.ndarray, tiles: np.ndarray, threads: int = 0):
         for p in (dp, bp, ep):
             lib.tgx_free(p)
     return descs, bins, entries
+
+
+HOST_ABI["tgx_batch_pair_ops"] = (C.c_int, [VP, SZ, VP, SZ, VP, SZ, VP, VP, VP, VP, SZ, C.c_uint,
+                                            C.POINTER(C.c_uint64), C.POINTER(C.c_uint64),
+                                            C.POINTER(C.c_double), CP, SZ])
+
+
+def batch_pair_ops(descs, bins, entries, foot: np.ndarray, cost: np.ndarray, threads: int = 0):
+    """``tgx_batch_pair_ops``: (pairs listed, pairs inside the footprint, summed cost of those)
+    for a binned batch, with `cost` one value per catalogue crater."""
+    lib = host_library()
+    foot = np.ascontiguousarray(foot, np.float64)
+    cx, cy, reach = (np.ascontiguousarray(foot[:, i]) for i in range(3))
+    cost = np.ascontiguousarray(cost, np.float64)
+    if cost.size != cx.size:
+        raise ValueError("one cost per catalogue crater")
+    descs = np.ascontiguousarray(descs, TILE_DESC_DTYPE)
+    bins = np.ascontiguousarray(bins, BIN_DTYPE)
+    entries = np.ascontiguousarray(entries, np.uint32)
+    listed, inside, ops = C.c_uint64(), C.c_uint64(), C.c_double()
+    err = C.create_string_buffer(512)
+    _check(lib.tgx_batch_pair_ops(descs.ctypes.data, descs.size, bins.ctypes.data, bins.size,
+                                  entries.ctypes.data, entries.size, cx.ctypes.data,
+                                  cy.ctypes.data, reach.ctypes.data, cost.ctypes.data, cx.size,
+                                  threads, C.byref(listed), C.byref(inside), C.byref(ops), err,
+                                  512), err)
+    return int(listed.value), int(inside.value), float(ops.value)

@@ -102,7 +102,7 @@ class PlaneTerrain:
         t2 = time.perf_counter()
         tiles, entries = N.bin_plane(g, foot[:, 0], foot[:, 1], foot[:, 2], self.tile_side)
         t3 = time.perf_counter()
-        self.kernel, self.records, self.data = kernel, records, data
+        self.kernel, self.records, self.data, self.foot = kernel, records, data, foot
         self.tiles, self.entries = tiles, entries
         frames = np.zeros(1, N.FRAME_DTYPE)
         out = C.c_void_p()
@@ -269,9 +269,10 @@ class TileBatch:
                       float(t["origin_y"]), float(t["pixel"]))
         d = self.descs[i]
         nb = int(d["bins_across"]) * ((int(d["height"]) + 15) // 16)
-        bins = self.bins[int(d["bin_first"]):int(d["bin_first"]) + nb]
-        idx = np.unique(np.concatenate([self.entries[b["first"]:b["first"] + b["count"]]
-                                        for b in bins] + [np.zeros(0, np.uint32)]))
+        first = self.bins[int(d["bin_first"])]
+        last = self.bins[int(d["bin_first"]) + nb - 1]
+        # A tile's bins list their entries one after another.
+        idx = np.unique(self.entries[int(first["first"]):int(last["first"]) + int(last["count"])])
         cx64, cy64 = N.grid_centers(grid)
         xs, xlo = N.split(cx64)
         ys, ylo = N.split(cy64)
@@ -324,3 +325,28 @@ def c4_tiles(count: int, side: float, seed: int = 4, min_k: int = 6, max_k: int 
     tiles["origin_x"] = rng.uniform(0.0, side, count)
     tiles["origin_y"] = rng.uniform(0.0, side, count)
     return tiles
+
+
+def plane_batch_view(grid: N.Grid, tiles: np.ndarray, entries: np.ndarray):
+    """One grid's 16 x 16 tile lists (``tgx_bin_plane``'s ``[tilesX, side, onBody, (first,
+    count)...]``) as a one-tile batch: (tile descriptors, bins, entries)."""
+    across, side = int(tiles[0]), int(tiles[1])
+    if side != 16:
+        raise ValueError("batched bins are 16 x 16 cells")
+    down = (grid.height + side - 1) // side
+    descs = np.zeros(1, N.TILE_DESC_DTYPE)
+    descs["origin_x"], descs["origin_y"], descs["pixel"] = (grid.origin_x, grid.origin_y,
+                                                            grid.pixel)
+    descs["width"], descs["height"], descs["bins_across"] = grid.width, grid.height, across
+    bins = np.zeros(across * down, N.BIN_DTYPE)
+    bins["first"] = tiles[3:3 + 2 * bins.size:2]
+    bins["count"] = tiles[4:4 + 2 * bins.size:2]
+    return descs, bins, np.ascontiguousarray(entries, np.uint32)
+
+
+def crater_pair_cost(records: np.ndarray, ops_pair: float, ops_harmonic: float,
+                     ops_ellipse: float) -> np.ndarray:
+    """Per-crater algorithmic cost of one (cell, crater) pair inside the reach: the base pair cost,
+    a term per outline harmonic, and one for an elongated outline."""
+    return (ops_pair + ops_harmonic * records["harmonic_count"].astype(np.float64) +
+            ops_ellipse * (records["ellipticity"] != 1.0))

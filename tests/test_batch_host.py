@@ -1,0 +1,106 @@
+# SPDX-License-Identifier: BSD-3-Clause
+"""Host side of the batched path, on the CPU: binning many tiles at once gives, tile for tile, the
+lists the per-grid binning (TileLists, src/layers/craters/shaping/apply.cpp:119-140) gives for the
+craters that tile resolves; the pair counter agrees with brute force."""
+import numpy as np
+import pytest
+
+from terraingen_amd import native as N
+
+
+def catalogue(n, side, seed):
+    rng = np.random.default_rng(seed)
+    d = np.exp(rng.uniform(np.log(0.6), np.log(60.0), n))
+    foot = np.empty((n, 4))
+    foot[:, 0] = rng.uniform(0.0, side, n)
+    foot[:, 1] = rng.uniform(0.0, side, n)
+    foot[:, 2] = 1.5 * 1.12 * 0.5 * d
+    foot[:, 3] = d
+    return foot
+
+
+def tiles_for(side, seed):
+    rng = np.random.default_rng(seed)
+    t = np.zeros(7, N.TILE_DTYPE)
+    t["width"] = [17, 33, 65, 16, 1, 40, 129]
+    t["height"] = [17, 20, 65, 48, 3, 40, 31]
+    t["pixel"] = [0.25, 0.5, 1.0, 2.0, 0.3, 4.0, 0.75]
+    t["origin_x"] = rng.uniform(0.0, side, t.size)
+    t["origin_y"] = rng.uniform(0.0, side, t.size)
+    return t
+
+
+def resolved(d, pixel):
+    """ResolutionWeight(D, pixel) != 0 with the 2-4 sample fade (src/layers/fade.cpp:48-52)."""
+    x = d / pixel
+    t = np.clip((x - 2.0) / 2.0, 0.0, 1.0)
+    w = 1.0 - (1.0 - t * t * (3.0 - 2.0 * t))
+    return np.where(x <= 2.0, False, np.where(x >= 4.0, True, w != 0.0))
+
+
+@pytest.mark.parametrize("threads", [1, 3])
+def test_batch_bins_equal_per_tile_binning(threads):
+    side = 200.0
+    foot = catalogue(3000, side, 1)
+    tiles = tiles_for(side, 2)
+    descs, bins, entries = N.batch_bin(foot, tiles, threads)
+    assert descs.size == tiles.size
+    offset = 0
+    for i, t in enumerate(tiles):
+        d = descs[i]
+        w, h = int(t["width"]), int(t["height"])
+        assert (int(d["width"]), int(d["height"])) == (w, h)
+        assert int(d["out_offset"]) == offset
+        offset += w * h
+        across, down = (w + 15) // 16, (h + 15) // 16
+        assert int(d["bins_across"]) == across
+        grid = N.Grid(w, h, float(t["origin_x"]), float(t["origin_y"]), float(t["pixel"]))
+        keep = np.flatnonzero(resolved(foot[:, 3], float(t["pixel"])))
+        ref_tiles, ref_entries = N.bin_plane(grid, foot[keep, 0], foot[keep, 1], foot[keep, 2])
+        assert int(ref_tiles[0]) == across
+        for b in range(across * down):
+            mine = bins[int(d["bin_first"]) + b]
+            assert int(mine["tile"]) == i
+            got = entries[int(mine["first"]):int(mine["first"]) + int(mine["count"])]
+            first, count = int(ref_tiles[3 + 2 * b]), int(ref_tiles[4 + 2 * b])
+            want = keep[ref_entries[first:first + count]]
+            assert np.array_equal(got, want), (i, b)
+    assert bins.size == sum(int(d["bins_across"]) * ((int(d["height"]) + 15) // 16)
+                            for d in descs)
+
+
+def test_batch_bin_rejects_malformed_tiles():
+    foot = catalogue(10, 50.0, 3)
+    tiles = tiles_for(50.0, 4)
+    tiles["pixel"][2] = 0.0
+    with pytest.raises(RuntimeError, match="malformed"):
+        N.batch_bin(foot, tiles)
+
+
+def test_pair_counter_matches_brute_force():
+    side = 120.0
+    foot = catalogue(800, side, 5)
+    tiles = tiles_for(side, 6)
+    descs, bins, entries = N.batch_bin(foot, tiles, 2)
+    cost = 100.0 + np.arange(foot.shape[0]) % 7
+    listed, inside, ops = N.batch_pair_ops(descs, bins, entries, foot, cost, 2)
+    want_listed = want_inside = 0
+    want_ops = 0.0
+    for b, bin_ in enumerate(bins):
+        d = descs[int(bin_["tile"])]
+        local = b - int(d["bin_first"])
+        c0 = (local % int(d["bins_across"])) * 16
+        r0 = (local // int(d["bins_across"])) * 16
+        c1, r1 = min(int(d["width"]), c0 + 16), min(int(d["height"]), r0 + 16)
+        xs = float(d["origin_x"]) + (np.arange(c0, c1) + 0.5) * float(d["pixel"])
+        ys = float(d["origin_y"]) - (np.arange(r0, r1) + 0.5) * float(d["pixel"])
+        for k in entries[int(bin_["first"]):int(bin_["first"]) + int(bin_["count"])]:
+            dx = xs[None, :] - foot[k, 0]
+            dy = ys[:, None] - foot[k, 1]
+            n_in = int((dx * dx + dy * dy < foot[k, 2] ** 2).sum())
+            want_listed += (c1 - c0) * (r1 - r0)
+            want_inside += n_in
+            want_ops += n_in * cost[k]
+    assert listed == want_listed and inside == want_inside
+    assert ops == pytest.approx(want_ops, rel=1e-12)
+    assert 0 < inside < listed
