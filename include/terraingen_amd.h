@@ -141,6 +141,21 @@ int tgx_batch_bin(const double* cx, const double* cy, const double* reach, const
                   tg_tile_desc** descs, tg_bin** bins, size_t* bin_count, uint32_t** entries,
                   size_t* entry_count, char* err, size_t err_len);
 
+/* A catalogue's spatial index for binning tiles, built once and kept (craters by diameter level
+ * and, within a level, by square cells): what tgx_batch_bin builds and discards on every call. */
+typedef struct tgx_index tgx_index;
+
+int tgx_index_create(const double* cx, const double* cy, const double* reach,
+                     const double* diameter, size_t n, tgx_index** out, char* err,
+                     size_t err_len);
+void tgx_index_free(tgx_index* index);
+size_t tgx_index_count(const tgx_index* index);
+
+/* tgx_batch_bin against a kept index. Thread-safe: the index is only read. */
+int tgx_index_bin(const tgx_index* index, const tgx_tile* tiles, size_t tile_count,
+                  unsigned threads, tg_tile_desc** descs, tg_bin** bins, size_t* bin_count,
+                  uint32_t** entries, size_t* entry_count, char* err, size_t err_len);
+
 /* Counts the (cell, crater) pairs of a binned batch (the output of tgx_batch_bin) whose FP64 cell
  * centre lies strictly inside the crater's footprint half-width `reach`: the pairs a crater kernel
  * evaluates in full. `listed` receives every (cell, listed crater) pair, `inside` those within
@@ -151,6 +166,68 @@ int tgx_batch_pair_ops(const tg_tile_desc* descs, size_t tile_count, const tg_bi
                        const double* cx, const double* cy, const double* reach,
                        const double* cost, size_t n, unsigned threads, uint64_t* listed,
                        uint64_t* inside, double* ops, char* err, size_t err_len);
+
+/* ---- The tile server: continuous batching onto the batched kernel ---------------------------
+ * No counterpart in the reference, which evaluates one grid per call. Requests (a tile and the host
+ * memory its heights go to) queue as they arrive; a dispatcher thread admits them, in arrival order,
+ * into whichever of `slots` batches is free, as many as fit `max_batch_tiles`, `max_batch_cells`
+ * and the HBM output pool of `pool_bytes` (the admission budget), bins them against the kept index,
+ * launches one persistent kernel per batch on the slot's stream and copies the heights back
+ * through pinned memory. Batches in different slots overlap on the device. The device handles
+ * (tg_dev, tg_noise_plan, tg_catalogue) come from libtg_hip.so, which this library opens on first
+ * use. */
+typedef struct tgx_server tgx_server;
+
+typedef struct tgx_server_config {
+    uint32_t slots;             /* batches in flight, 1-16 */
+    uint32_t max_batch_tiles;   /* tiles per batch */
+    uint64_t max_batch_cells;   /* cells per batch; also the pinned staging per slot (x4 bytes) */
+    uint64_t pool_bytes;        /* HBM for outputs of admitted batches */
+    uint32_t bin_threads;       /* host threads binning one batch (0: all) */
+    uint32_t poll_microseconds; /* how often the dispatcher looks at running batches (0: 20) */
+} tgx_server_config;
+
+typedef struct tgx_ticket_info {
+    double submitted; /* seconds on the steady clock */
+    double admitted;  /* into a batch */
+    double launched;  /* the batch's kernel queued */
+    double done;      /* heights in the request's memory */
+    uint32_t batch;   /* the batch it ran in, from 1 */
+    uint32_t batch_tiles;
+} tgx_ticket_info;
+
+typedef struct tgx_server_stats {
+    uint64_t batches;
+    uint64_t tiles;
+    uint64_t cells;
+    uint64_t failed;
+    uint32_t max_in_flight; /* most batches on the device at once */
+    uint32_t pad;
+    uint64_t pool_bytes;
+    uint64_t pool_peak_bytes;
+    uint64_t pending;
+} tgx_server_stats;
+
+/* `index` may be null (no craters); otherwise `catalogue` holds the same craters in the same
+ * order. The plan, catalogue and index must outlive the server. */
+int tgx_server_create(tg_dev* dev, const tg_noise_plan* plan, float elevation,
+                      const tg_catalogue* catalogue, const tgx_index* index,
+                      const tgx_server_config* config, tgx_server** out, char* err,
+                      size_t err_len);
+/* Thread-safe. `dst` receives width x height floats, row-major from the north-west cell, and must
+ * stay valid until the ticket is waited for. */
+int tgx_server_submit(tgx_server* server, const tgx_tile* tile, float* dst, uint64_t* ticket,
+                      char* err, size_t err_len);
+/* Blocks until the ticket's heights are in place (timeout < 0: no limit; TG_NOT_READY when the
+ * timeout passes first), then releases the ticket. Thread-safe. */
+int tgx_server_wait(tgx_server* server, uint64_t ticket, double timeout_seconds,
+                    tgx_ticket_info* info, char* err, size_t err_len);
+int tgx_server_stats_get(tgx_server* server, tgx_server_stats* out);
+/* Finishes every submitted request, then frees the server. No thread may be waiting on it. */
+int tgx_server_free(tgx_server* server);
+
+/* Empty when libtg_hip.so opens, else the loader's reason. */
+const char* tgx_hip_unavailable_reason(void);
 
 #ifdef __cplusplus
 }

@@ -27,7 +27,8 @@ enum {
     TG_OK = 0,
     TG_ERR_ARGUMENT = 1, /* a null pointer, a bad size or an unknown kernel id */
     TG_ERR_DEVICE = 2,   /* the HIP runtime reported an error; the message is the runtime's */
-    TG_ERR_NO_DEVICE = 3 /* no GPU at the ordinal */
+    TG_ERR_NO_DEVICE = 3, /* no GPU at the ordinal */
+    TG_NOT_READY = 4      /* tg_hip_event_query: the work before the event is still running */
 };
 
 /* The kernels of the reference the backend runs; one per kernel struct, as the reference keeps one
@@ -215,6 +216,17 @@ int tg_hip_event_record(tg_dev* dev, void* event, void* stream, char* err, size_
 /* Milliseconds between two recorded events, waiting for the second. */
 int tg_hip_event_elapsed(tg_dev* dev, void* start, void* stop, float* ms, char* err,
                          size_t err_len);
+/* Pinned (page-locked) host memory, for copies that overlap device work. */
+int tg_hip_host_alloc(tg_dev* dev, size_t bytes, void** out, char* err, size_t err_len);
+int tg_hip_host_free(tg_dev* dev, void* ptr);
+/* Asynchronous copies on `stream`: the host memory must be pinned (tg_hip_host_alloc) for the copy
+ * to overlap other work, and must stay valid until the stream has passed the copy. */
+int tg_hip_memcpy_d2h_async(tg_dev* dev, void* dst, const void* src, size_t bytes, void* stream,
+                            char* err, size_t err_len);
+int tg_hip_memcpy_h2d_async(tg_dev* dev, void* dst, const void* src, size_t bytes, void* stream,
+                            char* err, size_t err_len);
+/* TG_OK once the work recorded before `event` has finished, TG_NOT_READY while it runs. */
+int tg_hip_event_query(tg_dev* dev, void* event, char* err, size_t err_len);
 
 #ifdef __cplusplus
 }

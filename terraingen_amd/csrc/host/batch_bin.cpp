@@ -11,8 +11,8 @@
 // visits the cells its box (grown by the level's largest footprint) meets, and skips the levels
 // whose craters its pixel size cannot resolve.
 
+#include "batch.h"
 #include "common.h"
-#include "tg_hip.h"
 
 #include <atomic>
 #include <mutex>
@@ -21,14 +21,24 @@
 
 namespace tgh {
 
-namespace {
-
-struct Level {
+struct IndexLevel {
     double cell = 0.0;       // index cell side (m)
     double maxReach = 0.0;   // largest footprint half-width in the level (m)
     double maxDiameter = 0.0;
     std::unordered_map<std::uint64_t, std::vector<std::uint32_t>> cells;
 };
+
+}  // namespace tgh
+
+// The catalogue's spatial index: craters grouped by diameter level and, within a level, by cell.
+struct tgx_index {
+    std::vector<double> cx, cy, reach, diameter;
+    std::vector<tgh::IndexLevel> levels;
+};
+
+namespace tgh {
+
+namespace {
 
 std::uint64_t CellKey(std::int64_t i, std::int64_t j) {
     return (std::uint64_t(std::uint32_t(std::int32_t(i))) << 32) |
@@ -55,13 +65,13 @@ struct TileBins {
 
 }  // namespace
 
-void BinBatch(const double* cx, const double* cy, const double* reach, const double* diameter,
-              std::size_t n, const tgx_tile* tiles, std::size_t tileCount, unsigned threads,
-              std::vector<tg_tile_desc>& descs, std::vector<tg_bin>& bins,
-              std::vector<std::uint32_t>& entries) {
-    constexpr std::uint32_t kSide = 16;
-    // The index.
-    std::vector<Level> levels;
+IndexPtr MakeIndex(const double* cx, const double* cy, const double* reach,
+                                 const double* diameter, std::size_t n) {
+    IndexPtr index(new tgx_index());
+    index->cx.assign(cx, cx + n);
+    index->cy.assign(cy, cy + n);
+    index->reach.assign(reach, reach + n);
+    index->diameter.assign(diameter, diameter + n);
     int base = 0;
     if (n > 0) {
         double smallest = INFINITY;
@@ -70,6 +80,7 @@ void BinBatch(const double* cx, const double* cy, const double* reach, const dou
         }
         base = std::ilogb(smallest);
     }
+    std::vector<IndexLevel>& levels = index->levels;
     for (std::size_t k = 0; k < n; ++k) {
         if (!(diameter[k] > 0.0) || !std::isfinite(cx[k]) || !std::isfinite(cy[k]) ||
             !(reach[k] >= 0.0)) {
@@ -79,7 +90,7 @@ void BinBatch(const double* cx, const double* cy, const double* reach, const dou
         if (l >= levels.size()) {
             levels.resize(l + 1);
         }
-        Level& level = levels[l];
+        IndexLevel& level = levels[l];
         if (level.cell == 0.0) {
             level.cell = 8.0 * std::ldexp(1.0, base + int(l) + 1);
         }
@@ -89,86 +100,105 @@ void BinBatch(const double* cx, const double* cy, const double* reach, const dou
         const auto j = std::int64_t(std::floor(cy[k] / level.cell));
         level.cells[CellKey(i, j)].push_back(std::uint32_t(k));
     }
+    return index;
+}
+
+namespace {
+
+// One tile's bins against the index: per-bin counts and the entries in bin order.
+void BinTile(const tgx_index& index, const tgx_tile& tile, std::vector<std::uint32_t>& candidates,
+             TileBins& out) {
+    constexpr std::uint32_t kSide = 16;
+    const Grid g{tile.width, tile.height, tile.origin_x, tile.origin_y, tile.pixel};
+    const double minX = tile.origin_x, maxX = tile.origin_x + tile.width * tile.pixel;
+    const double maxY = tile.origin_y, minY = tile.origin_y - tile.height * tile.pixel;
+    candidates.clear();
+    for (const IndexLevel& level : index.levels) {
+        if (level.cell == 0.0 || !Resolved(level.maxDiameter, tile.pixel)) {
+            continue;
+        }
+        const double grow = level.maxReach;
+        const auto i0 = std::int64_t(std::floor((minX - grow) / level.cell));
+        const auto i1 = std::int64_t(std::floor((maxX + grow) / level.cell));
+        const auto j0 = std::int64_t(std::floor((minY - grow) / level.cell));
+        const auto j1 = std::int64_t(std::floor((maxY + grow) / level.cell));
+        if ((i1 - i0 + 1) * (j1 - j0 + 1) > std::int64_t(level.cells.size())) {
+            for (const auto& [key, list] : level.cells) {
+                candidates.insert(candidates.end(), list.begin(), list.end());
+            }
+            continue;
+        }
+        for (std::int64_t j = j0; j <= j1; ++j) {
+            for (std::int64_t i = i0; i <= i1; ++i) {
+                const auto it = level.cells.find(CellKey(i, j));
+                if (it != level.cells.end()) {
+                    candidates.insert(candidates.end(), it->second.begin(), it->second.end());
+                }
+            }
+        }
+    }
+    std::sort(candidates.begin(), candidates.end());
+    const std::size_t binsX = (tile.width + kSide - 1) / kSide;
+    const std::size_t binsY = (tile.height + kSide - 1) / kSide;
+    std::vector<std::vector<std::uint32_t>> lists(binsX * binsY);
+    for (const std::uint32_t k : candidates) {
+        if (!Resolved(index.diameter[k], tile.pixel)) {
+            continue;
+        }
+        const double r = index.reach[k];
+        const PixelWindow w = PixelWindowIn(g, index.cx[k] - r, index.cy[k] - r,
+                                            index.cx[k] + r, index.cy[k] + r);
+        if (w.Empty()) {
+            continue;
+        }
+        for (std::size_t by = w.rowBegin / kSide; by <= (w.rowEnd - 1) / kSide; ++by) {
+            for (std::size_t bx = w.colBegin / kSide; bx <= (w.colEnd - 1) / kSide; ++bx) {
+                lists[by * binsX + bx].push_back(k);
+            }
+        }
+    }
+    out.counts.resize(lists.size());
+    out.entries.clear();
+    for (std::size_t b = 0; b < lists.size(); ++b) {
+        out.counts[b] = std::uint32_t(lists[b].size());
+        out.entries.insert(out.entries.end(), lists[b].begin(), lists[b].end());
+    }
+}
+
+}  // namespace
+
+void CheckTile(const tgx_tile& tile, std::size_t t) {
+    if (tile.width == 0 || tile.height == 0 || !(tile.pixel > 0.0) ||
+        !std::isfinite(tile.origin_x) || !std::isfinite(tile.origin_y) ||
+        !std::isfinite(tile.pixel)) {
+        throw std::invalid_argument("tile " + std::to_string(t) + " is malformed");
+    }
+}
+
+void BinBatch(const tgx_index& index, const tgx_tile* tiles, std::size_t tileCount, unsigned threads,
+              std::vector<tg_tile_desc>& descs, std::vector<tg_bin>& bins,
+              std::vector<std::uint32_t>& entries) {
+    constexpr std::uint32_t kSide = 16;
+    for (std::size_t t = 0; t < tileCount; ++t) {
+        CheckTile(tiles[t], t);
+    }
     // Each tile's bins, in parallel.
     std::vector<TileBins> per(tileCount);
     std::atomic<std::size_t> next{0};
-    std::atomic<bool> failed{false};
-    std::mutex failureMutex;
-    std::string failure;
     auto work = [&] {
         std::vector<std::uint32_t> candidates;
         for (;;) {
             const std::size_t t = next.fetch_add(1);
-            if (t >= tileCount || failed.load()) {
+            if (t >= tileCount) {
                 return;
             }
-            const tgx_tile& tile = tiles[t];
-            if (tile.width == 0 || tile.height == 0 || !(tile.pixel > 0.0) ||
-                !std::isfinite(tile.origin_x) || !std::isfinite(tile.origin_y)) {
-                std::lock_guard<std::mutex> lock(failureMutex);
-                failed = true;
-                failure = "tile " + std::to_string(t) + " is malformed";
-                return;
-            }
-            const Grid g{tile.width, tile.height, tile.origin_x, tile.origin_y, tile.pixel};
-            const double minX = tile.origin_x, maxX = tile.origin_x + tile.width * tile.pixel;
-            const double maxY = tile.origin_y, minY = tile.origin_y - tile.height * tile.pixel;
-            candidates.clear();
-            for (const Level& level : levels) {
-                if (level.cell == 0.0 || !Resolved(level.maxDiameter, tile.pixel)) {
-                    continue;
-                }
-                const double grow = level.maxReach;
-                const auto i0 = std::int64_t(std::floor((minX - grow) / level.cell));
-                const auto i1 = std::int64_t(std::floor((maxX + grow) / level.cell));
-                const auto j0 = std::int64_t(std::floor((minY - grow) / level.cell));
-                const auto j1 = std::int64_t(std::floor((maxY + grow) / level.cell));
-                if ((i1 - i0 + 1) * (j1 - j0 + 1) > std::int64_t(level.cells.size())) {
-                    for (const auto& [key, list] : level.cells) {
-                        candidates.insert(candidates.end(), list.begin(), list.end());
-                    }
-                    continue;
-                }
-                for (std::int64_t j = j0; j <= j1; ++j) {
-                    for (std::int64_t i = i0; i <= i1; ++i) {
-                        const auto it = level.cells.find(CellKey(i, j));
-                        if (it != level.cells.end()) {
-                            candidates.insert(candidates.end(), it->second.begin(),
-                                              it->second.end());
-                        }
-                    }
-                }
-            }
-            std::sort(candidates.begin(), candidates.end());
-            const std::size_t binsX = (tile.width + kSide - 1) / kSide;
-            const std::size_t binsY = (tile.height + kSide - 1) / kSide;
-            std::vector<std::vector<std::uint32_t>> lists(binsX * binsY);
-            for (const std::uint32_t k : candidates) {
-                if (!Resolved(diameter[k], tile.pixel)) {
-                    continue;
-                }
-                const PixelWindow w = PixelWindowIn(g, cx[k] - reach[k], cy[k] - reach[k],
-                                                    cx[k] + reach[k], cy[k] + reach[k]);
-                if (w.Empty()) {
-                    continue;
-                }
-                for (std::size_t by = w.rowBegin / kSide; by <= (w.rowEnd - 1) / kSide; ++by) {
-                    for (std::size_t bx = w.colBegin / kSide; bx <= (w.colEnd - 1) / kSide; ++bx) {
-                        lists[by * binsX + bx].push_back(k);
-                    }
-                }
-            }
-            TileBins& out = per[t];
-            out.counts.resize(lists.size());
-            for (std::size_t b = 0; b < lists.size(); ++b) {
-                out.counts[b] = std::uint32_t(lists[b].size());
-                out.entries.insert(out.entries.end(), lists[b].begin(), lists[b].end());
-            }
+            BinTile(index, tiles[t], candidates, per[t]);
         }
     };
     if (threads == 0) {
         threads = std::max(1u, std::thread::hardware_concurrency());
     }
+    threads = unsigned(std::min<std::size_t>(threads, std::max<std::size_t>(tileCount, 1)));
     std::vector<std::thread> pool;
     for (unsigned i = 1; i < threads; ++i) {
         pool.emplace_back(work);
@@ -176,9 +206,6 @@ void BinBatch(const double* cx, const double* cy, const double* reach, const dou
     work();
     for (std::thread& th : pool) {
         th.join();
-    }
-    if (failed) {
-        throw std::invalid_argument(failure);
     }
     // Concatenated in tile order.
     descs.clear();
@@ -319,10 +346,52 @@ int tgx_batch_bin(const double* cx, const double* cy, const double* reach,
             (n > 0 && (cx == nullptr || cy == nullptr || reach == nullptr || diameter == nullptr))) {
             throw std::invalid_argument("tgx_batch_bin: null argument");
         }
+        const tgh::IndexPtr index = tgh::MakeIndex(cx, cy, reach, diameter, n);
         std::vector<tg_tile_desc> d;
         std::vector<tg_bin> b;
         std::vector<std::uint32_t> e;
-        tgh::BinBatch(cx, cy, reach, diameter, n, tiles, tileCount, threads, d, b, e);
+        tgh::BinBatch(*index, tiles, tileCount, threads, d, b, e);
+        *descs = tgh::Export(d);
+        *bins = tgh::Export(b);
+        *binCount = b.size();
+        *entries = tgh::Export(e);
+        *entryCount = e.size();
+    });
+}
+
+int tgx_index_create(const double* cx, const double* cy, const double* reach,
+                     const double* diameter, size_t n, tgx_index** out, char* err,
+                     size_t errLen) {
+    return tgh::Guard(err, errLen, [&] {
+        if (out == nullptr ||
+            (n > 0 && (cx == nullptr || cy == nullptr || reach == nullptr || diameter == nullptr))) {
+            throw std::invalid_argument("tgx_index_create: null argument");
+        }
+        *out = nullptr;
+        *out = tgh::MakeIndex(cx, cy, reach, diameter, n).release();
+    });
+}
+
+void tgx_index_free(tgx_index* index) {
+    delete index;
+}
+
+size_t tgx_index_count(const tgx_index* index) {
+    return index != nullptr ? index->cx.size() : 0;
+}
+
+int tgx_index_bin(const tgx_index* index, const tgx_tile* tiles, size_t tileCount,
+                  unsigned threads, tg_tile_desc** descs, tg_bin** bins, size_t* binCount,
+                  uint32_t** entries, size_t* entryCount, char* err, size_t errLen) {
+    return tgh::Guard(err, errLen, [&] {
+        if (index == nullptr || descs == nullptr || bins == nullptr || entries == nullptr ||
+            binCount == nullptr || entryCount == nullptr || (tileCount > 0 && tiles == nullptr)) {
+            throw std::invalid_argument("tgx_index_bin: null argument");
+        }
+        std::vector<tg_tile_desc> d;
+        std::vector<tg_bin> b;
+        std::vector<std::uint32_t> e;
+        tgh::BinBatch(*index, tiles, tileCount, threads, d, b, e);
         *descs = tgh::Export(d);
         *bins = tgh::Export(b);
         *binCount = b.size();

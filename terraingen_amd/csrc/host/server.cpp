@@ -1,0 +1,556 @@
+// SPDX-License-Identifier: BSD-3-Clause
+//
+// The tile server: continuous batching of tile requests onto the batched kernel.
+//
+// Requests (a tile and the host memory its heights go to) queue as they arrive. A dispatcher
+// thread owns a few slots, each a staged batch with its own stream, event and pinned staging
+// memory. Whenever a slot is free it admits the waiting requests in arrival order, as many as fit
+// the batch limits and the HBM output pool (the admission budget), bins them against the kept
+// catalogue index, stages the bins and launches one persistent kernel for the whole batch, then
+// copies the heights back asynchronously on the same stream. Batches in different slots overlap on
+// the device, and requests that arrive while others run are admitted as slots and pool memory
+// free up. A finished slot's heights are copied into each request's memory and its pool range is
+// returned. The crater catalogue, its index and the noise constants stay resident throughout.
+//
+// Each tile's heights depend only on the tile and the resident inputs (its bins are its own and
+// every cell is independent), so a tile gives the same heights alone and in any mix.
+
+#include "batch.h"
+#include "common.h"
+#include "hip_api.h"
+
+#include <chrono>
+#include <condition_variable>
+#include <deque>
+#include <map>
+#include <memory>
+#include <mutex>
+#include <thread>
+#include <unordered_map>
+
+namespace tgh {
+
+namespace {
+
+using Clock = std::chrono::steady_clock;
+
+double Now() {
+    return std::chrono::duration<double>(Clock::now().time_since_epoch()).count();
+}
+
+constexpr std::uint64_t kAlign = 256;
+
+std::uint64_t AlignUp(std::uint64_t bytes) {
+    return (bytes + kAlign - 1) / kAlign * kAlign;
+}
+
+// First-fit ranges of one device allocation.
+class RangePool {
+public:
+    explicit RangePool(std::uint64_t size) : size_(size) {
+        if (size > 0) {
+            free_[0] = size;
+        }
+    }
+
+    bool Take(std::uint64_t bytes, std::uint64_t& offset) {
+        for (auto it = free_.begin(); it != free_.end(); ++it) {
+            if (it->second >= bytes) {
+                offset = it->first;
+                const std::uint64_t rest = it->second - bytes;
+                free_.erase(it);
+                if (rest > 0) {
+                    free_[offset + bytes] = rest;
+                }
+                inUse_ += bytes;
+                peak_ = std::max(peak_, inUse_);
+                return true;
+            }
+        }
+        return false;
+    }
+
+    void Give(std::uint64_t offset, std::uint64_t bytes) {
+        inUse_ -= bytes;
+        auto it = free_.emplace(offset, bytes).first;
+        auto next = std::next(it);
+        if (next != free_.end() && it->first + it->second == next->first) {
+            it->second += next->second;
+            free_.erase(next);
+        }
+        if (it != free_.begin()) {
+            auto prev = std::prev(it);
+            if (prev->first + prev->second == it->first) {
+                prev->second += it->second;
+                free_.erase(it);
+            }
+        }
+    }
+
+    std::uint64_t Peak() const {
+        return peak_;
+    }
+    std::uint64_t Size() const {
+        return size_;
+    }
+
+private:
+    std::uint64_t size_ = 0;
+    std::map<std::uint64_t, std::uint64_t> free_;
+    std::uint64_t inUse_ = 0;
+    std::uint64_t peak_ = 0;
+};
+
+struct Request {
+    std::uint64_t ticket = 0;
+    tgx_tile tile{};
+    float* dst = nullptr;
+    tgx_ticket_info info{};
+    int status = TG_NOT_READY;
+    std::string error;
+};
+
+struct Slot {
+    tg_batch* batch = nullptr;
+    void* stream = nullptr;
+    void* event = nullptr;
+    float* staging = nullptr;  // pinned, max_batch_cells floats
+    std::size_t maxTiles = 0, maxBins = 0, maxEntries = 0;
+    bool busy = false;
+    std::vector<Request*> requests;  // in batch order
+    std::vector<std::uint64_t> offsets;
+    std::uint64_t poolOffset = 0, poolBytes = 0;
+};
+
+void Raise(int code, const char* err) {
+    if (code != TG_OK) {
+        throw std::runtime_error(err[0] != 0 ? err : "kernel library call failed");
+    }
+}
+
+}  // namespace
+
+}  // namespace tgh
+
+struct tgx_server {
+    const tgh::HipApi* hip = nullptr;
+    tg_dev* dev = nullptr;
+    const tg_noise_plan* plan = nullptr;
+    float elevation = 0.0f;
+    const tg_catalogue* catalogue = nullptr;
+    const tgx_index* index = nullptr;
+    tgh::IndexPtr emptyIndex;
+    tgx_server_config config{};
+    float* pool = nullptr;
+    std::unique_ptr<tgh::RangePool> ranges;
+    std::vector<tgh::Slot> slots;
+
+    std::mutex mutex;
+    std::condition_variable wake;  // the dispatcher: new requests or shutdown
+    std::condition_variable done;  // waiters: a request finished
+    std::deque<tgh::Request*> pending;
+    std::unordered_map<std::uint64_t, std::unique_ptr<tgh::Request>> requests;
+    std::uint64_t nextTicket = 1;
+    std::uint32_t nextBatch = 1;
+    bool stop = false;
+    tgx_server_stats stats{};
+    std::thread dispatcher;
+};
+
+namespace tgh {
+
+namespace {
+
+void FreeSlot(tgx_server& s, Slot& slot) {
+    const HipApi& h = *s.hip;
+    if (slot.batch != nullptr) {
+        h.batch_free(slot.batch);
+        slot.batch = nullptr;
+    }
+    if (slot.event != nullptr) {
+        h.event_destroy(s.dev, slot.event);
+        slot.event = nullptr;
+    }
+    if (slot.staging != nullptr) {
+        h.host_free(s.dev, slot.staging);
+        slot.staging = nullptr;
+    }
+}
+
+// (Re)creates the slot's batch with at least the given capacities; the slot is idle.
+void EnsureCapacity(tgx_server& s, Slot& slot, std::size_t tiles, std::size_t bins,
+                    std::size_t entries) {
+    if (slot.batch != nullptr && tiles <= slot.maxTiles && bins <= slot.maxBins &&
+        entries <= slot.maxEntries) {
+        return;
+    }
+    const HipApi& h = *s.hip;
+    char err[512] = {0};
+    if (slot.batch != nullptr) {
+        h.batch_free(slot.batch);
+        slot.batch = nullptr;
+        slot.stream = nullptr;
+    }
+    const std::size_t t = std::max(tiles, slot.maxTiles);
+    const std::size_t b = std::max(bins + bins / 2, slot.maxBins);
+    const std::size_t e = std::max(entries + entries / 2 + 1024, slot.maxEntries);
+    Raise(h.batch_create(s.dev, t, b, e, &slot.batch, err, sizeof err), err);
+    slot.maxTiles = t;
+    slot.maxBins = b;
+    slot.maxEntries = e;
+    slot.stream = h.batch_stream(slot.batch);
+}
+
+// Bins, stages and launches one batch in an idle slot, then queues the copy back and the event.
+void LaunchBatch(tgx_server& s, Slot& slot) {
+    const HipApi& h = *s.hip;
+    char err[512] = {0};
+    std::vector<tgx_tile> tiles(slot.requests.size());
+    for (std::size_t i = 0; i < tiles.size(); ++i) {
+        tiles[i] = slot.requests[i]->tile;
+    }
+    std::vector<tg_tile_desc> descs;
+    std::vector<tg_bin> bins;
+    std::vector<std::uint32_t> entries;
+    BinBatch(*s.index, tiles.data(), tiles.size(), s.config.bin_threads, descs, bins, entries);
+    EnsureCapacity(s, slot, descs.size(), bins.size(), entries.size());
+    Raise(h.batch_stage(slot.batch, descs.data(), descs.size(), bins.data(), bins.size(),
+                        entries.data(), entries.size(), err, sizeof err),
+          err);
+    std::uint64_t cells = 0;
+    slot.offsets.resize(descs.size());
+    for (std::size_t i = 0; i < descs.size(); ++i) {
+        slot.offsets[i] = descs[i].out_offset;
+        cells += std::uint64_t(descs[i].width) * descs[i].height;
+    }
+    float* region = s.pool + slot.poolOffset / sizeof(float);
+    const double launched = Now();
+    for (Request* r : slot.requests) {
+        r->info.launched = launched;
+    }
+    Raise(h.batch_run(slot.batch, s.plan, s.elevation, s.catalogue, region, cells, 0, err,
+                      sizeof err),
+          err);
+    Raise(h.memcpy_d2h_async(s.dev, slot.staging, region, cells * sizeof(float), slot.stream, err,
+                             sizeof err),
+          err);
+    Raise(h.event_record(s.dev, slot.event, slot.stream, err, sizeof err), err);
+}
+
+void FinishSlot(tgx_server& s, Slot& slot, int status, const std::string& error) {
+    const double now = Now();
+    for (Request* r : slot.requests) {
+        r->status = status;
+        r->error = error;
+        r->info.done = now;
+    }
+    if (status == TG_OK) {
+        s.stats.tiles += slot.requests.size();
+    } else {
+        s.stats.failed += slot.requests.size();
+    }
+    slot.requests.clear();
+    s.ranges->Give(slot.poolOffset, slot.poolBytes);
+    slot.busy = false;
+}
+
+// Takes waiting requests, in arrival order, for an idle slot: as many as fit the batch limits and
+// the pool. False when none can be admitted now.
+bool Admit(tgx_server& s, Slot& slot) {
+    std::vector<Request*> take;
+    std::uint64_t cells = 0;
+    while (!s.pending.empty() && take.size() < s.config.max_batch_tiles) {
+        Request* r = s.pending.front();
+        const std::uint64_t c = std::uint64_t(r->tile.width) * r->tile.height;
+        if (cells + c > s.config.max_batch_cells) {
+            break;
+        }
+        take.push_back(r);
+        cells += c;
+        s.pending.pop_front();
+    }
+    std::uint64_t offset = 0;
+    while (!take.empty() && !s.ranges->Take(AlignUp(cells * sizeof(float)), offset)) {
+        Request* r = take.back();  // over the budget: the newest waits for memory
+        take.pop_back();
+        cells -= std::uint64_t(r->tile.width) * r->tile.height;
+        s.pending.push_front(r);
+    }
+    if (take.empty()) {
+        return false;
+    }
+    const double now = Now();
+    const std::uint32_t id = s.nextBatch++;
+    for (Request* r : take) {
+        r->info.admitted = now;
+        r->info.batch = id;
+        r->info.batch_tiles = std::uint32_t(take.size());
+    }
+    slot.requests = std::move(take);
+    slot.poolOffset = offset;
+    slot.poolBytes = AlignUp(cells * sizeof(float));
+    slot.busy = true;
+    s.stats.batches += 1;
+    s.stats.cells += cells;
+    return true;
+}
+
+void Dispatch(tgx_server& s) {
+    const HipApi& h = *s.hip;
+    std::unique_lock<std::mutex> lock(s.mutex);
+    for (;;) {
+        bool progressed = false;
+        // Retire finished batches.
+        for (Slot& slot : s.slots) {
+            if (!slot.busy) {
+                continue;
+            }
+            char err[512] = {0};
+            const int q = h.event_query(s.dev, slot.event, err, sizeof err);
+            if (q == TG_NOT_READY) {
+                continue;
+            }
+            if (q == TG_OK) {
+                lock.unlock();  // the slot and its requests' memory belong to the dispatcher
+                for (std::size_t i = 0; i < slot.requests.size(); ++i) {
+                    const Request* r = slot.requests[i];
+                    std::memcpy(r->dst, slot.staging + slot.offsets[i],
+                                std::size_t(r->tile.width) * r->tile.height * sizeof(float));
+                }
+                lock.lock();
+                FinishSlot(s, slot, TG_OK, "");
+            } else {
+                FinishSlot(s, slot, TG_ERR_DEVICE, err);
+            }
+            progressed = true;
+            s.done.notify_all();
+        }
+        std::uint32_t inFlight = 0;
+        for (const Slot& slot : s.slots) {
+            inFlight += slot.busy ? 1u : 0u;
+        }
+        if (s.stop && s.pending.empty() && inFlight == 0) {
+            return;
+        }
+        // Admit waiting requests into free slots.
+        for (Slot& slot : s.slots) {
+            if (slot.busy || s.pending.empty()) {
+                continue;
+            }
+            if (!Admit(s, slot)) {
+                break;
+            }
+            inFlight += 1;
+            s.stats.max_in_flight = std::max(s.stats.max_in_flight, inFlight);
+            lock.unlock();
+            std::string failure;
+            try {
+                LaunchBatch(s, slot);
+            } catch (const std::exception& e) {
+                failure = e.what();
+            }
+            lock.lock();
+            if (!failure.empty()) {
+                // Nothing of the batch may still be queued: wait for the slot's stream first.
+                char err[512] = {0};
+                if (slot.stream != nullptr) {
+                    (void)h.stream_synchronize(s.dev, slot.stream, err, sizeof err);
+                }
+                FinishSlot(s, slot, TG_ERR_DEVICE, failure);
+                s.done.notify_all();
+            }
+            progressed = true;
+        }
+        if (progressed) {
+            continue;
+        }
+        if (inFlight > 0) {
+            s.wake.wait_for(lock, std::chrono::microseconds(s.config.poll_microseconds));
+        } else {
+            s.wake.wait(lock, [&] { return s.stop || !s.pending.empty(); });
+        }
+    }
+}
+
+void Fill(const Request& r, tgx_ticket_info* info) {
+    if (info != nullptr) {
+        *info = r.info;
+    }
+}
+
+}  // namespace
+
+}  // namespace tgh
+
+extern "C" {
+
+int tgx_server_create(tg_dev* dev, const tg_noise_plan* plan, float elevation,
+                      const tg_catalogue* catalogue, const tgx_index* index,
+                      const tgx_server_config* config, tgx_server** out, char* err,
+                      size_t errLen) {
+    if (out != nullptr) {
+        *out = nullptr;
+    }
+    auto s = std::make_unique<tgx_server>();
+    const int code = tgh::Guard(err, errLen, [&] {
+        if (dev == nullptr || config == nullptr || out == nullptr) {
+            throw std::invalid_argument("tgx_server_create: null argument");
+        }
+        const tgx_server_config& c = *config;
+        // A full batch must always fit the empty pool, so admission can always make progress.
+        if (c.slots == 0 || c.slots > 16 || c.max_batch_tiles == 0 || c.max_batch_cells == 0 ||
+            c.max_batch_cells > (std::uint64_t(1) << 40) ||
+            tgh::AlignUp(c.max_batch_cells * sizeof(float)) >
+                c.pool_bytes / tgh::kAlign * tgh::kAlign) {
+            throw std::invalid_argument(
+                "tgx_server_create: need 1-16 slots, batch limits above zero and a pool that "
+                "holds one full batch");
+        }
+        if (index != nullptr && catalogue == nullptr && tgx_index_count(index) > 0) {
+            throw std::invalid_argument("tgx_server_create: an index needs its catalogue");
+        }
+        s->hip = &tgh::Hip();
+        s->dev = dev;
+        s->plan = plan;
+        s->elevation = elevation;
+        s->catalogue = catalogue;
+        s->config = c;
+        if (s->config.poll_microseconds == 0) {
+            s->config.poll_microseconds = 20;
+        }
+        if (index == nullptr) {
+            s->emptyIndex = tgh::MakeIndex(nullptr, nullptr, nullptr, nullptr, 0);
+            index = s->emptyIndex.get();
+        }
+        s->index = index;
+        char e[512] = {0};
+        void* pool = nullptr;
+        tgh::Raise(s->hip->malloc(dev, c.pool_bytes, &pool, e, sizeof e), e);
+        s->pool = static_cast<float*>(pool);
+        s->ranges = std::make_unique<tgh::RangePool>(c.pool_bytes / tgh::kAlign * tgh::kAlign);
+        s->stats.pool_bytes = c.pool_bytes;
+        s->slots.resize(c.slots);
+        for (tgh::Slot& slot : s->slots) {
+            tgh::Raise(s->hip->event_create(dev, &slot.event, e, sizeof e), e);
+            void* staging = nullptr;
+            tgh::Raise(s->hip->host_alloc(dev, c.max_batch_cells * sizeof(float), &staging, e,
+                                          sizeof e),
+                       e);
+            slot.staging = static_cast<float*>(staging);
+            tgh::EnsureCapacity(*s, slot, std::min<std::size_t>(c.max_batch_tiles, 64), 4096,
+                                65536);
+        }
+        s->dispatcher = std::thread([srv = s.get()] { tgh::Dispatch(*srv); });
+    });
+    if (code != TG_OK) {
+        if (s->hip != nullptr) {
+            for (tgh::Slot& slot : s->slots) {
+                tgh::FreeSlot(*s, slot);
+            }
+            if (s->pool != nullptr) {
+                s->hip->free(dev, s->pool);
+            }
+        }
+        return code;
+    }
+    *out = s.release();
+    return TG_OK;
+}
+
+int tgx_server_submit(tgx_server* s, const tgx_tile* tile, float* dst, uint64_t* ticket,
+                      char* err, size_t errLen) {
+    return tgh::Guard(err, errLen, [&] {
+        if (s == nullptr || tile == nullptr || dst == nullptr || ticket == nullptr) {
+            throw std::invalid_argument("tgx_server_submit: null argument");
+        }
+        tgh::CheckTile(*tile, 0);
+        const std::uint64_t cells = std::uint64_t(tile->width) * tile->height;
+        if (cells > s->config.max_batch_cells) {
+            throw std::invalid_argument("tgx_server_submit: the tile has more cells than a batch "
+                                        "holds");
+        }
+        auto r = std::make_unique<tgh::Request>();
+        r->tile = *tile;
+        r->dst = dst;
+        r->info.submitted = tgh::Now();
+        std::lock_guard<std::mutex> lock(s->mutex);
+        if (s->stop) {
+            throw std::invalid_argument("tgx_server_submit: the server is shutting down");
+        }
+        r->ticket = s->nextTicket++;
+        *ticket = r->ticket;
+        s->pending.push_back(r.get());
+        s->requests.emplace(r->ticket, std::move(r));
+        s->wake.notify_one();
+    });
+}
+
+int tgx_server_wait(tgx_server* s, uint64_t ticket, double timeout_seconds,
+                    tgx_ticket_info* info, char* err, size_t errLen) {
+    if (s == nullptr) {
+        tgh::SetErr(err, errLen, "tgx_server_wait: null server");
+        return TG_ERR_ARGUMENT;
+    }
+    std::unique_lock<std::mutex> lock(s->mutex);
+    auto it = s->requests.find(ticket);
+    if (it == s->requests.end()) {
+        tgh::SetErr(err, errLen, "tgx_server_wait: unknown ticket");
+        return TG_ERR_ARGUMENT;
+    }
+    tgh::Request* r = it->second.get();
+    auto finished = [&] { return r->status != TG_NOT_READY; };
+    if (timeout_seconds < 0.0) {
+        s->done.wait(lock, finished);
+    } else if (!s->done.wait_for(lock, std::chrono::duration<double>(timeout_seconds),
+                                 finished)) {
+        return TG_NOT_READY;
+    }
+    tgh::Fill(*r, info);
+    const int status = r->status;
+    if (status != TG_OK) {
+        tgh::SetErr(err, errLen, r->error);
+    }
+    s->requests.erase(it);
+    return status;
+}
+
+int tgx_server_stats_get(tgx_server* s, tgx_server_stats* out) {
+    if (s == nullptr || out == nullptr) {
+        return TG_ERR_ARGUMENT;
+    }
+    std::lock_guard<std::mutex> lock(s->mutex);
+    *out = s->stats;
+    out->pool_peak_bytes = s->ranges->Peak();
+    out->pending = s->pending.size();
+    return TG_OK;
+}
+
+int tgx_server_free(tgx_server* s) {
+    if (s == nullptr) {
+        return TG_OK;
+    }
+    {
+        std::lock_guard<std::mutex> lock(s->mutex);
+        s->stop = true;
+        s->wake.notify_all();
+    }
+    if (s->dispatcher.joinable()) {
+        s->dispatcher.join();
+    }
+    for (tgh::Slot& slot : s->slots) {
+        tgh::FreeSlot(*s, slot);
+    }
+    if (s->pool != nullptr) {
+        s->hip->free(s->dev, s->pool);
+    }
+    delete s;
+    return TG_OK;
+}
+
+const char* tgx_hip_unavailable_reason(void) {
+    static thread_local std::string reason;
+    reason = tgh::HipUnavailableReason();
+    return reason.c_str();
+}
+
+}  // extern "C"

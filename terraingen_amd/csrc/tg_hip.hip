@@ -649,6 +649,77 @@ int tg_hip_event_elapsed(tg_dev* dev, void* start, void* stop, float* ms, char* 
     });
 }
 
+int tg_hip_host_alloc(tg_dev* dev, size_t bytes, void** out, char* err, size_t errLen) {
+    if (dev == nullptr || out == nullptr) {
+        SetErr(err, errLen, "tg_hip_host_alloc: null argument");
+        return TG_ERR_ARGUMENT;
+    }
+    *out = nullptr;
+    return Guard(err, errLen, [&] {
+        Check(hipSetDevice(dev->ordinal), "hipSetDevice");
+        Check(hipHostMalloc(out, bytes > 0 ? bytes : 1, hipHostMallocDefault), "hipHostMalloc");
+    });
+}
+
+int tg_hip_host_free(tg_dev* dev, void* ptr) {
+    if (dev == nullptr) {
+        return TG_ERR_ARGUMENT;
+    }
+    if (ptr != nullptr) {
+        (void)hipSetDevice(dev->ordinal);
+        (void)hipHostFree(ptr);
+    }
+    return TG_OK;
+}
+
+int tg_hip_memcpy_d2h_async(tg_dev* dev, void* dst, const void* src, size_t bytes, void* stream,
+                            char* err, size_t errLen) {
+    if (dev == nullptr || (bytes > 0 && (dst == nullptr || src == nullptr))) {
+        SetErr(err, errLen, "tg_hip_memcpy_d2h_async: null argument");
+        return TG_ERR_ARGUMENT;
+    }
+    return Guard(err, errLen, [&] {
+        Check(hipSetDevice(dev->ordinal), "hipSetDevice");
+        if (bytes > 0) {
+            Check(hipMemcpyAsync(dst, src, bytes, hipMemcpyDeviceToHost,
+                                 static_cast<hipStream_t>(stream)),
+                  "hipMemcpyAsync to host");
+        }
+    });
+}
+
+int tg_hip_memcpy_h2d_async(tg_dev* dev, void* dst, const void* src, size_t bytes, void* stream,
+                            char* err, size_t errLen) {
+    if (dev == nullptr || (bytes > 0 && (dst == nullptr || src == nullptr))) {
+        SetErr(err, errLen, "tg_hip_memcpy_h2d_async: null argument");
+        return TG_ERR_ARGUMENT;
+    }
+    return Guard(err, errLen, [&] {
+        Check(hipSetDevice(dev->ordinal), "hipSetDevice");
+        if (bytes > 0) {
+            Check(hipMemcpyAsync(dst, src, bytes, hipMemcpyHostToDevice,
+                                 static_cast<hipStream_t>(stream)),
+                  "hipMemcpyAsync to device");
+        }
+    });
+}
+
+int tg_hip_event_query(tg_dev* dev, void* event, char* err, size_t errLen) {
+    if (dev == nullptr || event == nullptr) {
+        SetErr(err, errLen, "tg_hip_event_query: null argument");
+        return TG_ERR_ARGUMENT;
+    }
+    const hipError_t status = hipEventQuery(static_cast<hipEvent_t>(event));
+    if (status == hipSuccess) {
+        return TG_OK;
+    }
+    if (status == hipErrorNotReady) {
+        return TG_NOT_READY;
+    }
+    SetErr(err, errLen, std::string("hipEventQuery: ") + hipGetErrorString(status));
+    return TG_ERR_DEVICE;
+}
+
 }  // extern "C"
 
 namespace tgi {

@@ -126,6 +126,11 @@ HIP_ABI = {
     "tg_hip_event_destroy": (C.c_int, [VP, VP]),
     "tg_hip_event_record": (C.c_int, [VP, VP, VP, CP, SZ]),
     "tg_hip_event_elapsed": (C.c_int, [VP, VP, VP, C.POINTER(C.c_float), CP, SZ]),
+    "tg_hip_host_alloc": (C.c_int, [VP, SZ, C.POINTER(VP), CP, SZ]),
+    "tg_hip_host_free": (C.c_int, [VP, VP]),
+    "tg_hip_memcpy_d2h_async": (C.c_int, [VP, VP, VP, SZ, VP, CP, SZ]),
+    "tg_hip_memcpy_h2d_async": (C.c_int, [VP, VP, VP, SZ, VP, CP, SZ]),
+    "tg_hip_event_query": (C.c_int, [VP, VP, CP, SZ]),
 }
 
 _hip = None
@@ -499,3 +504,86 @@ def batch_pair_ops(descs, bins, entries, foot: np.ndarray, cost: np.ndarray, thr
                                   threads, C.byref(listed), C.byref(inside), C.byref(ops), err,
                                   512), err)
     return int(listed.value), int(inside.value), float(ops.value)
+
+
+HOST_ABI["tgx_index_create"] = (C.c_int, [VP, VP, VP, VP, SZ, C.POINTER(VP), CP, SZ])
+HOST_ABI["tgx_index_free"] = (None, [VP])
+HOST_ABI["tgx_index_count"] = (SZ, [VP])
+HOST_ABI["tgx_index_bin"] = (C.c_int, [VP, VP, SZ, C.c_uint, C.POINTER(VP), C.POINTER(VP),
+                                       C.POINTER(SZ), C.POINTER(VP), C.POINTER(SZ), CP, SZ])
+
+
+def _take_batch(lib, tiles_n, dp, bp, bn, ep, en):
+    try:
+        descs = np.frombuffer(C.string_at(dp.value, tiles_n * 48), TILE_DESC_DTYPE).copy()
+        bins = np.frombuffer(C.string_at(bp.value, bn.value * 16), BIN_DTYPE).copy()
+        entries = np.frombuffer(C.string_at(ep.value, max(en.value, 1) * 4), np.uint32)[
+            :en.value].copy()
+    finally:
+        for p in (dp, bp, ep):
+            lib.tgx_free(p)
+    return descs, bins, entries
+
+
+class CatalogueIndex:
+    """``tgx_index``: a catalogue footprint (x, y, reach, D per crater, apply order) indexed once
+    for binning many batches of tiles."""
+
+    def __init__(self, foot: np.ndarray):
+        lib = host_library()
+        foot = np.ascontiguousarray(foot, np.float64)
+        cols = [np.ascontiguousarray(foot[:, i]) for i in range(4)]
+        h = VP()
+        err = C.create_string_buffer(512)
+        _check(lib.tgx_index_create(*(c.ctypes.data for c in cols), cols[0].size, C.byref(h),
+                                    err, 512), err)
+        self.handle = h.value
+        self.count = cols[0].size
+
+    def bin(self, tiles: np.ndarray, threads: int = 0):
+        """(tile descriptors, bins, entries) of a structured array of tiles."""
+        lib = host_library()
+        tiles = np.ascontiguousarray(tiles, TILE_DTYPE)
+        dp, bp, ep = VP(), VP(), VP()
+        bn, en = SZ(), SZ()
+        err = C.create_string_buffer(512)
+        _check(lib.tgx_index_bin(self.handle, tiles.ctypes.data, tiles.size, threads,
+                                 C.byref(dp), C.byref(bp), C.byref(bn), C.byref(ep),
+                                 C.byref(en), err, 512), err)
+        return _take_batch(lib, tiles.size, dp, bp, bn, ep, en)
+
+    def __del__(self):
+        if getattr(self, "handle", None):
+            host_library().tgx_index_free(self.handle)
+            self.handle = None
+
+
+# ---- The tile server (tgx_server_*) -----------------------------------------------------------
+
+ServerConfig = _struct("ServerConfig", [("slots", u32), ("max_batch_tiles", u32),
+                                        ("max_batch_cells", C.c_uint64),
+                                        ("pool_bytes", C.c_uint64), ("bin_threads", u32),
+                                        ("poll_microseconds", u32)])
+TicketInfo = _struct("TicketInfo", [("submitted", C.c_double), ("admitted", C.c_double),
+                                    ("launched", C.c_double), ("done", C.c_double),
+                                    ("batch", u32), ("batch_tiles", u32)])
+ServerStats = _struct("ServerStats", [("batches", C.c_uint64), ("tiles", C.c_uint64),
+                                      ("cells", C.c_uint64), ("failed", C.c_uint64),
+                                      ("max_in_flight", u32), ("pad", u32),
+                                      ("pool_bytes", C.c_uint64),
+                                      ("pool_peak_bytes", C.c_uint64),
+                                      ("pending", C.c_uint64)])
+TileDesc = _struct("TileDesc", [("origin_x", C.c_double), ("origin_y", C.c_double),
+                                ("pixel", C.c_double), ("width", u32), ("height", u32)])
+
+HOST_ABI["tgx_server_create"] = (C.c_int, [VP, VP, C.c_float, VP, VP, C.POINTER(ServerConfig),
+                                           C.POINTER(VP), CP, SZ])
+HOST_ABI["tgx_server_submit"] = (C.c_int, [VP, C.POINTER(TileDesc), VP, C.POINTER(C.c_uint64),
+                                           CP, SZ])
+HOST_ABI["tgx_server_wait"] = (C.c_int, [VP, C.c_uint64, C.c_double, C.POINTER(TicketInfo), CP,
+                                         SZ])
+HOST_ABI["tgx_server_stats_get"] = (C.c_int, [VP, C.POINTER(ServerStats)])
+HOST_ABI["tgx_server_free"] = (C.c_int, [VP])
+HOST_ABI["tgx_hip_unavailable_reason"] = (C.c_char_p, [])
+
+TG_NOT_READY = 4

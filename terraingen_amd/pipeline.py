@@ -201,6 +201,7 @@ class TileBatch:
             raise ValueError("the catalogue's finest pixel size drops craters: lower min_diameter "
                              "or finest_pixel")
         self.ordered = self.crater_list[apply_order(self.crater_list)]
+        self.index = N.CatalogueIndex(self.foot)
         t2 = time.perf_counter()
         diam = np.ascontiguousarray(self.foot[:, 3])
         cat = C.c_void_p()
@@ -224,7 +225,7 @@ class TileBatch:
     def stage(self, tiles: np.ndarray, threads: int = 0) -> None:
         t0 = time.perf_counter()
         self.tiles = np.ascontiguousarray(tiles, N.TILE_DTYPE)
-        self.descs, self.bins, self.entries = N.batch_bin(self.foot, self.tiles, threads)
+        self.descs, self.bins, self.entries = self.index.bin(self.tiles, threads)
         t1 = time.perf_counter()
         if self.batch is None or self._capacity < (self.tiles.size, self.bins.size,
                                                    self.entries.size):
@@ -350,3 +351,65 @@ def crater_pair_cost(records: np.ndarray, ops_pair: float, ops_harmonic: float,
     a term per outline harmonic, and one for an elongated outline."""
     return (ops_pair + ops_harmonic * records["harmonic_count"].astype(np.float64) +
             ops_ellipse * (records["ellipticity"] != 1.0))
+
+
+class TileServer:
+    """The native tile server (tgx_server_*) over a built TileBatch's resident noise plan,
+    catalogue and index: continuous batching of tile requests onto the batched kernel."""
+
+    def __init__(self, batch: TileBatch, slots: int = 3, max_batch_tiles: int = 64,
+                 max_batch_cells: int = 1 << 24, pool_bytes: int = 1 << 31,
+                 bin_threads: int = 0, poll_microseconds: int = 20):
+        self.batch = batch
+        self.lib = N.host_library()
+        self.config = N.ServerConfig(slots, max_batch_tiles, max_batch_cells, pool_bytes,
+                                     bin_threads, poll_microseconds)
+        h = C.c_void_p()
+        err = C.create_string_buffer(512)
+        code = self.lib.tgx_server_create(batch.dev.handle, batch.plan, batch.elevation,
+                                          batch.catalogue, batch.index.handle,
+                                          C.byref(self.config), C.byref(h), err, 512)
+        if code:
+            raise N.HipError(err.value.decode())
+        self.handle = h.value
+        self._dst = {}
+
+    def submit(self, origin_x: float, origin_y: float, pixel: float, width: int,
+               height: int) -> int:
+        """Queues a tile; returns its ticket. Heights land in a new array owned by the server
+        object until `wait` returns it."""
+        dst = np.empty((height, width), np.float32)
+        tile = N.TileDesc(origin_x, origin_y, pixel, width, height)
+        ticket = C.c_uint64()
+        err = C.create_string_buffer(512)
+        code = self.lib.tgx_server_submit(self.handle, C.byref(tile), dst.ctypes.data,
+                                          C.byref(ticket), err, 512)
+        if code:
+            raise N.HipError(err.value.decode())
+        self._dst[ticket.value] = dst
+        return ticket.value
+
+    def wait(self, ticket: int, timeout: float = -1.0):
+        """(heights, info dict) of a finished ticket; blocks until it finishes."""
+        info = N.TicketInfo()
+        err = C.create_string_buffer(512)
+        code = self.lib.tgx_server_wait(self.handle, ticket, timeout, C.byref(info), err, 512)
+        if code == N.TG_NOT_READY:
+            raise TimeoutError(f"ticket {ticket} not finished after {timeout} s")
+        dst = self._dst.pop(ticket)
+        if code:
+            raise N.HipError(err.value.decode())
+        return dst, {name: getattr(info, name) for name, _ in N.TicketInfo._fields_}
+
+    def stats(self) -> dict:
+        st = N.ServerStats()
+        self.lib.tgx_server_stats_get(self.handle, C.byref(st))
+        return {name: getattr(st, name) for name, _ in N.ServerStats._fields_ if name != "pad"}
+
+    def close(self) -> None:
+        if getattr(self, "handle", None):
+            self.lib.tgx_server_free(self.handle)
+            self.handle = None
+
+    def __del__(self):
+        self.close()

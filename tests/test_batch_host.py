@@ -104,3 +104,17 @@ def test_pair_counter_matches_brute_force():
     assert listed == want_listed and inside == want_inside
     assert ops == pytest.approx(want_ops, rel=1e-12)
     assert 0 < inside < listed
+
+
+def test_kept_index_bins_as_one_shot_binning():
+    foot = catalogue(2000, 150.0, 7)
+    tiles = tiles_for(150.0, 8)
+    index = N.CatalogueIndex(foot)
+    for threads in (1, 4):
+        got = index.bin(tiles, threads)
+        want = N.batch_bin(foot, tiles, threads)
+        for a, b in zip(got, want):
+            assert np.array_equal(a, b)
+    one = index.bin(tiles[2:3], 1)
+    alone = N.batch_bin(foot, tiles[2:3], 1)
+    assert all(np.array_equal(a, b) for a, b in zip(one, alone))
