@@ -15,8 +15,14 @@ are FP32, with FP64 cell centres.
 * ``plain``: Gcells/s of the plain per-tile path over the same tiles. Each tile goes through
   ``tg_hip_run``, once for the noise layer and once for the craters, synchronous with host memory
   in and out, as RunKernel drives a device. Host preparation of its inputs is excluded.
-* ``tile_latency_513_ms``: p50/p99 request-to-host latency of single 513 x 513 tiles through the
-  batched path: binning, staging, the launch and the copy back, one request at a time.
+* ``serving``: the served stream, run through the tile server (continuous batching, several
+  batches in flight, a 64 GB HBM admission pool). Tile requests arrive as a Poisson process at
+  2000/s for 10 s from a camera flying a 50 km straight ground track at 200 m altitude. Each
+  request is the quadtree tile, at levels 0-10 by slant range, that holds a ground point near the
+  camera, with edges of 129, 257, 513 or 1025 cells. The noise is as above, and the crater
+  catalogue (D >= 2 m, same SFD, seed 5) covers a 2 km swath along the track. It reports the
+  p50/p99 request-to-host latency (submission to heights in the caller's memory) for 513 x 513
+  tiles and for all tiles, and the served Gcells/s.
 
 ``--workload grid`` is one 4097 x 4097 grid at 0.5 m, with the same noise and craters (seed 1),
 evaluated by the strip kernels.
@@ -68,6 +74,26 @@ def roofline(ops: float, kernel_ms: float, bytes_written: float) -> dict:
             "frac": round(achieved / VALU_PEAK, 4), "traffic": None,
             "hbm_write_GBs": round(bytes_written / (kernel_ms * 1e-3) / 1e9, 1),
             "hbm_write_frac": round(bytes_written / (kernel_ms * 1e-3) / 1e9 / HBM_PEAK_GBS, 5)}
+
+
+def counted_traffic(kernel_ms: float) -> dict | None:
+    """HBM bytes per launch of the default tiles workload, from the committed counter runs
+    (FETCH_SIZE and WRITE_SIZE, each scaled by a 1 GiB copy's reading), with the VALU
+    instructions those runs counted, scaled to this run's launch time."""
+    path = ROOT / "profiles" / "batch_tiles_c4_1024_mixed_counters.md"
+    if not path.exists():
+        return None
+    for line in path.read_text().splitlines():
+        if line.strip().startswith("traffic: "):
+            meta = json.loads(line.strip()[len("traffic: "):])
+            lane = meta["valu_lane_instructions"]
+            return {"traffic": round(meta["read_bytes"] + meta["write_bytes"]),
+                    "traffic_read": round(meta["read_bytes"]),
+                    "traffic_write": round(meta["write_bytes"]),
+                    "valu_lane_instructions": lane,
+                    "valu_issue_frac": round(lane / (kernel_ms * 1e-3) / VALU_PEAK, 4),
+                    "counters": str(path.relative_to(ROOT))}
+    return None
 
 
 def percentile(values, q) -> float:
@@ -213,19 +239,28 @@ def bench_tiles(args, N, dev) -> dict:
     print(f"[bench] batched {step_ms:.3f} ms/step", file=sys.stderr, flush=True)
     got = out.download(np.empty(batch.cells, np.float32))
 
+    if args.copy_calibration_mib > 0:
+        # A copy of known size on the same stream, for scaling the HBM byte counters.
+        nbytes = args.copy_calibration_mib << 20
+        src, dst = N.DeviceArray(dev, nbytes), N.DeviceArray(dev, nbytes)
+        dev.call("tg_hip_memcpy_d2d_async", dev.handle, dst.ptr, src.ptr, nbytes, stream)
+        batch.wait()
+        del src, dst
+
     # The plain path over every tile: tg_hip_run per layer with host buffers.
     plain_s = 0.0
     plain_cells = 0
     worst_plain = 0.0
-    for i in range(tiles.size):
+    for i in range(tiles.size if not args.no_plain else 0):
         inp = batch.plain_inputs(i)
         t1 = time.perf_counter()
         h = batch.plain_run(inp)
         plain_s += time.perf_counter() - t1
         plain_cells += h.size
         worst_plain = max(worst_plain, float(np.abs(h - batch.tile_heights(got, i)).max()))
-    plain = plain_cells / plain_s / 1e9
-    print(f"[bench] plain {plain:.3f} Gcells/s", file=sys.stderr, flush=True)
+    plain = plain_cells / plain_s / 1e9 if plain_cells else None
+    if plain is not None:
+        print(f"[bench] plain {plain:.3f} Gcells/s", file=sys.stderr, flush=True)
 
     # Algorithmic work of one launch, counted on the host from the bins and footprints.
     cells = batch.cells
@@ -255,22 +290,6 @@ def bench_tiles(args, N, dev) -> dict:
         extra["error_vs_fp64_oracle_m"] = err
         extra["max_diff_vs_fp32_oracle_m"] = diff32
 
-    # Unloaded request latency of single 513 x 513 tiles through the batched path.
-    rng = np.random.default_rng(5)
-    lat = []
-    for r in range(args.latency_requests):
-        one = np.zeros(1, N.TILE_DTYPE)
-        one["width"] = one["height"] = 513
-        one["pixel"] = np.exp(rng.uniform(np.log(0.25), np.log(8.0)))
-        one["origin_x"], one["origin_y"] = rng.uniform(0.0, TILES_SIDE, 2)
-        host = np.empty(513 * 513, np.float32)
-        t1 = time.perf_counter()
-        batch.stage(one)
-        batch.run(out, use_graph=True)
-        out.download(host)  # synchronous: waits for the stream's work
-        lat.append((time.perf_counter() - t1) * 1e3)
-    lat = lat[min(3, len(lat) // 4):]  # the first requests capture a graph for the smaller grid
-
     result = {
         "metric": METRIC, "value": round(value, 3), "unit": "Gcells/s",
         "ms_per_step": round(step_ms, 4), "wall_ms_per_step": round(wall_ms, 4),
@@ -283,22 +302,70 @@ def bench_tiles(args, N, dev) -> dict:
                    "craters": int(batch.records.size), "bins": n_bins,
                    "entries": n_entries, "pairs_listed": listed,
                    "pairs_in_reach": inside},
-        "plain": {"value": round(plain, 4), "unit": "Gcells/s",
+        "plain": {"value": round(plain, 4) if plain else None, "unit": "Gcells/s",
                   "path": "tg_hip_run per tile and layer, host buffers, synchronous",
-                  "max_diff_vs_batched_m": worst_plain},
-        "batched_over_plain": round(value / plain, 2),
-        "tile_latency_513_ms": {"p50": round(percentile(lat, 50), 3),
-                                "p99": round(percentile(lat, 99), 3), "requests": len(lat),
-                                "mode": "one request at a time, to heights in host memory"},
+                  "max_diff_vs_batched_m": worst_plain if plain else None},
+        "batched_over_plain": round(value / plain, 2) if plain else None,
         "kernels_ms": {"batch_tiles": round(kernel_ms, 4)},
         "host_ms": {k: round(v, 1) for k, v in batch.timings.items()} | {
             "total_setup": round(host_ms, 1)},
         "roofline": roofline(ops_noise + ops_craters, kernel_ms, cells * 4.0) | {
             "kernel": "BatchTiles", "ops_noise": ops_noise, "ops_craters": ops_craters},
     }
+    if tiles.size == 1024 and (counted := counted_traffic(kernel_ms)) is not None:
+        result["roofline"] |= counted
     result |= extra
     batch.close()
+    if args.serve_seconds > 0:
+        result["serving"] = bench_serving(args, N, dev)
     return result
+
+
+def bench_serving(args, N, dev) -> dict:
+    from terraingen_amd.pipeline import CraterLayerSpec, TileBatch, TileServer
+    from terraingen_amd.serving import CameraStream, serve
+
+    stream = CameraStream(duration=args.serve_seconds)
+    requests = stream.requests()
+    t0 = time.perf_counter()
+    spec = CraterLayerSpec(seed=5, min_diameter=2.0)
+    swath = TileBatch(dev, [C2_FRACTAL], spec,
+                      region=(-2000.0, -1000.0, stream.path_length + 2000.0, 1000.0),
+                      finest_pixel=1.0)
+    swath.build()
+    server = TileServer(swath, slots=4, max_batch_tiles=32, max_batch_cells=1 << 24,
+                        pool_bytes=64 << 30, bin_threads=8)
+    setup_ms = (time.perf_counter() - t0) * 1e3
+    # Warm the server's slots before the timed stream.
+    for i in range(8):
+        r = requests[i]
+        server.wait(server.submit(float(r["origin_x"]), float(r["origin_y"]),
+                                  float(r["pixel"]), int(r["width"]), int(r["height"])))
+    print(f"[bench] serving {requests.size} requests over {stream.duration} s, "
+          f"{swath.records.size} craters", file=sys.stderr, flush=True)
+    rec = serve(server, requests)
+    stats = server.stats()
+    server.close()
+    swath.close()
+    lat_ms = rec["latency_s"] * 1e3
+    is513 = requests["width"] == 513
+    cells = float((requests["width"].astype(np.float64) * requests["height"]).sum())
+    return {
+        "requests": int(requests.size), "rate_per_s": stream.rate,
+        "duration_s": stream.duration,
+        "latency_513_ms": {"p50": round(percentile(lat_ms[is513], 50), 3),
+                           "p99": round(percentile(lat_ms[is513], 99), 3),
+                           "count": int(is513.sum())},
+        "latency_all_ms": {"p50": round(percentile(lat_ms, 50), 3),
+                           "p99": round(percentile(lat_ms, 99), 3),
+                           "max": round(float(lat_ms.max()), 3)},
+        "served_gcells_per_s": round(cells / rec["wall_s"] / 1e9, 3),
+        "batches": int(stats["batches"]), "max_in_flight": int(stats["max_in_flight"]),
+        "mean_batch_tiles": round(float(rec["batch_tiles"].mean()), 2),
+        "pool_peak_bytes": int(stats["pool_peak_bytes"]),
+        "levels": sorted(int(v) for v in np.unique(requests["level"])),
+        "craters": int(swath.records.size), "setup_ms": round(setup_ms, 1),
+    }
 
 
 def bench_grid(args, N, dev) -> dict:
@@ -383,10 +450,14 @@ def main() -> int:
     ap.add_argument("--workload", choices=("tiles", "grid"), default="tiles")
     ap.add_argument("--tiles", type=int, default=1024)
     ap.add_argument("--size", type=int, default=4097, help="grid workload edge")
-    ap.add_argument("--latency-requests", type=int, default=200)
+    ap.add_argument("--serve-seconds", type=float, default=10.0,
+                    help="length of the served stream (0: skip it)")
     ap.add_argument("--cpu-cells", type=int, default=6_000_000,
                     help="cells of the tiles workload timed on the host (CPU baseline)")
     ap.add_argument("--no-cpu-baseline", action="store_true")
+    ap.add_argument("--no-plain", action="store_true", help="skip the plain per-tile path")
+    ap.add_argument("--copy-calibration-mib", type=int, default=0,
+                    help="also run one device copy of this size (for counter calibration)")
     args = ap.parse_args()
     if args.gpus != 1:
         print(json.dumps({"error": "this build measures one GPU: tiles would shard across GPUs "

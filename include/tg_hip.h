@@ -207,6 +207,9 @@ int tg_hip_memcpy_h2d(tg_dev* dev, void* dst, const void* src, size_t bytes, cha
 int tg_hip_memcpy_d2h(tg_dev* dev, void* dst, const void* src, size_t bytes, char* err,
                       size_t err_len);
 int tg_hip_memset(tg_dev* dev, void* dst, int value, size_t bytes, char* err, size_t err_len);
+/* Device-to-device copy on `stream` (the runtime's copy kernel). */
+int tg_hip_memcpy_d2d_async(tg_dev* dev, void* dst, const void* src, size_t bytes, void* stream,
+                            char* err, size_t err_len);
 int tg_hip_stream_create(tg_dev* dev, void** out, char* err, size_t err_len);
 int tg_hip_stream_destroy(tg_dev* dev, void* stream);
 int tg_hip_stream_synchronize(tg_dev* dev, void* stream, char* err, size_t err_len);

@@ -704,6 +704,22 @@ int tg_hip_memcpy_h2d_async(tg_dev* dev, void* dst, const void* src, size_t byte
     });
 }
 
+int tg_hip_memcpy_d2d_async(tg_dev* dev, void* dst, const void* src, size_t bytes, void* stream,
+                            char* err, size_t errLen) {
+    if (dev == nullptr || (bytes > 0 && (dst == nullptr || src == nullptr))) {
+        SetErr(err, errLen, "tg_hip_memcpy_d2d_async: null argument");
+        return TG_ERR_ARGUMENT;
+    }
+    return Guard(err, errLen, [&] {
+        Check(hipSetDevice(dev->ordinal), "hipSetDevice");
+        if (bytes > 0) {
+            Check(hipMemcpyAsync(dst, src, bytes, hipMemcpyDeviceToDevice,
+                                 static_cast<hipStream_t>(stream)),
+                  "hipMemcpyAsync on the device");
+        }
+    });
+}
+
 int tg_hip_event_query(tg_dev* dev, void* event, char* err, size_t errLen) {
     if (dev == nullptr || event == nullptr) {
         SetErr(err, errLen, "tg_hip_event_query: null argument");

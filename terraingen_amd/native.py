@@ -131,6 +131,7 @@ HIP_ABI = {
     "tg_hip_memcpy_d2h_async": (C.c_int, [VP, VP, VP, SZ, VP, CP, SZ]),
     "tg_hip_memcpy_h2d_async": (C.c_int, [VP, VP, VP, SZ, VP, CP, SZ]),
     "tg_hip_event_query": (C.c_int, [VP, VP, CP, SZ]),
+    "tg_hip_memcpy_d2d_async": (C.c_int, [VP, VP, VP, SZ, VP, CP, SZ]),
 }
 
 _hip = None

@@ -1,0 +1,117 @@
+# SPDX-License-Identifier: BSD-3-Clause
+"""A served stream of tile requests from a moving camera, driven through the tile server.
+
+Requests arrive as a Poisson process. At each arrival, the camera has flown along a straight
+ground track at constant speed and altitude. The requested tile is the quadtree tile, at the level
+of detail the camera's distance calls for, that holds a ground point drawn around the camera.
+
+The level-of-detail rule (a screen-space-error rule of this module):
+* level 10 (the finest) covers tiles of ``finest_side`` metres; each coarser level doubles the
+  side;
+* the level for a point at slant range rho from the camera is ``10 - floor(log2(rho / altitude))``,
+  clamped to 0..10, so the tile's ground size grows in proportion to its distance;
+* the tile's edge is drawn from ``edges`` (cells, 2^k + 1), so its pixel size is
+  side / (edge - 1).
+"""
+from __future__ import annotations
+
+import threading
+import time
+from dataclasses import dataclass
+
+import numpy as np
+
+
+@dataclass
+class CameraStream:
+    rate: float = 2000.0            # requests per second
+    duration: float = 10.0          # seconds
+    path_length: float = 50_000.0   # metres of ground track, flown along +x from x = 0, y = 0
+    altitude: float = 200.0         # metres
+    finest_side: float = 128.0      # ground side of a level-10 tile, metres
+    levels: int = 10
+    edges: tuple = (129, 257, 513, 1025)
+    min_ground: float = 20.0        # ground distances from the camera, log-uniform, metres
+    max_ground: float = 20_000.0
+    seed: int = 5
+
+    def requests(self) -> np.ndarray:
+        """Arrival time (s from the start), level and tile of every request, in arrival order."""
+        rng = np.random.default_rng(self.seed)
+        n = rng.poisson(self.rate * self.duration)
+        t = np.sort(rng.uniform(0.0, self.duration, n))
+        cam_x = self.path_length * t / self.duration
+        ground = np.exp(rng.uniform(np.log(self.min_ground), np.log(self.max_ground), n))
+        bearing = rng.uniform(0.0, 2.0 * np.pi, n)
+        px = cam_x + ground * np.cos(bearing)
+        py = ground * np.sin(bearing)
+        rho = np.hypot(ground, self.altitude)
+        level = np.clip(self.levels - np.floor(np.log2(rho / self.altitude)), 0,
+                        self.levels).astype(np.int64)
+        side = self.finest_side * np.exp2(self.levels - level)
+        edge = np.asarray(self.edges)[rng.integers(0, len(self.edges), n)]
+        out = np.zeros(n, dtype=[("t", np.float64), ("level", np.int32), ("origin_x", np.float64),
+                                 ("origin_y", np.float64), ("pixel", np.float64),
+                                 ("width", np.uint32), ("height", np.uint32)])
+        out["t"] = t
+        out["level"] = level
+        out["origin_x"] = np.floor(px / side) * side
+        out["origin_y"] = (np.floor(py / side) + 1.0) * side
+        out["pixel"] = side / (edge - 1)
+        out["width"] = edge
+        out["height"] = edge
+        return out
+
+
+def serve(server, requests: np.ndarray, keep=None) -> dict:
+    """Submits `requests` at their arrival times (a submitting thread) while another thread waits
+    for them in order. Returns per-request (latency, batch, batch size) arrays and the wall time.
+    `keep(i, heights)` may look at each result; by default results are dropped once done."""
+    n = requests.size
+    tickets = np.zeros(n, np.uint64)
+    submitted = threading.Semaphore(0)
+    lat = np.zeros(n)
+    queue_wait = np.zeros(n)
+    batch = np.zeros(n, np.int64)
+    batch_tiles = np.zeros(n, np.int64)
+    errors = []
+    start = time.perf_counter() + 0.05
+
+    def submitter():
+        try:
+            for i in range(n):
+                due = start + float(requests["t"][i])
+                while True:
+                    left = due - time.perf_counter()
+                    if left <= 0.0:
+                        break
+                    if left > 0.0015:
+                        time.sleep(left - 0.001)
+                r = requests[i]
+                tickets[i] = server.submit(float(r["origin_x"]), float(r["origin_y"]),
+                                           float(r["pixel"]), int(r["width"]), int(r["height"]))
+                submitted.release()
+        except Exception as e:  # noqa: BLE001 - reported to the caller
+            errors.append(e)
+            for _ in range(n):
+                submitted.release()
+
+    th = threading.Thread(target=submitter)
+    th.start()
+    for i in range(n):
+        submitted.acquire()
+        if errors:
+            break
+        heights, info = server.wait(int(tickets[i]))
+        lat[i] = info["done"] - info["submitted"]
+        queue_wait[i] = info["admitted"] - info["submitted"]
+        batch[i] = info["batch"]
+        batch_tiles[i] = info["batch_tiles"]
+        if keep is not None:
+            keep(i, heights)
+    th.join()
+    if errors:
+        raise errors[0]
+    wall = time.perf_counter() - start
+    return {"latency_s": lat, "queue_s": queue_wait, "batch": batch, "batch_tiles": batch_tiles,
+            "wall_s": wall}
