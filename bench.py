@@ -59,7 +59,10 @@ OPS_PER_PERLIN_OCTAVE = 200
 OPS_PAIR = 110
 OPS_PER_HARMONIC = 12
 OPS_ELLIPSE = 12
-VALU_PEAK = 256 * 64 * 2.4e9  # lane-ops/s: 256 CUs x 64 lanes x 2.4 GHz
+# MI355X's published FP32 vector peak, 157.3 TFLOP/s, in lane operations: an FMA counts 2 flops
+# and the peak assumes the packed two-lane form, so 157.3e12 / 2 lane-ops/s (256 CUs x 64 lanes x
+# 2 x 2.4 GHz).
+VALU_PEAK = 157.3e12 / 2.0
 HBM_PEAK_GBS = 8000.0
 METRIC = "Gcells/s batched vs plain HIP at <=1e-3 m err; p50/p99 tile latency"
 
@@ -86,6 +89,10 @@ def counted_traffic(kernel_ms: float) -> dict | None:
     for line in path.read_text().splitlines():
         if line.strip().startswith("traffic: "):
             meta = json.loads(line.strip()[len("traffic: "):])
+            # Counts of another build of the kernel would not describe this one: they are used
+            # only when the counted launch took the time this one takes.
+            if abs(meta["kernel_ns"] * 1e-6 / kernel_ms - 1.0) > 0.05:
+                return {"traffic": None, "counters_stale": str(path.relative_to(ROOT))}
             lane = meta["valu_lane_instructions"]
             return {"traffic": round(meta["read_bytes"] + meta["write_bytes"]),
                     "traffic_read": round(meta["read_bytes"]),
@@ -94,6 +101,18 @@ def counted_traffic(kernel_ms: float) -> dict | None:
                     "valu_issue_frac": round(lane / (kernel_ms * 1e-3) / VALU_PEAK, 4),
                     "counters": str(path.relative_to(ROOT))}
     return None
+
+
+def valu_ceiling(N, dev) -> dict:
+    """The measured vector-ALU issue ceiling (tg_hip_valu_probe): lane-ops/s of FP32 FMA chains
+    and of 32-bit integer xor/add chains at full occupancy."""
+    out = {}
+    for kind, name in ((0, "fp32_fma"), (1, "int32_xor_add")):
+        ms, ops = C.c_float(), C.c_double()
+        dev.call("tg_hip_valu_probe", dev.handle, kind, 20000, 8192, C.byref(ms), C.byref(ops))
+        out[name] = round(ops.value / (ms.value * 1e-3) / 1e12, 2)
+    out["unit"] = "T lane-ops/s"
+    return out
 
 
 def percentile(values, q) -> float:
@@ -218,6 +237,14 @@ def bench_tiles(args, N, dev) -> dict:
           f" {batch.bins.size} bins, {batch.entries.size} entries", file=sys.stderr, flush=True)
     out = N.DeviceArray(dev, batch.cells * 4)
     stream = batch.stream
+    tuning = None
+    if not args.no_autotune:
+        timings = batch.autotune(out)
+        tuning = {"chosen": {"walk": ["together", "per_lane"][batch.tuned[0]],
+                             "groups_per_cu": batch.tuned[1]},
+                  "ms": {f"{['together', 'per_lane'][w]}/{g}": round(ms, 3)
+                         for (w, g), ms in timings.items()}}
+        print(f"[bench] autotune {tuning}", file=sys.stderr, flush=True)
     for _ in range(args.warmup):
         batch.run(out, use_graph=True)
     batch.wait()
@@ -307,6 +334,7 @@ def bench_tiles(args, N, dev) -> dict:
                   "max_diff_vs_batched_m": worst_plain if plain else None},
         "batched_over_plain": round(value / plain, 2) if plain else None,
         "kernels_ms": {"batch_tiles": round(kernel_ms, 4)},
+        "autotune": tuning,
         "host_ms": {k: round(v, 1) for k, v in batch.timings.items()} | {
             "total_setup": round(host_ms, 1)},
         "roofline": roofline(ops_noise + ops_craters, kernel_ms, cells * 4.0) | {
@@ -314,6 +342,7 @@ def bench_tiles(args, N, dev) -> dict:
     }
     if tiles.size == 1024 and (counted := counted_traffic(kernel_ms)) is not None:
         result["roofline"] |= counted
+    result["roofline"]["valu_ceiling_measured"] = valu_ceiling(N, dev)
     result |= extra
     batch.close()
     if args.serve_seconds > 0:
@@ -456,6 +485,8 @@ def main() -> int:
                     help="cells of the tiles workload timed on the host (CPU baseline)")
     ap.add_argument("--no-cpu-baseline", action="store_true")
     ap.add_argument("--no-plain", action="store_true", help="skip the plain per-tile path")
+    ap.add_argument("--no-autotune", action="store_true",
+                    help="launch with the default crater walk and workgroups per CU")
     ap.add_argument("--copy-calibration-mib", type=int, default=0,
                     help="also run one device copy of this size (for counter calibration)")
     args = ap.parse_args()

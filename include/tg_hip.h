@@ -192,6 +192,15 @@ int tg_hip_batch_stage(tg_batch* batch, const tg_tile_desc* tiles, size_t tile_c
 int tg_hip_batch_run(tg_batch* batch, const tg_noise_plan* plan, float elevation,
                      const tg_catalogue* catalogue, float* out_dev, size_t out_capacity,
                      int use_graph, char* err, size_t err_len);
+/* Launch choices of a batch (no counterpart in the reference), for the autotuner. The crater
+ * walk: TG_WALK_TOGETHER (the default), where a wave applies a listed crater whenever any of its
+ * cells is in reach, or TG_WALK_PER_LANE, where each cell walks the list to its own next crater in
+ * reach. Workgroups per CU of the persistent launch: 1-64 (default 8). `hoisted` (default 1) uses
+ * the per-crater constants of the MPD profile path where a crater allows it; 0 evaluates every
+ * crater in full. Every choice gives the same heights. */
+enum { TG_WALK_TOGETHER = 0, TG_WALK_PER_LANE = 1 };
+int tg_hip_batch_tune(tg_batch* batch, uint32_t crater_walk, uint32_t groups_per_cu,
+                      uint32_t hoisted, char* err, size_t err_len);
 int tg_hip_batch_wait(tg_batch* batch, char* err, size_t err_len);
 /* The batch's stream, for events and copies ordered with its runs. */
 void* tg_hip_batch_stream(tg_batch* batch);
@@ -219,6 +228,13 @@ int tg_hip_event_record(tg_dev* dev, void* event, void* stream, char* err, size_
 /* Milliseconds between two recorded events, waiting for the second. */
 int tg_hip_event_elapsed(tg_dev* dev, void* start, void* stop, float* ms, char* err,
                          size_t err_len);
+/* The vector-ALU issue ceiling, measured (no counterpart in the reference; a measurement aid):
+ * `blocks` workgroups of 256 lanes each run `iters` iterations of independent chains, kind 0 of
+ * eight FP32 FMAs, kind 1 of eight 32-bit xors and eight adds. `ms` is the timed launch (after a
+ * warm-up one), `lane_ops` the lane operations it issued. */
+int tg_hip_valu_probe(tg_dev* dev, int kind, uint32_t iters, uint32_t blocks, float* ms,
+                      double* lane_ops, char* err, size_t err_len);
+
 /* Pinned (page-locked) host memory, for copies that overlap device work. */
 int tg_hip_host_alloc(tg_dev* dev, size_t bytes, void** out, char* err, size_t err_len);
 int tg_hip_host_free(tg_dev* dev, void* ptr);

@@ -27,7 +27,9 @@ struct tg_catalogue {
     float* harm = nullptr;
     float4* outline = nullptr;
     double* diameters = nullptr;
+    tg::MpdTerms* terms = nullptr;
     size_t count = 0;
+    bool allFast = false;  // every crater takes the hoisted MPD path
 };
 
 struct tg_batch {
@@ -49,6 +51,12 @@ struct tg_batch {
     float* graphOut = nullptr;
     float graphElevation = 0.0f;
     uint32_t graphGroups = 0;
+    uint32_t graphWalk = 0;
+    bool graphHoisted = true;
+    // Launch choices (tg_hip_batch_tune).
+    uint32_t walk = 0;          // TG_WALK_TOGETHER
+    uint32_t groupsPerCu = 8;
+    bool hoisted = true;        // the hoisted MPD path where a crater allows it
 };
 
 namespace {
@@ -77,10 +85,23 @@ __device__ __forceinline__ float ResolutionAt(double diameter, double pixel) {
     return float(1.0 - falloff);
 }
 
+// The crater walks of a bin's list: how a wave visits the (cell, crater) pairs.
+enum CraterWalk : uint32_t {
+    // The wave steps through the list together; a crater is applied whenever any lane is in its
+    // reach (the lanes outside idle through it).
+    kWalkTogether = TG_WALK_TOGETHER,
+    // Each lane scans the list for its own next crater in reach and applies it; the wave repeats
+    // until no lane finds another. Every cell still applies its craters in list order.
+    kWalkPerLane = TG_WALK_PER_LANE,
+};
+
 // One bin per wave: lane 0 takes the next bin with one atomic add and the index is broadcast
-// with readfirstlane, so every branch below is uniform across the wave and no wave waits for
-// another (the kernel has no block barrier). The wave covers the bin as four 8 x 8 squares, the
-// shape the strip kernel uses, so its crater-reach tests stay coherent across lanes.
+// with readfirstlane, so the bin loop is uniform across the wave and no wave waits for another
+// (the kernel has no block barrier). The wave covers the bin as four 8 x 8 squares, the shape the
+// strip kernel uses, so its crater-reach tests stay coherent across lanes.
+// kAllFast: every crater of the catalogue takes the hoisted MPD path, so the full evaluation is
+// not compiled in; the kernel then holds far fewer registers and keeps more waves resident.
+template <uint32_t kWalk, bool kAllFast>
 __global__ void __launch_bounds__(kGroupThreads)
     BatchTiles(const tg_tile_desc* __restrict__ tiles, const tg_bin* __restrict__ bins,
                const uint32_t* __restrict__ entries, uint32_t* __restrict__ counters,
@@ -89,7 +110,7 @@ __global__ void __launch_bounds__(kGroupThreads)
                const tg_crater_record* __restrict__ records, const float* __restrict__ data,
                const float* __restrict__ cull, const float* __restrict__ harm,
                const float4* __restrict__ outline, const double* __restrict__ diameters,
-               float* __restrict__ out) {
+               const tg::MpdTerms* __restrict__ terms, float* __restrict__ out) {
     const uint32_t binCount = counters[1];
     const uint32_t lane = threadIdx.x & 63u;
     for (;;) {
@@ -122,15 +143,57 @@ __global__ void __launch_bounds__(kGroupThreads)
                     const float xLow = float(cx - double(x));
                     const float yLow = float(cy - double(y));
                     const tg::CraterShape shape(*kernel);
-                    for (uint32_t e = bin.first; e < bin.first + bin.count; ++e) {
-                        const uint32_t n = entries[e];
-                        const tg_crater_record& c = records[n];
-                        const float dx = tg::SplitDifference(x, xLow, c.x, c.x_low);
-                        const float dy = tg::SplitDifference(y, yLow, c.y, c.y_low);
-                        if (dx * dx + dy * dy < cull[n]) {
-                            const float weight = ResolutionAt(diameters[n], tile.pixel);
-                            h = tg::ApplyCrater(shape, data, c, h, dx, dy, harm, outline[n],
-                                                weight);
+                    const uint32_t end = bin.first + bin.count;
+                    if constexpr (kWalk == kWalkTogether) {
+                        for (uint32_t e = bin.first; e < end; ++e) {
+                            const uint32_t n = entries[e];
+                            const tg_crater_record& c = records[n];
+                            const float dx = tg::SplitDifference(x, xLow, c.x, c.x_low);
+                            const float dy = tg::SplitDifference(y, yLow, c.y, c.y_low);
+                            if (dx * dx + dy * dy < cull[n]) {
+                                const float weight = ResolutionAt(diameters[n], tile.pixel);
+                                if (kAllFast || (terms != nullptr && terms[n].fast != 0u)) {
+                                    h = tg::ApplyCraterMpd(c, terms[n], h, dx, dy, harm,
+                                                           outline[n], weight);
+                                } else if constexpr (!kAllFast) {
+                                    h = tg::ApplyCrater(shape, data, c, h, dx, dy, harm,
+                                                        outline[n], weight);
+                                }
+                            }
+                        }
+                    } else {
+                        uint32_t e = bin.first;
+                        for (;;) {
+                            uint32_t n = 0u;
+                            float dx = 0.0f, dy = 0.0f;
+                            bool hit = false;
+                            while (e < end) {
+                                const uint32_t m = entries[e];
+                                ++e;
+                                const tg_crater_record& c = records[m];
+                                const float ex = tg::SplitDifference(x, xLow, c.x, c.x_low);
+                                const float ey = tg::SplitDifference(y, yLow, c.y, c.y_low);
+                                if (ex * ex + ey * ey < cull[m]) {
+                                    n = m;
+                                    dx = ex;
+                                    dy = ey;
+                                    hit = true;
+                                    break;
+                                }
+                            }
+                            if (__ballot(hit) == 0ull) {
+                                break;  // no lane of the wave has a crater left
+                            }
+                            if (hit) {
+                                const float weight = ResolutionAt(diameters[n], tile.pixel);
+                                if (kAllFast || (terms != nullptr && terms[n].fast != 0u)) {
+                                    h = tg::ApplyCraterMpd(records[n], terms[n], h, dx, dy,
+                                                           harm, outline[n], weight);
+                                } else if constexpr (!kAllFast) {
+                                    h = tg::ApplyCrater(shape, data, records[n], h, dx, dy, harm,
+                                                        outline[n], weight);
+                                }
+                            }
                         }
                     }
                 }
@@ -168,7 +231,7 @@ void FreeCatalogue(tg_catalogue* c) {
     for (void* p : {static_cast<void*>(c->kernel), static_cast<void*>(c->records),
                     static_cast<void*>(c->data), static_cast<void*>(c->cull),
                     static_cast<void*>(c->harm), static_cast<void*>(c->outline),
-                    static_cast<void*>(c->diameters)}) {
+                    static_cast<void*>(c->diameters), static_cast<void*>(c->terms)}) {
         if (p != nullptr) {
             (void)hipFree(p);
         }
@@ -182,21 +245,29 @@ uint32_t GroupsFor(const tg_batch& b) {
     (void)hipDeviceGetAttribute(&cus, hipDeviceAttributeMultiprocessorCount, b.dev->ordinal);
     const size_t wavesPerGroup = kGroupThreads / 64;
     return uint32_t(std::max<size_t>(
-        1, std::min<size_t>((b.binCount + wavesPerGroup - 1) / wavesPerGroup, size_t(cus) * 8)));
+        1, std::min<size_t>((b.binCount + wavesPerGroup - 1) / wavesPerGroup,
+                            size_t(cus) * b.groupsPerCu)));
 }
 
 void Launch(tg_batch& b, const tg_noise_plan* plan, float elevation, const tg_catalogue* cat,
             float* out) {
     const uint32_t groups = GroupsFor(b);
     hipLaunchKernelGGL(ResetQueue, dim3(1), dim3(1), 0, b.stream, b.counters);
-    hipLaunchKernelGGL(BatchTiles, dim3(groups), dim3(kGroupThreads), 0, b.stream, b.tiles, b.bins,
+    // The compact kernel when every crater takes the hoisted path (or there are none).
+    const bool allFast = b.hoisted && (cat == nullptr || cat->allFast);
+    auto* fn = b.walk == TG_WALK_PER_LANE
+                   ? (allFast ? BatchTiles<kWalkPerLane, true> : BatchTiles<kWalkPerLane, false>)
+                   : (allFast ? BatchTiles<kWalkTogether, true>
+                              : BatchTiles<kWalkTogether, false>);
+    hipLaunchKernelGGL(fn, dim3(groups), dim3(kGroupThreads), 0, b.stream, b.tiles, b.bins,
                        b.entries, b.counters, plan != nullptr ? plan->device : nullptr,
                        elevation, cat != nullptr ? cat->kernel : nullptr,
                        cat != nullptr ? cat->records : nullptr,
                        cat != nullptr ? cat->data : nullptr, cat != nullptr ? cat->cull : nullptr,
                        cat != nullptr ? cat->harm : nullptr,
                        cat != nullptr ? cat->outline : nullptr,
-                       cat != nullptr ? cat->diameters : nullptr, out);
+                       cat != nullptr ? cat->diameters : nullptr,
+                       cat != nullptr && b.hoisted ? cat->terms : nullptr, out);
     Check(hipGetLastError(), "launching the batched tile kernel");
 }
 
@@ -245,6 +316,8 @@ int tg_hip_catalogue_create(tg_dev* dev, const tg_crater_shape* kernel,
         cat->harm = UploadArray(harm.data(), harm.size());
         cat->outline = UploadArray(outline.data(), outline.size());
         cat->diameters = UploadArray(diameters, count);
+        cat->terms = DeviceMpdTerms(cat->kernel, cat->records, count);
+        cat->allFast = AllMpdFast(*kernel, records, count);
     });
     if (code != TG_OK) {
         FreeCatalogue(cat.get());
@@ -413,7 +486,8 @@ int tg_hip_batch_run(tg_batch* b, const tg_noise_plan* plan, float elevation,
         // the bin count from device memory), so a restage of the same grid replays it.
         const uint32_t groups = GroupsFor(*b);
         if (b->graph == nullptr || b->graphPlan != plan || b->graphCatalogue != cat ||
-            b->graphOut != out || b->graphElevation != elevation || b->graphGroups != groups) {
+            b->graphOut != out || b->graphElevation != elevation || b->graphGroups != groups ||
+            b->graphWalk != b->walk || b->graphHoisted != b->hoisted) {
             DropGraph(*b);
             hipGraph_t graph = nullptr;
             Check(hipStreamBeginCapture(b->stream, hipStreamCaptureModeThreadLocal),
@@ -436,9 +510,27 @@ int tg_hip_batch_run(tg_batch* b, const tg_noise_plan* plan, float elevation,
             b->graphOut = out;
             b->graphElevation = elevation;
             b->graphGroups = groups;
+            b->graphWalk = b->walk;
+            b->graphHoisted = b->hoisted;
         }
         Check(hipGraphLaunch(b->graph, b->stream), "hipGraphLaunch");
     });
+}
+
+int tg_hip_batch_tune(tg_batch* b, uint32_t crater_walk, uint32_t groups_per_cu,
+                      uint32_t hoisted, char* err, size_t errLen) {
+    if (b == nullptr) {
+        SetErr(err, errLen, "tg_hip_batch_tune: null batch");
+        return TG_ERR_ARGUMENT;
+    }
+    if (crater_walk > TG_WALK_PER_LANE || groups_per_cu == 0 || groups_per_cu > 64) {
+        SetErr(err, errLen, "tg_hip_batch_tune: unknown crater walk or groups per CU not in 1-64");
+        return TG_ERR_ARGUMENT;
+    }
+    b->walk = crater_walk;
+    b->groupsPerCu = groups_per_cu;
+    b->hoisted = hoisted != 0u;
+    return TG_OK;
 }
 
 int tg_hip_batch_wait(tg_batch* b, char* err, size_t errLen) {

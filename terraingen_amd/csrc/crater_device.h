@@ -1084,4 +1084,114 @@ __device__ __forceinline__ float ApplyCrater(const CraterShape& shape, const flo
     return height + weight * shaped + weight * replaced;
 }
 
+// ---- The hoisted path for the Martin-Parkes-Dunstan profile ------------------------------------
+//
+// When only the simple MPD profile and the outline shape a crater (no diffusion, complex form, rim
+// rounding, crater noise or texture in the kernel block; no datum, marks, terraces, peaks, floor
+// shift, upslope or outline table in the record), most of ApplyCrater's work depends on the crater
+// alone. MpdTerms holds those values, computed once per crater by the same FP32 operations in the
+// same order as MpdHeight (craters.h:388-416), MartinParkesDunstanExterior (craters.h:374-386) and
+// KernelProfileDistance (craters.h:1034-1054) perform them, so ApplyCraterMpd gives the same bits
+// as ApplyCrater for such a crater.
+
+struct MpdTerms {
+    float radius;     // 0.5 D
+    float reach;      // reach radii x radius
+    float worn;       // KernelWornOutlineShare
+    float freshRim;   // fresh rim x D
+    float rim;        // current rim x D
+    float k;          // freshRim - rim + depth
+    float k2;         // 2 k
+    float alpha;      // (rim - freshRim) / k
+    float interior;   // k (alpha + 1) / alpha
+    float beta;       // min(1, 1.5 (freshRim - rim) / k)
+    float ejecta;     // the ejecta scale
+    float joinRise;   // the exterior law at beta, less the rim
+    float betaSlope;  // beta x the joint's slope
+    float offset;     // rim - ejecta
+    float gradient;   // -ejecta / 8 - offset
+    uint32_t fast;    // 1 when the hoisted path applies to this crater
+};
+static_assert(sizeof(MpdTerms) == 64, "MpdTerms is four float4");
+
+// The kernel block's part of the condition.
+__host__ __device__ inline bool MpdShapeOnly(const tg_crater_shape& k) {
+    return k.profile.simple == TG_PROFILE_MPD && k.profile.diffused == 0u &&
+           k.profile.complex_ == 0u && k.rim_sigma == 0.0f && k.noise_amplitude == 0.0f &&
+           k.noise_rim_amplitude == 0.0f && k.wall_streaks == 0.0f && k.ejecta == 0.0f;
+}
+
+__device__ inline MpdTerms MakeMpdTerms(const tg_crater_shape& k, const tg_crater_record& c) {
+    MpdTerms t{};
+    const float diameter = c.diameter;
+    const float depth = c.current_depth * diameter;
+    t.rim = c.current_rim * diameter;
+    t.freshRim = c.fresh_rim * diameter;
+    t.radius = 0.5f * diameter;
+    t.reach = k.reach_radii * t.radius;
+    t.worn = WornOutlineShare(c);
+    t.k = t.freshRim - t.rim + depth;
+    t.k2 = 2.0f * t.k;
+    t.alpha = (t.rim - t.freshRim) / t.k;
+    t.interior = t.k * (t.alpha + 1.0f) / t.alpha;
+    t.ejecta = t.freshRim > 0.0f ? EjectaScale(t.radius) * t.rim / t.freshRim
+                                 : EjectaScale(t.radius);
+    t.beta = StdMin(1.0f, 1.5f * (t.freshRim - t.rim) / t.k);
+    const ProfilePoint join = MpdExterior(t.beta, t.rim, t.ejecta);
+    const float drop = t.rim - join.height;
+    const float slope = drop > 0.0f ? StdMax(join.slope, -3.0f * drop / t.beta) : join.slope;
+    t.joinRise = join.height - t.rim;
+    t.betaSlope = t.beta * slope;
+    t.offset = t.rim - t.ejecta;
+    t.gradient = -t.ejecta / 8.0f - t.offset;
+    t.fast = (MpdShapeOnly(k) && c.has_datum != 1u && c.mark_count == 0u &&
+              c.terrace_count == 0u && c.peak_count == 0u && c.shift_x == 0.0f &&
+              c.shift_y == 0.0f && c.upslope_x == 0.0f && c.upslope_y == 0.0f &&
+              c.table_count == 0u)
+                 ? 1u
+                 : 0u;
+    return t;
+}
+
+// MpdHeight at `distance` from the hoisted terms.
+__device__ __forceinline__ float MpdHeightFrom(const MpdTerms& t, float distance) {
+    const float x = distance / t.radius - 1.0f;
+    if (x >= 1.0f) {
+        return 0.0f;
+    }
+    if (x <= t.alpha) {
+        return (t.k * x + t.k2) * x + t.freshRim;
+    }
+    if (x <= 0.0f) {
+        return t.interior * x * x + t.rim;
+    }
+    if (x <= t.beta) {
+        const float u = x / t.beta;
+        return t.rim + t.joinRise * u * u * (3.0f - 2.0f * u) + t.betaSlope * u * u * (u - 1.0f);
+    }
+    const float r = 1.0f / (x + 1.0f);
+    const float law = t.ejecta * r * r * r + t.offset + t.gradient * x;
+    const float taper = 1.0f - x * x;
+    return law * taper;
+}
+
+// ApplyCrater for a crater whose terms say `fast`, with the outline's harmonic terms in `harm`
+// and `ra` as FastOutlineScale takes them, and the resolution weight for the grid.
+__device__ __forceinline__ float ApplyCraterMpd(const tg_crater_record& c, const MpdTerms& t,
+                                                float height, float dx, float dy,
+                                                const float* harm, float4 ra, float weight) {
+    float distance = sqrtf(dx * dx + dy * dy);
+    if (c.ellipticity != 1.0f || c.harmonic_count > 0u) {
+        const float share = Smoothstep(distance / t.radius) * t.worn;
+        distance /= FastOutlineScale(harm, ra, c, dx, dy, distance, share);
+    }
+    if (distance >= t.reach) {
+        return height;
+    }
+    // The same sums as ApplyCrater's, whose other terms are zero here.
+    const float bare = MpdHeightFrom(t, distance) + 0.0f;
+    const float shaped = bare + 0.0f + 0.0f;
+    return height + weight * shaped + weight * 0.0f;
+}
+
 }  // namespace tg

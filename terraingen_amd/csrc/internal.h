@@ -17,6 +17,10 @@
 #include <string>
 #include <vector>
 
+namespace tg {
+struct MpdTerms;
+}
+
 struct tg_dev {
     int ordinal = 0;
     std::string name;
@@ -36,6 +40,8 @@ struct tg_crater_set {
     float* cull = nullptr;  // per record: squared round distance (m^2) beyond which it is inert
     float* harm = nullptr;  // A cos(phi), A sin(phi) in the data's harmonic slots
     float4* outline = nullptr;  // per record: cos and sin of the heading, sqrt(ellipticity), 1/that
+    tg::MpdTerms* terms = nullptr;  // per record: the hoisted MPD path's constants
+    bool allFast = false;           // every record takes the hoisted path
     size_t pointCount = 0;
     size_t lowCount = 0;
     uint32_t tilesAcross = 0;
@@ -114,6 +120,15 @@ std::vector<float> CullRadii(const tg_crater_shape& k, const tg_crater_record* r
                              const float* data, size_t dataCount);
 void OutlineTerms(const tg_crater_record* records, size_t n, const float* data, size_t dataCount,
                   std::vector<float>& harm, std::vector<float4>& outline);
+
+// Per-record MpdTerms of records already on the device, computed there (tg_hip.hip) into a new
+// device array of n entries (at least one).
+tg::MpdTerms* DeviceMpdTerms(const tg_crater_shape* kernelOnDevice,
+                             const tg_crater_record* recordsOnDevice, size_t n);
+
+// True when every record takes the hoisted MPD path (MakeMpdTerms's `fast`), decided on the host
+// by the same conditions (tg_hip.hip).
+bool AllMpdFast(const tg_crater_shape& k, const tg_crater_record* records, size_t n);
 
 // Throws HipError unless every index a crater batch holds stays inside its arrays (tg_hip.hip).
 void ValidateCraterBatch(const tg_crater_shape& k, const tg_crater_record* records, size_t n,

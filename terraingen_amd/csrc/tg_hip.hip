@@ -86,6 +86,8 @@ void LaunchNoise(const tg::NoisePlan* plan, float* heights, const float* xs, con
 // craters.slang's main(): each cell runs the crater list of its tile.
 constexpr uint32_t kCraterTile = 16;
 
+// kAllFast: every crater of the set takes the hoisted MPD path (see BatchTiles).
+template <bool kAllFast>
 __global__ void __launch_bounds__(kCraterTile * kCraterTile)
     CraterStrip(float* __restrict__ heights, const float* __restrict__ xs,
                 const float* __restrict__ ys, const tg_crater_shape* __restrict__ kernel,
@@ -94,7 +96,8 @@ __global__ void __launch_bounds__(kCraterTile * kCraterTile)
                 const tg_crater_frame* __restrict__ frames,
                 const tg_split_point* __restrict__ points, const float* __restrict__ lows,
                 const float* __restrict__ cull, const float* __restrict__ harm,
-                const float4* __restrict__ outline, tg_kernel_params params) {
+                const float4* __restrict__ outline, const tg::MpdTerms* __restrict__ terms,
+                tg_kernel_params params) {
     // Each 64-lane wave covers an 8 x 8 square of the 16 x 16 tile, so a disc-shaped reach
     // leaves fewer waves partly active than rows of the tile would.
     const uint32_t t = threadIdx.x;
@@ -133,9 +136,13 @@ __global__ void __launch_bounds__(kCraterTile * kCraterTile)
             if (dx * dx + dy * dy >= cull[n]) {
                 continue;  // beyond any reach its outline allows: ApplyCrater returns h as is
             }
-            h = tg::ApplyCrater(shape, data, c, h, dx, dy, harm, outline[n]);
+            if (kAllFast || terms[n].fast != 0u) {
+                h = tg::ApplyCraterMpd(c, terms[n], h, dx, dy, harm, outline[n], c.resolution);
+            } else if constexpr (!kAllFast) {
+                h = tg::ApplyCrater(shape, data, c, h, dx, dy, harm, outline[n]);
+            }
         }
-    } else {
+    } else if constexpr (!kAllFast) {
         const tg_split_point p = points[i];
         for (uint32_t e = first; e < first + count; ++e) {
             const uint32_t n = entries[e];
@@ -156,9 +163,11 @@ void LaunchCraters(const tg_crater_set& set, float* heights, const float* xs, co
     const dim3 block(kCraterTile * kCraterTile, 1, 1);
     const dim3 grid((params.width + kCraterTile - 1) / kCraterTile,
                     (params.rows + kCraterTile - 1) / kCraterTile, 1);
-    hipLaunchKernelGGL(CraterStrip, grid, block, 0, stream, heights, xs, ys, set.kernel,
+    // The compact kernel for a plane set whose craters all take the hoisted path.
+    auto* fn = set.allFast && !set.onBody ? CraterStrip<true> : CraterStrip<false>;
+    hipLaunchKernelGGL(fn, grid, block, 0, stream, heights, xs, ys, set.kernel,
                        set.records, set.data, set.tiles, set.entries, set.frames, set.points,
-                       set.lows, set.cull, set.harm, set.outline, params);
+                       set.lows, set.cull, set.harm, set.outline, set.terms, params);
     Check(hipGetLastError(), "launching the crater kernel");
 }
 
@@ -413,6 +422,9 @@ int CreateCraterSet(tg_dev* dev, const tg_crater_shape* kernel, const tg_crater_
         OutlineTerms(records, recordCount, data, dataCount, harm, outline);
         set->harm = Upload(harm.data(), harm.size(), owned);
         set->outline = Upload(outline.data(), outline.size(), owned);
+        set->terms = DeviceMpdTerms(set->kernel, set->records, recordCount);
+        owned.push_back(set->terms);
+        set->allFast = AllMpdFast(*kernel, records, recordCount);
         set->tileSide = tiles[1];
         set->tilesAcross = tiles[0];
         set->tileRows = uint32_t((tileWords - 3) / 2 / tiles[0]);
@@ -470,7 +482,7 @@ int tg_hip_crater_set_free(tg_crater_set* set) {
                     static_cast<void*>(set->entries), static_cast<void*>(set->frames),
                     static_cast<void*>(set->points), static_cast<void*>(set->lows),
                     static_cast<void*>(set->cull), static_cast<void*>(set->harm),
-                    static_cast<void*>(set->outline)}) {
+                    static_cast<void*>(set->outline), static_cast<void*>(set->terms)}) {
         (void)hipFree(p);
     }
     delete set;
@@ -738,7 +750,60 @@ int tg_hip_event_query(tg_dev* dev, void* event, char* err, size_t errLen) {
 
 }  // extern "C"
 
+namespace {
+
+__global__ void MakeMpdTermsKernel(const tg_crater_shape* __restrict__ kernel,
+                                   const tg_crater_record* __restrict__ records, uint32_t n,
+                                   tg::MpdTerms* __restrict__ terms) {
+    const uint32_t i = blockIdx.x * blockDim.x + threadIdx.x;
+    if (i < n) {
+        terms[i] = tg::MakeMpdTerms(*kernel, records[i]);
+    }
+}
+
+}  // namespace
+
 namespace tgi {
+
+tg::MpdTerms* DeviceMpdTerms(const tg_crater_shape* kernelOnDevice,
+                             const tg_crater_record* recordsOnDevice, size_t n) {
+    if (n > 0xFFFFFFFFull) {
+        throw HipError{"too many craters for the terms kernel"};
+    }
+    void* p = nullptr;
+    Check(hipMalloc(&p, (n > 0 ? n : 1) * sizeof(tg::MpdTerms)), "hipMalloc");
+    if (n > 0) {
+        const uint32_t threads = 256;
+        const uint32_t blocks = uint32_t((n + threads - 1) / threads);
+        hipLaunchKernelGGL(MakeMpdTermsKernel, dim3(blocks), dim3(threads), 0, nullptr,
+                           kernelOnDevice, recordsOnDevice, uint32_t(n),
+                           static_cast<tg::MpdTerms*>(p));
+        hipError_t status = hipGetLastError();
+        if (status == hipSuccess) {
+            status = hipDeviceSynchronize();
+        }
+        if (status != hipSuccess) {
+            (void)hipFree(p);
+            Check(status, "computing the crater terms");
+        }
+    }
+    return static_cast<tg::MpdTerms*>(p);
+}
+
+bool AllMpdFast(const tg_crater_shape& k, const tg_crater_record* records, size_t n) {
+    if (!tg::MpdShapeOnly(k)) {
+        return n == 0;
+    }
+    for (size_t i = 0; i < n; ++i) {
+        const tg_crater_record& c = records[i];
+        if (c.has_datum == 1u || c.mark_count != 0u || c.terrace_count != 0u ||
+            c.peak_count != 0u || c.shift_x != 0.0f || c.shift_y != 0.0f ||
+            c.upslope_x != 0.0f || c.upslope_y != 0.0f || c.table_count != 0u) {
+            return false;
+        }
+    }
+    return true;
+}
 
 // The squared round distance from a crater's centre beyond which ApplyCrater leaves every height
 // unchanged (craters.h:1186-1193): reach x radius x an upper bound of its outline scale

@@ -248,6 +248,33 @@ class TileBatch:
     def stream(self):
         return self.dev.lib.tg_hip_batch_stream(self.batch)
 
+    def tune(self, walk: int = N.TG_WALK_TOGETHER, groups_per_cu: int = 8,
+             hoisted: bool = True) -> None:
+        """The staged batch's launch choices (tg_hip_batch_tune)."""
+        self.dev.call("tg_hip_batch_tune", self.batch, walk, groups_per_cu, int(hoisted))
+
+    def autotune(self, out: N.DeviceArray, walks=(N.TG_WALK_TOGETHER, N.TG_WALK_PER_LANE),
+                 groups=(4, 8, 16), repeats: int = 2) -> dict:
+        """Times every (crater walk, workgroups per CU) on the staged batch with HIP events on
+        its stream, keeps the fastest and returns {(walk, groups): ms}. The walks give the same
+        heights, so the choice changes only the time."""
+        timings = {}
+        for walk in walks:
+            for g in groups:
+                self.tune(walk, g)
+                self.run(out, use_graph=False)  # warm: code objects, caches
+                a, b = N.Event(self.dev), N.Event(self.dev)
+                a.record(self.stream)
+                for _ in range(repeats):
+                    self.run(out, use_graph=False)
+                b.record(self.stream)
+                self.wait()
+                timings[(walk, g)] = a.elapsed_ms(b) / repeats
+        best = min(timings, key=timings.get)
+        self.tune(*best)
+        self.tuned = best
+        return timings
+
     def run(self, out: N.DeviceArray, use_graph: bool = True) -> None:
         self.dev.call("tg_hip_batch_run", self.batch, self.plan, self.elevation, self.catalogue,
                       out.ptr, out.nbytes // 4, 1 if use_graph else 0)

@@ -91,3 +91,22 @@ def test_stage_rejects_malformed_batches(hip, batch):
     hip.call("tg_hip_batch_stage", batch.batch, batch.descs.ctypes.data, batch.descs.size,
              batch.bins.ctypes.data, batch.bins.size, batch.entries.ctypes.data,
              batch.entries.size)
+
+
+def test_crater_walks_agree(hip, batch):
+    """Walking the bin lists together or per lane applies the same craters in the same order."""
+    batch.tune(N.TG_WALK_TOGETHER, 8)
+    together = run_batch(hip, batch, use_graph=False)
+    batch.tune(N.TG_WALK_PER_LANE, 4)
+    per_lane = run_batch(hip, batch, use_graph=True)
+    assert np.array_equal(together.view(np.uint32), per_lane.view(np.uint32))
+    batch.tune()
+
+
+def test_hoisted_mpd_path_is_bit_identical(hip, batch):
+    """The per-crater constants of the MPD profile path give the bits of the full evaluation."""
+    batch.tune(hoisted=False)
+    full = run_batch(hip, batch, use_graph=False)
+    batch.tune(hoisted=True)
+    hoisted = run_batch(hip, batch, use_graph=False)
+    assert np.array_equal(full.view(np.uint32), hoisted.view(np.uint32))
