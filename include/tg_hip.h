@@ -197,10 +197,11 @@ int tg_hip_batch_run(tg_batch* batch, const tg_noise_plan* plan, float elevation
  * cells is in reach, or TG_WALK_PER_LANE, where each cell walks the list to its own next crater in
  * reach. Workgroups per CU of the persistent launch: 1-64 (default 8). `hoisted` (default 1) uses
  * the per-crater constants of the MPD profile path where a crater allows it; 0 evaluates every
- * crater in full. Every choice gives the same heights. */
+ * crater in full. `chunk`: bins a wave takes from its queue per atomic add, 1-256 (default 8).
+ * Every choice gives the same heights. */
 enum { TG_WALK_TOGETHER = 0, TG_WALK_PER_LANE = 1 };
 int tg_hip_batch_tune(tg_batch* batch, uint32_t crater_walk, uint32_t groups_per_cu,
-                      uint32_t hoisted, char* err, size_t err_len);
+                      uint32_t hoisted, uint32_t chunk, char* err, size_t err_len);
 int tg_hip_batch_wait(tg_batch* batch, char* err, size_t err_len);
 /* The batch's stream, for events and copies ordered with its runs. */
 void* tg_hip_batch_stream(tg_batch* batch);

@@ -38,7 +38,7 @@ struct tg_batch {
     tg_tile_desc* tiles = nullptr;
     tg_bin* bins = nullptr;
     uint32_t* entries = nullptr;
-    uint32_t* counters = nullptr;  // [0] the queue's next bin, [1] the bin count
+    uint32_t* counters = nullptr;  // the queues' next offsets, then the bin count (kCountSlot)
     size_t maxTiles = 0, maxBins = 0, maxEntries = 0;
     // What the staged batch needs of a run: the largest entry and output end.
     size_t tileCount = 0, binCount = 0;
@@ -53,10 +53,12 @@ struct tg_batch {
     uint32_t graphGroups = 0;
     uint32_t graphWalk = 0;
     bool graphHoisted = true;
+    uint32_t graphChunk = 0;
     // Launch choices (tg_hip_batch_tune).
     uint32_t walk = 0;          // TG_WALK_TOGETHER
     uint32_t groupsPerCu = 8;
     bool hoisted = true;        // the hoisted MPD path where a crater allows it
+    uint32_t chunk = 8;         // bins a wave takes per atomic add
 };
 
 namespace {
@@ -64,8 +66,20 @@ namespace {
 constexpr uint32_t kBinSide = 16;
 constexpr uint32_t kGroupThreads = 256;  // four independent waves
 
+// The bins are split into kQueues contiguous ranges, one per XCD: workgroups are dispatched to the
+// XCDs in turn, so workgroup g starts on range g % kQueues, and neighbouring bins (which list the
+// same craters) are shaped by waves sharing one L2. Each range has its own counter in its own
+// 128-byte line; a wave takes `chunk` bins per atomic add, and when its range runs dry it moves to
+// the next one, so every bin is taken once and the device stays busy to the end.
+constexpr uint32_t kQueues = 8;
+constexpr uint32_t kQueueStride = 32;
+constexpr uint32_t kCountSlot = kQueues * kQueueStride;
+constexpr uint32_t kCounterWords = kCountSlot + 8;
+
 __global__ void ResetQueue(uint32_t* counters) {
-    counters[0] = 0u;
+    if (threadIdx.x < kQueues) {
+        counters[threadIdx.x * kQueueStride] = 0u;
+    }
 }
 
 // ResolutionWeight (src/layers/fade.cpp:48-52) with the smoothstep fade from 2 to 4 samples
@@ -110,18 +124,28 @@ __global__ void __launch_bounds__(kGroupThreads)
                const tg_crater_record* __restrict__ records, const float* __restrict__ data,
                const float* __restrict__ cull, const float* __restrict__ harm,
                const float4* __restrict__ outline, const double* __restrict__ diameters,
-               const tg::MpdTerms* __restrict__ terms, float* __restrict__ out) {
-    const uint32_t binCount = counters[1];
+               const tg::MpdTerms* __restrict__ terms, uint32_t chunk, float* __restrict__ out) {
+    const uint32_t binCount = counters[kCountSlot];
     const uint32_t lane = threadIdx.x & 63u;
+    uint32_t queue = blockIdx.x % kQueues;
+    uint32_t drained = 0u;  // ranges found empty; a range never refills
     for (;;) {
+        const uint32_t lo = uint32_t(uint64_t(binCount) * queue / kQueues);
+        const uint32_t hi = uint32_t(uint64_t(binCount) * (queue + 1u) / kQueues);
         uint32_t taken = 0u;
         if (lane == 0u) {
-            taken = atomicAdd(&counters[0], 1u);
+            taken = atomicAdd(&counters[queue * kQueueStride], chunk);
         }
-        const uint32_t b = __builtin_amdgcn_readfirstlane(taken);
-        if (b >= binCount) {
-            return;  // the queue only grows past binCount, so every wave ends here
+        const uint64_t first = uint64_t(lo) + __builtin_amdgcn_readfirstlane(taken);
+        if (first >= hi) {
+            if (++drained == kQueues) {
+                return;  // every range is empty, so every wave ends here
+            }
+            queue = (queue + 1u) % kQueues;
+            continue;
         }
+        const uint32_t last = uint32_t(std::min<uint64_t>(first + chunk, hi));
+        for (uint32_t b = uint32_t(first); b < last; ++b) {
         const tg_bin bin = bins[b];
         const tg_tile_desc tile = tiles[bin.tile];
         const uint32_t local = b - tile.bin_first;
@@ -200,6 +224,7 @@ __global__ void __launch_bounds__(kGroupThreads)
                 out[tile.out_offset + size_t(row) * tile.width + col] = h;
             }
         }
+        }
     }
 }
 
@@ -252,7 +277,7 @@ uint32_t GroupsFor(const tg_batch& b) {
 void Launch(tg_batch& b, const tg_noise_plan* plan, float elevation, const tg_catalogue* cat,
             float* out) {
     const uint32_t groups = GroupsFor(b);
-    hipLaunchKernelGGL(ResetQueue, dim3(1), dim3(1), 0, b.stream, b.counters);
+    hipLaunchKernelGGL(ResetQueue, dim3(1), dim3(64), 0, b.stream, b.counters);
     // The compact kernel when every crater takes the hoisted path (or there are none).
     const bool allFast = b.hoisted && (cat == nullptr || cat->allFast);
     auto* fn = b.walk == TG_WALK_PER_LANE
@@ -267,7 +292,7 @@ void Launch(tg_batch& b, const tg_noise_plan* plan, float elevation, const tg_ca
                        cat != nullptr ? cat->harm : nullptr,
                        cat != nullptr ? cat->outline : nullptr,
                        cat != nullptr ? cat->diameters : nullptr,
-                       cat != nullptr && b.hoisted ? cat->terms : nullptr, out);
+                       cat != nullptr && b.hoisted ? cat->terms : nullptr, b.chunk, out);
     Check(hipGetLastError(), "launching the batched tile kernel");
 }
 
@@ -356,7 +381,7 @@ int tg_hip_batch_create(tg_dev* dev, size_t maxTiles, size_t maxBins, size_t max
         b->tiles = Alloc<tg_tile_desc>(maxTiles);
         b->bins = Alloc<tg_bin>(maxBins);
         b->entries = Alloc<uint32_t>(maxEntries);
-        b->counters = UploadArray<uint32_t>(nullptr, 2);
+        b->counters = UploadArray<uint32_t>(nullptr, kCounterWords);
         b->maxTiles = maxTiles;
         b->maxBins = maxBins;
         b->maxEntries = maxEntries;
@@ -449,7 +474,8 @@ int tg_hip_batch_stage(tg_batch* b, const tg_tile_desc* tiles, size_t tileCount,
             Check(hipMemcpy(b->entries, entries, entryCount * sizeof(uint32_t),
                             hipMemcpyHostToDevice), "staging entries");
         }
-        const uint32_t counts[2] = {0u, uint32_t(binCount)};
+        uint32_t counts[kCounterWords] = {};
+        counts[kCountSlot] = uint32_t(binCount);
         Check(hipMemcpy(b->counters, counts, sizeof counts, hipMemcpyHostToDevice),
               "staging the bin count");
         b->tileCount = tileCount;
@@ -487,7 +513,8 @@ int tg_hip_batch_run(tg_batch* b, const tg_noise_plan* plan, float elevation,
         const uint32_t groups = GroupsFor(*b);
         if (b->graph == nullptr || b->graphPlan != plan || b->graphCatalogue != cat ||
             b->graphOut != out || b->graphElevation != elevation || b->graphGroups != groups ||
-            b->graphWalk != b->walk || b->graphHoisted != b->hoisted) {
+            b->graphWalk != b->walk || b->graphHoisted != b->hoisted ||
+            b->graphChunk != b->chunk) {
             DropGraph(*b);
             hipGraph_t graph = nullptr;
             Check(hipStreamBeginCapture(b->stream, hipStreamCaptureModeThreadLocal),
@@ -512,24 +539,29 @@ int tg_hip_batch_run(tg_batch* b, const tg_noise_plan* plan, float elevation,
             b->graphGroups = groups;
             b->graphWalk = b->walk;
             b->graphHoisted = b->hoisted;
+            b->graphChunk = b->chunk;
         }
         Check(hipGraphLaunch(b->graph, b->stream), "hipGraphLaunch");
     });
 }
 
 int tg_hip_batch_tune(tg_batch* b, uint32_t crater_walk, uint32_t groups_per_cu,
-                      uint32_t hoisted, char* err, size_t errLen) {
+                      uint32_t hoisted, uint32_t chunk, char* err, size_t errLen) {
     if (b == nullptr) {
         SetErr(err, errLen, "tg_hip_batch_tune: null batch");
         return TG_ERR_ARGUMENT;
     }
-    if (crater_walk > TG_WALK_PER_LANE || groups_per_cu == 0 || groups_per_cu > 64) {
-        SetErr(err, errLen, "tg_hip_batch_tune: unknown crater walk or groups per CU not in 1-64");
+    if (crater_walk > TG_WALK_PER_LANE || groups_per_cu == 0 || groups_per_cu > 64 ||
+        chunk == 0 || chunk > 256) {
+        SetErr(err, errLen,
+               "tg_hip_batch_tune: unknown crater walk, groups per CU not in 1-64 or chunk not "
+               "in 1-256");
         return TG_ERR_ARGUMENT;
     }
     b->walk = crater_walk;
     b->groupsPerCu = groups_per_cu;
     b->hoisted = hoisted != 0u;
+    b->chunk = chunk;
     return TG_OK;
 }
 

@@ -132,7 +132,7 @@ HIP_ABI = {
     "tg_hip_memcpy_h2d_async": (C.c_int, [VP, VP, VP, SZ, VP, CP, SZ]),
     "tg_hip_event_query": (C.c_int, [VP, VP, CP, SZ]),
     "tg_hip_memcpy_d2d_async": (C.c_int, [VP, VP, VP, SZ, VP, CP, SZ]),
-    "tg_hip_batch_tune": (C.c_int, [VP, u32, u32, u32, CP, SZ]),
+    "tg_hip_batch_tune": (C.c_int, [VP, u32, u32, u32, u32, CP, SZ]),
     "tg_hip_valu_probe": (C.c_int, [VP, C.c_int, u32, u32, C.POINTER(C.c_float),
                                     C.POINTER(C.c_double), CP, SZ]),
 }

@@ -249,29 +249,30 @@ class TileBatch:
         return self.dev.lib.tg_hip_batch_stream(self.batch)
 
     def tune(self, walk: int = N.TG_WALK_TOGETHER, groups_per_cu: int = 8,
-             hoisted: bool = True) -> None:
+             hoisted: bool = True, chunk: int = 8) -> None:
         """The staged batch's launch choices (tg_hip_batch_tune)."""
-        self.dev.call("tg_hip_batch_tune", self.batch, walk, groups_per_cu, int(hoisted))
+        self.dev.call("tg_hip_batch_tune", self.batch, walk, groups_per_cu, int(hoisted), chunk)
 
     def autotune(self, out: N.DeviceArray, walks=(N.TG_WALK_TOGETHER, N.TG_WALK_PER_LANE),
-                 groups=(4, 8, 16), repeats: int = 2) -> dict:
-        """Times every (crater walk, workgroups per CU) on the staged batch with HIP events on
-        its stream, keeps the fastest and returns {(walk, groups): ms}. The walks give the same
-        heights, so the choice changes only the time."""
+                 groups=(4, 8), chunks=(2, 8, 32), repeats: int = 2) -> dict:
+        """Times every (crater walk, workgroups per CU, bins per queue take) on the staged batch
+        with HIP events on its stream, keeps the fastest and returns {(walk, groups, chunk): ms}.
+        Every choice gives the same heights, so it changes only the time."""
         timings = {}
         for walk in walks:
             for g in groups:
-                self.tune(walk, g)
-                self.run(out, use_graph=False)  # warm: code objects, caches
-                a, b = N.Event(self.dev), N.Event(self.dev)
-                a.record(self.stream)
-                for _ in range(repeats):
-                    self.run(out, use_graph=False)
-                b.record(self.stream)
-                self.wait()
-                timings[(walk, g)] = a.elapsed_ms(b) / repeats
+                for chunk in chunks:
+                    self.tune(walk, g, True, chunk)
+                    self.run(out, use_graph=False)  # warm: code objects, caches
+                    a, b = N.Event(self.dev), N.Event(self.dev)
+                    a.record(self.stream)
+                    for _ in range(repeats):
+                        self.run(out, use_graph=False)
+                    b.record(self.stream)
+                    self.wait()
+                    timings[(walk, g, chunk)] = a.elapsed_ms(b) / repeats
         best = min(timings, key=timings.get)
-        self.tune(*best)
+        self.tune(best[0], best[1], True, best[2])
         self.tuned = best
         return timings
 

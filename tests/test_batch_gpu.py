@@ -110,3 +110,21 @@ def test_hoisted_mpd_path_is_bit_identical(hip, batch):
     batch.tune(hoisted=True)
     hoisted = run_batch(hip, batch, use_graph=False)
     assert np.array_equal(full.view(np.uint32), hoisted.view(np.uint32))
+
+
+@pytest.mark.parametrize("chunk,groups", [(1, 1), (3, 8), (256, 2)])
+def test_queue_chunks_cover_every_bin_once(hip, batch, chunk, groups):
+    """However the bins are taken (one per atomic or many, few workgroups or many), every cell of
+    every tile is written once with the same heights."""
+    batch.tune(N.TG_WALK_TOGETHER, 8, True, 8)
+    want = run_batch(hip, batch, use_graph=False)
+    batch.tune(N.TG_WALK_TOGETHER, groups, True, chunk)
+    out = N.DeviceArray(hip, batch.cells * 4)
+    poison = np.full(batch.cells, np.nan, np.float32)
+    out.upload(poison)
+    batch.run(out, use_graph=False)
+    batch.wait()
+    got = out.download(np.empty(batch.cells, np.float32))
+    batch.tune()
+    assert np.isfinite(got).all()
+    assert np.array_equal(got.view(np.uint32), want.view(np.uint32))
