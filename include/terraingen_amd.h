@@ -226,6 +226,36 @@ int tgx_server_stats_get(tgx_server* server, tgx_server_stats* out);
 /* Finishes every submitted request, then frees the server. No thread may be waiting on it. */
 int tgx_server_free(tgx_server* server);
 
+/* ---- The backend registry and RunKernel, with the hip backend ----------------------------------
+ * The reference's include/terraingen/gpu/backend.h and src/gpu/kernel.h for hosts that are not
+ * C++: backends are numbered as the reference's enum with Hip appended (0 auto, 1 cpu, 2 vulkan,
+ * 3 gpu, 4 cuda, 5 metal, 6 hip). The C++ forms are in the host library's backend.h. */
+
+/* BackendNamed (backend.cpp:103-111): ignores case; TG_ERR_ARGUMENT for an unknown name. */
+int tgx_backend_named(const char* name, int* backend);
+/* BackendName: writes the name ("unknown" out of range) and returns its length. */
+size_t tgx_backend_name(int backend, char* out, size_t len);
+/* What `terraingen backends` prints (app/cli.cpp:192-205): one line per Backends() entry, CPU then
+ * Vulkan, CUDA, Metal and hip, with the device or the reason, then the default. Returns the
+ * length of the full text. */
+size_t tgx_backends_report(char* out, size_t len);
+/* DefaultBackend / SetDefaultBackend (backend.cpp:136-153): TG_ERR_ARGUMENT for an unknown
+ * $TERRAINGEN_BACKEND. */
+int tgx_default_backend(int* backend, char* err, size_t err_len);
+void tgx_set_default_backend(int backend);
+/* ResolveBackend (backend.cpp:155-183): TG_ERR_NO_DEVICE with the reason when a requested GPU
+ * backend cannot run. */
+int tgx_resolve_backend(int requested, int* backend, char* err, size_t err_len);
+/* RunKernel (src/gpu/kernel.cpp:130-153): the program is a TG_KERNEL_* id; `values` holds the
+ * grid's heights, `no_data` (nullable) its no-data value; extras bind after the kernel struct,
+ * `per_cell` ones sliced per strip. `*ran` is 0, leaving `values` untouched, when the backend is
+ * not a GPU one, its device is unavailable, the program has no hip form, or the grid is empty or
+ * too wide; 1 when the kernel ran. `max_strip_bytes` 0 means 64 MiB, as the reference. */
+int tgx_run_kernel(int backend, int program, const void* kernel, size_t kernel_bytes,
+                   const tgx_grid* grid, float* values, const float* no_data, double origin_x,
+                   double origin_y, const tg_buf* extra, size_t n_extra, size_t max_strip_bytes,
+                   int* ran, char* err, size_t err_len);
+
 /* Empty when libtg_hip.so opens, else the loader's reason. */
 const char* tgx_hip_unavailable_reason(void);
 

@@ -218,7 +218,19 @@ int tg_hip_open(int ordinal, tg_dev** out, char* err, size_t errLen) {
     }
     auto dev = std::make_unique<tg_dev>();
     dev->ordinal = ordinal;
-    dev->name = std::string(prop.name) + " (" + prop.gcnArchName + ")";
+    // The marketing name where the runtime reports one, else a generic one; then the target.
+    char marketing[256] = {0};
+    std::string name;
+    if (hipDeviceGetName(marketing, sizeof marketing, ordinal) == hipSuccess) {
+        name = marketing;
+    }
+    if (name.empty()) {
+        name = prop.name;
+    }
+    if (name.empty()) {
+        name = "AMD GPU " + std::to_string(ordinal);
+    }
+    dev->name = name + " (" + prop.gcnArchName + ")";
     *out = dev.get();
     devices.push_back(std::move(dev));
     return TG_OK;

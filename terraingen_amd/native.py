@@ -592,3 +592,83 @@ HOST_ABI["tgx_server_free"] = (C.c_int, [VP])
 HOST_ABI["tgx_hip_unavailable_reason"] = (C.c_char_p, [])
 
 TG_NOT_READY = 4
+
+
+# ---- The backend registry and RunKernel (backend.cpp) ------------------------------------------
+
+BACKENDS = ("auto", "cpu", "vulkan", "gpu", "cuda", "metal", "hip")  # the enum's order
+HOST_ABI["tgx_backend_named"] = (C.c_int, [CP, C.POINTER(C.c_int)])
+HOST_ABI["tgx_backend_name"] = (SZ, [C.c_int, CP, SZ])
+HOST_ABI["tgx_backends_report"] = (SZ, [CP, SZ])
+HOST_ABI["tgx_default_backend"] = (C.c_int, [C.POINTER(C.c_int), CP, SZ])
+HOST_ABI["tgx_set_default_backend"] = (None, [C.c_int])
+HOST_ABI["tgx_resolve_backend"] = (C.c_int, [C.c_int, C.POINTER(C.c_int), CP, SZ])
+HOST_ABI["tgx_run_kernel"] = (C.c_int, [C.c_int, C.c_int, VP, SZ, C.POINTER(Grid), VP,
+                                        C.POINTER(C.c_float), C.c_double, C.c_double, VP, SZ, SZ,
+                                        C.POINTER(C.c_int), CP, SZ])
+
+
+def backend_named(name: str):
+    """BackendNamed: the backend's enum value, or None for an unknown name."""
+    b = C.c_int()
+    code = host_library().tgx_backend_named(name.encode(), C.byref(b))
+    return b.value if code == 0 else None
+
+
+def backend_name(backend: int) -> str:
+    out = C.create_string_buffer(64)
+    host_library().tgx_backend_name(backend, out, 64)
+    return out.value.decode()
+
+
+def backends_report() -> str:
+    lib = host_library()
+    n = lib.tgx_backends_report(None, 0)
+    out = C.create_string_buffer(n + 1)
+    lib.tgx_backends_report(out, n + 1)
+    return out.value.decode()
+
+
+def default_backend() -> int:
+    b = C.c_int()
+    err = C.create_string_buffer(512)
+    code = host_library().tgx_default_backend(C.byref(b), err, 512)
+    if code:
+        raise ValueError(err.value.decode())
+    return b.value
+
+
+def set_default_backend(backend: int) -> None:
+    host_library().tgx_set_default_backend(backend)
+
+
+def resolve_backend(requested: int) -> int:
+    b = C.c_int()
+    err = C.create_string_buffer(1024)
+    code = host_library().tgx_resolve_backend(requested, C.byref(b), err, 1024)
+    if code:
+        raise RuntimeError(err.value.decode())
+    return b.value
+
+
+def run_kernel(backend: int, program: int, kernel, grid: Grid, values: np.ndarray,
+               origin=(0.0, 0.0), extra=(), no_data=None, max_strip_bytes: int = 0) -> bool:
+    """RunKernel: runs `program` (a TG_KERNEL_* id, 0 for none) with the ctypes kernel struct over
+    `values` (float32, height x width, updated in place). `extra` holds (array, per_cell) pairs."""
+    assert values.dtype == np.float32 and values.flags.c_contiguous
+    assert values.size == grid.width * grid.height
+    keep = [np.ascontiguousarray(a) for a, _ in extra]
+    bufs = (TgBuf * max(1, len(keep)))()
+    for i, (a, (_, per_cell)) in enumerate(zip(keep, extra)):
+        bufs[i] = TgBuf(a.ctypes.data, a.nbytes, a.shape[0] if a.ndim else 1, per_cell)
+    ran = C.c_int()
+    err = C.create_string_buffer(1024)
+    nd = C.c_float(no_data) if no_data is not None else None
+    code = host_library().tgx_run_kernel(
+        backend, program, C.cast(C.byref(kernel), VP) if kernel is not None else None,
+        C.sizeof(kernel) if kernel is not None else 0, C.byref(grid), values.ctypes.data,
+        C.byref(nd) if nd is not None else None, origin[0], origin[1], C.cast(bufs, VP),
+        len(keep), max_strip_bytes, C.byref(ran), err, 1024)
+    if code:
+        raise RuntimeError(err.value.decode())
+    return bool(ran.value)
