@@ -121,6 +121,25 @@ int tgx_prepare_craters(const tgx_crater* craters, size_t n, const tgx_crater_mo
                         float** data, size_t* data_count, double** footprint, char* err,
                         size_t err_len);
 
+/* A boulder as the relief draws it: the fields of the reference's Boulder (include/terraingen/
+ * layers/boulders/boulder.h:52-104) the dome relief reads. `exposed` is ExposedHeight(); each
+ * facet is azimuth (rad from east), angle (rad from horizontal) and reach (m). */
+typedef struct tgx_boulder {
+    double x, y, diameter, exposed;
+    double facets[6];
+    uint32_t facet_count;
+    uint32_t model; /* 1: placed as a 3D model, not drawn into the heights */
+} tgx_boulder;
+
+/* BoulderLayer::PrepareRelief for one grid `spacing` (boulder_relief.cpp:373-423): the boulders
+ * drawn into heights (not models, resolution weight above 0) as device records, in list order,
+ * with the parameter block (`dome` 1: the dome averaged over a band one sample wide, the default
+ * look; 0: sampled at the point) and, when `footprint` is not null, four doubles per record
+ * (centre x, centre y, relief box half-width, diameter) for tgx_bin_plane. */
+int tgx_prepare_boulders(const tgx_boulder* boulders, size_t n, double spacing, int dome,
+                         tg_boulder_kernel* kernel, tg_boulder** records, size_t* count,
+                         double** footprint, char* err, size_t err_len);
+
 /* A tile of a batch: a north-up grid `width` x `height` of `pixel` meters, west and north edges
  * in FP64. */
 typedef struct tgx_tile {

@@ -35,7 +35,10 @@ enum {
  * `KernelProgram` per struct (include/terraingen/gpu/program.h:14-23). */
 enum {
     TG_KERNEL_NOISE_LAYER = 1, /* NoiseLayerKernel, src/layers/kernels/noise_layer.slang */
-    TG_KERNEL_CRATERS = 2      /* CraterShapeKernel, src/layers/kernels/craters.slang */
+    TG_KERNEL_CRATERS = 2,     /* CraterShapeKernel, src/layers/kernels/craters.slang */
+    TG_KERNEL_BOULDERS = 3     /* tg_boulder_kernel: the dome relief BoulderLayer bakes on the
+                                * CPU (boulder_relief.cpp:426-440); extras: boulders, tiles,
+                                * entries, lows */
 };
 
 typedef struct tg_dev tg_dev;
@@ -138,6 +141,22 @@ int tg_hip_crater_launch(tg_dev* dev, const tg_crater_set* set, float* heights_d
 /* A crater catalogue resident on the device: the kernel struct, one record per crater in apply
  * order, the data, and each crater's FP64 diameter (its ResolutionWeight is worked out per tile
  * from the tile's pixel size, src/layers/fade.cpp:48-52). */
+/* Boulders resident on the device for one grid spacing (no counterpart in the reference, which
+ * bakes boulders on the CPU): the parameter block, the boulders (tgx_prepare_boulders), their
+ * 16 x 16 tile lists (tgx_bin_plane over their relief boxes) and the grid's lows, as
+ * tg_hip_run(TG_KERNEL_BOULDERS) binds them. tg_hip_boulder_launch bakes them into heights on the
+ * device (each cell's height on entry is its base), on `stream`. */
+typedef struct tg_boulder_set tg_boulder_set;
+int tg_hip_boulder_set_create(tg_dev* dev, const tg_boulder_kernel* kernel,
+                              const tg_boulder* boulders, size_t count, const uint32_t* tiles,
+                              size_t tile_words, const uint32_t* entries, size_t entry_count,
+                              const float* lows, size_t low_count, tg_boulder_set** out,
+                              char* err, size_t err_len);
+int tg_hip_boulder_set_free(tg_boulder_set* set);
+int tg_hip_boulder_launch(tg_dev* dev, const tg_boulder_set* set, float* heights_dev,
+                          const float* xs_dev, const float* ys_dev, const tg_kernel_params* params,
+                          void* stream, char* err, size_t err_len);
+
 typedef struct tg_catalogue tg_catalogue;
 
 int tg_hip_catalogue_create(tg_dev* dev, const tg_crater_shape* kernel,

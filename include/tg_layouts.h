@@ -169,6 +169,30 @@ typedef struct tg_split_point {
     float x, x_low, y, y_low, z, z_low;
 } tg_split_point;
 
+/* One boulder as the dome relief reads it. The reference bakes boulders on the CPU in FP64
+ * (BoulderLayer::AddRelief and DomeHeight, src/layers/boulders/boulder_relief.cpp:44-57,244-275)
+ * and has no GPU layout for them; this is the device's, prepared on the host from a boulder list
+ * for one grid spacing. */
+typedef struct tg_boulder {
+    float x, y;          /* centre, FP32 high parts */
+    float x_low, y_low;  /* and the remainders (the split of MakeSplitPoint) */
+    float radius;        /* 0.5 D */
+    float exposed;       /* Boulder::ExposedHeight: height (1 - burial) */
+    float reach;         /* 0.5 (D + band width): half the side of its relief box */
+    float weight;        /* ResolutionWeight(D, spacing) of the grid */
+    float facet[8];      /* per ventifact: cos azimuth, sin azimuth, reach, tan angle */
+    uint32_t facet_count;
+    uint32_t pad[3];
+} tg_boulder;
+
+/* The boulder relief's parameter block: the band the dome is averaged over (the sample spacing
+ * for BoulderRelief::Dome, 0 for a point sample) and the number of records. */
+typedef struct tg_boulder_kernel {
+    float width;
+    uint32_t count;
+    uint32_t pad[2];
+} tg_boulder_kernel;
+
 /* No-data modes of tg_kernel_params.no_data (src/gpu/kernel.cpp:22-27). */
 enum { TG_NODATA_NONE = 0, TG_NODATA_NAN = 1, TG_NODATA_VALUE = 2 };
 
@@ -190,6 +214,8 @@ static_assert(sizeof(tg_crater_record) == 144, "tg_crater_record must be 144 byt
 static_assert(sizeof(tg_crater_frame) == 52, "tg_crater_frame must be 52 bytes");
 static_assert(sizeof(tg_split_point) == 24, "tg_split_point must be 24 bytes");
 static_assert(sizeof(tg_kernel_params) == 20, "tg_kernel_params must be 20 bytes");
+static_assert(sizeof(tg_boulder) == 80, "tg_boulder is 80 bytes");
+static_assert(sizeof(tg_boulder_kernel) == 16, "tg_boulder_kernel is 16 bytes");
 #endif
 
 #endif /* TG_LAYOUTS_H */

@@ -59,6 +59,21 @@ def _ref_case_type():
 RefCase = _ref_case_type()
 
 
+def _ref_boulder_case_type():
+    from terraingen_amd import native as N
+
+    u32, f64 = C.c_uint32, C.c_double
+    return type("RefBoulderCase", (C.Structure,), {"_fields_": [
+        ("width", u32), ("height", u32), ("pixel", f64), ("center_x", f64), ("center_y", f64),
+        ("elevation", C.c_float), ("source_kind", u32), ("fractal_count", u32),
+        ("fractals", N.Fractal * 8), ("sizes_kind", u32), ("sfd_coefficient", f64),
+        ("sfd_exponent", f64), ("min_d", f64), ("max_d", f64), ("ventifact_share", f64),
+        ("seed", u32), ("threads", u32)]})
+
+
+RefBoulderCase = _ref_boulder_case_type()
+
+
 def _declare(lib: C.CDLL) -> None:
     from terraingen_amd import native as N
 
@@ -96,6 +111,14 @@ def _declare(lib: C.CDLL) -> None:
         "ref_case_craters": (C.c_long, [vp, vp, vp, vp, vp, vp]),
         "ref_case_batch": (C.c_long, [vp, vp, C.POINTER(C.c_long)]),
         "ref_case_batch_copy": (None, [vp, vp, vp]),
+        "or_boulder_grid": (None, [vp, sz, vp, C.c_int, vp]),
+        "or_boulder_grid_f64": (None, [vp, sz, vp, C.c_int, vp]),
+        "or_boulder_relief_f64": (d, [vp, d, d, d]),
+        "ref_boulders_open": (vp, [C.POINTER(RefBoulderCase)]),
+        "ref_boulders_close": (None, [vp]),
+        "ref_boulders_evaluate": (C.c_int, [vp, vp]),
+        "ref_boulders_list": (C.c_long, [vp, vp]),
+        "ref_boulders_error": (C.c_char_p, []),
     }
     for name, (res, args) in sigs.items():
         fn = getattr(lib, name, None)

@@ -43,7 +43,7 @@ def _run(cmd: list[str]) -> None:
 
 
 def build_hip() -> Path:
-    sources = [CSRC / "tg_hip.hip", CSRC / "batch.hip", CSRC / "probe.hip"]
+    sources = [CSRC / "tg_hip.hip", CSRC / "batch.hip", CSRC / "boulders.hip", CSRC / "probe.hip"]
     tmp = HIP_SO.with_suffix(".so.tmp")
     _run([HIPCC, *HIP_FLAGS, f"-I{INCLUDE}", f"-I{CSRC}", *sources, "-o", tmp])
     os.replace(tmp, HIP_SO)

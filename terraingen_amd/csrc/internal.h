@@ -121,6 +121,13 @@ std::vector<float> CullRadii(const tg_crater_shape& k, const tg_crater_record* r
 void OutlineTerms(const tg_crater_record* records, size_t n, const float* data, size_t dataCount,
                   std::vector<float>& harm, std::vector<float4>& outline);
 
+// Boulder sets (boulders.hip): validation of a bound batch, and the strip launch.
+void ValidateBoulderBatch(const tg_boulder_kernel& k, const tg_boulder* boulders, size_t n,
+                          const uint32_t* tiles, size_t tileWords, const uint32_t* entries,
+                          size_t entryCount);
+void LaunchBoulders(const tg_boulder_set& set, float* heights, const float* xs, const float* ys,
+                    const tg_kernel_params& params, hipStream_t stream);
+
 // Per-record MpdTerms of records already on the device, computed there (tg_hip.hip) into a new
 // device array of n entries (at least one).
 tg::MpdTerms* DeviceMpdTerms(const tg_crater_shape* kernelOnDevice,

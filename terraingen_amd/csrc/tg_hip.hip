@@ -241,7 +241,8 @@ const char* tg_hip_device_name(const tg_dev* dev) {
 }
 
 int tg_hip_can_run(const tg_dev* dev, int kernelId) {
-    return dev != nullptr && (kernelId == TG_KERNEL_NOISE_LAYER || kernelId == TG_KERNEL_CRATERS);
+    return dev != nullptr && (kernelId == TG_KERNEL_NOISE_LAYER || kernelId == TG_KERNEL_CRATERS ||
+                              kernelId == TG_KERNEL_BOULDERS);
 }
 
 int tg_hip_close(tg_dev* dev) {
@@ -279,6 +280,16 @@ int tg_hip_run(tg_dev* dev, int kernelId, const void* kernel, size_t kernelBytes
                    "points and lows");
             return TG_ERR_ARGUMENT;
         }
+    } else if (kernelId == TG_KERNEL_BOULDERS) {
+        if (kernelBytes != sizeof(tg_boulder_kernel)) {
+            SetErr(err, errLen, "tg_hip_run: a tg_boulder_kernel is 16 bytes");
+            return TG_ERR_ARGUMENT;
+        }
+        if (nExtra != 4 || extra == nullptr) {
+            SetErr(err, errLen, "tg_hip_run: the boulder kernel binds boulders, tiles, entries "
+                                "and lows");
+            return TG_ERR_ARGUMENT;
+        }
     } else {
         SetErr(err, errLen, "tg_hip_run: unknown kernel id " + std::to_string(kernelId));
         return TG_ERR_ARGUMENT;
@@ -303,6 +314,23 @@ int tg_hip_run(tg_dev* dev, int kernelId, const void* kernel, size_t kernelBytes
             DeviceArray<tg::NoisePlan> dPlan(1);
             dPlan.Upload(plan.get(), sizeof(tg::NoisePlan));
             LaunchNoise(dPlan.ptr, dHeights.ptr, dXs.ptr, dYs.ptr, *params, nullptr);
+        } else if (kernelId == TG_KERNEL_BOULDERS) {
+            const tg_buf* b = extra;
+            tg_boulder_set* set = nullptr;
+            char inner[512] = {0};
+            const int code = tg_hip_boulder_set_create(
+                dev, static_cast<const tg_boulder_kernel*>(kernel),
+                static_cast<const tg_boulder*>(b[0].data), b[0].bytes / sizeof(tg_boulder),
+                static_cast<const uint32_t*>(b[1].data), b[1].bytes / sizeof(uint32_t),
+                static_cast<const uint32_t*>(b[2].data), b[2].bytes / sizeof(uint32_t),
+                static_cast<const float*>(b[3].data), b[3].bytes / sizeof(float), &set, inner,
+                sizeof inner);
+            if (code != TG_OK) {
+                throw HipError{inner};
+            }
+            std::unique_ptr<tg_boulder_set, int (*)(tg_boulder_set*)> owner(
+                set, tg_hip_boulder_set_free);
+            LaunchBoulders(*set, dHeights.ptr, dXs.ptr, dYs.ptr, *params, nullptr);
         } else {
             // Binding order of craters.slang: records, data, tiles, entries, frames, points, lows.
             const tg_buf* b = extra;

@@ -84,6 +84,7 @@ assert RECORD_DTYPE.itemsize == 144 and FRAME_DTYPE.itemsize == 52 and POINT_DTY
 
 TG_KERNEL_NOISE_LAYER = 1
 TG_KERNEL_CRATERS = 2
+TG_KERNEL_BOULDERS = 3
 NODATA_NONE, NODATA_NAN, NODATA_VALUE = 0, 1, 2
 
 # Every entry point include/tg_hip.h declares, with its ctypes signature.
@@ -133,6 +134,11 @@ HIP_ABI = {
     "tg_hip_event_query": (C.c_int, [VP, VP, CP, SZ]),
     "tg_hip_memcpy_d2d_async": (C.c_int, [VP, VP, VP, SZ, VP, CP, SZ]),
     "tg_hip_batch_tune": (C.c_int, [VP, u32, u32, u32, u32, CP, SZ]),
+    "tg_hip_boulder_set_create": (C.c_int, [VP, VP, VP, SZ, VP, SZ, VP, SZ, VP, SZ,
+                                            C.POINTER(VP), CP, SZ]),
+    "tg_hip_boulder_set_free": (C.c_int, [VP]),
+    "tg_hip_boulder_launch": (C.c_int, [VP, VP, VP, VP, VP, C.POINTER(KernelParams), VP, CP,
+                                        SZ]),
     "tg_hip_valu_probe": (C.c_int, [VP, C.c_int, u32, u32, C.POINTER(C.c_float),
                                     C.POINTER(C.c_double), CP, SZ]),
 }
@@ -672,3 +678,42 @@ def run_kernel(backend: int, program: int, kernel, grid: Grid, values: np.ndarra
     if code:
         raise RuntimeError(err.value.decode())
     return bool(ran.value)
+
+
+# ---- Boulders (tgx_prepare_boulders) -----------------------------------------------------------
+
+BOULDER_DTYPE = np.dtype([("x", np.float64), ("y", np.float64), ("diameter", np.float64),
+                          ("exposed", np.float64), ("facets", np.float64, 6),
+                          ("facet_count", np.uint32), ("model", np.uint32)])
+BOULDER_RECORD_DTYPE = np.dtype([("x", np.float32), ("y", np.float32), ("x_low", np.float32),
+                                 ("y_low", np.float32), ("radius", np.float32),
+                                 ("exposed", np.float32), ("reach", np.float32),
+                                 ("weight", np.float32), ("facet", np.float32, 8),
+                                 ("facet_count", np.uint32), ("pad", np.uint32, 3)])
+assert BOULDER_DTYPE.itemsize == 88 and BOULDER_RECORD_DTYPE.itemsize == 80
+BoulderKernel = _struct("BoulderKernel", [("width", f32), ("count", u32), ("pad", u32 * 2)])
+HOST_ABI["tgx_prepare_boulders"] = (C.c_int, [VP, SZ, C.c_double, C.c_int,
+                                              C.POINTER(BoulderKernel), C.POINTER(VP),
+                                              C.POINTER(SZ), C.POINTER(VP), CP, SZ])
+
+
+def prepare_boulders(boulders: np.ndarray, spacing: float, dome: bool = True):
+    """``tgx_prepare_boulders``: (kernel block, records, footprint[n, 4]: x, y, reach, D)."""
+    lib = host_library()
+    boulders = np.ascontiguousarray(boulders, BOULDER_DTYPE)
+    kernel = BoulderKernel()
+    rp, fp = VP(), VP()
+    n = SZ()
+    err = C.create_string_buffer(512)
+    _check(lib.tgx_prepare_boulders(boulders.ctypes.data, boulders.size, spacing, int(dome),
+                                    C.byref(kernel), C.byref(rp), C.byref(n), C.byref(fp),
+                                    err, 512), err)
+    try:
+        records = np.frombuffer(C.string_at(rp.value, max(n.value, 1) * 80),
+                                BOULDER_RECORD_DTYPE)[:n.value].copy()
+        foot = np.frombuffer(C.string_at(fp.value, max(n.value, 1) * 32), np.float64)[
+            :4 * n.value].reshape(n.value, 4).copy()
+    finally:
+        lib.tgx_free(rp)
+        lib.tgx_free(fp)
+    return kernel, records, foot
