@@ -59,6 +59,9 @@ OPS_PER_PERLIN_OCTAVE = 200
 OPS_PAIR = 110
 OPS_PER_HARMONIC = 12
 OPS_ELLIPSE = 12
+#   a (cell, boulder) pair inside the dome's reach: 40 (offset, hypot, two clamps, two
+#   sqrt + asin band areas, the max composite; boulder_relief.cpp:44-57,244-263).
+OPS_BOULDER_PAIR = 40
 # MI355X's published FP32 vector peak, 157.3 TFLOP/s, in lane operations: an FMA counts 2 flops
 # and the peak assumes the packed two-lane form, so 157.3e12 / 2 lane-ops/s (256 CUs x 64 lanes x
 # 2 x 2.4 GHz).
@@ -168,7 +171,8 @@ def cpu_tiles(batch, sample: list[int], cores: int) -> dict:
 
 
 def cpu_grid(terrain, rows: int, cores: int) -> dict:
-    """As cpu_tiles for the single grid, on a band of its rows."""
+    """As cpu_tiles for the single grid, on a band of its rows: the source, the craters and,
+    when the terrain has them, the boulders."""
     from oracle import oracle
     from terraingen_amd import native as N
 
@@ -176,33 +180,43 @@ def cpu_grid(terrain, rows: int, cores: int) -> dict:
     g = terrain.grid
     r0 = (g.height - rows) // 2
     sub_ys = np.ascontiguousarray(terrain.ys[r0:r0 + rows])
-    kernel = N.noise_kernel([C2_FRACTAL])
+    kernel = N.noise_kernel(terrain.source_fractals) if terrain.source_fractals else None
     params = N.KernelParams(g.width, rows, 0, 0.0, r0)
-    out = np.zeros((rows, g.width), np.float32)
+    out = np.full((rows, g.width), terrain.elevation, np.float32)
+    band = N.Grid(g.width, rows, g.origin_x, g.origin_y - r0 * g.pixel, g.pixel)
     t0 = time.perf_counter()
-    orc.or_noise_grid(C.byref(kernel), terrain.xs.ctypes.data, sub_ys.ctypes.data, g.width,
-                      rows, C.byref(params), out.ctypes.data, cores)
+    if kernel is not None:
+        orc.or_noise_grid(C.byref(kernel), terrain.xs.ctypes.data, sub_ys.ctypes.data, g.width,
+                          rows, C.byref(params), out.ctypes.data, cores)
     orc.or_shape_grid(C.byref(terrain.kernel), terrain.records.ctypes.data,
                       terrain.data.ctypes.data, terrain.tiles.ctypes.data,
                       terrain.entries.ctypes.data, terrain.xs.ctypes.data, sub_ys.ctypes.data,
                       terrain.lows.ctypes.data, g.width, rows, C.byref(params), out.ctypes.data,
                       cores)
+    if terrain.boulders is not None:
+        orc.or_boulder_grid(terrain.boulders.ctypes.data, terrain.boulders.size, C.byref(band),
+                            1, out.ctypes.data)
     elapsed = time.perf_counter() - t0
     cx64, cy64 = N.grid_centers(g)
     xs64 = np.ascontiguousarray(cx64)
     ys64 = np.ascontiguousarray(cy64[r0:r0 + rows])
-    ref = np.zeros((rows, g.width), np.float64)
-    orc.or_noise_grid_f64(C.byref(kernel), xs64.ctypes.data, ys64.ctypes.data, g.width, rows,
-                          ref.ctypes.data, cores)
+    ref = np.full((rows, g.width), float(terrain.elevation), np.float64)
+    if kernel is not None:
+        orc.or_noise_grid_f64(C.byref(kernel), xs64.ctypes.data, ys64.ctypes.data, g.width,
+                              rows, ref.ctypes.data, cores)
     orc.or_shape_grid_f64(C.byref(terrain.kernel), terrain.records.ctypes.data,
                           terrain.data.ctypes.data, terrain.tiles.ctypes.data,
                           terrain.entries.ctypes.data, xs64.ctypes.data, ys64.ctypes.data,
                           g.width, rows, r0, ref.ctypes.data, cores)
+    if terrain.boulders is not None:
+        orc.or_boulder_grid_f64(terrain.boulders.ctypes.data, terrain.boulders.size,
+                                C.byref(band), 1, ref.ctypes.data)
     cells = rows * g.width
     return {"value": round(cells / elapsed / 1e9, 6), "unit": "Gcells/s", "cores": cores,
             "kind": "port",
-            "sample": f"rows {r0}..{r0 + rows - 1} (x {g.width} cells) of the grid: noise then "
-                      f"craters, once, {platform.processor() or platform.machine()}",
+            "sample": f"rows {r0}..{r0 + rows - 1} (x {g.width} cells) of the grid: source, "
+                      f"craters{', boulders' if terrain.boulders is not None else ''}, once, "
+                      f"{platform.processor() or platform.machine()}",
             "_r0": r0, "_fp32": out, "_fp64": ref}
 
 
@@ -399,13 +413,29 @@ def bench_serving(args, N, dev) -> dict:
 
 def bench_grid(args, N, dev) -> dict:
     from terraingen_amd.pipeline import (CraterLayerSpec, PlaneTerrain, crater_pair_cost,
-                                         plane_batch_view)
+                                         plane_batch_view, scatter_boulders)
 
-    n = args.size
-    grid = N.north_up_grid(n, n, 0.5)
-    terrain = PlaneTerrain(dev, grid, [C2_FRACTAL], 0.0, CraterLayerSpec(seed=1))
+    c3 = args.workload == "c3"
+    n = args.size if args.size else (8193 if c3 else 4097)
+    pixel = 0.25 if c3 else 0.5
+    grid = N.north_up_grid(n, n, pixel)
+    if c3:
+        spec = CraterLayerSpec(sfd_coefficient=0.0156, sfd_exponent=2.2, min_diameter=0.5,
+                               max_diameter=300.0, complex_=1, transition_diameter=50.0,
+                               peaks=1, seed=2)
+        terrain = PlaneTerrain(dev, grid, [], 0.0, spec)
+    else:
+        terrain = PlaneTerrain(dev, grid, [C2_FRACTAL], 0.0, CraterLayerSpec(seed=1))
     t0 = time.perf_counter()
     terrain.prepare()
+    if c3:
+        # The boulder layer over the grid's box grown by the largest boulder: C3's sizes, 30 %
+        # around crater rims, drawn as dome relief at the grid spacing.
+        half = 0.5 * n * pixel + 3.0
+        terrain.boulders = scatter_boulders((-half, -half, half, half), 3.8e-3, 3.0, 0.2, 3.0,
+                                            seed=2, rim_share=0.3, craters=terrain.crater_list)
+        terrain.all_boulders = terrain.boulders.size
+        terrain._prepare_boulders()
     host_ms = (time.perf_counter() - t0) * 1e3
     heights = N.DeviceArray(dev, n * n * 4)
     sv = C.c_void_p()
@@ -435,27 +465,45 @@ def bench_grid(args, N, dev) -> dict:
 
     noise_ms = time_kernel(lambda: terrain.launch_noise(heights, stream))
     crater_ms = time_kernel(lambda: terrain.launch_craters(heights, stream))
+    boulder_ms = time_kernel(lambda: terrain.launch_boulders(heights, stream)) if c3 else 0.0
     terrain.launch(heights, stream)
     dev.synchronize(stream)
     got = heights.download(np.empty((n, n), np.float32))
     cells = n * n
-    noise_ops = cells * C2_FRACTAL["octaves"] * OPS_PER_PERLIN_OCTAVE
+    noise_ops = 0 if c3 else cells * C2_FRACTAL["octaves"] * OPS_PER_PERLIN_OCTAVE
     descs, bins, entries = plane_batch_view(grid, terrain.tiles, terrain.entries)
     cost = crater_pair_cost(terrain.records, OPS_PAIR, OPS_PER_HARMONIC, OPS_ELLIPSE)
     listed, inside, crater_ops = N.batch_pair_ops(descs, bins, entries, terrain.foot, cost)
-    dominant = "CraterStrip" if crater_ms >= noise_ms else "NoiseFillStrip"
-    dom_ms = max(crater_ms, noise_ms)
-    dom_ops = crater_ops if crater_ms >= noise_ms else noise_ops
+    kernels = {"NoiseFillStrip" if not c3 else "FillValue": (noise_ms, noise_ops),
+               "CraterStrip": (crater_ms, crater_ops)}
+    config = {"cells": cells, "craters": int(terrain.crater_count), "pairs_listed": listed,
+              "pairs_in_reach": inside}
+    if c3:
+        bdescs, bbins, bentries = plane_batch_view(grid, terrain.boulder_tiles,
+                                                   terrain.boulder_entries)
+        bcost = np.full(terrain.boulder_count, float(OPS_BOULDER_PAIR))
+        _, b_inside, boulder_ops = N.batch_pair_ops(bdescs, bbins, bentries, terrain.boulder_foot,
+                                                    bcost)
+        kernels["BoulderStrip"] = (boulder_ms, boulder_ops)
+        config |= {"boulders": int(terrain.all_boulders),
+                   "boulders_drawn": int(terrain.boulder_count),
+                   "boulder_pairs_in_reach": b_inside}
+        config["workload"] = (f"flat source {n}x{n} @0.25 m + power-law craters N(>D)=0.0156 "
+                              "D^-2.2, D 0.5-300 m, complex from 50 m with central peaks "
+                              "(terraces off), seed 2 + boulders N(>D)=3.8e-3 D^-3, D 0.2-3 m, "
+                              "30 % around crater rims (synthetic field), dome relief")
+    else:
+        config["workload"] = (f"synthetic source {n}x{n} @0.5 m: 8-octave Perlin fBm (512 m, "
+                              "lac 2, gain 0.5, 20 m, seed 1) + power-law craters "
+                              "N(>D)=0.05 D^-2, D 1-200 m, seed 1")
+    dominant = max(kernels, key=lambda k: kernels[k][0])
+    dom_ms, dom_ops = kernels[dominant]
     result = {
         "metric": METRIC, "value": round(cells / (step_ms * 1e-3) / 1e9, 3), "unit": "Gcells/s",
         "ms_per_step": round(step_ms, 4), "wall_ms_per_step": round(wall * 1e3 / args.steps, 4),
         "dtype": "f32+u32",
-        "config": {"workload": f"synthetic source {n}x{n} @0.5 m: 8-octave Perlin fBm (512 m, "
-                               "lac 2, gain 0.5, 20 m, seed 1) + power-law craters "
-                               "N(>D)=0.05 D^-2, D 1-200 m, seed 1",
-                   "cells": cells, "craters": int(terrain.crater_count),
-                   "pairs_listed": listed, "pairs_in_reach": inside},
-        "kernels_ms": {"noise_fill": round(noise_ms, 4), "craters": round(crater_ms, 4)},
+        "config": {"workload": config.pop("workload")} | config,
+        "kernels_ms": {k: round(v[0], 4) for k, v in kernels.items()},
         "host_ms": {k: round(v, 1) for k, v in terrain.timings.items()} | {
             "total_prepare": round(host_ms, 1)},
         "roofline": roofline(dom_ops, dom_ms, cells * 4.0) | {"kernel": dominant},
@@ -476,9 +524,10 @@ def main() -> int:
     ap.add_argument("--gpus", type=int, default=1)
     ap.add_argument("--steps", type=int, default=10)
     ap.add_argument("--warmup", type=int, default=2)
-    ap.add_argument("--workload", choices=("tiles", "grid"), default="tiles")
+    ap.add_argument("--workload", choices=("tiles", "grid", "c3"), default="tiles")
     ap.add_argument("--tiles", type=int, default=1024)
-    ap.add_argument("--size", type=int, default=4097, help="grid workload edge")
+    ap.add_argument("--size", type=int, default=0,
+                    help="grid workload edge (0: 4097 for grid, 8193 for c3)")
     ap.add_argument("--serve-seconds", type=float, default=10.0,
                     help="length of the served stream (0: skip it)")
     ap.add_argument("--cpu-cells", type=int, default=6_000_000,

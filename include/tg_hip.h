@@ -236,6 +236,10 @@ int tg_hip_memcpy_h2d(tg_dev* dev, void* dst, const void* src, size_t bytes, cha
 int tg_hip_memcpy_d2h(tg_dev* dev, void* dst, const void* src, size_t bytes, char* err,
                       size_t err_len);
 int tg_hip_memset(tg_dev* dev, void* dst, int value, size_t bytes, char* err, size_t err_len);
+/* dst[i] = value for i < count, on `stream`: FlatSource's fill (src/sources/flat_source.cpp:13-31)
+ * of a height buffer on the device, ordered with the stream's other work. */
+int tg_hip_fill(tg_dev* dev, float* dst, float value, size_t count, void* stream, char* err,
+                size_t err_len);
 /* Device-to-device copy on `stream` (the runtime's copy kernel). */
 int tg_hip_memcpy_d2d_async(tg_dev* dev, void* dst, const void* src, size_t bytes, void* stream,
                             char* err, size_t err_len);

@@ -26,6 +26,15 @@ using namespace tgi;
 namespace {
 
 
+// dst[i] = value for i < count, grid-stride: a flat source's fill, ordered on a stream (and safe
+// inside graph capture, where a memset node is not).
+__global__ void FillValue(float* __restrict__ dst, float value, size_t count) {
+    for (size_t i = size_t(blockIdx.x) * blockDim.x + threadIdx.x; i < count;
+         i += size_t(gridDim.x) * blockDim.x) {
+        dst[i] = value;
+    }
+}
+
 // Cells per block: 64 columns (one wave's lanes along a row, so stores coalesce) by 4 rows.
 constexpr uint32_t kNoiseBlockX = 64;
 constexpr uint32_t kNoiseBlockY = 4;
@@ -769,6 +778,24 @@ int tg_hip_memcpy_d2d_async(tg_dev* dev, void* dst, const void* src, size_t byte
                                  static_cast<hipStream_t>(stream)),
                   "hipMemcpyAsync on the device");
         }
+    });
+}
+
+int tg_hip_fill(tg_dev* dev, float* dst, float value, size_t count, void* stream, char* err,
+                size_t errLen) {
+    if (dev == nullptr || (count > 0 && dst == nullptr)) {
+        SetErr(err, errLen, "tg_hip_fill: null argument");
+        return TG_ERR_ARGUMENT;
+    }
+    return Guard(err, errLen, [&] {
+        Check(hipSetDevice(dev->ordinal), "hipSetDevice");
+        if (count == 0) {
+            return;
+        }
+        const size_t blocks = std::min<size_t>((count + 255) / 256, size_t(1) << 20);
+        hipLaunchKernelGGL(FillValue, dim3(uint32_t(blocks)), dim3(256), 0,
+                           static_cast<hipStream_t>(stream), dst, value, count);
+        Check(hipGetLastError(), "launching the fill kernel");
     });
 }
 

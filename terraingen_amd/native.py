@@ -134,6 +134,7 @@ HIP_ABI = {
     "tg_hip_event_query": (C.c_int, [VP, VP, CP, SZ]),
     "tg_hip_memcpy_d2d_async": (C.c_int, [VP, VP, VP, SZ, VP, CP, SZ]),
     "tg_hip_batch_tune": (C.c_int, [VP, u32, u32, u32, u32, CP, SZ]),
+    "tg_hip_fill": (C.c_int, [VP, VP, C.c_float, SZ, VP, CP, SZ]),
     "tg_hip_boulder_set_create": (C.c_int, [VP, VP, VP, SZ, VP, SZ, VP, SZ, VP, SZ,
                                             C.POINTER(VP), CP, SZ]),
     "tg_hip_boulder_set_free": (C.c_int, [VP]),

@@ -58,6 +58,9 @@ class PlaneTerrain:
     craters: CraterLayerSpec | None = None
     tile_side: int = 16
     timings: dict = field(default_factory=dict)
+    # A boulder layer after the craters: a list in BOULDER_DTYPE (the reference's own, or
+    # scatter_boulders), drawn as dome relief at the grid's spacing.
+    boulders: np.ndarray | None = None
 
     def prepare(self) -> None:
         g = self.grid
@@ -82,6 +85,28 @@ class PlaneTerrain:
         self.crater_count = 0
         if self.craters is not None:
             self._prepare_craters()
+        self.boulder_set = None
+        self.boulder_count = 0
+        if self.boulders is not None:
+            self._prepare_boulders()
+
+    def _prepare_boulders(self) -> None:
+        g = self.grid
+        t0 = time.perf_counter()
+        kernel, records, foot = N.prepare_boulders(self.boulders, g.pixel)
+        tiles, entries = N.bin_plane(g, foot[:, 0], foot[:, 1], foot[:, 2], self.tile_side)
+        t1 = time.perf_counter()
+        out = C.c_void_p()
+        self.dev.call("tg_hip_boulder_set_create", self.dev.handle, C.byref(kernel),
+                      records.ctypes.data, records.size, tiles.ctypes.data, tiles.size,
+                      entries.ctypes.data, entries.size, self.lows.ctypes.data, self.lows.size,
+                      C.byref(out))
+        self.boulder_set = out
+        self.boulder_kernel, self.boulder_records, self.boulder_foot = kernel, records, foot
+        self.boulder_tiles, self.boulder_entries = tiles, entries
+        self.boulder_count = records.size
+        self.timings.update(boulder_prepare_ms=(t1 - t0) * 1e3,
+                            boulder_upload_ms=(time.perf_counter() - t1) * 1e3)
 
     def _prepare_craters(self) -> None:
         spec = self.craters
@@ -128,8 +153,9 @@ class PlaneTerrain:
                                                   C.byref(self.params), stream, err, 512)
             if code:
                 raise N.HipError(err.value.decode())
-        else:
-            heights.zero()  # a flat source at 0 m
+        else:  # a flat source at `elevation`
+            self.dev.call("tg_hip_fill", self.dev.handle, heights.ptr, self.elevation,
+                          self.grid.width * self.grid.height, stream)
 
     def launch_craters(self, heights: N.DeviceArray, stream=None) -> None:
         if self.crater_set is None:
@@ -141,9 +167,16 @@ class PlaneTerrain:
         if code:
             raise N.HipError(err.value.decode())
 
+    def launch_boulders(self, heights: N.DeviceArray, stream=None) -> None:
+        if self.boulder_set is None:
+            return
+        self.dev.call("tg_hip_boulder_launch", self.dev.handle, self.boulder_set, heights.ptr,
+                      self.d_xs.ptr, self.d_ys.ptr, C.byref(self.params), stream)
+
     def launch(self, heights: N.DeviceArray, stream=None) -> None:
         self.launch_noise(heights, stream)
         self.launch_craters(heights, stream)
+        self.launch_boulders(heights, stream)
 
     def evaluate(self) -> np.ndarray:
         g = self.grid
@@ -159,6 +192,9 @@ class PlaneTerrain:
         if getattr(self, "crater_set", None) is not None:
             self.dev.lib.tg_hip_crater_set_free(self.crater_set)
             self.crater_set = None
+        if getattr(self, "boulder_set", None) is not None:
+            self.dev.lib.tg_hip_boulder_set_free(self.boulder_set)
+            self.boulder_set = None
 
 
 def apply_order(craters: np.ndarray) -> np.ndarray:
@@ -441,3 +477,40 @@ class TileServer:
 
     def __del__(self):
         self.close()
+
+
+def scatter_boulders(region: tuple, coefficient: float, exponent: float, min_d: float,
+                     max_d: float, seed: int, rim_share: float = 0.0, craters=None,
+                     min_crater_d: float = 10.0) -> np.ndarray:
+    """A synthetic boulder field in BOULDER_DTYPE, for measurement: Poisson count from the
+    cumulative power law N(>D) = coefficient D^-exponent per m^2 between min_d and max_d over
+    `region` (min_x, min_y, max_x, max_y), diameters by its inverse CDF, uniform centres, heights
+    and burial drawn as the reference's BoulderPose defaults draw them (height 0.4-0.6 D, buried
+    0.1-0.4 of it), no facets. A `rim_share` of them sit around the rims of `craters` (a
+    CRATER_DTYPE list) at least `min_crater_d` across, picked in proportion to rim length, 1-1.5
+    radii from the centre. Not the reference's placement (BoulderLayer thins candidates by its
+    abundance model); the same sizes, density and layout in distribution only."""
+    rng = np.random.default_rng(seed)
+    x0, y0, x1, y1 = region
+    area = (x1 - x0) * (y1 - y0)
+    lo, hi = min_d ** -exponent, max_d ** -exponent
+    n = int(rng.poisson(coefficient * (lo - hi) * area))
+    out = np.zeros(n, N.BOULDER_DTYPE)
+    u = rng.random(n)
+    out["diameter"] = (lo - u * (lo - hi)) ** (-1.0 / exponent)
+    out["x"] = rng.uniform(x0, x1, n)
+    out["y"] = rng.uniform(y0, y1, n)
+    ratio = rng.uniform(0.4, 0.6, n)
+    burial = rng.uniform(0.1, 0.4, n)
+    out["exposed"] = out["diameter"] * ratio * (1.0 - burial)
+    if rim_share > 0.0 and craters is not None and len(craters):
+        big = craters[craters["diameter"] >= min_crater_d]
+        if big.size:
+            near = rng.random(n) < rim_share
+            k = int(near.sum())
+            pick = rng.choice(big.size, k, p=big["diameter"] / big["diameter"].sum())
+            radius = 0.5 * big["diameter"][pick] * rng.uniform(1.0, 1.5, k)
+            angle = rng.uniform(0.0, 2.0 * np.pi, k)
+            out["x"][near] = big["x"][pick] + radius * np.cos(angle)
+            out["y"][near] = big["y"][pick] + radius * np.sin(angle)
+    return out
