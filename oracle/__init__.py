@@ -74,6 +74,18 @@ def _ref_boulder_case_type():
 RefBoulderCase = _ref_boulder_case_type()
 
 
+class RefSphereCase(C.Structure):
+    """``ref_sphere_case`` of ref_sphere.cpp: a crater layer on a sphere over a lon/lat raster."""
+
+    _fields_ = [
+        ("side", C.c_uint32), ("lon", C.c_double), ("lat", C.c_double), ("step", C.c_double),
+        ("radius", C.c_double), ("models", C.c_uint32), ("sfd_coefficient", C.c_double),
+        ("sfd_exponent", C.c_double), ("min_d", C.c_double), ("max_d", C.c_double),
+        ("lifetime", C.c_double), ("irregular", C.c_uint32), ("elongated_share", C.c_double),
+        ("seed", C.c_uint32), ("threads", C.c_uint32), ("manual_count", C.c_uint32),
+        ("manual", (C.c_double * 4) * 4)]
+
+
 def _declare(lib: C.CDLL) -> None:
     from terraingen_amd import native as N
 
@@ -103,6 +115,7 @@ def _declare(lib: C.CDLL) -> None:
         "or_shape_grid_f64": (None, [vp, vp, vp, vp, vp, vp, vp, sz, sz, u, vp, C.c_uint]),
         "or_shape_grid_body": (None, [vp, vp, vp, vp, vp, vp, vp, sz, sz,
                                       C.POINTER(N.KernelParams), vp, C.c_uint]),
+        "or_shape_grid_body_f64": (None, [vp, vp, vp, vp, vp, vp, vp, sz, sz, u, vp, C.c_uint]),
         "ref_apply_crater": (f, [vp, vp, vp, f, f, f]),
         "ref_last_error": (C.c_char_p, []),
         "ref_case_open": (vp, [C.POINTER(RefCase)]),
@@ -119,6 +132,11 @@ def _declare(lib: C.CDLL) -> None:
         "ref_boulders_evaluate": (C.c_int, [vp, vp]),
         "ref_boulders_list": (C.c_long, [vp, vp]),
         "ref_boulders_error": (C.c_char_p, []),
+        "ref_sphere_open": (vp, [C.POINTER(RefSphereCase)]),
+        "ref_sphere_close": (None, [vp]),
+        "ref_sphere_sizes": (C.c_long, [vp, C.POINTER(C.c_long)]),
+        "ref_sphere_copy": (None, [vp, vp, vp, vp, vp, vp, vp]),
+        "ref_sphere_error": (C.c_char_p, []),
     }
     for name, (res, args) in sigs.items():
         fn = getattr(lib, name, None)

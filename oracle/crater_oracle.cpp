@@ -167,4 +167,36 @@ void or_shape_grid_body(const tg_crater_shape* k, const tg_crater_record* record
     });
 }
 
+// The FP64 oracle of or_shape_grid_body: each cell's body-fixed position (x, y, z per cell) and
+// each crater's frame (centre x, y, z, east x, y, z, north x, y, z, sphere radius per crater) given
+// exactly in FP64, heights accumulated in FP64.
+void or_shape_grid_body_f64(const tg_crater_shape* k, const tg_crater_record* records,
+                            const float* data, const std::uint32_t* tiles,
+                            const std::uint32_t* entries, const double* frames,
+                            const double* positions, std::size_t width, std::size_t rows,
+                            std::uint32_t firstRow, double* heights, unsigned threads) {
+    const c64::Shp shape = c64::Widen(*k);
+    const c64::CraterShape view(shape);
+    Rows(rows, threads, [&](std::size_t r) {
+        const std::uint32_t row = firstRow + std::uint32_t(r);
+        for (std::size_t col = 0; col < width; ++col) {
+            const std::size_t i = r * width + col;
+            const std::uint32_t head = Head(tiles, std::uint32_t(col), row);
+            const std::uint32_t first = tiles[head], count = tiles[head + 1];
+            const double* p = positions + 3 * i;
+            double h = heights[i];
+            for (std::uint32_t e = first; e < first + count; ++e) {
+                const std::uint32_t n = entries[e];
+                const double* f = frames + 10 * std::size_t(n);
+                const c64::Frm frame{f[0], f[1], f[2], f[3], f[4], f[5], f[6], f[7], f[8], f[9],
+                                     0.0, 0.0, 0.0};
+                const c64::Offset at =
+                    c64::FrameOffset(frame, p[0] - f[0], p[1] - f[1], p[2] - f[2]);
+                h = c64::ApplyCrater(view, data, c64::Widen(records[n]), h, at.dx, at.dy);
+            }
+            heights[i] = h;
+        }
+    });
+}
+
 }  // extern "C"

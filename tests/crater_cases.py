@@ -86,3 +86,115 @@ def binding(case: CraterCase) -> list[tuple[np.ndarray, int]]:
     points = np.zeros(1, N.POINT_DTYPE)
     return [(case.records, 0), (case.data, 0), (case.tiles, 0), (case.entries, 0), (frames, 0),
             (points, 0), (case.lows, 0)]
+
+
+# Crater layers on a sphere (tests/golden/craters_sphere_*.npz, make_sphere_fixtures.py).
+SPHERE_NAMES = ["ft_midlat", "mpd_antimeridian", "ft_dense_south"]
+
+
+@dataclass
+class SphereCase:
+    name: str
+    kernel: N.CraterShape
+    records: np.ndarray
+    data: np.ndarray
+    frames: np.ndarray        # tg_crater_frame per crater (MakeCraterFrame)
+    points: np.ndarray        # tg_split_point per cell, row-major (MakeSplitPoint)
+    heights: np.ndarray       # the reference's FP32 heights: zeros shaped by the crater layer
+    side: int
+    lon: float                # north-west corner, degrees
+    lat: float
+    step: float               # cell side, degrees
+    radius: float             # sphere radius, meters
+    tiles: np.ndarray         # 16 x 16-cell tiles on a body, every tile listing every crater
+    entries: np.ndarray
+
+    @property
+    def shape(self):
+        return self.heights.shape
+
+    def cell_lonlat(self) -> tuple[np.ndarray, np.ndarray]:
+        """FP64 cell-centre longitude and latitude (degrees), each of shape (side, side)."""
+        k = np.arange(self.side, dtype=np.float64) + 0.5
+        lon = self.lon + k * self.step
+        lat = self.lat - k * self.step
+        return np.meshgrid(lon, lat)
+
+
+def load_sphere(name: str) -> SphereCase:
+    z = np.load(GOLDEN / f"craters_sphere_{name}.npz")
+    kernel = N.CraterShape.from_buffer_copy(z["kernel"].tobytes())
+    records = np.ascontiguousarray(z["records"]).view(N.RECORD_DTYPE)
+    spec = z["case"].tobytes()
+    # ref_sphere_case: side (u32, padded to 8), then lon, lat, step, radius (f64).
+    side = int(np.frombuffer(spec[:4], np.uint32)[0])
+    lon, lat, step, radius = np.frombuffer(spec[8:40], np.float64)
+    n = records.size
+    across = (side + 15) // 16
+    heads = [across, 16, 1]
+    for _ in range(across * across):
+        heads += [0, n]
+    return SphereCase(name, kernel, records, np.ascontiguousarray(z["data"], np.float32),
+                      np.ascontiguousarray(z["frames"]).view(N.FRAME_DTYPE),
+                      np.ascontiguousarray(z["points"]).view(N.POINT_DTYPE),
+                      np.ascontiguousarray(z["heights"]), side, float(lon), float(lat),
+                      float(step), float(radius), np.asarray(heads, np.uint32),
+                      np.arange(n, dtype=np.uint32))
+
+
+def sphere_position(lon: np.ndarray, lat: np.ndarray, radius: float) -> np.ndarray:
+    """Body-fixed position (x, y, z) in FP64 of longitude/latitude in degrees on a sphere."""
+    lam, phi = np.radians(lon), np.radians(lat)
+    return radius * np.stack([np.cos(phi) * np.cos(lam), np.cos(phi) * np.sin(lam),
+                              np.sin(phi)], axis=-1)
+
+
+def sphere_frames64(case: SphereCase) -> np.ndarray:
+    """Each crater's frame in FP64 from its centre's longitude and latitude (read off the
+    fixture's split centre): centre, local east, local north, sphere radius; 10 values each."""
+    f = case.frames
+    x = f["x"].astype(np.float64) + f["x_low"]
+    y = f["y"].astype(np.float64) + f["y_low"]
+    zc = f["z"].astype(np.float64) + f["z_low"]
+    lam = np.arctan2(y, x)
+    phi = np.arcsin(np.clip(zc / case.radius, -1.0, 1.0))
+    centre = sphere_position(np.degrees(lam), np.degrees(phi), case.radius)
+    east = np.stack([-np.sin(lam), np.cos(lam), np.zeros_like(lam)], axis=-1)
+    north = np.stack([-np.sin(phi) * np.cos(lam), -np.sin(phi) * np.sin(lam), np.cos(phi)],
+                     axis=-1)
+    out = np.concatenate([centre, east, north, np.full((lam.size, 1), case.radius)], axis=1)
+    return np.ascontiguousarray(out)
+
+
+def sphere_oracle_heights(orc, case: SphereCase) -> np.ndarray:
+    """The FP32 oracle on a body (or_shape_grid_body) over the fixture's own points and frames."""
+    out = np.zeros(case.shape, np.float32)
+    params = N.KernelParams(case.side, case.side, 0, 0.0, 0)
+    orc.or_shape_grid_body(C.byref(case.kernel), case.records.ctypes.data, case.data.ctypes.data,
+                           case.tiles.ctypes.data, case.entries.ctypes.data,
+                           case.frames.ctypes.data, case.points.ctypes.data, case.side,
+                           case.side, C.byref(params), out.ctypes.data, 0)
+    return out
+
+
+def sphere_oracle_heights_f64(orc, case: SphereCase) -> np.ndarray:
+    """The FP64 oracle on a body: cell positions and frames computed in FP64 from longitude and
+    latitude, independent of the fixture's FP32 splits."""
+    lon, lat = case.cell_lonlat()
+    pos = np.ascontiguousarray(sphere_position(lon, lat, case.radius).reshape(-1, 3))
+    frames = sphere_frames64(case)
+    out = np.zeros(case.shape, np.float64)
+    orc.or_shape_grid_body_f64(C.byref(case.kernel), case.records.ctypes.data,
+                               case.data.ctypes.data, case.tiles.ctypes.data,
+                               case.entries.ctypes.data, frames.ctypes.data, pos.ctypes.data,
+                               case.side, case.side, 0, out.ctypes.data, 0)
+    return out
+
+
+def sphere_binding(case: SphereCase, rows: slice | None = None) -> list[tuple[np.ndarray, int]]:
+    """craters.slang's buffers on a body: points are per cell (one per cell of the strip), lows
+    a placeholder."""
+    points = case.points if rows is None else np.ascontiguousarray(case.points[rows])
+    return [(case.records, 0), (case.data, 0), (case.tiles, 0), (case.entries, 0),
+            (case.frames, 0), (np.ascontiguousarray(points.reshape(-1)), 1),
+            (np.zeros(1, np.float32), 0)]
