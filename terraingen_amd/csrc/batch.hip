@@ -85,6 +85,15 @@ __global__ void ResetQueue(uint32_t* counters) {
 // ResolutionWeight (src/layers/fade.cpp:48-52) with the smoothstep fade from 2 to 4 samples
 // across, in FP64 as the reference computes it.
 __device__ __forceinline__ float ResolutionAt(double diameter, double pixel) {
+    // D >= 4 p (D <= 2 p) exactly puts D / p at or above 4 (at or below 2): the scalings by 4 and
+    // 2 are exact and division rounds monotonically. Most craters in reach fall there, and skip
+    // the FP64 division.
+    if (diameter >= 4.0 * pixel) {
+        return 1.0f;
+    }
+    if (diameter <= 2.0 * pixel) {
+        return 0.0f;
+    }
     const double x = diameter / pixel;
     double falloff;
     if (x <= 2.0) {
@@ -152,17 +161,27 @@ __global__ void __launch_bounds__(kGroupThreads)
         const uint32_t col0 = (local % tile.bins_across) * kBinSide;
         const uint32_t row0 = (local / tile.bins_across) * kBinSide;
         for (uint32_t square = 0; square < 4u; ++square) {
-            const uint32_t col = col0 + (square & 1u) * 8u + (lane & 7u);
-            const uint32_t row = row0 + (square >> 1u) * 8u + (lane >> 3u);
+            const uint32_t squareCol = col0 + (square & 1u) * 8u;
+            const uint32_t squareRow = row0 + (square >> 1u) * 8u;
+            if (squareCol >= tile.width || squareRow >= tile.height) {
+                continue;  // the square lies past the tile's edge (uniform across the wave)
+            }
+            const uint32_t col = squareCol + (lane & 7u);
+            const uint32_t row = squareRow + (lane >> 3u);
+            // Lanes past the tile's edge evaluate the noise of its last column or row and store
+            // nothing, so the whole wave takes part in the noise's shared corner hashing.
+            const uint32_t cellCol = std::min(col, tile.width - 1u);
+            const uint32_t cellRow = std::min(row, tile.height - 1u);
+            // GeoTransform::PixelToWorld of the cell centre in FP64, narrowed and split as
+            // ColumnOffsets/RowOffsets and MakeSplitPoint do (src/gpu/kernel.cpp:55-70,
+            // crater_kernel.cpp:229-236): the source origin is {0, 0}.
+            const double cx = tile.origin_x + (double(cellCol) + 0.5) * tile.pixel + 0.0;
+            const double cy = tile.origin_y + (double(cellRow) + 0.5) * -tile.pixel;
+            const float x = float(cx);
+            const float y = float(cy);
+            float h = plan != nullptr ? elevation + tg::EvaluateStack<true>(*plan, x, y, lane)
+                                      : elevation;
             if (col < tile.width && row < tile.height) {
-                // GeoTransform::PixelToWorld of the cell centre in FP64, narrowed and split as
-                // ColumnOffsets/RowOffsets and MakeSplitPoint do (src/gpu/kernel.cpp:55-70,
-                // crater_kernel.cpp:229-236): the source origin is {0, 0}.
-                const double cx = tile.origin_x + (double(col) + 0.5) * tile.pixel + 0.0;
-                const double cy = tile.origin_y + (double(row) + 0.5) * -tile.pixel;
-                const float x = float(cx);
-                const float y = float(cy);
-                float h = plan != nullptr ? elevation + tg::EvaluateStack(*plan, x, y) : elevation;
                 if (kernel != nullptr && bin.count > 0u) {
                     const float xLow = float(cx - double(x));
                     const float yLow = float(cy - double(y));
@@ -274,9 +293,18 @@ uint32_t GroupsFor(const tg_batch& b) {
                             size_t(cus) * b.groupsPerCu)));
 }
 
+// The bins a wave takes per atomic add: the tuned chunk, cut down on small batches so that each
+// wave gets at least four takes (a batch of one tile then spreads over as many waves as it has
+// bins, which keeps its latency low).
+uint32_t ChunkFor(const tg_batch& b, uint32_t groups) {
+    const uint64_t waves = uint64_t(groups) * (kGroupThreads / 64);
+    return uint32_t(std::clamp<uint64_t>(b.binCount / (waves * 4u), 1u, b.chunk));
+}
+
 void Launch(tg_batch& b, const tg_noise_plan* plan, float elevation, const tg_catalogue* cat,
             float* out) {
     const uint32_t groups = GroupsFor(b);
+    const uint32_t chunk = ChunkFor(b, groups);
     hipLaunchKernelGGL(ResetQueue, dim3(1), dim3(64), 0, b.stream, b.counters);
     // The compact kernel when every crater takes the hoisted path (or there are none).
     const bool allFast = b.hoisted && (cat == nullptr || cat->allFast);
@@ -292,7 +320,7 @@ void Launch(tg_batch& b, const tg_noise_plan* plan, float elevation, const tg_ca
                        cat != nullptr ? cat->harm : nullptr,
                        cat != nullptr ? cat->outline : nullptr,
                        cat != nullptr ? cat->diameters : nullptr,
-                       cat != nullptr && b.hoisted ? cat->terms : nullptr, b.chunk, out);
+                       cat != nullptr && b.hoisted ? cat->terms : nullptr, chunk, out);
     Check(hipGetLastError(), "launching the batched tile kernel");
 }
 
@@ -514,7 +542,7 @@ int tg_hip_batch_run(tg_batch* b, const tg_noise_plan* plan, float elevation,
         if (b->graph == nullptr || b->graphPlan != plan || b->graphCatalogue != cat ||
             b->graphOut != out || b->graphElevation != elevation || b->graphGroups != groups ||
             b->graphWalk != b->walk || b->graphHoisted != b->hoisted ||
-            b->graphChunk != b->chunk) {
+            b->graphChunk != ChunkFor(*b, groups)) {
             DropGraph(*b);
             hipGraph_t graph = nullptr;
             Check(hipStreamBeginCapture(b->stream, hipStreamCaptureModeThreadLocal),
@@ -539,7 +567,7 @@ int tg_hip_batch_run(tg_batch* b, const tg_noise_plan* plan, float elevation,
             b->graphGroups = groups;
             b->graphWalk = b->walk;
             b->graphHoisted = b->hoisted;
-            b->graphChunk = b->chunk;
+            b->graphChunk = ChunkFor(*b, groups);
         }
         Check(hipGraphLaunch(b->graph, b->stream), "hipGraphLaunch");
     });

@@ -861,7 +861,10 @@ __device__ __forceinline__ float TerracedRadii(const float* data, const tg_crate
 // crater's precomputed (A_k cos phi_k, A_k sin phi_k) in `harm` (same slots as its data), and the
 // kept shares share^(k / 2) are powers of sqrt(share) by repeated multiplication. Equal to the
 // reference's form to within FP32 rounding of a dozen products. `ra` holds the crater's
-// cos(heading), sin(heading), sqrt(ellipticity) and its reciprocal.
+// cos(heading), sin(heading), sqrt(ellipticity) and its reciprocal; `harm` is padded by
+// kHarmonicBlock floats past its end.
+constexpr uint32_t kHarmonicBlock = 16u;
+
 __device__ __forceinline__ float FastOutlineScale(const float* harm, float4 ra,
                                                   const tg_crater_record& c, float dx, float dy,
                                                   float d, float share) {
@@ -890,14 +893,28 @@ __device__ __forceinline__ float FastOutlineScale(const float* harm, float4 ra,
             ck = next;
             kept *= q;
         }
+        // The coefficients are read kHarmonicBlock floats at a time (one wide scalar load per
+        // block rather than one per harmonic); the table is padded by a block, so the last block
+        // stays in bounds. The sums are the same, in the same order.
         const float* h = harm + c.harmonic_offset;
-        for (uint32_t n = 0u; n < c.harmonic_count; ++n) {
-            const float wave = h[2u * n] * ck - h[2u * n + 1u] * sk;
-            wobble += (partial ? kept : 1.0f) * wave;
-            const float next = ck * ct - sk * st;
-            sk = sk * ct + ck * st;
-            ck = next;
-            kept *= q;
+        for (uint32_t base = 0u; base < c.harmonic_count; base += kHarmonicBlock / 2u) {
+            float coef[kHarmonicBlock];
+#pragma unroll
+            for (uint32_t i = 0u; i < kHarmonicBlock; ++i) {
+                coef[i] = h[2u * base + i];
+            }
+#pragma unroll
+            for (uint32_t j = 0u; j < kHarmonicBlock / 2u; ++j) {
+                if (base + j >= c.harmonic_count) {
+                    break;
+                }
+                const float wave = coef[2u * j] * ck - coef[2u * j + 1u] * sk;
+                wobble += (partial ? kept : 1.0f) * wave;
+                const float next = ck * ct - sk * st;
+                sk = sk * ct + ck * st;
+                ck = next;
+                kept *= q;
+            }
         }
     }
     return scale * wobble;

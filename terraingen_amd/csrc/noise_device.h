@@ -77,16 +77,22 @@ __device__ __forceinline__ uint32_t CornerX(uint32_t ax, uint32_t ay, uint32_t a
     return vx + vy * vz;
 }
 
-// PerlinGradient2 (perlin.h:17-26).
+// PerlinGradient2 (perlin.h:17-26). Both cases are computed and one is picked with a bitwise
+// select, so a wave whose lanes hold different hashes does not split into two branches; negation
+// is the sign-bit flip that unary minus is, so every result has the reference's bits.
 __device__ __forceinline__ float Grad2(uint32_t hash, float x, float y) {
     const float kInvSqrt2 = 0.707106781186547524f;
     const uint32_t h = hash & 7u;
+    // h < 4: the x or y axis, negated for odd h.
     const float axis = (h & 2u) != 0u ? y : x;
+    const uint32_t signedAxis = __float_as_uint(axis) ^ (h << 31u);
+    // h >= 4: the diagonals.
     const float sx = h == 5u ? -x : x;
     const float sy = h == 6u ? -y : y;
     const float sum = sx + sy;
     const float diagonal = (h == 7u ? -sum : sum) * kInvSqrt2;
-    return h < 4u ? ((h & 1u) != 0u ? -axis : axis) : diagonal;
+    const uint32_t pickAxis = 0u - uint32_t(h < 4u);
+    return __uint_as_float((signedAxis & pickAxis) | (__float_as_uint(diagonal) & ~pickAxis));
 }
 
 __device__ __forceinline__ float Fade(float t) {
@@ -119,6 +125,78 @@ __device__ __forceinline__ float Perlin2(float x, float y, uint32_t seed, uint32
     const float u = Fade(fx);
     const float v = Fade(fy);
     return 1.41421356237309505f * Mix(v, Mix(u, n00, n10), Mix(u, n01, n11));
+}
+
+// Perlin2 for a wave whose 64 lanes are the cells of an 8 x 8 square, every lane active: lane & 7
+// the cell's column, lane >> 3 its row, the cell centre's x non-decreasing along the columns and
+// its y non-increasing along the rows. The lattice coordinates (x cos - y sin, x sin + y cos)
+// times the frequency plus an offset are then monotone along the columns and along the rows (each
+// correctly rounded operation is monotone in its operands), so their floors are least and greatest
+// at corner lanes picked by the signs of cos and sin. When the square's lattice cells span at most
+// 7 x 7, its 8 x 8 lattice corners are hashed once each, one per lane, and every lane gathers its
+// four corner hashes with ds_bpermute: a quarter of the hashing, whose 32-bit multiplies are the
+// costliest VALU work of the noise. The hashes, and so the noise, are identical to Perlin2's.
+// `stepX` and `stepY` are the lane's column and row times the LCG multiplier (SquareSteps).
+__device__ __forceinline__ float Perlin2Square(float x, float y, uint32_t seedLcg, float turnCos,
+                                               float turnSin, uint32_t stepX, uint32_t stepY) {
+    const float cx = floorf(x);
+    const float cy = floorf(y);
+    const float fx = x - cx;
+    const float fy = y - cy;
+    const int32_t ix = int32_t(cx);
+    const int32_t iy = int32_t(cy);
+    // x' grows along the columns when cos >= 0 and along the rows (y falling) when sin >= 0;
+    // y' grows along the columns when sin >= 0 and along the rows when cos < 0.
+    const uint32_t lowX = (turnCos >= 0.0f ? 0u : 7u) + (turnSin >= 0.0f ? 0u : 56u);
+    const uint32_t lowY = (turnSin >= 0.0f ? 0u : 7u) + (turnCos >= 0.0f ? 56u : 0u);
+    const int32_t xMin = __builtin_amdgcn_readlane(ix, int(lowX));
+    const int32_t xMax = __builtin_amdgcn_readlane(ix, int(63u - lowX));
+    const int32_t yMin = __builtin_amdgcn_readlane(iy, int(lowY));
+    const int32_t yMax = __builtin_amdgcn_readlane(iy, int(63u - lowY));
+    uint32_t h00, h10, h01, h11;
+    if (uint32_t(xMax) - uint32_t(xMin) <= 6u && uint32_t(yMax) - uint32_t(yMin) <= 6u) {
+        // This lane hashes lattice corner (xMin + lane & 7, yMin + lane >> 3); the LCG step of a
+        // corner is linear in it, so its words are the least corner's plus the lane's steps.
+        const uint32_t ax = uint32_t(xMin) * kLcgMul + kLcgAdd + stepX;
+        const uint32_t ay = uint32_t(yMin) * kLcgMul + kLcgAdd + stepY;
+        const int32_t mine = int32_t(CornerX(ax, ay, seedLcg, ay * seedLcg));
+        const int32_t at = int32_t(((uint32_t(iy - yMin) << 3u) + uint32_t(ix - xMin)) << 2u);
+        h00 = uint32_t(__builtin_amdgcn_ds_bpermute(at, mine));
+        h10 = uint32_t(__builtin_amdgcn_ds_bpermute(at + 4, mine));
+        h01 = uint32_t(__builtin_amdgcn_ds_bpermute(at + 32, mine));
+        h11 = uint32_t(__builtin_amdgcn_ds_bpermute(at + 36, mine));
+    } else {
+        const uint32_t ax0 = uint32_t(ix) * kLcgMul + kLcgAdd;
+        const uint32_t ax1 = ax0 + kLcgMul;
+        const uint32_t ay0 = uint32_t(iy) * kLcgMul + kLcgAdd;
+        const uint32_t ay1 = ay0 + kLcgMul;
+        const uint32_t yz0 = ay0 * seedLcg;
+        const uint32_t yz1 = ay1 * seedLcg;
+        h00 = CornerX(ax0, ay0, seedLcg, yz0);
+        h10 = CornerX(ax1, ay0, seedLcg, yz0);
+        h01 = CornerX(ax0, ay1, seedLcg, yz1);
+        h11 = CornerX(ax1, ay1, seedLcg, yz1);
+    }
+    const float n00 = Grad2(h00, fx, fy);
+    const float n10 = Grad2(h10, fx - 1.0f, fy);
+    const float n01 = Grad2(h01, fx, fy - 1.0f);
+    const float n11 = Grad2(h11, fx - 1.0f, fy - 1.0f);
+    const float u = Fade(fx);
+    const float v = Fade(fy);
+    return 1.41421356237309505f * Mix(v, Mix(u, n00, n10), Mix(u, n01, n11));
+}
+
+// The lane's column and row in its 8 x 8 square times the LCG multiplier, for Perlin2Square. The
+// empty asm hides their origin, so the compiler adds them to the least corner's LCG word (one add
+// per coordinate) instead of re-multiplying the corner's index every octave.
+struct SquareSteps {
+    uint32_t x, y;
+};
+__device__ __forceinline__ SquareSteps MakeSquareSteps(uint32_t lane) {
+    uint32_t x = (lane & 7u) * kLcgMul;
+    uint32_t y = (lane >> 3u) * kLcgMul;
+    asm("" : "+v"(x), "+v"(y));
+    return {x, y};
 }
 
 // SimplexNoise2 (simplex.h:18-59).
@@ -232,19 +310,30 @@ __device__ __forceinline__ float Basis2(uint32_t basis, float x, float y, const 
 
 // Fractal2 (fractal.h:219-239) over a planned octave loop; AddFractalOctave and FinishFractal
 // (fractal.h:164-211) with the weights read from the plan.
-__device__ __forceinline__ float FractalLoop(const FractalLoopPlan& p, float x, float y) {
+// kSquare: the caller's wave is an 8 x 8 square of cells as Perlin2Square requires, and `lane` is
+// the lane's index in it.
+template <bool kSquare = false>
+__device__ __forceinline__ float FractalLoop(const FractalLoopPlan& p, float x, float y,
+                                             uint32_t lane = 0u) {
     const uint32_t type = p.type;
     const uint32_t basis = p.basis;
     const uint32_t octaves = p.octaves;
     float sum = 0.0f;
     float feedback = 1.0f;
     if (type == TG_FRACTAL_FBM && basis == TG_BASIS_PERLIN) {
+        const SquareSteps steps = MakeSquareSteps(lane);
         for (uint32_t o = 0; o < octaves; ++o) {
             const OctavePlan& q = p.octave[o];
             const float rx = x * q.turnCos - y * q.turnSin;
             const float ry = x * q.turnSin + y * q.turnCos;
-            const float n = Perlin2(rx * q.frequency + q.offsetX, ry * q.frequency + q.offsetY,
-                                    q.seed, q.seedLcg);
+            const float px = rx * q.frequency + q.offsetX;
+            const float py = ry * q.frequency + q.offsetY;
+            float n;
+            if constexpr (kSquare) {
+                n = Perlin2Square(px, py, q.seedLcg, q.turnCos, q.turnSin, steps.x, steps.y);
+            } else {
+                n = Perlin2(px, py, q.seed, q.seedLcg);
+            }
             sum += q.weight * n;
         }
         return p.amplitude * sum / p.totalWeight;
@@ -296,7 +385,11 @@ __device__ __forceinline__ float FractalLoop(const FractalLoopPlan& p, float x, 
 }
 
 // EvaluateFractals (fractals.h:106-135,151-196): the stack's height at (x, y).
-__device__ __forceinline__ float EvaluateStack(const NoisePlan& plan, float x, float y) {
+// kSquare: as for FractalLoop; a warped fractal's coordinates are not monotone across the square,
+// so it always takes the per-lane path.
+template <bool kSquare = false>
+__device__ __forceinline__ float EvaluateStack(const NoisePlan& plan, float x, float y,
+                                               uint32_t lane = 0u) {
     float height = 0.0f;
     const uint32_t count = plan.count;
     for (uint32_t i = 0; i < count; ++i) {
@@ -307,7 +400,7 @@ __device__ __forceinline__ float EvaluateStack(const NoisePlan& plan, float x, f
             const float dy = FractalLoop(f.warpY, x, y);
             value = FractalLoop(f.unit, x + dx, y + dy);
         } else {
-            value = FractalLoop(f.unit, x, y);
+            value = FractalLoop<kSquare>(f.unit, x, y, lane);
         }
         const float scaled = f.amplitude * value;
         if (i == 0) {

@@ -917,9 +917,11 @@ namespace tgi {
 
 // Per-crater outline terms the fast path reads: (A cos phi, A sin phi) in each harmonic's data slot
 // and (cos h, sin h, sqrt(ellipticity), 1 / sqrt(ellipticity)) per record, in FP64 then rounded.
+// The harmonics carry tg::kHarmonicBlock floats of zeros past the end: the kernels load them a
+// block at a time.
 void OutlineTerms(const tg_crater_record* records, size_t n, const float* data, size_t dataCount,
                   std::vector<float>& harm, std::vector<float4>& outline) {
-    harm.assign(dataCount > 0 ? dataCount : 1, 0.0f);
+    harm.assign(dataCount + tg::kHarmonicBlock, 0.0f);
     outline.assign(n > 0 ? n : 1, float4{1.0f, 0.0f, 1.0f, 1.0f});
     for (size_t i = 0; i < n; ++i) {
         const tg_crater_record& c = records[i];

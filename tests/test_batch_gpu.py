@@ -75,6 +75,41 @@ def test_batched_tile_matches_oracle(hip, orc, batch):
         assert d.max() <= 1e-4, (i, d.max())
 
 
+@pytest.mark.parametrize("fractals", [
+    [C2_FRACTAL],
+    [dict(seed=12, wavelength=40.0, octaves=12, amplitude=3.0),
+     dict(seed=3, type=1, offset=0.9, blend=1)],
+    [dict(seed=5, wavelength=90.0, octaves=6), dict(seed=6, warp_amplitude=25.0,
+                                                    warp_wavelength=300.0, octaves=7)],
+])
+def test_batched_noise_bit_exact_across_pixel_sizes(hip, orc, fractals):
+    """Tiles from 3 cm to 60 m a pixel, so the noise's octaves span from far under one lattice
+    cell per 8 x 8 square (corner hashes shared across the wave) to many (hashed per lane), on
+    every sign of the octave rotation; each tile's heights equal the oracle's bit for bit."""
+    b = TileBatch(hip, fractals, CraterLayerSpec(seed=4), region=(0.0, 0.0, 60.0, 60.0))
+    b.build()
+    tiles = c4_tiles(20, 3000.0, seed=3, min_k=3, max_k=7, pixel_range=(0.03, 60.0))
+    tiles["origin_x"] += 2.0e4  # away from the crater catalogue: noise only
+    tiles["origin_y"] -= 7.5e3
+    b.stage(tiles)
+    try:
+        got = run_batch(hip, b, use_graph=False)
+        for i in range(tiles.size):
+            inp = b.plain_inputs(i)
+            assert inp["craters"] is None
+            g = inp["grid"]
+            want = np.zeros((g.height, g.width), np.float32)
+            params = N.KernelParams(g.width, g.height, 0, 0.0, 0)
+            orc.or_noise_grid(C.byref(b.noise_kernel), inp["xs"].ctypes.data,
+                              inp["ys"].ctypes.data, g.width, g.height, C.byref(params),
+                              want.ctypes.data, 1)
+            mine = b.tile_heights(got, i)
+            assert np.array_equal(mine.view(np.uint32), want.view(np.uint32)), (
+                i, float(tiles["pixel"][i]), int(g.width))
+    finally:
+        b.close()
+
+
 def test_stage_rejects_malformed_batches(hip, batch):
     bins = batch.bins.copy()
     bins["count"][0] = 10 ** 8  # runs past the entries
