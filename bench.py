@@ -254,9 +254,9 @@ def bench_tiles(args, N, dev) -> dict:
     tuning = None
     if not args.no_autotune:
         timings = batch.autotune(out)
-        tuning = {"chosen": {"walk": ["together", "per_lane"][batch.tuned[0]],
+        tuning = {"chosen": {"walk": ["together", "per_lane", "culled"][batch.tuned[0]],
                              "groups_per_cu": batch.tuned[1], "chunk": batch.tuned[2]},
-                  "ms": {f"{['together', 'per_lane'][w]}/{g}/{c}": round(ms, 3)
+                  "ms": {f"{['together', 'per_lane', 'culled'][w]}/{g}/{c}": round(ms, 3)
                          for (w, g, c), ms in timings.items()}}
         print(f"[bench] autotune {tuning}", file=sys.stderr, flush=True)
     for _ in range(args.warmup):

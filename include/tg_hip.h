@@ -213,12 +213,14 @@ int tg_hip_batch_run(tg_batch* batch, const tg_noise_plan* plan, float elevation
                      int use_graph, char* err, size_t err_len);
 /* Launch choices of a batch (no counterpart in the reference), for the autotuner. The crater
  * walk: TG_WALK_TOGETHER (the default), where a wave applies a listed crater whenever any of its
- * cells is in reach, or TG_WALK_PER_LANE, where each cell walks the list to its own next crater in
- * reach. Workgroups per CU of the persistent launch: 1-64 (default 8). `hoisted` (default 1) uses
- * the per-crater constants of the MPD profile path where a crater allows it; 0 evaluates every
- * crater in full. `chunk`: bins a wave takes from its queue per atomic add, 1-256 (default 8).
- * Every choice gives the same heights. */
-enum { TG_WALK_TOGETHER = 0, TG_WALK_PER_LANE = 1 };
+ * cells is in reach; TG_WALK_PER_LANE, where each cell walks the list to its own next crater in
+ * reach; or TG_WALK_CULLED, where a wave first tests 64 entries at once against the box of its
+ * cells and steps only through those that can reach one. Workgroups per CU of the persistent
+ * launch: 1-64 (default 8). `hoisted` (default 1) uses the per-crater constants of the MPD profile
+ * path where a crater allows it; 0 evaluates every crater in full. `chunk`: bins a wave takes
+ * from its queue per atomic add, 1-256 (default 8; fewer on batches too small to give every wave
+ * four takes). Every choice gives the same heights. */
+enum { TG_WALK_TOGETHER = 0, TG_WALK_PER_LANE = 1, TG_WALK_CULLED = 2 };
 int tg_hip_batch_tune(tg_batch* batch, uint32_t crater_walk, uint32_t groups_per_cu,
                       uint32_t hoisted, uint32_t chunk, char* err, size_t err_len);
 int tg_hip_batch_wait(tg_batch* batch, char* err, size_t err_len);

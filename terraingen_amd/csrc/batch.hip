@@ -28,6 +28,8 @@ struct tg_catalogue {
     float4* outline = nullptr;
     double* diameters = nullptr;
     tg::MpdTerms* terms = nullptr;
+    tg::HotCrater* hot = nullptr;  // the fast path's record per crater
+    float* hotHarm = nullptr;      // its first tg::kHotHarmonics outline harmonics
     size_t count = 0;
     bool allFast = false;  // every crater takes the hoisted MPD path
 };
@@ -108,15 +110,58 @@ __device__ __forceinline__ float ResolutionAt(double diameter, double pixel) {
     return float(1.0 - falloff);
 }
 
-// The crater walks of a bin's list: how a wave visits the (cell, crater) pairs.
+// The crater walks of a bin's list: how a wave visits the (cell, crater) pairs. Every walk applies
+// each cell's craters in list order, so all give the same heights.
 enum CraterWalk : uint32_t {
     // The wave steps through the list together; a crater is applied whenever any lane is in its
     // reach (the lanes outside idle through it).
     kWalkTogether = TG_WALK_TOGETHER,
     // Each lane scans the list for its own next crater in reach and applies it; the wave repeats
-    // until no lane finds another. Every cell still applies its craters in list order.
+    // until no lane finds another.
     kWalkPerLane = TG_WALK_PER_LANE,
+    // The wave tests 64 entries at once, one per lane, against the box of its 8 x 8 cell centres
+    // and steps only through the entries that can reach one of them (a list of at most
+    // kShortList entries is stepped through as kWalkTogether does). One gather then replaces a
+    // chain of dependent scalar loads per entry.
+    kWalkCulled = TG_WALK_CULLED,
 };
+
+constexpr uint32_t kShortList = 4u;
+
+// Crater n's outline harmonics from its first: its slot of the hot table, or the padded harmonic
+// table when the outline has more than a slot holds.
+__device__ __forceinline__ const float* Coefficients(const tg::HotCrater& c, uint32_t n,
+                                                    const float* hotHarm, const float* harm) {
+    return c.harmonic_count <= tg::kHotHarmonics
+               ? hotHarm + size_t(n) * (2u * tg::kHotHarmonics)
+               : harm + c.harmonic_offset;
+}
+
+// Crater n at the cell (x + xLow, y + yLow) of height h, as the strip kernel applies it: nothing
+// beyond the crater's cull radius (or for a lane whose cell lies past the tile), else the hoisted
+// MPD evaluation where the crater allows it and the full one otherwise.
+template <bool kAllFast>
+__device__ __forceinline__ float VisitCrater(uint32_t n, bool inside, float h, float x, float xLow,
+                                             float y, float yLow, double pixel, bool hoisted,
+                                             const tg::HotCrater* hot, const float* hotHarm,
+                                             const tg_crater_shape* kernel, const float* data,
+                                             const tg_crater_record* records, const float* harm,
+                                             const float4* outline) {
+    const tg::HotCrater& c = hot[n];
+    const float dx = tg::SplitDifference(x, xLow, c.x, c.x_low);
+    const float dy = tg::SplitDifference(y, yLow, c.y, c.y_low);
+    if (inside && dx * dx + dy * dy < c.cull) {
+        const float weight = ResolutionAt(c.diameter, pixel);
+        if (kAllFast || (hoisted && c.terms.fast != 0u)) {
+            return tg::ApplyMpd(c.terms, c.ellipticity, c.first_harmonic, c.harmonic_count,
+                                Coefficients(c, n, hotHarm, harm), c.ra, h, dx, dy, weight);
+        } else if constexpr (!kAllFast) {
+            const tg::CraterShape shape(*kernel);
+            return tg::ApplyCrater(shape, data, records[n], h, dx, dy, harm, outline[n], weight);
+        }
+    }
+    return h;
+}
 
 // One bin per wave: lane 0 takes the next bin with one atomic add and the index is broadcast
 // with readfirstlane, so the bin loop is uniform across the wave and no wave waits for another
@@ -131,9 +176,9 @@ __global__ void __launch_bounds__(kGroupThreads)
                const tg::NoisePlan* __restrict__ plan, float elevation,
                const tg_crater_shape* __restrict__ kernel,
                const tg_crater_record* __restrict__ records, const float* __restrict__ data,
-               const float* __restrict__ cull, const float* __restrict__ harm,
-               const float4* __restrict__ outline, const double* __restrict__ diameters,
-               const tg::MpdTerms* __restrict__ terms, uint32_t chunk, float* __restrict__ out) {
+               const float* __restrict__ harm, const float4* __restrict__ outline,
+               const tg::HotCrater* __restrict__ hot, const float* __restrict__ hotHarm,
+               uint32_t hoisted, uint32_t chunk, float* __restrict__ out) {
     const uint32_t binCount = counters[kCountSlot];
     const uint32_t lane = threadIdx.x & 63u;
     uint32_t queue = blockIdx.x % kQueues;
@@ -168,8 +213,10 @@ __global__ void __launch_bounds__(kGroupThreads)
             }
             const uint32_t col = squareCol + (lane & 7u);
             const uint32_t row = squareRow + (lane >> 3u);
-            // Lanes past the tile's edge evaluate the noise of its last column or row and store
-            // nothing, so the whole wave takes part in the noise's shared corner hashing.
+            const bool inside = col < tile.width && row < tile.height;
+            // Lanes past the tile's edge evaluate its last column or row and store nothing, so
+            // the whole wave takes part in the noise's shared corner hashing and in the walks'
+            // ballots.
             const uint32_t cellCol = std::min(col, tile.width - 1u);
             const uint32_t cellRow = std::min(row, tile.height - 1u);
             // GeoTransform::PixelToWorld of the cell centre in FP64, narrowed and split as
@@ -181,69 +228,116 @@ __global__ void __launch_bounds__(kGroupThreads)
             const float y = float(cy);
             float h = plan != nullptr ? elevation + tg::EvaluateStack<true>(*plan, x, y, lane)
                                       : elevation;
-            if (col < tile.width && row < tile.height) {
-                if (kernel != nullptr && bin.count > 0u) {
-                    const float xLow = float(cx - double(x));
-                    const float yLow = float(cy - double(y));
-                    const tg::CraterShape shape(*kernel);
-                    const uint32_t end = bin.first + bin.count;
-                    if constexpr (kWalk == kWalkTogether) {
-                        for (uint32_t e = bin.first; e < end; ++e) {
-                            const uint32_t n = entries[e];
-                            const tg_crater_record& c = records[n];
-                            const float dx = tg::SplitDifference(x, xLow, c.x, c.x_low);
-                            const float dy = tg::SplitDifference(y, yLow, c.y, c.y_low);
-                            if (dx * dx + dy * dy < cull[n]) {
-                                const float weight = ResolutionAt(diameters[n], tile.pixel);
-                                if (kAllFast || (terms != nullptr && terms[n].fast != 0u)) {
-                                    h = tg::ApplyCraterMpd(c, terms[n], h, dx, dy, harm,
-                                                           outline[n], weight);
-                                } else if constexpr (!kAllFast) {
-                                    h = tg::ApplyCrater(shape, data, c, h, dx, dy, harm,
-                                                        outline[n], weight);
-                                }
+            if (kernel != nullptr && bin.count > 0u) {
+                const float xLow = float(cx - double(x));
+                const float yLow = float(cy - double(y));
+                const uint32_t end = bin.first + bin.count;
+                const auto visit = [&](uint32_t n) {
+                    h = VisitCrater<kAllFast>(n, inside, h, x, xLow, y, yLow, tile.pixel,
+                                              hoisted != 0u, hot, hotHarm, kernel, data, records,
+                                              harm, outline);
+                };
+                if constexpr (kWalk == kWalkPerLane) {
+                    uint32_t e = bin.first;
+                    for (;;) {
+                        uint32_t n = 0u;
+                        bool hit = false;
+                        while (inside && e < end) {
+                            const uint32_t m = entries[e];
+                            ++e;
+                            const tg::HotCrater& c = hot[m];
+                            const float ex = tg::SplitDifference(x, xLow, c.x, c.x_low);
+                            const float ey = tg::SplitDifference(y, yLow, c.y, c.y_low);
+                            if (ex * ex + ey * ey < c.cull) {
+                                n = m;
+                                hit = true;
+                                break;
                             }
                         }
-                    } else {
-                        uint32_t e = bin.first;
-                        for (;;) {
-                            uint32_t n = 0u;
-                            float dx = 0.0f, dy = 0.0f;
-                            bool hit = false;
-                            while (e < end) {
-                                const uint32_t m = entries[e];
-                                ++e;
-                                const tg_crater_record& c = records[m];
-                                const float ex = tg::SplitDifference(x, xLow, c.x, c.x_low);
-                                const float ey = tg::SplitDifference(y, yLow, c.y, c.y_low);
-                                if (ex * ex + ey * ey < cull[m]) {
-                                    n = m;
-                                    dx = ex;
-                                    dy = ey;
-                                    hit = true;
-                                    break;
-                                }
-                            }
-                            if (__ballot(hit) == 0ull) {
-                                break;  // no lane of the wave has a crater left
-                            }
-                            if (hit) {
-                                const float weight = ResolutionAt(diameters[n], tile.pixel);
-                                if (kAllFast || (terms != nullptr && terms[n].fast != 0u)) {
-                                    h = tg::ApplyCraterMpd(records[n], terms[n], h, dx, dy,
-                                                           harm, outline[n], weight);
-                                } else if constexpr (!kAllFast) {
-                                    h = tg::ApplyCrater(shape, data, records[n], h, dx, dy, harm,
-                                                        outline[n], weight);
-                                }
-                            }
+                        if (__ballot(hit) == 0ull) {
+                            break;  // no lane of the wave has a crater left
+                        }
+                        if (hit) {
+                            visit(n);
+                        }
+                    }
+                } else if (kWalk == kWalkTogether || bin.count <= kShortList) {
+                    for (uint32_t e = bin.first; e < end; ++e) {
+                        visit(entries[e]);
+                    }
+                } else {
+                    // The box of the square's cell centres (the same FP64 centres as above).
+                    const uint32_t lastCol = std::min(squareCol + 7u, tile.width - 1u);
+                    const uint32_t lastRow = std::min(squareRow + 7u, tile.height - 1u);
+                    const double west = tile.origin_x + (double(squareCol) + 0.5) * tile.pixel + 0.0;
+                    const double east = tile.origin_x + (double(lastCol) + 0.5) * tile.pixel + 0.0;
+                    const double north = tile.origin_y + (double(squareRow) + 0.5) * -tile.pixel;
+                    const double south = tile.origin_y + (double(lastRow) + 0.5) * -tile.pixel;
+                    for (uint32_t base = bin.first; base < end; base += 64u) {
+                        const uint32_t e = base + lane;
+                        uint32_t n = 0u;
+                        bool near = false;
+                        if (e < end) {
+                            n = entries[e];
+                            const tg::HotCrater& c = hot[n];
+                            // The centre's split sum is exact in FP64. A cell passing the FP32
+                            // test above lies within the cull radius to a relative 1e-6 or so,
+                            // and the box is no farther than any of its cells, so the margin
+                            // keeps every entry a cell of the square can take.
+                            const double px = double(c.x) + double(c.x_low);
+                            const double py = double(c.y) + double(c.y_low);
+                            const double ex = fmax(0.0, fmax(west - px, px - east));
+                            const double ey = fmax(0.0, fmax(south - py, py - north));
+                            near = ex * ex + ey * ey <= double(c.cull) * (1.0 + 1e-4) + 1e-6;
+                        }
+                        uint64_t candidates = __ballot(near);
+                        while (candidates != 0ull) {
+                            const uint32_t j = uint32_t(__builtin_ctzll(candidates));
+                            candidates &= candidates - 1ull;
+                            visit(uint32_t(__builtin_amdgcn_readlane(int(n), int(j))));
                         }
                     }
                 }
+            }
+            if (inside) {
                 out[tile.out_offset + size_t(row) * tile.width + col] = h;
             }
         }
         }
+    }
+}
+
+// One lane per crater: the hot record and harmonic slot from the catalogue's arrays.
+__global__ void MakeHotCraters(const tg_crater_record* __restrict__ records,
+                               const float* __restrict__ cull, const double* __restrict__ diameters,
+                               const tg::MpdTerms* __restrict__ terms,
+                               const float4* __restrict__ outline, const float* __restrict__ harm,
+                               size_t count, tg::HotCrater* __restrict__ hot,
+                               float* __restrict__ hotHarm) {
+    const size_t i = size_t(blockIdx.x) * blockDim.x + threadIdx.x;
+    if (i >= count) {
+        return;
+    }
+    const tg_crater_record& c = records[i];
+    tg::HotCrater h;
+    h.x = c.x;
+    h.y = c.y;
+    h.x_low = c.x_low;
+    h.y_low = c.y_low;
+    h.cull = cull[i];
+    h.ellipticity = c.ellipticity;
+    h.first_harmonic = c.first_harmonic;
+    h.harmonic_count = c.harmonic_count;
+    h.diameter = diameters[i];
+    h.harmonic_offset = c.harmonic_offset;
+    h.pad = 0u;
+    h.ra = outline[i];
+    h.terms = terms[i];
+    hot[i] = h;
+    float* slot = hotHarm + i * (2u * tg::kHotHarmonics);
+    const uint32_t held = 2u * std::min(c.harmonic_count, tg::kHotHarmonics);
+    for (uint32_t k = 0; k < 2u * tg::kHotHarmonics; ++k) {
+        slot[k] = k < held ? harm[c.harmonic_offset + k] : 0.0f;
     }
 }
 
@@ -275,7 +369,8 @@ void FreeCatalogue(tg_catalogue* c) {
     for (void* p : {static_cast<void*>(c->kernel), static_cast<void*>(c->records),
                     static_cast<void*>(c->data), static_cast<void*>(c->cull),
                     static_cast<void*>(c->harm), static_cast<void*>(c->outline),
-                    static_cast<void*>(c->diameters), static_cast<void*>(c->terms)}) {
+                    static_cast<void*>(c->diameters), static_cast<void*>(c->terms),
+                    static_cast<void*>(c->hot), static_cast<void*>(c->hotHarm)}) {
         if (p != nullptr) {
             (void)hipFree(p);
         }
@@ -310,17 +405,18 @@ void Launch(tg_batch& b, const tg_noise_plan* plan, float elevation, const tg_ca
     const bool allFast = b.hoisted && (cat == nullptr || cat->allFast);
     auto* fn = b.walk == TG_WALK_PER_LANE
                    ? (allFast ? BatchTiles<kWalkPerLane, true> : BatchTiles<kWalkPerLane, false>)
+               : b.walk == TG_WALK_CULLED
+                   ? (allFast ? BatchTiles<kWalkCulled, true> : BatchTiles<kWalkCulled, false>)
                    : (allFast ? BatchTiles<kWalkTogether, true>
                               : BatchTiles<kWalkTogether, false>);
     hipLaunchKernelGGL(fn, dim3(groups), dim3(kGroupThreads), 0, b.stream, b.tiles, b.bins,
                        b.entries, b.counters, plan != nullptr ? plan->device : nullptr,
                        elevation, cat != nullptr ? cat->kernel : nullptr,
                        cat != nullptr ? cat->records : nullptr,
-                       cat != nullptr ? cat->data : nullptr, cat != nullptr ? cat->cull : nullptr,
-                       cat != nullptr ? cat->harm : nullptr,
+                       cat != nullptr ? cat->data : nullptr, cat != nullptr ? cat->harm : nullptr,
                        cat != nullptr ? cat->outline : nullptr,
-                       cat != nullptr ? cat->diameters : nullptr,
-                       cat != nullptr && b.hoisted ? cat->terms : nullptr, chunk, out);
+                       cat != nullptr ? cat->hot : nullptr,
+                       cat != nullptr ? cat->hotHarm : nullptr, b.hoisted ? 1u : 0u, chunk, out);
     Check(hipGetLastError(), "launching the batched tile kernel");
 }
 
@@ -371,6 +467,17 @@ int tg_hip_catalogue_create(tg_dev* dev, const tg_crater_shape* kernel,
         cat->diameters = UploadArray(diameters, count);
         cat->terms = DeviceMpdTerms(cat->kernel, cat->records, count);
         cat->allFast = AllMpdFast(*kernel, records, count);
+        cat->hot = Alloc<tg::HotCrater>(count);
+        cat->hotHarm = Alloc<float>(count * 2u * tg::kHotHarmonics);
+        if (count > 0) {
+            const uint32_t block = 256;
+            hipLaunchKernelGGL(MakeHotCraters, dim3(uint32_t((count + block - 1) / block)),
+                               dim3(block), 0, nullptr, cat->records, cat->cull, cat->diameters,
+                               cat->terms, cat->outline, cat->harm, count, cat->hot,
+                               cat->hotHarm);
+            Check(hipGetLastError(), "launching the hot crater records");
+            Check(hipDeviceSynchronize(), "building the hot crater records");
+        }
     });
     if (code != TG_OK) {
         FreeCatalogue(cat.get());
@@ -579,7 +686,7 @@ int tg_hip_batch_tune(tg_batch* b, uint32_t crater_walk, uint32_t groups_per_cu,
         SetErr(err, errLen, "tg_hip_batch_tune: null batch");
         return TG_ERR_ARGUMENT;
     }
-    if (crater_walk > TG_WALK_PER_LANE || groups_per_cu == 0 || groups_per_cu > 64 ||
+    if (crater_walk > TG_WALK_CULLED || groups_per_cu == 0 || groups_per_cu > 64 ||
         chunk == 0 || chunk > 256) {
         SetErr(err, errLen,
                "tg_hip_batch_tune: unknown crater walk, groups per CU not in 1-64 or chunk not "

@@ -865,9 +865,11 @@ __device__ __forceinline__ float TerracedRadii(const float* data, const tg_crate
 // kHarmonicBlock floats past its end.
 constexpr uint32_t kHarmonicBlock = 16u;
 
-__device__ __forceinline__ float FastOutlineScale(const float* harm, float4 ra,
-                                                  const tg_crater_record& c, float dx, float dy,
-                                                  float d, float share) {
+// The fast outline over `coef`, the crater's (A cos phi, A sin phi) pairs from harmonic
+// `first`, `count` of them, padded by kHarmonicBlock floats.
+__device__ __forceinline__ float FastOutline(const float* coef, float4 ra, float ellipticity,
+                                             uint32_t first, uint32_t count, float dx, float dy,
+                                             float d, float share) {
     float ct = 1.0f, st = 0.0f;
     if (d > 0.0f) {
         const float inv = 1.0f / d;
@@ -875,7 +877,7 @@ __device__ __forceinline__ float FastOutlineScale(const float* harm, float4 ra,
         st = dy * inv;
     }
     float scale = 1.0f;
-    if (c.ellipticity != 1.0f) {
+    if (ellipticity != 1.0f) {
         const float cd = ct * ra.x + st * ra.y;  // cos(theta - heading)
         const float sd = st * ra.x - ct * ra.y;  // sin(theta - heading)
         const float u = cd / ra.z;
@@ -883,11 +885,11 @@ __device__ __forceinline__ float FastOutlineScale(const float* harm, float4 ra,
         scale = 1.0f / sqrtf(u * u + v * v);
     }
     float wobble = 1.0f;
-    if (share > 0.0f && c.harmonic_count > 0u) {
+    if (share > 0.0f && count > 0u) {
         const bool partial = share < 1.0f;
         const float q = partial ? sqrtf(share) : 1.0f;
         float ck = 1.0f, sk = 0.0f, kept = 1.0f;
-        for (uint32_t k = 0; k < c.first_harmonic; ++k) {
+        for (uint32_t k = 0; k < first; ++k) {
             const float next = ck * ct - sk * st;
             sk = sk * ct + ck * st;
             ck = next;
@@ -896,19 +898,18 @@ __device__ __forceinline__ float FastOutlineScale(const float* harm, float4 ra,
         // The coefficients are read kHarmonicBlock floats at a time (one wide scalar load per
         // block rather than one per harmonic); the table is padded by a block, so the last block
         // stays in bounds. The sums are the same, in the same order.
-        const float* h = harm + c.harmonic_offset;
-        for (uint32_t base = 0u; base < c.harmonic_count; base += kHarmonicBlock / 2u) {
-            float coef[kHarmonicBlock];
+        for (uint32_t base = 0u; base < count; base += kHarmonicBlock / 2u) {
+            float block[kHarmonicBlock];
 #pragma unroll
             for (uint32_t i = 0u; i < kHarmonicBlock; ++i) {
-                coef[i] = h[2u * base + i];
+                block[i] = coef[2u * base + i];
             }
 #pragma unroll
             for (uint32_t j = 0u; j < kHarmonicBlock / 2u; ++j) {
-                if (base + j >= c.harmonic_count) {
+                if (base + j >= count) {
                     break;
                 }
-                const float wave = coef[2u * j] * ck - coef[2u * j + 1u] * sk;
+                const float wave = block[2u * j] * ck - block[2u * j + 1u] * sk;
                 wobble += (partial ? kept : 1.0f) * wave;
                 const float next = ck * ct - sk * st;
                 sk = sk * ct + ck * st;
@@ -918,6 +919,13 @@ __device__ __forceinline__ float FastOutlineScale(const float* harm, float4 ra,
         }
     }
     return scale * wobble;
+}
+
+__device__ __forceinline__ float FastOutlineScale(const float* harm, float4 ra,
+                                                  const tg_crater_record& c, float dx, float dy,
+                                                  float d, float share) {
+    return FastOutline(harm + c.harmonic_offset, ra, c.ellipticity, c.first_harmonic,
+                       c.harmonic_count, dx, dy, d, share);
 }
 
 // KernelProfileDistance (craters.h:1034-1054).
@@ -1194,13 +1202,15 @@ __device__ __forceinline__ float MpdHeightFrom(const MpdTerms& t, float distance
 
 // ApplyCrater for a crater whose terms say `fast`, with the outline's harmonic terms in `harm`
 // and `ra` as FastOutlineScale takes them, and the resolution weight for the grid.
-__device__ __forceinline__ float ApplyCraterMpd(const tg_crater_record& c, const MpdTerms& t,
-                                                float height, float dx, float dy,
-                                                const float* harm, float4 ra, float weight) {
+// The hoisted MPD crater from its terms, outline (ellipticity, harmonics from `first`, `count`
+// of them in `coef`, `ra`) and resolution weight.
+__device__ __forceinline__ float ApplyMpd(const MpdTerms& t, float ellipticity, uint32_t first,
+                                          uint32_t count, const float* coef, float4 ra,
+                                          float height, float dx, float dy, float weight) {
     float distance = sqrtf(dx * dx + dy * dy);
-    if (c.ellipticity != 1.0f || c.harmonic_count > 0u) {
+    if (ellipticity != 1.0f || count > 0u) {
         const float share = Smoothstep(distance / t.radius) * t.worn;
-        distance /= FastOutlineScale(harm, ra, c, dx, dy, distance, share);
+        distance /= FastOutline(coef, ra, ellipticity, first, count, dx, dy, distance, share);
     }
     if (distance >= t.reach) {
         return height;
@@ -1209,6 +1219,34 @@ __device__ __forceinline__ float ApplyCraterMpd(const tg_crater_record& c, const
     const float bare = MpdHeightFrom(t, distance) + 0.0f;
     const float shaped = bare + 0.0f + 0.0f;
     return height + weight * shaped + weight * 0.0f;
+}
+
+// A crater as the batched kernel's fast path reads it, in one 128-byte line: the split centre and
+// squared cull radius its walks test, the outline's scalars, the FP64 diameter of the resolution
+// fade and the hoisted MPD terms. The first kHotHarmonics outline harmonics sit in a slot of a
+// parallel table at the same index, so a visit's loads do not wait on one another.
+struct HotCrater {
+    float x, y, x_low, y_low;
+    float cull;
+    float ellipticity;
+    uint32_t first_harmonic;
+    uint32_t harmonic_count;
+    double diameter;
+    uint32_t harmonic_offset;  // into the padded harmonic table, for outlines longer than a slot
+    uint32_t pad;
+    float4 ra;                 // cos and sin of the heading, sqrt(ellipticity) and its reciprocal
+    MpdTerms terms;
+};
+static_assert(sizeof(HotCrater) == 128, "HotCrater is one 128-byte line");
+
+// Harmonics a crater's slot of the hot harmonic table holds: 32 floats, 128 bytes.
+constexpr uint32_t kHotHarmonics = 16u;
+
+__device__ __forceinline__ float ApplyCraterMpd(const tg_crater_record& c, const MpdTerms& t,
+                                                float height, float dx, float dy,
+                                                const float* harm, float4 ra, float weight) {
+    return ApplyMpd(t, c.ellipticity, c.first_harmonic, c.harmonic_count,
+                    harm + c.harmonic_offset, ra, height, dx, dy, weight);
 }
 
 }  // namespace tg

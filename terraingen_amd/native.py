@@ -143,7 +143,7 @@ HIP_ABI = {
     "tg_hip_valu_probe": (C.c_int, [VP, C.c_int, u32, u32, C.POINTER(C.c_float),
                                     C.POINTER(C.c_double), CP, SZ]),
 }
-TG_WALK_TOGETHER, TG_WALK_PER_LANE = 0, 1
+TG_WALK_TOGETHER, TG_WALK_PER_LANE, TG_WALK_CULLED = 0, 1, 2
 
 _hip = None
 

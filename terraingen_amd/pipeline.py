@@ -289,7 +289,8 @@ class TileBatch:
         """The staged batch's launch choices (tg_hip_batch_tune)."""
         self.dev.call("tg_hip_batch_tune", self.batch, walk, groups_per_cu, int(hoisted), chunk)
 
-    def autotune(self, out: N.DeviceArray, walks=(N.TG_WALK_TOGETHER, N.TG_WALK_PER_LANE),
+    def autotune(self, out: N.DeviceArray,
+                 walks=(N.TG_WALK_TOGETHER, N.TG_WALK_PER_LANE, N.TG_WALK_CULLED),
                  groups=(4, 8), chunks=(2, 8, 32), repeats: int = 2) -> dict:
         """Times every (crater walk, workgroups per CU, bins per queue take) on the staged batch
         with HIP events on its stream, keeps the fastest and returns {(walk, groups, chunk): ms}.

@@ -129,12 +129,16 @@ def test_stage_rejects_malformed_batches(hip, batch):
 
 
 def test_crater_walks_agree(hip, batch):
-    """Walking the bin lists together or per lane applies the same craters in the same order."""
+    """Walking the bin lists together, per lane or culled 64 entries at a time applies the same
+    craters in the same order."""
     batch.tune(N.TG_WALK_TOGETHER, 8)
     together = run_batch(hip, batch, use_graph=False)
     batch.tune(N.TG_WALK_PER_LANE, 4)
     per_lane = run_batch(hip, batch, use_graph=True)
     assert np.array_equal(together.view(np.uint32), per_lane.view(np.uint32))
+    batch.tune(N.TG_WALK_CULLED, 8)
+    culled = run_batch(hip, batch, use_graph=False)
+    assert np.array_equal(together.view(np.uint32), culled.view(np.uint32))
     batch.tune()
 
 
