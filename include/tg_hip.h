@@ -202,6 +202,14 @@ int tg_hip_batch_free(tg_batch* batch);
 int tg_hip_batch_stage(tg_batch* batch, const tg_tile_desc* tiles, size_t tile_count,
                        const tg_bin* bins, size_t bin_count, const uint32_t* entries,
                        size_t entry_count, char* err, size_t err_len);
+/* tg_hip_batch_stage with the copies queued on the batch's stream instead of waited for (no
+ * counterpart in the reference): for a serving loop that bins the next batch while others run.
+ * The host arrays must stay unchanged until the batch's next run has completed, and should be
+ * pinned (tg_hip_host_alloc) for the copies to be asynchronous. The batch must be idle: a batch
+ * whose stream still has work queued is refused. */
+int tg_hip_batch_stage_async(tg_batch* batch, const tg_tile_desc* tiles, size_t tile_count,
+                             const tg_bin* bins, size_t bin_count, const uint32_t* entries,
+                             size_t entry_count, char* err, size_t err_len);
 
 /* Evaluates the staged batch into `out_dev` (at least `out_capacity` floats): each cell's height
  * is `elevation` plus the plan's noise (none when `plan` is null) at its centre, then the
@@ -212,10 +220,10 @@ int tg_hip_batch_run(tg_batch* batch, const tg_noise_plan* plan, float elevation
                      const tg_catalogue* catalogue, float* out_dev, size_t out_capacity,
                      int use_graph, char* err, size_t err_len);
 /* Launch choices of a batch (no counterpart in the reference), for the autotuner. The crater
- * walk: TG_WALK_TOGETHER (the default), where a wave applies a listed crater whenever any of its
- * cells is in reach; TG_WALK_PER_LANE, where each cell walks the list to its own next crater in
- * reach; or TG_WALK_CULLED, where a wave first tests 64 entries at once against the box of its
- * cells and steps only through those that can reach one. Workgroups per CU of the persistent
+ * walk: TG_WALK_TOGETHER, where a wave applies a listed crater whenever any of its cells is in
+ * reach; TG_WALK_PER_LANE, where each cell walks the list to its own next crater in reach; or
+ * TG_WALK_CULLED (the default), where a wave first tests 64 entries at once against the box of
+ * its cells and steps only through those that can reach one. Workgroups per CU of the persistent
  * launch: 1-64 (default 8). `hoisted` (default 1) uses the per-crater constants of the MPD profile
  * path where a crater allows it; 0 evaluates every crater in full. `chunk`: bins a wave takes
  * from its queue per atomic add, 1-256 (default 8; fewer on batches too small to give every wave

@@ -41,6 +41,7 @@ struct tg_batch {
     tg_bin* bins = nullptr;
     uint32_t* entries = nullptr;
     uint32_t* counters = nullptr;  // the queues' next offsets, then the bin count (kCountSlot)
+    uint32_t* hostCounts = nullptr;  // pinned: the counters a staging copies in
     size_t maxTiles = 0, maxBins = 0, maxEntries = 0;
     // What the staged batch needs of a run: the largest entry and output end.
     size_t tileCount = 0, binCount = 0;
@@ -57,7 +58,7 @@ struct tg_batch {
     bool graphHoisted = true;
     uint32_t graphChunk = 0;
     // Launch choices (tg_hip_batch_tune).
-    uint32_t walk = 0;          // TG_WALK_TOGETHER
+    uint32_t walk = TG_WALK_CULLED;
     uint32_t groupsPerCu = 8;
     bool hoisted = true;        // the hoisted MPD path where a crater allows it
     uint32_t chunk = 8;         // bins a wave takes per atomic add
@@ -170,7 +171,7 @@ __device__ __forceinline__ float VisitCrater(uint32_t n, bool inside, float h, f
 // kAllFast: every crater of the catalogue takes the hoisted MPD path, so the full evaluation is
 // not compiled in; the kernel then holds far fewer registers and keeps more waves resident.
 template <uint32_t kWalk, bool kAllFast>
-__global__ void __launch_bounds__(kGroupThreads)
+__global__ void __launch_bounds__(kGroupThreads, kAllFast ? 8 : 1)
     BatchTiles(const tg_tile_desc* __restrict__ tiles, const tg_bin* __restrict__ bins,
                const uint32_t* __restrict__ entries, uint32_t* __restrict__ counters,
                const tg::NoisePlan* __restrict__ plan, float elevation,
@@ -517,6 +518,11 @@ int tg_hip_batch_create(tg_dev* dev, size_t maxTiles, size_t maxBins, size_t max
         b->bins = Alloc<tg_bin>(maxBins);
         b->entries = Alloc<uint32_t>(maxEntries);
         b->counters = UploadArray<uint32_t>(nullptr, kCounterWords);
+        void* counts = nullptr;
+        Check(hipHostMalloc(&counts, kCounterWords * sizeof(uint32_t), hipHostMallocDefault),
+              "hipHostMalloc");
+        b->hostCounts = static_cast<uint32_t*>(counts);
+        std::memset(b->hostCounts, 0, kCounterWords * sizeof(uint32_t));
         b->maxTiles = maxTiles;
         b->maxBins = maxBins;
         b->maxEntries = maxEntries;
@@ -546,6 +552,9 @@ int tg_hip_batch_free(tg_batch* b) {
             (void)hipFree(p);
         }
     }
+    if (b->hostCounts != nullptr) {
+        (void)hipHostFree(b->hostCounts);
+    }
     if (b->stream != nullptr) {
         (void)hipStreamDestroy(b->stream);
     }
@@ -553,12 +562,19 @@ int tg_hip_batch_free(tg_batch* b) {
     return TG_OK;
 }
 
-int tg_hip_batch_stage(tg_batch* b, const tg_tile_desc* tiles, size_t tileCount,
-                       const tg_bin* bins, size_t binCount, const uint32_t* entries,
-                       size_t entryCount, char* err, size_t errLen) {
+}  // extern "C"
+
+namespace {
+
+// Validates a staging and copies it to the batch: synchronously from any host memory, or, with
+// `async`, queued on the batch's stream from memory the caller keeps unchanged until the batch's
+// next run has completed.
+int Stage(tg_batch* b, const tg_tile_desc* tiles, size_t tileCount, const tg_bin* bins,
+          size_t binCount, const uint32_t* entries, size_t entryCount, bool async, char* err,
+          size_t errLen) {
     if (b == nullptr || (tileCount > 0 && tiles == nullptr) || (binCount > 0 && bins == nullptr) ||
         (entryCount > 0 && entries == nullptr)) {
-        SetErr(err, errLen, "tg_hip_batch_stage: null argument");
+        SetErr(err, errLen, "staging a batch: null argument");
         return TG_ERR_ARGUMENT;
     }
     return Guard(err, errLen, [&] {
@@ -596,29 +612,68 @@ int tg_hip_batch_stage(tg_batch* b, const tg_tile_desc* tiles, size_t tileCount,
             maxEntry = std::max<uint64_t>(maxEntry, entries[i]);
         }
         Check(hipSetDevice(b->dev->ordinal), "hipSetDevice");
-        Check(hipStreamSynchronize(b->stream), "waiting for the batch's previous run");
-        if (tileCount > 0) {
-            Check(hipMemcpy(b->tiles, tiles, tileCount * sizeof(tg_tile_desc),
-                            hipMemcpyHostToDevice), "staging tiles");
+        if (async) {
+            // The pinned counts are rewritten here, so no earlier staging's copy may be pending.
+            const hipError_t idle = hipStreamQuery(b->stream);
+            if (idle == hipErrorNotReady) {
+                throw HipError{"the batch is still running: wait for it before staging again"};
+            }
+            Check(idle, "hipStreamQuery");
+            const auto put = [&](void* dst, const void* src, size_t bytes, const char* what) {
+                if (bytes > 0) {
+                    Check(hipMemcpyAsync(dst, src, bytes, hipMemcpyHostToDevice, b->stream), what);
+                }
+            };
+            put(b->tiles, tiles, tileCount * sizeof(tg_tile_desc), "staging tiles");
+            put(b->bins, bins, binCount * sizeof(tg_bin), "staging bins");
+            put(b->entries, entries, entryCount * sizeof(uint32_t), "staging entries");
+            std::memset(b->hostCounts, 0, kCounterWords * sizeof(uint32_t));
+            b->hostCounts[kCountSlot] = uint32_t(binCount);
+            put(b->counters, b->hostCounts, kCounterWords * sizeof(uint32_t),
+                "staging the bin count");
+        } else {
+            Check(hipStreamSynchronize(b->stream), "waiting for the batch's previous run");
+            if (tileCount > 0) {
+                Check(hipMemcpy(b->tiles, tiles, tileCount * sizeof(tg_tile_desc),
+                                hipMemcpyHostToDevice),
+                      "staging tiles");
+            }
+            if (binCount > 0) {
+                Check(hipMemcpy(b->bins, bins, binCount * sizeof(tg_bin), hipMemcpyHostToDevice),
+                      "staging bins");
+            }
+            if (entryCount > 0) {
+                Check(hipMemcpy(b->entries, entries, entryCount * sizeof(uint32_t),
+                                hipMemcpyHostToDevice),
+                      "staging entries");
+            }
+            uint32_t counts[kCounterWords] = {};
+            counts[kCountSlot] = uint32_t(binCount);
+            Check(hipMemcpy(b->counters, counts, sizeof counts, hipMemcpyHostToDevice),
+                  "staging the bin count");
         }
-        if (binCount > 0) {
-            Check(hipMemcpy(b->bins, bins, binCount * sizeof(tg_bin), hipMemcpyHostToDevice),
-                  "staging bins");
-        }
-        if (entryCount > 0) {
-            Check(hipMemcpy(b->entries, entries, entryCount * sizeof(uint32_t),
-                            hipMemcpyHostToDevice), "staging entries");
-        }
-        uint32_t counts[kCounterWords] = {};
-        counts[kCountSlot] = uint32_t(binCount);
-        Check(hipMemcpy(b->counters, counts, sizeof counts, hipMemcpyHostToDevice),
-              "staging the bin count");
         b->tileCount = tileCount;
         b->binCount = binCount;
         b->maxEntry = maxEntry;
         b->anyEntries = entryCount > 0 && any;
         b->outEnd = outEnd;
     });
+}
+
+}  // namespace
+
+extern "C" {
+
+int tg_hip_batch_stage(tg_batch* b, const tg_tile_desc* tiles, size_t tileCount,
+                       const tg_bin* bins, size_t binCount, const uint32_t* entries,
+                       size_t entryCount, char* err, size_t errLen) {
+    return Stage(b, tiles, tileCount, bins, binCount, entries, entryCount, false, err, errLen);
+}
+
+int tg_hip_batch_stage_async(tg_batch* b, const tg_tile_desc* tiles, size_t tileCount,
+                             const tg_bin* bins, size_t binCount, const uint32_t* entries,
+                             size_t entryCount, char* err, size_t errLen) {
+    return Stage(b, tiles, tileCount, bins, binCount, entries, entryCount, true, err, errLen);
 }
 
 int tg_hip_batch_run(tg_batch* b, const tg_noise_plan* plan, float elevation,

@@ -19,6 +19,7 @@ struct HipApi {
     decltype(&tg_hip_batch_create) batch_create;
     decltype(&tg_hip_batch_free) batch_free;
     decltype(&tg_hip_batch_stage) batch_stage;
+    decltype(&tg_hip_batch_stage_async) batch_stage_async;
     decltype(&tg_hip_batch_run) batch_run;
     decltype(&tg_hip_batch_stream) batch_stream;
     decltype(&tg_hip_malloc) malloc;

@@ -6,11 +6,14 @@
 // thread owns a few slots, each a staged batch with its own stream, event and pinned staging
 // memory. Whenever a slot is free it admits the waiting requests in arrival order, as many as fit
 // the batch limits and the HBM output pool (the admission budget), bins them against the kept
-// catalogue index, stages the bins and launches one persistent kernel for the whole batch, then
-// copies the heights back asynchronously on the same stream. Batches in different slots overlap on
-// the device, and requests that arrive while others run are admitted as slots and pool memory
-// free up. A finished slot's heights are copied into each request's memory and its pool range is
-// returned. The crater catalogue, its index and the noise constants stay resident throughout.
+// catalogue index into the slot's pinned buffers, queues their copies, one persistent kernel for
+// the whole batch and the copy of the heights back, all on the slot's stream, and moves on. Batches
+// in different slots overlap on the device, and requests that arrive while others run are admitted
+// as slots and pool memory free up. A finished slot goes to copier threads, which copy its heights
+// into each request's memory and return the slot and its pool range, so the dispatcher never waits
+// on a copy. Buffers outgrown by a batch are kept to the end rather than freed, since freeing
+// device memory waits for the whole device. The crater catalogue, its index and the noise
+// constants stay resident throughout.
 //
 // Each tile's heights depend only on the tile and the resident inputs (its bins are its own and
 // every cell is independent), so a tile gives the same heights alone and in any mix.
@@ -115,8 +118,13 @@ struct Slot {
     void* stream = nullptr;
     void* event = nullptr;
     float* staging = nullptr;  // pinned, max_batch_cells floats
+    // Pinned copies of the staged batch, read by the copies queued on the stream.
+    tg_tile_desc* hostDescs = nullptr;
+    tg_bin* hostBins = nullptr;
+    std::uint32_t* hostEntries = nullptr;
     std::size_t maxTiles = 0, maxBins = 0, maxEntries = 0;
     bool busy = false;
+    bool copying = false;  // finished on the device, its heights with a copier
     std::vector<Request*> requests;  // in batch order
     std::vector<std::uint64_t> offsets;
     std::uint64_t poolOffset = 0, poolBytes = 0;
@@ -155,6 +163,14 @@ struct tgx_server {
     bool stop = false;
     tgx_server_stats stats{};
     std::thread dispatcher;
+    // Finished slots waiting for a copier, and the copiers.
+    std::deque<tgh::Slot*> copyQueue;
+    std::condition_variable copyWake;
+    bool copiersStop = false;
+    std::vector<std::thread> copiers;
+    // Batches and pinned buffers outgrown by a slot, freed with the server.
+    std::vector<tg_batch*> retiredBatches;
+    std::vector<void*> retiredPinned;
 };
 
 namespace tgh {
@@ -171,13 +187,29 @@ void FreeSlot(tgx_server& s, Slot& slot) {
         h.event_destroy(s.dev, slot.event);
         slot.event = nullptr;
     }
-    if (slot.staging != nullptr) {
-        h.host_free(s.dev, slot.staging);
-        slot.staging = nullptr;
+    for (void* p : {static_cast<void*>(slot.staging), static_cast<void*>(slot.hostDescs),
+                    static_cast<void*>(slot.hostBins), static_cast<void*>(slot.hostEntries)}) {
+        if (p != nullptr) {
+            h.host_free(s.dev, p);
+        }
     }
+    slot.staging = nullptr;
+    slot.hostDescs = nullptr;
+    slot.hostBins = nullptr;
+    slot.hostEntries = nullptr;
 }
 
-// (Re)creates the slot's batch with at least the given capacities; the slot is idle.
+template <class T> T* PinnedArray(tgx_server& s, std::size_t count) {
+    char err[512] = {0};
+    void* p = nullptr;
+    Raise(s.hip->host_alloc(s.dev, std::max<std::size_t>(count, 1) * sizeof(T), &p, err,
+                            sizeof err),
+          err);
+    return static_cast<T*>(p);
+}
+
+// Gives the slot (idle) a batch and pinned buffers of at least the given capacities, keeping
+// what it outgrows until the server is freed.
 void EnsureCapacity(tgx_server& s, Slot& slot, std::size_t tiles, std::size_t bins,
                     std::size_t entries) {
     if (slot.batch != nullptr && tiles <= slot.maxTiles && bins <= slot.maxBins &&
@@ -186,19 +218,29 @@ void EnsureCapacity(tgx_server& s, Slot& slot, std::size_t tiles, std::size_t bi
     }
     const HipApi& h = *s.hip;
     char err[512] = {0};
-    if (slot.batch != nullptr) {
-        h.batch_free(slot.batch);
-        slot.batch = nullptr;
-        slot.stream = nullptr;
-    }
+    // A slot that outgrows its buffers gets twice what the batch needs.
+    const bool grow = slot.batch != nullptr;
     const std::size_t t = std::max(tiles, slot.maxTiles);
-    const std::size_t b = std::max(bins + bins / 2, slot.maxBins);
-    const std::size_t e = std::max(entries + entries / 2 + 1024, slot.maxEntries);
-    Raise(h.batch_create(s.dev, t, b, e, &slot.batch, err, sizeof err), err);
+    const std::size_t b = std::max(grow && bins > slot.maxBins ? bins * 2 : bins, slot.maxBins);
+    const std::size_t e = std::max(
+        grow && entries > slot.maxEntries ? entries * 2 + 1024 : entries, slot.maxEntries);
+    tg_batch* batch = nullptr;
+    Raise(h.batch_create(s.dev, t, b, e, &batch, err, sizeof err), err);
+    if (slot.batch != nullptr) {
+        s.retiredBatches.push_back(slot.batch);
+        s.retiredPinned.insert(s.retiredPinned.end(),
+                               {static_cast<void*>(slot.hostDescs),
+                                static_cast<void*>(slot.hostBins),
+                                static_cast<void*>(slot.hostEntries)});
+    }
+    slot.batch = batch;
+    slot.stream = h.batch_stream(batch);
+    slot.hostDescs = PinnedArray<tg_tile_desc>(s, t);
+    slot.hostBins = PinnedArray<tg_bin>(s, b);
+    slot.hostEntries = PinnedArray<std::uint32_t>(s, e);
     slot.maxTiles = t;
     slot.maxBins = b;
     slot.maxEntries = e;
-    slot.stream = h.batch_stream(slot.batch);
 }
 
 // Bins, stages and launches one batch in an idle slot, then queues the copy back and the event.
@@ -214,8 +256,11 @@ void LaunchBatch(tgx_server& s, Slot& slot) {
     std::vector<std::uint32_t> entries;
     BinBatch(*s.index, tiles.data(), tiles.size(), s.config.bin_threads, descs, bins, entries);
     EnsureCapacity(s, slot, descs.size(), bins.size(), entries.size());
-    Raise(h.batch_stage(slot.batch, descs.data(), descs.size(), bins.data(), bins.size(),
-                        entries.data(), entries.size(), err, sizeof err),
+    std::memcpy(slot.hostDescs, descs.data(), descs.size() * sizeof(tg_tile_desc));
+    std::memcpy(slot.hostBins, bins.data(), bins.size() * sizeof(tg_bin));
+    std::memcpy(slot.hostEntries, entries.data(), entries.size() * sizeof(std::uint32_t));
+    Raise(h.batch_stage_async(slot.batch, slot.hostDescs, descs.size(), slot.hostBins,
+                              bins.size(), slot.hostEntries, entries.size(), err, sizeof err),
           err);
     std::uint64_t cells = 0;
     slot.offsets.resize(descs.size());
@@ -252,6 +297,7 @@ void FinishSlot(tgx_server& s, Slot& slot, int status, const std::string& error)
     slot.requests.clear();
     s.ranges->Give(slot.poolOffset, slot.poolBytes);
     slot.busy = false;
+    slot.copying = false;
 }
 
 // Takes waiting requests, in arrival order, for an idle slot: as many as fit the batch limits and
@@ -300,9 +346,9 @@ void Dispatch(tgx_server& s) {
     std::unique_lock<std::mutex> lock(s.mutex);
     for (;;) {
         bool progressed = false;
-        // Retire finished batches.
+        // Hand finished batches to the copiers.
         for (Slot& slot : s.slots) {
-            if (!slot.busy) {
+            if (!slot.busy || slot.copying) {
                 continue;
             }
             char err[512] = {0};
@@ -311,19 +357,14 @@ void Dispatch(tgx_server& s) {
                 continue;
             }
             if (q == TG_OK) {
-                lock.unlock();  // the slot and its requests' memory belong to the dispatcher
-                for (std::size_t i = 0; i < slot.requests.size(); ++i) {
-                    const Request* r = slot.requests[i];
-                    std::memcpy(r->dst, slot.staging + slot.offsets[i],
-                                std::size_t(r->tile.width) * r->tile.height * sizeof(float));
-                }
-                lock.lock();
-                FinishSlot(s, slot, TG_OK, "");
+                slot.copying = true;
+                s.copyQueue.push_back(&slot);
+                s.copyWake.notify_one();
             } else {
                 FinishSlot(s, slot, TG_ERR_DEVICE, err);
+                s.done.notify_all();
             }
             progressed = true;
-            s.done.notify_all();
         }
         std::uint32_t inFlight = 0;
         for (const Slot& slot : s.slots) {
@@ -369,6 +410,29 @@ void Dispatch(tgx_server& s) {
         } else {
             s.wake.wait(lock, [&] { return s.stop || !s.pending.empty(); });
         }
+    }
+}
+
+// A copier: copies each finished slot's heights into its requests' memory, then returns the slot.
+void Copy(tgx_server& s) {
+    std::unique_lock<std::mutex> lock(s.mutex);
+    for (;;) {
+        s.copyWake.wait(lock, [&] { return s.copiersStop || !s.copyQueue.empty(); });
+        if (s.copyQueue.empty()) {
+            return;  // stopping, and nothing left
+        }
+        Slot& slot = *s.copyQueue.front();
+        s.copyQueue.pop_front();
+        lock.unlock();  // the slot and its requests' memory belong to this copier now
+        for (std::size_t i = 0; i < slot.requests.size(); ++i) {
+            const Request* r = slot.requests[i];
+            std::memcpy(r->dst, slot.staging + slot.offsets[i],
+                        std::size_t(r->tile.width) * r->tile.height * sizeof(float));
+        }
+        lock.lock();
+        FinishSlot(s, slot, TG_OK, "");
+        s.done.notify_all();
+        s.wake.notify_all();  // a slot is free for the dispatcher
     }
 }
 
@@ -437,10 +501,17 @@ int tgx_server_create(tg_dev* dev, const tg_noise_plan* plan, float elevation,
                                           sizeof e),
                        e);
             slot.staging = static_cast<float*>(staging);
-            tgh::EnsureCapacity(*s, slot, std::min<std::size_t>(c.max_batch_tiles, 64), 4096,
-                                65536);
+            // Room for a full batch of cells in 16 x 16 bins with ragged edges, and a few
+            // craters a bin, so that growing (which keeps the old buffers) is rare.
+            const std::size_t bins = std::size_t(c.max_batch_cells / 256 +
+                                                 c.max_batch_cells / 1024 +
+                                                 std::uint64_t(c.max_batch_tiles) * 64);
+            tgh::EnsureCapacity(*s, slot, c.max_batch_tiles, bins, bins * 8);
         }
         s->dispatcher = std::thread([srv = s.get()] { tgh::Dispatch(*srv); });
+        for (int i = 0; i < 2; ++i) {
+            s->copiers.emplace_back([srv = s.get()] { tgh::Copy(*srv); });
+        }
     });
     if (code != TG_OK) {
         if (s->hip != nullptr) {
@@ -537,8 +608,22 @@ int tgx_server_free(tgx_server* s) {
     if (s->dispatcher.joinable()) {
         s->dispatcher.join();
     }
+    {
+        std::lock_guard<std::mutex> lock(s->mutex);
+        s->copiersStop = true;
+        s->copyWake.notify_all();
+    }
+    for (std::thread& t : s->copiers) {
+        t.join();
+    }
     for (tgh::Slot& slot : s->slots) {
         tgh::FreeSlot(*s, slot);
+    }
+    for (tg_batch* b : s->retiredBatches) {
+        s->hip->batch_free(b);
+    }
+    for (void* p : s->retiredPinned) {
+        s->hip->host_free(s->dev, p);
     }
     if (s->pool != nullptr) {
         s->hip->free(s->dev, s->pool);

@@ -112,6 +112,7 @@ HIP_ABI = {
     "tg_hip_batch_create": (C.c_int, [VP, SZ, SZ, SZ, C.POINTER(VP), CP, SZ]),
     "tg_hip_batch_free": (C.c_int, [VP]),
     "tg_hip_batch_stage": (C.c_int, [VP, VP, SZ, VP, SZ, VP, SZ, CP, SZ]),
+    "tg_hip_batch_stage_async": (C.c_int, [VP, VP, SZ, VP, SZ, VP, SZ, CP, SZ]),
     "tg_hip_batch_run": (C.c_int, [VP, VP, C.c_float, VP, VP, SZ, C.c_int, CP, SZ]),
     "tg_hip_batch_wait": (C.c_int, [VP, CP, SZ]),
     "tg_hip_batch_stream": (VP, [VP]),

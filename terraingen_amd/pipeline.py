@@ -284,7 +284,7 @@ class TileBatch:
     def stream(self):
         return self.dev.lib.tg_hip_batch_stream(self.batch)
 
-    def tune(self, walk: int = N.TG_WALK_TOGETHER, groups_per_cu: int = 8,
+    def tune(self, walk: int = N.TG_WALK_CULLED, groups_per_cu: int = 8,
              hoisted: bool = True, chunk: int = 8) -> None:
         """The staged batch's launch choices (tg_hip_batch_tune)."""
         self.dev.call("tg_hip_batch_tune", self.batch, walk, groups_per_cu, int(hoisted), chunk)

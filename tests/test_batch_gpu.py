@@ -110,6 +110,18 @@ def test_batched_noise_bit_exact_across_pixel_sizes(hip, orc, fractals):
         b.close()
 
 
+def test_async_staging_matches_staging(hip, batch):
+    """tg_hip_batch_stage_async queues the same copies on the batch's stream: the next run gives
+    the heights of a synchronous staging."""
+    want = run_batch(hip, batch, use_graph=False)
+    descs, bins, entries = (np.ascontiguousarray(a) for a in (batch.descs, batch.bins,
+                                                              batch.entries))
+    hip.call("tg_hip_batch_stage_async", batch.batch, descs.ctypes.data, descs.size,
+             bins.ctypes.data, bins.size, entries.ctypes.data, entries.size)
+    got = run_batch(hip, batch, use_graph=False)
+    assert np.array_equal(got.view(np.uint32), want.view(np.uint32))
+
+
 def test_stage_rejects_malformed_batches(hip, batch):
     bins = batch.bins.copy()
     bins["count"][0] = 10 ** 8  # runs past the entries
