@@ -422,7 +422,8 @@ def bench_grid(args, N, dev) -> dict:
     if c3:
         spec = CraterLayerSpec(sfd_coefficient=0.0156, sfd_exponent=2.2, min_diameter=0.5,
                                max_diameter=300.0, complex_=1, transition_diameter=50.0,
-                               peaks=1, seed=2)
+                               peaks=1, terraces=1, terrace_onset=150.0, terrace_full=300.0,
+                               seed=2)
         terrain = PlaneTerrain(dev, grid, [], 0.0, spec)
     else:
         terrain = PlaneTerrain(dev, grid, [C2_FRACTAL], 0.0, CraterLayerSpec(seed=1))
@@ -489,8 +490,9 @@ def bench_grid(args, N, dev) -> dict:
                    "boulders_drawn": int(terrain.boulder_count),
                    "boulder_pairs_in_reach": b_inside}
         config["workload"] = (f"flat source {n}x{n} @0.25 m + power-law craters N(>D)=0.0156 "
-                              "D^-2.2, D 0.5-300 m, complex from 50 m with central peaks "
-                              "(terraces off), seed 2 + boulders N(>D)=3.8e-3 D^-3, D 0.2-3 m, "
+                              "D^-2.2, D 0.5-300 m, complex from 50 m with central peaks, "
+                              "terraces from 150 m (full at 300 m), seed 2 + boulders "
+                              "N(>D)=3.8e-3 D^-3, D 0.2-3 m, "
                               "30 % around crater rims (synthetic field), dome relief")
     else:
         config["workload"] = (f"synthetic source {n}x{n} @0.5 m: 8-octave Perlin fBm (512 m, "

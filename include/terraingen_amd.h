@@ -105,6 +105,9 @@ typedef struct tgx_apply {
     double noise_rim_amplitude;
     double wall_streaks;
     double ejecta;
+    /* CraterTerraces onset and full diameters (m); both 0 keep the defaults, 20 and 65 km. */
+    double terrace_onset;
+    double terrace_full;
 } tgx_apply;
 
 /* The crater layer's margin in radii: CraterReachRadii x MaxOutlineScale

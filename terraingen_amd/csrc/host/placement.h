@@ -5,6 +5,7 @@
 
 #include "random.h"
 
+#include <algorithm>
 #include <cmath>
 #include <cstdint>
 #include <stdexcept>

@@ -12,8 +12,9 @@
 //               (irregularity.cpp:124-150,240-284);
 //   state       FreshnessDegradation with the lunar fresh morphometry
 //               (freshness_degradation.cpp:26-32, morphometry.cpp);
-//   records     KernelStateRecord, split centre, outline data, noise and texture keys, central
-//               peak massifs (crater_kernel.cpp:181-199,461-519,581-606).
+//   records     KernelStateRecord, split centre, outline data, noise and texture keys, wall
+//               terraces (terraces.cpp), central peak massifs
+//               (crater_kernel.cpp:181-199,312-519,581-606).
 
 #include "common.h"
 #include "placement.h"
@@ -242,6 +243,7 @@ ApplyOptions SeededApply(ApplyOptions o, std::uint32_t layerSeed) {
     o.noiseSeed = SubstreamSeed(layerSeed, kCraterNoiseStream);
     o.textureSeed = SubstreamSeed(layerSeed, kCraterTextureStream);
     o.peaks.seed = SubstreamSeed(layerSeed, kCraterPeakStream);
+    o.terraces.seed = SubstreamSeed(layerSeed, kCraterTerraceStream);
     return o;
 }
 
@@ -369,6 +371,9 @@ Batch PrepareBatch(const std::vector<tgh::Crater>& craters, const CraterModels& 
         r.table_offset = static_cast<std::uint32_t>(b.data.size());
         r.mark_count = 0;
         r.mark_offset = static_cast<std::uint32_t>(b.data.size());
+        AppendTerraces(c.id, c.diameter, fresh.depthRatio, fresh.rimHeightRatio,
+                       current.depthRatio, current.rimHeightRatio, o.terraces, b.kernel, r,
+                       b.data);
         if (m.complex_) {
             AppendPeaks(c, fresh, current, o.peaks, m.transitionDiameter, r, b.data);
         }

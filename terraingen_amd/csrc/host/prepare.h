@@ -34,9 +34,20 @@ struct Peaks {
            roughnessWavelength = 0.35, warp = 0.12, heightScale = 1.0, widthScale = 1.0;
 };
 
+// CraterTerraces (apply_options.h:184-231) with its defaults.
+struct Terraces {
+    bool enabled = false;
+    std::uint32_t seed = 0;
+    double onsetDiameter = 20000.0, fullDiameter = 65000.0;
+    int minBenches = 1, maxBenches = 4;
+    double benchSlopeDegrees = 3.0, scarpShare = 0.25, scarpSlopeDegrees = 35.0, arcShare = 0.6,
+           rimVariation = 0.08, blockOffset = 0.6, blockGaps = 0.25, minFreshness = 0.3;
+};
+
 struct ApplyOptions {
     Irregularity irregularity;
     Peaks peaks;
+    Terraces terraces;
     std::uint32_t noiseSeed = 0, textureSeed = 0;
     double rimSigma = 0.0, noiseAmplitude = 0.0, noiseRimAmplitude = 0.0, wallStreaks = 0.0,
            ejecta = 0.0, mantling = 0.0, ejectaReachRadii = 3.0;
@@ -78,8 +89,13 @@ inline ApplyOptions ApplyFrom(const tgx_apply& a) {
     out.noiseRimAmplitude = a.noise_rim_amplitude;
     out.wallStreaks = a.wall_streaks;
     out.ejecta = a.ejecta;
-    if (a.terraces != 0) {
-        throw std::invalid_argument("crater terraces are not prepared by this host yet");
+    out.terraces.enabled = a.terraces != 0;
+    if (a.terrace_onset > 0.0 || a.terrace_full > 0.0) {
+        if (!(a.terrace_onset > 0.0 && a.terrace_full > a.terrace_onset)) {
+            throw std::invalid_argument("terraces need 0 < onset diameter < full diameter");
+        }
+        out.terraces.onsetDiameter = a.terrace_onset;
+        out.terraces.fullDiameter = a.terrace_full;
     }
     return out;
 }
@@ -90,5 +106,14 @@ double CraterReachRadii(const CraterModels& m, const ApplyOptions& o);
 double MaxOutlineScale(const Irregularity& irr);
 Batch PrepareBatch(const std::vector<Crater>& craters, const CraterModels& m,
                    const ApplyOptions& o, double spacing);
+
+// CraterProfile::Height of the kernel's closed-form profile (terraces.cpp).
+double ProfileHeight(const tg_crater_shape& k, double distance, double diameter,
+                     double freshDepth, double freshRim, double currentDepth, double currentRim);
+// AppendKernelTerraces (crater_kernel.cpp:312-459) for crater `id` (terraces.cpp).
+void AppendTerraces(std::uint64_t id, double diameter, double freshDepth, double freshRim,
+                    double currentDepth, double currentRim, const Terraces& terraces,
+                    const tg_crater_shape& kernel, tg_crater_record& record,
+                    std::vector<float>& data);
 
 }  // namespace tgh

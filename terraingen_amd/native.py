@@ -419,7 +419,7 @@ CraterModels = _struct("CraterModels", [("profile", u32), ("complex_", u32),
                                         ("lifetime_gyr", C.c_double)])
 Apply = _struct("Apply", [("irregular", u32), ("peaks", u32), ("terraces", u32)] + [
     (n, C.c_double) for n in ("rim_sigma", "noise_amplitude", "noise_rim_amplitude",
-                              "wall_streaks", "ejecta")])
+                              "wall_streaks", "ejecta", "terrace_onset", "terrace_full")])
 
 HOST_ABI["tgx_margin_radii"] = (C.c_double, [C.POINTER(CraterModels), C.POINTER(Apply)])
 HOST_ABI["tgx_prepare_craters"] = (C.c_int, [

@@ -32,6 +32,9 @@ class CraterLayerSpec:
     transition_diameter: float = 15000.0
     irregular: int = 1
     peaks: int = 0
+    terraces: int = 0
+    terrace_onset: float = 0.0   # m; 0 with terrace_full 0 keeps the defaults (20 km, 65 km)
+    terrace_full: float = 0.0
     seed: int = 1
     rim_sigma: float = 0.0
     noise_amplitude: float = 0.0
@@ -44,8 +47,9 @@ class CraterLayerSpec:
                               self.lifetime_gyr)
 
     def apply(self) -> N.Apply:
-        return N.Apply(self.irregular, self.peaks, 0, self.rim_sigma, self.noise_amplitude,
-                       self.noise_rim_amplitude, self.wall_streaks, self.ejecta)
+        return N.Apply(self.irregular, self.peaks, self.terraces, self.rim_sigma,
+                       self.noise_amplitude, self.noise_rim_amplitude, self.wall_streaks,
+                       self.ejecta, self.terrace_onset, self.terrace_full)
 
 
 @dataclass

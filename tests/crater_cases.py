@@ -12,7 +12,7 @@ import numpy as np
 from terraingen_amd import native as N
 
 GOLDEN = Path(__file__).resolve().parent / "golden"
-NAMES = ["c1_default", "c2_powerlaw", "c3_complex", "features_ft"]
+NAMES = ["c1_default", "c2_powerlaw", "c3_complex", "features_ft", "terraces_mpd", "terraces_ft"]
 
 
 @dataclass

@@ -47,6 +47,17 @@ CASES = {
                          source_kind=1, fractal_count=1, craters=1, models=1,
                          sfd_coefficient=0.05, sfd_exponent=2.0, min_d=1.0, max_d=200.0,
                          lifetime=1.0, irregular=1, seed=1),
+    # Wall terraces on simple craters: a dense power-law field (N(>D) = 0.5 D^-2, D in [4, 50] m)
+    # terraced from 12 m (full at 40 m), at 0.25 m, with the Martin-Parkes-Dunstan profile and
+    # freshness degradation, then with the Fassett-Thomson profile.
+    "terraces_mpd": dict(width=257, height=257, pixel=0.25, center_x=-60.0, center_y=35.0,
+                         craters=1, models=1, sfd_coefficient=0.5, sfd_exponent=2.0, min_d=4.0,
+                         max_d=50.0, lifetime=1.0, irregular=1, terraces=1, terrace_onset=12.0,
+                         terrace_full=40.0, seed=11),
+    "terraces_ft": dict(width=257, height=257, pixel=0.25, center_x=75.0, center_y=10.0,
+                        craters=1, models=3, sfd_coefficient=0.5, sfd_exponent=2.0, min_d=4.0,
+                        max_d=50.0, lifetime=0.5, irregular=1, terraces=1, terrace_onset=12.0,
+                        terrace_full=40.0, seed=12),
 }
 
 C2_FRACTAL = dict(seed=1, wavelength=512.0, amplitude=20.0, octaves=8, lacunarity=2.0, gain=0.5)
@@ -84,7 +95,11 @@ def run_case(name: str, spec: dict) -> dict:
 
 def main() -> None:
     out = Path(__file__).resolve().parent
+    # Names on the command line write only those fixtures.
+    wanted = sys.argv[1:] or list(CASES)
     for name, spec in CASES.items():
+        if name not in wanted:
+            continue
         arrays = run_case(name, spec)
         np.savez_compressed(out / f"craters_{name}.npz", **arrays)
         print(f"{name}: {arrays['x'].size} craters, {arrays['records'].size // 144} records, "

@@ -18,8 +18,12 @@ CASES = {
     "features_ft": (N.CraterModels(0, 0, 15000.0, 0.2), N.Apply(1, 0, 0, 0.08, 0.01, 0.005,
                                                                 0.004, 0.3), 7, 0.5,
                     (0.05, 2.0), (2.0, 60.0), 0.2),
-    "c3_complex": (N.CraterModels(1, 1, 50.0, 1.0), N.Apply(1, 1, 0), 2, 0.25, (0.0156, 2.2),
-                   (0.5, 300.0), 1.0),
+    "c3_complex": (N.CraterModels(1, 1, 50.0, 1.0), N.Apply(1, 1, 1, 0, 0, 0, 0, 0, 150.0, 300.0),
+                   2, 0.25, (0.0156, 2.2), (0.5, 300.0), 1.0),
+    "terraces_mpd": (N.CraterModels(1, 0, 15000.0, 1.0), N.Apply(1, 0, 1, 0, 0, 0, 0, 0, 12.0, 40.0),
+                     11, 0.25, (0.5, 2.0), (4.0, 50.0), 1.0),
+    "terraces_ft": (N.CraterModels(0, 0, 15000.0, 0.5), N.Apply(1, 0, 1, 0, 0, 0, 0, 0, 12.0, 40.0),
+                    12, 0.25, (0.5, 2.0), (4.0, 50.0), 0.5),
 }
 
 
@@ -49,8 +53,10 @@ def test_generated_list_is_the_references(name):
         assert np.array_equal(got[col], z[key]), col
 
 
-@pytest.mark.parametrize("name", ["c2_powerlaw", "features_ft"])
+@pytest.mark.parametrize("name", list(CASES))
 def test_prepared_batch_is_the_references(name):
+    """Records and data byte for byte, including complex craters' central peaks and the wall
+    terraces of MPD, Fassett-Thomson and complex craters (AppendKernelTerraces)."""
     models, apply, seed, spacing, *_ = CASES[name]
     z = fixture(name)
     cr = np.zeros(z["x"].size, N.CRATER_DTYPE)
@@ -63,22 +69,26 @@ def test_prepared_batch_is_the_references(name):
     assert np.array_equal(foot[:, 3].astype(np.float32), records["diameter"])
 
 
-def test_prepared_peaks_are_the_references():
-    """Complex craters' central peak massifs (records and data up to the first terraced crater,
-    terraces are not restated here)."""
-    models, apply, seed, spacing, *_ = CASES["c3_complex"]
-    z = fixture("c3_complex")
+def test_terraced_fixtures_hold_terraces_and_peaks():
+    """The fixtures above exercise what they claim: terraced craters of every bench count and
+    central peaks."""
+    for name in ("c3_complex", "terraces_mpd", "terraces_ft"):
+        ref = fixture(name)["records"].view(N.RECORD_DTYPE)
+        assert (ref["terrace_count"] > 0).any(), name
+    counts = fixture("terraces_mpd")["records"].view(N.RECORD_DTYPE)["terrace_count"]
+    assert set(np.unique(counts)) >= {1, 2, 3, 4}
+    assert (fixture("c3_complex")["records"].view(N.RECORD_DTYPE)["peak_count"] > 0).any()
+
+
+def test_terraces_can_be_turned_off():
+    """Without terraces the same list gives records with no terrace data."""
+    models, _, seed, spacing, *_ = CASES["terraces_mpd"]
+    z = fixture("terraces_mpd")
     cr = np.zeros(z["x"].size, N.CRATER_DTYPE)
     cr["x"], cr["y"], cr["diameter"], cr["age_gyr"], cr["id"] = z["x"], z["y"], z["d"], z["age"], z["id"]
-    kernel, records, data, _ = N.prepare_craters(cr, models, apply, seed, spacing)
-    ref = z["records"].view(N.RECORD_DTYPE)
-    first_terraced = int(np.argmax(ref["terrace_count"] > 0))
-    assert ref["terrace_count"][first_terraced] > 0
-    assert (ref["peak_count"][:first_terraced] > 0).any() or (records["peak_count"] > 0).any()
-    n = first_terraced
-    assert records[:n].tobytes() == ref[:n].tobytes()
-    end = int(ref["harmonic_offset"][n])
-    assert np.array_equal(data[:end].view(np.uint32), z["data"][:end].view(np.uint32))
+    _, records, data, _ = N.prepare_craters(cr, models, N.Apply(1, 0, 0), seed, spacing)
+    assert (records["terrace_count"] == 0).all()
+    assert data.size < z["data"].size
 
 
 @pytest.mark.parametrize("counter,key,want", [
