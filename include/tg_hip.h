@@ -36,9 +36,14 @@ enum {
 enum {
     TG_KERNEL_NOISE_LAYER = 1, /* NoiseLayerKernel, src/layers/kernels/noise_layer.slang */
     TG_KERNEL_CRATERS = 2,     /* CraterShapeKernel, src/layers/kernels/craters.slang */
-    TG_KERNEL_BOULDERS = 3     /* tg_boulder_kernel: the dome relief BoulderLayer bakes on the
+    TG_KERNEL_BOULDERS = 3,    /* tg_boulder_kernel: the dome relief BoulderLayer bakes on the
                                 * CPU (boulder_relief.cpp:426-440); extras: boulders, tiles,
                                 * entries, lows */
+    TG_KERNEL_NOISE_BODY = 4   /* NoiseLayerKernel on a body: heights += EvaluateFractals3 at each
+                                * cell's body-fixed position (src/noise/surface_noise.cpp:9-20),
+                                * which the reference runs on the CPU only (noise_layer.cpp:113);
+                                * one extra, per cell: x, y, z as doubles. Unwarped Perlin
+                                * fractals. */
 };
 
 typedef struct tg_dev tg_dev;

@@ -84,6 +84,47 @@ template <class R> R Stack(R x, R y, const tg_fractal* fractals, std::size_t cou
     return height;
 }
 
+// EvaluateFractals3 (include/terraingen/noise/fractals.h:235-269) for unwarped Perlin fractals at
+// body-fixed (x, y, z) meters; NaN for a fractal this restatement does not cover (warped, or
+// another basis), so a test that reaches one fails.
+float Stack3(double x, double y, double z, const tg_fractal* fractals, std::size_t count) {
+    float height = 0.0f;
+    bool started = false;
+    for (std::size_t i = 0; i < count; ++i) {
+        const tg_fractal& layer = fractals[i];
+        if (!layer.enabled) {
+            continue;
+        }
+        if (layer.warp_amplitude > 0.0f || layer.basis != TG_BASIS_PERLIN) {
+            return std::nanf("");
+        }
+        Params unit = ParamsOf(layer);
+        unit.amplitude = 1.0f;
+        const float value = Fractal3(x, y, z, unit);
+        const float scaled = layer.amplitude * value;
+        if (!started) {
+            height = scaled;
+            started = true;
+            continue;
+        }
+        switch (layer.blend) {
+        case TG_BLEND_MAX:
+            height = std::max(height, scaled);
+            break;
+        case TG_BLEND_MIN:
+            height = std::min(height, scaled);
+            break;
+        case TG_BLEND_MASK:
+            height *= 0.5f * (value + 1.0f);
+            break;
+        default:
+            height += scaled;
+            break;
+        }
+    }
+    return height;
+}
+
 bool IsNoData(const tg_kernel_params& p, float h) {
     if (p.no_data == TG_NODATA_NAN) {
         return std::isnan(h);
@@ -174,6 +215,36 @@ float or_eval_fractals(float x, float y, const tg_fractal* fractals, std::size_t
 
 double or_eval_fractals_f64(double x, double y, const tg_fractal* fractals, std::size_t count) {
     return Stack<double>(x, y, fractals, count);
+}
+
+// Fractal3 of one fractal (amplitude included), and the stack, at body-fixed (x, y, z) meters.
+float or_fractal3(double x, double y, double z, const tg_fractal* f) {
+    if (f->basis != TG_BASIS_PERLIN) {
+        return std::nanf("");
+    }
+    return Fractal3(x, y, z, ParamsOf(*f));
+}
+
+float or_eval_fractals3(double x, double y, double z, const tg_fractal* fractals,
+                        std::size_t count) {
+    return Stack3(x, y, z, fractals, count);
+}
+
+// The noise of a terrain on a body, as SyntheticSource and the noise layer add it there
+// (src/noise/surface_noise.cpp:9-20): heights[i] += stack at positions[3 i .. 3 i + 2].
+void or_body_noise_grid(const tg_noise_layer_kernel* kernel, const double* positions,
+                        std::size_t cells, const tg_kernel_params* params, float* heights,
+                        unsigned threads) {
+    const std::size_t rows = (cells + 1023) / 1024;
+    ForRows(rows, threads, [&](std::size_t r) {
+        for (std::size_t i = r * 1024; i < std::min(cells, (r + 1) * 1024); ++i) {
+            if (!IsNoData(*params, heights[i])) {
+                const double* p = positions + 3 * i;
+                heights[i] = heights[i] + Stack3(p[0], p[1], p[2], kernel->fractals,
+                                                 kernel->count);
+            }
+        }
+    });
 }
 
 // The noise layer over a grid, as the reference's CPU path computes it (noise_layer.cpp:28-58):

@@ -57,6 +57,19 @@ float ref_fractal2(float x, float y, const tg_fractal* f) {
     return Fractal2(x, y, FractalParamsFrom(FromLayout(*f)));
 }
 
+float ref_fractal3(double x, double y, double z, const tg_fractal* f) {
+    return Fractal3(x, y, z, FractalParamsFrom(FromLayout(*f)));
+}
+
+float ref_eval_fractals3(double x, double y, double z, const tg_fractal* fractals,
+                         std::size_t count) {
+    std::vector<Fractal> list;
+    for (std::size_t i = 0; i < count; ++i) {
+        list.push_back(FromLayout(fractals[i]));
+    }
+    return EvaluateFractals3(x, y, z, list.data(), list.size());
+}
+
 float ref_eval_fractals(float x, float y, const tg_fractal* fractals, std::size_t count) {
     std::vector<Fractal> list;
     for (std::size_t i = 0; i < count; ++i) {

@@ -425,4 +425,136 @@ __device__ __forceinline__ float EvaluateStack(const NoisePlan& plan, float x, f
     return height;
 }
 
+// ---- Noise on a body ---------------------------------------------------------------------------
+
+// PerlinGradient3 (perlin.h:38-43), branch-free: the two terms picked by selects and signed by the
+// hash's low bits (negation is the sign-bit flip).
+__device__ __forceinline__ float Grad3(uint32_t hash, float x, float y, float z) {
+    const uint32_t h = hash & 15u;
+    const float u = h < 8u ? x : y;
+    const float v = h < 4u ? y : (h == 12u || h == 14u ? x : z);
+    const float su = __uint_as_float(__float_as_uint(u) ^ ((h & 1u) << 31u));
+    const float sv = __uint_as_float(__float_as_uint(v) ^ ((h & 2u) << 30u));
+    return su + sv;
+}
+
+// PerlinNoise3 (perlin.h:56-63,113-128) at cell (ix, iy, iz) and fraction (fx, fy, fz). A corner's
+// hash is the x word of pcg3d(x, y, z + PcgHash(seed)) (lattice.h:99-102); the LCG step of each
+// coordinate is linear in it, so the eight corners share their three pairs of LCG words.
+__device__ __forceinline__ float Perlin3(int32_t ix, int32_t iy, int32_t iz, float fx, float fy,
+                                         float fz, uint32_t zKey) {
+    const uint32_t ax0 = uint32_t(ix) * kLcgMul + kLcgAdd;
+    const uint32_t ay0 = uint32_t(iy) * kLcgMul + kLcgAdd;
+    const uint32_t az0 = (uint32_t(iz) + zKey) * kLcgMul + kLcgAdd;
+    const uint32_t ax1 = ax0 + kLcgMul, ay1 = ay0 + kLcgMul, az1 = az0 + kLcgMul;
+    const float c000 = Grad3(CornerX(ax0, ay0, az0, ay0 * az0), fx, fy, fz);
+    const float c100 = Grad3(CornerX(ax1, ay0, az0, ay0 * az0), fx - 1.0f, fy, fz);
+    const float c010 = Grad3(CornerX(ax0, ay1, az0, ay1 * az0), fx, fy - 1.0f, fz);
+    const float c110 = Grad3(CornerX(ax1, ay1, az0, ay1 * az0), fx - 1.0f, fy - 1.0f, fz);
+    const float c001 = Grad3(CornerX(ax0, ay0, az1, ay0 * az1), fx, fy, fz - 1.0f);
+    const float c101 = Grad3(CornerX(ax1, ay0, az1, ay0 * az1), fx - 1.0f, fy, fz - 1.0f);
+    const float c011 = Grad3(CornerX(ax0, ay1, az1, ay1 * az1), fx, fy - 1.0f, fz - 1.0f);
+    const float c111 = Grad3(CornerX(ax1, ay1, az1, ay1 * az1), fx - 1.0f, fy - 1.0f, fz - 1.0f);
+    const float u = Fade(fx);
+    const float v = Fade(fy);
+    const float w = Fade(fz);
+    const float nearSide = Mix(v, Mix(u, c000, c100), Mix(u, c010, c110));
+    const float farSide = Mix(v, Mix(u, c001, c101), Mix(u, c011, c111));
+    return StdClamp(Mix(w, nearSide, farSide), -1.0f, 1.0f);
+}
+
+// Fractal3 (fractal.h:284-317) for the Perlin basis at body-fixed (x, y, z) meters: the octave's
+// FP64 turn and frequency from the plan, lattice coordinates split in FP64 before narrowing
+// (SplitLatticePrecise3, lattice.h:59-69), the octave sums as the 2D loop's.
+__device__ __forceinline__ float FractalLoop3(const FractalLoopPlan& p, const BodyOctave* body,
+                                              double x, double y, double z) {
+    const uint32_t type = p.type;
+    float sum = 0.0f;
+    float feedback = 1.0f;
+    for (uint32_t o = 0; o < p.octaves; ++o) {
+        const BodyOctave& b = body[o];
+        const double rx = b.turn[0] * x + b.turn[1] * y + b.turn[2] * z;
+        const double ry = b.turn[3] * x + b.turn[4] * y + b.turn[5] * z;
+        const double rz = b.turn[6] * x + b.turn[7] * y + b.turn[8] * z;
+        const double px = rx * b.frequency + double(b.offsetX);
+        const double py = ry * b.frequency + double(b.offsetY);
+        const double pz = rz * b.frequency + double(b.offsetZ);
+        const double cx = floor(px);
+        const double cy = floor(py);
+        const double cz = floor(pz);
+        const float n = Perlin3(int32_t(int64_t(cx)), int32_t(int64_t(cy)), int32_t(int64_t(cz)),
+                                float(px - cx), float(py - cy), float(pz - cz), Pcg(b.seed));
+        const float weight = p.octave[o].weight;
+        switch (type) {
+        case TG_FRACTAL_BILLOW:
+            sum += weight * (2.0f * fabsf(n) - 1.0f);
+            break;
+        case TG_FRACTAL_RIDGED: {
+            float signal = p.offset - fabsf(n);
+            signal *= signal * feedback;
+            feedback = Clamp01(2.0f * signal);
+            sum += weight * signal;
+            break;
+        }
+        case TG_FRACTAL_HYBRID: {
+            const float signal = (n + p.offset) * weight;
+            if (o == 0) {
+                sum = signal;
+                feedback = signal;
+            } else {
+                feedback = Clamp01(feedback);
+                sum += feedback * signal;
+                feedback *= signal;
+            }
+            break;
+        }
+        default:
+            sum += weight * n;
+            break;
+        }
+    }
+    switch (type) {
+    case TG_FRACTAL_RIDGED: {
+        const float ridge = StdMax(p.offset, 1.0f - p.offset);
+        return p.amplitude * (2.0f * sum / (ridge * ridge * p.totalWeight) - 1.0f);
+    }
+    case TG_FRACTAL_HYBRID:
+        return p.amplitude * sum / ((1.0f + p.offset) * p.totalWeight);
+    default:
+        return p.amplitude * sum / p.totalWeight;
+    }
+}
+
+// EvaluateFractals3 (fractals.h:235-269) for a plan whose fractals are unwarped Perlin ones
+// (NoisePlan::bodyReady): the stack's height at body-fixed (x, y, z) meters.
+__device__ __forceinline__ float EvaluateStack3(const NoisePlan& plan, double x, double y,
+                                                double z) {
+    float height = 0.0f;
+    const uint32_t count = plan.count;
+    for (uint32_t i = 0; i < count; ++i) {
+        const FractalPlan& f = plan.fractal[i];
+        const float value = FractalLoop3(f.unit, plan.body[i], x, y, z);
+        const float scaled = f.amplitude * value;
+        if (i == 0) {
+            height = scaled;
+            continue;
+        }
+        switch (f.blend) {
+        case TG_BLEND_MAX:
+            height = StdMax(height, scaled);
+            break;
+        case TG_BLEND_MIN:
+            height = StdMin(height, scaled);
+            break;
+        case TG_BLEND_MASK:
+            height *= 0.5f * (value + 1.0f);
+            break;
+        default:
+            height += scaled;
+            break;
+        }
+    }
+    return height;
+}
+
 }  // namespace tg

@@ -44,6 +44,18 @@ struct FractalLoopPlan {
     OctavePlan octave[TG_MAX_OCTAVES];
 };
 
+// One octave of a fractal on a body (Fractal3, fractal.h:284-317), in FP64 where the reference is:
+// the octave's turn (the golden angle about (1, 2, 3), composed once per octave) and frequency.
+struct BodyOctave {
+    double turn[9];      // row-major
+    double frequency;    // 1 / wavelength times lacunarity per octave
+    float offsetX;       // HashToUnitFloat(Pcg3dHash(seed, octave, 0).y)
+    float offsetY;       // HashToUnitFloat(Pcg3dHash(seed, octave, 0).z)
+    float offsetZ;       // HashToUnitFloat(Pcg3dHash(seed, octave, 1).x)
+    uint32_t seed;       // Pcg3dHash(seed, octave, 0).x
+};
+static_assert(sizeof(BodyOctave) == 96, "BodyOctave is 12 doubles");
+
 // One enabled fractal of a stack.
 struct FractalPlan {
     uint32_t blend;
@@ -59,8 +71,9 @@ struct NoisePlan {
     uint32_t count;  // enabled fractals, at most TG_MAX_FRACTALS
     uint32_t maxOctaves;
     uint32_t allPerlinFbm;  // 1 when every loop is Perlin fBm without warp (the fast path)
-    uint32_t pad;
+    uint32_t bodyReady;     // 1 when every fractal is an unwarped Perlin one (the body path's)
     FractalPlan fractal[TG_MAX_FRACTALS];
+    BodyOctave body[TG_MAX_FRACTALS][TG_MAX_OCTAVES];  // each fractal's octaves on a body
 };
 
 // Pcg3dHash of (seed, octave, 0): include/terraingen/noise/hash.h:40-57, lattice.h:126-129.
@@ -124,11 +137,53 @@ inline void PlanLoop(FractalLoopPlan& loop, uint32_t seed, float wavelength, flo
 
 // The plan of a noise-layer kernel: its enabled fractals in order (noise_layer.h:41-50 keeps only
 // enabled ones; a disabled one leaves the stack unchanged, so skipping it is exact).
+// Fractal3's per-octave constants (fractal.h:284-317), with the reference's FP64 recurrences;
+// must be built with FP contraction off, as the reference is.
+inline void PlanBody(BodyOctave* octaves, uint32_t count, uint32_t seed, float wavelength,
+                     float lacunarity) {
+    static constexpr double kTurn[3][3] = {
+        {-0.6132711010727254, -0.293401570821183, 0.7333580809050306},
+        {0.7897926788435601, -0.24097777005594262, 0.5640542870894418},
+        {0.011228581128535142, 0.9251190369776896, 0.37951111497202883}};
+    double frequency = 1.0 / static_cast<double>(wavelength);
+    double turn[3][3] = {{1.0, 0.0, 0.0}, {0.0, 1.0, 0.0}, {0.0, 0.0, 1.0}};
+    for (uint32_t o = 0; o < count; ++o) {
+        uint32_t h[3] = {seed, o, 0u};
+        uint32_t g[3] = {seed, o, 1u};
+        HostPcg3(h);
+        HostPcg3(g);
+        BodyOctave& b = octaves[o];
+        for (int r = 0; r < 3; ++r) {
+            for (int c = 0; c < 3; ++c) {
+                b.turn[3 * r + c] = turn[r][c];
+            }
+        }
+        b.frequency = frequency;
+        b.offsetX = float(h[1] >> 8u) * 0x1p-24f;
+        b.offsetY = float(h[2] >> 8u) * 0x1p-24f;
+        b.offsetZ = float(g[0] >> 8u) * 0x1p-24f;
+        b.seed = h[0];
+        frequency *= static_cast<double>(lacunarity);
+        double next[3][3] = {};
+        for (int r = 0; r < 3; ++r) {
+            for (int c = 0; c < 3; ++c) {
+                next[r][c] = kTurn[r][0] * turn[0][c] + kTurn[r][1] * turn[1][c] +
+                             kTurn[r][2] * turn[2][c];
+            }
+        }
+        for (int r = 0; r < 3; ++r) {
+            for (int c = 0; c < 3; ++c) {
+                turn[r][c] = next[r][c];
+            }
+        }
+    }
+}
+
 inline void MakeNoisePlan(const tg_noise_layer_kernel& k, NoisePlan& plan) {
     plan.count = 0;
     plan.maxOctaves = 0;
     plan.allPerlinFbm = 1;
-    plan.pad = 0;
+    plan.bodyReady = 1;
     const uint32_t n = k.count < TG_MAX_FRACTALS ? k.count : TG_MAX_FRACTALS;
     for (uint32_t i = 0; i < n; ++i) {
         const tg_fractal& f = k.fractals[i];
@@ -158,6 +213,10 @@ inline void MakeNoisePlan(const tg_noise_layer_kernel& k, NoisePlan& plan) {
         if (p.unit.basis != TG_BASIS_PERLIN || p.unit.type != TG_FRACTAL_FBM) {
             plan.allPerlinFbm = 0;
         }
+        if (p.warped || p.unit.basis != TG_BASIS_PERLIN) {
+            plan.bodyReady = 0;
+        }
+        PlanBody(plan.body[plan.count - 1], p.unit.octaves, f.seed, f.wavelength, f.lacunarity);
         if (p.unit.octaves > plan.maxOctaves) {
             plan.maxOctaves = p.unit.octaves;
         }

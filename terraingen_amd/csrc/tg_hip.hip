@@ -71,6 +71,25 @@ __global__ void __launch_bounds__(kNoiseBlockX * kNoiseBlockY)
         elevation + tg::EvaluateStack(*plan, xs[col], ys[row]);
 }
 
+// The noise on a body (SyntheticSource::NoiseAt / NoiseLayer::ApplySamples on a body,
+// src/noise/surface_noise.cpp:9-20): heights[i] += EvaluateFractals3 at positions[i], or with
+// `fill`, heights[i] = elevation + it.
+__global__ void __launch_bounds__(256)
+    NoiseBodyStrip(float* __restrict__ heights, const double* __restrict__ positions,
+                   const tg::NoisePlan* __restrict__ plan, tg_kernel_params params, int fill,
+                   float elevation) {
+    const size_t i = size_t(blockIdx.x) * blockDim.x + threadIdx.x;
+    if (i >= size_t(params.width) * params.rows) {
+        return;
+    }
+    const float height = fill ? elevation : heights[i];
+    if (!fill && IsNoData(params, height)) {
+        return;
+    }
+    const double* p = positions + 3 * i;
+    heights[i] = height + tg::EvaluateStack3(*plan, p[0], p[1], p[2]);
+}
+
 void LaunchNoise(const tg::NoisePlan* plan, float* heights, const float* xs, const float* ys,
                  const tg_kernel_params& params, hipStream_t stream, bool fill = false,
                  float elevation = 0.0f) {
@@ -251,7 +270,7 @@ const char* tg_hip_device_name(const tg_dev* dev) {
 
 int tg_hip_can_run(const tg_dev* dev, int kernelId) {
     return dev != nullptr && (kernelId == TG_KERNEL_NOISE_LAYER || kernelId == TG_KERNEL_CRATERS ||
-                              kernelId == TG_KERNEL_BOULDERS);
+                              kernelId == TG_KERNEL_BOULDERS || kernelId == TG_KERNEL_NOISE_BODY);
 }
 
 int tg_hip_close(tg_dev* dev) {
@@ -273,9 +292,15 @@ int tg_hip_run(tg_dev* dev, int kernelId, const void* kernel, size_t kernelBytes
         SetErr(err, errLen, "tg_hip_run: null grid buffer");
         return TG_ERR_ARGUMENT;
     }
-    if (kernelId == TG_KERNEL_NOISE_LAYER) {
+    if (kernelId == TG_KERNEL_NOISE_LAYER || kernelId == TG_KERNEL_NOISE_BODY) {
         if (kernelBytes != sizeof(tg_noise_layer_kernel)) {
             SetErr(err, errLen, "tg_hip_run: a NoiseLayerKernel is 420 bytes");
+            return TG_ERR_ARGUMENT;
+        }
+        if (kernelId == TG_KERNEL_NOISE_BODY &&
+            (nExtra != 1 || extra == nullptr ||
+             (cells > 0 && (extra[0].data == nullptr || extra[0].bytes < cells * 3 * sizeof(double))))) {
+            SetErr(err, errLen, "tg_hip_run: the body noise binds one position (3 doubles) per cell");
             return TG_ERR_ARGUMENT;
         }
     } else if (kernelId == TG_KERNEL_CRATERS) {
@@ -323,6 +348,23 @@ int tg_hip_run(tg_dev* dev, int kernelId, const void* kernel, size_t kernelBytes
             DeviceArray<tg::NoisePlan> dPlan(1);
             dPlan.Upload(plan.get(), sizeof(tg::NoisePlan));
             LaunchNoise(dPlan.ptr, dHeights.ptr, dXs.ptr, dYs.ptr, *params, nullptr);
+        } else if (kernelId == TG_KERNEL_NOISE_BODY) {
+            tg_noise_layer_kernel k;
+            std::memcpy(&k, kernel, sizeof k);
+            auto plan = std::make_unique<tg::NoisePlan>();
+            tg::MakeNoisePlan(k, *plan);
+            if (!plan->bodyReady) {
+                throw HipError{"the body noise covers unwarped Perlin fractals only"};
+            }
+            DeviceArray<tg::NoisePlan> dPlan(1);
+            dPlan.Upload(plan.get(), sizeof(tg::NoisePlan));
+            DeviceArray<double> dPositions(cells * 3);
+            dPositions.Upload(extra[0].data, cells * 3 * sizeof(double));
+            const uint32_t block = 256;
+            hipLaunchKernelGGL(NoiseBodyStrip, dim3(uint32_t((cells + block - 1) / block)),
+                               dim3(block), 0, nullptr, dHeights.ptr, dPositions.ptr, dPlan.ptr,
+                               *params, 0, 0.0f);
+            Check(hipGetLastError(), "launching the body noise kernel");
         } else if (kernelId == TG_KERNEL_BOULDERS) {
             const tg_buf* b = extra;
             tg_boulder_set* set = nullptr;

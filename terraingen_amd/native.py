@@ -85,6 +85,7 @@ assert RECORD_DTYPE.itemsize == 144 and FRAME_DTYPE.itemsize == 52 and POINT_DTY
 TG_KERNEL_NOISE_LAYER = 1
 TG_KERNEL_CRATERS = 2
 TG_KERNEL_BOULDERS = 3
+TG_KERNEL_NOISE_BODY = 4
 NODATA_NONE, NODATA_NAN, NODATA_VALUE = 0, 1, 2
 
 # Every entry point include/tg_hip.h declares, with its ctypes signature.
