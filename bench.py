@@ -62,6 +62,15 @@ OPS_ELLIPSE = 12
 #   a (cell, boulder) pair inside the dome's reach: 40 (offset, hypot, two clamps, two
 #   sqrt + asin band areas, the max composite; boulder_relief.cpp:44-57,244-263).
 OPS_BOULDER_PAIR = 40
+#   on a sphere: a Perlin3 octave per cell: rotation 15 and scale 6 in FP64, lattice split 12,
+#   8 corners x (key 4, hash 20, offsets 3, gradient 18), fades 21, lerps 21, clamp 2, weighted
+#   sum 2 (fractal.h:284-317, perlin.h:38-63,113-128, hash.h:40-57, lattice.h:59-69); the cell's
+#   body-fixed position 15 (two sines, two cosines, five products; surface.cpp:45-50); a
+#   (cell, crater) pair adds to the plane's the frame offset, 35 (three split differences, two
+#   dot products, the chord, its arcsine ratio and the rescale; craters.h:168-209).
+OPS_PER_PERLIN3_OCTAVE = 440
+OPS_BODY_POSITION = 15
+OPS_BODY_PAIR_EXTRA = 35
 # MI355X's published FP32 vector peak, 157.3 TFLOP/s, in lane operations: an FMA counts 2 flops
 # and the peak assumes the packed two-lane form, so 157.3e12 / 2 lane-ops/s (256 CUs x 64 lanes x
 # 2 x 2.4 GHz).
@@ -71,6 +80,9 @@ METRIC = "Gcells/s batched vs plain HIP at <=1e-3 m err; p50/p99 tile latency"
 
 C2_FRACTAL = dict(seed=1, wavelength=512.0, amplitude=20.0, octaves=8, lacunarity=2.0, gain=0.5)
 TILES_SIDE = 4472.0  # m: 0.05 x 4472^2 = 1.0e6 craters of D >= 1 m
+MOON = 1737400.0  # m
+GLOBAL_CRATERS = 1_000_000
+GLOBAL_MAX_D = 20000.0  # m
 
 
 def roofline(ops: float, kernel_ms: float, bytes_written: float) -> dict:
@@ -237,6 +249,30 @@ def error_summary(pairs) -> tuple[dict, float]:
 
 # ---- workloads --------------------------------------------------------------------------------
 
+def time_launches(args, N, dev, batch, out) -> tuple[float, float, float]:
+    """W untimed graph launches, then K timed ones on the batch's stream with an event pair around
+    each: (ms per step between the first and last event, wall ms per step, mean launch ms)."""
+    stream = batch.stream
+    for _ in range(args.warmup):
+        batch.run(out, use_graph=True)
+    batch.wait()
+    events = [(N.Event(dev), N.Event(dev)) for _ in range(args.steps)]
+    start, stop = N.Event(dev), N.Event(dev)
+    t0 = time.perf_counter()
+    start.record(stream)
+    for a, b in events:
+        a.record(stream)
+        batch.run(out, use_graph=True)
+        b.record(stream)
+    stop.record(stream)
+    batch.wait()
+    wall_ms = (time.perf_counter() - t0) * 1e3 / args.steps
+    step_ms = start.elapsed_ms(stop) / args.steps
+    kernel_ms = float(np.mean([a.elapsed_ms(b) for a, b in events]))
+    print(f"[bench] batched {step_ms:.3f} ms/step", file=sys.stderr, flush=True)
+    return step_ms, wall_ms, kernel_ms
+
+
 def bench_tiles(args, N, dev) -> dict:
     from terraingen_amd.pipeline import CraterLayerSpec, TileBatch, c4_tiles, crater_pair_cost
 
@@ -259,25 +295,7 @@ def bench_tiles(args, N, dev) -> dict:
                   "ms": {f"{['together', 'per_lane', 'culled'][w]}/{g}/{c}": round(ms, 3)
                          for (w, g, c), ms in timings.items()}}
         print(f"[bench] autotune {tuning}", file=sys.stderr, flush=True)
-    for _ in range(args.warmup):
-        batch.run(out, use_graph=True)
-    batch.wait()
-    # Timed: K graph launches on the batch's stream, an event pair around each.
-    events = [(N.Event(dev), N.Event(dev)) for _ in range(args.steps)]
-    start, stop = N.Event(dev), N.Event(dev)
-    t0 = time.perf_counter()
-    start.record(stream)
-    for a, b in events:
-        a.record(stream)
-        batch.run(out, use_graph=True)
-        b.record(stream)
-    stop.record(stream)
-    batch.wait()
-    wall_ms = (time.perf_counter() - t0) * 1e3 / args.steps
-    step_ms = start.elapsed_ms(stop) / args.steps
-    launch_ms = [a.elapsed_ms(b) for a, b in events]
-    kernel_ms = float(np.mean(launch_ms))
-    print(f"[bench] batched {step_ms:.3f} ms/step", file=sys.stderr, flush=True)
+    step_ms, wall_ms, kernel_ms = time_launches(args, N, dev, batch, out)
     got = out.download(np.empty(batch.cells, np.float32))
 
     if args.copy_calibration_mib > 0:
@@ -361,6 +379,164 @@ def bench_tiles(args, N, dev) -> dict:
     batch.close()
     if args.serve_seconds > 0:
         result["serving"] = bench_serving(args, N, dev)
+    return result
+
+
+def cpu_sphere_tiles(batch, sample: list[int], cores: int) -> dict:
+    """The reference's CPU path over sampled tiles on the sphere, timed by the oracle
+    restatement: the body noise (EvaluateFractals3 per cell) then the crater kernel's body branch,
+    FP32 on all host threads. Also the FP64 oracle of the same tiles (FP64 positions, frames,
+    noise and shaping), for the error of the hip result."""
+    from oracle import oracle
+    from terraingen_amd import native as N
+
+    orc = oracle()
+    elapsed = 0.0
+    cells = 0
+    fp32, fp64 = {}, {}
+    for i in sample:
+        inp = batch.plain_inputs(i)
+        g = inp["grid"]
+        params = N.KernelParams(g.width, g.height, 0, 0.0, 0)
+        pos = inp["positions"]
+        out = np.zeros((g.height, g.width), np.float32)
+        t0 = time.perf_counter()
+        orc.or_body_noise_grid(C.byref(batch.noise_kernel), pos.ctypes.data, pos.shape[0],
+                               C.byref(params), out.ctypes.data, cores)
+        if inp["craters"] is not None:
+            kernel, records, data, tiles, entries, frames, points = inp["craters"]
+            orc.or_shape_grid_body(C.byref(kernel), records.ctypes.data, data.ctypes.data,
+                                   tiles.ctypes.data, entries.ctypes.data, frames.ctypes.data,
+                                   points.ctypes.data, g.width, g.height, C.byref(params),
+                                   out.ctypes.data, cores)
+        elapsed += time.perf_counter() - t0
+        cells += g.width * g.height
+        fp32[i] = out
+        ref = np.zeros((g.height, g.width), np.float64)
+        orc.or_body_noise_grid_f64(C.byref(batch.noise_kernel), pos.ctypes.data, pos.shape[0],
+                                   ref.ctypes.data, cores)
+        if inp["craters"] is not None:
+            foot = inp["foot"]
+            lam, phi = np.radians(foot[:, 0]), np.radians(foot[:, 1])
+            frames64 = np.ascontiguousarray(np.concatenate([
+                batch.radius * np.stack([np.cos(phi) * np.cos(lam), np.cos(phi) * np.sin(lam),
+                                         np.sin(phi)], axis=1),
+                np.stack([-np.sin(lam), np.cos(lam), np.zeros_like(lam)], axis=1),
+                np.stack([-np.sin(phi) * np.cos(lam), -np.sin(phi) * np.sin(lam), np.cos(phi)],
+                         axis=1),
+                np.full((lam.size, 1), batch.radius)], axis=1))
+            orc.or_shape_grid_body_f64(C.byref(kernel), records.ctypes.data, data.ctypes.data,
+                                       tiles.ctypes.data, entries.ctypes.data,
+                                       frames64.ctypes.data, pos.ctypes.data, g.width, g.height,
+                                       0, ref.ctypes.data, cores)
+        fp64[i] = ref
+    return {"value": round(cells / elapsed / 1e9, 6), "unit": "Gcells/s", "cores": cores,
+            "kind": "port",
+            "sample": f"{len(sample)} of the tiles ({cells} cells), body noise then craters, "
+                      f"once each, {platform.processor() or platform.machine()}",
+            "_fp32": fp32, "_fp64": fp64}
+
+
+def bench_sphere(args, N, dev) -> dict:
+    """The batched tiles on the Moon: tile origins uniform on a 1737.4 km sphere in FP64, each
+    cell's noise at its body-fixed position, a global catalogue of 1e6 craters."""
+    from terraingen_amd.pipeline import (CraterLayerSpec, SphereTileBatch, crater_pair_cost,
+                                         global_craters, sphere_min_diameter, sphere_tiles)
+
+    t0 = time.perf_counter()
+    craters = global_craters(GLOBAL_CRATERS, MOON, max_diameter=GLOBAL_MAX_D, seed=4)
+    batch = SphereTileBatch(dev, [C2_FRACTAL], CraterLayerSpec(seed=4), region=None, radius=MOON,
+                            craters_given=craters)
+    batch.build()
+    tiles = sphere_tiles(args.tiles, MOON, seed=4)
+    batch.stage(tiles)
+    host_ms = (time.perf_counter() - t0) * 1e3
+    print(f"[bench] staged {tiles.size} sphere tiles, {batch.cells} cells, "
+          f"{batch.records.size} craters, {batch.bins.size} bins, {batch.entries.size} entries",
+          file=sys.stderr, flush=True)
+    out = N.DeviceArray(dev, batch.cells * 4)
+    tuning = None
+    if not args.no_autotune:
+        timings = batch.autotune(out, walks=(N.TG_WALK_CULLED,))
+        tuning = {"chosen": {"groups_per_cu": batch.tuned[1], "chunk": batch.tuned[2]},
+                  "ms": {f"{g}/{c}": round(ms, 3) for (_, g, c), ms in timings.items()}}
+        print(f"[bench] autotune {tuning}", file=sys.stderr, flush=True)
+    step_ms, wall_ms, kernel_ms = time_launches(args, N, dev, batch, out)
+    got = out.download(np.empty(batch.cells, np.float32))
+
+    # The plain path over a sample of the tiles (every 4th): tg_hip_run per layer, host buffers.
+    plain_s = 0.0
+    plain_cells = 0
+    worst_plain = 0.0
+    for i in range(0, tiles.size if not args.no_plain else 0, 4):
+        inp = batch.plain_inputs(i)
+        t1 = time.perf_counter()
+        h = batch.plain_run(inp)
+        plain_s += time.perf_counter() - t1
+        plain_cells += h.size
+        worst_plain = max(worst_plain, float(np.abs(h - batch.tile_heights(got, i)).max()))
+        del inp
+    plain = plain_cells / plain_s / 1e9 if plain_cells else None
+    if plain is not None:
+        print(f"[bench] plain {plain:.3f} Gcells/s", file=sys.stderr, flush=True)
+
+    cells = batch.cells
+    ops_noise = cells * (C2_FRACTAL["octaves"] * OPS_PER_PERLIN3_OCTAVE + OPS_BODY_POSITION)
+    cost = crater_pair_cost(batch.records, OPS_PAIR + OPS_BODY_PAIR_EXTRA, OPS_PER_HARMONIC,
+                            OPS_ELLIPSE)
+    listed, inside, ops_craters = N.sphere_pair_ops(batch.descs, batch.bins, batch.entries,
+                                                    batch.foot, cost, MOON)
+    value = cells / (step_ms * 1e-3) / 1e9
+
+    extra = {}
+    if not args.no_cpu_baseline:
+        cores = os.cpu_count() or 1
+        budget = args.cpu_cells
+        sample, total = [], 0
+        for i in range(0, tiles.size, 7):
+            n = int(tiles["width"][i]) * int(tiles["height"][i])
+            if total + n > budget:
+                continue
+            sample.append(i)
+            total += n
+        base = cpu_sphere_tiles(batch, sample, cores)
+        fp32, fp64 = base.pop("_fp32"), base.pop("_fp64")
+        err, diff32 = error_summary((batch.tile_heights(got, i), fp32[i], fp64[i])
+                                    for i in sample)
+        extra["cpu_baseline"] = base
+        extra["error_vs_fp64_oracle_m"] = err
+        extra["max_diff_vs_fp32_oracle_m"] = diff32
+
+    d_min = sphere_min_diameter(GLOBAL_CRATERS, MOON, 0.05, 2.0, GLOBAL_MAX_D)
+    result = {
+        "metric": METRIC, "value": round(value, 3), "unit": "Gcells/s",
+        "ms_per_step": round(step_ms, 4), "wall_ms_per_step": round(wall_ms, 4),
+        "dtype": "f32+f64+u32",
+        "config": {"workload": f"{tiles.size} mixed tiles on a 1737.4 km sphere, edges 2^k+1 "
+                               "(k 6-11), pixel 0.25-8 m, north-west corners uniform on the "
+                               "sphere, FP64 body-fixed cell positions; 8-octave 3D Perlin fBm "
+                               "(512 m, lac 2, gain 0.5, 20 m, seed 1) + a global catalogue of "
+                               f"{GLOBAL_CRATERS} power-law craters N(>D)=0.05 D^-2, D "
+                               f"{d_min:.0f}-{GLOBAL_MAX_D:.0f} m, seed 4",
+                   "tiles": int(tiles.size), "cells": cells,
+                   "craters": int(batch.records.size), "bins": int(batch.bins.size),
+                   "entries": int(batch.entries.size), "pairs_listed": listed,
+                   "pairs_in_reach": inside},
+        "plain": {"value": round(plain, 4) if plain else None, "unit": "Gcells/s",
+                  "path": "tg_hip_run per tile and layer (body noise, crater body branch), host "
+                          "buffers, synchronous, over every 4th tile",
+                  "max_diff_vs_batched_m": worst_plain if plain else None},
+        "batched_over_plain": round(value / plain, 2) if plain else None,
+        "kernels_ms": {"batch_body_tiles": round(kernel_ms, 4)},
+        "autotune": tuning,
+        "host_ms": {k: round(v, 1) for k, v in batch.timings.items()} | {
+            "total_setup": round(host_ms, 1)},
+        "roofline": roofline(ops_noise + ops_craters, kernel_ms, cells * 4.0) | {
+            "kernel": "BatchBodyTiles", "ops_noise": ops_noise, "ops_craters": ops_craters},
+    }
+    result["roofline"]["valu_ceiling_measured"] = valu_ceiling(N, dev)
+    result |= extra
+    batch.close()
     return result
 
 
@@ -526,7 +702,7 @@ def main() -> int:
     ap.add_argument("--gpus", type=int, default=1)
     ap.add_argument("--steps", type=int, default=10)
     ap.add_argument("--warmup", type=int, default=2)
-    ap.add_argument("--workload", choices=("tiles", "grid", "c3"), default="tiles")
+    ap.add_argument("--workload", choices=("tiles", "sphere", "grid", "c3"), default="tiles")
     ap.add_argument("--tiles", type=int, default=1024)
     ap.add_argument("--size", type=int, default=0,
                     help="grid workload edge (0: 4097 for grid, 8193 for c3)")
@@ -552,6 +728,8 @@ def main() -> int:
     dev = N.HipDevice(0)
     if args.workload == "tiles":
         body = bench_tiles(args, N, dev)
+    elif args.workload == "sphere":
+        body = bench_sphere(args, N, dev)
     else:
         body = bench_grid(args, N, dev)
     result = {"metric": body.pop("metric"), "value": body.pop("value"), "unit": body.pop("unit"),

@@ -189,6 +189,43 @@ int tgx_batch_pair_ops(const tg_tile_desc* descs, size_t tile_count, const tg_bi
                        const double* cost, size_t n, unsigned threads, uint64_t* listed,
                        uint64_t* inside, double* ops, char* err, size_t err_len);
 
+/* ---- Tiles on a sphere ---------------------------------------------------------------------
+ * Longitude-latitude rasters on a sphere of `radius` meters: `west` and `north` are a tile's edges
+ * and `step` its cell size, in degrees; cells run east then south from the north-west one. */
+
+/* Surface::Position of each cell centre (src/geometry/surface.cpp:45-50,261-267), width x height
+ * x 3 doubles. The sine and cosine are the library's own, the same bits as the batched kernel's on
+ * the device, within about an ulp of the C library's. */
+int tgx_sphere_positions(double west, double north, double step, uint32_t width, uint32_t height,
+                         double radius, double* xyz, char* err, size_t err_len);
+/* MakeSplitPoint of each cell centre (crater_kernel.cpp:303-310) from those positions. */
+int tgx_sphere_split_points(double west, double north, double step, uint32_t width,
+                            uint32_t height, double radius, tg_split_point* out, char* err,
+                            size_t err_len);
+/* MakeCraterFrame (crater_kernel.cpp:285-301) of craters centred at (lon, lat) degrees, in FP64
+ * with the C library's sine and cosine as the reference computes them. */
+int tgx_sphere_frames(const double* lon, const double* lat, size_t n, double radius,
+                      tg_crater_frame* out, char* err, size_t err_len);
+/* tgx_index_bin for tiles on a sphere (tgx_tile's origin and pixel in degrees): crater k, centred
+ * at (lon[k], lat[k]) with footprint half-width reach[k] meters of arc, joins every 16 x 16 bin of
+ * a tile with a cell centre within reach of it, when it is resolved at the tile's spacing (pixel
+ * times radius times pi / 180, RasterGrid::Spacing, src/raster/grid.cpp:28-33), in catalogue
+ * order. The bounds are conservative: a bin may list a crater none of its cells reaches. */
+int tgx_sphere_bin(const double* lon, const double* lat, const double* reach,
+                   const double* diameter, size_t n, double radius, const tgx_tile* tiles,
+                   size_t tile_count, unsigned threads, tg_tile_desc** descs, tg_bin** bins,
+                   size_t* bin_count, uint32_t** entries, size_t* entry_count, char* err,
+                   size_t err_len);
+/* tgx_batch_pair_ops for tiles on a sphere (the output of tgx_sphere_bin): the (cell, listed
+ * crater) pairs whose cell centre lies strictly within the crater's reach along the sphere. A
+ * measurement aid with no counterpart in the reference; it never touches a device. */
+int tgx_sphere_pair_ops(const tg_tile_desc* descs, size_t tile_count, const tg_bin* bins,
+                        size_t bin_count, const uint32_t* entries, size_t entry_count,
+                        const double* lon, const double* lat, const double* reach,
+                        const double* cost, size_t n, double radius, unsigned threads,
+                        uint64_t* listed, uint64_t* inside, double* ops, char* err,
+                        size_t err_len);
+
 /* ---- The tile server: continuous batching onto the batched kernel ---------------------------
  * No counterpart in the reference, which evaluates one grid per call. Requests (a tile and the host
  * memory its heights go to) queue as they arrive; a dispatcher thread admits them, in arrival order,

@@ -169,6 +169,10 @@ int tg_hip_catalogue_create(tg_dev* dev, const tg_crater_shape* kernel,
                             const float* data, size_t data_count, const double* diameters,
                             tg_catalogue** out, char* err, size_t err_len);
 int tg_hip_catalogue_free(tg_catalogue* catalogue);
+/* The crater frames (MakeCraterFrame, crater_kernel.cpp:285-301; tgx_sphere_frames makes them)
+ * of a catalogue for tiles on a sphere: one per crater, in catalogue order, set once. */
+int tg_hip_catalogue_frames(tg_catalogue* catalogue, const tg_crater_frame* frames, size_t count,
+                            char* err, size_t err_len);
 
 /* One tile: a north-up grid `width` x `height` of `pixel` meters, west and north edges in FP64
  * (cell centres are worked out in FP64 on the device and narrowed as RunKernel narrows them,
@@ -236,6 +240,15 @@ int tg_hip_batch_run(tg_batch* batch, const tg_noise_plan* plan, float elevation
 enum { TG_WALK_TOGETHER = 0, TG_WALK_PER_LANE = 1, TG_WALK_CULLED = 2 };
 int tg_hip_batch_tune(tg_batch* batch, uint32_t crater_walk, uint32_t groups_per_cu,
                       uint32_t hoisted, uint32_t chunk, char* err, size_t err_len);
+/* The surface of a batch's tiles: 0 (the default) for a plane, else a sphere of `sphere_radius`
+ * meters whose tiles are longitude-latitude rasters (a tile's origin_x, origin_y and pixel are its
+ * west and north edges and cell size in degrees). On a sphere each cell's noise is the stack's at
+ * its body-fixed position (EvaluateFractals3, src/noise/surface_noise.cpp:9-20; the plan's
+ * fractals must be unwarped Perlin ones, as tg_hip_run(TG_KERNEL_NOISE_BODY) requires), craters
+ * are placed by their frames (tg_hip_catalogue_frames) and weighted at the tile's spacing, pixel
+ * times radius times pi / 180 (RasterGrid::Spacing, src/raster/grid.cpp:28-33). The bins come
+ * from tgx_sphere_bin. The crater walk is always the culled one. */
+int tg_hip_batch_surface(tg_batch* batch, double sphere_radius, char* err, size_t err_len);
 int tg_hip_batch_wait(tg_batch* batch, char* err, size_t err_len);
 /* The batch's stream, for events and copies ordered with its runs. */
 void* tg_hip_batch_stream(tg_batch* batch);

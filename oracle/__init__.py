@@ -104,8 +104,11 @@ def _declare(lib: C.CDLL) -> None:
         "or_eval_fractals_f64": (d, [d, d, fp, sz]),
         "or_fractal3": (f, [d, d, d, fp]), "ref_fractal3": (f, [d, d, d, fp]),
         "or_eval_fractals3": (f, [d, d, d, fp, sz]), "ref_eval_fractals3": (f, [d, d, d, fp, sz]),
+        "or_eval_fractals3_f64": (d, [d, d, d, fp, sz]),
         "or_body_noise_grid": (None, [C.POINTER(N.NoiseLayerKernel), vp, sz,
                                       C.POINTER(N.KernelParams), vp, C.c_uint]),
+        "or_body_noise_grid_f64": (None, [C.POINTER(N.NoiseLayerKernel), vp, sz, vp,
+                                          C.c_uint]),
         "ref_make_noise_layer_kernel": (None, [fp, sz, vp]),
         "or_noise_grid": (None, [C.POINTER(N.NoiseLayerKernel), vp, vp, sz, sz,
                                  C.POINTER(N.KernelParams), vp, C.c_uint]),

@@ -306,41 +306,42 @@ template <class R> inline R Fractal2(R x, R y, const Params& p) {
 // ---- 3D noise on a body (Perlin basis) ---------------------------------------------------------
 
 // PerlinGradient3 (include/terraingen/noise/perlin.h:38-43).
-inline float Grad3(u32 hash, float x, float y, float z) {
+template <class R> inline R Grad3(u32 hash, R x, R y, R z) {
     const u32 h = hash & 15u;
-    const float u = h < 8u ? x : y;
-    const float v = h < 4u ? y : (h == 12u || h == 14u ? x : z);
+    const R u = h < 8u ? x : y;
+    const R v = h < 4u ? y : (h == 12u || h == 14u ? x : z);
     return ((h & 1u) == 0u ? u : -u) + ((h & 2u) == 0u ? v : -v);
 }
 
 // PerlinNoise3 over a split lattice point (perlin.h:56-63,113-128): each corner's hash is the x
 // word of pcg3d(x, y, z + PcgHash(seed)) (PerlinLatticeKey3, lattice.h:99-102).
-inline float Perlin3(std::int32_t ix, std::int32_t iy, std::int32_t iz, float fx, float fy,
-                     float fz, u32 seed) {
+template <class R>
+inline R Perlin3(std::int32_t ix, std::int32_t iy, std::int32_t iz, R fx, R fy, R fz, u32 seed) {
     const u32 zKey = Pcg(seed);
-    float c[8];
+    R c[8];
     for (u32 corner = 0; corner < 8u; ++corner) {
         const u32 dx = corner & 1u, dy = (corner >> 1u) & 1u, dz = corner >> 2u;
         const u32 hash = Pcg3({u32(ix) + dx, u32(iy) + dy, u32(iz) + dz + zKey}).x;
-        c[corner] = Grad3(hash, fx - float(dx), fy - float(dy), fz - float(dz));
+        c[corner] = Grad3<R>(hash, fx - R(dx), fy - R(dy), fz - R(dz));
     }
-    const float u = Fade(fx), v = Fade(fy), w = Fade(fz);
-    const float nearSide = Mix(v, Mix(u, c[0], c[1]), Mix(u, c[2], c[3]));
-    const float farSide = Mix(v, Mix(u, c[4], c[5]), Mix(u, c[6], c[7]));
-    return std::clamp(Mix(w, nearSide, farSide), -1.0f, 1.0f);
+    const R u = Fade(fx), v = Fade(fy), w = Fade(fz);
+    const R nearSide = Mix(v, Mix(u, c[0], c[1]), Mix(u, c[2], c[3]));
+    const R farSide = Mix(v, Mix(u, c[4], c[5]), Mix(u, c[6], c[7]));
+    return std::clamp(Mix(w, nearSide, farSide), R(-1), R(1));
 }
 
 // Fractal3 (include/terraingen/noise/fractal.h:284-317) for the Perlin basis: FP64 turns about the
 // axis (1, 2, 3), FP64 frequency by repeated multiplication, FP64 lattice coordinates split before
-// narrowing (SplitLatticePrecise3, lattice.h:59-69), FP32 octave sums.
-inline float Fractal3(double x, double y, double z, const Params& p) {
+// narrowing (SplitLatticePrecise3, lattice.h:59-69), octave sums in R (FP32 as the reference;
+// FP64 for the FP64 oracle, where the fractions are not narrowed either).
+template <class R = float> inline R Fractal3(double x, double y, double z, const Params& p) {
     static constexpr double kTurn[3][3] = {
         {-0.6132711010727254, -0.293401570821183, 0.7333580809050306},
         {0.7897926788435601, -0.24097777005594262, 0.5640542870894418},
         {0.011228581128535142, 0.9251190369776896, 0.37951111497202883}};
     double frequency = 1.0 / static_cast<double>(p.wavelength);
     double turn[3][3] = {{1.0, 0.0, 0.0}, {0.0, 1.0, 0.0}, {0.0, 0.0, 1.0}};
-    Sums<float> sums;
+    Sums<R> sums;
     for (int o = 0; o < p.octaves; ++o) {
         const U3 h = Pcg3({p.seed, u32(o), 0u});
         const U3 g = Pcg3({p.seed, u32(o), 1u});
@@ -351,10 +352,10 @@ inline float Fractal3(double x, double y, double z, const Params& p) {
         const double py = ry * frequency + static_cast<double>(UnitFloat(h.z));
         const double pz = rz * frequency + static_cast<double>(UnitFloat(g.x));
         const double cx = std::floor(px), cy = std::floor(py), cz = std::floor(pz);
-        const float n = Perlin3(std::int32_t(static_cast<std::int64_t>(cx)),
-                                std::int32_t(static_cast<std::int64_t>(cy)),
-                                std::int32_t(static_cast<std::int64_t>(cz)), float(px - cx),
-                                float(py - cy), float(pz - cz), h.x);
+        const R n = Perlin3<R>(std::int32_t(static_cast<std::int64_t>(cx)),
+                               std::int32_t(static_cast<std::int64_t>(cy)),
+                               std::int32_t(static_cast<std::int64_t>(cz)), R(px - cx),
+                               R(py - cy), R(pz - cz), h.x);
         AddOctave(sums, p, o, n);
         frequency *= static_cast<double>(p.lacunarity);
         double next[3][3] = {};

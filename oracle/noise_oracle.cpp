@@ -85,10 +85,11 @@ template <class R> R Stack(R x, R y, const tg_fractal* fractals, std::size_t cou
 }
 
 // EvaluateFractals3 (include/terraingen/noise/fractals.h:235-269) for unwarped Perlin fractals at
-// body-fixed (x, y, z) meters; NaN for a fractal this restatement does not cover (warped, or
-// another basis), so a test that reaches one fails.
-float Stack3(double x, double y, double z, const tg_fractal* fractals, std::size_t count) {
-    float height = 0.0f;
+// body-fixed (x, y, z) meters, in precision R; NaN for a fractal this restatement does not cover
+// (warped, or another basis), so a test that reaches one fails.
+template <class R>
+R Stack3(double x, double y, double z, const tg_fractal* fractals, std::size_t count) {
+    R height = R(0);
     bool started = false;
     for (std::size_t i = 0; i < count; ++i) {
         const tg_fractal& layer = fractals[i];
@@ -96,12 +97,12 @@ float Stack3(double x, double y, double z, const tg_fractal* fractals, std::size
             continue;
         }
         if (layer.warp_amplitude > 0.0f || layer.basis != TG_BASIS_PERLIN) {
-            return std::nanf("");
+            return R(std::nan(""));
         }
         Params unit = ParamsOf(layer);
         unit.amplitude = 1.0f;
-        const float value = Fractal3(x, y, z, unit);
-        const float scaled = layer.amplitude * value;
+        const R value = Fractal3<R>(x, y, z, unit);
+        const R scaled = R(layer.amplitude) * value;
         if (!started) {
             height = scaled;
             started = true;
@@ -115,7 +116,7 @@ float Stack3(double x, double y, double z, const tg_fractal* fractals, std::size
             height = std::min(height, scaled);
             break;
         case TG_BLEND_MASK:
-            height *= 0.5f * (value + 1.0f);
+            height *= R(0.5) * (value + R(1));
             break;
         default:
             height += scaled;
@@ -222,12 +223,17 @@ float or_fractal3(double x, double y, double z, const tg_fractal* f) {
     if (f->basis != TG_BASIS_PERLIN) {
         return std::nanf("");
     }
-    return Fractal3(x, y, z, ParamsOf(*f));
+    return Fractal3<float>(x, y, z, ParamsOf(*f));
 }
 
 float or_eval_fractals3(double x, double y, double z, const tg_fractal* fractals,
                         std::size_t count) {
-    return Stack3(x, y, z, fractals, count);
+    return Stack3<float>(x, y, z, fractals, count);
+}
+
+double or_eval_fractals3_f64(double x, double y, double z, const tg_fractal* fractals,
+                             std::size_t count) {
+    return Stack3<double>(x, y, z, fractals, count);
 }
 
 // The noise of a terrain on a body, as SyntheticSource and the noise layer add it there
@@ -240,9 +246,21 @@ void or_body_noise_grid(const tg_noise_layer_kernel* kernel, const double* posit
         for (std::size_t i = r * 1024; i < std::min(cells, (r + 1) * 1024); ++i) {
             if (!IsNoData(*params, heights[i])) {
                 const double* p = positions + 3 * i;
-                heights[i] = heights[i] + Stack3(p[0], p[1], p[2], kernel->fractals,
-                                                 kernel->count);
+                heights[i] = heights[i] + Stack3<float>(p[0], p[1], p[2], kernel->fractals,
+                                                        kernel->count);
             }
+        }
+    });
+}
+
+// The FP64 oracle of or_body_noise_grid: heights[i] += the stack in FP64 (every cell holds data).
+void or_body_noise_grid_f64(const tg_noise_layer_kernel* kernel, const double* positions,
+                            std::size_t cells, double* heights, unsigned threads) {
+    const std::size_t rows = (cells + 1023) / 1024;
+    ForRows(rows, threads, [&](std::size_t r) {
+        for (std::size_t i = r * 1024; i < std::min(cells, (r + 1) * 1024); ++i) {
+            const double* p = positions + 3 * i;
+            heights[i] += Stack3<double>(p[0], p[1], p[2], kernel->fractals, kernel->count);
         }
     });
 }

@@ -10,6 +10,7 @@
 #include "crater_device.h"
 #include "internal.h"
 #include "noise_device.h"
+#include "sphere_math.h"
 
 #include <hip/hip_runtime.h>
 
@@ -30,6 +31,7 @@ struct tg_catalogue {
     tg::MpdTerms* terms = nullptr;
     tg::HotCrater* hot = nullptr;  // the fast path's record per crater
     float* hotHarm = nullptr;      // its first tg::kHotHarmonics outline harmonics
+    tg_crater_frame* frames = nullptr;  // MakeCraterFrame per crater, for tiles on a sphere
     size_t count = 0;
     bool allFast = false;  // every crater takes the hoisted MPD path
 };
@@ -62,6 +64,9 @@ struct tg_batch {
     uint32_t groupsPerCu = 8;
     bool hoisted = true;        // the hoisted MPD path where a crater allows it
     uint32_t chunk = 8;         // bins a wave takes per atomic add
+    // The tiles' surface (tg_hip_batch_surface): 0 for a plane, else a sphere of this radius.
+    double radius = 0.0;
+    double graphRadius = 0.0;
 };
 
 namespace {
@@ -164,22 +169,11 @@ __device__ __forceinline__ float VisitCrater(uint32_t n, bool inside, float h, f
     return h;
 }
 
-// One bin per wave: lane 0 takes the next bin with one atomic add and the index is broadcast
-// with readfirstlane, so the bin loop is uniform across the wave and no wave waits for another
-// (the kernel has no block barrier). The wave covers the bin as four 8 x 8 squares, the shape the
-// strip kernel uses, so its crater-reach tests stay coherent across lanes.
-// kAllFast: every crater of the catalogue takes the hoisted MPD path, so the full evaluation is
-// not compiled in; the kernel then holds far fewer registers and keeps more waves resident.
-template <uint32_t kWalk, bool kAllFast>
-__global__ void __launch_bounds__(kGroupThreads, kAllFast ? 8 : 1)
-    BatchTiles(const tg_tile_desc* __restrict__ tiles, const tg_bin* __restrict__ bins,
-               const uint32_t* __restrict__ entries, uint32_t* __restrict__ counters,
-               const tg::NoisePlan* __restrict__ plan, float elevation,
-               const tg_crater_shape* __restrict__ kernel,
-               const tg_crater_record* __restrict__ records, const float* __restrict__ data,
-               const float* __restrict__ harm, const float4* __restrict__ outline,
-               const tg::HotCrater* __restrict__ hot, const float* __restrict__ hotHarm,
-               uint32_t hoisted, uint32_t chunk, float* __restrict__ out) {
+// Takes bins off the queues until every range is empty, calling `bin(b)` for each bin b taken:
+// lane 0 takes `chunk` bins with one atomic add and the first is broadcast with readfirstlane, so
+// the calls are uniform across the wave and no wave waits for another.
+template <class F>
+__device__ __forceinline__ void DrainBins(uint32_t* counters, uint32_t chunk, F&& bin) {
     const uint32_t binCount = counters[kCountSlot];
     const uint32_t lane = threadIdx.x & 63u;
     uint32_t queue = blockIdx.x % kQueues;
@@ -201,6 +195,29 @@ __global__ void __launch_bounds__(kGroupThreads, kAllFast ? 8 : 1)
         }
         const uint32_t last = uint32_t(std::min<uint64_t>(first + chunk, hi));
         for (uint32_t b = uint32_t(first); b < last; ++b) {
+            bin(b);
+        }
+    }
+}
+
+// One bin per wave: lane 0 takes the next bin with one atomic add and the index is broadcast
+// with readfirstlane, so the bin loop is uniform across the wave and no wave waits for another
+// (the kernel has no block barrier). The wave covers the bin as four 8 x 8 squares, the shape the
+// strip kernel uses, so its crater-reach tests stay coherent across lanes.
+// kAllFast: every crater of the catalogue takes the hoisted MPD path, so the full evaluation is
+// not compiled in; the kernel then holds far fewer registers and keeps more waves resident.
+template <uint32_t kWalk, bool kAllFast>
+__global__ void __launch_bounds__(kGroupThreads, kAllFast ? 8 : 1)
+    BatchTiles(const tg_tile_desc* __restrict__ tiles, const tg_bin* __restrict__ bins,
+               const uint32_t* __restrict__ entries, uint32_t* __restrict__ counters,
+               const tg::NoisePlan* __restrict__ plan, float elevation,
+               const tg_crater_shape* __restrict__ kernel,
+               const tg_crater_record* __restrict__ records, const float* __restrict__ data,
+               const float* __restrict__ harm, const float4* __restrict__ outline,
+               const tg::HotCrater* __restrict__ hot, const float* __restrict__ hotHarm,
+               uint32_t hoisted, uint32_t chunk, float* __restrict__ out) {
+    const uint32_t lane = threadIdx.x & 63u;
+    DrainBins(counters, chunk, [&](uint32_t b) {
         const tg_bin bin = bins[b];
         const tg_tile_desc tile = tiles[bin.tile];
         const uint32_t local = b - tile.bin_first;
@@ -304,8 +321,150 @@ __global__ void __launch_bounds__(kGroupThreads, kAllFast ? 8 : 1)
                 out[tile.out_offset + size_t(row) * tile.width + col] = h;
             }
         }
+    });
+}
+
+// The largest of v over the wave (every lane active).
+__device__ __forceinline__ double WaveMax(double v) {
+    for (int offset = 32; offset > 0; offset >>= 1) {
+        v = fmax(v, __shfl_xor(v, offset, 64));
+    }
+    return v;
+}
+
+// Crater n at the body-fixed cell point p, as the strip kernel applies it on a body: the offset in
+// the crater's frame (BodySplitOffset), nothing beyond the crater's cull radius (or for a lane
+// whose cell lies past the tile), else the hoisted MPD evaluation where the crater allows it and
+// the full one otherwise, at the tile's resolution weight.
+template <bool kAllFast>
+__device__ __forceinline__ float VisitBodyCrater(uint32_t n, bool inside, float h,
+                                                 const tg_split_point& p, double spacing,
+                                                 bool hoisted, const tg::HotCrater* hot,
+                                                 const float* hotHarm,
+                                                 const tg_crater_frame* frames,
+                                                 const tg_crater_shape* kernel, const float* data,
+                                                 const tg_crater_record* records,
+                                                 const float* harm, const float4* outline) {
+    const tg::HotCrater& c = hot[n];
+    const tg::Offset at = tg::BodySplitOffset(frames[n], p);
+    if (inside && at.dx * at.dx + at.dy * at.dy < c.cull) {
+        const float weight = ResolutionAt(c.diameter, spacing);
+        if (kAllFast || (hoisted && c.terms.fast != 0u)) {
+            return tg::ApplyMpd(c.terms, c.ellipticity, c.first_harmonic, c.harmonic_count,
+                                Coefficients(c, n, hotHarm, harm), c.ra, h, at.dx, at.dy,
+                                weight);
+        } else if constexpr (!kAllFast) {
+            const tg::CraterShape shape(*kernel);
+            return tg::ApplyCrater(shape, data, records[n], h, at.dx, at.dy, harm, outline[n],
+                                   weight);
         }
     }
+    return h;
+}
+
+// BatchTiles for longitude-latitude tiles on a sphere of `radius` meters (a tile's origin and
+// pixel in degrees). Each cell's body-fixed position is computed in FP64 (Surface::Position,
+// src/geometry/surface.cpp:45-50,261-267, with sphere_math.h's sine and cosine), the noise is the
+// stack's at that position (EvaluateFractals3, src/noise/surface_noise.cpp:9-20), and the craters
+// take the position split into FP32 pairs (MakeSplitPoint, crater_kernel.cpp:303-310) and their
+// offsets in each crater's frame (craters.h:168-209), at the resolution weight of the tile's
+// spacing, pixel times unitLength (RasterGrid::Spacing, src/raster/grid.cpp:28-33). The crater
+// lists are walked 64 entries at a time: an entry is visited only when its centre lies within its
+// cull radius (a bound on the arc, and so on the chord, to any cell it shapes) of the ball around
+// the wave's 8 x 8 cells.
+template <bool kAllFast>
+__global__ void __launch_bounds__(kGroupThreads, kAllFast ? 8 : 1)
+    BatchBodyTiles(const tg_tile_desc* __restrict__ tiles, const tg_bin* __restrict__ bins,
+                   const uint32_t* __restrict__ entries, uint32_t* __restrict__ counters,
+                   const tg::NoisePlan* __restrict__ plan, float elevation, double radius,
+                   double unitLength, const tg_crater_shape* __restrict__ kernel,
+                   const tg_crater_record* __restrict__ records, const float* __restrict__ data,
+                   const float* __restrict__ harm, const float4* __restrict__ outline,
+                   const tg::HotCrater* __restrict__ hot, const float* __restrict__ hotHarm,
+                   const tg_crater_frame* __restrict__ frames, uint32_t hoisted, uint32_t chunk,
+                   float* __restrict__ out) {
+    const uint32_t lane = threadIdx.x & 63u;
+    DrainBins(counters, chunk, [&](uint32_t b) {
+        const tg_bin bin = bins[b];
+        const tg_tile_desc tile = tiles[bin.tile];
+        const uint32_t local = b - tile.bin_first;
+        const uint32_t col0 = (local % tile.bins_across) * kBinSide;
+        const uint32_t row0 = (local / tile.bins_across) * kBinSide;
+        const double spacing = tile.pixel * unitLength;
+        for (uint32_t square = 0; square < 4u; ++square) {
+            const uint32_t squareCol = col0 + (square & 1u) * 8u;
+            const uint32_t squareRow = row0 + (square >> 1u) * 8u;
+            if (squareCol >= tile.width || squareRow >= tile.height) {
+                continue;  // the square lies past the tile's edge (uniform across the wave)
+            }
+            const uint32_t col = squareCol + (lane & 7u);
+            const uint32_t row = squareRow + (lane >> 3u);
+            const bool inside = col < tile.width && row < tile.height;
+            // Lanes past the tile's edge evaluate its last column or row and store nothing.
+            const uint32_t cellCol = std::min(col, tile.width - 1u);
+            const uint32_t cellRow = std::min(row, tile.height - 1u);
+            const tg::Position at = tg::CellPosition(tile.origin_x, tile.origin_y, tile.pixel,
+                                                     cellCol, cellRow, radius);
+            float h = plan != nullptr ? elevation + tg::EvaluateStack3(*plan, at.x, at.y, at.z)
+                                      : elevation;
+            if (kernel != nullptr && bin.count > 0u) {
+                tg_split_point p;
+                p.x = float(at.x);
+                p.x_low = float(at.x - double(p.x));
+                p.y = float(at.y);
+                p.y_low = float(at.y - double(p.y));
+                p.z = float(at.z);
+                p.z_low = float(at.z - double(p.z));
+                const uint32_t end = bin.first + bin.count;
+                const auto visit = [&](uint32_t n) {
+                    h = VisitBodyCrater<kAllFast>(n, inside, h, p, spacing, hoisted != 0u, hot,
+                                                  hotHarm, frames, kernel, data, records, harm,
+                                                  outline);
+                };
+                if (bin.count <= kShortList) {
+                    for (uint32_t e = bin.first; e < end; ++e) {
+                        visit(entries[e]);
+                    }
+                } else {
+                    // The ball around the square's cells: centred on lane 27's cell (row 3,
+                    // column 3), its radius the farthest of the wave's cells from it.
+                    const double ox = __shfl(at.x, 27, 64);
+                    const double oy = __shfl(at.y, 27, 64);
+                    const double oz = __shfl(at.z, 27, 64);
+                    const double reach = WaveMax(sqrt((at.x - ox) * (at.x - ox) +
+                                                      (at.y - oy) * (at.y - oy) +
+                                                      (at.z - oz) * (at.z - oz)));
+                    for (uint32_t base = bin.first; base < end; base += 64u) {
+                        const uint32_t e = base + lane;
+                        uint32_t n = 0u;
+                        bool near = false;
+                        if (e < end) {
+                            n = entries[e];
+                            const tg_crater_frame& f = frames[n];
+                            // A cell passing the FP32 test lies within the cull radius of the
+                            // centre along the sphere to a relative 1e-6 or so, and the chord is
+                            // no longer than the arc; the margins keep every entry a cell of the
+                            // square can take.
+                            const double qx = double(f.x) + double(f.x_low) - ox;
+                            const double qy = double(f.y) + double(f.y_low) - oy;
+                            const double qz = double(f.z) + double(f.z_low) - oz;
+                            const double d = sqrt(qx * qx + qy * qy + qz * qz);
+                            near = d <= sqrt(double(hot[n].cull)) * (1.0 + 1e-4) + 1e-3 + reach;
+                        }
+                        uint64_t candidates = __ballot(near);
+                        while (candidates != 0ull) {
+                            const uint32_t j = uint32_t(__builtin_ctzll(candidates));
+                            candidates &= candidates - 1ull;
+                            visit(uint32_t(__builtin_amdgcn_readlane(int(n), int(j))));
+                        }
+                    }
+                }
+            }
+            if (inside) {
+                out[tile.out_offset + size_t(row) * tile.width + col] = h;
+            }
+        }
+    });
 }
 
 // One lane per crater: the hot record and harmonic slot from the catalogue's arrays.
@@ -371,7 +530,8 @@ void FreeCatalogue(tg_catalogue* c) {
                     static_cast<void*>(c->data), static_cast<void*>(c->cull),
                     static_cast<void*>(c->harm), static_cast<void*>(c->outline),
                     static_cast<void*>(c->diameters), static_cast<void*>(c->terms),
-                    static_cast<void*>(c->hot), static_cast<void*>(c->hotHarm)}) {
+                    static_cast<void*>(c->hot), static_cast<void*>(c->hotHarm),
+                    static_cast<void*>(c->frames)}) {
         if (p != nullptr) {
             (void)hipFree(p);
         }
@@ -404,6 +564,25 @@ void Launch(tg_batch& b, const tg_noise_plan* plan, float elevation, const tg_ca
     hipLaunchKernelGGL(ResetQueue, dim3(1), dim3(64), 0, b.stream, b.counters);
     // The compact kernel when every crater takes the hoisted path (or there are none).
     const bool allFast = b.hoisted && (cat == nullptr || cat->allFast);
+    if (b.radius > 0.0) {
+        // The length of a degree of arc, as Surface::Sphere sets it (surface.cpp:49).
+        const double unitLength = b.radius * (3.141592653589793 / 180.0);
+        auto* body = allFast ? BatchBodyTiles<true> : BatchBodyTiles<false>;
+        hipLaunchKernelGGL(body, dim3(groups), dim3(kGroupThreads), 0, b.stream, b.tiles, b.bins,
+                           b.entries, b.counters, plan != nullptr ? plan->device : nullptr,
+                           elevation, b.radius, unitLength,
+                           cat != nullptr ? cat->kernel : nullptr,
+                           cat != nullptr ? cat->records : nullptr,
+                           cat != nullptr ? cat->data : nullptr,
+                           cat != nullptr ? cat->harm : nullptr,
+                           cat != nullptr ? cat->outline : nullptr,
+                           cat != nullptr ? cat->hot : nullptr,
+                           cat != nullptr ? cat->hotHarm : nullptr,
+                           cat != nullptr ? cat->frames : nullptr, b.hoisted ? 1u : 0u, chunk,
+                           out);
+        Check(hipGetLastError(), "launching the batched body tile kernel");
+        return;
+    }
     auto* fn = b.walk == TG_WALK_PER_LANE
                    ? (allFast ? BatchTiles<kWalkPerLane, true> : BatchTiles<kWalkPerLane, false>)
                : b.walk == TG_WALK_CULLED
@@ -690,6 +869,15 @@ int tg_hip_batch_run(tg_batch* b, const tg_noise_plan* plan, float elevation,
         if (b->anyEntries && (cat == nullptr || b->maxEntry >= cat->count)) {
             throw HipError{"the batch's bins name craters the catalogue does not hold"};
         }
+        if (b->radius > 0.0) {
+            if (plan != nullptr && !plan->host.bodyReady) {
+                throw HipError{"tiles on a sphere take noise of unwarped Perlin fractals only"};
+            }
+            if (b->anyEntries && cat->frames == nullptr) {
+                throw HipError{"tiles on a sphere need the catalogue's crater frames "
+                               "(tg_hip_catalogue_frames)"};
+            }
+        }
         if (b->binCount == 0) {
             return;
         }
@@ -704,6 +892,7 @@ int tg_hip_batch_run(tg_batch* b, const tg_noise_plan* plan, float elevation,
         if (b->graph == nullptr || b->graphPlan != plan || b->graphCatalogue != cat ||
             b->graphOut != out || b->graphElevation != elevation || b->graphGroups != groups ||
             b->graphWalk != b->walk || b->graphHoisted != b->hoisted ||
+            b->graphRadius != b->radius ||
             b->graphChunk != ChunkFor(*b, groups)) {
             DropGraph(*b);
             hipGraph_t graph = nullptr;
@@ -730,6 +919,7 @@ int tg_hip_batch_run(tg_batch* b, const tg_noise_plan* plan, float elevation,
             b->graphWalk = b->walk;
             b->graphHoisted = b->hoisted;
             b->graphChunk = ChunkFor(*b, groups);
+            b->graphRadius = b->radius;
         }
         Check(hipGraphLaunch(b->graph, b->stream), "hipGraphLaunch");
     });
@@ -753,6 +943,47 @@ int tg_hip_batch_tune(tg_batch* b, uint32_t crater_walk, uint32_t groups_per_cu,
     b->hoisted = hoisted != 0u;
     b->chunk = chunk;
     return TG_OK;
+}
+
+int tg_hip_batch_surface(tg_batch* b, double sphere_radius, char* err, size_t errLen) {
+    if (b == nullptr) {
+        SetErr(err, errLen, "tg_hip_batch_surface: null batch");
+        return TG_ERR_ARGUMENT;
+    }
+    if (!(sphere_radius >= 0.0) || !std::isfinite(sphere_radius)) {
+        SetErr(err, errLen, "tg_hip_batch_surface: the radius must be 0 or positive and finite");
+        return TG_ERR_ARGUMENT;
+    }
+    b->radius = sphere_radius;
+    return TG_OK;
+}
+
+int tg_hip_catalogue_frames(tg_catalogue* cat, const tg_crater_frame* frames, size_t count,
+                            char* err, size_t errLen) {
+    if (cat == nullptr || (count > 0 && frames == nullptr)) {
+        SetErr(err, errLen, "tg_hip_catalogue_frames: null argument");
+        return TG_ERR_ARGUMENT;
+    }
+    if (count != cat->count) {
+        SetErr(err, errLen, "tg_hip_catalogue_frames: one frame per crater of the catalogue");
+        return TG_ERR_ARGUMENT;
+    }
+    if (cat->frames != nullptr) {
+        // Captured graphs hold the frames' address, so a catalogue's frames are set once.
+        SetErr(err, errLen, "tg_hip_catalogue_frames: the catalogue already has its frames");
+        return TG_ERR_ARGUMENT;
+    }
+    return Guard(err, errLen, [&] {
+        for (size_t i = 0; i < count; ++i) {
+            const tg_crater_frame& f = frames[i];
+            if (!(f.sphere_radius > 0.0f) || !std::isfinite(f.x) || !std::isfinite(f.y) ||
+                !std::isfinite(f.z)) {
+                throw HipError{"frame " + std::to_string(i) + " is not a frame on a sphere"};
+            }
+        }
+        Check(hipSetDevice(cat->dev->ordinal), "hipSetDevice");
+        cat->frames = UploadArray(frames, count);
+    });
 }
 
 int tg_hip_batch_wait(tg_batch* b, char* err, size_t errLen) {

@@ -171,11 +171,21 @@ __global__ void __launch_bounds__(kCraterTile * kCraterTile)
             }
         }
     } else if constexpr (!kAllFast) {
+        // On a body the offset is the cell's in the crater's frame, and the rest is the plane's.
         const tg_split_point p = points[i];
         for (uint32_t e = first; e < first + count; ++e) {
             const uint32_t n = entries[e];
             const tg::Offset at = tg::BodySplitOffset(frames[n], p);
-            h = tg::ApplyCrater(shape, data, records[n], h, at.dx, at.dy);
+            if (at.dx * at.dx + at.dy * at.dy >= cull[n]) {
+                continue;
+            }
+            const tg_crater_record& c = records[n];
+            if (terms[n].fast != 0u) {
+                h = tg::ApplyCraterMpd(c, terms[n], h, at.dx, at.dy, harm, outline[n],
+                                       c.resolution);
+            } else {
+                h = tg::ApplyCrater(shape, data, c, h, at.dx, at.dy, harm, outline[n]);
+            }
         }
     }
     heights[i] = h;

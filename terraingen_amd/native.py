@@ -136,6 +136,8 @@ HIP_ABI = {
     "tg_hip_event_query": (C.c_int, [VP, VP, CP, SZ]),
     "tg_hip_memcpy_d2d_async": (C.c_int, [VP, VP, VP, SZ, VP, CP, SZ]),
     "tg_hip_batch_tune": (C.c_int, [VP, u32, u32, u32, u32, CP, SZ]),
+    "tg_hip_batch_surface": (C.c_int, [VP, C.c_double, CP, SZ]),
+    "tg_hip_catalogue_frames": (C.c_int, [VP, VP, SZ, CP, SZ]),
     "tg_hip_fill": (C.c_int, [VP, VP, C.c_float, SZ, VP, CP, SZ]),
     "tg_hip_boulder_set_create": (C.c_int, [VP, VP, VP, SZ, VP, SZ, VP, SZ, VP, SZ,
                                             C.POINTER(VP), CP, SZ]),
@@ -537,6 +539,92 @@ def _take_batch(lib, tiles_n, dp, bp, bn, ep, en):
         for p in (dp, bp, ep):
             lib.tgx_free(p)
     return descs, bins, entries
+
+
+HOST_ABI["tgx_sphere_positions"] = (C.c_int, [C.c_double, C.c_double, C.c_double, u32, u32,
+                                              C.c_double, VP, CP, SZ])
+HOST_ABI["tgx_sphere_split_points"] = (C.c_int, [C.c_double, C.c_double, C.c_double, u32, u32,
+                                                 C.c_double, VP, CP, SZ])
+HOST_ABI["tgx_sphere_frames"] = (C.c_int, [VP, VP, SZ, C.c_double, VP, CP, SZ])
+HOST_ABI["tgx_sphere_bin"] = (C.c_int, [VP, VP, VP, VP, SZ, C.c_double, VP, SZ, C.c_uint,
+                                        C.POINTER(VP), C.POINTER(VP), C.POINTER(SZ),
+                                        C.POINTER(VP), C.POINTER(SZ), CP, SZ])
+
+
+def sphere_positions(g: Grid, radius: float) -> np.ndarray:
+    """``tgx_sphere_positions``: the body-fixed FP64 centres of a longitude-latitude grid's cells,
+    (height * width, 3)."""
+    out = np.empty((g.height * g.width, 3), np.float64)
+    err = C.create_string_buffer(512)
+    _check(host_library().tgx_sphere_positions(g.origin_x, g.origin_y, g.pixel, g.width,
+                                               g.height, radius, out.ctypes.data, err, 512), err)
+    return out
+
+
+def sphere_split_points(g: Grid, radius: float) -> np.ndarray:
+    """``tgx_sphere_split_points``: MakeSplitPoint of each cell centre, POINT_DTYPE."""
+    out = np.empty(g.height * g.width, POINT_DTYPE)
+    err = C.create_string_buffer(512)
+    _check(host_library().tgx_sphere_split_points(g.origin_x, g.origin_y, g.pixel, g.width,
+                                                  g.height, radius, out.ctypes.data, err, 512),
+           err)
+    return out
+
+
+def sphere_frames(lon: np.ndarray, lat: np.ndarray, radius: float) -> np.ndarray:
+    """``tgx_sphere_frames``: MakeCraterFrame of craters at (lon, lat) degrees, FRAME_DTYPE."""
+    lon, lat = (np.ascontiguousarray(a, np.float64) for a in (lon, lat))
+    out = np.empty(max(lon.size, 1), FRAME_DTYPE)
+    err = C.create_string_buffer(512)
+    _check(host_library().tgx_sphere_frames(lon.ctypes.data, lat.ctypes.data, lon.size, radius,
+                                            out.ctypes.data, err, 512), err)
+    return out[:lon.size]
+
+
+def sphere_bin(foot: np.ndarray, radius: float, tiles: np.ndarray, threads: int = 0):
+    """``tgx_sphere_bin``: (tile descriptors, bins, entries) for a catalogue footprint (lon, lat,
+    reach in meters of arc, D per crater, apply order) and tiles in degrees."""
+    lib = host_library()
+    foot = np.ascontiguousarray(foot, np.float64)
+    lon, lat, reach, d = (np.ascontiguousarray(foot[:, i]) for i in range(4))
+    tiles = np.ascontiguousarray(tiles, TILE_DTYPE)
+    dp, bp, ep = VP(), VP(), VP()
+    bn, en = SZ(), SZ()
+    err = C.create_string_buffer(512)
+    _check(lib.tgx_sphere_bin(lon.ctypes.data, lat.ctypes.data, reach.ctypes.data, d.ctypes.data,
+                              lon.size, radius, tiles.ctypes.data, tiles.size, threads,
+                              C.byref(dp), C.byref(bp), C.byref(bn), C.byref(ep), C.byref(en),
+                              err, 512), err)
+    return _take_batch(lib, tiles.size, dp, bp, bn, ep, en)
+
+
+HOST_ABI["tgx_sphere_pair_ops"] = (C.c_int, [VP, SZ, VP, SZ, VP, SZ, VP, VP, VP, VP, SZ,
+                                             C.c_double, C.c_uint, C.POINTER(C.c_uint64),
+                                             C.POINTER(C.c_uint64), C.POINTER(C.c_double), CP,
+                                             SZ])
+
+
+def sphere_pair_ops(descs, bins, entries, foot: np.ndarray, cost: np.ndarray, radius: float,
+                    threads: int = 0):
+    """``tgx_sphere_pair_ops``: (pairs listed, pairs within reach along the sphere, summed cost
+    of those) for a batch binned by ``sphere_bin``."""
+    lib = host_library()
+    foot = np.ascontiguousarray(foot, np.float64)
+    lon, lat, reach = (np.ascontiguousarray(foot[:, i]) for i in range(3))
+    cost = np.ascontiguousarray(cost, np.float64)
+    if cost.size != lon.size:
+        raise ValueError("one cost per catalogue crater")
+    descs = np.ascontiguousarray(descs, TILE_DESC_DTYPE)
+    bins = np.ascontiguousarray(bins, BIN_DTYPE)
+    entries = np.ascontiguousarray(entries, np.uint32)
+    listed, inside, ops = C.c_uint64(), C.c_uint64(), C.c_double()
+    err = C.create_string_buffer(512)
+    _check(lib.tgx_sphere_pair_ops(descs.ctypes.data, descs.size, bins.ctypes.data, bins.size,
+                                   entries.ctypes.data, entries.size, lon.ctypes.data,
+                                   lat.ctypes.data, reach.ctypes.data, cost.ctypes.data,
+                                   lon.size, radius, threads, C.byref(listed), C.byref(inside),
+                                   C.byref(ops), err, 512), err)
+    return int(listed.value), int(inside.value), float(ops.value)
 
 
 class CatalogueIndex:

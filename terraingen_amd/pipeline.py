@@ -382,6 +382,187 @@ class TileBatch:
                 setattr(self, name, None)
 
 
+MOON_RADIUS = 1737400.0  # meters
+
+
+def body_tile_lists(descs: np.ndarray, bins: np.ndarray, entries: np.ndarray):
+    """A one-tile batch's bins as the crater kernel's tile lists on a body (``[tilesX, side,
+    onBody = 1, (first, count)...]``, craters.h:1341-1344) and entries."""
+    d = descs[0]
+    heads = [int(d["bins_across"]), 16, 1]
+    for b in bins:
+        heads += [int(b["first"]), int(b["count"])]
+    return (np.asarray(heads, np.uint32),
+            np.ascontiguousarray(entries if entries.size else np.zeros(1, np.uint32), np.uint32))
+
+
+@dataclass
+class SphereTileBatch(TileBatch):
+    """TileBatch for longitude-latitude tiles on a sphere of `radius` meters (a tile's origin and
+    pixel in degrees; tg_hip_batch_surface): each cell's noise is the stack's at its body-fixed
+    position (EvaluateFractals3, src/noise/surface_noise.cpp:9-20) and the craters are placed by
+    their frames (MakeCraterFrame, crater_kernel.cpp:285-301). The catalogue is a given crater
+    list with longitude and latitude in its x and y columns (``global_craters`` makes one). The
+    plain path per tile is tg_hip_run of TG_KERNEL_NOISE_BODY then TG_KERNEL_CRATERS on a body,
+    with the tile's positions, split points and frames worked out on the host."""
+    radius: float = MOON_RADIUS
+    craters_given: np.ndarray | None = None
+
+    @property
+    def unit_length(self) -> float:
+        """Surface::UnitLength of the sphere: meters per degree of arc (surface.cpp:49)."""
+        return self.radius * (np.pi / 180.0)
+
+    def build(self) -> None:
+        spec = self.craters
+        t0 = time.perf_counter()
+        if self.craters_given is None:
+            raise ValueError("a sphere batch takes its catalogue as craters_given")
+        self.crater_list = np.ascontiguousarray(self.craters_given, N.CRATER_DTYPE)
+        self.kernel, self.records, self.data, self.foot = N.prepare_craters(
+            self.crater_list, spec.models(), spec.apply(), spec.seed, self.finest_pixel)
+        if self.records.size != self.crater_list.size:
+            raise ValueError("the catalogue's finest spacing drops craters")
+        self.ordered = self.crater_list[apply_order(self.crater_list)]
+        self.frames = N.sphere_frames(self.foot[:, 0], self.foot[:, 1], self.radius)
+        t1 = time.perf_counter()
+        diam = np.ascontiguousarray(self.foot[:, 3])
+        cat = C.c_void_p()
+        self.dev.call("tg_hip_catalogue_create", self.dev.handle, C.byref(self.kernel),
+                      self.records.ctypes.data, self.records.size, self.data.ctypes.data,
+                      self.data.size, diam.ctypes.data, C.byref(cat))
+        self.catalogue = cat
+        self.dev.call("tg_hip_catalogue_frames", cat, self.frames.ctypes.data, self.frames.size)
+        t2 = time.perf_counter()
+        self.plan = None
+        if self.source_fractals:
+            kernel = N.noise_kernel(self.source_fractals)
+            plan = C.c_void_p()
+            self.dev.call("tg_hip_noise_plan_create", self.dev.handle, C.byref(kernel),
+                          C.byref(plan))
+            self.plan = plan
+            self.noise_kernel = kernel
+        self.batch = None
+        self.timings.update(prepare_ms=(t1 - t0) * 1e3, catalogue_upload_ms=(t2 - t1) * 1e3)
+
+    def stage(self, tiles: np.ndarray, threads: int = 0) -> None:
+        t0 = time.perf_counter()
+        self.tiles = np.ascontiguousarray(tiles, N.TILE_DTYPE)
+        self.descs, self.bins, self.entries = N.sphere_bin(self.foot, self.radius, self.tiles,
+                                                           threads)
+        t1 = time.perf_counter()
+        if self.batch is None or self._capacity < (self.tiles.size, self.bins.size,
+                                                   self.entries.size):
+            if self.batch is not None:
+                self.dev.lib.tg_hip_batch_free(self.batch)
+            cap = (int(self.tiles.size * 1.25) + 1, int(self.bins.size * 1.25) + 1,
+                   int(self.entries.size * 1.25) + 1)
+            b = C.c_void_p()
+            self.dev.call("tg_hip_batch_create", self.dev.handle, *cap, C.byref(b))
+            self.batch, self._capacity = b, cap
+            self.dev.call("tg_hip_batch_surface", self.batch, self.radius)
+        self.dev.call("tg_hip_batch_stage", self.batch, self.descs.ctypes.data, self.descs.size,
+                      self.bins.ctypes.data, self.bins.size, self.entries.ctypes.data,
+                      self.entries.size)
+        t2 = time.perf_counter()
+        self.cells = int((self.tiles["width"].astype(np.int64) *
+                          self.tiles["height"].astype(np.int64)).sum())
+        self.timings.update(bin_ms=(t1 - t0) * 1e3, stage_ms=(t2 - t1) * 1e3)
+
+    def plain_inputs(self, i: int) -> dict:
+        """Tile i's inputs as the reference's per-grid path on a body builds them: its craters
+        prepared at its spacing (RasterGrid::Spacing: pixel times the unit length), binned into
+        its 16 x 16 tiles, their frames, and each cell's position and split point."""
+        t = self.tiles[i]
+        grid = N.Grid(int(t["width"]), int(t["height"]), float(t["origin_x"]),
+                      float(t["origin_y"]), float(t["pixel"]))
+        d = self.descs[i]
+        nb = int(d["bins_across"]) * ((int(d["height"]) + 15) // 16)
+        first = self.bins[int(d["bin_first"])]
+        last = self.bins[int(d["bin_first"]) + nb - 1]
+        idx = np.unique(self.entries[int(first["first"]):int(last["first"]) + int(last["count"])])
+        inputs = dict(grid=grid, positions=N.sphere_positions(grid, self.radius), craters=None)
+        if idx.size > 0:
+            spec = self.craters
+            spacing = grid.pixel * self.unit_length
+            kernel, records, data, foot = N.prepare_craters(
+                self.ordered[idx], spec.models(), spec.apply(), spec.seed, spacing)
+            one = np.zeros(1, N.TILE_DTYPE)
+            for k in ("origin_x", "origin_y", "pixel", "width", "height"):
+                one[k] = t[k]
+            descs, bins, entries = N.sphere_bin(foot, self.radius, one, 1)
+            tiles, entries = body_tile_lists(descs, bins, entries)
+            frames = N.sphere_frames(foot[:, 0], foot[:, 1], self.radius)
+            inputs["craters"] = (kernel, records, data, tiles, entries, frames,
+                                 N.sphere_split_points(grid, self.radius))
+            inputs["foot"] = foot
+        return inputs
+
+    def plain_run(self, inputs: dict) -> np.ndarray:
+        """The tile through tg_hip_run: the body noise kernel then the crater kernel's body
+        branch, each a synchronous call with host buffers."""
+        g = inputs["grid"]
+        h = np.full((g.height, g.width), self.elevation, np.float32)
+        params = N.KernelParams(g.width, g.height, N.NODATA_NONE, 0.0, 0)
+        xs, ys = np.zeros(g.width, np.float32), np.zeros(g.height, np.float32)
+        if self.plan is not None:
+            self.dev.run(N.TG_KERNEL_NOISE_BODY, self.noise_kernel, h, xs, ys, params,
+                         [(inputs["positions"], 1)])
+        if inputs["craters"] is not None:
+            kernel, records, data, tiles, entries, frames, points = inputs["craters"]
+            extra = [(records, 0), (data, 0), (tiles, 0), (entries, 0), (frames, 0),
+                     (points, 1), (np.zeros(1, np.float32), 0)]
+            self.dev.run(N.TG_KERNEL_CRATERS, kernel, h, xs, ys, params, extra)
+        return h
+
+
+def sphere_min_diameter(count: int, radius: float, coefficient: float, exponent: float,
+                        max_diameter: float) -> float:
+    """The smallest diameter for which the cumulative power law N(>=D) = coefficient D^-exponent
+    per m^2, truncated at `max_diameter`, puts `count` craters on a sphere of `radius` meters."""
+    area = 4.0 * np.pi * radius * radius
+    return float((count / (coefficient * area) + max_diameter ** -exponent) ** (-1.0 / exponent))
+
+
+def global_craters(count: int, radius: float = MOON_RADIUS, coefficient: float = 0.05,
+                   exponent: float = 2.0, max_diameter: float = 20000.0,
+                   seed: int = 4) -> np.ndarray:
+    """A synthetic global catalogue (CRATER_DTYPE, longitude and latitude in degrees in x and y):
+    `count` craters with centres uniform on the sphere, diameters from the cumulative power law
+    N(>=D) = coefficient D^-exponent per m^2 between ``sphere_min_diameter`` and `max_diameter`
+    (inverse transform), ages uniform over 1 Gyr, IDs 1..count."""
+    rng = np.random.default_rng(seed)
+    d_min = sphere_min_diameter(count, radius, coefficient, exponent, max_diameter)
+    out = np.zeros(count, N.CRATER_DTYPE)
+    out["x"] = rng.uniform(-180.0, 180.0, count)
+    out["y"] = np.degrees(np.arcsin(rng.uniform(-1.0, 1.0, count)))
+    lo, hi = d_min ** -exponent, max_diameter ** -exponent
+    out["diameter"] = (lo - rng.uniform(0.0, 1.0, count) * (lo - hi)) ** (-1.0 / exponent)
+    out["age_gyr"] = rng.uniform(0.0, 1.0, count)
+    out["id"] = np.arange(1, count + 1, dtype=np.uint64)
+    return out
+
+
+def sphere_tiles(count: int, radius: float = MOON_RADIUS, seed: int = 4, min_k: int = 6,
+                 max_k: int = 11, pixel_range=(0.25, 8.0)) -> np.ndarray:
+    """Mixed longitude-latitude tiles on a sphere: edges 2^k + 1 with k uniform in [min_k,
+    max_k], cell sizes log-uniform over `pixel_range` meters (in degrees, over the unit length),
+    north-west corners uniform on the sphere (moved south of the pole where a tile would
+    cross it)."""
+    rng = np.random.default_rng(seed)
+    tiles = np.zeros(count, N.TILE_DTYPE)
+    edge = 2 ** rng.integers(min_k, max_k + 1, count) + 1
+    meters = np.exp(rng.uniform(np.log(pixel_range[0]), np.log(pixel_range[1]), count))
+    step = meters / (radius * (np.pi / 180.0))
+    tiles["width"] = edge
+    tiles["height"] = edge
+    tiles["pixel"] = step
+    tiles["origin_x"] = rng.uniform(-180.0, 180.0, count)
+    north = np.degrees(np.arcsin(rng.uniform(-1.0, 1.0, count)))
+    tiles["origin_y"] = np.clip(north, -90.0 + edge * step, 90.0)
+    return tiles
+
+
 def c4_tiles(count: int, side: float, seed: int = 4, min_k: int = 6, max_k: int = 11,
              pixel_range=(0.25, 8.0)) -> np.ndarray:
     """Mixed tiles: edges 2^k + 1 with k uniform in [min_k, max_k], pixel sizes log-uniform over
