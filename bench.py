@@ -1,26 +1,35 @@
 # SPDX-License-Identifier: BSD-3-Clause
 """Throughput of the hip backend's noise + crater shaping hot path on one MI355X.
 
-Default workload (``--workload tiles``): 1024 tiles of mixed size and pixel size, evaluated in one
-persistent launch. Edges are 2^k + 1 with k uniform in 6..11, pixel sizes log-uniform in
-0.25-8 m, and west/north edges uniform over a 4472 m square. The synthetic-source noise is an
-8-octave Perlin fBm (base wavelength 512 m, lacunarity 2, gain 0.5, amplitude 20 m, seed 1, origin
-{0, 0}). A shared catalogue of about 1e6 craters covers the square. It follows the power-law
-cumulative SFD N(>D) = 0.05 D^-2 per m^2 with D in [1, 200] m and ages uniform over 1 Gyr, seed 4,
-using the Martin-Parkes-Dunstan profile with freshness degradation and irregular outlines. Heights
-are FP32, with FP64 cell centres.
+Default workload (``--workload sphere``): 1024 tiles of mixed size and cell size on the Moon,
+evaluated in one persistent launch. Tiles are longitude-latitude rasters on a 1737.4 km sphere:
+edges 2^k + 1 with k uniform in 6..11, cells log-uniform in 0.25-8 m (in degrees over the
+sphere's unit length), north-west corners uniform on the sphere. Each cell's body-fixed position
+is computed in FP64 on the device. The noise is an 8-octave 3D Perlin fBm at that position
+(EvaluateFractals3: base wavelength 512 m, lacunarity 2, gain 0.5, amplitude 20 m, seed 1). A
+global catalogue of 1e6 craters follows the power-law cumulative SFD N(>D) = 0.05 D^-2 per m^2
+over the sphere (D from 1374 m to 20 km), centres uniform on the sphere, ages uniform over 1 Gyr,
+seed 4; each crater is placed by its FP64 frame and offsets are taken along the sphere. The
+profile is Martin-Parkes-Dunstan with freshness degradation and irregular outlines. Heights are
+FP32.
+
+``--workload tiles`` is the same batch on a plane: west/north edges uniform over a 4472 m square,
+the 2D noise with origin {0, 0}, and a catalogue of about 1e6 craters with D in [1, 200] m over
+the square.
 
 * ``value``: Gcells/s of the batched path. One step is one graph launch of the persistent kernel
   over every tile, with the catalogue, bins and outputs resident in HBM.
-* ``plain``: Gcells/s of the plain per-tile path over the same tiles. Each tile goes through
-  ``tg_hip_run``, once for the noise layer and once for the craters, synchronous with host memory
-  in and out, as RunKernel drives a device. Host preparation of its inputs is excluded.
+* ``plain``: Gcells/s of the plain per-tile path over the same tiles (every 4th tile on the
+  sphere). Each tile goes through ``tg_hip_run``, once for the noise layer and once for the
+  craters, synchronous with host memory in and out, as RunKernel drives a device. Host preparation
+  of its inputs is excluded.
 * ``serving``: the served stream, run through the tile server (continuous batching, several
   batches in flight, a 64 GB HBM admission pool). Tile requests arrive as a Poisson process at
   2000/s for 10 s from a camera flying a 50 km straight ground track at 200 m altitude. Each
   request is the quadtree tile, at levels 0-10 by slant range, that holds a ground point near the
-  camera, with edges of 129, 257, 513 or 1025 cells. The noise is as above, and the crater
-  catalogue (D >= 2 m, same SFD, seed 5) covers a 2 km swath along the track. It reports the
+  camera, with edges of 129, 257, 513 or 1025 cells. It runs on a plane: the 2D noise of the
+  ``tiles`` workload, and a crater catalogue (D >= 2 m, same SFD, seed 5) over a 2 km swath
+  along the track. It reports the
   p50/p99 request-to-host latency (submission to heights in the caller's memory) for 513 x 513
   tiles and for all tiles, and the served Gcells/s.
 
@@ -94,11 +103,11 @@ def roofline(ops: float, kernel_ms: float, bytes_written: float) -> dict:
             "hbm_write_frac": round(bytes_written / (kernel_ms * 1e-3) / 1e9 / HBM_PEAK_GBS, 5)}
 
 
-def counted_traffic(kernel_ms: float) -> dict | None:
-    """HBM bytes per launch of the default tiles workload, from the committed counter runs
-    (FETCH_SIZE and WRITE_SIZE, each scaled by a 1 GiB copy's reading), with the VALU
-    instructions those runs counted, scaled to this run's launch time."""
-    path = ROOT / "profiles" / "batch_tiles_c4_1024_mixed_counters.md"
+def counted_traffic(kernel_ms: float, name: str) -> dict | None:
+    """HBM bytes per launch of a 1024-tile workload, from its committed counter runs
+    (profiles/`name`: FETCH_SIZE and WRITE_SIZE, each scaled by a 1 GiB copy's reading), with the
+    VALU instructions those runs counted, scaled to this run's launch time."""
+    path = ROOT / "profiles" / name
     if not path.exists():
         return None
     for line in path.read_text().splitlines():
@@ -372,13 +381,12 @@ def bench_tiles(args, N, dev) -> dict:
         "roofline": roofline(ops_noise + ops_craters, kernel_ms, cells * 4.0) | {
             "kernel": "BatchTiles", "ops_noise": ops_noise, "ops_craters": ops_craters},
     }
-    if tiles.size == 1024 and (counted := counted_traffic(kernel_ms)) is not None:
+    if tiles.size == 1024 and (counted := counted_traffic(
+            kernel_ms, "batch_tiles_c4_1024_mixed_counters.md")) is not None:
         result["roofline"] |= counted
     result["roofline"]["valu_ceiling_measured"] = valu_ceiling(N, dev)
     result |= extra
     batch.close()
-    if args.serve_seconds > 0:
-        result["serving"] = bench_serving(args, N, dev)
     return result
 
 
@@ -463,6 +471,13 @@ def bench_sphere(args, N, dev) -> dict:
         print(f"[bench] autotune {tuning}", file=sys.stderr, flush=True)
     step_ms, wall_ms, kernel_ms = time_launches(args, N, dev, batch, out)
     got = out.download(np.empty(batch.cells, np.float32))
+    if args.copy_calibration_mib > 0:
+        # A copy of known size on the same stream, for scaling the HBM byte counters.
+        nbytes = args.copy_calibration_mib << 20
+        src, dst = N.DeviceArray(dev, nbytes), N.DeviceArray(dev, nbytes)
+        dev.call("tg_hip_memcpy_d2d_async", dev.handle, dst.ptr, src.ptr, nbytes, batch.stream)
+        batch.wait()
+        del src, dst
 
     # The plain path over a sample of the tiles (every 4th): tg_hip_run per layer, host buffers.
     plain_s = 0.0
@@ -534,6 +549,9 @@ def bench_sphere(args, N, dev) -> dict:
         "roofline": roofline(ops_noise + ops_craters, kernel_ms, cells * 4.0) | {
             "kernel": "BatchBodyTiles", "ops_noise": ops_noise, "ops_craters": ops_craters},
     }
+    if tiles.size == 1024 and (counted := counted_traffic(
+            kernel_ms, "batch_body_tiles_c4_sphere_1024_counters.md")) is not None:
+        result["roofline"] |= counted
     result["roofline"]["valu_ceiling_measured"] = valu_ceiling(N, dev)
     result |= extra
     batch.close()
@@ -702,7 +720,7 @@ def main() -> int:
     ap.add_argument("--gpus", type=int, default=1)
     ap.add_argument("--steps", type=int, default=10)
     ap.add_argument("--warmup", type=int, default=2)
-    ap.add_argument("--workload", choices=("tiles", "sphere", "grid", "c3"), default="tiles")
+    ap.add_argument("--workload", choices=("sphere", "tiles", "grid", "c3"), default="sphere")
     ap.add_argument("--tiles", type=int, default=1024)
     ap.add_argument("--size", type=int, default=0,
                     help="grid workload edge (0: 4097 for grid, 8193 for c3)")
@@ -732,6 +750,8 @@ def main() -> int:
         body = bench_sphere(args, N, dev)
     else:
         body = bench_grid(args, N, dev)
+    if args.workload in ("tiles", "sphere") and args.serve_seconds > 0:
+        body["serving"] = bench_serving(args, N, dev)
     result = {"metric": body.pop("metric"), "value": body.pop("value"), "unit": body.pop("unit"),
               "n_gpus": 1, "steps": args.steps, "warmup": args.warmup,
               "ms_per_step": body.pop("ms_per_step"), "higher_is_better": True,
