@@ -372,7 +372,9 @@ __device__ __forceinline__ float VisitBodyCrater(uint32_t n, bool inside, float 
 // lists are walked 64 entries at a time: an entry is visited only when its centre lies within its
 // cull radius (a bound on the arc, and so on the chord, to any cell it shapes) of the ball around
 // the wave's 8 x 8 cells.
-template <bool kAllFast>
+// kSmall: every lattice coordinate of the launch lies within 2^31 of the origin (checked on the
+// host from the radius and the plan's highest frequency).
+template <bool kAllFast, bool kSmall>
 __global__ void __launch_bounds__(kGroupThreads, kAllFast ? 8 : 1)
     BatchBodyTiles(const tg_tile_desc* __restrict__ tiles, const tg_bin* __restrict__ bins,
                    const uint32_t* __restrict__ entries, uint32_t* __restrict__ counters,
@@ -384,6 +386,7 @@ __global__ void __launch_bounds__(kGroupThreads, kAllFast ? 8 : 1)
                    const tg_crater_frame* __restrict__ frames, uint32_t hoisted, uint32_t chunk,
                    float* __restrict__ out) {
     const uint32_t lane = threadIdx.x & 63u;
+    const tg::CubeSteps steps = tg::MakeCubeSteps(lane);
     DrainBins(counters, chunk, [&](uint32_t b) {
         const tg_bin bin = bins[b];
         const tg_tile_desc tile = tiles[bin.tile];
@@ -401,11 +404,27 @@ __global__ void __launch_bounds__(kGroupThreads, kAllFast ? 8 : 1)
             const uint32_t row = squareRow + (lane >> 3u);
             const bool inside = col < tile.width && row < tile.height;
             // Lanes past the tile's edge evaluate its last column or row and store nothing.
-            const uint32_t cellCol = std::min(col, tile.width - 1u);
-            const uint32_t cellRow = std::min(row, tile.height - 1u);
-            const tg::Position at = tg::CellPosition(tile.origin_x, tile.origin_y, tile.pixel,
-                                                     cellCol, cellRow, radius);
-            float h = plan != nullptr ? elevation + tg::EvaluateStack3(*plan, at.x, at.y, at.z)
+            // A cell's longitude depends on its column and its latitude on its row, so lanes 0-7
+            // take the sine and cosine of the square's columns, lanes 8-15 of its rows, and every
+            // lane gathers its own: the position is CellPosition's, bit for bit.
+            const uint32_t which = lane & 15u;
+            const double angle =
+                which < 8u
+                    ? tile.origin_x +
+                          (double(std::min(squareCol + which, tile.width - 1u)) + 0.5) *
+                              tile.pixel +
+                          0.0
+                    : tile.origin_y +
+                          (double(std::min(squareRow + which - 8u, tile.height - 1u)) + 0.5) *
+                              -tile.pixel;
+            const tg::SinCos mine = tg::SinCosDegrees(angle);
+            const tg::SinCos lon{__shfl(mine.sin, int(lane & 7u), 64),
+                                 __shfl(mine.cos, int(lane & 7u), 64)};
+            const tg::SinCos lat{__shfl(mine.sin, int(8u + (lane >> 3u)), 64),
+                                 __shfl(mine.cos, int(8u + (lane >> 3u)), 64)};
+            const tg::Position at = tg::SpherePositionOf(lon, lat, radius);
+            float h = plan != nullptr ? elevation + tg::EvaluateStack3<true, kSmall>(
+                                                        *plan, at.x, at.y, at.z, lane, steps)
                                       : elevation;
             if (kernel != nullptr && bin.count > 0u) {
                 tg_split_point p;
@@ -567,7 +586,19 @@ void Launch(tg_batch& b, const tg_noise_plan* plan, float elevation, const tg_ca
     if (b.radius > 0.0) {
         // The length of a degree of arc, as Surface::Sphere sets it (surface.cpp:49).
         const double unitLength = b.radius * (3.141592653589793 / 180.0);
-        auto* body = allFast ? BatchBodyTiles<true> : BatchBodyTiles<false>;
+        // Lattice coordinates are at most |position| x frequency + 1 in magnitude (the turns
+        // are rotations), and positions are at most the radius.
+        double frequency = 0.0;
+        if (plan != nullptr) {
+            for (uint32_t i = 0; i < plan->host.count; ++i) {
+                for (uint32_t o = 0; o < plan->host.fractal[i].unit.octaves; ++o) {
+                    frequency = std::max(frequency, plan->host.body[i][o].frequency);
+                }
+            }
+        }
+        const bool small = b.radius * frequency * (1.0 + 1e-9) + 2.0 < 2147483648.0;
+        auto* body = allFast ? (small ? BatchBodyTiles<true, true> : BatchBodyTiles<true, false>)
+                             : (small ? BatchBodyTiles<false, true> : BatchBodyTiles<false, false>);
         hipLaunchKernelGGL(body, dim3(groups), dim3(kGroupThreads), 0, b.stream, b.tiles, b.bins,
                            b.entries, b.counters, plan != nullptr ? plan->device : nullptr,
                            elevation, b.radius, unitLength,

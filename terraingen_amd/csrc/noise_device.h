@@ -463,11 +463,86 @@ __device__ __forceinline__ float Perlin3(int32_t ix, int32_t iy, int32_t iz, flo
     return StdClamp(Mix(w, nearSide, farSide), -1.0f, 1.0f);
 }
 
+// The lane's point in a 4 x 4 x 4 block of lattice points (lane & 3, (lane >> 2) & 3, lane >> 4)
+// times the LCG multiplier, for Perlin3Shared; the empty asm keeps the compiler from
+// re-multiplying them every octave (as SquareSteps does in 2D).
+struct CubeSteps {
+    uint32_t x, y, z;
+};
+__device__ __forceinline__ CubeSteps MakeCubeSteps(uint32_t lane) {
+    uint32_t x = (lane & 3u) * kLcgMul;
+    uint32_t y = ((lane >> 2u) & 3u) * kLcgMul;
+    uint32_t z = (lane >> 4u) * kLcgMul;
+    asm("" : "+v"(x), "+v"(y), "+v"(z));
+    return {x, y, z};
+}
+
+// Perlin3 for a wave of 64 active lanes whose cells lie close together (a tile's 8 x 8 square on
+// a body). When every lane's lattice cell lies within one cell of lane 27's along each axis, the
+// 4 x 4 x 4 lattice points from one below lane 27's cell hold every corner of the wave: each lane
+// hashes one of them (one PCG hash instead of eight, whose 32-bit multiplies are the costliest
+// instructions of the noise), the gradient codes (the hash's low four bits, all PerlinGradient3
+// reads) of each point's cell are packed into one word by three lane exchanges, and each lane reads
+// its cell's word with a fourth. Otherwise every lane hashes its own eight corners. The hashes'
+// low bits, and so the noise, are Perlin3's.
+__device__ __forceinline__ float Perlin3Shared(int32_t ix, int32_t iy, int32_t iz, float fx,
+                                               float fy, float fz, uint32_t zKey, uint32_t lane,
+                                               const CubeSteps& steps) {
+    const int32_t bx = __builtin_amdgcn_readlane(ix, 27) - 1;
+    const int32_t by = __builtin_amdgcn_readlane(iy, 27) - 1;
+    const int32_t bz = __builtin_amdgcn_readlane(iz, 27) - 1;
+    const uint32_t rx = uint32_t(ix - bx), ry = uint32_t(iy - by), rz = uint32_t(iz - bz);
+    if (__ballot(rx > 2u || ry > 2u || rz > 2u) != 0ull) {
+        return Perlin3(ix, iy, iz, fx, fy, fz, zKey);
+    }
+    const uint32_t ax = uint32_t(bx) * kLcgMul + kLcgAdd + steps.x;
+    const uint32_t ay = uint32_t(by) * kLcgMul + kLcgAdd + steps.y;
+    const uint32_t az = (uint32_t(bz) + zKey) * kLcgMul + kLcgAdd + steps.z;
+    // Nibble k of a point's word: the code of corner (k & 1, (k >> 1) & 1, k >> 2) of its cell.
+    // Lanes whose point has no cell in the block (an index of 3) pack words no lane reads.
+    const auto from = [](uint32_t byteAddress, uint32_t value) {
+        return uint32_t(__builtin_amdgcn_ds_bpermute(int32_t(byteAddress), int32_t(value)));
+    };
+    uint32_t word = CornerX(ax, ay, az, ay * az) & 15u;
+    const uint32_t byteAt = lane << 2u;
+    word |= from(byteAt + 4u, word) << 4u;    // the point one along x
+    word |= from(byteAt + 16u, word) << 8u;   // one along y
+    word |= from(byteAt + 64u, word) << 16u;  // one along z
+    const uint32_t codes = from((rx + 4u * ry + 16u * rz) << 2u, word);
+    const float c000 = Grad3(codes, fx, fy, fz);
+    const float c100 = Grad3(codes >> 4u, fx - 1.0f, fy, fz);
+    const float c010 = Grad3(codes >> 8u, fx, fy - 1.0f, fz);
+    const float c110 = Grad3(codes >> 12u, fx - 1.0f, fy - 1.0f, fz);
+    const float c001 = Grad3(codes >> 16u, fx, fy, fz - 1.0f);
+    const float c101 = Grad3(codes >> 20u, fx - 1.0f, fy, fz - 1.0f);
+    const float c011 = Grad3(codes >> 24u, fx, fy - 1.0f, fz - 1.0f);
+    const float c111 = Grad3(codes >> 28u, fx - 1.0f, fy - 1.0f, fz - 1.0f);
+    const float u = Fade(fx);
+    const float v = Fade(fy);
+    const float w = Fade(fz);
+    const float nearSide = Mix(v, Mix(u, c000, c100), Mix(u, c010, c110));
+    const float farSide = Mix(v, Mix(u, c001, c101), Mix(u, c011, c111));
+    return StdClamp(Mix(w, nearSide, farSide), -1.0f, 1.0f);
+}
+
 // Fractal3 (fractal.h:284-317) for the Perlin basis at body-fixed (x, y, z) meters: the octave's
 // FP64 turn and frequency from the plan, lattice coordinates split in FP64 before narrowing
 // (SplitLatticePrecise3, lattice.h:59-69), the octave sums as the 2D loop's.
+// kShared: the caller's 64 lanes are all active and their cells close together, as
+// Perlin3Shared requires, and `lane` and `steps` are the lane's.
+// kSmall: every lattice coordinate lies within 2^31 of the origin (the caller has bounded the
+// positions and frequencies), so int32_t(int64_t(c)) is one conversion of the double.
+template <bool kShared = false, bool kSmall = false>
 __device__ __forceinline__ float FractalLoop3(const FractalLoopPlan& p, const BodyOctave* body,
-                                              double x, double y, double z) {
+                                              double x, double y, double z, uint32_t lane = 0u,
+                                              CubeSteps steps = {}) {
+    const auto index = [](double c) {
+        if constexpr (kSmall) {
+            return int32_t(c);
+        } else {
+            return int32_t(int64_t(c));
+        }
+    };
     const uint32_t type = p.type;
     float sum = 0.0f;
     float feedback = 1.0f;
@@ -476,14 +551,20 @@ __device__ __forceinline__ float FractalLoop3(const FractalLoopPlan& p, const Bo
         const double rx = b.turn[0] * x + b.turn[1] * y + b.turn[2] * z;
         const double ry = b.turn[3] * x + b.turn[4] * y + b.turn[5] * z;
         const double rz = b.turn[6] * x + b.turn[7] * y + b.turn[8] * z;
-        const double px = rx * b.frequency + double(b.offsetX);
-        const double py = ry * b.frequency + double(b.offsetY);
-        const double pz = rz * b.frequency + double(b.offsetZ);
+        const double px = rx * b.frequency + b.offsetX;
+        const double py = ry * b.frequency + b.offsetY;
+        const double pz = rz * b.frequency + b.offsetZ;
         const double cx = floor(px);
         const double cy = floor(py);
         const double cz = floor(pz);
-        const float n = Perlin3(int32_t(int64_t(cx)), int32_t(int64_t(cy)), int32_t(int64_t(cz)),
-                                float(px - cx), float(py - cy), float(pz - cz), Pcg(b.seed));
+        float n;
+        if constexpr (kShared) {
+            n = Perlin3Shared(index(cx), index(cy), index(cz), float(px - cx), float(py - cy),
+                              float(pz - cz), b.zKey, lane, steps);
+        } else {
+            n = Perlin3(index(cx), index(cy), index(cz), float(px - cx), float(py - cy),
+                        float(pz - cz), b.zKey);
+        }
         const float weight = p.octave[o].weight;
         switch (type) {
         case TG_FRACTAL_BILLOW:
@@ -527,13 +608,16 @@ __device__ __forceinline__ float FractalLoop3(const FractalLoopPlan& p, const Bo
 
 // EvaluateFractals3 (fractals.h:235-269) for a plan whose fractals are unwarped Perlin ones
 // (NoisePlan::bodyReady): the stack's height at body-fixed (x, y, z) meters.
+template <bool kShared = false, bool kSmall = false>
 __device__ __forceinline__ float EvaluateStack3(const NoisePlan& plan, double x, double y,
-                                                double z) {
+                                                double z, uint32_t lane = 0u,
+                                                CubeSteps steps = {}) {
     float height = 0.0f;
     const uint32_t count = plan.count;
     for (uint32_t i = 0; i < count; ++i) {
         const FractalPlan& f = plan.fractal[i];
-        const float value = FractalLoop3(f.unit, plan.body[i], x, y, z);
+        const float value =
+            FractalLoop3<kShared, kSmall>(f.unit, plan.body[i], x, y, z, lane, steps);
         const float scaled = f.amplitude * value;
         if (i == 0) {
             height = scaled;

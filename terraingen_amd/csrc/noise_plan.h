@@ -49,12 +49,13 @@ struct FractalLoopPlan {
 struct BodyOctave {
     double turn[9];      // row-major
     double frequency;    // 1 / wavelength times lacunarity per octave
-    float offsetX;       // HashToUnitFloat(Pcg3dHash(seed, octave, 0).y)
-    float offsetY;       // HashToUnitFloat(Pcg3dHash(seed, octave, 0).z)
-    float offsetZ;       // HashToUnitFloat(Pcg3dHash(seed, octave, 1).x)
-    uint32_t seed;       // Pcg3dHash(seed, octave, 0).x
+    double offsetX;      // HashToUnitFloat(Pcg3dHash(seed, octave, 0).y), widened exactly
+    double offsetY;      // HashToUnitFloat(Pcg3dHash(seed, octave, 0).z)
+    double offsetZ;      // HashToUnitFloat(Pcg3dHash(seed, octave, 1).x)
+    uint32_t zKey;       // PcgHash(Pcg3dHash(seed, octave, 0).x): PerlinLatticeKey3's z term
+    uint32_t pad;
 };
-static_assert(sizeof(BodyOctave) == 96, "BodyOctave is 12 doubles");
+static_assert(sizeof(BodyOctave) == 112, "BodyOctave is 14 doubles");
 
 // One enabled fractal of a stack.
 struct FractalPlan {
@@ -159,10 +160,11 @@ inline void PlanBody(BodyOctave* octaves, uint32_t count, uint32_t seed, float w
             }
         }
         b.frequency = frequency;
-        b.offsetX = float(h[1] >> 8u) * 0x1p-24f;
-        b.offsetY = float(h[2] >> 8u) * 0x1p-24f;
-        b.offsetZ = float(g[0] >> 8u) * 0x1p-24f;
-        b.seed = h[0];
+        b.offsetX = double(float(h[1] >> 8u) * 0x1p-24f);
+        b.offsetY = double(float(h[2] >> 8u) * 0x1p-24f);
+        b.offsetZ = double(float(g[0] >> 8u) * 0x1p-24f);
+        b.zKey = HostPcg(h[0]);
+        b.pad = 0u;
         frequency *= static_cast<double>(lacunarity);
         double next[3][3] = {};
         for (int r = 0; r < 3; ++r) {

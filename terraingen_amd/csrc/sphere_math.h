@@ -6,10 +6,11 @@
 // cosine. Here the sine and cosine are this file's own: the angle is reduced in degrees, which is
 // exact (x - 90 k is representable whenever it is no larger than x), then converted to radians and
 // summed as Taylor series on [-pi/4, pi/4] by Horner's rule in a fixed order, so the host and the
-// device (both built with FP contraction off) compute identical positions. They differ from the C
-// library's by about an ulp, 2e-10 m at a lunar radius.
+// device (both built with FP contraction off, both rounding to nearest) compute identical
+// positions. They differ from the C library's by about an ulp, 2e-10 m at a lunar radius.
 #pragma once
 
+#include <cmath>
 #include <cstdint>
 
 #if defined(__HIPCC__)
@@ -26,10 +27,9 @@ struct SinCos {
 
 // sin and cos of `degrees`.
 TG_HOST_DEVICE inline SinCos SinCosDegrees(double degrees) {
-    const double quarter = degrees / 90.0;
-    // Nearest quarter turn, rounded half away from zero.
-    const double k = quarter >= 0.0 ? static_cast<double>(static_cast<int64_t>(quarter + 0.5))
-                                    : -static_cast<double>(static_cast<int64_t>(0.5 - quarter));
+    // The nearest quarter turn k (ties to even; any k within a turn of the nearest keeps the
+    // series accurate), and the remainder r in degrees, which is exact.
+    const double k = rint(degrees * (1.0 / 90.0));
     const double r = (degrees - 90.0 * k) * 0.017453292519943295;  // pi / 180
     const double r2 = r * r;
     // r - r^3/3! + ... - r^15/15!, and 1 - r^2/2! + ... + r^16/16!.
@@ -51,28 +51,28 @@ TG_HOST_DEVICE inline SinCos SinCosDegrees(double degrees) {
                                             r2 * (1.0 / 479001600.0 +
                                                   r2 * (-1.0 / 87178291200.0 +
                                                         r2 * (1.0 / 20922789888000.0))))))));
-    const int64_t q = static_cast<int64_t>(k) & 3;
-    switch (q) {
-    case 0:
-        return {s, c};
-    case 1:
-        return {c, -s};
-    case 2:
-        return {-s, -c};
-    default:
-        return {-c, s};
-    }
+    // k mod 4, exactly (every step is exact for an integral k), picks the quadrant by selects.
+    const double quadrant = k - 4.0 * floor(k * 0.25);
+    const bool odd = quadrant == 1.0 || quadrant == 3.0;
+    const double a = odd ? c : s;
+    const double b = odd ? s : c;
+    const bool negateSin = quadrant >= 2.0;
+    const bool negateCos = quadrant == 1.0 || quadrant == 2.0;
+    return {negateSin ? -a : a, negateCos ? -b : b};
 }
 
 struct Position {
     double x, y, z;
 };
 
+// Surface::Position from the sine and cosine of the longitude and of the latitude.
+TG_HOST_DEVICE inline Position SpherePositionOf(SinCos lon, SinCos lat, double radius) {
+    return {radius * (lat.cos * lon.cos), radius * (lat.cos * lon.sin), radius * lat.sin};
+}
+
 // Surface::Position of (`lonDeg`, `latDeg`) on a sphere of `radius` meters.
 TG_HOST_DEVICE inline Position SpherePosition(double lonDeg, double latDeg, double radius) {
-    const SinCos lon = SinCosDegrees(lonDeg);
-    const SinCos lat = SinCosDegrees(latDeg);
-    return {radius * (lat.cos * lon.cos), radius * (lat.cos * lon.sin), radius * lat.sin};
+    return SpherePositionOf(SinCosDegrees(lonDeg), SinCosDegrees(latDeg), radius);
 }
 
 // GeoTransform::PixelToWorld of a north-up raster's cell centre, in degrees

@@ -76,6 +76,40 @@ def test_sphere_hoisted_path_is_bit_identical(hip, batch):
     assert np.array_equal(full.view(np.uint32), hoisted.view(np.uint32))
 
 
+@pytest.mark.parametrize("fractals", [
+    [C2_FRACTAL],
+    [dict(seed=7, wavelength=40.0, amplitude=3.0, octaves=12, lacunarity=2.2, gain=0.55),
+     dict(seed=3, type=1, offset=0.9, wavelength=900.0, octaves=6, blend=1)],
+])
+def test_sphere_noise_bit_exact_across_cell_sizes(hip, orc, fractals):
+    """Tiles on the Moon from 3 cm to 60 m a cell, noise only, so an octave's lattice cells per
+    8 x 8 square range from far under one (corner hashes shared across the wave) to many (hashed
+    per lane); each tile's heights equal the oracle's bit for bit."""
+    moon = 1737400.0
+    b = SphereTileBatch(hip, fractals, CraterLayerSpec(seed=4), region=None, radius=moon,
+                        craters_given=global_craters(1, moon, max_diameter=2000.0, seed=1))
+    b.build()
+    tiles = sphere_tiles(16, moon, seed=5, min_k=3, max_k=7, pixel_range=(0.03, 60.0))
+    b.stage(tiles)
+    try:
+        got = run_batch(hip, b, use_graph=False)
+        for i in range(tiles.size):
+            inp = b.plain_inputs(i)
+            g = inp["grid"]
+            want = np.zeros((g.height, g.width), np.float32)
+            params = N.KernelParams(g.width, g.height, 0, 0.0, 0)
+            pos = inp["positions"]
+            orc.or_body_noise_grid(C.byref(b.noise_kernel), pos.ctypes.data, pos.shape[0],
+                                   C.byref(params), want.ctypes.data, 0)
+            if inp["craters"] is not None:
+                continue  # the one crater reaches this tile
+            mine = b.tile_heights(got, i)
+            assert np.array_equal(mine.view(np.uint32), want.view(np.uint32)), (
+                i, float(tiles["pixel"][i]) * moon * np.pi / 180.0, int(g.width))
+    finally:
+        b.close()
+
+
 def frames64(lon, lat, radius):
     """MakeCraterFrame in FP64 (centre, east, north, radius; 10 values per crater)."""
     lam, phi = np.radians(lon), np.radians(lat)
