@@ -59,11 +59,14 @@ typedef struct tgx_crater {
     uint64_t id;
 } tgx_crater;
 
-/* Generation of craters from a power-law density N(>=D) = sfd_coefficient D^-sfd_exponent per m^2
- * over the region [min_x, max_x) x [min_y, max_y) meters: GenerateCraters
+/* Generation of craters over the region [min_x, max_x) x [min_y, max_y) meters: GenerateCraters
  * (src/layers/craters/generation/population.cpp:414-424) with CraterGenerationOptions'
- * minDiameter, maxDiameter, marginRadii, cellSizeFactor (0 = 16) and minResolvedDiameter, ages
- * uniform over lifetime_gyr (0 = fresh). */
+ * minDiameter, maxDiameter, marginRadii, cellSizeFactor (0 = 16) and minResolvedDiameter.
+ * `model` 0: a power-law density N(>=D) = sfd_coefficient D^-sfd_exponent per m^2 and ages uniform
+ * over lifetime_gyr (0 = fresh). `model` 1: the density and ages of DefaultCraterModels
+ * (crater_layer.cpp:194-203): the lunar equilibrium surface (Neukum production over 3.5 Gyr
+ * capped by Hartmann's equilibrium) with steady-state ages; the sfd and lifetime fields are
+ * ignored. */
 typedef struct tgx_generation {
     double min_x, min_y, max_x, max_y;
     double sfd_coefficient, sfd_exponent;
@@ -73,6 +76,7 @@ typedef struct tgx_generation {
     double min_resolved_diameter;
     double lifetime_gyr;
     uint32_t seed;
+    uint32_t model;
 } tgx_generation;
 
 /* Philox4x32-10 of a 4-word counter under a 2-word key (include/terraingen/random/philox.h:79-91),
@@ -83,15 +87,20 @@ void tgx_philox4x32(const uint32_t* counter, const uint32_t* key, uint32_t* out)
 int tgx_generate_craters(const tgx_generation* spec, tgx_crater** craters, size_t* count,
                          char* err, size_t err_len);
 
-/* The crater models a layer shapes with: `profile` TG_PROFILE_MPD or TG_PROFILE_FASSETT_THOMSON,
- * complex craters (ComplexCraterProfile with the lunar shape) from `transition_diameter` when
- * `complex_` is 1, and FreshnessDegradation with the lunar fresh morphometry and ages uniform over
- * `lifetime_gyr` (include/terraingen/layers/craters/crater_layer.h:29-46). */
+/* The crater models a layer shapes with (include/terraingen/layers/craters/crater_layer.h:29-46).
+ * `model` 0: `profile` TG_PROFILE_MPD or TG_PROFILE_FASSETT_THOMSON, complex craters
+ * (ComplexCraterProfile with the lunar shape) from `transition_diameter` when `complex_` is 1, and
+ * FreshnessDegradation with the lunar fresh morphometry and ages uniform over `lifetime_gyr`.
+ * `model` 1: DefaultCraterModels (crater_layer.cpp:194-203): the Martin-Parkes-Dunstan profile
+ * diffused in time with matched diffusion over the lunar steady-state ages (the other fields are
+ * ignored). */
 typedef struct tgx_crater_models {
     uint32_t profile;
     uint32_t complex_;
     double transition_diameter;
     double lifetime_gyr;
+    uint32_t model;
+    uint32_t pad;
 } tgx_crater_models;
 
 /* The CraterApplyOptions switches of the hot path (apply_options.h:296-318): irregular outlines,

@@ -18,6 +18,8 @@
 #include "random.h"
 
 #include <cstdlib>
+#include <limits>
+#include <utility>
 #include <memory>
 
 namespace tgh {
@@ -42,11 +44,182 @@ double PowerLawDensity::SampleDiameter(Stream& random, double minD, double maxD)
     return std::clamp(Invert(target), minD, maxD);
 }
 
-double Ages::Sample(Stream& random) const {
-    if (lifetimeGyr == 0.0) {
-        return 0.0;
+double Ages::Lifetime(double diameter) const {
+    if (!(diameter > 0.0 && std::isfinite(diameter))) {
+        throw std::domain_error("Diameter must be positive and finite");
     }
-    return std::clamp(random.NextDouble() * lifetimeGyr, 0.0, lifetimeGyr);
+    return surface != nullptr ? surface->LifetimeGyr(diameter) : lifetimeGyr;
+}
+
+namespace {
+
+// BisectRising (src/util/numerics.h:63-127) with the Illinois bracket (:30-61): the interval of
+// `Bisect` on `value(x) >= target` for a non-decreasing value, reached with fewer evaluations.
+struct Illinois {
+    double a, b, fa, fb;
+    int side = 0;
+    double Next() const {
+        const double c = b - fb * (b - a) / (fb - fa);
+        return c > a && c < b ? c : 0.5 * (a + b);
+    }
+    void Update(double c, double fc) {
+        if ((fc < 0.0) == (fa < 0.0)) {
+            a = c;
+            fa = fc;
+            fb *= side == -1 ? 0.5 : 1.0;
+            side = -1;
+        } else {
+            b = c;
+            fb = fc;
+            fa *= side == 1 ? 0.5 : 1.0;
+            side = 1;
+        }
+    }
+};
+
+template <class Value>
+std::pair<double, double> BisectRising(double low, double high, int steps, double tolerance,
+                                       double target, const Value& value) {
+    constexpr int kEstimateSteps = 12;
+    const double margin = 1e-13 * (high - low);
+    Illinois bracket{low, high, value(low) - target, value(high) - target};
+    const bool belowKnown = bracket.fa < 0.0;
+    const bool aboveKnown = !(bracket.fb < 0.0);
+    double below = low;
+    double above = high;
+    if (belowKnown && aboveKnown) {
+        double c = low;
+        for (int i = 0; i < kEstimateSteps; ++i) {
+            const double previous = c;
+            c = bracket.Next();
+            bracket.Update(c, value(c) - target);
+            if (i > 0 && std::fabs(c - previous) <= margin) {
+                break;
+            }
+        }
+        const double a = bracket.a;
+        const double b = bracket.b;
+        below = a;
+        above = b;
+        for (const double spread : {margin, 64.0 * margin}) {
+            if (b - a <= 2.0 * spread) {
+                break;
+            }
+            if (c - spread > a && value(c - spread) < target) {
+                below = c - spread;
+            }
+            if (c + spread < b && !(value(c + spread) < target)) {
+                above = c + spread;
+            }
+            if (above - below <= 4.0 * spread) {
+                break;
+            }
+        }
+    }
+    const double aboveFrom = aboveKnown ? above + margin : std::numeric_limits<double>::infinity();
+    const double belowTo = belowKnown ? below - margin : -std::numeric_limits<double>::infinity();
+    for (int i = 0; i < steps && high - low > tolerance; ++i) {
+        const double middle = 0.5 * (low + high);
+        bool isAbove = middle >= aboveFrom;
+        if (!isAbove && !(middle <= belowTo)) {
+            isAbove = !(value(middle) < target);
+        }
+        high = isAbove ? middle : high;
+        low = isAbove ? low : middle;
+    }
+    return {low, high};
+}
+
+}  // namespace
+
+double Ages::Sample(Stream& random, double diameter) const {
+    const double lifetime = Lifetime(diameter);
+    double age = 0.0;
+    if (surface != nullptr) {
+        // SteadyStateAges::SampleAgeAt (steady_state.cpp:53-59): a formation age by bisection on
+        // the chronology's N1(T) (SampleFormationAge, :17-24).
+        if (lifetime != 0.0) {
+            const double target = random.NextDouble() * NeukumChronology(lifetime);
+            const auto found = BisectRising(0.0, lifetime, 100, 1e-15 * lifetime, target,
+                                            [](double t) { return NeukumChronology(t); });
+            age = 0.5 * (found.first + found.second);
+        }
+    } else if (lifetime != 0.0) {
+        age = random.NextDouble() * lifetime;  // UniformAges::SampleAgeAt
+    }
+    return std::clamp(age, 0.0, lifetime);
+}
+
+double LunarDensity::ExpectedCount(double area, double minD, double maxD) const {
+    if (!(minD > 0.0 && std::isfinite(minD) && maxD > minD) || std::isnan(maxD)) {
+        throw std::invalid_argument("Diameter interval must satisfy 0 < minDiameter < maxDiameter");
+    }
+    if (!(area >= 0.0 && std::isfinite(area))) {
+        throw std::invalid_argument("Area must be finite and not negative");
+    }
+    if (std::isinf(maxD)) {
+        throw std::domain_error("An infinite maxDiameter needs an unbounded valid range");
+    }
+    const double upper = m_surface.DensityChecked(maxD);
+    return area * (m_surface.DensityChecked(minD) - upper);
+}
+
+double LunarDensity::Invert(double target, double minD, double maxD) const {
+    const auto cumulative = [this](double d) { return m_surface.Density(d); };
+    double low = std::log(minD);
+    double high = std::isinf(maxD) ? low + 1.0 : std::log(maxD);
+    while (std::isinf(maxD) && cumulative(std::exp(high)) > target) {
+        high += 1.0;
+    }
+    if (target > 0.0) {
+        // InvertBySecant (size_frequency.cpp:34-61): Illinois steps on log N over log D.
+        const double logTarget = std::log(target);
+        const auto excess = [&](double u) { return std::log(cumulative(std::exp(u))) - logTarget; };
+        Illinois bracket{low, high, excess(low), excess(high)};
+        if (!(bracket.fa > 0.0)) {
+            return std::exp(low);
+        }
+        if (!(bracket.fb < 0.0)) {
+            return std::exp(high);
+        }
+        double u = low;
+        bool converged = false;
+        for (int i = 0; i < 100; ++i) {
+            const double previous = u;
+            u = bracket.Next();
+            const double g = excess(u);
+            if (std::isnan(g)) {
+                break;
+            }
+            if (g == 0.0 || std::fabs(u - previous) <= 1e-14 * std::fmax(1.0, std::fabs(u))) {
+                converged = true;
+                break;
+            }
+            bracket.Update(u, g);
+        }
+        if (converged) {
+            return std::exp(u);
+        }
+        low = bracket.a;
+        high = bracket.b;
+    }
+    // Bisect (numerics.h:22-29) on !(N(exp u) > target).
+    const double tolerance = 1e-15 * std::fmax(1.0, std::fabs(high));
+    for (int i = 0; i < 200 && high - low > tolerance; ++i) {
+        const double middle = 0.5 * (low + high);
+        (!(cumulative(std::exp(middle)) > target) ? high : low) = middle;
+    }
+    return std::exp(0.5 * (low + high));
+}
+
+double LunarDensity::SampleDiameter(Stream& random, double minD, double maxD) const {
+    const double count = ExpectedCount(1.0, minD, maxD);
+    if (!(count > 0.0)) {
+        throw std::invalid_argument("The distribution has no mass in the diameter interval");
+    }
+    const double top = m_surface.DensityChecked(minD);
+    const double target = top - random.NextDouble() * count;
+    return std::clamp(Invert(target, minD, maxD), minD, maxD);
 }
 
 std::vector<Band> CraterBands(double minD, double maxD, double cellSizeFactor) {
@@ -100,7 +273,7 @@ std::vector<CellIndex> CellsCovering(const Box& r, double cellSize) {
 
 class Generator {
   public:
-    Generator(const Box& region, const PowerLawDensity& density, const Ages& ages,
+    Generator(const Box& region, const Density& density, const Ages& ages,
               std::uint32_t seed, const GenerationOptions& o)
         : m_region(region), m_density(density), m_ages(ages), m_seed(seed), m_o(o) {
         if (!(o.maxDiameter > o.minDiameter) || !std::isfinite(o.maxDiameter)) {
@@ -170,7 +343,7 @@ class Generator {
             const double v = geometry.NextDouble();
             const double x = (cell.i + u) * band.cellSize;
             const double y = (cell.j + v) * band.cellSize;
-            const double age = m_ages.Sample(ageStream);
+            const double age = m_ages.Sample(ageStream, d);
             const Words4 id = Philox({i, j, k, static_cast<std::uint32_t>(index)}, m_seed,
                                      kCraterIdStream);
             out.push_back({x, y, d, age, JoinWords(id.x, id.y)});
@@ -209,7 +382,7 @@ class Generator {
     }
 
     Box m_region;
-    const PowerLawDensity& m_density;
+    const Density& m_density;
     const Ages& m_ages;
     std::uint32_t m_seed;
     GenerationOptions m_o;
@@ -218,7 +391,7 @@ class Generator {
 }  // namespace
 
 std::vector<Crater> GenerateCraters(double minX, double minY, double maxX, double maxY,
-                                    const PowerLawDensity& density, const Ages& ages,
+                                    const Density& density, const Ages& ages,
                                     std::uint32_t seed, const GenerationOptions& options) {
     if (!(maxX > minX && maxY > minY) || !std::isfinite(minX + minY + maxX + maxY)) {
         throw std::invalid_argument("Region must be finite and not empty");
@@ -245,8 +418,15 @@ int tgx_generate_craters(const tgx_generation* g, tgx_crater** out, size_t* coun
         if (g == nullptr || out == nullptr || count == nullptr) {
             throw std::invalid_argument("tgx_generate_craters: null argument");
         }
-        const tgh::PowerLawDensity density{g->sfd_coefficient, g->sfd_exponent};
-        const tgh::Ages ages{g->lifetime_gyr};
+        // model 1: DefaultCraterModels' lunar equilibrium surface and steady-state ages.
+        if (g->model > 1u) {
+            throw std::invalid_argument("tgx_generate_craters: unknown crater model");
+        }
+        const tgh::PowerLawDensity powerLaw{g->sfd_coefficient, g->sfd_exponent};
+        const tgh::LunarDensity lunar;
+        const tgh::Density& density =
+            g->model == 1u ? static_cast<const tgh::Density&>(lunar) : powerLaw;
+        tgh::Ages ages{g->lifetime_gyr, g->model == 1u ? &lunar.Surface() : nullptr};
         tgh::GenerationOptions o;
         o.minDiameter = g->min_diameter;
         o.maxDiameter = g->max_diameter;

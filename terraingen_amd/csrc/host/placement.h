@@ -3,6 +3,7 @@
 // Host placement of craters and boulder candidates (placement.cpp).
 #pragma once
 
+#include "lunar_models.h"
 #include "random.h"
 
 #include <algorithm>
@@ -13,10 +14,23 @@
 
 namespace tgh {
 
+// A cumulative crater density: the expected count per area in a diameter interval, and a
+// diameter drawn from it (SizeFrequencyDistribution::ExpectedCount and SampleDiameter,
+// size_frequency.cpp:116-140).
+class Density {
+  public:
+    virtual ~Density() = default;
+    virtual double ExpectedCount(double area, double minD, double maxD) const = 0;
+    virtual double SampleDiameter(Stream& random, double minD, double maxD) const = 0;
+};
+
 // N(>=D) = coefficient D^-exponent per m^2 (PowerLawSegment, power_law.h:15-33).
-struct PowerLawDensity {
+struct PowerLawDensity final : Density {
     double coefficient = 1.0;
     double exponent = 2.0;
+
+    PowerLawDensity() = default;
+    PowerLawDensity(double c, double e) : coefficient(c), exponent(e) {}
 
     double Cumulative(double d) const {
         if (!(d > 0.0 && std::isfinite(d))) {
@@ -27,20 +41,43 @@ struct PowerLawDensity {
     double Invert(double target) const {
         return std::pow(target / coefficient, -1.0 / exponent);
     }
-    double ExpectedCount(double area, double minD, double maxD) const;
-    double SampleDiameter(Stream& random, double minD, double maxD) const;
+    double ExpectedCount(double area, double minD, double maxD) const override;
+    double SampleDiameter(Stream& random, double minD, double maxD) const override;
 };
 
-// Crater ages: uniform over a lifetime in Gyr (UniformAges), or all fresh at 0 (FreshAges).
+// The density of the reference's default crater models (lunar_models.h).
+class LunarDensity final : public Density {
+  public:
+    double ExpectedCount(double area, double minD, double maxD) const override;
+    double SampleDiameter(Stream& random, double minD, double maxD) const override;
+    const LunarSurface& Surface() const {
+        return m_surface;
+    }
+
+  private:
+    // SizeFrequencyDistribution::InvertCumulative (size_frequency.cpp:95-113).
+    double Invert(double target, double minD, double maxD) const;
+    LunarSurface m_surface;
+};
+
+// Crater ages (CraterAgeModel, age_model.cpp:9-51): uniform over a lifetime in Gyr (UniformAges),
+// all fresh at 0 (FreshAges, a lifetime of 0), or the lunar steady state (SteadyStateAges of the
+// default models, when `surface` is set).
 struct Ages {
     double lifetimeGyr = 0.0;
-    double Sample(Stream& random) const;
-    // CraterAgeModel::RemainingLifetime (src/layers/craters/ages/age_model.cpp).
-    double Remaining(double ageGyr) const {
-        if (lifetimeGyr == 0.0) {
+    const LunarSurface* surface = nullptr;
+
+    // CraterAgeModel::LifetimeGyr.
+    double Lifetime(double diameter) const;
+    // CraterAgeModel::SampleAgeGyr.
+    double Sample(Stream& random, double diameter) const;
+    // CraterAgeModel::RemainingLifetime.
+    double Remaining(double diameter, double ageGyr) const {
+        const double lifetime = Lifetime(diameter);
+        if (lifetime == 0.0) {
             return 1.0;
         }
-        return std::clamp(1.0 - ageGyr / lifetimeGyr, 0.0, 1.0);
+        return std::clamp(1.0 - ageGyr / lifetime, 0.0, 1.0);
     }
 };
 
@@ -66,7 +103,7 @@ struct Crater {
 };
 
 std::vector<Crater> GenerateCraters(double minX, double minY, double maxX, double maxY,
-                                    const PowerLawDensity& density, const Ages& ages,
+                                    const Density& density, const Ages& ages,
                                     std::uint32_t seed, const GenerationOptions& options);
 
 }  // namespace tgh

@@ -17,12 +17,15 @@
 //               (crater_kernel.cpp:181-199,312-519,581-606).
 
 #include "common.h"
+#include "lunar_models.h"
 #include "placement.h"
 #include "prepare.h"
 #include "random.h"
 
 #include <algorithm>
 #include <cstdlib>
+#include <map>
+#include <memory>
 #include <numeric>
 
 namespace tgh {
@@ -217,6 +220,9 @@ tg_crater_shape DefaultShapeKernel() {
 }
 
 double ProfileReachRadii(const CraterModels& m) {
+    if (m.diffused) {
+        return DiffusionTables::kReachRadii;  // DiffusedProfile::ReachRadii (diffusion.h:65-67)
+    }
     const double simple = m.profile == TG_PROFILE_MPD ? 2.0 : 1.5;
     return m.complex_ ? std::fmax(simple, 3.0) : simple;
 }
@@ -250,6 +256,15 @@ ApplyOptions SeededApply(ApplyOptions o, std::uint32_t layerSeed) {
 tg_crater_shape MakeShapeKernel(const CraterModels& m, const ApplyOptions& o) {
     tg_crater_shape k = DefaultShapeKernel();
     k.profile.simple = m.profile;
+    if (m.diffused) {
+        // DiffusedProfile's kernel block (diffusion.cpp:176-181): tables of 600 cells over half
+        // the reach, the ages first in the data.
+        k.profile.diffused = 1u;
+        k.profile.table_extent = static_cast<float>(0.5 * DiffusionTables::kReachRadii);
+        k.profile.table_cells = DiffusionTables::kCells;
+        k.profile.ages_offset = 0u;
+        k.profile.age_count = static_cast<std::uint32_t>(DiffusionAges().size());
+    }
     if (m.complex_) {
         // SetKernelComplexShape (complex.cpp:29-43) with the lunar complex shape's laws.
         k.profile.complex_ = 1u;
@@ -323,6 +338,17 @@ Batch PrepareBatch(const std::vector<tgh::Crater>& craters, const CraterModels& 
     Batch b;
     b.kernel = MakeShapeKernel(m, o);
     const double reachRadii = CraterReachRadii(m, o);
+    // The default models' steady state and diffusion tables (ProfileTables::AppendAges puts the
+    // tables' ages first in the data, crater_kernel.cpp:215-222).
+    std::unique_ptr<LunarSurface> surface;
+    DiffusionTables tables;
+    std::map<std::pair<double, double>, std::uint32_t> tableOffsets;
+    if (m.diffused) {
+        surface = std::make_unique<LunarSurface>();
+        for (const double age : DiffusionAges()) {
+            b.data.push_back(static_cast<float>(age));
+        }
+    }
     for (const std::size_t k : ApplyOrder(craters)) {
         const tgh::Crater& c = craters[k];
         const double resolution = 1.0 - FalloffSmoothstep(c.diameter / spacing, 2.0, 4.0);
@@ -330,14 +356,41 @@ Batch PrepareBatch(const std::vector<tgh::Crater>& craters, const CraterModels& 
             continue;
         }
         const Outline outline = IrregularOutline(c, o.irregularity);
-        // FreshnessDegradation::State.
         const Morphometry fresh = LunarFresh(c.diameter);
-        const double floor = std::fmin(m.cutoffDepthRatio / fresh.depthRatio, 1.0);
-        const double erosion = floor + (1.0 - floor) * m.ages.Remaining(c.ageGyr);
-        const Morphometry current{erosion * fresh.depthRatio, erosion * fresh.rimHeightRatio};
+        Morphometry current;
+        double tau = 0.0;
+        if (m.diffused) {
+            // DiffusionDegradation::WornState (diffusion.cpp:299-316) with the matched diffusion
+            // ages (EquilibriumDiffusionAges, diffusivities.cpp:28-46): worn to the visibility
+            // depth ratio at the end of the crater's equilibrium lifetime.
+            if (!(c.ageGyr >= 0.0 && std::isfinite(c.ageGyr))) {
+                throw std::domain_error("Crater age must be finite and not negative");
+            }
+            const Morphometry2 f{fresh.depthRatio, fresh.rimHeightRatio};
+            const double cut = tables.AgeAtDepthRatio(f, m.cutoffDepthRatio);
+            const double kappaT = 1.0 * (cut * c.diameter * c.diameter * c.ageGyr /
+                                         surface->EquilibriumLifetimeGyr(c.diameter));
+            if (!(kappaT >= 0.0 && std::isfinite(kappaT))) {
+                throw std::domain_error(
+                    "A crater's diffusion age must be finite and not negative");
+            }
+            tau = kappaT / (c.diameter * c.diameter);
+            const Morphometry2 worn = tables.MorphometryAt(f, tau);
+            current = {worn.depthRatio, worn.rimHeightRatio};
+        } else {
+            // FreshnessDegradation::State.
+            const double floor = std::fmin(m.cutoffDepthRatio / fresh.depthRatio, 1.0);
+            const double erosion =
+                floor + (1.0 - floor) * m.ages.Remaining(c.diameter, c.ageGyr);
+            current = {erosion * fresh.depthRatio, erosion * fresh.rimHeightRatio};
+        }
+        // CheckProfileState (crater_kernel.cpp:172-184): a crater the MPD law shapes directly,
+        // in the fresh state under a diffused profile.
+        const Morphometry& shaped = m.diffused ? fresh : current;
         if (m.profile == TG_PROFILE_MPD && !(m.complex_ && c.diameter >= m.transitionDiameter) &&
-            !(current.depthRatio > 0.0 && current.rimHeightRatio >= 0.0 &&
-              current.rimHeightRatio <= fresh.rimHeightRatio)) {
+            !(m.diffused && tau > 0.0) &&
+            !(shaped.depthRatio > 0.0 && shaped.rimHeightRatio >= 0.0 &&
+              shaped.rimHeightRatio <= fresh.rimHeightRatio)) {
             throw std::invalid_argument(
                 "Crater state needs a positive depth and 0 <= rim height <= fresh rim height");
         }
@@ -347,7 +400,7 @@ Batch PrepareBatch(const std::vector<tgh::Crater>& craters, const CraterModels& 
         r.fresh_rim = static_cast<float>(fresh.rimHeightRatio);
         r.current_depth = static_cast<float>(current.depthRatio);
         r.current_rim = static_cast<float>(current.rimHeightRatio);
-        r.diffusion = 0.0f;
+        r.diffusion = static_cast<float>(tau);
         r.resolution = static_cast<float>(resolution);
         r.ellipticity = 1.0f;
         r.first_harmonic = 2u;
@@ -376,6 +429,19 @@ Batch PrepareBatch(const std::vector<tgh::Crater>& craters, const CraterModels& 
                        b.data);
         if (m.complex_) {
             AppendPeaks(c, fresh, current, o.peaks, m.transitionDiameter, r, b.data);
+        }
+        if (m.diffused && tau > 0.0) {
+            // ProfileTables::Append (crater_kernel.cpp:224-254): the crater's table, appended
+            // the first time a crater uses it.
+            const Morphometry2 f{fresh.depthRatio, fresh.rimHeightRatio};
+            const auto key = std::make_pair(f.depthRatio, f.rimHeightRatio);
+            const auto [at, added] =
+                tableOffsets.try_emplace(key, static_cast<std::uint32_t>(b.data.size()));
+            if (added) {
+                const std::vector<float>& heights = tables.TableFor(f).heights;
+                b.data.insert(b.data.end(), heights.begin(), heights.end());
+            }
+            r.profile_offset = at->second;
         }
         b.records.push_back(r);
         b.cx.push_back(c.x);

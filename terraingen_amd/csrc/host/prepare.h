@@ -59,6 +59,9 @@ struct CraterModels {
     double transitionDiameter = 15000.0;
     double cutoffDepthRatio = 0.04;  // kVisibilityDepthRatio (degradation/state.h:13)
     Ages ages;
+    // DefaultCraterModels: the MPD profile diffused in time (DiffusedProfile) under matched
+    // diffusion over the lunar steady-state ages, in place of freshness degradation.
+    bool diffused = false;
 };
 
 struct Batch {
@@ -70,6 +73,14 @@ struct Batch {
 
 inline CraterModels ModelsFrom(const tgx_crater_models& m) {
     CraterModels out;
+    if (m.model > 1u) {
+        throw std::invalid_argument("unknown crater model");
+    }
+    if (m.model == 1u) {
+        out.profile = TG_PROFILE_MPD;
+        out.diffused = true;
+        return out;
+    }
     out.profile = m.profile;
     out.complex_ = m.complex_ != 0;
     out.transitionDiameter = m.transition_diameter;

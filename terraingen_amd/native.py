@@ -397,7 +397,7 @@ CRATER_DTYPE = np.dtype([("x", np.float64), ("y", np.float64), ("diameter", np.f
 Generation = _struct("Generation", [(n, C.c_double) for n in (
     "min_x", "min_y", "max_x", "max_y", "sfd_coefficient", "sfd_exponent", "min_diameter",
     "max_diameter", "margin_radii", "cell_size_factor", "min_resolved_diameter",
-    "lifetime_gyr")] + [("seed", u32)])
+    "lifetime_gyr")] + [("seed", u32), ("model", u32)])
 
 HOST_ABI["tgx_philox4x32"] = (None, [VP, VP, VP])
 HOST_ABI["tgx_generate_craters"] = (C.c_int, [C.POINTER(Generation), C.POINTER(VP),
@@ -419,7 +419,10 @@ def generate_craters(spec: Generation) -> np.ndarray:
 
 CraterModels = _struct("CraterModels", [("profile", u32), ("complex_", u32),
                                         ("transition_diameter", C.c_double),
-                                        ("lifetime_gyr", C.c_double)])
+                                        ("lifetime_gyr", C.c_double), ("model", u32),
+                                        ("pad", u32)])
+# tgx_crater_models.model / tgx_generation.model: 1 selects the reference's DefaultCraterModels.
+MODELS_DEFAULT = 1
 Apply = _struct("Apply", [("irregular", u32), ("peaks", u32), ("terraces", u32)] + [
     (n, C.c_double) for n in ("rim_sigma", "noise_amplitude", "noise_rim_amplitude",
                               "wall_streaks", "ejecta", "terrace_onset", "terrace_full")])

@@ -21,7 +21,8 @@ from . import native as N
 
 @dataclass
 class CraterLayerSpec:
-    """A crater layer with a power-law density N(>=D) = sfd_coefficient D^-sfd_exponent per m^2."""
+    """A crater layer with a power-law density N(>=D) = sfd_coefficient D^-sfd_exponent per m^2, or
+    the reference's default models."""
     sfd_coefficient: float = 0.05
     sfd_exponent: float = 2.0
     min_diameter: float = 1.0
@@ -41,10 +42,14 @@ class CraterLayerSpec:
     noise_rim_amplitude: float = 0.0
     wall_streaks: float = 0.0
     ejecta: float = 0.0
+    # 1: the reference's DefaultCraterModels (lunar equilibrium density, steady-state ages,
+    # diffused MPD profile, matched diffusion); the SFD, profile and lifetime fields are then
+    # unused, and the diameters are the layer's defaults unless set.
+    model: int = 0
 
     def models(self) -> N.CraterModels:
         return N.CraterModels(self.profile, self.complex_, self.transition_diameter,
-                              self.lifetime_gyr)
+                              self.lifetime_gyr, self.model)
 
     def apply(self) -> N.Apply:
         return N.Apply(self.irregular, self.peaks, self.terraces, self.rim_sigma,
@@ -123,7 +128,7 @@ class PlaneTerrain:
                            min_diameter=spec.min_diameter, max_diameter=spec.max_diameter,
                            margin_radii=N.margin_radii(spec.models(), spec.apply()),
                            cell_size_factor=16.0, min_resolved_diameter=2.0 * g.pixel,
-                           lifetime_gyr=spec.lifetime_gyr, seed=spec.seed)
+                           lifetime_gyr=spec.lifetime_gyr, seed=spec.seed, model=spec.model)
         self.crater_list = N.generate_craters(gen)
         t1 = time.perf_counter()
         kernel, records, data, foot = N.prepare_craters(self.crater_list, spec.models(),

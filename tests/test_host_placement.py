@@ -13,6 +13,11 @@ from crater_cases import GOLDEN
 
 # (fixture, models, apply, layer seed, spacing, sfd (c, e), diameters, lifetime, grid)
 CASES = {
+    # The quick-start terrain: DefaultCraterModels (lunar equilibrium surface, steady-state ages,
+    # diffused Martin-Parkes-Dunstan profile, matched diffusion) with the default diameters
+    # (2.5-250 m) and apply options (irregular outlines; peaks and terraces on but inactive).
+    "c1_default": (N.CraterModels(1, 0, 15000.0, 0.0, N.MODELS_DEFAULT), N.Apply(1, 1, 1), 0,
+                   1.0, None, (2.5, 250.0), 0.0),
     "c2_powerlaw": (N.CraterModels(1, 0, 15000.0, 1.0), N.Apply(1, 0, 0), 1, 0.5, (0.05, 2.0),
                     (1.0, 200.0), 1.0),
     "features_ft": (N.CraterModels(0, 0, 15000.0, 0.2), N.Apply(1, 0, 0, 0.08, 0.01, 0.005,
@@ -40,13 +45,15 @@ def grid_bounds(z):
 
 @pytest.mark.parametrize("name", list(CASES))
 def test_generated_list_is_the_references(name):
-    models, apply, seed, spacing, (c, e), (dmin, dmax), life = CASES[name]
+    models, apply, seed, spacing, sfd, (dmin, dmax), life = CASES[name]
     z = fixture(name)
     x0, y0, x1, y1 = grid_bounds(z)
-    g = N.Generation(min_x=x0, min_y=y0, max_x=x1, max_y=y1, sfd_coefficient=c, sfd_exponent=e,
-                     min_diameter=dmin, max_diameter=dmax,
-                     margin_radii=N.margin_radii(models, apply), cell_size_factor=16.0,
-                     min_resolved_diameter=2.0 * spacing, lifetime_gyr=life, seed=seed)
+    g = N.Generation(min_x=x0, min_y=y0, max_x=x1, max_y=y1, min_diameter=dmin,
+                     max_diameter=dmax, margin_radii=N.margin_radii(models, apply),
+                     cell_size_factor=16.0, min_resolved_diameter=2.0 * spacing,
+                     lifetime_gyr=life, seed=seed, model=models.model)
+    if sfd is not None:
+        g.sfd_coefficient, g.sfd_exponent = sfd
     got = N.generate_craters(g)
     assert got.size == z["x"].size
     for col, key in [("x", "x"), ("y", "y"), ("diameter", "d"), ("age_gyr", "age"), ("id", "id")]:

@@ -1,6 +1,7 @@
 # SPDX-License-Identifier: BSD-3-Clause
-"""A whole terrain on the hip backend against the reference's CPU evaluation: synthetic fBm
-source plus power-law craters, placed, prepared and binned by this repository's host code."""
+"""A whole terrain on the hip backend against the reference's CPU evaluation, its craters placed,
+prepared and binned by this repository's host code: the quick-start terrain (flat source and the
+default crater models, with diffused profiles), and synthetic fBm plus power-law craters."""
 import numpy as np
 import pytest
 
@@ -28,4 +29,28 @@ def test_c2_recipe_matches_reference_terrain(hip):
     d = np.abs(got - z["heights"])
     print(f"[c2_synthetic] GPU terrain vs reference CPU terrain: max {d.max():.3g} m, "
           f"rms {np.sqrt((d ** 2).mean()):.3g} m over {got.size} cells", flush=True)
+    assert d.max() <= 1e-3
+
+
+def test_c1_quick_start_matches_reference_terrain(hip):
+    """FlatSource 257 x 257 at 1 m, height 0, one CraterLayer with the reference's default models
+    and apply options, seed 0: the same crater list, and the reference's CPU heights bit for bit
+    (the diffusion tables, states and records are byte-equal to the reference's)."""
+    z = np.load(GOLDEN / "craters_c1_default.npz")
+    grid = N.north_up_grid(257, 257, 1.0, (0.0, 0.0))
+    spec = CraterLayerSpec(model=N.MODELS_DEFAULT, seed=0, min_diameter=2.5, max_diameter=250.0,
+                           irregular=1, peaks=1, terraces=1)
+    terrain = PlaneTerrain(hip, grid, [], 0.0, spec)
+    terrain.prepare()
+    try:
+        for col, key in [("x", "x"), ("y", "y"), ("diameter", "d"), ("age_gyr", "age"),
+                         ("id", "id")]:
+            assert np.array_equal(terrain.crater_list[col], z[key]), col
+        assert terrain.records.view(np.uint8).tobytes() == z["records"].tobytes()
+        got = terrain.evaluate()
+    finally:
+        terrain.close()
+    d = np.abs(got - z["heights"])
+    print(f"[c1_default] GPU terrain vs reference CPU terrain: max {d.max():.3g} m over "
+          f"{got.size} cells, {terrain.crater_count} craters", flush=True)
     assert d.max() <= 1e-3
