@@ -112,3 +112,47 @@ def test_philox_known_answers(counter, key, want):
     out = (C.c_uint32 * 4)()
     lib.tgx_philox4x32(c, k, out)
     assert tuple(out) == want
+
+
+@pytest.mark.parametrize("seed,pixel,center", [(3, 2.0, (500.0, -300.0)), (17, 0.5, (-80.0, 40.0))])
+def test_default_models_match_the_reference_elsewhere(ref, seed, pixel, center):
+    """The default crater models on other seeds, spacings and regions, against the reference's
+    own pipeline run here (oracle/_ref): the same list, and the same kernel struct, records and
+    data, diffusion tables included. A 2 m spacing fades the smallest craters; 0.5 m keeps
+    them all."""
+    from oracle import RefCase
+
+    case = RefCase(width=129, height=129, pixel=pixel, center_x=center[0], center_y=center[1],
+                   craters=1, models=0, irregular=1, peaks=1, terraces=1, seed=seed, threads=4)
+    h = ref.ref_case_open(C.byref(case))
+    assert h, ref.ref_last_error()
+    try:
+        n = ref.ref_case_craters(h, None, None, None, None, None)
+        cols = [np.zeros(n) for _ in range(4)] + [np.zeros(n, np.uint64)]
+        ref.ref_case_craters(h, *[a.ctypes.data for a in cols])
+        kernel = N.CraterShape()
+        data_count = C.c_long()
+        nrec = ref.ref_case_batch(h, C.byref(kernel), C.byref(data_count))
+        assert nrec >= 0, ref.ref_last_error()
+        records = np.zeros(nrec, N.RECORD_DTYPE)
+        data = np.zeros(data_count.value, np.float32)
+        ref.ref_case_batch_copy(h, records.ctypes.data, data.ctypes.data)
+    finally:
+        ref.ref_case_close(h)
+    models = N.CraterModels(1, 0, 15000.0, 0.0, N.MODELS_DEFAULT)
+    apply = N.Apply(1, 1, 1)
+    grid = N.north_up_grid(129, 129, pixel, center)
+    g = N.Generation(min_x=grid.origin_x, min_y=grid.origin_y - 129 * pixel,
+                     max_x=grid.origin_x + 129 * pixel, max_y=grid.origin_y, min_diameter=2.5,
+                     max_diameter=250.0, margin_radii=N.margin_radii(models, apply),
+                     cell_size_factor=16.0, min_resolved_diameter=2.0 * pixel, seed=seed,
+                     model=N.MODELS_DEFAULT)
+    got = N.generate_craters(g)
+    assert got.size == n and n > 0
+    for col, want in zip(("x", "y", "diameter", "age_gyr", "id"), cols):
+        assert np.array_equal(got[col], want), col
+    k, recs, dat, _ = N.prepare_craters(got, models, apply, seed, pixel)
+    assert bytes(k) == bytes(kernel)
+    assert recs.view(np.uint8).tobytes() == records.view(np.uint8).tobytes()
+    assert dat.view(np.uint32).tobytes() == data.view(np.uint32).tobytes()
+    assert (recs["diffusion"] > 0).any() and (recs["profile_offset"] > 0).any()
