@@ -25,11 +25,12 @@ the square.
   of its inputs is excluded.
 * ``serving``: the served stream, run through the tile server (continuous batching, several
   batches in flight, a 64 GB HBM admission pool). Tile requests arrive as a Poisson process at
-  2000/s for 10 s from a camera flying a 50 km straight ground track at 200 m altitude. Each
-  request is the quadtree tile, at levels 0-10 by slant range, that holds a ground point near the
-  camera, with edges of 129, 257, 513 or 1025 cells. It runs on a plane: the 2D noise of the
-  ``tiles`` workload, and a crater catalogue (D >= 2 m, same SFD, seed 5) over a 2 km swath
-  along the track. It reports the
+  2000/s for 10 s from a camera flying 50 km at 200 m altitude. Each request is the quadtree
+  tile, at levels 0-10 by slant range, that holds a ground point near the camera, with edges of
+  129, 257, 513 or 1025 cells. With the default workload the track is a great circle on the
+  Moon and the tiles are longitude-latitude ones (3D body noise, frame-placed craters); with
+  ``tiles`` it is a straight track on a plane. The crater catalogue (D >= 2 m, same SFD, seed 5)
+  covers a 2 km swath along the track. It reports the
   p50/p99 request-to-host latency (submission to heights in the caller's memory) for 513 x 513
   tiles and for all tiles, and the served Gcells/s.
 
@@ -559,16 +560,22 @@ def bench_sphere(args, N, dev) -> dict:
 
 
 def bench_serving(args, N, dev) -> dict:
-    from terraingen_amd.pipeline import CraterLayerSpec, TileBatch, TileServer
-    from terraingen_amd.serving import CameraStream, serve
+    from terraingen_amd.pipeline import CraterLayerSpec, SphereTileBatch, TileBatch, TileServer
+    from terraingen_amd.serving import CameraStream, GreatCircleStream, serve
 
-    stream = CameraStream(duration=args.serve_seconds)
+    sphere = args.workload == "sphere"
+    stream = (GreatCircleStream(duration=args.serve_seconds, radius=MOON) if sphere
+              else CameraStream(duration=args.serve_seconds))
     requests = stream.requests()
     t0 = time.perf_counter()
     spec = CraterLayerSpec(seed=5, min_diameter=2.0)
-    swath = TileBatch(dev, [C2_FRACTAL], spec,
-                      region=(-2000.0, -1000.0, stream.path_length + 2000.0, 1000.0),
-                      finest_pixel=1.0)
+    if sphere:
+        swath = SphereTileBatch(dev, [C2_FRACTAL], spec, region=None, finest_pixel=1.0,
+                                radius=MOON, craters_given=stream.swath_craters(1000.0))
+    else:
+        swath = TileBatch(dev, [C2_FRACTAL], spec,
+                          region=(-2000.0, -1000.0, stream.path_length + 2000.0, 1000.0),
+                          finest_pixel=1.0)
     swath.build()
     server = TileServer(swath, slots=4, max_batch_tiles=32, max_batch_cells=1 << 24,
                         pool_bytes=64 << 30, bin_threads=8)
@@ -588,6 +595,10 @@ def bench_serving(args, N, dev) -> dict:
     is513 = requests["width"] == 513
     cells = float((requests["width"].astype(np.float64) * requests["height"]).sum())
     return {
+        "stream": ("a 50 km great-circle track on the 1737.4 km sphere at 200 m, lon/lat "
+                   "quadtree tiles, 3D body noise, a 2 km swath of craters D >= 2 m" if sphere
+                   else "a 50 km straight track on a plane at 200 m, a 2 km swath of craters "
+                        "D >= 2 m"),
         "requests": int(requests.size), "rate_per_s": stream.rate,
         "duration_s": stream.duration,
         "latency_513_ms": {"p50": round(percentile(lat_ms[is513], 50), 3),

@@ -225,6 +225,12 @@ int tgx_sphere_bin(const double* lon, const double* lat, const double* reach,
                    size_t tile_count, unsigned threads, tg_tile_desc** descs, tg_bin** bins,
                    size_t* bin_count, uint32_t** entries, size_t* entry_count, char* err,
                    size_t err_len);
+/* A kept index of a catalogue on a sphere (the columns of tgx_sphere_bin), for tgx_index_bin and
+ * the tile server: tiles binned against it are longitude-latitude tiles, and a server given it
+ * runs its batches on that sphere (tg_hip_batch_surface). Freed with tgx_index_free. */
+int tgx_sphere_index_create(const double* lon, const double* lat, const double* reach,
+                            const double* diameter, size_t n, double radius, tgx_index** out,
+                            char* err, size_t err_len);
 /* tgx_batch_pair_ops for tiles on a sphere (the output of tgx_sphere_bin): the (cell, listed
  * crater) pairs whose cell centre lies strictly within the crater's reach along the sphere. A
  * measurement aid with no counterpart in the reference; it never touches a device. */

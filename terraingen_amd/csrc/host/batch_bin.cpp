@@ -21,23 +21,6 @@
 
 namespace tgh {
 
-struct IndexLevel {
-    double cell = 0.0;       // index cell side (m)
-    double maxReach = 0.0;   // largest footprint half-width in the level (m)
-    double maxDiameter = 0.0;
-    std::unordered_map<std::uint64_t, std::vector<std::uint32_t>> cells;
-};
-
-}  // namespace tgh
-
-// The catalogue's spatial index: craters grouped by diameter level and, within a level, by cell.
-struct tgx_index {
-    std::vector<double> cx, cy, reach, diameter;
-    std::vector<tgh::IndexLevel> levels;
-};
-
-namespace tgh {
-
 namespace {
 
 std::uint64_t CellKey(std::int64_t i, std::int64_t j) {
@@ -58,10 +41,6 @@ bool Resolved(double diameter, double pixel) {
     return 1.0 - (1.0 - t * t * (3.0 - 2.0 * t)) != 0.0;
 }
 
-struct TileBins {
-    std::vector<std::uint32_t> counts;  // per bin
-    std::vector<std::uint32_t> entries;
-};
 
 }  // namespace
 
@@ -106,12 +85,16 @@ IndexPtr MakeIndex(const double* cx, const double* cy, const double* reach,
 namespace {
 
 // One tile's bins against the index: per-bin counts and the entries in bin order.
-void BinTile(const tgx_index& index, const tgx_tile& tile, std::vector<std::uint32_t>& candidates,
-             TileBins& out) {
+// The bins of bin rows [row0, row1) of `tile`: each crater's pixel window is the whole tile's, so
+// a strip's bins are those of the whole tile; only its candidates are gathered from the strip.
+void BinTile(const tgx_index& index, const tgx_tile& tile, std::size_t row0, std::size_t row1,
+             std::vector<std::uint32_t>& candidates, TileBins& out) {
     constexpr std::uint32_t kSide = 16;
     const Grid g{tile.width, tile.height, tile.origin_x, tile.origin_y, tile.pixel};
     const double minX = tile.origin_x, maxX = tile.origin_x + tile.width * tile.pixel;
-    const double maxY = tile.origin_y, minY = tile.origin_y - tile.height * tile.pixel;
+    const double maxY = tile.origin_y - double(row0 * kSide) * tile.pixel;
+    const double minY =
+        tile.origin_y - double(std::min<std::size_t>(row1 * kSide, tile.height)) * tile.pixel;
     candidates.clear();
     for (const IndexLevel& level : index.levels) {
         if (level.cell == 0.0 || !Resolved(level.maxDiameter, tile.pixel)) {
@@ -138,9 +121,9 @@ void BinTile(const tgx_index& index, const tgx_tile& tile, std::vector<std::uint
         }
     }
     std::sort(candidates.begin(), candidates.end());
+    candidates.erase(std::unique(candidates.begin(), candidates.end()), candidates.end());
     const std::size_t binsX = (tile.width + kSide - 1) / kSide;
-    const std::size_t binsY = (tile.height + kSide - 1) / kSide;
-    std::vector<std::vector<std::uint32_t>> lists(binsX * binsY);
+    std::vector<std::vector<std::uint32_t>> lists(binsX * (row1 - row0));
     for (const std::uint32_t k : candidates) {
         if (!Resolved(index.diameter[k], tile.pixel)) {
             continue;
@@ -151,9 +134,11 @@ void BinTile(const tgx_index& index, const tgx_tile& tile, std::vector<std::uint
         if (w.Empty()) {
             continue;
         }
-        for (std::size_t by = w.rowBegin / kSide; by <= (w.rowEnd - 1) / kSide; ++by) {
+        const std::size_t by0 = std::max(w.rowBegin / kSide, row0);
+        const std::size_t by1 = std::min((w.rowEnd - 1) / kSide + 1, row1);
+        for (std::size_t by = by0; by < by1; ++by) {
             for (std::size_t bx = w.colBegin / kSide; bx <= (w.colEnd - 1) / kSide; ++bx) {
-                lists[by * binsX + bx].push_back(k);
+                lists[(by - row0) * binsX + bx].push_back(k);
             }
         }
     }
@@ -178,27 +163,33 @@ void CheckTile(const tgx_tile& tile, std::size_t t) {
 void BinBatch(const tgx_index& index, const tgx_tile* tiles, std::size_t tileCount, unsigned threads,
               std::vector<tg_tile_desc>& descs, std::vector<tg_bin>& bins,
               std::vector<std::uint32_t>& entries) {
+    if (index.sphere) {
+        BinSphereBatch(*index.sphere, tiles, tileCount, threads, descs, bins, entries);
+        return;
+    }
     constexpr std::uint32_t kSide = 16;
     for (std::size_t t = 0; t < tileCount; ++t) {
         CheckTile(tiles[t], t);
     }
-    // Each tile's bins, in parallel.
-    std::vector<TileBins> per(tileCount);
+    // Each tile's bins in strips of bin rows, in parallel, so one large tile spreads over threads.
+    const std::vector<Strip> strips = MakeStrips(tiles, tileCount);
+    std::vector<TileBins> per(strips.size());
     std::atomic<std::size_t> next{0};
     auto work = [&] {
         std::vector<std::uint32_t> candidates;
         for (;;) {
-            const std::size_t t = next.fetch_add(1);
-            if (t >= tileCount) {
+            const std::size_t n = next.fetch_add(1);
+            if (n >= strips.size()) {
                 return;
             }
-            BinTile(index, tiles[t], candidates, per[t]);
+            const Strip& s = strips[n];
+            BinTile(index, tiles[s.tile], s.row0, s.row1, candidates, per[n]);
         }
     };
     if (threads == 0) {
         threads = std::max(1u, std::thread::hardware_concurrency());
     }
-    threads = unsigned(std::min<std::size_t>(threads, std::max<std::size_t>(tileCount, 1)));
+    threads = unsigned(std::min<std::size_t>(threads, std::max<std::size_t>(strips.size(), 1)));
     std::vector<std::thread> pool;
     for (unsigned i = 1; i < threads; ++i) {
         pool.emplace_back(work);
@@ -212,6 +203,7 @@ void BinBatch(const tgx_index& index, const tgx_tile* tiles, std::size_t tileCou
     bins.clear();
     entries.clear();
     std::uint64_t out = 0;
+    std::uint64_t binCursor = 0;
     for (std::size_t t = 0; t < tileCount; ++t) {
         const tgx_tile& tile = tiles[t];
         tg_tile_desc d{};
@@ -221,16 +213,39 @@ void BinBatch(const tgx_index& index, const tgx_tile* tiles, std::size_t tileCou
         d.width = tile.width;
         d.height = tile.height;
         d.out_offset = out;
-        d.bin_first = std::uint32_t(bins.size());
+        d.bin_first = std::uint32_t(binCursor);
         d.bins_across = (tile.width + kSide - 1) / kSide;
         descs.push_back(d);
         out += std::uint64_t(tile.width) * tile.height;
-        std::uint32_t first = std::uint32_t(entries.size());
-        for (const std::uint32_t count : per[t].counts) {
-            bins.push_back({std::uint32_t(t), first, count, 0u});
+        binCursor += std::uint64_t(d.bins_across) * ((tile.height + kSide - 1) / kSide);
+        if (binCursor > 0xFFFFFFFFull) {
+            throw std::invalid_argument("batch too large for 32-bit bin and entry indices");
+        }
+    }
+    AppendStrips(strips, per, bins, entries);
+}
+
+std::vector<Strip> MakeStrips(const tgx_tile* tiles, std::size_t tileCount) {
+    constexpr std::size_t kSide = 16, kRows = 8;
+    std::vector<Strip> strips;
+    for (std::size_t t = 0; t < tileCount; ++t) {
+        const std::size_t rows = (tiles[t].height + kSide - 1) / kSide;
+        for (std::size_t r = 0; r < rows; r += kRows) {
+            strips.push_back({t, r, std::min(rows, r + kRows)});
+        }
+    }
+    return strips;
+}
+
+void AppendStrips(const std::vector<Strip>& strips, const std::vector<TileBins>& per,
+                  std::vector<tg_bin>& bins, std::vector<std::uint32_t>& entries) {
+    for (std::size_t n = 0; n < strips.size(); ++n) {
+        std::uint64_t first = entries.size();
+        for (const std::uint32_t count : per[n].counts) {
+            bins.push_back({std::uint32_t(strips[n].tile), std::uint32_t(first), count, 0u});
             first += count;
         }
-        entries.insert(entries.end(), per[t].entries.begin(), per[t].entries.end());
+        entries.insert(entries.end(), per[n].entries.begin(), per[n].entries.end());
         if (entries.size() > 0xFFFFFFFFull || bins.size() > 0xFFFFFFFFull) {
             throw std::invalid_argument("batch too large for 32-bit bin and entry indices");
         }

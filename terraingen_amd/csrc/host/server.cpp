@@ -226,6 +226,14 @@ void EnsureCapacity(tgx_server& s, Slot& slot, std::size_t tiles, std::size_t bi
         grow && entries > slot.maxEntries ? entries * 2 + 1024 : entries, slot.maxEntries);
     tg_batch* batch = nullptr;
     Raise(h.batch_create(s.dev, t, b, e, &batch, err, sizeof err), err);
+    if (const double radius = IndexRadius(*s.index); radius > 0.0) {
+        // Tiles binned against a sphere's index are longitude-latitude tiles on it.
+        const int code = h.batch_surface(batch, radius, err, sizeof err);
+        if (code != TG_OK) {
+            h.batch_free(batch);
+            Raise(code, err);
+        }
+    }
     if (slot.batch != nullptr) {
         s.retiredBatches.push_back(slot.batch);
         s.retiredPinned.insert(s.retiredPinned.end(),

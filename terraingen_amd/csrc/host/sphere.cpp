@@ -17,10 +17,45 @@
 
 #include <atomic>
 #include <cmath>
+#include <map>
 #include <thread>
 #include <unordered_map>
 
 namespace tgh {
+
+// The craters of one diameter level (a power of two), grouped by longitude-latitude cell of
+// `cellDeg` degrees, twice the level's largest footprint: a tile visits the cells its box grown
+// by that footprint meets.
+struct SphereLevel {
+    double cellDeg = 0.0;
+    double maxReachDeg = 0.0;
+    double maxDiameter = 0.0;
+    std::int64_t lonCells = 0, latCells = 0;
+    std::unordered_map<std::int64_t, std::vector<std::uint32_t>> cells;
+
+    std::int64_t LonCell(double lon) const {
+        double l = std::fmod(lon + 180.0, 360.0);
+        if (l < 0.0) {
+            l += 360.0;
+        }
+        return std::min(lonCells - 1, std::int64_t(std::floor(l / cellDeg)));
+    }
+    std::int64_t LatCell(double lat) const {
+        return std::clamp<std::int64_t>(std::int64_t(std::floor((lat + 90.0) / cellDeg)), 0,
+                                        latCells - 1);
+    }
+    std::int64_t Key(std::int64_t i, std::int64_t j) const {
+        return i * latCells + j;
+    }
+};
+
+// Crater centres in degrees with footprints in degrees of arc, indexed by diameter level.
+struct SphereCatalogue {
+    std::vector<double> lon, lat, reachDeg, diameter;
+    std::vector<double> lonHalfWidth;  // LonHalfWidth(lat, reachDeg) per crater
+    double radius = 0.0;
+    std::vector<SphereLevel> levels;
+};
 
 namespace {
 
@@ -55,32 +90,6 @@ double LonDelta(double a, double b) {
     return d - 180.0;
 }
 
-struct SphereIndex {
-    const double* lon;
-    const double* lat;
-    const double* reachDeg;  // footprint half-width in degrees of arc
-    const double* diameter;
-    std::size_t n;
-    double maxReachDeg = 0.0;
-    std::unordered_map<std::int64_t, std::vector<std::uint32_t>> cells;  // 1-degree cells
-
-    static std::int64_t Key(std::int64_t i, std::int64_t j) {
-        return i * 1000 + j;
-    }
-};
-
-std::int64_t LonCell(double lon) {
-    double l = std::fmod(lon + 180.0, 360.0);
-    if (l < 0.0) {
-        l += 360.0;
-    }
-    return std::min<std::int64_t>(359, std::int64_t(std::floor(l)));
-}
-
-std::int64_t LatCell(double lat) {
-    return std::clamp<std::int64_t>(std::int64_t(std::floor(lat + 90.0)), 0, 179);
-}
-
 // The half-width in degrees of longitude of every point within `arc` degrees of arc of a point at
 // latitude `lat` or nearer the equator than |lat| + arc: |sin dLon| <= sin arc / cos lat' for a
 // point at latitude lat', so asin(sin arc / cos(|lat| + arc)) bounds it; 360 when the cap holds a
@@ -98,35 +107,62 @@ double LonHalfWidth(double lat, double arc) {
 }
 
 // One tile's bins: per-bin counts and the entries in bin order.
-void BinSphereTile(const SphereIndex& index, const tgx_tile& tile, double unitLength,
-                   std::vector<std::uint32_t>& candidates, std::vector<std::uint32_t>& counts,
-                   std::vector<std::uint32_t>& entries) {
-    const double west = tile.origin_x, north = tile.origin_y, step = tile.pixel;
+// Bin rows [row0, row1) of the tile (strips bin exactly as the whole tile: every bound below is
+// the tile's; only the candidates are gathered from the strip's latitudes).
+void BinSphereTile(const SphereCatalogue& index, const tgx_tile& tile, std::size_t row0,
+                   std::size_t row1, double unitLength, std::vector<std::uint32_t>& candidates,
+                   std::vector<std::uint32_t>& counts, std::vector<std::uint32_t>& entries) {
+    const double west = tile.origin_x, step = tile.pixel;
+    const double north = tile.origin_y;
     const double span = tile.width * step;
-    const double south = north - tile.height * step;
+    const double stripNorth = north - double(row0 * kSide) * step;
+    const double stripSouth =
+        north - double(std::min<std::size_t>(row1 * kSide, tile.height)) * step;
     const double spacing = step * unitLength;
-    // Candidates: the 1-degree cells of the tile's box grown by the largest reach.
-    const double grow = index.maxReachDeg;
-    const double latLo = std::max(-90.0, south - grow), latHi = std::min(90.0, north + grow);
-    const double lonGrow = LonHalfWidth(std::max(std::fabs(south), std::fabs(north)), grow);
+    // Candidates: per diameter level whose largest crater the tile can resolve, the cells of the
+    // tile's box grown by the level's largest footprint.
     candidates.clear();
-    const bool allLon = span + 2.0 * lonGrow >= 359.0;
-    const std::int64_t lonCells = allLon ? 360 : std::int64_t(std::ceil(span + 2.0 * lonGrow)) + 2;
-    const std::int64_t lon0 = allLon ? 0 : LonCell(west - lonGrow);
-    for (std::int64_t j = LatCell(latLo); j <= LatCell(latHi); ++j) {
-        for (std::int64_t s = 0; s < lonCells; ++s) {
-            const std::int64_t i = (lon0 + s) % 360;
-            const auto it = index.cells.find(SphereIndex::Key(i, j));
-            if (it != index.cells.end()) {
-                candidates.insert(candidates.end(), it->second.begin(), it->second.end());
+    for (const SphereLevel& level : index.levels) {
+        if (!Resolved(level.maxDiameter, spacing)) {
+            continue;
+        }
+        const double grow = level.maxReachDeg;
+        const double latLo = std::max(-90.0, stripSouth - grow);
+        const double latHi = std::min(90.0, stripNorth + grow);
+        const double lonGrow =
+            LonHalfWidth(std::max(std::fabs(stripSouth), std::fabs(stripNorth)), grow);
+        const std::int64_t j0 = level.LatCell(latLo), j1 = level.LatCell(latHi);
+        const double lonSpan = span + 2.0 * lonGrow;
+        const std::int64_t lonSteps =
+            lonSpan >= 359.0 ? level.lonCells
+                             : std::min(level.lonCells,
+                                        std::int64_t(std::ceil(lonSpan / level.cellDeg)) + 2);
+        if (lonSteps * (j1 - j0 + 1) > std::int64_t(level.cells.size())) {
+            // Fewer occupied cells than cells to visit: filter the occupied ones by row.
+            for (const auto& [key, list] : level.cells) {
+                const std::int64_t j = key % level.latCells;
+                if (j < j0 || j > j1) {
+                    continue;
+                }
+                candidates.insert(candidates.end(), list.begin(), list.end());
+            }
+            continue;
+        }
+        const std::int64_t i0 = lonSpan >= 359.0 ? 0 : level.LonCell(west - lonGrow);
+        for (std::int64_t j = j0; j <= j1; ++j) {
+            for (std::int64_t k = 0; k < lonSteps; ++k) {
+                const std::int64_t i = (i0 + k) % level.lonCells;
+                const auto it = level.cells.find(level.Key(i, j));
+                if (it != level.cells.end()) {
+                    candidates.insert(candidates.end(), it->second.begin(), it->second.end());
+                }
             }
         }
     }
     std::sort(candidates.begin(), candidates.end());
     candidates.erase(std::unique(candidates.begin(), candidates.end()), candidates.end());
     const std::size_t binsX = (tile.width + kSide - 1) / kSide;
-    const std::size_t binsY = (tile.height + kSide - 1) / kSide;
-    std::vector<std::vector<std::uint32_t>> lists(binsX * binsY);
+    std::vector<std::vector<std::uint32_t>> lists(binsX * (row1 - row0));
     for (const std::uint32_t k : candidates) {
         if (!Resolved(index.diameter[k], spacing)) {
             continue;
@@ -140,7 +176,7 @@ void BinSphereTile(const SphereIndex& index, const tgx_tile& tile, double unitLe
         if (rowB < -1.0 || rowA > double(tile.height)) {
             continue;
         }
-        const double dLon = LonHalfWidth(lat, r);
+        const double dLon = index.lonHalfWidth[k];
         double colA = -1.0, colB = double(tile.width);
         if (span + 2.0 * dLon < 359.0) {
             // The crater's longitude east of the tile's middle, folded into [-180, 180).
@@ -160,9 +196,11 @@ void BinSphereTile(const SphereIndex& index, const tgx_tile& tile, double unitLe
         };
         const std::size_t c0 = first(colA, tile.width), c1 = last(colB, tile.width);
         const std::size_t r0 = first(rowA, tile.height), r1 = last(rowB, tile.height);
-        for (std::size_t by = r0 / kSide; by <= r1 / kSide; ++by) {
+        const std::size_t by0 = std::max(r0 / kSide, row0);
+        const std::size_t by1 = std::min(r1 / kSide + 1, row1);
+        for (std::size_t by = by0; by < by1; ++by) {
             for (std::size_t bx = c0 / kSide; bx <= c1 / kSide; ++bx) {
-                lists[by * binsX + bx].push_back(k);
+                lists[(by - row0) * binsX + bx].push_back(k);
             }
         }
     }
@@ -223,6 +261,124 @@ void SpherePairs(const tg_tile_desc* descs, std::size_t first, std::size_t last,
 }
 
 }  // namespace
+
+std::shared_ptr<const SphereCatalogue> MakeSphereCatalogue(const double* lon, const double* lat,
+                                                           const double* reach,
+                                                           const double* diameter,
+                                                           std::size_t n, double radius) {
+    if ((n > 0 && (lon == nullptr || lat == nullptr || reach == nullptr || diameter == nullptr)) ||
+        !(radius > 0.0 && std::isfinite(radius))) {
+        throw std::invalid_argument("a sphere catalogue needs its columns and a positive radius");
+    }
+    auto c = std::make_shared<SphereCatalogue>();
+    c->radius = radius;
+    c->lon.assign(lon, lon + n);
+    c->lat.assign(lat, lat + n);
+    c->diameter.assign(diameter, diameter + n);
+    c->reachDeg.resize(n);
+    c->lonHalfWidth.resize(n);
+    std::map<int, std::vector<std::uint32_t>> byLevel;
+    for (std::size_t k = 0; k < n; ++k) {
+        if (!std::isfinite(lon[k]) || !(std::fabs(lat[k]) <= 90.0) || !(reach[k] >= 0.0) ||
+            !(diameter[k] > 0.0 && std::isfinite(diameter[k]))) {
+            throw std::invalid_argument("crater " + std::to_string(k) + " is malformed");
+        }
+        // Footprints in degrees of arc: a crater's offsets are arc lengths (KernelFrameOffset).
+        c->reachDeg[k] = reach[k] / radius / kDegree;
+        c->lonHalfWidth[k] = LonHalfWidth(lat[k], c->reachDeg[k]);
+        byLevel[std::ilogb(diameter[k])].push_back(std::uint32_t(k));
+    }
+    for (const auto& [exponent, members] : byLevel) {
+        (void)exponent;
+        SphereLevel level;
+        for (const std::uint32_t k : members) {
+            level.maxReachDeg = std::max(level.maxReachDeg, c->reachDeg[k]);
+            level.maxDiameter = std::max(level.maxDiameter, c->diameter[k]);
+        }
+        // Cells at least twice the largest footprint, doubled until they hold a few craters
+        // each on average, so that a tile visits few cells and reads few craters per cell.
+        level.cellDeg = std::clamp(2.0 * level.maxReachDeg, 1e-7, 1.0);
+        for (;;) {
+            level.lonCells = std::int64_t(std::ceil(360.0 / level.cellDeg));
+            level.latCells = std::int64_t(std::ceil(180.0 / level.cellDeg)) + 1;
+            level.cells.clear();
+            for (const std::uint32_t k : members) {
+                level.cells[level.Key(level.LonCell(c->lon[k]), level.LatCell(c->lat[k]))]
+                    .push_back(k);
+            }
+            if (level.cellDeg >= 1.0 || members.size() >= 8 * level.cells.size()) {
+                break;
+            }
+            level.cellDeg = std::min(1.0, 2.0 * level.cellDeg);
+        }
+        c->levels.push_back(std::move(level));
+    }
+    return c;
+}
+
+void BinSphereBatch(const SphereCatalogue& catalogue, const tgx_tile* tiles,
+                    std::size_t tileCount, unsigned threads, std::vector<tg_tile_desc>& d,
+                    std::vector<tg_bin>& b, std::vector<std::uint32_t>& e) {
+    for (std::size_t t = 0; t < tileCount; ++t) {
+        CheckTile(tiles[t], t);
+    }
+    const double unitLength = catalogue.radius * kDegree;
+    const std::vector<Strip> strips = MakeStrips(tiles, tileCount);
+    std::vector<TileBins> per(strips.size());
+    std::atomic<std::size_t> next{0};
+    auto work = [&] {
+        std::vector<std::uint32_t> candidates;
+        for (;;) {
+            const std::size_t n = next.fetch_add(1);
+            if (n >= strips.size()) {
+                return;
+            }
+            const Strip& s = strips[n];
+            BinSphereTile(catalogue, tiles[s.tile], s.row0, s.row1, unitLength, candidates,
+                          per[n].counts, per[n].entries);
+        }
+    };
+    if (threads == 0) {
+        threads = std::max(1u, std::thread::hardware_concurrency());
+    }
+    threads = unsigned(std::min<std::size_t>(threads, std::max<std::size_t>(strips.size(), 1)));
+    std::vector<std::thread> pool;
+    for (unsigned i = 1; i < threads; ++i) {
+        pool.emplace_back(work);
+    }
+    work();
+    for (std::thread& th : pool) {
+        th.join();
+    }
+    d.clear();
+    b.clear();
+    e.clear();
+    std::uint64_t out = 0;
+    std::uint64_t binCursor = 0;
+    for (std::size_t t = 0; t < tileCount; ++t) {
+        const tgx_tile& tile = tiles[t];
+        tg_tile_desc desc{};
+        desc.origin_x = tile.origin_x;
+        desc.origin_y = tile.origin_y;
+        desc.pixel = tile.pixel;
+        desc.width = tile.width;
+        desc.height = tile.height;
+        desc.out_offset = out;
+        desc.bin_first = std::uint32_t(binCursor);
+        desc.bins_across = (tile.width + kSide - 1) / kSide;
+        d.push_back(desc);
+        out += std::uint64_t(tile.width) * tile.height;
+        binCursor += std::uint64_t(desc.bins_across) * ((tile.height + kSide - 1) / kSide);
+        if (binCursor > 0xFFFFFFFFull) {
+            throw std::invalid_argument("batch too large for 32-bit bin and entry indices");
+        }
+    }
+    AppendStrips(strips, per, b, e);
+}
+
+double IndexRadius(const tgx_index& index) {
+    return index.sphere ? index.sphere->radius : 0.0;
+}
 
 }  // namespace tgh
 
@@ -298,84 +454,37 @@ int tgx_sphere_bin(const double* lon, const double* lat, const double* reach,
                    size_t errLen) {
     return tgh::Guard(err, errLen, [&] {
         if (descs == nullptr || bins == nullptr || entries == nullptr || binCount == nullptr ||
-            entryCount == nullptr || (tileCount > 0 && tiles == nullptr) ||
-            (n > 0 && (lon == nullptr || lat == nullptr || reach == nullptr ||
-                       diameter == nullptr)) ||
-            !(radius > 0.0)) {
-            throw std::invalid_argument("tgx_sphere_bin: null argument or radius not positive");
+            entryCount == nullptr || (tileCount > 0 && tiles == nullptr)) {
+            throw std::invalid_argument("tgx_sphere_bin: null argument");
         }
-        for (size_t t = 0; t < tileCount; ++t) {
-            tgh::CheckTile(tiles[t], t);
-        }
-        // Footprints in degrees of arc: a crater's offsets are arc lengths (KernelFrameOffset).
-        std::vector<double> reachDeg(n);
-        tgh::SphereIndex index{lon, lat, reachDeg.data(), diameter, n, 0.0, {}};
-        for (size_t k = 0; k < n; ++k) {
-            if (!std::isfinite(lon[k]) || !(std::fabs(lat[k]) <= 90.0) || !(reach[k] >= 0.0)) {
-                throw std::invalid_argument("crater " + std::to_string(k) + " is malformed");
-            }
-            reachDeg[k] = reach[k] / radius / tgh::kDegree;
-            index.maxReachDeg = std::max(index.maxReachDeg, reachDeg[k]);
-            index.cells[tgh::SphereIndex::Key(tgh::LonCell(lon[k]), tgh::LatCell(lat[k]))]
-                .push_back(uint32_t(k));
-        }
-        const double unitLength = radius * tgh::kDegree;
-        std::vector<std::vector<uint32_t>> counts(tileCount), lists(tileCount);
-        std::atomic<size_t> next{0};
-        auto work = [&] {
-            std::vector<uint32_t> candidates;
-            for (;;) {
-                const size_t t = next.fetch_add(1);
-                if (t >= tileCount) {
-                    return;
-                }
-                tgh::BinSphereTile(index, tiles[t], unitLength, candidates, counts[t], lists[t]);
-            }
-        };
-        if (threads == 0) {
-            threads = std::max(1u, std::thread::hardware_concurrency());
-        }
-        threads = unsigned(std::min<size_t>(threads, std::max<size_t>(tileCount, 1)));
-        std::vector<std::thread> pool;
-        for (unsigned i = 1; i < threads; ++i) {
-            pool.emplace_back(work);
-        }
-        work();
-        for (std::thread& th : pool) {
-            th.join();
-        }
+        const auto catalogue = tgh::MakeSphereCatalogue(lon, lat, reach, diameter, n, radius);
         std::vector<tg_tile_desc> d;
         std::vector<tg_bin> b;
         std::vector<uint32_t> e;
-        uint64_t out = 0;
-        for (size_t t = 0; t < tileCount; ++t) {
-            const tgx_tile& tile = tiles[t];
-            tg_tile_desc desc{};
-            desc.origin_x = tile.origin_x;
-            desc.origin_y = tile.origin_y;
-            desc.pixel = tile.pixel;
-            desc.width = tile.width;
-            desc.height = tile.height;
-            desc.out_offset = out;
-            desc.bin_first = uint32_t(b.size());
-            desc.bins_across = (tile.width + tgh::kSide - 1) / tgh::kSide;
-            d.push_back(desc);
-            out += uint64_t(tile.width) * tile.height;
-            uint32_t first = uint32_t(e.size());
-            for (const uint32_t count : counts[t]) {
-                b.push_back({uint32_t(t), first, count, 0u});
-                first += count;
-            }
-            e.insert(e.end(), lists[t].begin(), lists[t].end());
-            if (e.size() > 0xFFFFFFFFull || b.size() > 0xFFFFFFFFull) {
-                throw std::invalid_argument("batch too large for 32-bit bin and entry indices");
-            }
-        }
+        tgh::BinSphereBatch(*catalogue, tiles, tileCount, threads, d, b, e);
         *descs = tgh::Export(d);
         *bins = tgh::Export(b);
         *binCount = b.size();
         *entries = tgh::Export(e);
         *entryCount = e.size();
+    });
+}
+
+int tgx_sphere_index_create(const double* lon, const double* lat, const double* reach,
+                            const double* diameter, size_t n, double radius, tgx_index** out,
+                            char* err, size_t errLen) {
+    return tgh::Guard(err, errLen, [&] {
+        if (out == nullptr) {
+            throw std::invalid_argument("tgx_sphere_index_create: null argument");
+        }
+        *out = nullptr;
+        auto index = std::make_unique<tgx_index>();
+        index->sphere = tgh::MakeSphereCatalogue(lon, lat, reach, diameter, n, radius);
+        index->cx = index->sphere->lon;
+        index->cy = index->sphere->lat;
+        index->reach.assign(reach, reach + n);
+        index->diameter = index->sphere->diameter;
+        *out = index.release();
     });
 }
 

@@ -482,9 +482,9 @@ __device__ __forceinline__ CubeSteps MakeCubeSteps(uint32_t lane) {
 // 4 x 4 x 4 lattice points from one below lane 27's cell hold every corner of the wave: each lane
 // hashes one of them (one PCG hash instead of eight, whose 32-bit multiplies are the costliest
 // instructions of the noise), the gradient codes (the hash's low four bits, all PerlinGradient3
-// reads) of each point's cell are packed into one word by three lane exchanges, and each lane reads
-// its cell's word with a fourth. Otherwise every lane hashes its own eight corners. The hashes'
-// low bits, and so the noise, are Perlin3's.
+// reads) of the four points of each point's cell face are packed into one word by two lane
+// exchanges, and each lane reads its cell's two faces with two more. Otherwise every lane hashes
+// its own eight corners. The hashes' low bits, and so the noise, are Perlin3's.
 __device__ __forceinline__ float Perlin3Shared(int32_t ix, int32_t iy, int32_t iz, float fx,
                                                float fy, float fz, uint32_t zKey, uint32_t lane,
                                                const CubeSteps& steps) {
@@ -498,7 +498,7 @@ __device__ __forceinline__ float Perlin3Shared(int32_t ix, int32_t iy, int32_t i
     const uint32_t ax = uint32_t(bx) * kLcgMul + kLcgAdd + steps.x;
     const uint32_t ay = uint32_t(by) * kLcgMul + kLcgAdd + steps.y;
     const uint32_t az = (uint32_t(bz) + zKey) * kLcgMul + kLcgAdd + steps.z;
-    // Nibble k of a point's word: the code of corner (k & 1, (k >> 1) & 1, k >> 2) of its cell.
+    // Nibble k of `codes`: the code of corner (k & 1, (k >> 1) & 1, k >> 2) of the lane's cell.
     // Lanes whose point has no cell in the block (an index of 3) pack words no lane reads.
     const auto from = [](uint32_t byteAddress, uint32_t value) {
         return uint32_t(__builtin_amdgcn_ds_bpermute(int32_t(byteAddress), int32_t(value)));
@@ -507,8 +507,9 @@ __device__ __forceinline__ float Perlin3Shared(int32_t ix, int32_t iy, int32_t i
     const uint32_t byteAt = lane << 2u;
     word |= from(byteAt + 4u, word) << 4u;    // the point one along x
     word |= from(byteAt + 16u, word) << 8u;   // one along y
-    word |= from(byteAt + 64u, word) << 16u;  // one along z
-    const uint32_t codes = from((rx + 4u * ry + 16u * rz) << 2u, word);
+    // The cell's four corners at its z and the four one along z, read independently.
+    const uint32_t cellAt = (rx + 4u * ry + 16u * rz) << 2u;
+    const uint32_t codes = from(cellAt, word) | (from(cellAt + 64u, word) << 16u);
     const float c000 = Grad3(codes, fx, fy, fz);
     const float c100 = Grad3(codes >> 4u, fx - 1.0f, fy, fz);
     const float c010 = Grad3(codes >> 8u, fx, fy - 1.0f, fz);

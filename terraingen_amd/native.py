@@ -527,6 +527,8 @@ def batch_pair_ops(descs, bins, entries, foot: np.ndarray, cost: np.ndarray, thr
 
 HOST_ABI["tgx_index_create"] = (C.c_int, [VP, VP, VP, VP, SZ, C.POINTER(VP), CP, SZ])
 HOST_ABI["tgx_index_free"] = (None, [VP])
+HOST_ABI["tgx_sphere_index_create"] = (C.c_int, [VP, VP, VP, VP, SZ, C.c_double, C.POINTER(VP),
+                                                 CP, SZ])
 HOST_ABI["tgx_index_count"] = (SZ, [VP])
 HOST_ABI["tgx_index_bin"] = (C.c_int, [VP, VP, SZ, C.c_uint, C.POINTER(VP), C.POINTER(VP),
                                        C.POINTER(SZ), C.POINTER(VP), C.POINTER(SZ), CP, SZ])
@@ -634,16 +636,23 @@ class CatalogueIndex:
     """``tgx_index``: a catalogue footprint (x, y, reach, D per crater, apply order) indexed once
     for binning many batches of tiles."""
 
-    def __init__(self, foot: np.ndarray):
+    def __init__(self, foot: np.ndarray, radius: float = 0.0):
+        """On a sphere of `radius` meters (``tgx_sphere_index_create``) the footprint's x and y
+        are longitude and latitude in degrees and its reach is in meters of arc."""
         lib = host_library()
         foot = np.ascontiguousarray(foot, np.float64)
         cols = [np.ascontiguousarray(foot[:, i]) for i in range(4)]
         h = VP()
         err = C.create_string_buffer(512)
-        _check(lib.tgx_index_create(*(c.ctypes.data for c in cols), cols[0].size, C.byref(h),
-                                    err, 512), err)
+        if radius > 0.0:
+            _check(lib.tgx_sphere_index_create(*(c.ctypes.data for c in cols), cols[0].size,
+                                               radius, C.byref(h), err, 512), err)
+        else:
+            _check(lib.tgx_index_create(*(c.ctypes.data for c in cols), cols[0].size,
+                                        C.byref(h), err, 512), err)
         self.handle = h.value
         self.count = cols[0].size
+        self.radius = radius
 
     def bin(self, tiles: np.ndarray, threads: int = 0):
         """(tile descriptors, bins, entries) of a structured array of tiles."""

@@ -438,6 +438,7 @@ class SphereTileBatch(TileBatch):
                       self.data.size, diam.ctypes.data, C.byref(cat))
         self.catalogue = cat
         self.dev.call("tg_hip_catalogue_frames", cat, self.frames.ctypes.data, self.frames.size)
+        self.index = N.CatalogueIndex(self.foot, radius=self.radius)
         t2 = time.perf_counter()
         self.plan = None
         if self.source_fractals:

@@ -2,7 +2,7 @@
 """A served stream of tile requests from a moving camera, driven through the tile server.
 
 Requests arrive as a Poisson process. At each arrival, the camera has flown along a straight
-ground track at constant speed and altitude. The requested tile is the quadtree tile, at the level
+ground track (on a sphere, a great circle) at constant speed and altitude. The requested tile is the quadtree tile, at the level
 of detail the camera's distance calls for, that holds a ground point drawn around the camera.
 
 The level-of-detail rule (a screen-space-error rule of this module):
@@ -60,6 +60,96 @@ class CameraStream:
         out["pixel"] = side / (edge - 1)
         out["width"] = edge
         out["height"] = edge
+        return out
+
+
+def destination(lon, lat, bearing, arc):
+    """The point `arc` radians along the great circle leaving (lon, lat) (degrees) at `bearing`
+    (radians clockwise from north), in degrees."""
+    phi, lam = np.radians(lat), np.radians(lon)
+    sin_phi2 = np.sin(phi) * np.cos(arc) + np.cos(phi) * np.sin(arc) * np.cos(bearing)
+    phi2 = np.arcsin(np.clip(sin_phi2, -1.0, 1.0))
+    lam2 = lam + np.arctan2(np.sin(bearing) * np.sin(arc) * np.cos(phi),
+                            np.cos(arc) - np.sin(phi) * sin_phi2)
+    lon2 = (np.degrees(lam2) + 180.0) % 360.0 - 180.0
+    return lon2, np.degrees(phi2)
+
+
+@dataclass
+class GreatCircleStream(CameraStream):
+    """The stream on a sphere: the camera flies `path_length` meters along the great circle that
+    leaves (`start_lon`, `start_lat`) at `heading` (degrees clockwise from north), and tiles are
+    longitude-latitude quadtree tiles whose side in degrees is the level's ground side over the
+    sphere's meters per degree."""
+    radius: float = 1737400.0
+    start_lon: float = 20.0
+    start_lat: float = 5.0
+    heading: float = 60.0
+
+    def unit_length(self) -> float:
+        return self.radius * np.pi / 180.0
+
+    def track(self, along: np.ndarray, across: np.ndarray | float = 0.0):
+        """Longitude and latitude `along` meters down the track and `across` meters to its right."""
+        lon, lat = destination(self.start_lon, self.start_lat, np.radians(self.heading),
+                               np.asarray(along) / self.radius)
+        if np.all(np.asarray(across) == 0.0):
+            return lon, lat
+        # The track's bearing there: back to the start, turned half a circle.
+        back = np.arctan2(
+            np.sin(np.radians(self.start_lon - lon)) * np.cos(np.radians(self.start_lat)),
+            np.cos(np.radians(lat)) * np.sin(np.radians(self.start_lat)) -
+            np.sin(np.radians(lat)) * np.cos(np.radians(self.start_lat)) *
+            np.cos(np.radians(self.start_lon - lon)))
+        forward = np.where(np.asarray(along) > 0.0, back + np.pi, np.radians(self.heading))
+        return destination(lon, lat, forward + 0.5 * np.pi, np.asarray(across) / self.radius)
+
+    def requests(self) -> np.ndarray:
+        rng = np.random.default_rng(self.seed)
+        n = rng.poisson(self.rate * self.duration)
+        t = np.sort(rng.uniform(0.0, self.duration, n))
+        cam_lon, cam_lat = self.track(self.path_length * t / self.duration)
+        ground = np.exp(rng.uniform(np.log(self.min_ground), np.log(self.max_ground), n))
+        bearing = rng.uniform(0.0, 2.0 * np.pi, n)
+        plon, plat = destination(cam_lon, cam_lat, bearing, ground / self.radius)
+        rho = np.hypot(ground, self.altitude)
+        level = np.clip(self.levels - np.floor(np.log2(rho / self.altitude)), 0,
+                        self.levels).astype(np.int64)
+        side = self.finest_side * np.exp2(self.levels - level) / self.unit_length()
+        edge = np.asarray(self.edges)[rng.integers(0, len(self.edges), n)]
+        out = np.zeros(n, dtype=[("t", np.float64), ("level", np.int32), ("origin_x", np.float64),
+                                 ("origin_y", np.float64), ("pixel", np.float64),
+                                 ("width", np.uint32), ("height", np.uint32)])
+        out["t"] = t
+        out["level"] = level
+        out["origin_x"] = np.floor(plon / side) * side
+        out["origin_y"] = (np.floor(plat / side) + 1.0) * side
+        out["pixel"] = side / (edge - 1)
+        out["width"] = edge
+        out["height"] = edge
+        return out
+
+    def swath_craters(self, half_width: float, coefficient: float = 0.05,
+                      exponent: float = 2.0, min_diameter: float = 2.0,
+                      max_diameter: float = 200.0, margin: float = 2000.0,
+                      seed: int = 5) -> np.ndarray:
+        """A synthetic catalogue along the track (CRATER_DTYPE, longitude and latitude in x and
+        y): the power law N(>=D) = coefficient D^-exponent per m^2 over the strip `half_width`
+        meters each side of the track and `margin` meters beyond its ends, centres uniform in
+        (along, across), diameters by inverse transform, ages uniform over 1 Gyr."""
+        from . import native as N
+
+        rng = np.random.default_rng(seed)
+        length = self.path_length + 2.0 * margin
+        lo, hi = min_diameter ** -exponent, max_diameter ** -exponent
+        count = rng.poisson(coefficient * length * 2.0 * half_width * (lo - hi))
+        along = rng.uniform(-margin, self.path_length + margin, count)
+        across = rng.uniform(-half_width, half_width, count)
+        out = np.zeros(count, N.CRATER_DTYPE)
+        out["x"], out["y"] = self.track(along, across)
+        out["diameter"] = (lo - rng.uniform(0.0, 1.0, count) * (lo - hi)) ** (-1.0 / exponent)
+        out["age_gyr"] = rng.uniform(0.0, 1.0, count)
+        out["id"] = np.arange(1, count + 1, dtype=np.uint64)
         return out
 
 

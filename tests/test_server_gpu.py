@@ -100,3 +100,43 @@ def test_submit_rejects_oversized_and_malformed_tiles(hip, batch):
     h, _ = server.wait(server.submit(10.0, 300.0, 0.5, 33, 17))
     assert h.shape == (17, 33) and np.isfinite(h).all()
     server.close()
+
+
+def test_sphere_server_alone_equals_mix_equals_batched(hip):
+    """A server over a sphere's index runs longitude-latitude tiles on that sphere: each tile's
+    heights are the same alone, in a mix, and through the batched path."""
+    from terraingen_amd import native as N
+    from terraingen_amd.pipeline import SphereTileBatch, global_craters, sphere_tiles
+
+    body = 20000.0
+    b = SphereTileBatch(hip, [C2_FRACTAL], CraterLayerSpec(seed=4), region=None, radius=body,
+                        craters_given=global_craters(8000, body, max_diameter=2000.0, seed=3))
+    b.build()
+    arr = sphere_tiles(24, body, seed=5, min_k=4, max_k=7, pixel_range=(0.25, 8.0))
+    tiles = [(float(x["origin_x"]), float(x["origin_y"]), float(x["pixel"]), int(x["width"]),
+              int(x["height"])) for x in arr]
+    server = TileServer(b, slots=3, max_batch_tiles=8, max_batch_cells=1 << 20,
+                        pool_bytes=1 << 26)
+    try:
+        alone = [server.wait(server.submit(*t))[0] for t in tiles]
+        tickets = [server.submit(*t) for t in tiles]
+        sizes = []
+        for i, ticket in enumerate(tickets):
+            h, info = server.wait(ticket)
+            sizes.append(info["batch_tiles"])
+            assert np.array_equal(h.view(np.uint32), alone[i].view(np.uint32)), i
+        assert server.stats()["failed"] == 0 and max(sizes) > 1
+    finally:
+        server.close()
+    b.stage(arr)
+    out = N.DeviceArray(hip, b.cells * 4)
+    b.run(out, use_graph=False)
+    b.wait()
+    got = out.download(np.empty(b.cells, np.float32))
+    shaped = 0
+    for i in range(len(tiles)):
+        assert np.array_equal(b.tile_heights(got, i).view(np.uint32),
+                              alone[i].view(np.uint32)), i
+        shaped += b.plain_inputs(i)["craters"] is not None
+    b.close()
+    assert shaped > 0

@@ -176,3 +176,17 @@ def test_global_catalogue_density():
     assert np.all(tiles["origin_y"] <= 90.0)
     meters = tiles["pixel"] * MOON_RADIUS * np.pi / 180.0
     assert meters.min() >= 0.25 and meters.max() <= 8.0
+
+
+def test_kept_sphere_index_bins_as_the_one_shot_binning():
+    """tgx_sphere_index_create + tgx_index_bin (what the tile server uses) bins exactly as
+    tgx_sphere_bin."""
+    cat = global_craters(30000, 20000.0, max_diameter=2000.0, seed=8)
+    foot = np.stack([cat["x"], cat["y"], 1.07 * cat["diameter"], cat["diameter"]], axis=1)
+    tiles = sphere_tiles(40, 20000.0, seed=2, min_k=4, max_k=7, pixel_range=(0.25, 20.0))
+    want = N.sphere_bin(foot, 20000.0, tiles, 2)
+    index = N.CatalogueIndex(foot, radius=20000.0)
+    got = index.bin(tiles, 3)
+    for a, b in zip(want, got):
+        assert a.tobytes() == b.tobytes()
+    assert got[2].size > 0
