@@ -548,7 +548,11 @@ def bench_sphere(args, N, dev) -> dict:
         "host_ms": {k: round(v, 1) for k, v in batch.timings.items()} | {
             "total_setup": round(host_ms, 1)},
         "roofline": roofline(ops_noise + ops_craters, kernel_ms, cells * 4.0) | {
-            "kernel": "BatchBodyTiles", "ops_noise": ops_noise, "ops_craters": ops_craters},
+            "kernel": "BatchBodyTiles", "ops_noise": ops_noise, "ops_craters": ops_craters,
+            "note": "achieved counts the reference's algorithm (eight corner hashes per 3D "
+                    "octave); the kernel hashes each lattice point once per wave where a "
+                    "square's corners fit a 4x4x4 block, so it can exceed the peak. "
+                    "valu_issue_frac is the fraction of the peak the hardware issued"},
     }
     if tiles.size == 1024 and (counted := counted_traffic(
             kernel_ms, "batch_body_tiles_c4_sphere_1024_counters.md")) is not None:
