@@ -29,8 +29,9 @@ the square.
   tile, at levels 0-10 by slant range, that holds a ground point near the camera, with edges of
   129, 257, 513 or 1025 cells. With the default workload the track is a great circle on the
   Moon and the tiles are longitude-latitude ones (3D body noise, frame-placed craters); with
-  ``tiles`` it is a straight track on a plane. The crater catalogue (D >= 2 m, same SFD, seed 5)
-  covers a 2 km swath along the track. It reports the
+  ``tiles`` it is a straight track on a plane. The crater catalogue (C3's population: N(>D) =
+  0.0156 D^-2.2 per m^2, D 0.5-300 m, complex from 50 m with central peaks and wall terraces from
+  150 m, seed 5) covers a 2 km swath along the track; boulders are not served. It reports the
   p50/p99 request-to-host latency (submission to heights in the caller's memory) for 513 x 513
   tiles and for all tiles, and the served Gcells/s.
 
@@ -572,14 +573,22 @@ def bench_serving(args, N, dev) -> dict:
               else CameraStream(duration=args.serve_seconds))
     requests = stream.requests()
     t0 = time.perf_counter()
-    spec = CraterLayerSpec(seed=5, min_diameter=2.0)
+    # The crater population of the dense C3 field: slope -2.2, D 0.5-300 m, complex from 50 m
+    # with central peaks, wall terraces from 150 m (full at 300 m). Prepared at the finest cell
+    # served (a level-10 tile of 1025 cells: 0.125 m), so no crater is dropped.
+    spec = CraterLayerSpec(sfd_coefficient=0.0156, sfd_exponent=2.2, min_diameter=0.5,
+                           max_diameter=300.0, complex_=1, transition_diameter=50.0, peaks=1,
+                           terraces=1, terrace_onset=150.0, terrace_full=300.0, seed=5)
+    finest = stream.finest_side / (max(stream.edges) - 1)
     if sphere:
-        swath = SphereTileBatch(dev, [C2_FRACTAL], spec, region=None, finest_pixel=1.0,
-                                radius=MOON, craters_given=stream.swath_craters(1000.0))
+        craters = stream.swath_craters(1000.0, coefficient=0.0156, exponent=2.2,
+                                       min_diameter=0.5, max_diameter=300.0)
+        swath = SphereTileBatch(dev, [C2_FRACTAL], spec, region=None, finest_pixel=finest,
+                                radius=MOON, craters_given=craters)
     else:
         swath = TileBatch(dev, [C2_FRACTAL], spec,
                           region=(-2000.0, -1000.0, stream.path_length + 2000.0, 1000.0),
-                          finest_pixel=1.0)
+                          finest_pixel=finest)
     swath.build()
     server = TileServer(swath, slots=4, max_batch_tiles=32, max_batch_cells=1 << 24,
                         pool_bytes=64 << 30, bin_threads=8)
@@ -600,9 +609,10 @@ def bench_serving(args, N, dev) -> dict:
     cells = float((requests["width"].astype(np.float64) * requests["height"]).sum())
     return {
         "stream": ("a 50 km great-circle track on the 1737.4 km sphere at 200 m, lon/lat "
-                   "quadtree tiles, 3D body noise, a 2 km swath of craters D >= 2 m" if sphere
-                   else "a 50 km straight track on a plane at 200 m, a 2 km swath of craters "
-                        "D >= 2 m"),
+                   "quadtree tiles, 3D body noise" if sphere
+                   else "a 50 km straight track on a plane at 200 m") +
+                  "; a 2 km swath of C3's craters (N(>D)=0.0156 D^-2.2, D 0.5-300 m, complex "
+                  "from 50 m with peaks, terraces from 150 m); no boulders",
         "requests": int(requests.size), "rate_per_s": stream.rate,
         "duration_s": stream.duration,
         "latency_513_ms": {"p50": round(percentile(lat_ms[is513], 50), 3),

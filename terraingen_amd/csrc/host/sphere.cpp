@@ -298,18 +298,28 @@ std::shared_ptr<const SphereCatalogue> MakeSphereCatalogue(const double* lon, co
         // Cells at least twice the largest footprint, doubled until they hold a few craters
         // each on average, so that a tile visits few cells and reads few craters per cell.
         level.cellDeg = std::clamp(2.0 * level.maxReachDeg, 1e-7, 1.0);
+        std::vector<std::int64_t> keys(members.size());
         for (;;) {
             level.lonCells = std::int64_t(std::ceil(360.0 / level.cellDeg));
             level.latCells = std::int64_t(std::ceil(180.0 / level.cellDeg)) + 1;
-            level.cells.clear();
-            for (const std::uint32_t k : members) {
-                level.cells[level.Key(level.LonCell(c->lon[k]), level.LatCell(c->lat[k]))]
-                    .push_back(k);
+            for (std::size_t m = 0; m < members.size(); ++m) {
+                const std::uint32_t k = members[m];
+                keys[m] = level.Key(level.LonCell(c->lon[k]), level.LatCell(c->lat[k]));
             }
-            if (level.cellDeg >= 1.0 || members.size() >= 8 * level.cells.size()) {
+            if (level.cellDeg >= 1.0) {
+                break;
+            }
+            std::vector<std::int64_t> distinct = keys;
+            std::sort(distinct.begin(), distinct.end());
+            const std::size_t occupied = std::size_t(
+                std::unique(distinct.begin(), distinct.end()) - distinct.begin());
+            if (members.size() >= 8 * occupied) {
                 break;
             }
             level.cellDeg = std::min(1.0, 2.0 * level.cellDeg);
+        }
+        for (std::size_t m = 0; m < members.size(); ++m) {
+            level.cells[keys[m]].push_back(members[m]);
         }
         c->levels.push_back(std::move(level));
     }
