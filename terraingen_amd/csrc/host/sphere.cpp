@@ -331,6 +331,12 @@ void BinSphereBatch(const SphereCatalogue& catalogue, const tgx_tile* tiles,
                     std::vector<tg_bin>& b, std::vector<std::uint32_t>& e) {
     for (std::size_t t = 0; t < tileCount; ++t) {
         CheckTile(tiles[t], t);
+        // A longitude-latitude tile lies between the poles.
+        const tgx_tile& tile = tiles[t];
+        if (!(tile.origin_y <= 90.0 && tile.origin_y - tile.height * tile.pixel >= -90.0)) {
+            throw std::invalid_argument("tile " + std::to_string(t) +
+                                        " reaches past a pole (its latitudes leave [-90, 90])");
+        }
     }
     const double unitLength = catalogue.radius * kDegree;
     const std::vector<Strip> strips = MakeStrips(tiles, tileCount);

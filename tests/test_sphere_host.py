@@ -190,3 +190,14 @@ def test_kept_sphere_index_bins_as_the_one_shot_binning():
     for a, b in zip(want, got):
         assert a.tobytes() == b.tobytes()
     assert got[2].size > 0
+
+
+def test_sphere_tiles_past_a_pole_are_refused():
+    foot = np.array([[10.0, 5.0, 50.0, 100.0]])
+    tiles = np.zeros(1, N.TILE_DTYPE)
+    tiles[0] = (0.0, 90.5, 0.01, 16, 16)
+    with pytest.raises(Exception, match="pole"):
+        N.sphere_bin(foot, MOON_RADIUS, tiles)
+    tiles[0] = (0.0, -89.99, 0.01, 16, 16)
+    with pytest.raises(Exception, match="pole"):
+        N.sphere_bin(foot, MOON_RADIUS, tiles)
