@@ -42,8 +42,8 @@ enum {
     TG_KERNEL_NOISE_BODY = 4   /* NoiseLayerKernel on a body: heights += EvaluateFractals3 at each
                                 * cell's body-fixed position (src/noise/surface_noise.cpp:9-20),
                                 * which the reference runs on the CPU only (noise_layer.cpp:113);
-                                * one extra, per cell: x, y, z as doubles. Unwarped Perlin
-                                * fractals. */
+                                * one extra, per cell: x, y, z as doubles. Every basis, fractal
+                                * type, blend and 3D domain warp (fractals.h:221-269). */
 };
 
 typedef struct tg_dev tg_dev;
@@ -243,8 +243,7 @@ int tg_hip_batch_tune(tg_batch* batch, uint32_t crater_walk, uint32_t groups_per
 /* The surface of a batch's tiles: 0 (the default) for a plane, else a sphere of `sphere_radius`
  * meters whose tiles are longitude-latitude rasters (a tile's origin_x, origin_y and pixel are its
  * west and north edges and cell size in degrees). On a sphere each cell's noise is the stack's at
- * its body-fixed position (EvaluateFractals3, src/noise/surface_noise.cpp:9-20; the plan's
- * fractals must be unwarped Perlin ones, as tg_hip_run(TG_KERNEL_NOISE_BODY) requires), craters
+ * its body-fixed position (EvaluateFractals3, src/noise/surface_noise.cpp:9-20), craters
  * are placed by their frames (tg_hip_catalogue_frames) and weighted at the tile's spacing, pixel
  * times radius times pi / 180 (RasterGrid::Spacing, src/raster/grid.cpp:28-33). The bins come
  * from tgx_sphere_bin. The crater walk is always the culled one. */

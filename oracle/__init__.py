@@ -103,6 +103,8 @@ def _declare(lib: C.CDLL) -> None:
         "or_eval_fractals": (f, [f, f, fp, sz]), "ref_eval_fractals": (f, [f, f, fp, sz]),
         "or_eval_fractals_f64": (d, [d, d, fp, sz]),
         "or_fractal3": (f, [d, d, d, fp]), "ref_fractal3": (f, [d, d, d, fp]),
+        "or_basis3": (f, [u, f, f, f, u]), "ref_basis3": (f, [u, f, f, f, u]),
+        "or_worley3": (None, [f, f, f, u, vp]),
         "or_eval_fractals3": (f, [d, d, d, fp, sz]), "ref_eval_fractals3": (f, [d, d, d, fp, sz]),
         "or_eval_fractals3_f64": (d, [d, d, d, fp, sz]),
         "or_body_noise_grid": (None, [C.POINTER(N.NoiseLayerKernel), vp, sz,

@@ -303,7 +303,7 @@ template <class R> inline R Fractal2(R x, R y, const Params& p) {
     return Finish(sums, p);
 }
 
-// ---- 3D noise on a body (Perlin basis) ---------------------------------------------------------
+// ---- 3D noise on a body ------------------------------------------------------------------------
 
 // PerlinGradient3 (include/terraingen/noise/perlin.h:38-43).
 template <class R> inline R Grad3(u32 hash, R x, R y, R z) {
@@ -330,7 +330,158 @@ inline R Perlin3(std::int32_t ix, std::int32_t iy, std::int32_t iz, R fx, R fy, 
     return std::clamp(Mix(w, nearSide, farSide), R(-1), R(1));
 }
 
-// Fractal3 (include/terraingen/noise/fractal.h:284-317) for the Perlin basis: FP64 turns about the
+// SimplexCorner3 (include/terraingen/noise/simplex.h:27-35); the corner hash is Perlin's
+// (PerlinCornerHash3, perlin.h:49-51).
+template <class R>
+inline R SimplexCorner3(u32 ci, u32 cj, u32 ck, R dx, R dy, R dz, u32 zKey) {
+    const R t = R(0.5) - dx * dx - dy * dy - dz * dz;
+    if (t <= R(0)) {
+        return R(0);
+    }
+    const R t2 = t * t;
+    return t2 * t2 * Grad3<R>(Pcg3({ci, cj, ck + zKey}).x, dx, dy, dz);
+}
+
+// SimplexNoise3 over a split lattice point (simplex.h:74-114): the skew by (x + y + z) / 3 split
+// into the cell sum's integer third and a fraction; the unskew constant is the reference's float
+// 1 / 6.
+template <class R>
+inline R Simplex3(std::int32_t ix, std::int32_t iy, std::int32_t iz, R fx, R fy, R fz, u32 seed) {
+    const u32 zKey = Pcg(seed);
+    const std::int32_t cellSum = std::int32_t(u32(ix) + u32(iy) + u32(iz));
+    std::int32_t third = cellSum / 3;
+    if (cellSum - 3 * third < 0) {
+        third -= 1;
+    }
+    const R remainder = R(cellSum - 3 * third);
+    const R skew = (remainder + fx + fy + fz) / R(3);
+    const R stepX = std::floor(fx + skew);
+    const R stepY = std::floor(fy + skew);
+    const R stepZ = std::floor(fz + skew);
+    const R unskew = (remainder + stepX + stepY + stepZ) / R(6);
+    const R x0 = fx - stepX + unskew;
+    const R y0 = fy - stepY + unskew;
+    const R z0 = fz - stepZ + unskew;
+    const u32 i1 = x0 >= y0 && x0 >= z0 ? 1u : 0u;
+    const u32 j1 = y0 > x0 && y0 >= z0 ? 1u : 0u;
+    const u32 k1 = z0 > x0 && z0 > y0 ? 1u : 0u;
+    const u32 i2 = x0 >= y0 || x0 >= z0 ? 1u : 0u;
+    const u32 j2 = y0 > x0 || y0 >= z0 ? 1u : 0u;
+    const u32 k2 = z0 > x0 || z0 > y0 ? 1u : 0u;
+    const R g = R(1.0f / 6.0f);
+    const R g2 = R(2.0f * (1.0f / 6.0f));
+    const R g3 = R(3.0f * (1.0f / 6.0f));
+    const u32 i = u32(ix) + u32(third) + u32(std::int32_t(stepX));
+    const u32 j = u32(iy) + u32(third) + u32(std::int32_t(stepY));
+    const u32 k = u32(iz) + u32(third) + u32(std::int32_t(stepZ));
+    const R sum = SimplexCorner3<R>(i, j, k, x0, y0, z0, zKey) +
+                  SimplexCorner3<R>(i + i1, j + j1, k + k1, x0 - R(i1) + g, y0 - R(j1) + g,
+                                    z0 - R(k1) + g, zKey) +
+                  SimplexCorner3<R>(i + i2, j + j2, k + k2, x0 - R(i2) + g2, y0 - R(j2) + g2,
+                                    z0 - R(k2) + g2, zKey) +
+                  SimplexCorner3<R>(i + 1u, j + 1u, k + 1u, x0 - R(1) + g3, y0 - R(1) + g3,
+                                    z0 - R(1) + g3, zKey);
+    return std::clamp(R(76) * sum, R(-1), R(1));
+}
+
+// ValueNoise3 (value.h:22-26,57-73): corner values from pcg3d(x, y, z ^ PcgHash(seed)).y
+// (ValueLatticeKey3, lattice.h:108-111), blended with the quintic fade.
+template <class R>
+inline R Value3(std::int32_t ix, std::int32_t iy, std::int32_t iz, R fx, R fy, R fz, u32 seed) {
+    const u32 key = Pcg(seed);
+    R c[8];
+    for (u32 corner = 0; corner < 8u; ++corner) {
+        c[corner] = LatticeValue<R>({u32(ix) + (corner & 1u), u32(iy) + ((corner >> 1u) & 1u),
+                                     (u32(iz) + (corner >> 2u)) ^ key});
+    }
+    const R u = Fade(fx), v = Fade(fy), w = Fade(fz);
+    const R nearSide = Mix(v, Mix(u, c[0], c[1]), Mix(u, c[2], c[3]));
+    const R farSide = Mix(v, Mix(u, c[4], c[5]), Mix(u, c[6], c[7]));
+    return Mix(w, nearSide, farSide);
+}
+
+// CellGapSquared (worley.h:24-32): the squared gap along one axis from the point to a cell d
+// cells away.
+template <class R> inline R CellGap2(std::int32_t d, R f) {
+    R gap = R(0);
+    if (d > 0) {
+        gap = R(d) - f;
+    } else if (d < 0) {
+        gap = f - R(d + 1);
+    }
+    return gap * gap;
+}
+
+// Worley3 (worley.h:75-114): the 3 x 3 x 3 cells around the point, then the outer shell of the
+// 5 x 5 x 5 cells not yet farther than F2; feature points from pcg3d(x ^ PcgHash(seed), y, z)
+// (CellularLatticeKey3, lattice.h:116-119), the nearest cell's value from PcgHash of its words.
+template <class R>
+inline Worley<R> Worley3(std::int32_t ix, std::int32_t iy, std::int32_t iz, R fx, R fy, R fz,
+                         u32 seed) {
+    const u32 seedKey = Pcg(seed);
+    R d1 = R(1e30f), d2 = R(1e30f);
+    u32 nearest = 0u;
+    for (std::int32_t shell = 0; shell < 2; ++shell) {
+        for (std::int32_t dz = -2; dz <= 2; ++dz) {
+            for (std::int32_t dy = -2; dy <= 2; ++dy) {
+                for (std::int32_t dx = -2; dx <= 2; ++dx) {
+                    const bool inner = std::abs(dx) <= 1 && std::abs(dy) <= 1 && std::abs(dz) <= 1;
+                    if (inner != (shell == 0)) {
+                        continue;
+                    }
+                    if (shell == 1 &&
+                        CellGap2(dx, fx) + CellGap2(dy, fy) + CellGap2(dz, fz) >= d2) {
+                        continue;
+                    }
+                    const U3 h =
+                        Pcg3({(u32(ix) + u32(dx)) ^ seedKey, u32(iy) + u32(dy), u32(iz) + u32(dz)});
+                    const R px = R(dx) + R(UnitFloat(h.x)) - fx;
+                    const R py = R(dy) + R(UnitFloat(h.y)) - fy;
+                    const R pz = R(dz) + R(UnitFloat(h.z)) - fz;
+                    const R d = px * px + py * py + pz * pz;
+                    if (d < d1) {
+                        d2 = d1;
+                        d1 = d;
+                        nearest = Pcg(h.x ^ h.y ^ h.z);
+                    } else if (d < d2) {
+                        d2 = d;
+                    }
+                }
+            }
+        }
+    }
+    return {std::sqrt(d1), std::sqrt(d2), R(2) * R(UnitFloat(nearest)) - R(1)};
+}
+
+// BasisNoise3 (fractal.h:115-136): Worley distances over their 3D bounds sqrt(3) and sqrt(6).
+template <class R>
+inline R Basis3(u32 basis, std::int32_t ix, std::int32_t iy, std::int32_t iz, R fx, R fy, R fz,
+                u32 seed) {
+    const R invSqrt3 = R(0.577350269189625765f);
+    const R invSqrt6 = R(0.408248290463863016f);
+    switch (basis) {
+    case TG_BASIS_SIMPLEX:
+        return Simplex3<R>(ix, iy, iz, fx, fy, fz, seed);
+    case TG_BASIS_VALUE:
+        return Value3<R>(ix, iy, iz, fx, fy, fz, seed);
+    case TG_BASIS_WORLEY_F1:
+        return R(2) * std::min(Worley3<R>(ix, iy, iz, fx, fy, fz, seed).f1 * invSqrt3, R(1)) -
+               R(1);
+    case TG_BASIS_WORLEY_F2:
+        return R(2) * std::min(Worley3<R>(ix, iy, iz, fx, fy, fz, seed).f2 * invSqrt6, R(1)) -
+               R(1);
+    case TG_BASIS_WORLEY_F2_MINUS_F1: {
+        const Worley<R> w = Worley3<R>(ix, iy, iz, fx, fy, fz, seed);
+        return R(2) * std::min((w.f2 - w.f1) * invSqrt6, R(1)) - R(1);
+    }
+    case TG_BASIS_VORONOI:
+        return Worley3<R>(ix, iy, iz, fx, fy, fz, seed).value;
+    default:
+        return Perlin3<R>(ix, iy, iz, fx, fy, fz, seed);
+    }
+}
+
+// Fractal3 (include/terraingen/noise/fractal.h:284-317) over every basis: FP64 turns about the
 // axis (1, 2, 3), FP64 frequency by repeated multiplication, FP64 lattice coordinates split before
 // narrowing (SplitLatticePrecise3, lattice.h:59-69), octave sums in R (FP32 as the reference;
 // FP64 for the FP64 oracle, where the fractions are not narrowed either).
@@ -352,10 +503,10 @@ template <class R = float> inline R Fractal3(double x, double y, double z, const
         const double py = ry * frequency + static_cast<double>(UnitFloat(h.z));
         const double pz = rz * frequency + static_cast<double>(UnitFloat(g.x));
         const double cx = std::floor(px), cy = std::floor(py), cz = std::floor(pz);
-        const R n = Perlin3<R>(std::int32_t(static_cast<std::int64_t>(cx)),
-                               std::int32_t(static_cast<std::int64_t>(cy)),
-                               std::int32_t(static_cast<std::int64_t>(cz)), R(px - cx),
-                               R(py - cy), R(pz - cz), h.x);
+        const R n = Basis3<R>(p.basis, std::int32_t(static_cast<std::int64_t>(cx)),
+                              std::int32_t(static_cast<std::int64_t>(cy)),
+                              std::int32_t(static_cast<std::int64_t>(cz)), R(px - cx), R(py - cy),
+                              R(pz - cz), h.x);
         AddOctave(sums, p, o, n);
         frequency *= static_cast<double>(p.lacunarity);
         double next[3][3] = {};

@@ -84,9 +84,27 @@ template <class R> R Stack(R x, R y, const tg_fractal* fractals, std::size_t cou
     return height;
 }
 
-// EvaluateFractals3 (include/terraingen/noise/fractals.h:235-269) for unwarped Perlin fractals at
-// body-fixed (x, y, z) meters, in precision R; NaN for a fractal this restatement does not cover
-// (warped, or another basis), so a test that reaches one fails.
+// EvaluateFractal3 (include/terraingen/noise/fractals.h:212-243): the fractal's unit value at
+// body-fixed (x, y, z) meters, warped by three Fractal3 fields (DomainWarp3) when warpAmplitude
+// > 0; the offsets are added in FP64.
+template <class R> R FractalValue3(double x, double y, double z, const tg_fractal& layer) {
+    Params unit = ParamsOf(layer);
+    unit.amplitude = 1.0f;
+    if (layer.warp_amplitude > 0.0f) {
+        Params warp{Pcg(layer.seed ^ 0x5bd1e995u), layer.warp_wavelength, layer.warp_amplitude, 4,
+                    2.0f, 0.5f, TG_BASIS_PERLIN, TG_FRACTAL_FBM, 1.0f};
+        const double dx = static_cast<double>(Fractal3<R>(x, y, z, warp));
+        warp.seed = Pcg(warp.seed);
+        const double dy = static_cast<double>(Fractal3<R>(x, y, z, warp));
+        warp.seed = Pcg(warp.seed);
+        const double dz = static_cast<double>(Fractal3<R>(x, y, z, warp));
+        return Fractal3<R>(x + dx, y + dy, z + dz, unit);
+    }
+    return Fractal3<R>(x, y, z, unit);
+}
+
+// EvaluateFractals3 (include/terraingen/noise/fractals.h:245-269) at body-fixed (x, y, z) meters,
+// in precision R.
 template <class R>
 R Stack3(double x, double y, double z, const tg_fractal* fractals, std::size_t count) {
     R height = R(0);
@@ -96,12 +114,7 @@ R Stack3(double x, double y, double z, const tg_fractal* fractals, std::size_t c
         if (!layer.enabled) {
             continue;
         }
-        if (layer.warp_amplitude > 0.0f || layer.basis != TG_BASIS_PERLIN) {
-            return R(std::nan(""));
-        }
-        Params unit = ParamsOf(layer);
-        unit.amplitude = 1.0f;
-        const R value = Fractal3<R>(x, y, z, unit);
+        const R value = FractalValue3<R>(x, y, z, layer);
         const R scaled = R(layer.amplitude) * value;
         if (!started) {
             height = scaled;
@@ -220,10 +233,24 @@ double or_eval_fractals_f64(double x, double y, const tg_fractal* fractals, std:
 
 // Fractal3 of one fractal (amplitude included), and the stack, at body-fixed (x, y, z) meters.
 float or_fractal3(double x, double y, double z, const tg_fractal* f) {
-    if (f->basis != TG_BASIS_PERLIN) {
-        return std::nanf("");
-    }
     return Fractal3<float>(x, y, z, ParamsOf(*f));
+}
+
+// BasisNoise3 at a point in lattice units split in FP32 (SplitLattice3, lattice.h:39-46).
+float or_basis3(u32 basis, float x, float y, float z, u32 seed) {
+    const float cx = std::floor(x), cy = std::floor(y), cz = std::floor(z);
+    return Basis3<float>(basis, std::int32_t(cx), std::int32_t(cy), std::int32_t(cz), x - cx,
+                         y - cy, z - cz, seed);
+}
+
+// Worley3 at a point in lattice units: out = {F1, F2, the nearest cell's value}.
+void or_worley3(float x, float y, float z, u32 seed, float* out) {
+    const float cx = std::floor(x), cy = std::floor(y), cz = std::floor(z);
+    const Worley<float> w = Worley3<float>(std::int32_t(cx), std::int32_t(cy), std::int32_t(cz),
+                                           x - cx, y - cy, z - cz, seed);
+    out[0] = w.f1;
+    out[1] = w.f2;
+    out[2] = w.value;
 }
 
 float or_eval_fractals3(double x, double y, double z, const tg_fractal* fractals,

@@ -57,6 +57,10 @@ float ref_fractal2(float x, float y, const tg_fractal* f) {
     return Fractal2(x, y, FractalParamsFrom(FromLayout(*f)));
 }
 
+float ref_basis3(std::uint32_t basis, float x, float y, float z, std::uint32_t seed) {
+    return BasisNoise3(static_cast<NoiseBasis>(basis), SplitLattice3(x, y, z), seed);
+}
+
 float ref_fractal3(double x, double y, double z, const tg_fractal* f) {
     return Fractal3(x, y, z, FractalParamsFrom(FromLayout(*f)));
 }

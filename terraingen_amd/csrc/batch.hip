@@ -374,7 +374,9 @@ __device__ __forceinline__ float VisitBodyCrater(uint32_t n, bool inside, float 
 // the wave's 8 x 8 cells.
 // kSmall: every lattice coordinate of the launch lies within 2^31 of the origin (checked on the
 // host from the radius and the plan's highest frequency).
-template <bool kAllFast, bool kSmall>
+// kGeneral: the plan holds warped fractals or bases other than Perlin (one variant, with the
+// general crater walk and 64-bit lattice indices, carries their code).
+template <bool kAllFast, bool kSmall, bool kGeneral = false>
 __global__ void __launch_bounds__(kGroupThreads, kAllFast ? 8 : 1)
     BatchBodyTiles(const tg_tile_desc* __restrict__ tiles, const tg_bin* __restrict__ bins,
                    const uint32_t* __restrict__ entries, uint32_t* __restrict__ counters,
@@ -423,7 +425,7 @@ __global__ void __launch_bounds__(kGroupThreads, kAllFast ? 8 : 1)
             const tg::SinCos lat{__shfl(mine.sin, int(8u + (lane >> 3u)), 64),
                                  __shfl(mine.cos, int(8u + (lane >> 3u)), 64)};
             const tg::Position at = tg::SpherePositionOf(lon, lat, radius);
-            float h = plan != nullptr ? elevation + tg::EvaluateStack3<true, kSmall>(
+            float h = plan != nullptr ? elevation + tg::EvaluateStack3<true, kSmall, kGeneral>(
                                                         *plan, at.x, at.y, at.z, lane, steps)
                                       : elevation;
             if (kernel != nullptr && bin.count > 0u) {
@@ -586,19 +588,14 @@ void Launch(tg_batch& b, const tg_noise_plan* plan, float elevation, const tg_ca
     if (b.radius > 0.0) {
         // The length of a degree of arc, as Surface::Sphere sets it (surface.cpp:49).
         const double unitLength = b.radius * (3.141592653589793 / 180.0);
-        // Lattice coordinates are at most |position| x frequency + 1 in magnitude (the turns
-        // are rotations), and positions are at most the radius.
-        double frequency = 0.0;
-        if (plan != nullptr) {
-            for (uint32_t i = 0; i < plan->host.count; ++i) {
-                for (uint32_t o = 0; o < plan->host.fractal[i].unit.octaves; ++o) {
-                    frequency = std::max(frequency, plan->host.body[i][o].frequency);
-                }
-            }
-        }
-        const bool small = b.radius * frequency * (1.0 + 1e-9) + 2.0 < 2147483648.0;
+        // Cell positions lie on the sphere of the batch's radius.
+        const bool small =
+            plan == nullptr || tg::BodyLatticeBound(plan->host, b.radius) < 2147483648.0;
         auto* body = allFast ? (small ? BatchBodyTiles<true, true> : BatchBodyTiles<true, false>)
                              : (small ? BatchBodyTiles<false, true> : BatchBodyTiles<false, false>);
+        if (plan != nullptr && !plan->host.bodyPerlin) {
+            body = BatchBodyTiles<false, false, true>;
+        }
         hipLaunchKernelGGL(body, dim3(groups), dim3(kGroupThreads), 0, b.stream, b.tiles, b.bins,
                            b.entries, b.counters, plan != nullptr ? plan->device : nullptr,
                            elevation, b.radius, unitLength,
@@ -901,9 +898,6 @@ int tg_hip_batch_run(tg_batch* b, const tg_noise_plan* plan, float elevation,
             throw HipError{"the batch's bins name craters the catalogue does not hold"};
         }
         if (b->radius > 0.0) {
-            if (plan != nullptr && !plan->host.bodyReady) {
-                throw HipError{"tiles on a sphere take noise of unwarped Perlin fractals only"};
-            }
             if (b->anyEntries && cat->frames == nullptr) {
                 throw HipError{"tiles on a sphere need the catalogue's crater frames "
                                "(tg_hip_catalogue_frames)"};

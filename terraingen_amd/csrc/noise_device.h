@@ -526,14 +526,177 @@ __device__ __forceinline__ float Perlin3Shared(int32_t ix, int32_t iy, int32_t i
     return StdClamp(Mix(w, nearSide, farSide), -1.0f, 1.0f);
 }
 
-// Fractal3 (fractal.h:284-317) for the Perlin basis at body-fixed (x, y, z) meters: the octave's
+// SimplexNoise3 (simplex.h:27-35,74-114) at cell (ix, iy, iz) and fraction (fx, fy, fz): the skew
+// split into the cell sum's integer third and a fraction, corner hashes as Perlin's (zKey =
+// PcgHash(seed)). A corner beyond the kernel's radius is zero, picked by a select as in Simplex2.
+__device__ __forceinline__ float SimplexCorner3(uint32_t ci, uint32_t cj, uint32_t ck, float dx,
+                                                float dy, float dz, uint32_t zKey) {
+    const float t = 0.5f - dx * dx - dy * dy - dz * dz;
+    const float t2 = t * t;
+    const float corner = t2 * t2 * Grad3(Pcg3({ci, cj, ck + zKey}).x, dx, dy, dz);
+    return !(t <= 0.0f) ? corner : 0.0f;
+}
+
+__device__ __forceinline__ float Simplex3(int32_t ix, int32_t iy, int32_t iz, float fx, float fy,
+                                          float fz, uint32_t zKey) {
+    const int32_t cellSum = int32_t(uint32_t(ix) + uint32_t(iy) + uint32_t(iz));
+    int32_t third = cellSum / 3;
+    if (cellSum - 3 * third < 0) {
+        third -= 1;
+    }
+    const float remainder = float(cellSum - 3 * third);
+    const float skew = (remainder + fx + fy + fz) / 3.0f;
+    const float stepX = floorf(fx + skew);
+    const float stepY = floorf(fy + skew);
+    const float stepZ = floorf(fz + skew);
+    const float unskew = (remainder + stepX + stepY + stepZ) / 6.0f;
+    const float x0 = fx - stepX + unskew;
+    const float y0 = fy - stepY + unskew;
+    const float z0 = fz - stepZ + unskew;
+    const uint32_t i1 = x0 >= y0 && x0 >= z0 ? 1u : 0u;
+    const uint32_t j1 = y0 > x0 && y0 >= z0 ? 1u : 0u;
+    const uint32_t k1 = z0 > x0 && z0 > y0 ? 1u : 0u;
+    const uint32_t i2 = x0 >= y0 || x0 >= z0 ? 1u : 0u;
+    const uint32_t j2 = y0 > x0 || y0 >= z0 ? 1u : 0u;
+    const uint32_t k2 = z0 > x0 || z0 > y0 ? 1u : 0u;
+    const float kUnskew = 1.0f / 6.0f;
+    const float x1 = x0 - float(i1) + kUnskew;
+    const float y1 = y0 - float(j1) + kUnskew;
+    const float z1 = z0 - float(k1) + kUnskew;
+    const float x2 = x0 - float(i2) + 2.0f * kUnskew;
+    const float y2 = y0 - float(j2) + 2.0f * kUnskew;
+    const float z2 = z0 - float(k2) + 2.0f * kUnskew;
+    const float x3 = x0 - 1.0f + 3.0f * kUnskew;
+    const float y3 = y0 - 1.0f + 3.0f * kUnskew;
+    const float z3 = z0 - 1.0f + 3.0f * kUnskew;
+    const uint32_t i = uint32_t(ix) + uint32_t(third) + uint32_t(int32_t(stepX));
+    const uint32_t j = uint32_t(iy) + uint32_t(third) + uint32_t(int32_t(stepY));
+    const uint32_t k = uint32_t(iz) + uint32_t(third) + uint32_t(int32_t(stepZ));
+    const float sum = SimplexCorner3(i, j, k, x0, y0, z0, zKey) +
+                      SimplexCorner3(i + i1, j + j1, k + k1, x1, y1, z1, zKey) +
+                      SimplexCorner3(i + i2, j + j2, k + k2, x2, y2, z2, zKey) +
+                      SimplexCorner3(i + 1u, j + 1u, k + 1u, x3, y3, z3, zKey);
+    return StdClamp(76.0f * sum, -1.0f, 1.0f);
+}
+
+// ValueNoise3 (value.h:22-26,57-73): each corner's value from pcg3d(x, y, z ^ key).y with key =
+// PcgHash(seed) (ValueLatticeKey3, lattice.h:108-111), blended with the quintic fade.
+__device__ __forceinline__ float Value3(int32_t ix, int32_t iy, int32_t iz, float fx, float fy,
+                                        float fz, uint32_t key) {
+    const uint32_t x0 = uint32_t(ix), y0 = uint32_t(iy), z0 = uint32_t(iz);
+    const auto value = [key](uint32_t x, uint32_t y, uint32_t z) {
+        return 2.0f * UnitFloat(Pcg3({x, y, z ^ key}).y) - 1.0f;
+    };
+    const float v000 = value(x0, y0, z0);
+    const float v100 = value(x0 + 1u, y0, z0);
+    const float v010 = value(x0, y0 + 1u, z0);
+    const float v110 = value(x0 + 1u, y0 + 1u, z0);
+    const float v001 = value(x0, y0, z0 + 1u);
+    const float v101 = value(x0 + 1u, y0, z0 + 1u);
+    const float v011 = value(x0, y0 + 1u, z0 + 1u);
+    const float v111 = value(x0 + 1u, y0 + 1u, z0 + 1u);
+    const float u = Fade(fx);
+    const float v = Fade(fy);
+    const float w = Fade(fz);
+    const float nearSide = Mix(v, Mix(u, v000, v100), Mix(u, v010, v110));
+    const float farSide = Mix(v, Mix(u, v001, v101), Mix(u, v011, v111));
+    return Mix(w, nearSide, farSide);
+}
+
+// CellGapSquared (worley.h:24-32): the squared gap along one axis to a cell d cells away.
+__device__ __forceinline__ float CellGap2(int32_t d, float f) {
+    float gap = 0.0f;
+    if (d > 0) {
+        gap = float(d) - f;
+    } else if (d < 0) {
+        gap = f - float(d + 1);
+    }
+    return gap * gap;
+}
+
+// Worley3 (worley.h:75-114): the 3 x 3 x 3 cells around the point, then the outer shell of the
+// 5 x 5 x 5 cells whose gap to the point is below F2; feature points from
+// pcg3d(x ^ key, y, z) with key = PcgHash(seed) (CellularLatticeKey3, lattice.h:116-119), the
+// nearest cell's value from PcgHash of its three words.
+__device__ __forceinline__ Worley Worley3(int32_t ix, int32_t iy, int32_t iz, float fx, float fy,
+                                          float fz, uint32_t key) {
+    float d1 = 1e30f;
+    float d2 = 1e30f;
+    uint32_t nearest = 0u;
+    const auto visit = [&](int32_t dx, int32_t dy, int32_t dz) {
+        const U3 h = Pcg3({(uint32_t(ix) + uint32_t(dx)) ^ key, uint32_t(iy) + uint32_t(dy),
+                           uint32_t(iz) + uint32_t(dz)});
+        const float px = float(dx) + UnitFloat(h.x) - fx;
+        const float py = float(dy) + UnitFloat(h.y) - fy;
+        const float pz = float(dz) + UnitFloat(h.z) - fz;
+        const float d = px * px + py * py + pz * pz;
+        if (d < d1) {
+            d2 = d1;
+            d1 = d;
+            nearest = Pcg(h.x ^ h.y ^ h.z);
+        } else if (d < d2) {
+            d2 = d;
+        }
+    };
+    for (int32_t dz = -1; dz <= 1; ++dz) {
+        for (int32_t dy = -1; dy <= 1; ++dy) {
+            for (int32_t dx = -1; dx <= 1; ++dx) {
+                visit(dx, dy, dz);
+            }
+        }
+    }
+    for (int32_t dz = -2; dz <= 2; ++dz) {
+        for (int32_t dy = -2; dy <= 2; ++dy) {
+            for (int32_t dx = -2; dx <= 2; ++dx) {
+                if (abs(dx) <= 1 && abs(dy) <= 1 && abs(dz) <= 1) {
+                    continue;
+                }
+                if (CellGap2(dx, fx) + CellGap2(dy, fy) + CellGap2(dz, fz) >= d2) {
+                    continue;
+                }
+                visit(dx, dy, dz);
+            }
+        }
+    }
+    return {sqrtf(d1), sqrtf(d2), 2.0f * UnitFloat(nearest) - 1.0f};
+}
+
+// BasisNoise3 (fractal.h:115-136) for the bases other than Perlin, with key = PcgHash of the
+// octave's seed; Worley distances over their 3D bounds sqrt(3) and sqrt(6).
+__device__ __forceinline__ float Basis3(uint32_t basis, int32_t ix, int32_t iy, int32_t iz,
+                                        float fx, float fy, float fz, uint32_t key) {
+    const float kInvSqrt3 = 0.577350269189625765f;
+    const float kInvSqrt6 = 0.408248290463863016f;
+    switch (basis) {
+    case TG_BASIS_SIMPLEX:
+        return Simplex3(ix, iy, iz, fx, fy, fz, key);
+    case TG_BASIS_VALUE:
+        return Value3(ix, iy, iz, fx, fy, fz, key);
+    case TG_BASIS_WORLEY_F1:
+        return 2.0f * StdMin(Worley3(ix, iy, iz, fx, fy, fz, key).f1 * kInvSqrt3, 1.0f) - 1.0f;
+    case TG_BASIS_WORLEY_F2:
+        return 2.0f * StdMin(Worley3(ix, iy, iz, fx, fy, fz, key).f2 * kInvSqrt6, 1.0f) - 1.0f;
+    case TG_BASIS_WORLEY_F2_MINUS_F1: {
+        const Worley w = Worley3(ix, iy, iz, fx, fy, fz, key);
+        return 2.0f * StdMin((w.f2 - w.f1) * kInvSqrt6, 1.0f) - 1.0f;
+    }
+    case TG_BASIS_VORONOI:
+        return Worley3(ix, iy, iz, fx, fy, fz, key).value;
+    default:
+        return Perlin3(ix, iy, iz, fx, fy, fz, key);
+    }
+}
+
+// Fractal3 (fractal.h:284-317) at body-fixed (x, y, z) meters: the octave's
 // FP64 turn and frequency from the plan, lattice coordinates split in FP64 before narrowing
 // (SplitLatticePrecise3, lattice.h:59-69), the octave sums as the 2D loop's.
 // kShared: the caller's 64 lanes are all active and their cells close together, as
 // Perlin3Shared requires, and `lane` and `steps` are the lane's.
 // kSmall: every lattice coordinate lies within 2^31 of the origin (the caller has bounded the
 // positions and frequencies), so int32_t(int64_t(c)) is one conversion of the double.
-template <bool kShared = false, bool kSmall = false>
+// kGeneral: any basis; when false the caller has checked that the basis is Perlin
+// (NoisePlan::bodyPerlin), and the other bases' code stays out of the kernel.
+template <bool kShared = false, bool kSmall = false, bool kGeneral = true>
 __device__ __forceinline__ float FractalLoop3(const FractalLoopPlan& p, const BodyOctave* body,
                                               double x, double y, double z, uint32_t lane = 0u,
                                               CubeSteps steps = {}) {
@@ -559,7 +722,10 @@ __device__ __forceinline__ float FractalLoop3(const FractalLoopPlan& p, const Bo
         const double cy = floor(py);
         const double cz = floor(pz);
         float n;
-        if constexpr (kShared) {
+        if (kGeneral && p.basis != TG_BASIS_PERLIN) {
+            n = Basis3(p.basis, index(cx), index(cy), index(cz), float(px - cx), float(py - cy),
+                       float(pz - cz), b.zKey);
+        } else if constexpr (kShared) {
             n = Perlin3Shared(index(cx), index(cy), index(cz), float(px - cx), float(py - cy),
                               float(pz - cz), b.zKey, lane, steps);
         } else {
@@ -607,9 +773,11 @@ __device__ __forceinline__ float FractalLoop3(const FractalLoopPlan& p, const Bo
     }
 }
 
-// EvaluateFractals3 (fractals.h:235-269) for a plan whose fractals are unwarped Perlin ones
-// (NoisePlan::bodyReady): the stack's height at body-fixed (x, y, z) meters.
-template <bool kShared = false, bool kSmall = false>
+// EvaluateFractals3 (fractals.h:221-269): the stack's height at body-fixed (x, y, z) meters, a
+// warped fractal evaluated at the point moved by its three warp fields (DomainWarp3), each a
+// Fractal3 whose FP32 value is added in FP64. kGeneral as for FractalLoop3: when false, every
+// fractal is an unwarped Perlin one.
+template <bool kShared = false, bool kSmall = false, bool kGeneral = true>
 __device__ __forceinline__ float EvaluateStack3(const NoisePlan& plan, double x, double y,
                                                 double z, uint32_t lane = 0u,
                                                 CubeSteps steps = {}) {
@@ -617,8 +785,17 @@ __device__ __forceinline__ float EvaluateStack3(const NoisePlan& plan, double x,
     const uint32_t count = plan.count;
     for (uint32_t i = 0; i < count; ++i) {
         const FractalPlan& f = plan.fractal[i];
+        double wx = x, wy = y, wz = z;
+        if (kGeneral && f.warped) {
+            wx = x + double(FractalLoop3<kShared, kSmall>(f.warpX, plan.warpBody[i][0], x, y, z,
+                                                          lane, steps));
+            wy = y + double(FractalLoop3<kShared, kSmall>(f.warpX, plan.warpBody[i][1], x, y, z,
+                                                          lane, steps));
+            wz = z + double(FractalLoop3<kShared, kSmall>(f.warpX, plan.warpBody[i][2], x, y, z,
+                                                          lane, steps));
+        }
         const float value =
-            FractalLoop3<kShared, kSmall>(f.unit, plan.body[i], x, y, z, lane, steps);
+            FractalLoop3<kShared, kSmall, kGeneral>(f.unit, plan.body[i], wx, wy, wz, lane, steps);
         const float scaled = f.amplitude * value;
         if (i == 0) {
             height = scaled;

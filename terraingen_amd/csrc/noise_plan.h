@@ -72,9 +72,12 @@ struct NoisePlan {
     uint32_t count;  // enabled fractals, at most TG_MAX_FRACTALS
     uint32_t maxOctaves;
     uint32_t allPerlinFbm;  // 1 when every loop is Perlin fBm without warp (the fast path)
-    uint32_t bodyReady;     // 1 when every fractal is an unwarped Perlin one (the body path's)
+    uint32_t bodyPerlin;    // 1 when every fractal is an unwarped Perlin one
     FractalPlan fractal[TG_MAX_FRACTALS];
     BodyOctave body[TG_MAX_FRACTALS][TG_MAX_OCTAVES];  // each fractal's octaves on a body
+    // The octaves on a body of a warped fractal's three warp fields (DomainWarp3,
+    // fractals.h:221-230); their sums and weights are warpX's.
+    BodyOctave warpBody[TG_MAX_FRACTALS][3][4];
 };
 
 // Pcg3dHash of (seed, octave, 0): include/terraingen/noise/hash.h:40-57, lattice.h:126-129.
@@ -185,7 +188,7 @@ inline void MakeNoisePlan(const tg_noise_layer_kernel& k, NoisePlan& plan) {
     plan.count = 0;
     plan.maxOctaves = 0;
     plan.allPerlinFbm = 1;
-    plan.bodyReady = 1;
+    plan.bodyPerlin = 1;
     const uint32_t n = k.count < TG_MAX_FRACTALS ? k.count : TG_MAX_FRACTALS;
     for (uint32_t i = 0; i < n; ++i) {
         const tg_fractal& f = k.fractals[i];
@@ -216,13 +219,44 @@ inline void MakeNoisePlan(const tg_noise_layer_kernel& k, NoisePlan& plan) {
             plan.allPerlinFbm = 0;
         }
         if (p.warped || p.unit.basis != TG_BASIS_PERLIN) {
-            plan.bodyReady = 0;
+            plan.bodyPerlin = 0;
         }
         PlanBody(plan.body[plan.count - 1], p.unit.octaves, f.seed, f.wavelength, f.lacunarity);
+        if (p.warped) {
+            // Each further 3D warp field is seeded with PcgHash of the one before.
+            uint32_t seed = HostPcg(f.seed ^ 0x5bd1e995u);
+            for (uint32_t w = 0; w < 3u; ++w) {
+                PlanBody(plan.warpBody[plan.count - 1][w], 4u, seed, f.warp_wavelength, 2.0f);
+                seed = HostPcg(seed);
+            }
+        }
         if (p.unit.octaves > plan.maxOctaves) {
             plan.maxOctaves = p.unit.octaves;
         }
     }
+}
+
+// A bound on the magnitude of every lattice coordinate the plan's body noise reaches at points
+// within `radius` of the origin: the turns are rotations, the offsets below 1 and the floor one
+// more, and a warp moves a point by at most its amplitude along each axis (so by less than twice
+// it).
+inline double BodyLatticeBound(const NoisePlan& plan, double radius) {
+    double bound = 0.0;
+    for (uint32_t i = 0; i < plan.count; ++i) {
+        const FractalPlan& f = plan.fractal[i];
+        const double reach = f.warped ? radius + 2.0 * double(f.warpX.amplitude) : radius;
+        for (uint32_t o = 0; o < f.unit.octaves; ++o) {
+            const double c = reach * plan.body[i][o].frequency;
+            bound = c > bound ? c : bound;
+        }
+        if (f.warped) {
+            for (uint32_t o = 0; o < 4u; ++o) {
+                const double c = radius * plan.warpBody[i][0][o].frequency;
+                bound = c > bound ? c : bound;
+            }
+        }
+    }
+    return bound * (1.0 + 1e-9) + 2.0;
 }
 
 }  // namespace tg

@@ -363,9 +363,6 @@ int tg_hip_run(tg_dev* dev, int kernelId, const void* kernel, size_t kernelBytes
             std::memcpy(&k, kernel, sizeof k);
             auto plan = std::make_unique<tg::NoisePlan>();
             tg::MakeNoisePlan(k, *plan);
-            if (!plan->bodyReady) {
-                throw HipError{"the body noise covers unwarped Perlin fractals only"};
-            }
             DeviceArray<tg::NoisePlan> dPlan(1);
             dPlan.Upload(plan.get(), sizeof(tg::NoisePlan));
             DeviceArray<double> dPositions(cells * 3);

@@ -70,11 +70,33 @@ def test_body_noise_skips_no_data_and_joins_strips(hip, orc):
     assert np.all(got[::6, ::5] == -9999.0)
 
 
-def test_body_noise_refuses_warped_fractals(hip):
-    kernel = N.noise_kernel([dict(C2_FBM, warp_amplitude=30.0)])
-    w, h = 8, 8
-    pos = lonlat_positions(0.0, 0.0, 0.001, w, h)
-    with pytest.raises(N.HipError, match="unwarped Perlin"):
-        hip.run(N.TG_KERNEL_NOISE_BODY, kernel, np.zeros((h, w), np.float32),
-                np.zeros(w, np.float32), np.zeros(h, np.float32),
-                N.KernelParams(w, h, N.NODATA_NONE, 0.0, 0), [(pos, 1)])
+MIXED_BASES = [
+    dict(C2_FBM, warp_amplitude=40.0, warp_wavelength=800.0),
+    dict(seed=11, basis=1, wavelength=150.0, octaves=6, amplitude=4.0),
+    dict(seed=12, basis=3, wavelength=60.0, octaves=3, amplitude=2.0, blend=1,
+         warp_amplitude=10.0, warp_wavelength=90.0),
+    dict(seed=13, basis=2, type=2, wavelength=250.0, octaves=4, blend=3),
+    dict(seed=14, basis=6, wavelength=30.0, octaves=2, amplitude=1.5, blend=2),
+]
+
+
+@pytest.mark.parametrize("fractals", [
+    MIXED_BASES,
+    [dict(seed=21, basis=4, type=1, offset=0.8, wavelength=70.0, octaves=3, amplitude=5.0),
+     dict(seed=22, basis=5, type=3, offset=0.6, wavelength=45.0, octaves=3, amplitude=2.0,
+          warp_amplitude=25.0, warp_wavelength=300.0)],
+])
+def test_body_noise_every_basis_and_warp_bit_exact(hip, orc, fractals):
+    """Simplex, value, Worley F1/F2/F2-F1 and Voronoi bases and 3D domain warp (DomainWarp3,
+    include/terraingen/noise/fractals.h:221-243) on a lunar-radius body, bit for bit."""
+    kernel = N.noise_kernel(fractals)
+    w, h = 61, 43
+    pos = lonlat_positions(-77.3, 12.9, 0.0004, w, h)
+    base = np.full((h, w), -2.0, np.float32)
+    params = N.KernelParams(w, h, N.NODATA_NONE, 0.0, 0)
+    got = base.copy()
+    hip.run(N.TG_KERNEL_NOISE_BODY, kernel, got, np.zeros(w, np.float32),
+            np.zeros(h, np.float32), params, [(pos, 1)])
+    want = oracle(orc, kernel, pos, base, params)
+    assert np.array_equal(got.view(np.uint32), want.view(np.uint32))
+    assert np.ptp(want) > 1.0

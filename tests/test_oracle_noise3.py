@@ -1,15 +1,20 @@
 # SPDX-License-Identifier: BSD-3-Clause
-"""The noise oracle on a body pinned: Fractal3 and EvaluateFractals3 (Perlin basis, every fractal
-type, stacks with every blend) bit for bit against the reference's own headers compiled into
-oracle/_ref, at points on a lunar-radius sphere and near the origin. The reference's golden
-Fractal3 values (tests/golden/noise/kernels_test.cpp:91-102) are for the Voronoi, Worley and
-simplex bases, which this build does not evaluate in 3D."""
+"""The noise oracle on a body pinned: Fractal3 and EvaluateFractals3 (every basis, every fractal
+type, stacks with every blend, 3D domain warp) bit for bit against the reference's own headers
+compiled into oracle/_ref, at points on a lunar-radius sphere and near the origin; the 3D bases
+against the reference's golden values (tests/golden/noise/kernels_test.cpp:79-103), and Worley3's
+F1 and cell value against a brute-force search over 7 x 7 x 7 cells."""
+import ctypes as C
+import struct
+
 import numpy as np
 import pytest
 
 from terraingen_amd import native as N
 
 MOON = 1737400.0
+BASES = dict(perlin=0, simplex=1, value=2, worley_f1=3, worley_f2=4, worley_f2_minus_f1=5,
+             voronoi=6)  # NoiseBasis (include/terraingen/noise/fractal.h)
 
 FRACTALS = [
     dict(seed=1, wavelength=512.0, amplitude=20.0, octaves=8, lacunarity=2.0, gain=0.5),
@@ -60,3 +65,94 @@ def test_fractal3_is_continuous_and_bounded(orc):
         a = orc.or_fractal3(*p, f)
         b = orc.or_fractal3(*(p + step), f)
         assert abs(a) <= 20.0 and abs(a - b) < 0.05
+
+
+def hx(text):
+    return float.fromhex(text)
+
+
+def f32(v):
+    return struct.unpack("<f", struct.pack("<f", v))[0]
+
+
+def test_3d_bases_golden(orc):
+    """tests/golden/noise/kernels_test.cpp:79-103."""
+    out = (C.c_float * 3)()
+    orc.or_worley3(hx("0x1p-2"), hx("0x1.8p-1"), hx("-0x1.8p+0"), 1234, out)
+    assert list(out) == [hx("0x1.113fap-1"), hx("0x1.dc0d32p-1"), hx("-0x1.20f528p-1")]
+    simplex = BASES["simplex"]
+    assert orc.or_basis3(simplex, hx("0x1p-2"), hx("0x1.8p-1"), hx("-0x1.8p+0"),
+                         1234) == hx("0x1.62c8a8p-3")
+    assert orc.or_basis3(simplex, hx("-0x1.9p+3"), hx("0x1.9p+1"), hx("0x1.c4p+2"),
+                         1234) == hx("0x1.24874ep-6")
+    for basis, want in (("voronoi", "-0x1.d05156p+5"), ("worley_f2_minus_f1", "-0x1.605be8p+6"),
+                        ("simplex", "-0x1.e90b4p+4")):
+        f = N.make_fractal(seed=42, basis=BASES[basis])
+        assert orc.or_fractal3(512.0625, -1025.0, 300.5, f) == hx(want), basis
+
+
+def test_worley3_f1_matches_brute_force(orc):
+    """F1 and the nearest cell's value from every cell within 3 of the point's cell, with the
+    feature points hashed here from pcg3d(x ^ PcgHash(seed), y, z)."""
+    seed = 77
+    key = orc.or_pcg_hash(seed)
+    rng = np.random.default_rng(5)
+    out = (C.c_float * 3)()
+    for x, y, z in rng.uniform(-40.0, 40.0, (40, 3)).astype(np.float32):
+        ix, iy, iz = int(np.floor(x)), int(np.floor(y)), int(np.floor(z))
+        fx, fy, fz = f32(x - ix), f32(y - iy), f32(z - iz)
+        best, value = None, None
+        for dz in range(-3, 4):
+            for dy in range(-3, 4):
+                for dx in range(-3, 4):
+                    words = (C.c_uint32 * 3)(((ix + dx) & 0xFFFFFFFF) ^ key,
+                                             (iy + dy) & 0xFFFFFFFF, (iz + dz) & 0xFFFFFFFF)
+                    h = (C.c_uint32 * 3)()
+                    orc.or_pcg3d(words, h)
+                    p = [dx + (h[0] >> 8) * 2.0**-24 - fx, dy + (h[1] >> 8) * 2.0**-24 - fy,
+                         dz + (h[2] >> 8) * 2.0**-24 - fz]
+                    d = p[0] ** 2 + p[1] ** 2 + p[2] ** 2
+                    if best is None or d < best:
+                        best = d
+                        nearest = orc.or_pcg_hash(h[0] ^ h[1] ^ h[2])
+                        value = 2.0 * ((nearest >> 8) * 2.0**-24) - 1.0
+        orc.or_worley3(float(x), float(y), float(z), seed, out)
+        assert abs(out[0] - np.sqrt(best)) < 1e-5
+        assert out[2] == np.float32(value)
+        assert out[1] >= out[0]
+
+
+@pytest.mark.parametrize("basis", ["simplex", "value", "worley_f1", "worley_f2",
+                                   "worley_f2_minus_f1", "voronoi"])
+def test_3d_bases_match_reference_bitwise(orc, ref, basis):
+    b = BASES[basis]
+    rng = np.random.default_rng(6)
+    for x, y, z in rng.uniform(-300.0, 300.0, (300, 3)).astype(np.float32):
+        a = orc.or_basis3(b, float(x), float(y), float(z), 99)
+        r = ref.ref_basis3(b, float(x), float(y), float(z), 99)
+        assert np.float32(a).tobytes() == np.float32(r).tobytes(), (x, y, z)
+    for type_ in range(4):
+        f = N.make_fractal(seed=8 + type_, basis=b, type=type_, offset=0.8, wavelength=700.0,
+                           octaves=5, amplitude=9.0)
+        for x, y, z in sphere_points(60, 7 + type_):
+            a = orc.or_fractal3(x, y, z, f)
+            r = ref.ref_fractal3(x, y, z, f)
+            assert np.float32(a).tobytes() == np.float32(r).tobytes(), (type_, x, y, z)
+
+
+def test_warped_mixed_stacks3_match_reference_bitwise(orc, ref):
+    """Stacks mixing bases with 3D domain warp (DomainWarp3) on a lunar sphere."""
+    specs = [dict(FRACTALS[0], warp_amplitude=40.0, warp_wavelength=800.0),
+             dict(seed=11, basis=BASES["simplex"], wavelength=150.0, octaves=6, amplitude=4.0),
+             dict(seed=12, basis=BASES["worley_f1"], wavelength=60.0, octaves=3, amplitude=2.0,
+                  blend=1, warp_amplitude=10.0, warp_wavelength=90.0),
+             dict(seed=13, basis=BASES["value"], type=2, wavelength=250.0, octaves=4, blend=3),
+             dict(seed=14, basis=BASES["voronoi"], wavelength=30.0, octaves=2, amplitude=1.5,
+                  blend=2)]
+    stack = (N.Fractal * len(specs))(*[N.make_fractal(**s) for s in specs])
+    for x, y, z in sphere_points(300, 8):
+        a = orc.or_eval_fractals3(x, y, z, stack, len(specs))
+        b = ref.ref_eval_fractals3(x, y, z, stack, len(specs))
+        assert np.float32(a).tobytes() == np.float32(b).tobytes(), (x, y, z)
+        c = orc.or_eval_fractals3_f64(x, y, z, stack, len(specs))
+        assert abs(c - a) < 1e-3
