@@ -1,8 +1,9 @@
 // SPDX-License-Identifier: BSD-3-Clause
 //
-// ORACLE — test infrastructure only (see noise_oracle.cpp). The 2D noise core shared by the noise
-// and crater oracles: PCG hashes, Perlin/simplex/value/Worley bases and the fractal octave loop,
-// templated on the precision of the per-cell arithmetic.
+// ORACLE — test infrastructure only (see noise_oracle.cpp). The noise core shared by the noise
+// and crater oracles: PCG hashes, Perlin/simplex/value/Worley bases in 2D and 3D and the fractal
+// octave loops (Fractal2 on a plane, Fractal3 on a body), templated on the precision of the
+// per-cell arithmetic.
 #pragma once
 
 #include "tg_layouts.h"

@@ -4,8 +4,9 @@
 // cpu_baseline leg may load this library, and only as the checker or the CPU baseline; the hip
 // backend never calls it.
 //
-// A CPU restatement of the reference's 2D noise (PCG hashes, Perlin/simplex/value/Worley bases,
-// fBm/ridged/billow/hybrid fractals, stacks with domain warp and blend) in two precisions:
+// A CPU restatement of the reference's noise (PCG hashes, Perlin/simplex/value/Worley bases,
+// fBm/ridged/billow/hybrid fractals, stacks with domain warp and blend; in 2D on a plane and in 3D
+// at body-fixed positions) in two precisions:
 //   * `float`  — the reference-equivalent path: every operation in the reference's order, built
 //                with -ffp-contract=off as the reference is (CMakeLists.txt:32-38), so its bits are
 //                the reference CPU backend's bits;
