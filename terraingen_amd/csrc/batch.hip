@@ -34,6 +34,8 @@ struct tg_catalogue {
     tg_crater_frame* frames = nullptr;  // MakeCraterFrame per crater, for tiles on a sphere
     size_t count = 0;
     bool allFast = false;  // every crater takes the hoisted MPD path
+    // Changes whenever what a launch reads of the catalogue changes (creation, frames).
+    uint64_t generation = tgi::NextGeneration();
 };
 
 struct tg_batch {
@@ -51,8 +53,8 @@ struct tg_batch {
     bool anyEntries = false;
     // The captured graph and the arguments it was captured for.
     hipGraphExec_t graph = nullptr;
-    const void* graphPlan = nullptr;
-    const void* graphCatalogue = nullptr;
+    uint64_t graphPlan = 0;       // generations of the plan and catalogue it was captured with
+    uint64_t graphCatalogue = 0;
     float* graphOut = nullptr;
     float graphElevation = 0.0f;
     uint32_t graphGroups = 0;
@@ -914,7 +916,10 @@ int tg_hip_batch_run(tg_batch* b, const tg_noise_plan* plan, float elevation,
         // The graph holds the launch's arguments and grid, not the staged batch (the kernel reads
         // the bin count from device memory), so a restage of the same grid replays it.
         const uint32_t groups = GroupsFor(*b);
-        if (b->graph == nullptr || b->graphPlan != plan || b->graphCatalogue != cat ||
+        const uint64_t planGeneration = plan != nullptr ? plan->generation : 0;
+        const uint64_t catalogueGeneration = cat != nullptr ? cat->generation : 0;
+        if (b->graph == nullptr || b->graphPlan != planGeneration ||
+            b->graphCatalogue != catalogueGeneration ||
             b->graphOut != out || b->graphElevation != elevation || b->graphGroups != groups ||
             b->graphWalk != b->walk || b->graphHoisted != b->hoisted ||
             b->graphRadius != b->radius ||
@@ -936,8 +941,8 @@ int tg_hip_batch_run(tg_batch* b, const tg_noise_plan* plan, float elevation,
             const hipError_t made = hipGraphInstantiate(&b->graph, graph, nullptr, nullptr, 0);
             (void)hipGraphDestroy(graph);
             Check(made, "hipGraphInstantiate");
-            b->graphPlan = plan;
-            b->graphCatalogue = cat;
+            b->graphPlan = planGeneration;
+            b->graphCatalogue = catalogueGeneration;
             b->graphOut = out;
             b->graphElevation = elevation;
             b->graphGroups = groups;
@@ -1008,6 +1013,7 @@ int tg_hip_catalogue_frames(tg_catalogue* cat, const tg_crater_frame* frames, si
         }
         Check(hipSetDevice(cat->dev->ordinal), "hipSetDevice");
         cat->frames = UploadArray(frames, count);
+        cat->generation = NextGeneration();  // graphs captured without the frames are stale
     });
 }
 

@@ -209,7 +209,8 @@ template <class T> T* PinnedArray(tgx_server& s, std::size_t count) {
 }
 
 // Gives the slot (idle) a batch and pinned buffers of at least the given capacities, keeping
-// what it outgrows until the server is freed.
+// what it outgrows until the server is freed. The new buffers are made first and the slot takes
+// them only once all exist, so a failed allocation leaves the slot as it was.
 void EnsureCapacity(tgx_server& s, Slot& slot, std::size_t tiles, std::size_t bins,
                     std::size_t entries) {
     if (slot.batch != nullptr && tiles <= slot.maxTiles && bins <= slot.maxBins &&
@@ -225,14 +226,29 @@ void EnsureCapacity(tgx_server& s, Slot& slot, std::size_t tiles, std::size_t bi
     const std::size_t e = std::max(
         grow && entries > slot.maxEntries ? entries * 2 + 1024 : entries, slot.maxEntries);
     tg_batch* batch = nullptr;
-    Raise(h.batch_create(s.dev, t, b, e, &batch, err, sizeof err), err);
-    if (const double radius = IndexRadius(*s.index); radius > 0.0) {
-        // Tiles binned against a sphere's index are longitude-latitude tiles on it.
-        const int code = h.batch_surface(batch, radius, err, sizeof err);
-        if (code != TG_OK) {
-            h.batch_free(batch);
-            Raise(code, err);
+    tg_tile_desc* descs = nullptr;
+    tg_bin* binBuf = nullptr;
+    std::uint32_t* entryBuf = nullptr;
+    try {
+        Raise(h.batch_create(s.dev, t, b, e, &batch, err, sizeof err), err);
+        if (const double radius = IndexRadius(*s.index); radius > 0.0) {
+            // Tiles binned against a sphere's index are longitude-latitude tiles on it.
+            Raise(h.batch_surface(batch, radius, err, sizeof err), err);
         }
+        descs = PinnedArray<tg_tile_desc>(s, t);
+        binBuf = PinnedArray<tg_bin>(s, b);
+        entryBuf = PinnedArray<std::uint32_t>(s, e);
+    } catch (...) {
+        for (void* p : {static_cast<void*>(descs), static_cast<void*>(binBuf),
+                        static_cast<void*>(entryBuf)}) {
+            if (p != nullptr) {
+                h.host_free(s.dev, p);
+            }
+        }
+        if (batch != nullptr) {
+            h.batch_free(batch);
+        }
+        throw;
     }
     if (slot.batch != nullptr) {
         s.retiredBatches.push_back(slot.batch);
@@ -243,16 +259,18 @@ void EnsureCapacity(tgx_server& s, Slot& slot, std::size_t tiles, std::size_t bi
     }
     slot.batch = batch;
     slot.stream = h.batch_stream(batch);
-    slot.hostDescs = PinnedArray<tg_tile_desc>(s, t);
-    slot.hostBins = PinnedArray<tg_bin>(s, b);
-    slot.hostEntries = PinnedArray<std::uint32_t>(s, e);
+    slot.hostDescs = descs;
+    slot.hostBins = binBuf;
+    slot.hostEntries = entryBuf;
     slot.maxTiles = t;
     slot.maxBins = b;
     slot.maxEntries = e;
 }
 
 // Bins, stages and launches one batch in an idle slot, then queues the copy back and the event.
-void LaunchBatch(tgx_server& s, Slot& slot) {
+// Runs without the server's lock (the slot is the dispatcher's while it is being launched) and
+// returns the launch time, which the dispatcher records under the lock.
+double LaunchBatch(tgx_server& s, Slot& slot) {
     const HipApi& h = *s.hip;
     char err[512] = {0};
     std::vector<tgx_tile> tiles(slot.requests.size());
@@ -278,9 +296,6 @@ void LaunchBatch(tgx_server& s, Slot& slot) {
     }
     float* region = s.pool + slot.poolOffset / sizeof(float);
     const double launched = Now();
-    for (Request* r : slot.requests) {
-        r->info.launched = launched;
-    }
     Raise(h.batch_run(slot.batch, s.plan, s.elevation, s.catalogue, region, cells, 0, err,
                       sizeof err),
           err);
@@ -288,6 +303,7 @@ void LaunchBatch(tgx_server& s, Slot& slot) {
                              sizeof err),
           err);
     Raise(h.event_record(s.dev, slot.event, slot.stream, err, sizeof err), err);
+    return launched;
 }
 
 void FinishSlot(tgx_server& s, Slot& slot, int status, const std::string& error) {
@@ -389,16 +405,27 @@ void Dispatch(tgx_server& s) {
             if (!Admit(s, slot)) {
                 break;
             }
-            inFlight += 1;
+            // Counted from the slots themselves: a copier may have returned a slot while the lock
+            // was down for the previous launch.
+            inFlight = 0;
+            for (const Slot& other : s.slots) {
+                inFlight += other.busy ? 1u : 0u;
+            }
             s.stats.max_in_flight = std::max(s.stats.max_in_flight, inFlight);
             lock.unlock();
             std::string failure;
+            double launched = 0.0;
             try {
-                LaunchBatch(s, slot);
+                launched = LaunchBatch(s, slot);
             } catch (const std::exception& e) {
                 failure = e.what();
             }
             lock.lock();
+            // The slot's event is queried only by this thread, below, so its requests are still
+            // the slot's here.
+            for (Request* r : slot.requests) {
+                r->info.launched = launched;
+            }
             if (!failure.empty()) {
                 // Nothing of the batch may still be queued: wait for the slot's stream first.
                 char err[512] = {0};

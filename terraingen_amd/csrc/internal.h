@@ -11,6 +11,7 @@
 
 #include <hip/hip_runtime.h>
 
+#include <atomic>
 #include <cstdio>
 #include <memory>
 #include <mutex>
@@ -50,10 +51,23 @@ struct tg_crater_set {
     bool onBody = false;
 };
 
+namespace tgi {
+
+// A number no other object of the library has had or will have: captured graphs are keyed on the
+// generations of what they read rather than on addresses, which a freed object's successor may
+// reuse.
+inline uint64_t NextGeneration() {
+    static std::atomic<uint64_t> next{1};
+    return next.fetch_add(1, std::memory_order_relaxed);
+}
+
+}  // namespace tgi
+
 struct tg_noise_plan {
     tg_dev* dev = nullptr;
     tg::NoisePlan* device = nullptr;
     tg::NoisePlan host{};
+    uint64_t generation = tgi::NextGeneration();
 };
 
 namespace tgi {

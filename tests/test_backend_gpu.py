@@ -54,4 +54,4 @@ def test_craters_through_run_kernel_match_the_reference(hip):
     assert N.run_kernel(HIP, N.TG_KERNEL_CRATERS, case.kernel, g, strips, (0.0, 0.0),
                         binding(case), max_strip_bytes=w * 4 * 37)
     assert np.array_equal(whole.view(np.uint32), strips.view(np.uint32))
-    assert np.abs(whole - case.heights).max() <= 1e-3
+    assert np.abs(whole - case.heights).max() <= 2e-5  # FP32 rounding of the fast outline

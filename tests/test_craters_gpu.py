@@ -39,7 +39,9 @@ def test_crater_kernel_matches_reference_and_oracles(hip, orc, case):
           f"{d32.max():.3g}; vs FP64 oracle max {d64.max():.3g} rms "
           f"{np.sqrt((d64 ** 2).mean()):.3g} m", flush=True)
     assert d64.max() <= TOL_FP64
-    assert d_ref.max() <= TOL_FP64
+    # The reference's FP32 arithmetic, apart from FP32 rounding of the fast outline (1.6e-6 m
+    # measured): far inside the bound.
+    assert d_ref.max() <= 2e-5
 
 
 def test_strips_join(hip, case):

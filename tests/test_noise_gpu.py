@@ -123,5 +123,5 @@ def test_full_size_c2_noise_against_fp64(hip, orc):
                           want64.ctypes.data, 0)
     d = np.abs(got[rows] - want64)
     print(f"[noise C2] vs FP64: max {d.max():.3g} m, rms {np.sqrt((d**2).mean()):.3g} m", flush=True)
-    assert d.max() <= 1e-3
+    assert d.max() <= 2e-4  # FP32 rounding of eight octaves at 20 m (1.8e-5 m measured)
     assert np.isfinite(got).all() and got.std() > 1.0

@@ -29,7 +29,8 @@ def test_c2_recipe_matches_reference_terrain(hip):
     d = np.abs(got - z["heights"])
     print(f"[c2_synthetic] GPU terrain vs reference CPU terrain: max {d.max():.3g} m, "
           f"rms {np.sqrt((d ** 2).mean()):.3g} m over {got.size} cells", flush=True)
-    assert d.max() <= 1e-3
+    # FP32 rounding of the fast outline and of the noise's sum order only (4.5e-6 m measured).
+    assert d.max() <= 5e-5
 
 
 def test_c1_quick_start_matches_reference_terrain(hip):
@@ -53,4 +54,5 @@ def test_c1_quick_start_matches_reference_terrain(hip):
     d = np.abs(got - z["heights"])
     print(f"[c1_default] GPU terrain vs reference CPU terrain: max {d.max():.3g} m over "
           f"{got.size} cells, {terrain.crater_count} craters", flush=True)
-    assert d.max() <= 1e-3
+    # FP32 rounding of the fast outline only (9.5e-7 m measured).
+    assert d.max() <= 1e-5

@@ -225,3 +225,47 @@ def test_sphere_batch_refuses_what_it_cannot_run(hip, batch):
     with pytest.raises(N.HipError, match="already has its frames"):
         hip.call("tg_hip_catalogue_frames", batch.catalogue, batch.frames.ctypes.data,
                  batch.frames.size)
+
+
+def test_graph_recaptures_after_frames_are_attached(hip, batch):
+    """A graph captured on a batch whose bins list no crater, before the catalogue had frames, is
+    not replayed once frames are attached and a batch with craters is staged: the run re-captures
+    and equals the eager path."""
+    cat = C.c_void_p()
+    diam = np.ascontiguousarray(batch.foot[:, 3])
+    hip.call("tg_hip_catalogue_create", hip.handle, C.byref(batch.kernel),
+             batch.records.ctypes.data, batch.records.size, batch.data.ctypes.data,
+             batch.data.size, diam.ctypes.data, C.byref(cat))
+    b = C.c_void_p()
+    try:
+        near = batch.tiles[:6].copy()
+        descs, bins, entries = N.sphere_bin(batch.foot, batch.radius, near, 1)
+        assert entries.size > 0
+        empty = bins.copy()
+        empty["first"] = 0
+        empty["count"] = 0
+        hip.call("tg_hip_batch_create", hip.handle, descs.size, bins.size, entries.size,
+                 C.byref(b))
+        hip.call("tg_hip_batch_surface", b, batch.radius)
+        cells = int(sum(int(d["width"]) * int(d["height"]) for d in descs))
+        out = N.DeviceArray(hip, cells * 4)
+        one = np.zeros(1, np.uint32)
+        hip.call("tg_hip_batch_stage", b, descs.ctypes.data, descs.size, empty.ctypes.data,
+                 empty.size, one.ctypes.data, 0)
+        hip.call("tg_hip_batch_run", b, batch.plan, 0.0, cat, out.ptr, cells, 1)
+        hip.call("tg_hip_batch_wait", b)
+        hip.call("tg_hip_catalogue_frames", cat, batch.frames.ctypes.data, batch.frames.size)
+        hip.call("tg_hip_batch_stage", b, descs.ctypes.data, descs.size, bins.ctypes.data,
+                 bins.size, entries.ctypes.data, entries.size)
+        hip.call("tg_hip_batch_run", b, batch.plan, 0.0, cat, out.ptr, cells, 0)
+        hip.call("tg_hip_batch_wait", b)
+        eager = out.download(np.empty(cells, np.float32))
+        for _ in range(2):
+            hip.call("tg_hip_batch_run", b, batch.plan, 0.0, cat, out.ptr, cells, 1)
+            hip.call("tg_hip_batch_wait", b)
+            graphed = out.download(np.empty(cells, np.float32))
+            assert np.array_equal(graphed.view(np.uint32), eager.view(np.uint32))
+    finally:
+        if b.value:
+            hip.lib.tg_hip_batch_free(b)
+        hip.lib.tg_hip_catalogue_free(cat)
