@@ -1,22 +1,29 @@
 // SPDX-License-Identifier: BSD-3-Clause
 //
-// Wall terraces on the host: the reference's AppendKernelTerraces
-// (src/layers/craters/shaping/crater_kernel.cpp:312-459) for the closed-form profiles this host
-// prepares. A terraced crater's data block is its header (inner wall edge, slope share, strength,
-// scarp rounding, arc cut, arc width), five arc harmonics, the rim scallop terms, then per inner
-// scarp its base and scallop, wobble and block terms, then per step its bench level, scarp share
-// and kept share, every value drawn and derived as the reference does so the block is byte-equal.
+// Wall terraces: the data block a terraced crater's kernel reads (KernelTerracedRadii,
+// include/terraingen/layers/kernels/craters.h:986-1030), prepared on the host as the reference's
+// AppendKernelTerraces prepares it (src/layers/craters/shaping/crater_kernel.cpp:312-459).
 //
-// The wall is sampled through CraterProfile::Height, which for these profiles is the FP32 kernel
-// law evaluated on the host with float arguments and the C++ float overloads of pow and sqrt
-// (martin_parkes_dunstan.cpp:9-19, fassett_thomson.cpp:14-18, complex.cpp:69-77 over
-// include/terraingen/layers/kernels/craters.h:306-507): the host functions below restate those
-// laws in the same FP32 operation order.
+// The preparation runs in two phases:
+//   1. TerraceDraws takes every value the crater's terrace stream gives, in the stream's order:
+//      the bench count's jitter, five arc phases, the rim scallop's three draws, one rise per
+//      step, then per inner scarp five doubles and a block key. Nothing drawn depends on anything
+//      derived except the bench count, which the first draw settles.
+//   2. The derivation works from those draws and the crater's wall: WallTable samples the
+//      profile from the floor edge to the rim crest (monotone), maps a share of the wall's rise to
+//      a share of its width and back; the step bases split the rise by the drawn rises; each step
+//      gets the scarp share its slope needs and the bench level that keeps its area-weighted mean
+//      height (bisection).
+// The block is then written section by section: header, arc harmonics, rim scallop, scarps,
+// steps. Byte-equal data needs the draws in stream order and each FP64 expression evaluated as
+// the reference evaluates it before the narrowing to float; both are kept here.
 
 #include "prepare.h"
+#include "profile_laws.h"
 #include "random.h"
 
 #include <algorithm>
+#include <array>
 #include <cmath>
 #include <stdexcept>
 
@@ -25,204 +32,237 @@ namespace tgh {
 namespace {
 
 constexpr double kPi = 3.141592653589793238462643383279502884;
-constexpr double kDegree = kPi / 180.0;
 
-// The constants of the terrace preparation (crater_kernel.cpp:26-60).
-constexpr double kFullTerraceCover = 0.9;
-constexpr double kTerraceArcWidth = 0.3;
-constexpr int kTerraceWallSamples = 256;
-constexpr double kNarrowestTerraceStep = 0.25;
-constexpr double kTerraceScarpRounding = 0.4;
-constexpr double kMostTerraceScarpShare = 0.9;
-constexpr double kSteepTerraceStep = 0.7;
-constexpr double kTerraceScarpSqueeze = 0.75;
-constexpr int kTerraceMeanSamples = 64;
-constexpr int kTerraceLevelIterations = 30;
-constexpr double kTerraceLevelMargin = 0.1;
-constexpr double kTerraceScarpRoom = 0.4;
-constexpr double kTerraceScarpReach = 1.2;
-constexpr double kMinTerraceScallops = 6.0;
-constexpr double kMaxTerraceScallops = 12.0;
-constexpr double kTerraceRimScallop = 0.5;
+// Tuning of the terrace preparation. The values are the reference's (crater_kernel.cpp:26-60);
+// the kernel's reading of the block depends on them.
+namespace tune {
+constexpr double kFullCover = 0.9;          // arc share reached at the full diameter
+constexpr double kArcWidth = 0.3;           // written to the header for the kernel
+constexpr int kWallSamples = 256;           // the wall table's intervals
+constexpr double kNarrowestStep = 0.25;     // of the wall width, divided by the bench count
+constexpr double kScarpRounding = 0.4;      // the ease of a scarp's foot and crest
+constexpr double kMostScarpShare = 0.9;
+constexpr double kSteepStep = 0.7;          // above this scarp need a bench fades
+constexpr double kScarpSqueeze = 0.75;
+constexpr int kMeanSamples = 64;            // midpoint samples of a step's mean height
+constexpr int kLevelIterations = 30;
+constexpr double kLevelMargin = 0.1;        // bench levels stay inside [0.1, 0.9]
+constexpr double kScarpRoom = 0.4;          // of the narrower neighbouring step
+constexpr double kScarpReach = 1.2;         // of the room: scallop, wobble and block together
+constexpr double kFewestScallops = 6.0;
+constexpr double kMostScallops = 12.0;
+constexpr double kRimScallop = 0.5;
+constexpr int kArcHarmonics = 5;            // orders 2..6
+constexpr std::size_t kArcBearings = 360;   // bearings of the arc cut's quantile
+}  // namespace tune
 
-// The record fields the profile laws read (KernelStateRecord, profile.h:34-43).
-struct ShapeState {
-    float freshDepth, freshRim, currentDepth, currentRim;
-};
-
-// KernelEjectaScale (craters.h:309-311).
-float EjectaScale(float radius) {
-    return 0.14f * std::pow(radius, 0.74f);
-}
-
-struct Point {
-    float height, slope;
-};
-
-// MartinParkesDunstanExterior (craters.h:376-387).
-Point MpdExterior(float x, float rim, float ejectaScale) {
-    const float offset = rim - ejectaScale;
-    const float gradient = -ejectaScale / 8.0f - offset;
-    const float r = 1.0f / (x + 1.0f);
-    const float law = ejectaScale * r * r * r + offset + gradient * x;
-    const float lawSlope = -3.0f * ejectaScale * r * r * r * r + gradient;
-    const float taper = 1.0f - x * x;
-    return {law * taper, lawSlope * taper - 2.0f * x * law};
-}
-
-// MartinParkesDunstanHeight (craters.h:389-419).
-float MpdHeight(float distance, float diameter, const ShapeState& c) {
-    const float depth = c.currentDepth * diameter;
-    const float rim = c.currentRim * diameter;
-    const float freshRim = c.freshRim * diameter;
-    const float radius = 0.5f * diameter;
-    const float x = distance / radius - 1.0f;
-    if (x >= 1.0f) {
-        return 0.0f;
-    }
-    const float k = freshRim - rim + depth;
-    const float alpha = (rim - freshRim) / k;
-    if (x <= alpha) {
-        return (k * x + 2.0f * k) * x + freshRim;
-    }
-    if (x <= 0.0f) {
-        return k * (alpha + 1.0f) / alpha * x * x + rim;
-    }
-    const float ejectaScale =
-        freshRim > 0.0f ? EjectaScale(radius) * rim / freshRim : EjectaScale(radius);
-    const float beta = std::min(1.0f, 1.5f * (freshRim - rim) / k);
-    if (x <= beta) {
-        const Point join = MpdExterior(beta, rim, ejectaScale);
-        const float drop = rim - join.height;
-        const float slope = drop > 0.0f ? std::max(join.slope, -3.0f * drop / beta) : join.slope;
-        const float t = x / beta;
-        return rim + (join.height - rim) * t * t * (3.0f - 2.0f * t) +
-               beta * slope * t * t * (t - 1.0f);
-    }
-    return MpdExterior(x, rim, ejectaScale).height;
-}
-
-// FassettThomsonExterior and FassettThomsonShape (craters.h:330-362).
-float FtExterior(float x, float start) {
-    const float polynomial =
-        0.188253307f + x * (-0.187050452f + x * (0.01844746f + x * 0.01505647f));
-    const float u = (x - start) / (1.5f - start);
-    return polynomial * (1.0f - u * u);
-}
-
-float FtShape(float x, std::uint32_t rim, float freshRim, float currentDepth) {
-    if (x > 1.5f) {
-        return 0.0f;
-    }
-    const float interior =
-        -0.228809953f + x * (0.227533882f + x * (0.083116795f - x * 0.039499407f));
-    float h;
-    if (rim == 2u) {
-        if (x <= 0.98f) {
-            h = interior;
-        } else if (x <= 1.02f) {
-            h = freshRim;
-        } else {
-            h = FtExterior(x, 1.02f);
-        }
-    } else {
-        const float edge = rim == 0u ? 0.98f : 1.0f;
-        if (x <= 0.2f) {
-            h = -0.181f;
-        } else if (x <= edge) {
-            h = interior;
-        } else {
-            h = FtExterior(x, edge);
-        }
-    }
-    return std::max(h, freshRim - currentDepth);
-}
-
-// KilometerLaw (craters.h:432-435).
-float KilometerLaw(float coefficient, float exponent, float diameter, bool volume) {
-    const float kilometers = coefficient * std::pow(diameter / 1000.0f, exponent);
-    return volume ? kilometers * 1e9f : kilometers * 1000.0f;
-}
-
-// ComplexProfileHeight with KernelComplexPeak, KernelPeakDome and KernelKeptDepth
-// (craters.h:444-494).
-float ComplexHeight(const tg_crater_profile& p, float distance, float diameter,
-                    const ShapeState& c) {
-    const float radius = 0.5f * diameter;
-    const float x = distance / radius;
-    const float depth = c.currentDepth * diameter;
-    const float rim = c.currentRim * diameter;
-    if (x >= 1.0f) {
-        if (x >= p.complex_reach_radii) {
-            return 0.0f;
-        }
-        const float edge = std::pow(p.complex_reach_radii, -p.ejecta_exponent);
-        return rim * (std::pow(x, -p.ejecta_exponent) - edge) / (1.0f - edge);
-    }
-    const float floorShare = std::min(
-        0.9f, KilometerLaw(p.floor_coefficient, p.floor_exponent, diameter, false) / diameter);
-    float height = rim - depth;
-    if (x > floorShare) {
-        const float t = (x - floorShare) / (1.0f - floorShare);
-        height += depth * t * t;
-    }
-    // The fresh central peak's height and base, scaled past the laws' fitted range.
-    const float fitted =
-        p.peak_fit_max_diameter > 0.0f ? std::min(diameter, p.peak_fit_max_diameter) : diameter;
-    const float peakHeight =
-        KilometerLaw(p.peak_height_coefficient, p.peak_height_exponent, fitted, false);
-    float base;
-    if (p.peak_from_volume == 1u) {
-        const float volume =
-            KilometerLaw(p.peak_width_coefficient, p.peak_width_exponent, fitted, true);
-        base = std::sqrt(3.0f * volume / (3.14159265358979f * peakHeight));
-    } else {
-        base = 0.5f * KilometerLaw(p.peak_width_coefficient, p.peak_width_exponent, fitted, false);
-    }
-    base *= diameter / fitted;
-    float dome = 0.0f;
-    if (distance < base) {
-        const float u = 1.0f - (distance / base) * (distance / base);
-        dome = peakHeight * u * u;
-    }
-    const float kept = c.freshDepth > 0.0f ? c.currentDepth / c.freshDepth : 1.0f;
-    return height + kept * dome;
-}
-
-// KernelTerraceEase (craters.h:968-974).
-float TerraceEase(float v, float rounding) {
-    const float steepest = 1.0f / (1.0f - 0.5f * rounding);
-    if (v < rounding) {
-        return steepest * v * v / (2.0f * rounding);
-    }
-    return steepest * (v - 0.5f * rounding);
-}
-
-double SmoothstepD(double t) {
+double Clamp01Smooth(double t) {
     t = std::clamp(t, 0.0, 1.0);
     return t * t * (3.0 - 2.0 * t);
+}
+
+// The drawn part of one crater's terraces (phase 1).
+struct TerraceDraws {
+    int benches = 0;
+    std::array<double, tune::kArcHarmonics> arcPhase{};
+    double rimAmount = 0.0, rimCount = 0.0, rimPhase = 0.0;
+    std::vector<double> rises;  // one per step, each in [0.5, 1.5)
+    struct Scarp {
+        double amount, count, phase, wobble, wobblePhase;
+        std::uint32_t key;
+    };
+    std::vector<Scarp> scarps;  // one per inner scarp: benches - 1
+};
+
+// A scallop count: an integer from the fewest to the most, uniform.
+double ScallopCount(double u) {
+    return tune::kFewestScallops +
+           std::floor((tune::kMostScallops - tune::kFewestScallops + 1.0) * u);
+}
+
+TerraceDraws DrawTerraces(Stream& s, const Terraces& opts, double size) {
+    TerraceDraws d;
+    const double spread = opts.maxBenches - opts.minBenches;
+    const double jitter = s.NextDouble();
+    d.benches = std::clamp(
+        static_cast<int>(std::lround(opts.minBenches + spread * size + jitter - 0.5)),
+        opts.minBenches, opts.maxBenches);
+    for (double& phase : d.arcPhase) {
+        phase = 2.0 * kPi * s.NextDouble();
+    }
+    d.rimAmount = 0.6 + 0.8 * s.NextDouble();
+    d.rimCount = ScallopCount(s.NextDouble());
+    d.rimPhase = s.NextDouble();
+    for (int i = 0; i < d.benches; ++i) {
+        d.rises.push_back(0.5 + s.NextDouble());
+    }
+    for (int i = 1; i < d.benches; ++i) {
+        TerraceDraws::Scarp c{};
+        c.amount = 0.6 + 0.8 * s.NextDouble();
+        c.count = ScallopCount(s.NextDouble());
+        c.phase = s.NextDouble();
+        c.wobble = s.NextDouble();
+        c.wobblePhase = 2.0 * kPi * s.NextDouble();
+        c.key = s.NextU32() >> 8u;
+        d.scarps.push_back(c);
+    }
+    return d;
+}
+
+// The crater's wall from the floor edge (t = 0) to the rim crest (t = 1), sampled at
+// tune::kWallSamples + 1 points and made non-decreasing.
+class WallTable {
+  public:
+    template <class Height>
+    WallTable(double inner, double radius, Height&& height) : h_(tune::kWallSamples + 1) {
+        for (std::size_t n = 0; n < h_.size(); ++n) {
+            const double t = static_cast<double>(n) / tune::kWallSamples;
+            const double v = height((inner + t * (1.0 - inner)) * radius);
+            h_[n] = n == 0 ? v : std::max(v, h_[n - 1]);
+        }
+    }
+
+    // Height at the share t of the wall's width, linear between samples.
+    double At(double t) const {
+        const double at = std::clamp(t, 0.0, 1.0) * tune::kWallSamples;
+        const std::size_t n = std::min(static_cast<std::size_t>(at), h_.size() - 2);
+        return h_[n] + (at - static_cast<double>(n)) * (h_[n + 1] - h_[n]);
+    }
+
+    // The share of the wall's width at which it has risen `share` of its rise: the first sample
+    // not below the target, interpolated back to the crossing.
+    double WidthFor(double share) const {
+        const double rise = h_.back() - h_.front();
+        if (!(rise > 0.0)) {
+            return share;
+        }
+        const double target = h_.front() + share * rise;
+        std::size_t lo = 0, hi = h_.size();  // h_[lo..hi) holds the first sample >= target
+        while (lo < hi) {
+            const std::size_t mid = lo + (hi - lo) / 2;
+            if (h_[mid] < target) {
+                lo = mid + 1;
+            } else {
+                hi = mid;
+            }
+        }
+        if (lo == 0) {
+            return 0.0;
+        }
+        if (lo == h_.size()) {
+            return 1.0;
+        }
+        const double gap = h_[lo] - h_[lo - 1];
+        const double part = gap > 0.0 ? (h_[lo] - target) / gap : 0.0;
+        return (static_cast<double>(lo) - part) / tune::kWallSamples;
+    }
+
+  private:
+    std::vector<double> h_;
+};
+
+// One step of the wall, from base `from` to base `to` (shares of the wall's width), shaped as a
+// bench at `level` between eased scarps, tilted by the slope share.
+struct Step {
+    const WallTable& wall;
+    double inner, slopeShare, from, to;
+
+    // The stair's height share at `within` the step (0..1) for bench level and scarp share.
+    double Stair(double within, double level, double scarp) const {
+        const auto rounding = static_cast<float>(tune::kScarpRounding);
+        const double lower = scarp * level;
+        const double upper = scarp * (1.0 - level);
+        double stair = level;
+        if (within < lower) {
+            const auto v = static_cast<float>(1.0 - within / lower);
+            stair = level * (1.0 - double{tg::TerraceEase(v, rounding)});
+        } else if (within > 1.0 - upper) {
+            const auto v = static_cast<float>((within - (1.0 - upper)) / upper);
+            stair = level + (1.0 - level) * double{tg::TerraceEase(v, rounding)};
+        }
+        return slopeShare * within + (1.0 - slopeShare) * stair;
+    }
+
+    // Area-weighted (by radius) mean height over the step, stepped at `level`; the unstepped
+    // wall for a negative level.
+    double MeanHeight(double level, double scarp) const {
+        double weighted = 0.0;
+        double weights = 0.0;
+        for (int n = 0; n < tune::kMeanSamples; ++n) {
+            const double within = (n + 0.5) / tune::kMeanSamples;
+            const double r = inner + (from + within * (to - from)) * (1.0 - inner);
+            const double shaped = level < 0.0 ? within : Stair(within, level, scarp);
+            weighted += r * wall.At(from + shaped * (to - from));
+            weights += r;
+        }
+        return weighted / weights;
+    }
+
+    // The bench level in [margin, 1 - margin] whose stair keeps the wall's mean height.
+    double Level(double scarp) const {
+        const double target = MeanHeight(-1.0, scarp);
+        double low = tune::kLevelMargin;
+        double high = 1.0 - tune::kLevelMargin;
+        for (int i = 0; i < tune::kLevelIterations && high - low > 0.0; ++i) {
+            const double middle = 0.5 * (low + high);
+            if (MeanHeight(middle, scarp) < target) {
+                low = middle;
+            } else {
+                high = middle;
+            }
+        }
+        return 0.5 * (low + high);
+    }
+};
+
+// The arc cut: the value the arc harmonics' sum exceeds over a share `cover` of the bearings
+// (-2, below any sum, when every bearing is terraced).
+double ArcCut(const std::array<double, 2 * tune::kArcHarmonics>& arcs, double cover) {
+    if (!(cover < 1.0)) {
+        return -2.0;
+    }
+    std::vector<double> sums(tune::kArcBearings);
+    for (std::size_t n = 0; n < sums.size(); ++n) {
+        const double theta = 2.0 * kPi * static_cast<double>(n) / double{tune::kArcBearings};
+        double sum = 0.0;
+        for (std::size_t k = 0; k < tune::kArcHarmonics; ++k) {
+            sum += arcs[2 * k] * std::cos((static_cast<double>(k) + 2.0) * theta + arcs[2 * k + 1]);
+        }
+        sums[n] = sum;
+    }
+    std::sort(sums.begin(), sums.end());
+    return sums[static_cast<std::size_t>((1.0 - cover) * double{tune::kArcBearings - 1})];
+}
+
+void Put(std::vector<float>& data, std::initializer_list<double> values) {
+    for (const double v : values) {
+        data.push_back(static_cast<float>(v));
+    }
 }
 
 }  // namespace
 
 double ProfileHeight(const tg_crater_shape& k, double distance, double diameter,
                      double freshDepth, double freshRim, double currentDepth, double currentRim) {
-    const ShapeState s{static_cast<float>(freshDepth), static_cast<float>(freshRim),
-                       static_cast<float>(currentDepth), static_cast<float>(currentRim)};
-    const float d = static_cast<float>(distance);
-    const float D = static_cast<float>(diameter);
+    const auto D = static_cast<float>(diameter);
+    const tg::ProfileState s{D, static_cast<float>(freshDepth), static_cast<float>(freshRim),
+                             static_cast<float>(currentDepth), static_cast<float>(currentRim)};
+    const auto d = static_cast<float>(distance);
     const tg_crater_profile& p = k.profile;
+    // CraterProfile::Height of the closed-form profiles (complex.cpp:69-77,
+    // martin_parkes_dunstan.cpp:9-19, fassett_thomson.cpp:14-18): the kernel's FP32 laws.
     if (p.complex_ == 1u && diameter >= double(p.transition_diameter)) {
-        return static_cast<double>(ComplexHeight(p, d, D, s));
+        return static_cast<double>(tg::ComplexProfileHeight(p, d, D, s));
     }
     if (p.simple == TG_PROFILE_MPD) {
         if (!(currentDepth > 0.0 && currentRim >= 0.0 && currentRim <= freshRim)) {
             throw std::invalid_argument(
                 "Crater state needs a positive depth and 0 <= rim height <= fresh rim height");
         }
-        return static_cast<double>(MpdHeight(d, D, s));
+        return static_cast<double>(tg::MpdHeight(d, D, s));
     }
-    return static_cast<double>(FtShape(d / (0.5f * D), p.rim, s.freshRim, s.currentDepth) * D);
+    return static_cast<double>(
+        tg::FassettThomsonShape(d / (0.5f * D), p.rim, s.freshRim, s.currentDepth) * D);
 }
 
 void AppendTerraces(std::uint64_t id, double diameter, double freshDepth, double freshRim,
@@ -234,7 +274,7 @@ void AppendTerraces(std::uint64_t id, double diameter, double freshDepth, double
     }
     const double freshness = freshDepth > 0.0 ? currentDepth / freshDepth : 1.0;
     const double strength =
-        SmoothstepD((freshness - terraces.minFreshness) / (1.0 - terraces.minFreshness));
+        Clamp01Smooth((freshness - terraces.minFreshness) / (1.0 - terraces.minFreshness));
     if (!(strength > 0.0)) {
         return;
     }
@@ -242,213 +282,115 @@ void AppendTerraces(std::uint64_t id, double diameter, double freshDepth, double
         return ProfileHeight(kernel, distance, diameter, freshDepth, freshRim, currentDepth,
                              currentRim);
     };
-    // The floor edge: the complex shape's floor law, which for the lunar shape is the default
-    // (0.187 km D^1.249).
-    const tg_crater_profile& p = kernel.profile;
-    const float floorCoefficient = p.complex_ == 1u ? p.floor_coefficient : 0.187f;
-    const float floorExponent = p.complex_ == 1u ? p.floor_exponent : 1.249f;
-    const double floor = static_cast<double>(
-        KilometerLaw(floorCoefficient, floorExponent, static_cast<float>(diameter), false));
-    const double inner = std::clamp(floor / diameter, 0.2, 0.9);
     const double radius = 0.5 * diameter;
+    // The floor edge, as a share of the radius: the complex shape's floor law, or the lunar one
+    // (0.187 km D^1.249) for simple profiles.
+    const tg_crater_profile& p = kernel.profile;
+    const bool complex = p.complex_ == 1u;
+    const double floorWidth = static_cast<double>(
+        tg::KilometerLaw(complex ? p.floor_coefficient : 0.187f,
+                         complex ? p.floor_exponent : 1.249f, static_cast<float>(diameter),
+                         false));
+    const double inner = std::clamp(floorWidth / diameter, 0.2, 0.9);
+    // Slopes: the wall's mean, a bench's and a rounded scarp's; benches take a share of the mean.
     const double drop = height(radius) - height(inner * radius);
     const double meanSlope = drop / ((1.0 - inner) * radius);
-    const double benchSlope = std::tan(terraces.benchSlopeDegrees * kDegree);
-    const double scarpSlope = std::tan(terraces.scarpSlopeDegrees * kDegree) *
-                              (1.0 - 0.5 * kTerraceScarpRounding) * kTerraceScarpSqueeze;
+    const double degree = kPi / 180.0;
+    const double benchSlope = std::tan(terraces.benchSlopeDegrees * degree);
+    const double scarpSlope = std::tan(terraces.scarpSlopeDegrees * degree) *
+                              (1.0 - 0.5 * tune::kScarpRounding) * tune::kScarpSqueeze;
     const double slopeShare =
         meanSlope > 0.0 ? std::clamp(benchSlope / meanSlope, 0.02, 0.5) : 0.5;
     const double size = std::clamp((diameter - terraces.onsetDiameter) /
                                        (terraces.fullDiameter - terraces.onsetDiameter),
                                    0.0, 1.0);
-    Stream random(terraces.seed, kCraterTerraceStream, std::uint32_t(id),
+
+    // Phase 1: every draw of the crater's terrace stream.
+    Stream stream(terraces.seed, kCraterTerraceStream, std::uint32_t(id),
                   std::uint32_t(id >> 32));
-    const double spread = terraces.maxBenches - terraces.minBenches;
-    const int benches = std::clamp(
-        static_cast<int>(std::lround(terraces.minBenches + spread * size + random.NextDouble() -
-                                     0.5)),
-        terraces.minBenches, terraces.maxBenches);
-    // Five arc harmonics, orders 2-6, weights 1/k normalized, random phases.
-    double arcs[10];
-    {
-        double total = 0.0;
-        for (int k = 2; k < 7; ++k) {
-            total += 1.0 / k;
-        }
-        for (int k = 2; k < 7; ++k) {
-            arcs[2 * (k - 2)] = 1.0 / k / total;
-            arcs[2 * (k - 2) + 1] = 2.0 * kPi * random.NextDouble();
-        }
+    const TerraceDraws draws = DrawTerraces(stream, terraces, size);
+
+    // Phase 2: the block. Header and arc harmonics (weights 1/k normalized, k = 2..6).
+    std::array<double, 2 * tune::kArcHarmonics> arcs{};
+    double weightSum = 0.0;
+    for (int k = 2; k < 2 + tune::kArcHarmonics; ++k) {
+        weightSum += 1.0 / k;
+    }
+    for (int k = 2; k < 2 + tune::kArcHarmonics; ++k) {
+        arcs[2 * (k - 2)] = 1.0 / k / weightSum;
+        arcs[2 * (k - 2) + 1] = draws.arcPhase[k - 2];
     }
     const double cover =
         terraces.arcShare >= 1.0
             ? 1.0
-            : terraces.arcShare + std::max(0.0, kFullTerraceCover - terraces.arcShare) * size;
-    double cut = -2.0;
-    if (cover < 1.0) {
-        // The (1 - cover) quantile of the arc sum over 360 bearings.
-        constexpr std::size_t kSamples = 360;
-        std::vector<double> values(kSamples);
-        for (std::size_t n = 0; n < kSamples; ++n) {
-            const double theta = 2.0 * kPi * static_cast<double>(n) / double{kSamples};
-            double sum = 0.0;
-            for (std::size_t k = 0; k < 5; ++k) {
-                sum += arcs[2 * k] *
-                       std::cos((static_cast<double>(k) + 2.0) * theta + arcs[2 * k + 1]);
-            }
-            values[n] = sum;
-        }
-        const std::size_t at = static_cast<std::size_t>((1.0 - cover) * double{kSamples - 1});
-        std::nth_element(values.begin(), values.begin() + std::ptrdiff_t(at), values.end());
-        cut = values[at];
-    }
-    record.terrace_count = static_cast<std::uint32_t>(benches);
+            : terraces.arcShare + std::max(0.0, tune::kFullCover - terraces.arcShare) * size;
+    record.terrace_count = static_cast<std::uint32_t>(draws.benches);
     record.terrace_offset = static_cast<std::uint32_t>(data.size());
-    for (const double v : {inner, slopeShare, strength, kTerraceScarpRounding, cut,
-                           kTerraceArcWidth}) {
-        data.push_back(static_cast<float>(v));
-    }
+    Put(data, {inner, slopeShare, strength, tune::kScarpRounding, ArcCut(arcs, cover),
+               tune::kArcWidth});
     for (const double v : arcs) {
         data.push_back(static_cast<float>(v));
     }
-    const auto scallopCount = [&random] {
-        return kMinTerraceScallops +
-               std::floor((kMaxTerraceScallops - kMinTerraceScallops + 1.0) * random.NextDouble());
-    };
-    const double rimScallop = kTerraceRimScallop * terraces.rimVariation * strength *
-                              (0.6 + 0.8 * random.NextDouble());
-    // Drawn in the order written: the scallop count, then the phase.
-    const double rimScallops = scallopCount();
-    const double rimPhase = random.NextDouble();
-    for (const double v : {rimScallop, rimScallops, rimPhase, terraces.blockGaps}) {
-        data.push_back(static_cast<float>(v));
+    // The rim crest's scallops.
+    Put(data, {tune::kRimScallop * terraces.rimVariation * strength * draws.rimAmount,
+               draws.rimCount, draws.rimPhase, terraces.blockGaps});
+
+    // Step bases: the wall's rise split by the drawn rises, mapped to widths and kept at least
+    // the narrowest step apart (forward, then backward).
+    const WallTable wall(inner, radius, height);
+    std::vector<double> bases{wall.WidthFor(0.0)};
+    double risen = 0.0;
+    for (const double r : draws.rises) {
+        risen += r;
     }
-    // The wall's height from the floor edge to the rim crest, made non-decreasing.
-    std::vector<double> wall(kTerraceWallSamples + 1);
-    for (std::size_t n = 0; n < wall.size(); ++n) {
-        const double t = static_cast<double>(n) / kTerraceWallSamples;
-        const double h = height((inner + t * (1.0 - inner)) * radius);
-        wall[n] = n == 0 ? h : std::max(h, wall[n - 1]);
-    }
-    const double rise = wall.back() - wall.front();
-    // The share of the wall's width below which it reaches `share` of its rise.
-    const auto across = [&](double share) {
-        if (!(rise > 0.0)) {
-            return share;
+    {
+        double cumulative = 0.0;
+        for (const double r : draws.rises) {
+            cumulative += r;
+            bases.push_back(wall.WidthFor(cumulative / risen));
         }
-        const double target = wall.front() + share * rise;
-        const auto above = std::lower_bound(wall.begin(), wall.end(), target);
-        if (above == wall.begin()) {
-            return 0.0;
-        }
-        if (above == wall.end()) {
-            return 1.0;
-        }
-        const auto n = static_cast<double>(above - wall.begin());
-        const double gap = *above - *(above - 1);
-        const double part = gap > 0.0 ? (*above - target) / gap : 0.0;
-        return (n - part) / kTerraceWallSamples;
-    };
-    // Step heights: random rises, normalized; their bases across the wall, kept apart.
-    std::vector<double> heights{0.0};
-    double total = 0.0;
-    for (int i = 0; i < benches; ++i) {
-        total += 0.5 + random.NextDouble();
-        heights.push_back(total);
-    }
-    std::vector<double> bases;
-    for (double& h : heights) {
-        h /= total;
-        bases.push_back(across(h));
     }
     bases.back() = 1.0;
-    const double narrowest = kNarrowestTerraceStep / benches;
+    const double narrowest = tune::kNarrowestStep / draws.benches;
     for (std::size_t k = 1; k + 1 < bases.size(); ++k) {
         bases[k] = std::max(bases[k], bases[k - 1] + narrowest);
     }
     for (std::size_t k = bases.size() - 2; k > 0; --k) {
         bases[k] = std::min(bases[k], bases[k + 1] - narrowest);
     }
-    // Each inner scarp: base, scallops, wobble and block offset within its room.
+
+    // Inner scarps: base, scallops, wobble and block offset, scaled together to fit the room
+    // the neighbouring steps leave.
     const double variation = terraces.rimVariation / (1.0 - inner);
-    for (std::size_t k = 1; k < bases.size() - 1; ++k) {
+    for (std::size_t k = 1; k + 1 < bases.size(); ++k) {
+        const TerraceDraws::Scarp& c = draws.scarps[k - 1];
         const double room =
-            kTerraceScarpRoom * std::min(bases[k] - bases[k - 1], bases[k + 1] - bases[k]);
-        double scallop = std::min(variation * (0.6 + 0.8 * random.NextDouble()), 0.9 * room);
-        const double scallops = scallopCount();
-        const double phase = random.NextDouble();
-        double wobble = std::min(0.5 * variation * random.NextDouble(), 0.4 * room);
-        const double wobblePhase = 2.0 * kPi * random.NextDouble();
+            tune::kScarpRoom * std::min(bases[k] - bases[k - 1], bases[k + 1] - bases[k]);
+        double scallop = std::min(variation * c.amount, 0.9 * room);
+        double wobble = std::min(0.5 * variation * c.wobble, 0.4 * room);
         double block = terraces.blockOffset * room;
         const double reach = 2.0 / 3.0 * scallop + wobble + block;
-        if (reach > kTerraceScarpReach * room) {
-            const double fit = kTerraceScarpReach * room / reach;
+        if (reach > tune::kScarpReach * room) {
+            const double fit = tune::kScarpReach * room / reach;
             scallop *= fit;
             wobble *= fit;
             block *= fit;
         }
-        const double key = static_cast<double>(random.NextU32() >> 8u);
-        for (const double v : {bases[k], scallop, scallops, phase, wobble, wobblePhase, block, key}) {
-            data.push_back(static_cast<float>(v));
-        }
+        Put(data, {bases[k], scallop, c.count, c.phase, wobble, c.wobblePhase, block,
+                   static_cast<double>(c.key)});
     }
-    const auto wallAt = [&](double t) {
-        const double at = std::clamp(t, 0.0, 1.0) * kTerraceWallSamples;
-        const auto n = std::min(static_cast<std::size_t>(at), wall.size() - 2);
-        return wall[n] + (at - static_cast<double>(n)) * (wall[n + 1] - wall[n]);
-    };
-    // A step's stair at `within` its width: the bench at `level`, eased into its scarps.
-    const auto stairAt = [&](double within, double level, double scarp) {
-        const auto rounding = static_cast<float>(kTerraceScarpRounding);
-        const double lower = scarp * level;
-        const double upper = scarp * (1.0 - level);
-        double stair = level;
-        if (within < lower) {
-            stair = level * (1.0 - double{TerraceEase(static_cast<float>(1.0 - within / lower),
-                                                      rounding)});
-        } else if (within > 1.0 - upper) {
-            stair = level + (1.0 - level) *
-                                double{TerraceEase(
-                                    static_cast<float>((within - (1.0 - upper)) / upper),
-                                    rounding)};
-        }
-        return slopeShare * within + (1.0 - slopeShare) * stair;
-    };
-    // The area-weighted mean height of a step, stepped at `level` (unstepped below 0).
-    const auto meanHeight = [&](double from, double to, double level, double scarp) {
-        double weighted = 0.0;
-        double weights = 0.0;
-        for (int n = 0; n < kTerraceMeanSamples; ++n) {
-            const double within = (n + 0.5) / kTerraceMeanSamples;
-            const double r = inner + (from + within * (to - from)) * (1.0 - inner);
-            const double shaped = level < 0.0 ? within : stairAt(within, level, scarp);
-            weighted += r * wallAt(from + shaped * (to - from));
-            weights += r;
-        }
-        return weighted / weights;
-    };
-    // Each step: the scarp share its slope needs, the share of its bench kept, and the bench
-    // level that keeps its mean height, by bisection.
+
+    // Steps: the scarp share the step's slope needs, the share of its bench kept, its level.
     for (std::size_t k = 0; k + 1 < bases.size(); ++k) {
-        const double from = bases[k];
-        const double to = bases[k + 1];
-        const double stepSlope =
-            (wallAt(to) - wallAt(from)) / ((to - from) * (1.0 - inner) * radius);
+        const Step step{wall, inner, slopeShare, bases[k], bases[k + 1]};
+        const double slope = (wall.At(step.to) - wall.At(step.from)) /
+                             ((step.to - step.from) * (1.0 - inner) * radius);
         const double needed =
-            scarpSlope > benchSlope ? (stepSlope - benchSlope) / (scarpSlope - benchSlope) : 1.0;
-        const double scarp = std::clamp(needed, terraces.scarpShare, kMostTerraceScarpShare);
+            scarpSlope > benchSlope ? (slope - benchSlope) / (scarpSlope - benchSlope) : 1.0;
+        const double scarp = std::clamp(needed, terraces.scarpShare, tune::kMostScarpShare);
         const double kept =
-            1.0 - SmoothstepD((needed - kSteepTerraceStep) / (1.0 - kSteepTerraceStep));
-        const double target = meanHeight(from, to, -1.0, scarp);
-        double low = kTerraceLevelMargin;
-        double high = 1.0 - kTerraceLevelMargin;
-        for (int i = 0; i < kTerraceLevelIterations && high - low > 0.0; ++i) {
-            const double middle = 0.5 * (low + high);
-            (!(meanHeight(from, to, middle, scarp) < target) ? high : low) = middle;
-        }
-        for (const double v : {0.5 * (low + high), scarp, kept}) {
-            data.push_back(static_cast<float>(v));
-        }
+            1.0 - Clamp01Smooth((needed - tune::kSteepStep) / (1.0 - tune::kSteepStep));
+        Put(data, {step.Level(scarp), scarp, kept});
     }
 }
 
