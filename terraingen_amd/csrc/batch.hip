@@ -32,8 +32,10 @@ struct tg_catalogue {
     tg::HotCrater* hot = nullptr;  // the fast path's record per crater
     float* hotHarm = nullptr;      // its first tg::kHotHarmonics outline harmonics
     tg_crater_frame* frames = nullptr;  // MakeCraterFrame per crater, for tiles on a sphere
+    tg::PreciseTerms* precise = nullptr;  // FP64 terms of km-scale craters (and frame axes)
     size_t count = 0;
     bool allFast = false;  // every crater takes the hoisted MPD path
+    bool anyPrecise = false;  // some crater takes it in FP64 (tg::kFastPrecise)
     // Changes whenever what a launch reads of the catalogue changes (creation, frames).
     uint64_t generation = tgi::NextGeneration();
 };
@@ -145,30 +147,73 @@ __device__ __forceinline__ const float* Coefficients(const tg::HotCrater& c, uin
                : harm + c.harmonic_offset;
 }
 
-// Crater n at the cell (x + xLow, y + yLow) of height h, as the strip kernel applies it: nothing
-// beyond the crater's cull radius (or for a lane whose cell lies past the tile), else the hoisted
-// MPD evaluation where the crater allows it and the full one otherwise.
-template <bool kAllFast>
-__device__ __forceinline__ float VisitCrater(uint32_t n, bool inside, float h, float x, float xLow,
-                                             float y, float yLow, double pixel, bool hoisted,
-                                             const tg::HotCrater* hot, const float* hotHarm,
-                                             const tg_crater_shape* kernel, const float* data,
-                                             const tg_crater_record* records, const float* harm,
-                                             const float4* outline) {
+// A km-scale crater's FP64 contribution at a plane cell, before it joins the cell's precise sum.
+__device__ __forceinline__ double PlanePrecise(uint32_t n, float x, float xLow, float y,
+                                               float yLow, double pixel, const tg::HotCrater* hot,
+                                               const float* hotHarm, const float* harm,
+                                               const tg::PreciseTerms* precise) {
     const tg::HotCrater& c = hot[n];
+    const tg::Offset64 o = tg::PlaneOffset64(precise[n], x, xLow, y, yLow);
+    if (!(o.distance * o.distance < double(c.cull))) {
+        return 0.0;
+    }
+    return double(ResolutionAt(c.diameter, pixel)) *
+           tg::MpdPrecise(precise[n], c.ellipticity, c.first_harmonic, c.harmonic_count,
+                          Coefficients(c, n, hotHarm, harm), o);
+}
+
+// The same at a body-fixed cell point, the offset taken in the crater's frame.
+__device__ __forceinline__ double BodyPrecise(uint32_t n, const tg_split_point& p, double spacing,
+                                              const tg::HotCrater* hot, const float* hotHarm,
+                                              const float* harm,
+                                              const tg::PreciseTerms* precise) {
+    const tg::HotCrater& c = hot[n];
+    const tg::Offset64 o = tg::BodyOffset64(precise[n], p);
+    if (!(o.distance * o.distance < double(c.cull))) {
+        return 0.0;
+    }
+    return double(ResolutionAt(c.diameter, spacing)) *
+           tg::MpdPrecise(precise[n], c.ellipticity, c.first_harmonic, c.harmonic_count,
+                          Coefficients(c, n, hotHarm, harm), o);
+}
+
+// Crater n at the cell (x + xLow, y + yLow), as the strip kernel applies it: nothing beyond the
+// crater's cull radius (or for a lane whose cell lies past the tile); a km-scale crater's
+// contribution in FP64 into the cell's precise sum; else the hoisted MPD evaluation where the
+// crater allows it and the full one, after folding the precise sum in, otherwise.
+template <bool kAllFast, bool kPrecise>
+__device__ __forceinline__ void VisitCrater(uint32_t n, bool inside, tg::PreciseSum& cell, float x,
+                                            float xLow, float y, float yLow, double pixel,
+                                            bool hoisted, const tg::HotCrater* hot,
+                                            const float* hotHarm, const tg_crater_shape* kernel,
+                                            const float* data, const tg_crater_record* records,
+                                            const float* harm, const float4* outline,
+                                            const tg::PreciseTerms* precise) {
+    const tg::HotCrater& c = hot[n];
+    if constexpr (kPrecise) {
+        if (c.terms.fast == tg::kFastPrecise) {
+            if (inside) {
+                cell.pending +=
+                    PlanePrecise(n, x, xLow, y, yLow, pixel, hot, hotHarm, harm, precise);
+            }
+            return;
+        }
+    }
     const float dx = tg::SplitDifference(x, xLow, c.x, c.x_low);
     const float dy = tg::SplitDifference(y, yLow, c.y, c.y_low);
     if (inside && dx * dx + dy * dy < c.cull) {
         const float weight = ResolutionAt(c.diameter, pixel);
         if (kAllFast || (hoisted && c.terms.fast != 0u)) {
-            return tg::ApplyMpd(c.terms, c.ellipticity, c.first_harmonic, c.harmonic_count,
-                                Coefficients(c, n, hotHarm, harm), c.ra, h, dx, dy, weight);
+            cell.h = tg::ApplyMpd(c.terms, c.ellipticity, c.first_harmonic, c.harmonic_count,
+                                  Coefficients(c, n, hotHarm, harm), c.ra, cell.h, dx, dy, weight);
         } else if constexpr (!kAllFast) {
             const tg::CraterShape shape(*kernel);
-            return tg::ApplyCrater(shape, data, records[n], h, dx, dy, harm, outline[n], weight);
+            // A crater outside the hoisted path may read the height: the precise sum joins it
+            // first. A hoisted one only adds, as its full evaluation here does.
+            const float h = c.terms.fast == 0u ? cell.Folded() : cell.h;
+            cell.h = tg::ApplyCrater(shape, data, records[n], h, dx, dy, harm, outline[n], weight);
         }
     }
-    return h;
 }
 
 // Takes bins off the queues until every range is empty, calling `bin(b)` for each bin b taken:
@@ -208,8 +253,10 @@ __device__ __forceinline__ void DrainBins(uint32_t* counters, uint32_t chunk, F&
 // strip kernel uses, so its crater-reach tests stay coherent across lanes.
 // kAllFast: every crater of the catalogue takes the hoisted MPD path, so the full evaluation is
 // not compiled in; the kernel then holds far fewer registers and keeps more waves resident.
-template <uint32_t kWalk, bool kAllFast>
-__global__ void __launch_bounds__(kGroupThreads, kAllFast ? 8 : 1)
+// kPrecise: some crater takes that path in FP64 (a catalogue without km-scale craters runs the
+// variant without the FP64 code, whose registers cost the other variants residency).
+template <uint32_t kWalk, bool kAllFast, bool kPrecise>
+__global__ void __launch_bounds__(kGroupThreads, kAllFast ? (kPrecise ? 6 : 8) : 1)
     BatchTiles(const tg_tile_desc* __restrict__ tiles, const tg_bin* __restrict__ bins,
                const uint32_t* __restrict__ entries, uint32_t* __restrict__ counters,
                const tg::NoisePlan* __restrict__ plan, float elevation,
@@ -217,7 +264,8 @@ __global__ void __launch_bounds__(kGroupThreads, kAllFast ? 8 : 1)
                const tg_crater_record* __restrict__ records, const float* __restrict__ data,
                const float* __restrict__ harm, const float4* __restrict__ outline,
                const tg::HotCrater* __restrict__ hot, const float* __restrict__ hotHarm,
-               uint32_t hoisted, uint32_t chunk, float* __restrict__ out) {
+               const tg::PreciseTerms* __restrict__ precise, uint32_t hoisted, uint32_t chunk,
+               float* __restrict__ out) {
     const uint32_t lane = threadIdx.x & 63u;
     DrainBins(counters, chunk, [&](uint32_t b) {
         const tg_bin bin = bins[b];
@@ -246,16 +294,17 @@ __global__ void __launch_bounds__(kGroupThreads, kAllFast ? 8 : 1)
             const double cy = tile.origin_y + (double(cellRow) + 0.5) * -tile.pixel;
             const float x = float(cx);
             const float y = float(cy);
-            float h = plan != nullptr ? elevation + tg::EvaluateStack<true>(*plan, x, y, lane)
-                                      : elevation;
+            tg::PreciseSum cell{plan != nullptr
+                                    ? elevation + tg::EvaluateStack<true>(*plan, x, y, lane)
+                                    : elevation};
             if (kernel != nullptr && bin.count > 0u) {
                 const float xLow = float(cx - double(x));
                 const float yLow = float(cy - double(y));
                 const uint32_t end = bin.first + bin.count;
                 const auto visit = [&](uint32_t n) {
-                    h = VisitCrater<kAllFast>(n, inside, h, x, xLow, y, yLow, tile.pixel,
-                                              hoisted != 0u, hot, hotHarm, kernel, data, records,
-                                              harm, outline);
+                    VisitCrater<kAllFast, kPrecise>(n, inside, cell, x, xLow, y, yLow, tile.pixel,
+                                          hoisted != 0u, hot, hotHarm, kernel, data, records, harm,
+                                          outline, precise);
                 };
                 if constexpr (kWalk == kWalkPerLane) {
                     uint32_t e = bin.first;
@@ -320,7 +369,7 @@ __global__ void __launch_bounds__(kGroupThreads, kAllFast ? 8 : 1)
                 }
             }
             if (inside) {
-                out[tile.out_offset + size_t(row) * tile.width + col] = h;
+                out[tile.out_offset + size_t(row) * tile.width + col] = cell.Folded();
             }
         }
     });
@@ -336,32 +385,42 @@ __device__ __forceinline__ double WaveMax(double v) {
 
 // Crater n at the body-fixed cell point p, as the strip kernel applies it on a body: the offset in
 // the crater's frame (BodySplitOffset), nothing beyond the crater's cull radius (or for a lane
-// whose cell lies past the tile), else the hoisted MPD evaluation where the crater allows it and
-// the full one otherwise, at the tile's resolution weight.
-template <bool kAllFast>
-__device__ __forceinline__ float VisitBodyCrater(uint32_t n, bool inside, float h,
-                                                 const tg_split_point& p, double spacing,
-                                                 bool hoisted, const tg::HotCrater* hot,
-                                                 const float* hotHarm,
-                                                 const tg_crater_frame* frames,
-                                                 const tg_crater_shape* kernel, const float* data,
-                                                 const tg_crater_record* records,
-                                                 const float* harm, const float4* outline) {
+// whose cell lies past the tile); a km-scale crater's contribution in FP64 (offset included) into
+// the cell's precise sum; else the hoisted MPD evaluation where the crater allows it and the full
+// one otherwise, at the tile's resolution weight.
+template <bool kAllFast, bool kPrecise>
+__device__ __forceinline__ void VisitBodyCrater(uint32_t n, bool inside, tg::PreciseSum& cell,
+                                                const tg_split_point& p, double spacing,
+                                                bool hoisted, const tg::HotCrater* hot,
+                                                const float* hotHarm,
+                                                const tg_crater_frame* frames,
+                                                const tg_crater_shape* kernel, const float* data,
+                                                const tg_crater_record* records,
+                                                const float* harm, const float4* outline,
+                                                const tg::PreciseTerms* precise) {
     const tg::HotCrater& c = hot[n];
+    if constexpr (kPrecise) {
+        if (c.terms.fast == tg::kFastPrecise) {
+            if (inside) {
+                cell.pending += BodyPrecise(n, p, spacing, hot, hotHarm, harm, precise);
+            }
+            return;
+        }
+    }
     const tg::Offset at = tg::BodySplitOffset(frames[n], p);
     if (inside && at.dx * at.dx + at.dy * at.dy < c.cull) {
         const float weight = ResolutionAt(c.diameter, spacing);
         if (kAllFast || (hoisted && c.terms.fast != 0u)) {
-            return tg::ApplyMpd(c.terms, c.ellipticity, c.first_harmonic, c.harmonic_count,
-                                Coefficients(c, n, hotHarm, harm), c.ra, h, at.dx, at.dy,
-                                weight);
+            cell.h = tg::ApplyMpd(c.terms, c.ellipticity, c.first_harmonic, c.harmonic_count,
+                                  Coefficients(c, n, hotHarm, harm), c.ra, cell.h, at.dx, at.dy,
+                                  weight);
         } else if constexpr (!kAllFast) {
             const tg::CraterShape shape(*kernel);
-            return tg::ApplyCrater(shape, data, records[n], h, at.dx, at.dy, harm, outline[n],
-                                   weight);
+            const float h = c.terms.fast == 0u ? cell.Folded() : cell.h;
+            cell.h = tg::ApplyCrater(shape, data, records[n], h, at.dx, at.dy, harm, outline[n],
+                                     weight);
         }
     }
-    return h;
 }
 
 // BatchTiles for longitude-latitude tiles on a sphere of `radius` meters (a tile's origin and
@@ -378,8 +437,9 @@ __device__ __forceinline__ float VisitBodyCrater(uint32_t n, bool inside, float 
 // host from the radius and the plan's highest frequency).
 // kGeneral: the plan holds warped fractals or bases other than Perlin (one variant, with the
 // general crater walk and 64-bit lattice indices, carries their code).
-template <bool kAllFast, bool kSmall, bool kGeneral = false>
-__global__ void __launch_bounds__(kGroupThreads, kAllFast ? 8 : 1)
+// kPrecise: as for BatchTiles.
+template <bool kAllFast, bool kSmall, bool kGeneral, bool kPrecise>
+__global__ void __launch_bounds__(kGroupThreads, kAllFast ? (kPrecise ? 6 : 8) : 1)
     BatchBodyTiles(const tg_tile_desc* __restrict__ tiles, const tg_bin* __restrict__ bins,
                    const uint32_t* __restrict__ entries, uint32_t* __restrict__ counters,
                    const tg::NoisePlan* __restrict__ plan, float elevation, double radius,
@@ -387,7 +447,8 @@ __global__ void __launch_bounds__(kGroupThreads, kAllFast ? 8 : 1)
                    const tg_crater_record* __restrict__ records, const float* __restrict__ data,
                    const float* __restrict__ harm, const float4* __restrict__ outline,
                    const tg::HotCrater* __restrict__ hot, const float* __restrict__ hotHarm,
-                   const tg_crater_frame* __restrict__ frames, uint32_t hoisted, uint32_t chunk,
+                   const tg_crater_frame* __restrict__ frames,
+                   const tg::PreciseTerms* __restrict__ precise, uint32_t hoisted, uint32_t chunk,
                    float* __restrict__ out) {
     const uint32_t lane = threadIdx.x & 63u;
     const tg::CubeSteps steps = tg::MakeCubeSteps(lane);
@@ -427,9 +488,10 @@ __global__ void __launch_bounds__(kGroupThreads, kAllFast ? 8 : 1)
             const tg::SinCos lat{__shfl(mine.sin, int(8u + (lane >> 3u)), 64),
                                  __shfl(mine.cos, int(8u + (lane >> 3u)), 64)};
             const tg::Position at = tg::SpherePositionOf(lon, lat, radius);
-            float h = plan != nullptr ? elevation + tg::EvaluateStack3<true, kSmall, kGeneral>(
-                                                        *plan, at.x, at.y, at.z, lane, steps)
-                                      : elevation;
+            tg::PreciseSum cell{plan != nullptr
+                                    ? elevation + tg::EvaluateStack3<true, kSmall, kGeneral>(
+                                                      *plan, at.x, at.y, at.z, lane, steps)
+                                    : elevation};
             if (kernel != nullptr && bin.count > 0u) {
                 tg_split_point p;
                 p.x = float(at.x);
@@ -440,9 +502,9 @@ __global__ void __launch_bounds__(kGroupThreads, kAllFast ? 8 : 1)
                 p.z_low = float(at.z - double(p.z));
                 const uint32_t end = bin.first + bin.count;
                 const auto visit = [&](uint32_t n) {
-                    h = VisitBodyCrater<kAllFast>(n, inside, h, p, spacing, hoisted != 0u, hot,
-                                                  hotHarm, frames, kernel, data, records, harm,
-                                                  outline);
+                    VisitBodyCrater<kAllFast, kPrecise>(n, inside, cell, p, spacing, hoisted != 0u, hot,
+                                              hotHarm, frames, kernel, data, records, harm,
+                                              outline, precise);
                 };
                 if (bin.count <= kShortList) {
                     for (uint32_t e = bin.first; e < end; ++e) {
@@ -484,7 +546,7 @@ __global__ void __launch_bounds__(kGroupThreads, kAllFast ? 8 : 1)
                 }
             }
             if (inside) {
-                out[tile.out_offset + size_t(row) * tile.width + col] = h;
+                out[tile.out_offset + size_t(row) * tile.width + col] = cell.Folded();
             }
         }
     });
@@ -554,7 +616,7 @@ void FreeCatalogue(tg_catalogue* c) {
                     static_cast<void*>(c->harm), static_cast<void*>(c->outline),
                     static_cast<void*>(c->diameters), static_cast<void*>(c->terms),
                     static_cast<void*>(c->hot), static_cast<void*>(c->hotHarm),
-                    static_cast<void*>(c->frames)}) {
+                    static_cast<void*>(c->frames), static_cast<void*>(c->precise)}) {
         if (p != nullptr) {
             (void)hipFree(p);
         }
@@ -580,6 +642,37 @@ uint32_t ChunkFor(const tg_batch& b, uint32_t groups) {
     return uint32_t(std::clamp<uint64_t>(b.binCount / (waves * 4u), 1u, b.chunk));
 }
 
+using PlaneKernelFn = decltype(&BatchTiles<kWalkCulled, true, false>);
+using BodyKernelFn = decltype(&BatchBodyTiles<true, true, false, false>);
+
+template <uint32_t kWalk>
+PlaneKernelFn PlaneKernelOf(bool allFast, bool precise) {
+    return allFast ? (precise ? BatchTiles<kWalk, true, true> : BatchTiles<kWalk, true, false>)
+                   : (precise ? BatchTiles<kWalk, false, true> : BatchTiles<kWalk, false, false>);
+}
+
+PlaneKernelFn PlaneKernel(uint32_t walk, bool allFast, bool precise) {
+    return walk == TG_WALK_PER_LANE ? PlaneKernelOf<kWalkPerLane>(allFast, precise)
+           : walk == TG_WALK_CULLED ? PlaneKernelOf<kWalkCulled>(allFast, precise)
+                                    : PlaneKernelOf<kWalkTogether>(allFast, precise);
+}
+
+template <bool kPrecise>
+BodyKernelFn BodyKernelOf(bool allFast, bool small, bool general) {
+    if (general) {  // other bases or warped fractals: one variant carries their code
+        return BatchBodyTiles<false, false, true, kPrecise>;
+    }
+    return allFast ? (small ? BatchBodyTiles<true, true, false, kPrecise>
+                            : BatchBodyTiles<true, false, false, kPrecise>)
+                   : (small ? BatchBodyTiles<false, true, false, kPrecise>
+                            : BatchBodyTiles<false, false, false, kPrecise>);
+}
+
+BodyKernelFn BodyKernel(bool allFast, bool small, bool general, bool precise) {
+    return precise ? BodyKernelOf<true>(allFast, small, general)
+                   : BodyKernelOf<false>(allFast, small, general);
+}
+
 void Launch(tg_batch& b, const tg_noise_plan* plan, float elevation, const tg_catalogue* cat,
             float* out) {
     const uint32_t groups = GroupsFor(b);
@@ -593,11 +686,9 @@ void Launch(tg_batch& b, const tg_noise_plan* plan, float elevation, const tg_ca
         // Cell positions lie on the sphere of the batch's radius.
         const bool small =
             plan == nullptr || tg::BodyLatticeBound(plan->host, b.radius) < 2147483648.0;
-        auto* body = allFast ? (small ? BatchBodyTiles<true, true> : BatchBodyTiles<true, false>)
-                             : (small ? BatchBodyTiles<false, true> : BatchBodyTiles<false, false>);
-        if (plan != nullptr && !plan->host.bodyPerlin) {
-            body = BatchBodyTiles<false, false, true>;
-        }
+        const bool precise = cat != nullptr && cat->anyPrecise;
+        auto* body = BodyKernel(allFast, small, plan != nullptr && !plan->host.bodyPerlin,
+                                precise);
         hipLaunchKernelGGL(body, dim3(groups), dim3(kGroupThreads), 0, b.stream, b.tiles, b.bins,
                            b.entries, b.counters, plan != nullptr ? plan->device : nullptr,
                            elevation, b.radius, unitLength,
@@ -608,17 +699,13 @@ void Launch(tg_batch& b, const tg_noise_plan* plan, float elevation, const tg_ca
                            cat != nullptr ? cat->outline : nullptr,
                            cat != nullptr ? cat->hot : nullptr,
                            cat != nullptr ? cat->hotHarm : nullptr,
-                           cat != nullptr ? cat->frames : nullptr, b.hoisted ? 1u : 0u, chunk,
+                           cat != nullptr ? cat->frames : nullptr,
+                           cat != nullptr ? cat->precise : nullptr, b.hoisted ? 1u : 0u, chunk,
                            out);
         Check(hipGetLastError(), "launching the batched body tile kernel");
         return;
     }
-    auto* fn = b.walk == TG_WALK_PER_LANE
-                   ? (allFast ? BatchTiles<kWalkPerLane, true> : BatchTiles<kWalkPerLane, false>)
-               : b.walk == TG_WALK_CULLED
-                   ? (allFast ? BatchTiles<kWalkCulled, true> : BatchTiles<kWalkCulled, false>)
-                   : (allFast ? BatchTiles<kWalkTogether, true>
-                              : BatchTiles<kWalkTogether, false>);
+    auto* fn = PlaneKernel(b.walk, allFast, cat != nullptr && cat->anyPrecise);
     hipLaunchKernelGGL(fn, dim3(groups), dim3(kGroupThreads), 0, b.stream, b.tiles, b.bins,
                        b.entries, b.counters, plan != nullptr ? plan->device : nullptr,
                        elevation, cat != nullptr ? cat->kernel : nullptr,
@@ -626,7 +713,8 @@ void Launch(tg_batch& b, const tg_noise_plan* plan, float elevation, const tg_ca
                        cat != nullptr ? cat->data : nullptr, cat != nullptr ? cat->harm : nullptr,
                        cat != nullptr ? cat->outline : nullptr,
                        cat != nullptr ? cat->hot : nullptr,
-                       cat != nullptr ? cat->hotHarm : nullptr, b.hoisted ? 1u : 0u, chunk, out);
+                       cat != nullptr ? cat->hotHarm : nullptr,
+                       cat != nullptr ? cat->precise : nullptr, b.hoisted ? 1u : 0u, chunk, out);
     Check(hipGetLastError(), "launching the batched tile kernel");
 }
 
@@ -676,7 +764,9 @@ int tg_hip_catalogue_create(tg_dev* dev, const tg_crater_shape* kernel,
         cat->outline = UploadArray(outline.data(), outline.size());
         cat->diameters = UploadArray(diameters, count);
         cat->terms = DeviceMpdTerms(cat->kernel, cat->records, count);
+        cat->precise = DevicePreciseTerms(cat->kernel, cat->records, nullptr, count);
         cat->allFast = AllMpdFast(*kernel, records, count);
+        cat->anyPrecise = AnyPrecise(*kernel, records, count);
         cat->hot = Alloc<tg::HotCrater>(count);
         cat->hotHarm = Alloc<float>(count * 2u * tg::kHotHarmonics);
         if (count > 0) {
@@ -1012,7 +1102,17 @@ int tg_hip_catalogue_frames(tg_catalogue* cat, const tg_crater_frame* frames, si
             }
         }
         Check(hipSetDevice(cat->dev->ordinal), "hipSetDevice");
-        cat->frames = UploadArray(frames, count);
+        tg_crater_frame* uploaded = UploadArray(frames, count);
+        tg::PreciseTerms* precise = nullptr;
+        try {
+            precise = DevicePreciseTerms(cat->kernel, cat->records, uploaded, count);
+        } catch (...) {
+            (void)hipFree(uploaded);
+            throw;
+        }
+        (void)hipFree(cat->precise);  // the plane's terms, without frame axes
+        cat->frames = uploaded;
+        cat->precise = precise;
         cat->generation = NextGeneration();  // graphs captured without the frames are stale
     });
 }

@@ -928,6 +928,17 @@ __device__ __forceinline__ float ApplyCrater(const CraterShape& shape, const flo
 // KernelProfileDistance (craters.h:1034-1054) perform them, so ApplyCraterMpd gives the same bits
 // as ApplyCrater for such a crater.
 
+// MpdTerms::fast: 0 for the full evaluation, else the hoisted path in FP32 (kFastSingle) or, for a
+// crater of kPreciseDiameter or more, in FP64 (kFastPrecise, PreciseTerms below).
+constexpr uint32_t kFastSingle = 1u;
+constexpr uint32_t kFastPrecise = 2u;
+// FP32 carries a relative 6e-8: for a crater of diameter D the distances, depths and rim heights
+// of one pair round to about 1.5e-7 D of height (2e-3 m at 20 km, 1.5e-4 m at 1 km, measured
+// against the FP64 oracle with the reference's arithmetic). Pairs with craters of a kilometre and
+// more are evaluated in FP64, which keeps a cell within the 1e-3 m bound wherever km-scale
+// craters overlap.
+constexpr float kPreciseDiameter = 1000.0f;
+
 struct MpdTerms {
     float radius;     // 0.5 D
     float reach;      // reach radii x radius
@@ -955,6 +966,13 @@ __host__ __device__ inline bool MpdShapeOnly(const tg_crater_shape& k) {
            k.noise_rim_amplitude == 0.0f && k.wall_streaks == 0.0f && k.ejecta == 0.0f;
 }
 
+// The record's part of the condition: nothing the hoisted path leaves out.
+__host__ __device__ inline bool MpdRecordFast(const tg_crater_record& c) {
+    return c.has_datum != 1u && c.mark_count == 0u && c.terrace_count == 0u &&
+           c.peak_count == 0u && c.shift_x == 0.0f && c.shift_y == 0.0f && c.upslope_x == 0.0f &&
+           c.upslope_y == 0.0f && c.table_count == 0u;
+}
+
 __device__ inline MpdTerms MakeMpdTerms(const tg_crater_shape& k, const tg_crater_record& c) {
     MpdTerms t{};
     const float diameter = c.diameter;
@@ -978,11 +996,8 @@ __device__ inline MpdTerms MakeMpdTerms(const tg_crater_shape& k, const tg_crate
     t.betaSlope = t.beta * slope;
     t.offset = t.rim - t.ejecta;
     t.gradient = -t.ejecta / 8.0f - t.offset;
-    t.fast = (MpdShapeOnly(k) && c.has_datum != 1u && c.mark_count == 0u &&
-              c.terrace_count == 0u && c.peak_count == 0u && c.shift_x == 0.0f &&
-              c.shift_y == 0.0f && c.upslope_x == 0.0f && c.upslope_y == 0.0f &&
-              c.table_count == 0u)
-                 ? 1u
+    t.fast = MpdShapeOnly(k) && MpdRecordFast(c)
+                 ? (c.diameter >= kPreciseDiameter ? kFastPrecise : kFastSingle)
                  : 0u;
     return t;
 }
@@ -1057,5 +1072,279 @@ __device__ __forceinline__ float ApplyCraterMpd(const tg_crater_record& c, const
     return ApplyMpd(t, c.ellipticity, c.first_harmonic, c.harmonic_count,
                     harm + c.harmonic_offset, ra, height, dx, dy, weight);
 }
+
+// ---- The hoisted MPD path in FP64, for craters of kPreciseDiameter and more ---------------------
+//
+// The same profile law, outline and blend as ApplyMpd, with every per-pair quantity in FP64: the
+// cell's offset from the split cell point and split crater centre (on a body, in the crater's
+// frame with FP64 axes), the distance, the outline scale and the height. The record fields the
+// terms start from are the FP32 ones the FP64 oracle widens, so only rounding differs from it.
+// A cell's precise contributions are summed in FP64 (PreciseSum) and reach its FP32 height once,
+// since a km-scale height rounds to 1e-4 m or more on every FP32 store.
+
+struct PreciseTerms {
+    double invRadius, reach, worn, freshRim, rim, k, alpha, interior, beta, invBeta, ejecta,
+        joinRise, betaSlope, offset, gradient;
+    double cosHeading, sinHeading, along, alongInv;  // the outline's ellipse
+    double centreX, centreY, centreZ;  // the split centre, summed (z 0 on a plane)
+    double eastX, eastY, northX, northY, northZ;     // on a body: the frame's axes (east z = 0)
+    double halfInvSphere;              // on a body: 0.5 / the sphere's radius
+    double pad[4];
+};
+static_assert(sizeof(PreciseTerms) == 256, "PreciseTerms is two 128-byte lines");
+
+// 1 / v to within an ulp or so: the hardware estimate and two Newton steps.
+__device__ __forceinline__ double Reciprocal64(double v) {
+    double r = __builtin_amdgcn_rcp(v);
+    r = fma(r, fma(-v, r, 1.0), r);
+    return fma(r, fma(-v, r, 1.0), r);
+}
+
+// 1 / sqrt(v) for a positive normal v: the hardware estimate and two Newton steps.
+__device__ __forceinline__ double InvSqrt64(double v) {
+    double y = __builtin_amdgcn_rsq(v);
+    const double half = 0.5 * v;
+    y = y * fma(-half * y, y, 1.5);
+    return y * fma(-half * y, y, 1.5);
+}
+
+// sqrt(v) for v >= 0 to within an ulp or so: v / sqrt(v) from InvSqrt64, then one correction.
+__device__ __forceinline__ double Sqrt64(double v) {
+    if (!(v > 0.0)) {
+        return 0.0;
+    }
+    const double y = InvSqrt64(v);
+    const double s = v * y;
+    return fma(fma(-s, s, v), 0.5 * y, s);
+}
+
+__device__ __forceinline__ double Smoothstep64(double t) {
+    const double u = t < 0.0 ? 0.0 : (t > 1.0 ? 1.0 : t);
+    return u * u * (3.0 - 2.0 * u);
+}
+
+// MartinParkesDunstanExterior (craters.h:374-386) in FP64.
+__device__ __forceinline__ void MpdExterior64(double x, double rim, double ejectaScale,
+                                              double& height, double& slope) {
+    const double offset = rim - ejectaScale;
+    const double gradient = -ejectaScale / 8.0 - offset;
+    const double r = 1.0 / (x + 1.0);
+    const double law = ejectaScale * r * r * r + offset + gradient * x;
+    const double lawSlope = -3.0 * ejectaScale * r * r * r * r + gradient;
+    const double taper = 1.0 - x * x;
+    height = law * taper;
+    slope = lawSlope * taper - 2.0 * x * law;
+}
+
+// The terms of crater c (MakeMpdTerms's, in FP64 from the same record fields), its ellipse and,
+// with a frame, FP64 east and north axes from the frame's split centre.
+__device__ inline PreciseTerms MakePreciseTerms(const tg_crater_shape& k, const tg_crater_record& c,
+                                                const tg_crater_frame* frame) {
+    PreciseTerms t{};
+    const double diameter = c.diameter;
+    const double radius = 0.5 * diameter;
+    const double depth = double(c.current_depth) * diameter;
+    t.rim = double(c.current_rim) * diameter;
+    t.freshRim = double(c.fresh_rim) * diameter;
+    t.invRadius = 1.0 / radius;
+    t.reach = double(k.reach_radii) * radius;
+    t.worn = c.fresh_rim > 0.0f ? fmin(fmax(double(c.current_rim) / double(c.fresh_rim), 0.0), 1.0)
+                                : 1.0;
+    t.k = t.freshRim - t.rim + depth;
+    t.alpha = (t.rim - t.freshRim) / t.k;
+    t.interior = t.k * (t.alpha + 1.0) / t.alpha;
+    const double scale = 0.14 * pow(radius, 0.74);
+    t.ejecta = t.freshRim > 0.0 ? scale * t.rim / t.freshRim : scale;
+    t.beta = fmin(1.0, 1.5 * (t.freshRim - t.rim) / t.k);
+    t.invBeta = 1.0 / t.beta;
+    double joinHeight, joinSlope;
+    MpdExterior64(t.beta, t.rim, t.ejecta, joinHeight, joinSlope);
+    const double drop = t.rim - joinHeight;
+    const double slope = drop > 0.0 ? fmax(joinSlope, -3.0 * drop / t.beta) : joinSlope;
+    t.joinRise = joinHeight - t.rim;
+    t.betaSlope = t.beta * slope;
+    t.offset = t.rim - t.ejecta;
+    t.gradient = -t.ejecta / 8.0 - t.offset;
+    t.cosHeading = cos(double(c.heading));
+    t.sinHeading = sin(double(c.heading));
+    t.along = sqrt(double(c.ellipticity));
+    t.alongInv = 1.0 / t.along;
+    t.centreX = double(c.x) + double(c.x_low);
+    t.centreY = double(c.y) + double(c.y_low);
+    if (frame != nullptr) {
+        const double cx = double(frame->x) + double(frame->x_low);
+        const double cy = double(frame->y) + double(frame->y_low);
+        const double cz = double(frame->z) + double(frame->z_low);
+        t.centreX = cx;
+        t.centreY = cy;
+        t.centreZ = cz;
+        t.halfInvSphere = frame->sphere_radius > 0.0f ? 0.5 / double(frame->sphere_radius) : 0.0;
+        const double rho = sqrt(cx * cx + cy * cy);
+        const double full = sqrt(rho * rho + cz * cz);
+        if (rho > 1e-6 * full) {
+            t.eastX = -cy / rho;
+            t.eastY = cx / rho;
+            t.northX = -cz * cx / (full * rho);
+            t.northY = -cz * cy / (full * rho);
+            t.northZ = rho / full;
+        } else {  // at a pole the centre fixes no longitude: the frame's own axes
+            t.eastX = frame->east_x;
+            t.eastY = frame->east_y;
+            t.northX = frame->north_x;
+            t.northY = frame->north_y;
+            t.northZ = frame->north_z;
+        }
+    }
+    return t;
+}
+
+// MpdHeightFrom in FP64.
+__device__ __forceinline__ double MpdHeight64(const PreciseTerms& t, double distance) {
+    const double x = distance * t.invRadius - 1.0;
+    if (x >= 1.0) {
+        return 0.0;
+    }
+    if (x <= t.alpha) {
+        return fma(fma(t.k, x, 2.0 * t.k), x, t.freshRim);
+    }
+    if (x <= 0.0) {
+        return fma(t.interior * x, x, t.rim);
+    }
+    if (x <= t.beta) {
+        const double u = x * t.invBeta;
+        const double u2 = u * u;
+        return fma(t.betaSlope * u2, u - 1.0, fma(t.joinRise * u2, fma(-2.0, u, 3.0), t.rim));
+    }
+    const double r = Reciprocal64(x + 1.0);
+    const double law = fma(t.ejecta * r, r * r, fma(t.gradient, x, t.offset));
+    return law * fma(-x, x, 1.0);
+}
+
+// The offset of a cell from a crater centre: its distance and direction cosines (1, 0 at the
+// centre).
+struct Offset64 {
+    double distance, ct, st;
+};
+
+// FastOutline in FP64 over the same (A cos phi, A sin phi) coefficients. cos k theta and
+// sin k theta follow the three-term recurrence x_(k+1) = 2 cos theta x_k - x_(k-1), which for
+// the dozen or so harmonics of an outline stays within a few ulps in FP64.
+__device__ __forceinline__ double FastOutline64(const float* coef, const PreciseTerms& t,
+                                                float ellipticity, uint32_t first, uint32_t count,
+                                                const Offset64& o, double share) {
+    const double ct = o.ct, st = o.st;
+    double scale = 1.0;
+    if (ellipticity != 1.0f) {
+        const double cd = fma(ct, t.cosHeading, st * t.sinHeading);
+        const double sd = fma(st, t.cosHeading, -(ct * t.sinHeading));
+        const double u = cd * t.alongInv;
+        const double v = sd * t.along;
+        scale = InvSqrt64(fma(u, u, v * v));
+    }
+    double wobble = 1.0;
+    if (share > 0.0 && count > 0u) {
+        const bool partial = share < 1.0;
+        const double q = partial ? Sqrt64(share) : 1.0;
+        const double twoCos = 2.0 * ct;
+        // (cos, sin) of (k - 1) theta and k theta, from k = 0.
+        double cPrev = ct, sPrev = -st, ck = 1.0, sk = 0.0, kept = 1.0;
+        for (uint32_t k = 0; k < first; ++k) {
+            const double cNext = fma(twoCos, ck, -cPrev);
+            const double sNext = fma(twoCos, sk, -sPrev);
+            cPrev = ck;
+            sPrev = sk;
+            ck = cNext;
+            sk = sNext;
+            kept *= q;
+        }
+        for (uint32_t j = 0; j < count; ++j) {
+            const double wave = fma(double(coef[2u * j]), ck, -(double(coef[2u * j + 1u]) * sk));
+            wobble = fma(partial ? kept : 1.0, wave, wobble);
+            const double cNext = fma(twoCos, ck, -cPrev);
+            const double sNext = fma(twoCos, sk, -sPrev);
+            cPrev = ck;
+            sPrev = sk;
+            ck = cNext;
+            sk = sNext;
+            kept *= q;
+        }
+    }
+    return scale * wobble;
+}
+
+// ApplyMpd's contribution in FP64 at offset o, before the resolution weight: 0 beyond the
+// outline-scaled reach.
+__device__ __forceinline__ double MpdPrecise(const PreciseTerms& t, float ellipticity,
+                                             uint32_t first, uint32_t count, const float* coef,
+                                             const Offset64& o) {
+    double distance = o.distance;
+    if (ellipticity != 1.0f || count > 0u) {
+        const double share = Smoothstep64(distance * t.invRadius) * t.worn;
+        distance *= Reciprocal64(FastOutline64(coef, t, ellipticity, first, count, o, share));
+    }
+    if (distance >= t.reach) {
+        return 0.0;
+    }
+    return MpdHeight64(t, distance);
+}
+
+// The plane offset of the split cell point (x + xLow, y + yLow) from the crater's centre.
+__device__ __forceinline__ Offset64 PlaneOffset64(const PreciseTerms& t, float x, float xLow,
+                                                  float y, float yLow) {
+    const double dx = (double(x) + double(xLow)) - t.centreX;
+    const double dy = (double(y) + double(yLow)) - t.centreY;
+    Offset64 o{Sqrt64(dx * dx + dy * dy), 1.0, 0.0};
+    if (o.distance > 0.0) {
+        const double inv = Reciprocal64(o.distance);
+        o.ct = dx * inv;
+        o.st = dy * inv;
+    }
+    return o;
+}
+
+// KernelFrameOffset (craters.h:168-186) in FP64: the split cell point's offset in the crater's
+// frame, its length the arc along the sphere rather than the chord.
+__device__ __forceinline__ Offset64 BodyOffset64(const PreciseTerms& t, const tg_split_point& p) {
+    const double qx = (double(p.x) + double(p.x_low)) - t.centreX;
+    const double qy = (double(p.y) + double(p.y_low)) - t.centreY;
+    const double qz = (double(p.z) + double(p.z_low)) - t.centreZ;
+    const double ox = qx * t.eastX + qy * t.eastY;
+    const double oy = qx * t.northX + qy * t.northY + qz * t.northZ;
+    const double chord = Sqrt64(qx * qx + qy * qy + qz * qz);
+    const double s = fmin(chord * t.halfInvSphere, 1.0);
+    double ratio;
+    if (s < 0.25) {
+        const double s2 = s * s;
+        ratio = 1.0 + s2 * (1.0 / 6.0 +
+                            s2 * (3.0 / 40.0 + s2 * (5.0 / 112.0 + s2 * (35.0 / 1152.0 +
+                                                                        s2 * (63.0 / 2816.0)))));
+    } else {
+        ratio = atan2(s, sqrt(1.0 - s * s)) / s;
+    }
+    Offset64 o{chord * ratio, 0.0, 1.0};
+    const double across = Sqrt64(ox * ox + oy * oy);
+    if (across > 0.0) {
+        const double inv = Reciprocal64(across);
+        o.ct = ox * inv;
+        o.st = oy * inv;
+    }
+    return o;
+}
+
+// A cell's height as its craters are applied: the FP32 height of the reference's arithmetic and
+// the FP64 sum of the precise craters not yet added to it. Both the batched and the strip kernels
+// fold the sum in before a crater whose evaluation reads the height, and at the end.
+struct PreciseSum {
+    float h;
+    double pending = 0.0;
+
+    __device__ __forceinline__ float Folded() {
+        if (pending != 0.0) {
+            h = float(double(h) + pending);
+            pending = 0.0;
+        }
+        return h;
+    }
+};
 
 }  // namespace tg

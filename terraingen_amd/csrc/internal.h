@@ -20,6 +20,7 @@
 
 namespace tg {
 struct MpdTerms;
+struct PreciseTerms;
 }
 
 struct tg_dev {
@@ -42,6 +43,7 @@ struct tg_crater_set {
     float* harm = nullptr;  // A cos(phi), A sin(phi) in the data's harmonic slots
     float4* outline = nullptr;  // per record: cos and sin of the heading, sqrt(ellipticity), 1/that
     tg::MpdTerms* terms = nullptr;  // per record: the hoisted MPD path's constants
+    tg::PreciseTerms* precise = nullptr;  // per record: FP64 terms of km-scale craters
     bool allFast = false;           // every record takes the hoisted path
     size_t pointCount = 0;
     size_t lowCount = 0;
@@ -146,10 +148,16 @@ void LaunchBoulders(const tg_boulder_set& set, float* heights, const float* xs, 
 // device array of n entries (at least one).
 tg::MpdTerms* DeviceMpdTerms(const tg_crater_shape* kernelOnDevice,
                              const tg_crater_record* recordsOnDevice, size_t n);
+// FP64 terms of every crater (tg::MakePreciseTerms), with frame axes when frames are given.
+tg::PreciseTerms* DevicePreciseTerms(const tg_crater_shape* kernelOnDevice,
+                                     const tg_crater_record* recordsOnDevice,
+                                     const tg_crater_frame* framesOnDevice, size_t n);
 
 // True when every record takes the hoisted MPD path (MakeMpdTerms's `fast`), decided on the host
 // by the same conditions (tg_hip.hip).
 bool AllMpdFast(const tg_crater_shape& k, const tg_crater_record* records, size_t n);
+// Whether any record takes the hoisted path in FP64 (tg::kFastPrecise).
+bool AnyPrecise(const tg_crater_shape& k, const tg_crater_record* records, size_t n);
 
 // Throws HipError unless every index a crater batch holds stays inside its arrays (tg_hip.hip).
 void ValidateCraterBatch(const tg_crater_shape& k, const tg_crater_record* records, size_t n,

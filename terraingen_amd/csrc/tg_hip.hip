@@ -125,7 +125,7 @@ __global__ void __launch_bounds__(kCraterTile * kCraterTile)
                 const tg_split_point* __restrict__ points, const float* __restrict__ lows,
                 const float* __restrict__ cull, const float* __restrict__ harm,
                 const float4* __restrict__ outline, const tg::MpdTerms* __restrict__ terms,
-                tg_kernel_params params) {
+                const tg::PreciseTerms* __restrict__ precise, tg_kernel_params params) {
     // Each 64-lane wave covers an 8 x 8 square of the 16 x 16 tile, so a disc-shaped reach
     // leaves fewer waves partly active than rows of the tile would.
     const uint32_t t = threadIdx.x;
@@ -147,7 +147,7 @@ __global__ void __launch_bounds__(kCraterTile * kCraterTile)
     const uint32_t first = tiles[head];
     const uint32_t count = tiles[head + 1];
     const tg::CraterShape shape(*kernel);
-    float h = height;
+    tg::PreciseSum cell{height};
     if (tiles[2] == 0u) {
         tg_split_point p;
         p.x = xs[col];
@@ -159,15 +159,25 @@ __global__ void __launch_bounds__(kCraterTile * kCraterTile)
         for (uint32_t e = first; e < first + count; ++e) {
             const uint32_t n = entries[e];
             const tg_crater_record& c = records[n];
+            if (terms[n].fast == tg::kFastPrecise) {
+                const tg::Offset64 o = tg::PlaneOffset64(precise[n], p.x, p.x_low, p.y, p.y_low);
+                if (o.distance * o.distance < double(cull[n])) {
+                    cell.pending += double(c.resolution) *
+                                    tg::MpdPrecise(precise[n], c.ellipticity, c.first_harmonic,
+                                                   c.harmonic_count, harm + c.harmonic_offset, o);
+                }
+                continue;
+            }
             const float dx = tg::SplitDifference(p.x, p.x_low, c.x, c.x_low);
             const float dy = tg::SplitDifference(p.y, p.y_low, c.y, c.y_low);
             if (dx * dx + dy * dy >= cull[n]) {
                 continue;  // beyond any reach its outline allows: ApplyCrater returns h as is
             }
             if (kAllFast || terms[n].fast != 0u) {
-                h = tg::ApplyCraterMpd(c, terms[n], h, dx, dy, harm, outline[n], c.resolution);
+                cell.h = tg::ApplyCraterMpd(c, terms[n], cell.h, dx, dy, harm, outline[n],
+                                            c.resolution);
             } else if constexpr (!kAllFast) {
-                h = tg::ApplyCrater(shape, data, c, h, dx, dy, harm, outline[n]);
+                cell.h = tg::ApplyCrater(shape, data, c, cell.Folded(), dx, dy, harm, outline[n]);
             }
         }
     } else if constexpr (!kAllFast) {
@@ -175,20 +185,30 @@ __global__ void __launch_bounds__(kCraterTile * kCraterTile)
         const tg_split_point p = points[i];
         for (uint32_t e = first; e < first + count; ++e) {
             const uint32_t n = entries[e];
+            const tg_crater_record& c = records[n];
+            if (terms[n].fast == tg::kFastPrecise) {
+                const tg::Offset64 o = tg::BodyOffset64(precise[n], p);
+                if (o.distance * o.distance < double(cull[n])) {
+                    cell.pending += double(c.resolution) *
+                                    tg::MpdPrecise(precise[n], c.ellipticity, c.first_harmonic,
+                                                   c.harmonic_count, harm + c.harmonic_offset, o);
+                }
+                continue;
+            }
             const tg::Offset at = tg::BodySplitOffset(frames[n], p);
             if (at.dx * at.dx + at.dy * at.dy >= cull[n]) {
                 continue;
             }
-            const tg_crater_record& c = records[n];
             if (terms[n].fast != 0u) {
-                h = tg::ApplyCraterMpd(c, terms[n], h, at.dx, at.dy, harm, outline[n],
-                                       c.resolution);
+                cell.h = tg::ApplyCraterMpd(c, terms[n], cell.h, at.dx, at.dy, harm, outline[n],
+                                            c.resolution);
             } else {
-                h = tg::ApplyCrater(shape, data, c, h, at.dx, at.dy, harm, outline[n]);
+                cell.h = tg::ApplyCrater(shape, data, c, cell.Folded(), at.dx, at.dy, harm,
+                                         outline[n]);
             }
         }
     }
-    heights[i] = h;
+    heights[i] = cell.Folded();
 }
 
 
@@ -205,7 +225,8 @@ void LaunchCraters(const tg_crater_set& set, float* heights, const float* xs, co
     auto* fn = set.allFast && !set.onBody ? CraterStrip<true> : CraterStrip<false>;
     hipLaunchKernelGGL(fn, grid, block, 0, stream, heights, xs, ys, set.kernel,
                        set.records, set.data, set.tiles, set.entries, set.frames, set.points,
-                       set.lows, set.cull, set.harm, set.outline, set.terms, params);
+                       set.lows, set.cull, set.harm, set.outline, set.terms, set.precise,
+                       params);
     Check(hipGetLastError(), "launching the crater kernel");
 }
 
@@ -522,6 +543,11 @@ int CreateCraterSet(tg_dev* dev, const tg_crater_shape* kernel, const tg_crater_
         set->outline = Upload(outline.data(), outline.size(), owned);
         set->terms = DeviceMpdTerms(set->kernel, set->records, recordCount);
         owned.push_back(set->terms);
+        set->precise = DevicePreciseTerms(set->kernel, set->records,
+                                          tiles[2] != 0 && frames != nullptr ? set->frames
+                                                                             : nullptr,
+                                          recordCount);
+        owned.push_back(set->precise);
         set->allFast = AllMpdFast(*kernel, records, recordCount);
         set->tileSide = tiles[1];
         set->tilesAcross = tiles[0];
@@ -580,7 +606,8 @@ int tg_hip_crater_set_free(tg_crater_set* set) {
                     static_cast<void*>(set->entries), static_cast<void*>(set->frames),
                     static_cast<void*>(set->points), static_cast<void*>(set->lows),
                     static_cast<void*>(set->cull), static_cast<void*>(set->harm),
-                    static_cast<void*>(set->outline), static_cast<void*>(set->terms)}) {
+                    static_cast<void*>(set->outline), static_cast<void*>(set->terms),
+                    static_cast<void*>(set->precise)}) {
         (void)hipFree(p);
     }
     delete set;
@@ -877,9 +904,45 @@ __global__ void MakeMpdTermsKernel(const tg_crater_shape* __restrict__ kernel,
     }
 }
 
+__global__ void MakePreciseTermsKernel(const tg_crater_shape* __restrict__ kernel,
+                                       const tg_crater_record* __restrict__ records,
+                                       const tg_crater_frame* __restrict__ frames, uint32_t n,
+                                       tg::PreciseTerms* __restrict__ out) {
+    const uint32_t i = blockIdx.x * blockDim.x + threadIdx.x;
+    if (i < n) {
+        out[i] = tg::MakePreciseTerms(*kernel, records[i], frames != nullptr ? frames + i : nullptr);
+    }
+}
+
 }  // namespace
 
 namespace tgi {
+
+tg::PreciseTerms* DevicePreciseTerms(const tg_crater_shape* kernelOnDevice,
+                                     const tg_crater_record* recordsOnDevice,
+                                     const tg_crater_frame* framesOnDevice, size_t n) {
+    if (n > 0xFFFFFFFFull) {
+        throw HipError{"too many craters for the terms kernel"};
+    }
+    void* p = nullptr;
+    Check(hipMalloc(&p, (n > 0 ? n : 1) * sizeof(tg::PreciseTerms)), "hipMalloc");
+    if (n > 0) {
+        const uint32_t threads = 256;
+        const uint32_t blocks = uint32_t((n + threads - 1) / threads);
+        hipLaunchKernelGGL(MakePreciseTermsKernel, dim3(blocks), dim3(threads), 0, nullptr,
+                           kernelOnDevice, recordsOnDevice, framesOnDevice, uint32_t(n),
+                           static_cast<tg::PreciseTerms*>(p));
+        hipError_t status = hipGetLastError();
+        if (status == hipSuccess) {
+            status = hipDeviceSynchronize();
+        }
+        if (status != hipSuccess) {
+            (void)hipFree(p);
+            Check(status, "computing the precise crater terms");
+        }
+    }
+    return static_cast<tg::PreciseTerms*>(p);
+}
 
 tg::MpdTerms* DeviceMpdTerms(const tg_crater_shape* kernelOnDevice,
                              const tg_crater_record* recordsOnDevice, size_t n) {
@@ -911,14 +974,23 @@ bool AllMpdFast(const tg_crater_shape& k, const tg_crater_record* records, size_
         return n == 0;
     }
     for (size_t i = 0; i < n; ++i) {
-        const tg_crater_record& c = records[i];
-        if (c.has_datum == 1u || c.mark_count != 0u || c.terrace_count != 0u ||
-            c.peak_count != 0u || c.shift_x != 0.0f || c.shift_y != 0.0f ||
-            c.upslope_x != 0.0f || c.upslope_y != 0.0f || c.table_count != 0u) {
+        if (!tg::MpdRecordFast(records[i])) {
             return false;
         }
     }
     return true;
+}
+
+bool AnyPrecise(const tg_crater_shape& k, const tg_crater_record* records, size_t n) {
+    if (!tg::MpdShapeOnly(k)) {
+        return false;
+    }
+    for (size_t i = 0; i < n; ++i) {
+        if (tg::MpdRecordFast(records[i]) && records[i].diameter >= tg::kPreciseDiameter) {
+            return true;
+        }
+    }
+    return false;
 }
 
 // The squared round distance from a crater's centre beyond which ApplyCrater leaves every height
