@@ -82,10 +82,12 @@ OPS_BOULDER_PAIR = 40
 OPS_PER_PERLIN3_OCTAVE = 440
 OPS_BODY_POSITION = 15
 OPS_BODY_PAIR_EXTRA = 35
-# MI355X's published FP32 vector peak, 157.3 TFLOP/s, in lane operations: an FMA counts 2 flops
-# and the peak assumes the packed two-lane form, so 157.3e12 / 2 lane-ops/s (256 CUs x 64 lanes x
-# 2 x 2.4 GHz).
-VALU_PEAK = 157.3e12 / 2.0
+# MI355X's published FP32 vector peak. It counts a packed FMA as 4 flops per lane (2 lanes x 2
+# flops), so it bounds any count of one lane-op per operation from above: the kernels are built
+# without packed FP32, and their issue ceiling is a quarter of it, 39.3e12 lane-instructions/s
+# (256 CUs x 64 lanes x 2.4 GHz).
+VALU_PEAK_TFLOPS = 157.3
+ISSUE_CEILING = 256 * 64 * 2.4e9
 HBM_PEAK_GBS = 8000.0
 METRIC = "Gcells/s batched vs plain HIP at <=1e-3 m err; p50/p99 tile latency"
 
@@ -97,10 +99,15 @@ GLOBAL_MAX_D = 20000.0  # m
 
 
 def roofline(ops: float, kernel_ms: float, bytes_written: float) -> dict:
+    """The VALU roofline of one launch: algorithmic lane operations per second against the
+    published FP32 vector peak, and the compulsory height writes against HBM bandwidth."""
     achieved = ops / (kernel_ms * 1e-3)
     return {"bound": "valu", "achieved": round(achieved / 1e12, 3),
-            "peak": round(VALU_PEAK / 1e12, 3), "unit": "Tops/s",
-            "frac": round(achieved / VALU_PEAK, 4), "traffic": None,
+            "peak": VALU_PEAK_TFLOPS, "unit": "TFLOP/s",
+            "frac": round(achieved / (VALU_PEAK_TFLOPS * 1e12), 4),
+            "peak_basis": "MI355X published FP32 vector peak (157.3 TFLOP/s, packed FMA as 4 "
+                          "flops a lane); achieved counts one per algorithmic lane operation",
+            "traffic": None,
             "hbm_write_GBs": round(bytes_written / (kernel_ms * 1e-3) / 1e9, 1),
             "hbm_write_frac": round(bytes_written / (kernel_ms * 1e-3) / 1e9 / HBM_PEAK_GBS, 5)}
 
@@ -120,12 +127,19 @@ def counted_traffic(kernel_ms: float, name: str) -> dict | None:
             if abs(meta["kernel_ns"] * 1e-6 / kernel_ms - 1.0) > 0.05:
                 return {"traffic": None, "counters_stale": str(path.relative_to(ROOT))}
             lane = meta["valu_lane_instructions"]
-            return {"traffic": round(meta["read_bytes"] + meta["write_bytes"]),
-                    "traffic_read": round(meta["read_bytes"]),
-                    "traffic_write": round(meta["write_bytes"]),
-                    "valu_lane_instructions": lane,
-                    "valu_issue_frac": round(lane / (kernel_ms * 1e-3) / VALU_PEAK, 4),
-                    "counters": str(path.relative_to(ROOT))}
+            out = {"traffic": round(meta["read_bytes"] + meta["write_bytes"]),
+                   "traffic_read": round(meta["read_bytes"]),
+                   "traffic_write": round(meta["write_bytes"]),
+                   "valu_lane_instructions": lane,
+                   "counters": str(path.relative_to(ROOT))}
+            if "probe_valu_rate" in meta:
+                # Executed work against the issue rate the same counter reads for the probe's
+                # independent chains, in the same run.
+                rate = lane / (meta["kernel_ns"] * 1e-9)
+                out["executed_frac"] = round(rate / meta["probe_valu_rate"], 4)
+                out["executed_basis"] = ("SQ_INSTS_VALU x 64 per second of the kernel over the "
+                                         "same for the VALU probe's chains, one counter run")
+            return out
     return None
 
 
@@ -145,52 +159,220 @@ def percentile(values, q) -> float:
     return float(np.percentile(np.asarray(values), q))
 
 
-# ---- the CPU baseline leg (oracle restatement of the reference's CPU path) -------------------
+# ---- the CPU baseline leg and the FP64 error (oracle/: test infrastructure) ---------------------
 
-def cpu_tiles(batch, sample: list[int], cores: int) -> dict:
-    """The reference's CPU path over sampled tiles, timed by the oracle restatement: noise rows
-    then crater tiles, FP32 kernel math on all host threads. Also the FP64 oracle of the same
-    tiles, for the error of the hip result."""
+def host_cpu() -> tuple[str, int]:
+    """The host CPU's model name and the threads this process may run on."""
+    model = platform.processor() or platform.machine()
+    try:
+        with open("/proc/cpuinfo") as f:
+            for line in f:
+                if line.startswith("model name"):
+                    model = line.split(":", 1)[1].strip()
+                    break
+    except OSError:
+        pass
+    try:
+        cores = len(os.sched_getaffinity(0))
+    except AttributeError:
+        cores = os.cpu_count() or 1
+    return model, cores
+
+
+def cpu_baseline(batch, sample: list[int], sphere: bool, repeats: int = 5) -> dict:
+    """The reference's CPU path over sampled tiles, on this host's cores: oracle/_ref, the
+    reference's own code built from its sources, runs its noise rows (FractalStackRows on
+    ParallelFor, noise_layer.cpp:28-43) then its crater tiles (ShapeCraters / ShapeCratersOnBody
+    on ForEachTile, apply.cpp:199-222). The oracle's restatement runs the same tiles beside it.
+    Inputs (crater batches, bins, positions) are prepared before the timing; one warm-up pass,
+    then the best of `repeats` passes of each. Falls back to the restatement alone (kind "port")
+    where oracle/_ref was not built."""
+    from oracle import oracle, reference
+    from terraingen_amd import native as N
+
+    orc, ref = oracle(), reference()
+    model, cores = host_cpu()
+    inputs = [(i, batch.plain_inputs(i)) for i in sample]
+    cells = sum(inp["grid"].width * inp["grid"].height for _, inp in inputs)
+    k = batch.noise_kernel
+    fractals = C.cast(C.byref(k.fractals), C.c_void_p)
+
+    def run_reference():
+        outs = {}
+        for i, inp in inputs:
+            g = inp["grid"]
+            out = np.empty((g.height, g.width), np.float32)
+            cr = inp["craters"]
+            if sphere:
+                args = ((C.byref(cr[0]), cr[1].ctypes.data, cr[2].ctypes.data, cr[5].ctypes.data,
+                         cr[3].ctypes.data, cr[4].ctypes.data) if cr is not None
+                        else (None, None, None, None, None, None))
+                code = ref.ref_body_tile(fractals, k.count, g.origin_x, g.origin_y, g.pixel,
+                                         g.width, g.height, batch.radius, *args, cores,
+                                         out.ctypes.data)
+            else:
+                args = ((C.byref(cr[0]), cr[1].ctypes.data, cr[2].ctypes.data, cr[3].ctypes.data,
+                         cr[4].ctypes.data) if cr is not None else (None,) * 5)
+                code = ref.ref_plane_tile(fractals, k.count, g.origin_x, g.origin_y, g.pixel,
+                                          g.width, g.height, *args, cores, out.ctypes.data)
+            if code != 0:
+                raise RuntimeError(ref.ref_bench_error().decode())
+            outs[i] = out
+        return outs
+
+    def run_port():
+        outs = {}
+        for i, inp in inputs:
+            g = inp["grid"]
+            params = N.KernelParams(g.width, g.height, 0, 0.0, 0)
+            out = np.zeros((g.height, g.width), np.float32)
+            cr = inp["craters"]
+            if sphere:
+                pos = inp["positions"]
+                orc.or_body_noise_grid(C.byref(k), pos.ctypes.data, pos.shape[0],
+                                       C.byref(params), out.ctypes.data, cores)
+                if cr is not None:
+                    orc.or_shape_grid_body(C.byref(cr[0]), cr[1].ctypes.data, cr[2].ctypes.data,
+                                           cr[3].ctypes.data, cr[4].ctypes.data,
+                                           cr[5].ctypes.data, cr[6].ctypes.data, g.width,
+                                           g.height, C.byref(params), out.ctypes.data, cores)
+            else:
+                orc.or_noise_grid(C.byref(k), inp["xs"].ctypes.data, inp["ys"].ctypes.data,
+                                  g.width, g.height, C.byref(params), out.ctypes.data, cores)
+                if cr is not None:
+                    orc.or_shape_grid(C.byref(cr[0]), cr[1].ctypes.data, cr[2].ctypes.data,
+                                      cr[3].ctypes.data, cr[4].ctypes.data, inp["xs"].ctypes.data,
+                                      inp["ys"].ctypes.data, inp["lows"].ctypes.data, g.width,
+                                      g.height, C.byref(params), out.ctypes.data, cores)
+            outs[i] = out
+        return outs
+
+    def best(fn):
+        fn()
+        times = []
+        for _ in range(repeats):
+            t0 = time.perf_counter()
+            outs = fn()
+            times.append(time.perf_counter() - t0)
+        return min(times), outs
+
+    port_s, port_out = best(run_port)
+    result = {"unit": "Gcells/s", "cores": cores, "cpu_model": model,
+              "sample": f"{len(sample)} tiles ({cells} cells): noise then craters, inputs "
+                        f"prepared beforehand; best of {repeats} after 1 warm-up",
+              "port_value": round(cells / port_s / 1e9, 6)}
+    if ref is not None:
+        ref_s, ref_out = best(run_reference)
+        diff = max(float(np.abs(ref_out[i] - port_out[i]).max()) for i in ref_out)
+        result |= {"value": round(cells / ref_s / 1e9, 6), "kind": "reference",
+                   "calibration_ratio": round(port_s / ref_s, 3),
+                   "calibration_basis": "restatement time / reference time, same tiles, "
+                                        "same cores",
+                   "port_vs_reference_max_m": diff}
+    else:
+        result |= {"value": result["port_value"], "kind": "port"}
+    result["_fp32"] = port_out
+    return result
+
+
+def bands_of(height: int, rows: int = 8) -> list[tuple[int, int]]:
+    """Three bands of `rows` rows (top, middle, bottom) of a tile, or all of it when small."""
+    if height <= 3 * rows:
+        return [(0, height)]
+    mid = height // 2 - rows // 2
+    return [(0, rows), (mid, rows), (height - rows, rows)]
+
+
+DIAMETER_BANDS = [("none in reach", 0.0, 1.0), ("< 2 km", 1.0, 2000.0),
+                  ("2-5 km", 2000.0, 5000.0), ("5-10 km", 5000.0, 10000.0),
+                  (">= 10 km", 10000.0, np.inf)]
+
+
+def fp64_error(batch, got: np.ndarray, regular: list[int], sphere: bool, cores: int) -> dict:
+    """Max/RMS |hip - FP64 oracle| over row bands (bands_of) of the `regular` tiles and of every
+    tile a crater of 5 km or more reaches, reported per band of the largest crater in reach."""
     from oracle import oracle
     from terraingen_amd import native as N
 
     orc = oracle()
-    elapsed = 0.0
-    cells = 0
-    fp32, fp64 = {}, {}
-    for i in sample:
-        inp = batch.plain_inputs(i)
-        g = inp["grid"]
-        params = N.KernelParams(g.width, g.height, 0, 0.0, 0)
-        out = np.zeros((g.height, g.width), np.float32)
-        t0 = time.perf_counter()
-        orc.or_noise_grid(C.byref(batch.noise_kernel), inp["xs"].ctypes.data,
-                          inp["ys"].ctypes.data, g.width, g.height, C.byref(params),
-                          out.ctypes.data, cores)
-        if inp["craters"] is not None:
-            kernel, records, data, tiles, entries = inp["craters"]
-            orc.or_shape_grid(C.byref(kernel), records.ctypes.data, data.ctypes.data,
-                              tiles.ctypes.data, entries.ctypes.data, inp["xs"].ctypes.data,
-                              inp["ys"].ctypes.data, inp["lows"].ctypes.data, g.width,
-                              g.height, C.byref(params), out.ctypes.data, cores)
-        elapsed += time.perf_counter() - t0
-        cells += g.width * g.height
-        fp32[i] = out
-        cx64, cy64 = N.grid_centers(g)
-        cx64, cy64 = np.ascontiguousarray(cx64), np.ascontiguousarray(cy64)
-        ref = np.zeros((g.height, g.width), np.float64)
-        orc.or_noise_grid_f64(C.byref(batch.noise_kernel), cx64.ctypes.data, cy64.ctypes.data,
-                              g.width, g.height, ref.ctypes.data, cores)
-        if inp["craters"] is not None:
-            orc.or_shape_grid_f64(C.byref(kernel), records.ctypes.data, data.ctypes.data,
-                                  tiles.ctypes.data, entries.ctypes.data, cx64.ctypes.data,
-                                  cy64.ctypes.data, g.width, g.height, 0, ref.ctypes.data, cores)
-        fp64[i] = ref
-    return {"value": round(cells / elapsed / 1e9, 6), "unit": "Gcells/s", "cores": cores,
-            "kind": "port",
-            "sample": f"{len(sample)} of the tiles ({cells} cells), noise then craters, once "
-                      f"each, {platform.processor() or platform.machine()}",
-            "_fp32": fp32, "_fp64": fp64}
+    largest = batch.largest_in_reach()
+    big = [i for i in range(batch.tiles.size) if largest[i] >= 5000.0]
+    chosen = sorted(set(regular) | set(big))
+    k = batch.noise_kernel
+    stats = {name: [0.0, 0.0, 0, 0] for name, _, _ in DIAMETER_BANDS}  # max, sum sq, cells, tiles
+    for i in chosen:
+        t = batch.tiles[i]
+        w, h = int(t["width"]), int(t["height"])
+        prepared = batch.tile_craters(i)
+        mine = batch.tile_heights(got, i)
+        if prepared is not None:
+            kernel, records, data, foot = prepared
+            if sphere:
+                lists, entries = batch.band_lists(foot, float(t["origin_x"]),
+                                                  float(t["origin_y"]), float(t["pixel"]), w, h)
+                frames = frames64(foot[:, 0], foot[:, 1], batch.radius)
+            else:
+                grid = N.Grid(w, h, float(t["origin_x"]), float(t["origin_y"]),
+                              float(t["pixel"]))
+                lists, entries = N.bin_plane(grid, foot[:, 0], foot[:, 1], foot[:, 2], 16)
+        worst, sq, n = 0.0, 0.0, 0
+        for r0, rows in bands_of(h):
+            band = N.Grid(w, rows, float(t["origin_x"]),
+                          float(t["origin_y"]) - r0 * float(t["pixel"]), float(t["pixel"]))
+            f64 = np.zeros((rows, w), np.float64)
+            if sphere:
+                pos = N.sphere_positions(band, batch.radius)
+                orc.or_body_noise_grid_f64(C.byref(k), pos.ctypes.data, pos.shape[0],
+                                           f64.ctypes.data, cores)
+                if prepared is not None:
+                    orc.or_shape_grid_body_f64(C.byref(kernel), records.ctypes.data,
+                                               data.ctypes.data, lists.ctypes.data,
+                                               entries.ctypes.data, frames.ctypes.data,
+                                               pos.ctypes.data, w, rows, r0, f64.ctypes.data,
+                                               cores)
+            else:
+                cx64, cy64 = (np.ascontiguousarray(a) for a in N.grid_centers(band))
+                orc.or_noise_grid_f64(C.byref(k), cx64.ctypes.data, cy64.ctypes.data, w, rows,
+                                      f64.ctypes.data, cores)
+                if prepared is not None:
+                    orc.or_shape_grid_f64(C.byref(kernel), records.ctypes.data,
+                                          data.ctypes.data, lists.ctypes.data,
+                                          entries.ctypes.data, cx64.ctypes.data, cy64.ctypes.data,
+                                          w, rows, r0, f64.ctypes.data, cores)
+            d = np.abs(mine[r0:r0 + rows].astype(np.float64) - f64)
+            worst = max(worst, float(d.max()))
+            sq += float((d ** 2).sum())
+            n += d.size
+        for name, lo, hi in DIAMETER_BANDS:
+            if lo <= largest[i] < hi:
+                st = stats[name]
+                st[0] = max(st[0], worst)
+                st[1] += sq
+                st[2] += n
+                st[3] += 1
+    total = [max(v[0] for v in stats.values()), sum(v[1] for v in stats.values()),
+             sum(v[2] for v in stats.values()), sum(v[3] for v in stats.values())]
+    return {"max": float(f"{total[0]:.3g}"),
+            "rms": float(f"{np.sqrt(total[1] / max(total[2], 1)):.3g}"),
+            "cells": total[2], "tiles": total[3],
+            "tiles_with_craters_of_5km_or_more": len(big),
+            "sampled": "3 bands of 8 rows per tile (all rows of small tiles): every tile a "
+                       "crater of 5 km or more reaches, and every 7th tile within the CPU budget",
+            "by_largest_crater_in_reach": {
+                name: {"max": float(f"{v[0]:.3g}"), "tiles": v[3], "cells": v[2]}
+                for name, v in stats.items() if v[3] > 0}}
+
+
+def frames64(lon, lat, radius):
+    """MakeCraterFrame in FP64 (centre, east, north, radius; 10 values per crater)."""
+    lam, phi = np.radians(lon), np.radians(lat)
+    centre = radius * np.stack([np.cos(phi) * np.cos(lam), np.cos(phi) * np.sin(lam),
+                                np.sin(phi)], axis=1)
+    east = np.stack([-np.sin(lam), np.cos(lam), np.zeros_like(lam)], axis=1)
+    north = np.stack([-np.sin(phi) * np.cos(lam), -np.sin(phi) * np.sin(lam), np.cos(phi)],
+                     axis=1)
+    return np.ascontiguousarray(np.concatenate([centre, east, north,
+                                                np.full((lam.size, 1), radius)], axis=1))
 
 
 def cpu_grid(terrain, rows: int, cores: int) -> dict:
@@ -237,9 +419,9 @@ def cpu_grid(terrain, rows: int, cores: int) -> dict:
     cells = rows * g.width
     return {"value": round(cells / elapsed / 1e9, 6), "unit": "Gcells/s", "cores": cores,
             "kind": "port",
+            "cpu_model": host_cpu()[0],
             "sample": f"rows {r0}..{r0 + rows - 1} (x {g.width} cells) of the grid: source, "
-                      f"craters{', boulders' if terrain.boulders is not None else ''}, once, "
-                      f"{platform.processor() or platform.machine()}",
+                      f"craters{', boulders' if terrain.boulders is not None else ''}, once",
             "_r0": r0, "_fp32": out, "_fp64": ref}
 
 
@@ -282,6 +464,32 @@ def time_launches(args, N, dev, batch, out) -> tuple[float, float, float]:
     kernel_ms = float(np.mean([a.elapsed_ms(b) for a, b in events]))
     print(f"[bench] batched {step_ms:.3f} ms/step", file=sys.stderr, flush=True)
     return step_ms, wall_ms, kernel_ms
+
+
+def cpu_and_error(args, batch, got: np.ndarray, tiles: np.ndarray, sphere: bool) -> dict:
+    """The CPU baseline over every 7th tile within the CPU budget, the FP32 restatement's
+    difference there, and the banded FP64 error (fp64_error)."""
+    if args.no_cpu_baseline:
+        return {}
+    budget = args.cpu_cells
+    sample, total = [], 0
+    for i in range(0, tiles.size, 7):
+        n = int(tiles["width"][i]) * int(tiles["height"][i])
+        if total + n > budget:
+            continue
+        sample.append(i)
+        total += n
+    base = cpu_baseline(batch, sample, sphere)
+    fp32 = base.pop("_fp32")
+    diff32 = max(float(np.abs(batch.tile_heights(got, i) - fp32[i]).max()) for i in sample)
+    print(f"[bench] cpu baseline {base}", file=sys.stderr, flush=True)
+    err = fp64_error(batch, got, sample, sphere, base["cores"])
+    print(f"[bench] fp64 error {err}", file=sys.stderr, flush=True)
+    return {"cpu_baseline": base, "error_vs_fp64_oracle_m": err,
+            "max_diff_vs_fp32_oracle_m": float(f"{diff32:.3g}"),
+            "fp32_oracle_note": "the FP32 restatement of the reference's arithmetic; craters of "
+                                "1 km and more are evaluated here in FP64, so the difference "
+                                "there is the FP32 arithmetic's own rounding"}
 
 
 def bench_tiles(args, N, dev) -> dict:
@@ -341,28 +549,13 @@ def bench_tiles(args, N, dev) -> dict:
     n_bins, n_entries = int(batch.bins.size), int(batch.entries.size)
     value = cells / (step_ms * 1e-3) / 1e9
 
-    extra = {}
-    if not args.no_cpu_baseline:
-        cores = os.cpu_count() or 1
-        budget = args.cpu_cells
-        sample, total = [], 0
-        for i in range(0, tiles.size, 7):
-            n = int(tiles["width"][i]) * int(tiles["height"][i])
-            if total + n > budget:
-                continue
-            sample.append(i)
-            total += n
-        base = cpu_tiles(batch, sample, cores)
-        fp32, fp64 = base.pop("_fp32"), base.pop("_fp64")
-        err, diff32 = error_summary((batch.tile_heights(got, i), fp32[i], fp64[i])
-                                    for i in sample)
-        extra["cpu_baseline"] = base
-        extra["error_vs_fp64_oracle_m"] = err
-        extra["max_diff_vs_fp32_oracle_m"] = diff32
+    extra = cpu_and_error(args, batch, got, tiles, sphere=False)
 
     result = {
         "metric": METRIC, "value": round(value, 3), "unit": "Gcells/s",
         "ms_per_step": round(step_ms, 4), "wall_ms_per_step": round(wall_ms, 4),
+        "end_to_end_gcells_per_s": round(cells / ((step_ms + batch.timings["bin_ms"]) * 1e-3)
+                                         / 1e9, 3),
         "dtype": "f32+u32",
         "config": {"workload": f"{tiles.size} mixed tiles, edges 2^k+1 (k 6-11), pixel "
                                f"0.25-8 m, over a {TILES_SIDE:.0f} m square; 8-octave Perlin fBm "
@@ -390,61 +583,6 @@ def bench_tiles(args, N, dev) -> dict:
     result |= extra
     batch.close()
     return result
-
-
-def cpu_sphere_tiles(batch, sample: list[int], cores: int) -> dict:
-    """The reference's CPU path over sampled tiles on the sphere, timed by the oracle
-    restatement: the body noise (EvaluateFractals3 per cell) then the crater kernel's body branch,
-    FP32 on all host threads. Also the FP64 oracle of the same tiles (FP64 positions, frames,
-    noise and shaping), for the error of the hip result."""
-    from oracle import oracle
-    from terraingen_amd import native as N
-
-    orc = oracle()
-    elapsed = 0.0
-    cells = 0
-    fp32, fp64 = {}, {}
-    for i in sample:
-        inp = batch.plain_inputs(i)
-        g = inp["grid"]
-        params = N.KernelParams(g.width, g.height, 0, 0.0, 0)
-        pos = inp["positions"]
-        out = np.zeros((g.height, g.width), np.float32)
-        t0 = time.perf_counter()
-        orc.or_body_noise_grid(C.byref(batch.noise_kernel), pos.ctypes.data, pos.shape[0],
-                               C.byref(params), out.ctypes.data, cores)
-        if inp["craters"] is not None:
-            kernel, records, data, tiles, entries, frames, points = inp["craters"]
-            orc.or_shape_grid_body(C.byref(kernel), records.ctypes.data, data.ctypes.data,
-                                   tiles.ctypes.data, entries.ctypes.data, frames.ctypes.data,
-                                   points.ctypes.data, g.width, g.height, C.byref(params),
-                                   out.ctypes.data, cores)
-        elapsed += time.perf_counter() - t0
-        cells += g.width * g.height
-        fp32[i] = out
-        ref = np.zeros((g.height, g.width), np.float64)
-        orc.or_body_noise_grid_f64(C.byref(batch.noise_kernel), pos.ctypes.data, pos.shape[0],
-                                   ref.ctypes.data, cores)
-        if inp["craters"] is not None:
-            foot = inp["foot"]
-            lam, phi = np.radians(foot[:, 0]), np.radians(foot[:, 1])
-            frames64 = np.ascontiguousarray(np.concatenate([
-                batch.radius * np.stack([np.cos(phi) * np.cos(lam), np.cos(phi) * np.sin(lam),
-                                         np.sin(phi)], axis=1),
-                np.stack([-np.sin(lam), np.cos(lam), np.zeros_like(lam)], axis=1),
-                np.stack([-np.sin(phi) * np.cos(lam), -np.sin(phi) * np.sin(lam), np.cos(phi)],
-                         axis=1),
-                np.full((lam.size, 1), batch.radius)], axis=1))
-            orc.or_shape_grid_body_f64(C.byref(kernel), records.ctypes.data, data.ctypes.data,
-                                       tiles.ctypes.data, entries.ctypes.data,
-                                       frames64.ctypes.data, pos.ctypes.data, g.width, g.height,
-                                       0, ref.ctypes.data, cores)
-        fp64[i] = ref
-    return {"value": round(cells / elapsed / 1e9, 6), "unit": "Gcells/s", "cores": cores,
-            "kind": "port",
-            "sample": f"{len(sample)} of the tiles ({cells} cells), body noise then craters, "
-                      f"once each, {platform.processor() or platform.machine()}",
-            "_fp32": fp32, "_fp64": fp64}
 
 
 def bench_sphere(args, N, dev) -> dict:
@@ -505,29 +643,14 @@ def bench_sphere(args, N, dev) -> dict:
                                                     batch.foot, cost, MOON)
     value = cells / (step_ms * 1e-3) / 1e9
 
-    extra = {}
-    if not args.no_cpu_baseline:
-        cores = os.cpu_count() or 1
-        budget = args.cpu_cells
-        sample, total = [], 0
-        for i in range(0, tiles.size, 7):
-            n = int(tiles["width"][i]) * int(tiles["height"][i])
-            if total + n > budget:
-                continue
-            sample.append(i)
-            total += n
-        base = cpu_sphere_tiles(batch, sample, cores)
-        fp32, fp64 = base.pop("_fp32"), base.pop("_fp64")
-        err, diff32 = error_summary((batch.tile_heights(got, i), fp32[i], fp64[i])
-                                    for i in sample)
-        extra["cpu_baseline"] = base
-        extra["error_vs_fp64_oracle_m"] = err
-        extra["max_diff_vs_fp32_oracle_m"] = diff32
+    extra = cpu_and_error(args, batch, got, tiles, sphere=True)
 
     d_min = sphere_min_diameter(GLOBAL_CRATERS, MOON, 0.05, 2.0, GLOBAL_MAX_D)
     result = {
         "metric": METRIC, "value": round(value, 3), "unit": "Gcells/s",
         "ms_per_step": round(step_ms, 4), "wall_ms_per_step": round(wall_ms, 4),
+        "end_to_end_gcells_per_s": round(cells / ((step_ms + batch.timings["bin_ms"]) * 1e-3)
+                                         / 1e9, 3),
         "dtype": "f32+f64+u32",
         "config": {"workload": f"{tiles.size} mixed tiles on a 1737.4 km sphere, edges 2^k+1 "
                                "(k 6-11), pixel 0.25-8 m, north-west corners uniform on the "
@@ -552,8 +675,9 @@ def bench_sphere(args, N, dev) -> dict:
             "kernel": "BatchBodyTiles", "ops_noise": ops_noise, "ops_craters": ops_craters,
             "note": "achieved counts the reference's algorithm (eight corner hashes per 3D "
                     "octave); the kernel hashes each lattice point once per wave where a "
-                    "square's corners fit a 4x4x4 block, so it can exceed the peak. "
-                    "valu_issue_frac is the fraction of the peak the hardware issued"},
+                    "square's corners fit a 4x4x4 block, so it issues fewer instructions than "
+                    "that count. executed_frac is what the hardware issued against the issue "
+                    "rate the same counter reads for the VALU probe"},
     }
     if tiles.size == 1024 and (counted := counted_traffic(
             kernel_ms, "batch_body_tiles_c4_sphere_1024_counters.md")) is not None:

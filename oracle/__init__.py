@@ -146,6 +146,10 @@ def _declare(lib: C.CDLL) -> None:
         "ref_sphere_sizes": (C.c_long, [vp, C.POINTER(C.c_long)]),
         "ref_sphere_copy": (None, [vp, vp, vp, vp, vp, vp, vp]),
         "ref_sphere_error": (C.c_char_p, []),
+        "ref_body_tile": (C.c_int, [vp, sz, d, d, d, u, u, d, vp, vp, vp, vp, vp, vp, C.c_uint,
+                                    vp]),
+        "ref_plane_tile": (C.c_int, [vp, sz, d, d, d, u, u, vp, vp, vp, vp, vp, C.c_uint, vp]),
+        "ref_bench_error": (C.c_char_p, []),
     }
     for name, (res, args) in sigs.items():
         fn = getattr(lib, name, None)

@@ -336,28 +336,55 @@ class TileBatch:
         return out_host[o:o + n].reshape(int(d["height"]), int(d["width"]))
 
     # ---- the plain per-tile path ---------------------------------------------------------
+    def tile_entries(self, i: int) -> np.ndarray:
+        """Catalogue rows (apply order) that tile i's bins list, each once, in order."""
+        d = self.descs[i]
+        nb = int(d["bins_across"]) * ((int(d["height"]) + 15) // 16)
+        first = self.bins[int(d["bin_first"])]
+        last = self.bins[int(d["bin_first"]) + nb - 1]
+        # A tile's bins list their entries one after another.
+        return np.unique(self.entries[int(first["first"]):int(last["first"]) +
+                                      int(last["count"])])
+
+    def largest_in_reach(self) -> np.ndarray:
+        """Per tile, the largest diameter among the craters its bins list (0 when none)."""
+        out = np.zeros(self.descs.size)
+        for i in range(self.descs.size):
+            idx = self.tile_entries(i)
+            if idx.size:
+                out[i] = float(self.foot[idx, 3].max())
+        return out
+
+    def tile_spacing(self, i: int) -> float:
+        """The cell spacing tile i's craters are prepared at (meters)."""
+        return float(self.tiles["pixel"][i])
+
+    def tile_craters(self, i: int):
+        """Tile i's craters as the reference's per-grid path prepares them, at its spacing
+        (resolution baked into the records): (kernel, records, data, footprint) or None."""
+        idx = self.tile_entries(i)
+        if idx.size == 0:
+            return None
+        spec = self.craters
+        return N.prepare_craters(self.ordered[idx], spec.models(), spec.apply(), spec.seed,
+                                 self.tile_spacing(i))
+
     def plain_inputs(self, i: int) -> dict:
         """Tile i's inputs as the reference's per-grid path builds them: its craters prepared at
         its pixel size (resolution baked into the records), binned into its 16 x 16 tiles."""
         t = self.tiles[i]
         grid = N.Grid(int(t["width"]), int(t["height"]), float(t["origin_x"]),
                       float(t["origin_y"]), float(t["pixel"]))
-        d = self.descs[i]
-        nb = int(d["bins_across"]) * ((int(d["height"]) + 15) // 16)
-        first = self.bins[int(d["bin_first"])]
-        last = self.bins[int(d["bin_first"]) + nb - 1]
-        # A tile's bins list their entries one after another.
-        idx = np.unique(self.entries[int(first["first"]):int(last["first"]) + int(last["count"])])
         cx64, cy64 = N.grid_centers(grid)
         xs, xlo = N.split(cx64)
         ys, ylo = N.split(cy64)
         inputs = dict(grid=grid, xs=xs, ys=ys, lows=np.concatenate([xlo, ylo]), craters=None)
-        if idx.size > 0:
-            spec = self.craters
-            kernel, records, data, foot = N.prepare_craters(
-                self.ordered[idx], spec.models(), spec.apply(), spec.seed, grid.pixel)
+        prepared = self.tile_craters(i)
+        if prepared is not None:
+            kernel, records, data, foot = prepared
             tiles, entries = N.bin_plane(grid, foot[:, 0], foot[:, 1], foot[:, 2], 16)
             inputs["craters"] = (kernel, records, data, tiles, entries)
+            inputs["foot"] = foot
         return inputs
 
     def plain_run(self, inputs: dict) -> np.ndarray:
@@ -475,6 +502,20 @@ class SphereTileBatch(TileBatch):
                           self.tiles["height"].astype(np.int64)).sum())
         self.timings.update(bin_ms=(t1 - t0) * 1e3, stage_ms=(t2 - t1) * 1e3)
 
+    def tile_spacing(self, i: int) -> float:
+        """RasterGrid::Spacing of tile i: its pixel (degrees) times the unit length."""
+        return float(self.tiles["pixel"][i]) * self.unit_length
+
+    def band_lists(self, foot: np.ndarray, origin_x: float, origin_y: float, pixel: float,
+                   width: int, height: int):
+        """The 16 x 16 tile lists on a body (``[tilesX, 16, 1, (first, count)...]``) and entries
+        of a grid of `width` x `height` cells from (origin_x, origin_y), degrees, for craters of
+        footprint `foot`."""
+        one = np.zeros(1, N.TILE_DTYPE)
+        one[0] = (origin_x, origin_y, pixel, width, height)
+        descs, bins, entries = N.sphere_bin(foot, self.radius, one, 1)
+        return body_tile_lists(descs, bins, entries)
+
     def plain_inputs(self, i: int) -> dict:
         """Tile i's inputs as the reference's per-grid path on a body builds them: its craters
         prepared at its spacing (RasterGrid::Spacing: pixel times the unit length), binned into
@@ -482,22 +523,12 @@ class SphereTileBatch(TileBatch):
         t = self.tiles[i]
         grid = N.Grid(int(t["width"]), int(t["height"]), float(t["origin_x"]),
                       float(t["origin_y"]), float(t["pixel"]))
-        d = self.descs[i]
-        nb = int(d["bins_across"]) * ((int(d["height"]) + 15) // 16)
-        first = self.bins[int(d["bin_first"])]
-        last = self.bins[int(d["bin_first"]) + nb - 1]
-        idx = np.unique(self.entries[int(first["first"]):int(last["first"]) + int(last["count"])])
         inputs = dict(grid=grid, positions=N.sphere_positions(grid, self.radius), craters=None)
-        if idx.size > 0:
-            spec = self.craters
-            spacing = grid.pixel * self.unit_length
-            kernel, records, data, foot = N.prepare_craters(
-                self.ordered[idx], spec.models(), spec.apply(), spec.seed, spacing)
-            one = np.zeros(1, N.TILE_DTYPE)
-            for k in ("origin_x", "origin_y", "pixel", "width", "height"):
-                one[k] = t[k]
-            descs, bins, entries = N.sphere_bin(foot, self.radius, one, 1)
-            tiles, entries = body_tile_lists(descs, bins, entries)
+        prepared = self.tile_craters(i)
+        if prepared is not None:
+            kernel, records, data, foot = prepared
+            tiles, entries = self.band_lists(foot, grid.origin_x, grid.origin_y, grid.pixel,
+                                             grid.width, grid.height)
             frames = N.sphere_frames(foot[:, 0], foot[:, 1], self.radius)
             inputs["craters"] = (kernel, records, data, tiles, entries, frames,
                                  N.sphere_split_points(grid, self.radius))
