@@ -77,7 +77,18 @@ typedef struct tgx_generation {
     double lifetime_gyr;
     uint32_t seed;
     uint32_t model;
+    /* 0: a plane (the region in meters); else a sphere of this radius in meters, the region in
+     * longitude and latitude degrees, cells the reference's cube-sphere cells. */
+    double sphere_radius;
+    /* Cells drawn on this many threads (0: every hardware thread); the list is the same on any. */
+    uint32_t threads;
+    uint32_t pad;
 } tgx_generation;
+
+/* The threads host routines (placement, preparation, binning) run on when their own count is 0:
+ * 0 (the default) for every hardware thread. Results never depend on it. No counterpart in the
+ * reference, whose routines take a thread count each (ParallelFor, src/util/parallel.cpp:126). */
+void tgx_set_host_threads(uint32_t threads);
 
 /* Philox4x32-10 of a 4-word counter under a 2-word key (include/terraingen/random/philox.h:79-91),
  * the generator every placement draw comes from. */
@@ -86,6 +97,16 @@ void tgx_philox4x32(const uint32_t* counter, const uint32_t* key, uint32_t* out)
 /* The craters in band, cell and index order, bit-identical to the reference's. */
 int tgx_generate_craters(const tgx_generation* spec, tgx_crater** craters, size_t* count,
                          char* err, size_t err_len);
+
+/* On a sphere, the union of the lists of the spec's region and `region_count` more regions
+ * (min_lon, min_lat, max_lon, max_lat each): each cell of a band is drawn once, and a crater is
+ * kept when any region's generation keeps it; in band, cell and index order. No single
+ * counterpart in the reference; its GenerateCraters (population.cpp:414-424) per region gives the
+ * same craters. For catalogues along a path, such as a camera's track, that a box would
+ * overcover. */
+int tgx_generate_craters_regions(const tgx_generation* spec, const double* more_regions,
+                                 size_t region_count, tgx_crater** craters, size_t* count,
+                                 char* err, size_t err_len);
 
 /* The crater models a layer shapes with (include/terraingen/layers/craters/crater_layer.h:29-46).
  * `model` 0: `profile` TG_PROFILE_MPD or TG_PROFILE_FASSETT_THOMSON, complex craters

@@ -12,6 +12,8 @@ import ctypes as C
 import os
 from pathlib import Path
 
+import numpy as np
+
 HERE = Path(__file__).resolve().parent
 ORACLE_SO = HERE / "liboracle.so"
 REF_SO = HERE / "_ref" / "libtgref.so"
@@ -86,6 +88,30 @@ class RefSphereCase(C.Structure):
         ("manual", (C.c_double * 4) * 4)]
 
 
+# ref_placement.cpp's ref_sphere_generation: GenerateCraters on a sphere's cube-sphere cells.
+RefSphereGeneration = type("RefSphereGeneration", (C.Structure,), {"_fields_": [
+    (n, C.c_double) for n in ("radius", "min_lon", "min_lat", "max_lon", "max_lat",
+                              "sfd_coefficient", "sfd_exponent", "min_d", "max_d",
+                              "margin_radii", "min_resolved", "lifetime")] + [
+    ("seed", C.c_uint32), ("threads", C.c_uint32)]})
+
+
+def reference_sphere_craters(lib: C.CDLL, **spec) -> np.ndarray:
+    """The reference's crater list on a sphere (ref_generate_sphere), in CRATER_DTYPE."""
+    from terraingen_amd import native as N
+
+    g = RefSphereGeneration(**spec)
+    n = lib.ref_generate_sphere(C.byref(g), None, None, None, None, None, 0)
+    if n < 0:
+        raise RuntimeError(lib.ref_placement_error().decode())
+    cols = [np.zeros(n) for _ in range(4)] + [np.zeros(n, np.uint64)]
+    lib.ref_generate_sphere(C.byref(g), *[c.ctypes.data for c in cols], n)
+    out = np.zeros(n, N.CRATER_DTYPE)
+    for name, col in zip(("x", "y", "diameter", "age_gyr", "id"), cols):
+        out[name] = col
+    return out
+
+
 def _declare(lib: C.CDLL) -> None:
     from terraingen_amd import native as N
 
@@ -150,6 +176,9 @@ def _declare(lib: C.CDLL) -> None:
                                     vp]),
         "ref_plane_tile": (C.c_int, [vp, sz, d, d, d, u, u, vp, vp, vp, vp, vp, C.c_uint, vp]),
         "ref_bench_error": (C.c_char_p, []),
+        "ref_generate_sphere": (C.c_long, [C.POINTER(RefSphereGeneration), vp, vp, vp, vp, vp,
+                                           C.c_long]),
+        "ref_placement_error": (C.c_char_p, []),
     }
     for name, (res, args) in sigs.items():
         fn = getattr(lib, name, None)

@@ -7,13 +7,17 @@
 #include "terraingen_amd.h"
 
 #include <algorithm>
+#include <atomic>
 #include <cmath>
 #include <cstddef>
 #include <cstdint>
 #include <cstdio>
 #include <cstring>
+#include <exception>
+#include <mutex>
 #include <stdexcept>
 #include <string>
+#include <thread>
 #include <vector>
 
 namespace tgh {
@@ -110,6 +114,83 @@ template <class F> int Guard(char* err, std::size_t len, F body) {
     } catch (const std::exception& e) {
         SetErr(err, len, e.what());
         return 2;
+    }
+}
+
+// The thread count host routines use when asked for 0 ("all"): tgx_set_host_threads, else every
+// hardware thread.
+inline std::atomic<unsigned>& HostThreads() {
+    static std::atomic<unsigned> threads{0};
+    return threads;
+}
+
+// Runs task(n) for n in [0, count) on `threads` threads (0: HostThreads' count).
+template <class Task>
+void ForEachIndex(std::size_t count, unsigned threads, const Task& task) {
+    if (threads == 0) {
+        threads = HostThreads().load();
+    }
+    if (threads == 0) {
+        threads = std::max(1u, std::thread::hardware_concurrency());
+    }
+    threads = static_cast<unsigned>(std::min<std::size_t>(threads, count));
+    if (threads <= 1) {
+        for (std::size_t n = 0; n < count; ++n) {
+            task(n);
+        }
+        return;
+    }
+    std::atomic<std::size_t> next{0};
+    std::exception_ptr failure;
+    std::mutex failed;
+    std::vector<std::thread> pool;
+    for (unsigned t = 0; t < threads; ++t) {
+        pool.emplace_back([&] {
+            for (std::size_t n = next++; n < count; n = next++) {
+                try {
+                    task(n);
+                } catch (...) {
+                    std::lock_guard<std::mutex> lock(failed);
+                    if (!failure) {
+                        failure = std::current_exception();
+                    }
+                    next = count;  // the rest are skipped
+                }
+            }
+        });
+    }
+    for (std::thread& th : pool) {
+        th.join();
+    }
+    if (failure) {
+        std::rethrow_exception(failure);
+    }
+}
+
+// Sorts `v` by `less`, which must be a strict total order (so the result is the one std::sort
+// gives): runs sorted on threads, then merged pairwise, each level's merges on threads.
+template <class T, class Less>
+void ParallelSort(std::vector<T>& v, const Less& less, unsigned threads) {
+    constexpr std::size_t kRun = std::size_t{1} << 16;
+    const std::size_t n = v.size();
+    if (n <= kRun) {
+        std::sort(v.begin(), v.end(), less);
+        return;
+    }
+    const auto at = [&](std::vector<T>& w, std::size_t k) {
+        return w.begin() + static_cast<std::ptrdiff_t>(std::min(k, n));
+    };
+    ForEachIndex((n + kRun - 1) / kRun, threads, [&](std::size_t r) {
+        std::sort(at(v, r * kRun), at(v, (r + 1) * kRun), less);
+    });
+    std::vector<T> other(n);
+    for (std::size_t width = kRun; width < n; width *= 2) {
+        ForEachIndex((n + 2 * width - 1) / (2 * width), threads, [&](std::size_t p) {
+            const std::size_t lo = p * 2 * width;
+            std::merge(at(v, lo), at(v, lo + width), at(v, lo + width), at(v, lo + 2 * width),
+                       at(other, lo), less);
+        });
+        v.swap(other);
     }
 }
 

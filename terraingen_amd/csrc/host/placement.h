@@ -3,6 +3,7 @@
 // Host placement of craters and boulder candidates (placement.cpp).
 #pragma once
 
+#include "cube_cells.h"
 #include "lunar_models.h"
 #include "random.h"
 
@@ -95,6 +96,7 @@ struct GenerationOptions {
     double cellSizeFactor = 16.0;
     double minResolvedDiameter = 0.0;
     double maxExpectedCraters = 5e7;
+    unsigned threads = 1;  // cells drawn on this many threads (0: every hardware thread)
 };
 
 struct Crater {
@@ -105,5 +107,12 @@ struct Crater {
 std::vector<Crater> GenerateCraters(double minX, double minY, double maxX, double maxY,
                                     const Density& density, const Ages& ages,
                                     std::uint32_t seed, const GenerationOptions& options);
+
+// The same on a sphere of `radius` meters, over longitude-latitude regions (degrees), in
+// cube-sphere cells (GenerateCraters with CubeSphereCells, population.cpp:414-424): for one
+// region the reference's list; for several, the union of theirs, each cell drawn once.
+std::vector<Crater> GenerateCratersOnSphere(double radius, const std::vector<Box>& regions,
+                                            const Density& density, const Ages& ages,
+                                            std::uint32_t seed, const GenerationOptions& options);
 
 }  // namespace tgh

@@ -24,6 +24,7 @@
 
 #include <algorithm>
 #include <cstdlib>
+#include <limits>
 #include <map>
 #include <memory>
 #include <numeric>
@@ -313,48 +314,52 @@ tg_crater_shape MakeShapeKernel(const CraterModels& m, const ApplyOptions& o) {
     return k;
 }
 
-std::vector<std::size_t> ApplyOrder(const std::vector<tgh::Crater>& craters) {
-    std::vector<std::size_t> index(craters.size());
-    std::iota(index.begin(), index.end(), std::size_t{0});
-    std::sort(index.begin(), index.end(), [&](std::size_t a, std::size_t b) {
-        const tgh::Crater& p = craters[a];
-        const tgh::Crater& q = craters[b];
-        if (p.ageGyr != q.ageGyr) {
-            return p.ageGyr > q.ageGyr;
+std::vector<std::size_t> ApplyOrder(const std::vector<tgh::Crater>& craters, unsigned threads) {
+    struct Key {
+        double age, diameter;
+        std::uint64_t id;
+        std::size_t index;
+    };
+    std::vector<Key> keys(craters.size());
+    for (std::size_t k = 0; k < craters.size(); ++k) {
+        keys[k] = {craters[k].ageGyr, craters[k].diameter, craters[k].id, k};
+    }
+    ParallelSort(keys, [](const Key& p, const Key& q) {
+        if (p.age != q.age) {
+            return p.age > q.age;
         }
         if (p.diameter != q.diameter) {
             return p.diameter > q.diameter;
         }
-        return p.id != q.id ? p.id < q.id : a < b;
-    });
+        return p.id != q.id ? p.id < q.id : p.index < q.index;
+    }, threads);
+    std::vector<std::size_t> index(craters.size());
+    for (std::size_t k = 0; k < keys.size(); ++k) {
+        index[k] = keys[k].index;
+    }
     return index;
 }
 
-Batch PrepareBatch(const std::vector<tgh::Crater>& craters, const CraterModels& m,
-                   const ApplyOptions& o, double spacing) {
-    if (!(spacing > 0.0)) {
-        throw std::invalid_argument("Sample spacing must be positive");
+namespace {
+
+// One crater's record and data, appended to `b` (CraterLayer's preparation of a crater:
+// PrepareCraters, prepared.cpp:60-87, then MakeCraterKernelBatch's record and data,
+// crater_kernel.cpp:94-283). `tables`, `tableOffsets` and `surface` are the default models' shared
+// diffusion state (null otherwise).
+struct CraterPrep {
+    const CraterModels& m;
+    const ApplyOptions& o;
+    double spacing, reachRadii;
+    LunarSurface* surface;
+    DiffusionTables* tables;
+    std::map<std::pair<double, double>, std::uint32_t>* tableOffsets;
+
+    // The crater's resolution weight at the spacing: 0 drops it.
+    double Resolution(const tgh::Crater& c) const {
+        return 1.0 - FalloffSmoothstep(c.diameter / spacing, 2.0, 4.0);
     }
-    Batch b;
-    b.kernel = MakeShapeKernel(m, o);
-    const double reachRadii = CraterReachRadii(m, o);
-    // The default models' steady state and diffusion tables (ProfileTables::AppendAges puts the
-    // tables' ages first in the data, crater_kernel.cpp:215-222).
-    std::unique_ptr<LunarSurface> surface;
-    DiffusionTables tables;
-    std::map<std::pair<double, double>, std::uint32_t> tableOffsets;
-    if (m.diffused) {
-        surface = std::make_unique<LunarSurface>();
-        for (const double age : DiffusionAges()) {
-            b.data.push_back(static_cast<float>(age));
-        }
-    }
-    for (const std::size_t k : ApplyOrder(craters)) {
-        const tgh::Crater& c = craters[k];
-        const double resolution = 1.0 - FalloffSmoothstep(c.diameter / spacing, 2.0, 4.0);
-        if (resolution == 0.0) {
-            continue;
-        }
+
+    void Append(const tgh::Crater& c, double resolution, Batch& b) const {
         const Outline outline = IrregularOutline(c, o.irregularity);
         const Morphometry fresh = LunarFresh(c.diameter);
         Morphometry current;
@@ -367,7 +372,7 @@ Batch PrepareBatch(const std::vector<tgh::Crater>& craters, const CraterModels& 
                 throw std::domain_error("Crater age must be finite and not negative");
             }
             const Morphometry2 f{fresh.depthRatio, fresh.rimHeightRatio};
-            const double cut = tables.AgeAtDepthRatio(f, m.cutoffDepthRatio);
+            const double cut = tables->AgeAtDepthRatio(f, m.cutoffDepthRatio);
             const double kappaT = 1.0 * (cut * c.diameter * c.diameter * c.ageGyr /
                                          surface->EquilibriumLifetimeGyr(c.diameter));
             if (!(kappaT >= 0.0 && std::isfinite(kappaT))) {
@@ -375,7 +380,7 @@ Batch PrepareBatch(const std::vector<tgh::Crater>& craters, const CraterModels& 
                     "A crater's diffusion age must be finite and not negative");
             }
             tau = kappaT / (c.diameter * c.diameter);
-            const Morphometry2 worn = tables.MorphometryAt(f, tau);
+            const Morphometry2 worn = tables->MorphometryAt(f, tau);
             current = {worn.depthRatio, worn.rimHeightRatio};
         } else {
             // FreshnessDegradation::State.
@@ -424,6 +429,8 @@ Batch PrepareBatch(const std::vector<tgh::Crater>& craters, const CraterModels& 
         r.table_offset = static_cast<std::uint32_t>(b.data.size());
         r.mark_count = 0;
         r.mark_offset = static_cast<std::uint32_t>(b.data.size());
+        r.terrace_offset = kUnset;
+        r.peak_offset = kUnset;
         AppendTerraces(c.id, c.diameter, fresh.depthRatio, fresh.rimHeightRatio,
                        current.depthRatio, current.rimHeightRatio, o.terraces, b.kernel, r,
                        b.data);
@@ -436,9 +443,9 @@ Batch PrepareBatch(const std::vector<tgh::Crater>& craters, const CraterModels& 
             const Morphometry2 f{fresh.depthRatio, fresh.rimHeightRatio};
             const auto key = std::make_pair(f.depthRatio, f.rimHeightRatio);
             const auto [at, added] =
-                tableOffsets.try_emplace(key, static_cast<std::uint32_t>(b.data.size()));
+                tableOffsets->try_emplace(key, static_cast<std::uint32_t>(b.data.size()));
             if (added) {
-                const std::vector<float>& heights = tables.TableFor(f).heights;
+                const std::vector<float>& heights = tables->TableFor(f).heights;
                 b.data.insert(b.data.end(), heights.begin(), heights.end());
             }
             r.profile_offset = at->second;
@@ -448,6 +455,100 @@ Batch PrepareBatch(const std::vector<tgh::Crater>& craters, const CraterModels& 
         b.cy.push_back(c.y);
         b.reach.push_back(outline.MaxScale() * reachRadii * (0.5 * c.diameter));
         b.diameter.push_back(c.diameter);
+    }
+
+    // An offset nothing set: terraces and peaks set theirs only when the crater has them.
+    static constexpr std::uint32_t kUnset = 0xFFFFFFFFu;
+};
+
+// Craters per chunk of a parallel preparation.
+constexpr std::size_t kPrepareChunk = 4096;
+
+}  // namespace
+
+Batch PrepareBatch(const std::vector<tgh::Crater>& craters, const CraterModels& m,
+                   const ApplyOptions& o, double spacing, unsigned threads) {
+    if (!(spacing > 0.0)) {
+        throw std::invalid_argument("Sample spacing must be positive");
+    }
+    Batch b;
+    b.kernel = MakeShapeKernel(m, o);
+    // The default models' steady state and diffusion tables (ProfileTables::AppendAges puts the
+    // tables' ages first in the data, crater_kernel.cpp:215-222).
+    std::unique_ptr<LunarSurface> surface;
+    DiffusionTables tables;
+    std::map<std::pair<double, double>, std::uint32_t> tableOffsets;
+    if (m.diffused) {
+        surface = std::make_unique<LunarSurface>();
+        for (const double age : DiffusionAges()) {
+            b.data.push_back(static_cast<float>(age));
+        }
+    }
+    const CraterPrep prep{m, o, spacing, CraterReachRadii(m, o), surface.get(), &tables,
+                          &tableOffsets};
+    std::vector<std::size_t> order;
+    for (const std::size_t k : ApplyOrder(craters, threads)) {
+        if (prep.Resolution(craters[k]) != 0.0) {
+            order.push_back(k);
+        }
+    }
+    const auto finish = [](tg_crater_record& r, std::uint32_t base) {
+        r.harmonic_offset += base;
+        r.table_offset += base;
+        r.mark_offset += base;
+        r.terrace_offset = r.terrace_offset == CraterPrep::kUnset ? 0u : r.terrace_offset + base;
+        r.peak_offset = r.peak_offset == CraterPrep::kUnset ? 0u : r.peak_offset + base;
+    };
+    // The default models share diffusion tables in first-use order: one crater after another.
+    // Otherwise each crater's record and data depend on it alone, so chunks of the apply order
+    // are prepared on threads and joined in order, their data offsets moved past what precedes.
+    if (m.diffused || order.size() <= kPrepareChunk || threads == 1) {
+        for (const std::size_t k : order) {
+            prep.Append(craters[k], prep.Resolution(craters[k]), b);
+            finish(b.records.back(), 0u);
+        }
+    } else {
+        const std::size_t chunks = (order.size() + kPrepareChunk - 1) / kPrepareChunk;
+        std::vector<Batch> parts(chunks);
+        ForEachIndex(chunks, threads, [&](std::size_t n) {
+            Batch& part = parts[n];
+            part.kernel = b.kernel;
+            const std::size_t end = std::min(order.size(), (n + 1) * kPrepareChunk);
+            for (std::size_t i = n * kPrepareChunk; i < end; ++i) {
+                const tgh::Crater& c = craters[order[i]];
+                prep.Append(c, prep.Resolution(c), part);
+            }
+        });
+        // Each part's place in the joined batch, then the parts copied there on threads.
+        std::vector<std::size_t> recordAt(chunks + 1, 0), dataAt(chunks + 1, b.data.size());
+        for (std::size_t n = 0; n < chunks; ++n) {
+            recordAt[n + 1] = recordAt[n] + parts[n].records.size();
+            dataAt[n + 1] = dataAt[n] + parts[n].data.size();
+        }
+        if (dataAt[chunks] > std::numeric_limits<std::uint32_t>::max()) {
+            throw std::length_error("crater data exceeds 2^32 floats");
+        }
+        b.records.resize(recordAt[chunks]);
+        b.data.resize(dataAt[chunks]);
+        for (std::vector<double>* column : {&b.cx, &b.cy, &b.reach, &b.diameter}) {
+            column->resize(recordAt[chunks]);
+        }
+        ForEachIndex(chunks, threads, [&](std::size_t n) {
+            Batch& part = parts[n];
+            const auto base = static_cast<std::uint32_t>(dataAt[n]);
+            const auto at = static_cast<std::ptrdiff_t>(recordAt[n]);
+            for (tg_crater_record& r : part.records) {
+                finish(r, base);
+            }
+            std::copy(part.records.begin(), part.records.end(), b.records.begin() + at);
+            std::copy(part.data.begin(), part.data.end(),
+                      b.data.begin() + static_cast<std::ptrdiff_t>(dataAt[n]));
+            std::copy(part.cx.begin(), part.cx.end(), b.cx.begin() + at);
+            std::copy(part.cy.begin(), part.cy.end(), b.cy.begin() + at);
+            std::copy(part.reach.begin(), part.reach.end(), b.reach.begin() + at);
+            std::copy(part.diameter.begin(), part.diameter.end(), b.diameter.begin() + at);
+            part = Batch{};
+        });
     }
     if (b.data.empty()) {
         b.data.push_back(0.0f);
@@ -476,7 +577,7 @@ int tgx_prepare_craters(const tgx_crater* craters, size_t n, const tgx_crater_mo
         }
         const tgh::CraterModels m = tgh::ModelsFrom(*models);
         const tgh::ApplyOptions o = tgh::SeededApply(tgh::ApplyFrom(*apply), layer_seed);
-        const tgh::Batch b = tgh::PrepareBatch(list, m, o, spacing);
+        const tgh::Batch b = tgh::PrepareBatch(list, m, o, spacing, 0);
         *kernel = b.kernel;
         *records = tgh::Export(b.records);
         *record_count = b.records.size();
@@ -484,6 +585,7 @@ int tgx_prepare_craters(const tgx_crater* craters, size_t n, const tgx_crater_mo
         *data_count = b.data.size();
         if (footprint != nullptr) {
             std::vector<double> f;
+            f.reserve(4 * b.records.size());
             for (std::size_t i = 0; i < b.records.size(); ++i) {
                 f.push_back(b.cx[i]);
                 f.push_back(b.cy[i]);

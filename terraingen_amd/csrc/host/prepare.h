@@ -115,8 +115,10 @@ tg_crater_shape MakeShapeKernel(const CraterModels& m, const ApplyOptions& o);
 ApplyOptions SeededApply(ApplyOptions o, std::uint32_t layerSeed);
 double CraterReachRadii(const CraterModels& m, const ApplyOptions& o);
 double MaxOutlineScale(const Irregularity& irr);
+// The craters prepared in apply order (a crater the spacing does not resolve is dropped), on
+// `threads` threads (0: every hardware thread; the batch is the same on any).
 Batch PrepareBatch(const std::vector<Crater>& craters, const CraterModels& m,
-                   const ApplyOptions& o, double spacing);
+                   const ApplyOptions& o, double spacing, unsigned threads = 0);
 
 // CraterProfile::Height of the kernel's closed-form profile (terraces.cpp).
 double ProfileHeight(const tg_crater_shape& k, double distance, double diameter,

@@ -397,24 +397,55 @@ CRATER_DTYPE = np.dtype([("x", np.float64), ("y", np.float64), ("diameter", np.f
 Generation = _struct("Generation", [(n, C.c_double) for n in (
     "min_x", "min_y", "max_x", "max_y", "sfd_coefficient", "sfd_exponent", "min_diameter",
     "max_diameter", "margin_radii", "cell_size_factor", "min_resolved_diameter",
-    "lifetime_gyr")] + [("seed", u32), ("model", u32)])
+    "lifetime_gyr")] + [("seed", u32), ("model", u32), ("sphere_radius", C.c_double),
+                        ("threads", u32), ("pad", u32)])
 
 HOST_ABI["tgx_philox4x32"] = (None, [VP, VP, VP])
+HOST_ABI["tgx_set_host_threads"] = (None, [u32])
 HOST_ABI["tgx_generate_craters"] = (C.c_int, [C.POINTER(Generation), C.POINTER(VP),
                                               C.POINTER(SZ), CP, SZ])
+HOST_ABI["tgx_generate_craters_regions"] = (C.c_int, [C.POINTER(Generation), VP, SZ,
+                                                      C.POINTER(VP), C.POINTER(SZ), CP, SZ])
 
 
-def generate_craters(spec: Generation) -> np.ndarray:
-    """``tgx_generate_craters``: a structured array of x, y, diameter, age_gyr, id."""
+def generate_craters(spec: Generation, more_regions=None) -> np.ndarray:
+    """``tgx_generate_craters``: a structured array of x, y, diameter, age_gyr, id; with
+    `more_regions` (an (n, 4) array of boxes, on a sphere), ``tgx_generate_craters_regions``."""
     lib = host_library()
     ptr, n = VP(), SZ()
     err = C.create_string_buffer(512)
-    _check(lib.tgx_generate_craters(C.byref(spec), C.byref(ptr), C.byref(n), err, 512), err)
+    if more_regions is not None and len(more_regions):
+        boxes = np.ascontiguousarray(more_regions, np.float64).reshape(-1, 4)
+        _check(lib.tgx_generate_craters_regions(C.byref(spec), boxes.ctypes.data,
+                                                boxes.shape[0], C.byref(ptr), C.byref(n),
+                                                err, 512), err)
+    else:
+        _check(lib.tgx_generate_craters(C.byref(spec), C.byref(ptr), C.byref(n), err, 512),
+               err)
     try:
-        raw = C.string_at(ptr.value, n.value * CRATER_DTYPE.itemsize)
+        out = _owned_copy(ptr, n.value * CRATER_DTYPE.itemsize).view(CRATER_DTYPE)
     finally:
         lib.tgx_free(ptr)
-    return np.frombuffer(raw, CRATER_DTYPE).copy()
+    return out
+
+
+def generate_sphere_craters(radius: float, region: tuple, sfd_coefficient: float,
+                            sfd_exponent: float, min_diameter: float, max_diameter: float,
+                            margin_radii: float = 0.0, min_resolved_diameter: float = 0.0,
+                            lifetime_gyr: float = 1.0, seed: int = 1, model: int = 0,
+                            threads: int = 0) -> np.ndarray:
+    """``tgx_generate_craters`` on a sphere of `radius` meters over a longitude-latitude
+    `region` (min_lon, min_lat, max_lon, max_lat, degrees), or the union over several (an (n, 4)
+    array): the reference's cube-sphere placement; x and y of the list are longitude and
+    latitude."""
+    boxes = np.asarray(region, np.float64).reshape(-1, 4)
+    lon0, lat0, lon1, lat1 = (float(v) for v in boxes[0])
+    return generate_craters(Generation(
+        min_x=lon0, min_y=lat0, max_x=lon1, max_y=lat1, sfd_coefficient=sfd_coefficient,
+        sfd_exponent=sfd_exponent, min_diameter=min_diameter, max_diameter=max_diameter,
+        margin_radii=margin_radii, cell_size_factor=16.0,
+        min_resolved_diameter=min_resolved_diameter, lifetime_gyr=lifetime_gyr, seed=seed,
+        model=model, sphere_radius=radius, threads=threads), boxes[1:])
 
 
 CraterModels = _struct("CraterModels", [("profile", u32), ("complex_", u32),
@@ -433,6 +464,14 @@ HOST_ABI["tgx_prepare_craters"] = (C.c_int, [
     C.POINTER(VP), C.POINTER(SZ), C.POINTER(VP), C.POINTER(SZ), C.POINTER(VP), CP, SZ])
 
 
+def _owned_copy(ptr: VP, nbytes: int) -> np.ndarray:
+    """One copy of `nbytes` at a library-allocated pointer into a new byte array."""
+    out = np.empty(nbytes, np.uint8)
+    if nbytes:
+        C.memmove(out.ctypes.data, ptr.value, nbytes)
+    return out
+
+
 def prepare_craters(craters: np.ndarray, models, apply, layer_seed: int, spacing: float):
     """``tgx_prepare_craters``: (kernel, records, data, footprint[n, 4]: x, y, reach, D)."""
     lib = host_library()
@@ -446,9 +485,9 @@ def prepare_craters(craters: np.ndarray, models, apply, layer_seed: int, spacing
                                    C.byref(rp), C.byref(rn), C.byref(dp), C.byref(dn),
                                    C.byref(fp), err, 512), err)
     try:
-        records = np.frombuffer(C.string_at(rp.value, rn.value * 144), RECORD_DTYPE).copy()
-        data = np.frombuffer(C.string_at(dp.value, dn.value * 4), np.float32).copy()
-        foot = np.frombuffer(C.string_at(fp.value, rn.value * 32), np.float64).reshape(-1, 4).copy()
+        records = _owned_copy(rp, rn.value * 144).view(RECORD_DTYPE)
+        data = _owned_copy(dp, dn.value * 4).view(np.float32)
+        foot = _owned_copy(fp, rn.value * 32).view(np.float64).reshape(-1, 4)
     finally:
         for p in (rp, dp, fp):
             lib.tgx_free(p)
@@ -488,8 +527,8 @@ def batch_bin(foot: np.ndarray, tiles: np.ndarray, threads: int = 0):
                              cx.size, tiles.ctypes.data, tiles.size, threads, C.byref(dp),
                              C.byref(bp), C.byref(bn), C.byref(ep), C.byref(en), err, 512), err)
     try:
-        descs = np.frombuffer(C.string_at(dp.value, tiles.size * 48), TILE_DESC_DTYPE).copy()
-        bins = np.frombuffer(C.string_at(bp.value, bn.value * 16), BIN_DTYPE).copy()
+        descs = _owned_copy(dp, tiles.size * 48).view(TILE_DESC_DTYPE)
+        bins = _owned_copy(bp, bn.value * 16).view(BIN_DTYPE)
         entries = np.frombuffer(C.string_at(ep.value, max(en.value, 1) * 4), np.uint32)[
             :en.value].copy()
     finally:
@@ -536,8 +575,8 @@ HOST_ABI["tgx_index_bin"] = (C.c_int, [VP, VP, SZ, C.c_uint, C.POINTER(VP), C.PO
 
 def _take_batch(lib, tiles_n, dp, bp, bn, ep, en):
     try:
-        descs = np.frombuffer(C.string_at(dp.value, tiles_n * 48), TILE_DESC_DTYPE).copy()
-        bins = np.frombuffer(C.string_at(bp.value, bn.value * 16), BIN_DTYPE).copy()
+        descs = _owned_copy(dp, tiles_n * 48).view(TILE_DESC_DTYPE)
+        bins = _owned_copy(bp, bn.value * 16).view(BIN_DTYPE)
         entries = np.frombuffer(C.string_at(ep.value, max(en.value, 1) * 4), np.uint32)[
             :en.value].copy()
     finally:

@@ -153,6 +153,49 @@ class GreatCircleStream(CameraStream):
         return out
 
 
+    def placed_swath(self, half_width: float, coefficient: float, exponent: float,
+                     min_diameter: float, max_diameter: float, margin_radii: float = 0.0,
+                     min_resolved_diameter: float = 0.0, lifetime_gyr: float = 1.0,
+                     margin: float = 2000.0, seed: int = 5, segment: float = 250.0,
+                     threads: int = 0) -> np.ndarray:
+        """The catalogue along the track as the reference places it (CRATER_DTYPE, longitude and
+        latitude in x and y): GenerateCraters on the sphere's cube-sphere cells over the
+        longitude-latitude box of each `segment` meters of the strip `half_width` meters each
+        side of the track (and `margin` beyond its ends), in one generation over the union of the
+        boxes (each cell drawn once, a crater kept when any box keeps it), keeping the craters
+        whose centres lie in the strip. The density is N(>=D) = coefficient D^-exponent per m^2
+        with ages uniform over `lifetime_gyr`."""
+        from . import native as N
+
+        boxes = []
+        for a0 in np.arange(-margin, self.path_length + margin, segment):
+            along = np.array([a0, a0, a0 + segment, a0 + segment])
+            lon, lat = self.track(along, np.array([-half_width, half_width] * 2))
+            boxes.append((lon.min(), lat.min(), lon.max(), lat.max()))
+        craters = N.generate_sphere_craters(
+            self.radius, np.array(boxes), coefficient, exponent, min_diameter, max_diameter,
+            margin_radii=margin_radii, min_resolved_diameter=min_resolved_diameter,
+            lifetime_gyr=lifetime_gyr, seed=seed, threads=threads)
+        # Along-track and cross-track position against the track's great circle.
+        start = np.array(unit_vector(self.start_lon, self.start_lat))
+        ahead = np.array(unit_vector(*self.track(np.array([1000.0]))))[:, 0]
+        normal = np.cross(start, ahead)
+        normal /= np.linalg.norm(normal)
+        p = np.stack(unit_vector(craters["x"], craters["y"]), axis=1)
+        across = np.arcsin(np.clip(p @ normal, -1.0, 1.0)) * self.radius
+        forward = np.cross(normal, start)
+        along = np.arctan2(p @ forward, p @ start) * self.radius
+        keep = (np.abs(across) <= half_width) & (along >= -margin) & (
+            along <= self.path_length + margin)
+        return craters[keep]
+
+
+def unit_vector(lon, lat):
+    """The unit vector(s) at longitude and latitude in degrees, as (x, y, z)."""
+    lam, phi = np.radians(lon), np.radians(lat)
+    return np.cos(phi) * np.cos(lam), np.cos(phi) * np.sin(lam), np.sin(phi)
+
+
 def serve(server, requests: np.ndarray, keep=None) -> dict:
     """Submits `requests` at their arrival times (a submitting thread) while another thread waits
     for them in order. Returns per-request (latency, batch, batch size) arrays and the wall time.

@@ -156,3 +156,31 @@ def test_default_models_match_the_reference_elsewhere(ref, seed, pixel, center):
     assert recs.view(np.uint8).tobytes() == records.view(np.uint8).tobytes()
     assert dat.view(np.uint32).tobytes() == data.view(np.uint32).tobytes()
     assert (recs["diffusion"] > 0).any() and (recs["profile_offset"] > 0).any()
+
+
+def test_parallel_preparation_is_byte_identical():
+    """Above a chunk's worth of craters, preparation runs in chunks on threads and joins them:
+    records (with their data offsets) and data are the bytes of the one-thread preparation, for
+    complex craters with peaks and terraces."""
+    lib = N.host_library()
+    gen = N.Generation(min_x=0.0, min_y=0.0, max_x=1500.0, max_y=1500.0,
+                       sfd_coefficient=0.0156, sfd_exponent=2.2, min_diameter=0.5,
+                       max_diameter=300.0, margin_radii=3.0, cell_size_factor=16.0,
+                       min_resolved_diameter=0.25, lifetime_gyr=1.0, seed=5)
+    craters = N.generate_craters(gen)
+    assert craters.size > 20000 and (craters["diameter"] > 150.0).any()
+    models = N.CraterModels(1, 1, 50.0, 1.0)
+    apply = N.Apply(1, 1, 1, 0, 0, 0, 0, 0, 150.0, 300.0)
+    try:
+        lib.tgx_set_host_threads(1)
+        one = N.prepare_craters(craters, models, apply, 5, 0.125)
+        lib.tgx_set_host_threads(8)
+        many = N.prepare_craters(craters, models, apply, 5, 0.125)
+    finally:
+        lib.tgx_set_host_threads(0)
+    assert bytes(one[0]) == bytes(many[0])
+    assert one[1].tobytes() == many[1].tobytes()
+    assert one[2].tobytes() == many[2].tobytes()
+    assert np.array_equal(one[3], many[3])
+    records = one[1]
+    assert (records["terrace_count"] > 0).any() and (records["peak_count"] > 0).any()
