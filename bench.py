@@ -7,9 +7,10 @@ edges 2^k + 1 with k uniform in 6..11, cells log-uniform in 0.25-8 m (in degrees
 sphere's unit length), north-west corners uniform on the sphere. Each cell's body-fixed position
 is computed in FP64 on the device. The noise is an 8-octave 3D Perlin fBm at that position
 (EvaluateFractals3: base wavelength 512 m, lacunarity 2, gain 0.5, amplitude 20 m, seed 1). A
-global catalogue of 1e6 craters follows the power-law cumulative SFD N(>D) = 0.05 D^-2 per m^2
-over the sphere (D from 1374 m to 20 km), centres uniform on the sphere, ages uniform over 1 Gyr,
-seed 4; each crater is placed by its FP64 frame and offsets are taken along the sphere. The
+global catalogue of about 1e6 craters follows the power-law cumulative SFD N(>D) = 0.05 D^-2 per
+m^2 over the sphere (D from 1374 m to 20 km), ages uniform over 1 Gyr, seed 4, placed as the
+reference places them (GenerateCraters in cube-sphere cells over the whole Moon, bit for bit);
+each crater is placed by its FP64 frame and offsets are taken along the sphere. The
 profile is Martin-Parkes-Dunstan with freshness degradation and irregular outlines. Heights are
 FP32.
 
@@ -31,7 +32,8 @@ the square.
   Moon and the tiles are longitude-latitude ones (3D body noise, frame-placed craters); with
   ``tiles`` it is a straight track on a plane. The crater catalogue (C3's population: N(>D) =
   0.0156 D^-2.2 per m^2, D 0.5-300 m, complex from 50 m with central peaks and wall terraces from
-  150 m, seed 5) covers a 2 km swath along the track; boulders are not served. It reports the
+  150 m, seed 5) covers a 2 km swath along the track, placed on the sphere as the reference places
+it (one generation over the union of the track's boxes); boulders are not served. It reports the
   p50/p99 request-to-host latency (submission to heights in the caller's memory) for 513 x 513
   tiles and for all tiles, and the served Gcells/s.
 
@@ -589,10 +591,12 @@ def bench_sphere(args, N, dev) -> dict:
     """The batched tiles on the Moon: tile origins uniform on a 1737.4 km sphere in FP64, each
     cell's noise at its body-fixed position, a global catalogue of 1e6 craters."""
     from terraingen_amd.pipeline import (CraterLayerSpec, SphereTileBatch, crater_pair_cost,
-                                         global_craters, sphere_min_diameter, sphere_tiles)
+                                         placed_global_craters, sphere_min_diameter,
+                                         sphere_tiles)
 
     t0 = time.perf_counter()
-    craters = global_craters(GLOBAL_CRATERS, MOON, max_diameter=GLOBAL_MAX_D, seed=4)
+    craters = placed_global_craters(GLOBAL_CRATERS, MOON, max_diameter=GLOBAL_MAX_D, seed=4)
+    place_ms = (time.perf_counter() - t0) * 1e3
     batch = SphereTileBatch(dev, [C2_FRACTAL], CraterLayerSpec(seed=4), region=None, radius=MOON,
                             craters_given=craters)
     batch.build()
@@ -656,8 +660,9 @@ def bench_sphere(args, N, dev) -> dict:
                                "(k 6-11), pixel 0.25-8 m, north-west corners uniform on the "
                                "sphere, FP64 body-fixed cell positions; 8-octave 3D Perlin fBm "
                                "(512 m, lac 2, gain 0.5, 20 m, seed 1) + a global catalogue of "
-                               f"{GLOBAL_CRATERS} power-law craters N(>D)=0.05 D^-2, D "
-                               f"{d_min:.0f}-{GLOBAL_MAX_D:.0f} m, seed 4",
+                               f"{craters.size} power-law craters N(>D)=0.05 D^-2, D "
+                               f"{d_min:.0f}-{GLOBAL_MAX_D:.0f} m, seed 4, placed as the "
+                               "reference places them (cube-sphere cells over the whole Moon)",
                    "tiles": int(tiles.size), "cells": cells,
                    "craters": int(batch.records.size), "bins": int(batch.bins.size),
                    "entries": int(batch.entries.size), "pairs_listed": listed,
@@ -705,8 +710,10 @@ def bench_serving(args, N, dev) -> dict:
                            terraces=1, terrace_onset=150.0, terrace_full=300.0, seed=5)
     finest = stream.finest_side / (max(stream.edges) - 1)
     if sphere:
-        craters = stream.swath_craters(1000.0, coefficient=0.0156, exponent=2.2,
-                                       min_diameter=0.5, max_diameter=300.0)
+        craters = stream.placed_swath(1000.0, 0.0156, 2.2, 0.5, 300.0,
+                                      margin_radii=N.margin_radii(spec.models(), spec.apply()),
+                                      seed=5)
+        place_ms = (time.perf_counter() - t0) * 1e3
         swath = SphereTileBatch(dev, [C2_FRACTAL], spec, region=None, finest_pixel=finest,
                                 radius=MOON, craters_given=craters)
     else:
@@ -736,7 +743,9 @@ def bench_serving(args, N, dev) -> dict:
                    "quadtree tiles, 3D body noise" if sphere
                    else "a 50 km straight track on a plane at 200 m") +
                   "; a 2 km swath of C3's craters (N(>D)=0.0156 D^-2.2, D 0.5-300 m, complex "
-                  "from 50 m with peaks, terraces from 150 m); no boulders",
+                  "from 50 m with peaks, terraces from 150 m" +
+                  (", placed as the reference places them on the sphere)" if sphere else ")") +
+                  "; no boulders",
         "requests": int(requests.size), "rate_per_s": stream.rate,
         "duration_s": stream.duration,
         "latency_513_ms": {"p50": round(percentile(lat_ms[is513], 50), 3),
@@ -751,6 +760,7 @@ def bench_serving(args, N, dev) -> dict:
         "pool_peak_bytes": int(stats["pool_peak_bytes"]),
         "levels": sorted(int(v) for v in np.unique(requests["level"])),
         "craters": int(swath.records.size), "setup_ms": round(setup_ms, 1),
+        "placement_ms": round(place_ms, 1) if sphere else None,
     }
 
 

@@ -580,6 +580,20 @@ def global_craters(count: int, radius: float = MOON_RADIUS, coefficient: float =
     return out
 
 
+def placed_global_craters(count: int, radius: float = MOON_RADIUS, coefficient: float = 0.05,
+                          exponent: float = 2.0, max_diameter: float = 20000.0, seed: int = 4,
+                          lifetime_gyr: float = 1.0, threads: int = 0) -> np.ndarray:
+    """A global catalogue as the reference places it: GenerateCraters over the whole sphere in
+    cube-sphere cells (src/layers/craters/generation/population.cpp:381-424), diameters from the
+    cumulative power law N(>=D) = coefficient D^-exponent per m^2 between ``sphere_min_diameter``
+    (about `count` craters expected) and `max_diameter`, ages uniform over `lifetime_gyr`;
+    longitude and latitude in x and y, in the reference's order."""
+    d_min = sphere_min_diameter(count, radius, coefficient, exponent, max_diameter)
+    return N.generate_sphere_craters(radius, (-180.0, -90.0, 180.0, 90.0), coefficient, exponent,
+                                     d_min, max_diameter, lifetime_gyr=lifetime_gyr, seed=seed,
+                                     threads=threads)
+
+
 def sphere_tiles(count: int, radius: float = MOON_RADIUS, seed: int = 4, min_k: int = 6,
                  max_k: int = 11, pixel_range=(0.25, 8.0)) -> np.ndarray:
     """Mixed longitude-latitude tiles on a sphere: edges 2^k + 1 with k uniform in [min_k,
