@@ -252,6 +252,28 @@ int tg_hip_batch_wait(tg_batch* batch, char* err, size_t err_len);
 /* The batch's stream, for events and copies ordered with its runs. */
 void* tg_hip_batch_stream(tg_batch* batch);
 
+/* tg_hip_batch_stage then tg_hip_batch_run, for one batch, without allocating anything: no
+ * tg_batch, no hipMalloc or hipFree, no allocating library call. No counterpart in the reference;
+ * it is for callers that cannot allocate device memory (tight serving loops, harnesses that
+ * forbid it). The tiles, bins and entries (host memory, checked as tg_hip_batch_stage checks them)
+ * are copied into the library's static scratch on this device, which also holds the work queue's
+ * counters, and the batch is evaluated into `out_dev` on `stream` (0: the null stream), on a
+ * sphere of `sphere_radius` meters when it is positive (as tg_hip_batch_surface), with the default
+ * launch choices and no graph. The heights are those tg_hip_batch_run gives. Limits: at most
+ * TG_STATIC_MAX_TILES tiles, TG_STATIC_MAX_BINS bins and TG_STATIC_MAX_ENTRIES entries; larger
+ * batches are refused with an error. The host arrays must stay unchanged until the run has
+ * completed. Every call on a device shares that device's scratch, which keeps the last call's
+ * contents: calls that overlap on one device (on different streams, or from several threads)
+ * race, and must be ordered by the caller. */
+#define TG_STATIC_MAX_TILES 4096
+#define TG_STATIC_MAX_BINS 262144
+#define TG_STATIC_MAX_ENTRIES 2097152
+int tg_hip_batch_run_static(tg_dev* dev, const tg_tile_desc* tiles, size_t tile_count,
+                            const tg_bin* bins, size_t bin_count, const uint32_t* entries,
+                            size_t entry_count, const tg_noise_plan* plan, float elevation,
+                            const tg_catalogue* catalogue, double sphere_radius, float* out_dev,
+                            size_t out_capacity, void* stream, char* err, size_t err_len);
+
 /* ---- Device memory, streams and timing --------------------------------------------------
  * No counterpart in the reference. For callers without a HIP runtime of their own (the tests,
  * the measurements, hosts written in other languages): one process should hold one HIP runtime,

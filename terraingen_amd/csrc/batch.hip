@@ -87,6 +87,18 @@ constexpr uint32_t kQueues = 8;
 constexpr uint32_t kQueueStride = 32;
 constexpr uint32_t kCountSlot = kQueues * kQueueStride;
 constexpr uint32_t kCounterWords = kCountSlot + 8;
+static_assert(kCounterWords * sizeof(uint32_t) <= tgi::kStaticCounterBytes,
+              "the static scratch holds the counters");
+
+// The counters of a batch in the static scratch: every queue offset cleared, the bin count set.
+__global__ void StaticCounters(uint32_t* counters, uint32_t binCount) {
+    if (threadIdx.x < kQueues) {
+        counters[threadIdx.x * kQueueStride] = 0u;
+    }
+    if (threadIdx.x == 0) {
+        counters[kCountSlot] = binCount;
+    }
+}
 
 __global__ void ResetQueue(uint32_t* counters) {
     if (threadIdx.x < kQueues) {
@@ -865,6 +877,53 @@ int tg_hip_batch_free(tg_batch* b) {
 
 namespace {
 
+// What a run needs to know of a checked batch: the end of its output, its largest entry and
+// whether any bin lists a crater.
+struct Checked {
+    uint64_t outEnd = 0;
+    uint64_t maxEntry = 0;
+    bool any = false;
+};
+
+// Checks every index of a batch's tiles, bins and entries against the others.
+Checked CheckBatch(const tg_tile_desc* tiles, size_t tileCount, const tg_bin* bins,
+                   size_t binCount, const uint32_t* entries, size_t entryCount) {
+    uint64_t outEnd = 0;
+    for (size_t i = 0; i < tileCount; ++i) {
+        const tg_tile_desc& t = tiles[i];
+        const uint64_t cells = uint64_t(t.width) * t.height;
+        if (t.width == 0 || t.height == 0 ||
+            t.bins_across != (t.width + kBinSide - 1) / kBinSide || !(t.pixel > 0.0) ||
+            !std::isfinite(t.origin_x) || !std::isfinite(t.origin_y) ||
+            t.out_offset > ~uint64_t(0) - cells) {
+            throw HipError{"tile " + std::to_string(i) + " is malformed"};
+        }
+        const uint64_t binsDown = (t.height + kBinSide - 1) / kBinSide;
+        if (uint64_t(t.bin_first) + uint64_t(t.bins_across) * binsDown > binCount) {
+            throw HipError{"tile " + std::to_string(i) + "'s bins run past the bin list"};
+        }
+        outEnd = std::max<uint64_t>(outEnd, t.out_offset + cells);
+    }
+    uint64_t maxEntry = 0;
+    bool any = false;
+    for (size_t i = 0; i < binCount; ++i) {
+        const tg_bin& bin = bins[i];
+        if (bin.tile >= tileCount || uint64_t(bin.first) + bin.count > entryCount) {
+            throw HipError{"bin " + std::to_string(i) + " is malformed"};
+        }
+        const tg_tile_desc& t = tiles[bin.tile];
+        const uint64_t binsDown = (t.height + kBinSide - 1) / kBinSide;
+        if (i < t.bin_first || i >= t.bin_first + uint64_t(t.bins_across) * binsDown) {
+            throw HipError{"bin " + std::to_string(i) + " lies outside its tile's bins"};
+        }
+        any = any || bin.count > 0;
+    }
+    for (size_t i = 0; i < entryCount; ++i) {
+        maxEntry = std::max<uint64_t>(maxEntry, entries[i]);
+    }
+    return {outEnd, maxEntry, entryCount > 0 && any};
+}
+
 // Validates a staging and copies it to the batch: synchronously from any host memory, or, with
 // `async`, queued on the batch's stream from memory the caller keeps unchanged until the batch's
 // next run has completed.
@@ -880,36 +939,7 @@ int Stage(tg_batch* b, const tg_tile_desc* tiles, size_t tileCount, const tg_bin
         if (tileCount > b->maxTiles || binCount > b->maxBins || entryCount > b->maxEntries) {
             throw HipError{"batch exceeds the capacities it was created with"};
         }
-        uint64_t outEnd = 0;
-        for (size_t i = 0; i < tileCount; ++i) {
-            const tg_tile_desc& t = tiles[i];
-            if (t.width == 0 || t.height == 0 || t.bins_across != (t.width + kBinSide - 1) / kBinSide ||
-                !(t.pixel > 0.0) || !std::isfinite(t.origin_x) || !std::isfinite(t.origin_y)) {
-                throw HipError{"tile " + std::to_string(i) + " is malformed"};
-            }
-            const uint64_t binsDown = (t.height + kBinSide - 1) / kBinSide;
-            if (uint64_t(t.bin_first) + uint64_t(t.bins_across) * binsDown > binCount) {
-                throw HipError{"tile " + std::to_string(i) + "'s bins run past the bin list"};
-            }
-            outEnd = std::max<uint64_t>(outEnd, t.out_offset + uint64_t(t.width) * t.height);
-        }
-        uint64_t maxEntry = 0;
-        bool any = false;
-        for (size_t i = 0; i < binCount; ++i) {
-            const tg_bin& bin = bins[i];
-            if (bin.tile >= tileCount || uint64_t(bin.first) + bin.count > entryCount) {
-                throw HipError{"bin " + std::to_string(i) + " is malformed"};
-            }
-            const tg_tile_desc& t = tiles[bin.tile];
-            const uint64_t binsDown = (t.height + kBinSide - 1) / kBinSide;
-            if (i < t.bin_first || i >= t.bin_first + uint64_t(t.bins_across) * binsDown) {
-                throw HipError{"bin " + std::to_string(i) + " lies outside its tile's bins"};
-            }
-            any = any || bin.count > 0;
-        }
-        for (size_t i = 0; i < entryCount; ++i) {
-            maxEntry = std::max<uint64_t>(maxEntry, entries[i]);
-        }
+        const Checked checked = CheckBatch(tiles, tileCount, bins, binCount, entries, entryCount);
         Check(hipSetDevice(b->dev->ordinal), "hipSetDevice");
         if (async) {
             // The pinned counts are rewritten here, so no earlier staging's copy may be pending.
@@ -953,10 +983,24 @@ int Stage(tg_batch* b, const tg_tile_desc* tiles, size_t tileCount, const tg_bin
         }
         b->tileCount = tileCount;
         b->binCount = binCount;
-        b->maxEntry = maxEntry;
-        b->anyEntries = entryCount > 0 && any;
-        b->outEnd = outEnd;
+        b->maxEntry = checked.maxEntry;
+        b->anyEntries = checked.any;
+        b->outEnd = checked.outEnd;
     });
+}
+
+// Checks a run's output and catalogue against the staged batch.
+void CheckRun(const tg_batch& b, const tg_catalogue* cat, size_t outCapacity) {
+    if (b.outEnd > outCapacity) {
+        throw HipError{"the batch's tiles run past the output buffer"};
+    }
+    if (b.anyEntries && (cat == nullptr || b.maxEntry >= cat->count)) {
+        throw HipError{"the batch's bins name craters the catalogue does not hold"};
+    }
+    if (b.radius > 0.0 && b.anyEntries && cat->frames == nullptr) {
+        throw HipError{"tiles on a sphere need the catalogue's crater frames "
+                       "(tg_hip_catalogue_frames)"};
+    }
 }
 
 }  // namespace
@@ -983,18 +1027,7 @@ int tg_hip_batch_run(tg_batch* b, const tg_noise_plan* plan, float elevation,
         return TG_ERR_ARGUMENT;
     }
     return Guard(err, errLen, [&] {
-        if (b->outEnd > outCapacity) {
-            throw HipError{"the batch's tiles run past the output buffer"};
-        }
-        if (b->anyEntries && (cat == nullptr || b->maxEntry >= cat->count)) {
-            throw HipError{"the batch's bins name craters the catalogue does not hold"};
-        }
-        if (b->radius > 0.0) {
-            if (b->anyEntries && cat->frames == nullptr) {
-                throw HipError{"tiles on a sphere need the catalogue's crater frames "
-                               "(tg_hip_catalogue_frames)"};
-            }
-        }
+        CheckRun(*b, cat, outCapacity);
         if (b->binCount == 0) {
             return;
         }
@@ -1076,6 +1109,71 @@ int tg_hip_batch_surface(tg_batch* b, double sphere_radius, char* err, size_t er
     }
     b->radius = sphere_radius;
     return TG_OK;
+}
+
+int tg_hip_batch_run_static(tg_dev* dev, const tg_tile_desc* tiles, size_t tileCount,
+                            const tg_bin* bins, size_t binCount, const uint32_t* entries,
+                            size_t entryCount, const tg_noise_plan* plan, float elevation,
+                            const tg_catalogue* cat, double sphereRadius, float* out,
+                            size_t outCapacity, void* stream, char* err, size_t errLen) {
+    if (dev == nullptr || out == nullptr || (tileCount > 0 && tiles == nullptr) ||
+        (binCount > 0 && bins == nullptr) || (entryCount > 0 && entries == nullptr)) {
+        SetErr(err, errLen, "tg_hip_batch_run_static: null argument");
+        return TG_ERR_ARGUMENT;
+    }
+    if (tileCount > TG_STATIC_MAX_TILES || binCount > TG_STATIC_MAX_BINS ||
+        entryCount > TG_STATIC_MAX_ENTRIES) {
+        SetErr(err, errLen,
+               "tg_hip_batch_run_static: the batch exceeds the static scratch (at most " +
+                   std::to_string(TG_STATIC_MAX_TILES) + " tiles, " +
+                   std::to_string(TG_STATIC_MAX_BINS) + " bins, " +
+                   std::to_string(TG_STATIC_MAX_ENTRIES) + " entries)");
+        return TG_ERR_ARGUMENT;
+    }
+    if (!(sphereRadius >= 0.0) || !std::isfinite(sphereRadius)) {
+        SetErr(err, errLen,
+               "tg_hip_batch_run_static: the radius must be 0 or positive and finite");
+        return TG_ERR_ARGUMENT;
+    }
+    if (dev->scratch == nullptr) {
+        SetErr(err, errLen, "tg_hip_batch_run_static: " + dev->scratchError);
+        return TG_ERR_DEVICE;
+    }
+    return Guard(err, errLen, [&] {
+        const Checked checked = CheckBatch(tiles, tileCount, bins, binCount, entries, entryCount);
+        // A batch view of the scratch, with the default launch choices; nothing of it is freed.
+        tg_batch b;
+        b.dev = dev;
+        b.stream = static_cast<hipStream_t>(stream);
+        b.counters = reinterpret_cast<uint32_t*>(dev->scratch);
+        b.tiles = reinterpret_cast<tg_tile_desc*>(dev->scratch + kStaticTilesAt);
+        b.bins = reinterpret_cast<tg_bin*>(dev->scratch + kStaticBinsAt);
+        b.entries = reinterpret_cast<uint32_t*>(dev->scratch + kStaticEntriesAt);
+        b.tileCount = tileCount;
+        b.binCount = binCount;
+        b.maxEntry = checked.maxEntry;
+        b.anyEntries = checked.any;
+        b.outEnd = checked.outEnd;
+        b.radius = sphereRadius;
+        CheckRun(b, cat, outCapacity);
+        if (binCount == 0) {
+            return;
+        }
+        Check(hipSetDevice(dev->ordinal), "hipSetDevice");
+        const auto put = [&](void* dst, const void* src, size_t bytes, const char* what) {
+            Check(hipMemcpyAsync(dst, src, bytes, hipMemcpyHostToDevice, b.stream), what);
+        };
+        put(b.tiles, tiles, tileCount * sizeof(tg_tile_desc), "staging tiles");
+        put(b.bins, bins, binCount * sizeof(tg_bin), "staging bins");
+        if (entryCount > 0) {
+            put(b.entries, entries, entryCount * sizeof(uint32_t), "staging entries");
+        }
+        // The counters from the previous call are cleared and the bin count set by a kernel.
+        hipLaunchKernelGGL(StaticCounters, dim3(1), dim3(64), 0, b.stream, b.counters,
+                           uint32_t(binCount));
+        Check(hipGetLastError(), "setting the static queue counters");
+        Launch(b, plan, elevation, cat, out);
+    });
 }
 
 int tg_hip_catalogue_frames(tg_catalogue* cat, const tg_crater_frame* frames, size_t count,

@@ -27,7 +27,25 @@ struct tg_dev {
     int ordinal = 0;
     std::string name;
     std::mutex mutex;  // runs are serialized, as ComputeDevice requires (src/gpu/device.h:48)
+    // This device's copy of the library's static scratch (tg_hip_batch_run_static), from
+    // hipGetSymbolAddress at tg_hip_open; null, with the reason, if that failed.
+    unsigned char* scratch = nullptr;
+    std::string scratchError;
 };
+
+namespace tgi {
+
+// The static scratch of tg_hip_batch_run_static, one per device: the queue counters, then the
+// staged tiles, bins and entries of one batch, at these byte offsets.
+constexpr size_t kStaticCounterBytes = 2048;
+constexpr size_t kStaticTilesAt = kStaticCounterBytes;
+constexpr size_t kStaticBinsAt = kStaticTilesAt + size_t(TG_STATIC_MAX_TILES) * sizeof(tg_tile_desc);
+constexpr size_t kStaticEntriesAt = kStaticBinsAt + size_t(TG_STATIC_MAX_BINS) * sizeof(tg_bin);
+constexpr size_t kStaticScratchBytes =
+    kStaticEntriesAt + size_t(TG_STATIC_MAX_ENTRIES) * sizeof(uint32_t);
+static_assert(kStaticBinsAt % 256 == 0 && kStaticEntriesAt % 256 == 0, "scratch alignment");
+
+}  // namespace tgi
 
 struct tg_crater_set {
     tg_dev* dev = nullptr;

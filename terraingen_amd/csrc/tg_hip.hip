@@ -23,6 +23,12 @@
 
 using namespace tgi;
 
+// The library's static scratch (tg_hip_batch_run_static): each device holds its own copy until the
+// process ends. Host code never passes this name to a kernel or a copy: it stands in for the
+// device address only on the device side. tg_hip_open looks up each device's address of it with
+// hipGetSymbolAddress and keeps that in tg_dev::scratch, which is what host code uses.
+static __device__ __align__(256) unsigned char scratch[kStaticScratchBytes];
+
 namespace {
 
 
@@ -290,6 +296,19 @@ int tg_hip_open(int ordinal, tg_dev** out, char* err, size_t errLen) {
         name = "AMD GPU " + std::to_string(ordinal);
     }
     dev->name = name + " (" + prop.gcnArchName + ")";
+    // This device's address of the static scratch, with the device current.
+    void* at = nullptr;
+    hipError_t found = hipSetDevice(ordinal);
+    if (found == hipSuccess) {
+        found = hipGetSymbolAddress(&at, HIP_SYMBOL(scratch));
+    }
+    if (found == hipSuccess && at != nullptr) {
+        dev->scratch = static_cast<unsigned char*>(at);
+    } else {
+        dev->scratchError = std::string("the static scratch has no device address: ") +
+                            hipGetErrorString(found);
+        (void)hipGetLastError();
+    }
     *out = dev.get();
     devices.push_back(std::move(dev));
     return TG_OK;

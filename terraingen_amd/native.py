@@ -137,6 +137,8 @@ HIP_ABI = {
     "tg_hip_memcpy_d2d_async": (C.c_int, [VP, VP, VP, SZ, VP, CP, SZ]),
     "tg_hip_batch_tune": (C.c_int, [VP, u32, u32, u32, u32, CP, SZ]),
     "tg_hip_batch_surface": (C.c_int, [VP, C.c_double, CP, SZ]),
+    "tg_hip_batch_run_static": (C.c_int, [VP, VP, SZ, VP, SZ, VP, SZ, VP, C.c_float, VP,
+                                          C.c_double, VP, SZ, VP, CP, SZ]),
     "tg_hip_catalogue_frames": (C.c_int, [VP, VP, SZ, CP, SZ]),
     "tg_hip_fill": (C.c_int, [VP, VP, C.c_float, SZ, VP, CP, SZ]),
     "tg_hip_boulder_set_create": (C.c_int, [VP, VP, VP, SZ, VP, SZ, VP, SZ, VP, SZ,
