@@ -354,6 +354,58 @@ int tgx_run_kernel(int backend, int program, const void* kernel, size_t kernel_b
 /* Empty when libtg_hip.so opens, else the loader's reason. */
 const char* tgx_hip_unavailable_reason(void);
 
+/* ---- Output files --------------------------------------------------------------------------
+ * A heightmap (or several bands of one grid) as the reference's WriteGeoTiff writes it
+ * (src/raster/geotiff/geotiff_write.cpp:221-275, GeoTiffWriteOptions in
+ * include/terraingen/raster/geotiff/geotiff.h:185-192): tiled little-endian float32, edge tiles
+ * padded with the no-data value or 0, compressed tiles through the floating-point predictor,
+ * BigTIFF when the uncompressed tiles could pass 4 GiB, the grid's transform (GDAL order:
+ * origin x, x per column, x per row, origin y, y per column, y per row) and projection keys
+ * (GeoKeyDirectory shorts, GeoDoubleParams, GeoAsciiParams, as ReadGeoTiff returns them) and the
+ * no-data value as GDAL_NODATA. With no compression or LZW the file is the reference's byte for
+ * byte; Deflate (zlib, level 6) decodes to the same values. `values` holds `bands` planes of
+ * width x height floats, row-major from the north-west cell. Tiles are encoded on `threads`
+ * threads (0: all), which does not change the file. */
+enum { TGX_TIFF_NONE = 0, TGX_TIFF_LZW = 1, TGX_TIFF_DEFLATE = 2 };
+typedef struct tgx_geotiff {
+    const char* path;
+    uint32_t width;
+    uint32_t height;
+    uint32_t bands;
+    uint32_t compression; /* TGX_TIFF_*; the reference's default is LZW */
+    uint32_t tile_size;   /* a positive multiple of 16; the reference's default is 256 */
+    uint32_t big_tiff;    /* 1: always BigTIFF */
+    uint32_t has_nodata;
+    float nodata;
+    double transform[6];
+    const uint16_t* geokeys;
+    size_t geokey_count;
+    const double* geo_doubles;
+    size_t geo_double_count;
+    const char* geo_ascii; /* null or empty for none */
+    uint32_t threads;
+    uint32_t pad;
+} tgx_geotiff;
+int tgx_write_geotiff(const tgx_geotiff* spec, const float* values, char* err, size_t err_len);
+
+/* CraterAttributes (include/terraingen/layers/craters/shaping/attributes.h:14-21). */
+typedef struct tgx_crater_attributes {
+    double floor_depth;
+    double rim_height;
+    double depth_ratio;
+    double freshness;
+    uint8_t complex_;
+    uint8_t pad[7];
+} tgx_crater_attributes;
+
+/* A crater list as the reference's WriteCraterList writes CSV (src/io/feature_lists.cpp:124-140,
+ * 431-445): columns x, y, diameter, age_gyr, id, manual, replace (each double as "%.17g", flags as
+ * 0 or 1), then floor_depth, rim_height, depth_ratio, freshness and complex when `attributes`
+ * (one per crater) is given. `manual` and `replace` may be null (all 0). */
+int tgx_write_crater_csv(const char* path, const tgx_crater* craters, size_t count,
+                         const uint8_t* manual, const uint8_t* replace,
+                         const tgx_crater_attributes* attributes, char* err, size_t err_len);
+
 #ifdef __cplusplus
 }
 #endif

@@ -179,9 +179,53 @@ def _declare(lib: C.CDLL) -> None:
         "ref_generate_sphere": (C.c_long, [C.POINTER(RefSphereGeneration), vp, vp, vp, vp, vp,
                                            C.c_long]),
         "ref_placement_error": (C.c_char_p, []),
+        "ref_write_geotiff": (C.c_int, [C.c_char_p, vp, u, u, u, vp, C.c_int, f, vp, sz, vp, sz,
+                                        C.c_char_p, C.c_int, u, C.c_int]),
+        "ref_write_crater_csv": (C.c_int, [C.c_char_p, vp, vp, vp, vp, vp, vp, vp, sz, vp, vp]),
+        "ref_output_error": (C.c_char_p, []),
     }
     for name, (res, args) in sigs.items():
         fn = getattr(lib, name, None)
         if fn is not None:
             fn.restype = res
             fn.argtypes = args
+
+
+def ref_write_geotiff(lib: C.CDLL, path, values: np.ndarray, transform, nodata=None,
+                      geokeys=None, compression: int = 1, tile_size: int = 256,
+                      big_tiff: bool = False) -> None:
+    """The reference's WriteGeoTiff of `values` ((bands,) height x width float32)."""
+    v = np.ascontiguousarray(values, np.float32)
+    bands = 1 if v.ndim == 2 else v.shape[0]
+    h, w = v.shape[-2:]
+    t = np.ascontiguousarray(transform, np.float64)
+    directory, doubles, ascii_ = geokeys if geokeys is not None else ((), (), "")
+    dk = np.ascontiguousarray(directory, np.uint16)
+    dd = np.ascontiguousarray(doubles, np.float64)
+    rc = lib.ref_write_geotiff(str(path).encode(), v.ctypes.data, w, h, bands, t.ctypes.data,
+                               int(nodata is not None),
+                               float(nodata) if nodata is not None else 0.0,
+                               dk.ctypes.data, dk.size, dd.ctypes.data, dd.size,
+                               ascii_.encode(), compression, tile_size, int(big_tiff))
+    if rc != 0:
+        raise RuntimeError(lib.ref_output_error().decode())
+
+
+def ref_write_crater_csv(lib: C.CDLL, path, craters: np.ndarray, manual=None, replace=None,
+                         attributes=None) -> None:
+    """The reference's WriteCraterList (CSV) of a CRATER_DTYPE list; `attributes` is a
+    structured array with floor_depth, rim_height, depth_ratio, freshness and complex_."""
+    cols = [np.ascontiguousarray(craters[k]) for k in ("x", "y", "diameter", "age_gyr", "id")]
+    flags = [np.ascontiguousarray(a, np.uint8) if a is not None else None
+             for a in (manual, replace)]
+    if attributes is not None:
+        attrs = np.ascontiguousarray(np.stack([attributes[k] for k in (
+            "floor_depth", "rim_height", "depth_ratio", "freshness")], axis=1), np.float64)
+        cx = np.ascontiguousarray(attributes["complex_"], np.uint8)
+    rc = lib.ref_write_crater_csv(str(path).encode(), *[c.ctypes.data for c in cols],
+                                  *[a.ctypes.data if a is not None else None for a in flags],
+                                  craters.size,
+                                  attrs.ctypes.data if attributes is not None else None,
+                                  cx.ctypes.data if attributes is not None else None)
+    if rc != 0:
+        raise RuntimeError(lib.ref_output_error().decode())

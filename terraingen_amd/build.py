@@ -61,7 +61,7 @@ def build_host() -> Path | None:
     tmp = HOST_SO.with_suffix(".so.tmp")
     _run(["g++", "-std=c++17", "-O2", "-fPIC", "-shared", "-ffp-contract=off", "-Wall",
           "-Wextra", "-pthread", f"-I{INCLUDE}", f"-I{CSRC}", f"-I{CSRC / 'host'}", *sources,
-          "-o", tmp, "-ldl"])
+          "-o", tmp, "-ldl", "-lz"])
     os.replace(tmp, HOST_SO)
     return HOST_SO
 

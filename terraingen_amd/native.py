@@ -431,6 +431,89 @@ def generate_craters(spec: Generation, more_regions=None) -> np.ndarray:
     return out
 
 
+class GeoTiffSpec(C.Structure):
+    _fields_ = [("path", C.c_char_p), ("width", u32), ("height", u32), ("bands", u32),
+                ("compression", u32), ("tile_size", u32), ("big_tiff", u32),
+                ("has_nodata", u32), ("nodata", C.c_float), ("transform", C.c_double * 6),
+                ("geokeys", VP), ("geokey_count", SZ), ("geo_doubles", VP),
+                ("geo_double_count", SZ), ("geo_ascii", C.c_char_p), ("threads", u32),
+                ("pad", u32)]
+
+
+CRATER_ATTRIBUTES_DTYPE = np.dtype([("floor_depth", np.float64), ("rim_height", np.float64),
+                                    ("depth_ratio", np.float64), ("freshness", np.float64),
+                                    ("complex_", np.uint8), ("pad", np.uint8, 7)])
+assert CRATER_ATTRIBUTES_DTYPE.itemsize == 40
+TIFF_COMPRESSION = {"none": 0, "lzw": 1, "deflate": 2}
+
+HOST_ABI["tgx_write_geotiff"] = (C.c_int, [C.POINTER(GeoTiffSpec), VP, CP, SZ])
+HOST_ABI["tgx_write_crater_csv"] = (C.c_int, [C.c_char_p, VP, SZ, VP, VP, VP, CP, SZ])
+
+
+def north_up_transform(origin_x: float, origin_y: float, pixel: float) -> tuple:
+    """GeoTransform::NorthUp in GDAL order: the north-west corner, square cells, rows south."""
+    return (origin_x, pixel, 0.0, origin_y, 0.0, -pixel)
+
+
+def sphere_geokeys(radius: float) -> tuple:
+    """GeographicSphereKeys (src/raster/geotiff/geotiff.cpp:103-120): geographic longitude and
+    latitude in degrees on a user-defined sphere of `radius` meters, pixels as areas, as
+    (directory, doubles, ascii)."""
+    if not radius > 0.0:
+        raise ValueError("A sphere's radius must be positive")
+    keys = [(1024, 0, 2), (1025, 0, 1), (2048, 0, 32767), (2050, 0, 32767), (2054, 0, 9102),
+            (2056, 0, 32767), (2057, 34736, 0), (2058, 34736, 1)]
+    directory = [1, 1, 0, len(keys)]
+    for key, location, value in keys:
+        directory += [key, location, 1, value]
+    return directory, [radius, radius], ""
+
+
+def write_geotiff(path, values: np.ndarray, transform=(0.0, 1.0, 0.0, 0.0, 0.0, 1.0),
+                  nodata=None, geokeys=None, compression: str = "lzw", tile_size: int = 256,
+                  big_tiff: bool = False, threads: int = 0) -> None:
+    """``tgx_write_geotiff``: `values` (height x width, or bands x height x width, float32) as
+    the reference's WriteGeoTiff writes them, with a GDAL-order `transform`, an optional no-data
+    value and projection keys (directory, doubles, ascii)."""
+    v = np.ascontiguousarray(values, np.float32)
+    if v.ndim not in (2, 3):
+        raise ValueError("values must be height x width or bands x height x width")
+    directory, doubles, text = geokeys if geokeys is not None else ((), (), "")
+    dk = np.ascontiguousarray(directory, np.uint16)
+    dd = np.ascontiguousarray(doubles, np.float64)
+    spec = GeoTiffSpec(path=str(path).encode(), width=v.shape[-1], height=v.shape[-2],
+                       bands=1 if v.ndim == 2 else v.shape[0],
+                       compression=TIFF_COMPRESSION[compression], tile_size=tile_size,
+                       big_tiff=int(big_tiff), has_nodata=int(nodata is not None),
+                       nodata=float(nodata) if nodata is not None else 0.0,
+                       transform=(C.c_double * 6)(*transform),
+                       geokeys=dk.ctypes.data if dk.size else None, geokey_count=dk.size,
+                       geo_doubles=dd.ctypes.data if dd.size else None,
+                       geo_double_count=dd.size, geo_ascii=text.encode() if text else None,
+                       threads=threads)
+    err = C.create_string_buffer(512)
+    _check(host_library().tgx_write_geotiff(C.byref(spec), v.ctypes.data, err, 512), err)
+
+
+def write_crater_csv(path, craters: np.ndarray, manual=None, replace=None,
+                     attributes=None) -> None:
+    """``tgx_write_crater_csv``: a CRATER_DTYPE list as the reference's crater list CSV, with
+    optional manual/replace flags and CRATER_ATTRIBUTES_DTYPE attributes."""
+    c = np.ascontiguousarray(craters, CRATER_DTYPE)
+    flags = [np.ascontiguousarray(a, np.uint8) if a is not None else None
+             for a in (manual, replace)]
+    attrs = (np.ascontiguousarray(attributes, CRATER_ATTRIBUTES_DTYPE)
+             if attributes is not None else None)
+    for a in (*flags, attrs):
+        if a is not None and a.size != c.size:
+            raise ValueError("flags and attributes must have one entry per crater")
+    err = C.create_string_buffer(512)
+    _check(host_library().tgx_write_crater_csv(
+        str(path).encode(), c.ctypes.data, c.size,
+        *[a.ctypes.data if a is not None else None for a in flags],
+        attrs.ctypes.data if attrs is not None else None, err, 512), err)
+
+
 def generate_sphere_craters(radius: float, region: tuple, sfd_coefficient: float,
                             sfd_exponent: float, min_diameter: float, max_diameter: float,
                             margin_radii: float = 0.0, min_resolved_diameter: float = 0.0,
