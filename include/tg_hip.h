@@ -39,11 +39,22 @@ enum {
     TG_KERNEL_BOULDERS = 3,    /* tg_boulder_kernel: the dome relief BoulderLayer bakes on the
                                 * CPU (boulder_relief.cpp:426-440); extras: boulders, tiles,
                                 * entries, lows */
-    TG_KERNEL_NOISE_BODY = 4   /* NoiseLayerKernel on a body: heights += EvaluateFractals3 at each
+    TG_KERNEL_NOISE_BODY = 4,  /* NoiseLayerKernel on a body: heights += EvaluateFractals3 at each
                                 * cell's body-fixed position (src/noise/surface_noise.cpp:9-20),
                                 * which the reference runs on the CPU only (noise_layer.cpp:113);
                                 * one extra, per cell: x, y, z as doubles. Every basis, fractal
                                 * type, blend and 3D domain warp (fractals.h:221-269). */
+    TG_KERNEL_CRATER_DATUMS = 5, /* tg_crater_datum_kernel, src/layers/kernels/crater_datums.slang:
+                                  * a row per fitted crater, a column per lattice point of its
+                                  * datum footprint, each adding KernelDatumRelief (craters.h:
+                                  * 1266-1305); extras: records, data, band floats, band cells,
+                                  * cells, entries, targets (datum_fit.cpp:306-314) */
+    TG_KERNEL_CRATER_BODY_DATUMS = 6 /* tg_crater_body_datum_kernel, crater_body_datums.slang:
+                                      * KernelBodyDatumRelief (craters.h:1313-1326) at each
+                                      * lattice point on a body, no-data points left alone;
+                                      * extras: records, data, frames, bounds, lists (first and
+                                      * count per row), entries, positions (per cell, split
+                                      * points) (datum_fit.cpp:561-565) */
 };
 
 typedef struct tg_dev tg_dev;

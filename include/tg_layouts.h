@@ -169,6 +169,21 @@ typedef struct tg_split_point {
     float x, x_low, y, y_low, z, z_low;
 } tg_split_point;
 
+/* CraterDatumKernel (include/terraingen/layers/kernels/craters.h:1214-1221): what fitting crater
+ * datums on a plane takes: the craters' shapes, the fit footprint in crater radii, lattice points
+ * per side and the size bands of the crater index. */
+typedef struct tg_crater_datum_kernel {
+    tg_crater_shape shape;
+    float fit_radii;
+    uint32_t lattice_side;
+    uint32_t band_count;
+} tg_crater_datum_kernel;
+
+/* CraterBodyDatumKernel (craters.h:1308-1311): the craters' shapes, for datums on a body. */
+typedef struct tg_crater_body_datum_kernel {
+    tg_crater_shape shape;
+} tg_crater_body_datum_kernel;
+
 /* One boulder as the dome relief reads it. The reference bakes boulders on the CPU in FP64
  * (BoulderLayer::AddRelief and DomeHeight, src/layers/boulders/boulder_relief.cpp:44-57,244-275)
  * and has no GPU layout for them; this is the device's, prepared on the host from a boulder list
@@ -213,6 +228,9 @@ static_assert(sizeof(tg_crater_shape) == 188, "tg_crater_shape must be 188 bytes
 static_assert(sizeof(tg_crater_record) == 144, "tg_crater_record must be 144 bytes");
 static_assert(sizeof(tg_crater_frame) == 52, "tg_crater_frame must be 52 bytes");
 static_assert(sizeof(tg_split_point) == 24, "tg_split_point must be 24 bytes");
+static_assert(sizeof(tg_crater_datum_kernel) == 200, "tg_crater_datum_kernel must be 200 bytes");
+static_assert(sizeof(tg_crater_body_datum_kernel) == 188,
+              "tg_crater_body_datum_kernel must be 188 bytes");
 static_assert(sizeof(tg_kernel_params) == 20, "tg_kernel_params must be 20 bytes");
 static_assert(sizeof(tg_boulder) == 80, "tg_boulder is 80 bytes");
 static_assert(sizeof(tg_boulder_kernel) == 16, "tg_boulder_kernel is 16 bytes");

@@ -183,6 +183,14 @@ def _declare(lib: C.CDLL) -> None:
                                         C.c_char_p, C.c_int, u, C.c_int]),
         "ref_write_crater_csv": (C.c_int, [C.c_char_p, vp, vp, vp, vp, vp, vp, vp, sz, vp, vp]),
         "ref_output_error": (C.c_char_p, []),
+        "or_datum_grid": (None, [vp, vp, vp, vp, vp, vp, vp, vp, C.POINTER(N.KernelParams), vp,
+                                 C.c_uint]),
+        "or_body_datum_grid": (None, [vp, vp, vp, vp, vp, vp, vp, vp,
+                                      C.POINTER(N.KernelParams), vp, C.c_uint]),
+        "ref_datum_grid": (None, [vp, vp, vp, vp, vp, vp, vp, vp, C.POINTER(N.KernelParams),
+                                  vp]),
+        "ref_body_datum_grid": (None, [vp, vp, vp, vp, vp, vp, vp, vp,
+                                       C.POINTER(N.KernelParams), vp]),
     }
     for name, (res, args) in sigs.items():
         fn = getattr(lib, name, None)

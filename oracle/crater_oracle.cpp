@@ -199,4 +199,95 @@ void or_shape_grid_body_f64(const tg_crater_shape* k, const tg_crater_record* re
     });
 }
 
+// crater_datums.slang over a strip (datum_fit.cpp:306-328): row r of the lattice holds crater
+// targets[first_row + r]'s footprint, column p its lattice point p; each adds KernelDatumRelief
+// (craters.h:1266-1305), the relief of the craters before the target found through the size
+// bands of the crater index.
+void or_datum_grid(const tg_crater_datum_kernel* kd, const tg_crater_record* records,
+                   const float* data, const float* bandFloats, const std::uint32_t* bandCells,
+                   const std::uint32_t* cells, const std::uint32_t* entries,
+                   const std::uint32_t* targets, const tg_kernel_params* params, float* heights,
+                   unsigned threads) {
+    const c32::Shp shape = c32::Widen(kd->shape);
+    Rows(params->rows, threads, [&](std::size_t r) {
+        const std::uint32_t k = targets[params->first_row + r];
+        const c32::Rec target = c32::Widen(records[k]);
+        const std::uint32_t side = kd->lattice_side;
+        const float fitRadius = kd->fit_radii * 0.5f * target.diameter;
+        for (std::uint32_t col = 0; col < params->width; ++col) {
+            const float dx = fitRadius * ((2.0f * float(col % side) + 1.0f) / float(side) - 1.0f);
+            const float dy = fitRadius * ((2.0f * float(col / side) + 1.0f) / float(side) - 1.0f);
+            float relief = 0.0f;
+            if (dx * dx + dy * dy < fitRadius * fitRadius) {
+                const float x = target.x, xLow = target.x_low + dx;
+                const float y = target.y, yLow = target.y_low + dy;
+                const float px = x + xLow, py = y + yLow;
+                const float slackX = 4.8e-7f * std::fabs(px), slackY = 4.8e-7f * std::fabs(py);
+                // KernelCellsNear (craters.h:1256-1264).
+                const auto near = [](float p, float size, float slack, float start,
+                                     std::int32_t count, std::int32_t& lo, std::int32_t& hi) {
+                    lo = std::max(std::int32_t(std::floor((p - size - slack) / size) - start), 0);
+                    hi = std::min(std::int32_t(std::floor((p + size + slack) / size) - start),
+                                  count - 1);
+                };
+                for (std::uint32_t b = 0; b < kd->band_count; ++b) {
+                    const float size = bandFloats[3 * b];
+                    const std::int32_t across = std::int32_t(bandCells[3 * b]);
+                    std::int32_t i0, i1, j0, j1;
+                    near(px, size, slackX, bandFloats[3 * b + 1], across, i0, i1);
+                    near(py, size, slackY, bandFloats[3 * b + 2],
+                         std::int32_t(bandCells[3 * b + 1]), j0, j1);
+                    for (std::int32_t j = j0; j <= j1; ++j) {
+                        for (std::int32_t i = i0; i <= i1; ++i) {
+                            const std::uint32_t cell =
+                                bandCells[3 * b + 2] + std::uint32_t(j * across + i);
+                            const std::uint32_t first = cells[2 * cell];
+                            const std::uint32_t end = first + cells[2 * cell + 1];
+                            for (std::uint32_t e = first; e < end && entries[e] < k; ++e) {
+                                const c32::Rec c = c32::Widen(records[entries[e]]);
+                                relief += c32::CraterRelief(
+                                    shape, data, c, size,
+                                    c32::SplitDifference(x, xLow, c.x, c.x_low),
+                                    c32::SplitDifference(y, yLow, c.y, c.y_low));
+                            }
+                        }
+                    }
+                }
+            }
+            float& h = heights[r * params->width + col];
+            h = h + relief;
+        }
+    });
+}
+
+// crater_body_datums.slang over a strip (datum_fit.cpp:544-574): each lattice point with data adds
+// KernelBodyDatumRelief (craters.h:1313-1326), the relief of its row's listed craters in order,
+// each read in its frame within its bound.
+void or_body_datum_grid(const tg_crater_body_datum_kernel* kd, const tg_crater_record* records,
+                        const float* data, const tg_crater_frame* frames, const float* bounds,
+                        const std::uint32_t* lists, const std::uint32_t* entries,
+                        const tg_split_point* positions, const tg_kernel_params* params,
+                        float* heights, unsigned threads) {
+    const c32::Shp shape = c32::Widen(kd->shape);
+    Rows(params->rows, threads, [&](std::size_t r) {
+        const std::uint32_t row = params->first_row + std::uint32_t(r);
+        for (std::uint32_t col = 0; col < params->width; ++col) {
+            float& h = heights[r * params->width + col];
+            if (IsNoData(*params, h)) {
+                continue;
+            }
+            const tg_split_point& s = positions[r * params->width + col];
+            const c32::Pt p{s.x, s.x_low, s.y, s.y_low, s.z, s.z_low};
+            float relief = 0.0f;
+            for (std::uint32_t e = lists[2 * row]; e < lists[2 * row] + lists[2 * row + 1]; ++e) {
+                const std::uint32_t n = entries[e];
+                const c32::Offset o = c32::BodySplitOffset(c32::Widen(frames[n]), p);
+                relief += c32::CraterRelief(shape, data, c32::Widen(records[n]), bounds[n], o.dx,
+                                            o.dy);
+            }
+            h = h + relief;
+        }
+    });
+}
+
 }  // extern "C"

@@ -1082,5 +1082,24 @@ inline R ApplyCrater(const CraterShape& shape, const float* data,
     return height + c.resolution * shaped + c.resolution * replaced;
 }
 
+// CraterRelief (craters.h:1225-1238): the relief crater `c` adds at (dx, dy) without its datum, as
+// datums are fitted to it: resolution times the bare profile and the central peak, within the
+// profile's reach; nothing beyond `bound`.
+inline R CraterRelief(const Shp& k, const float* data, const Rec& c, R bound, R dx, R dy) {
+    const R squared = dx * dx + dy * dy;
+    if (!(squared < bound * bound)) {
+        return 0.0f;
+    }
+    const R distance = ShiftedDistance(data, c, dx, dy);
+    if (!(distance < k.profile_reach_radii * 0.5f * c.diameter)) {
+        return 0.0f;
+    }
+    CraterView view{c,
+                    {c.diameter, c.fresh_depth, c.fresh_rim, c.current_depth, c.current_rim},
+                    {c.diameter, c.fresh_depth, c.fresh_rim, c.fresh_depth, c.fresh_rim}};
+    return c.resolution * (UpslopeHeight(c, ProfileHeight(k, data, distance, c.diameter, view),
+                                         dx, dy) +
+                           PeakRelief(k, data, c, distance, dx, dy));
+}
 
 }  // namespace ORC_NS

@@ -217,7 +217,130 @@ __global__ void __launch_bounds__(kCraterTile * kCraterTile)
     heights[i] = cell.Folded();
 }
 
+// CraterRelief (craters.h:1225-1238): the relief crater `c` adds at (dx, dy) from its centre
+// without its datum, as datums are fitted to it: its resolution times the bare profile and the
+// central peak, within the profile's reach; nothing beyond `bound` meters.
+__device__ __forceinline__ float CraterRelief(const tg_crater_shape& k, const float* data,
+                                              const tg_crater_record& c, float bound, float dx,
+                                              float dy) {
+    const float squared = dx * dx + dy * dy;
+    if (!(squared < bound * bound)) {
+        return 0.0f;
+    }
+    const float distance = tg::ShiftedDistance(data, c, dx, dy);
+    if (!(distance < k.profile_reach_radii * 0.5f * c.diameter)) {
+        return 0.0f;
+    }
+    const tg::CraterView view{
+        c,
+        {c.diameter, c.fresh_depth, c.fresh_rim, c.current_depth, c.current_rim},
+        {c.diameter, c.fresh_depth, c.fresh_rim, c.fresh_depth, c.fresh_rim}};
+    return c.resolution *
+           (tg::UpslopeHeight(c, tg::ProfileHeight(k, data, distance, c.diameter, view), dx, dy) +
+            tg::PeakRelief(k, data, c, distance, dx, dy));
+}
 
+// KernelCellsNear (craters.h:1256-1264): the cells of a band's row or column within one cell (and
+// the float slack of a cell lookup) of position `p`, clamped to the band.
+__device__ __forceinline__ int2 CellsNear(float p, float size, float slack, float start,
+                                          int32_t count) {
+    const int32_t first = int32_t(floorf((p - size - slack) / size) - start);
+    const int32_t last = int32_t(floorf((p + size + slack) / size) - start);
+    return int2{first > 0 ? first : 0, last < count - 1 ? last : count - 1};
+}
+
+// crater_datums.slang's main(): a row per fitted crater (`targets`), a column per lattice point of
+// its datum footprint; each lattice point adds KernelDatumRelief (craters.h:1266-1305), the relief
+// of the craters applied before its crater, found in the size bands of the crater index. One wave
+// per row: a row's points read the same cells and entries, so the walk is uniform across the wave.
+__global__ __launch_bounds__(64) void CraterDatumStrip(
+    float* __restrict__ heights, const tg_crater_datum_kernel* __restrict__ kernel,
+    const tg_crater_record* __restrict__ records, const float* __restrict__ data,
+    const float* __restrict__ bandFloats, const uint32_t* __restrict__ bandCells,
+    const uint32_t* __restrict__ cells, const uint32_t* __restrict__ entries,
+    const uint32_t* __restrict__ targets, tg_kernel_params params) {
+    const uint32_t col = blockIdx.y * 64u + threadIdx.x;
+    const uint32_t localRow = blockIdx.x;
+    if (col >= params.width || localRow >= params.rows) {
+        return;
+    }
+    const size_t at = size_t(localRow) * params.width + col;
+    const tg_crater_datum_kernel& kd = *kernel;
+    const uint32_t k = targets[params.first_row + localRow];
+    const tg_crater_record& target = records[k];
+    const uint32_t side = kd.lattice_side;
+    const float fitRadius = kd.fit_radii * 0.5f * target.diameter;
+    // KernelDatumOffset (craters.h:1241-1243) of the point's column and row in the lattice.
+    const float dx = fitRadius * ((2.0f * float(col % side) + 1.0f) / float(side) - 1.0f);
+    const float dy = fitRadius * ((2.0f * float(col / side) + 1.0f) / float(side) - 1.0f);
+    float relief = 0.0f;
+    if (dx * dx + dy * dy < fitRadius * fitRadius) {
+        const float x = target.x, xLow = target.x_low + dx;
+        const float y = target.y, yLow = target.y_low + dy;
+        const float px = x + xLow;
+        const float py = y + yLow;
+        const float slackX = 4.8e-7f * fabsf(px);  // kDatumCellSlack (craters.h:1246)
+        const float slackY = 4.8e-7f * fabsf(py);
+        for (uint32_t b = 0u; b < kd.band_count; ++b) {
+            const float size = bandFloats[3u * b];
+            const int32_t across = int32_t(bandCells[3u * b]);
+            const int2 columns = CellsNear(px, size, slackX, bandFloats[3u * b + 1u], across);
+            const int2 rows = CellsNear(py, size, slackY, bandFloats[3u * b + 2u],
+                                        int32_t(bandCells[3u * b + 1u]));
+            for (int32_t j = rows.x; j <= rows.y; ++j) {
+                for (int32_t i = columns.x; i <= columns.y; ++i) {
+                    const uint32_t cell = bandCells[3u * b + 2u] + uint32_t(j * across + i);
+                    const uint32_t first = cells[2u * cell];
+                    const uint32_t end = first + cells[2u * cell + 1u];
+                    for (uint32_t e = first; e < end; ++e) {
+                        const uint32_t n = entries[e];
+                        if (n >= k) {
+                            break;
+                        }
+                        const tg_crater_record& c = records[n];
+                        relief += CraterRelief(kd.shape, data, c, size,
+                                               tg::SplitDifference(x, xLow, c.x, c.x_low),
+                                               tg::SplitDifference(y, yLow, c.y, c.y_low));
+                    }
+                }
+            }
+        }
+    }
+    heights[at] = heights[at] + relief;
+}
+
+// crater_body_datums.slang's main(): at each lattice point of a fitted crater's footprint on a
+// body (a row per crater, a column per point), KernelBodyDatumRelief (craters.h:1313-1326): the
+// relief of the row's listed craters, in list order, each read in its frame within its bound.
+// Points with no data are left as they are.
+__global__ __launch_bounds__(64) void CraterBodyDatumStrip(
+    float* __restrict__ heights, const tg_crater_body_datum_kernel* __restrict__ kernel,
+    const tg_crater_record* __restrict__ records, const float* __restrict__ data,
+    const tg_crater_frame* __restrict__ frames, const float* __restrict__ bounds,
+    const uint32_t* __restrict__ lists, const uint32_t* __restrict__ entries,
+    const tg_split_point* __restrict__ positions, tg_kernel_params params) {
+    const uint32_t col = blockIdx.y * 64u + threadIdx.x;
+    const uint32_t localRow = blockIdx.x;
+    if (col >= params.width || localRow >= params.rows) {
+        return;
+    }
+    const size_t at = size_t(localRow) * params.width + col;
+    const float height = heights[at];
+    if (IsNoData(params, height)) {
+        return;
+    }
+    const uint32_t row = params.first_row + localRow;
+    const uint32_t first = lists[2u * row];
+    const uint32_t count = lists[2u * row + 1u];
+    const tg_split_point p = positions[at];
+    float relief = 0.0f;
+    for (uint32_t e = first; e < first + count; ++e) {
+        const uint32_t n = entries[e];
+        const tg::Offset o = tg::BodySplitOffset(frames[n], p);
+        relief += CraterRelief(kernel->shape, data, records[n], bounds[n], o.dx, o.dy);
+    }
+    heights[at] = height + relief;
+}
 
 void LaunchCraters(const tg_crater_set& set, float* heights, const float* xs, const float* ys,
                    const tg_kernel_params& params, hipStream_t stream) {
@@ -244,6 +367,143 @@ int CreateCraterSet(tg_dev* dev, const tg_crater_shape* kernel, const tg_crater_
                     size_t lowCount, tg_crater_set** out, char* err, size_t errLen);
 void CheckLaunch(const tg_crater_set& set, const tg_kernel_params& params);
 
+
+}  // namespace
+
+namespace {
+
+// The element count of buffer `b` holding items of `size` bytes, checked to be whole.
+size_t Items(const tg_buf& b, size_t size, const char* what) {
+    if (b.bytes % size != 0 || (b.bytes > 0 && b.data == nullptr)) {
+        throw HipError{std::string("tg_hip_run: the ") + what + " buffer is malformed"};
+    }
+    return b.bytes / size;
+}
+
+// crater_datums.slang and crater_body_datums.slang through RunKernel's binding: every index the
+// kernels follow checked on the host, the buffers uploaded, one wave per row launched, the strip's
+// heights copied back.
+int RunDatums(tg_dev* dev, int kernelId, const void* kernel, float* heights,
+              const tg_kernel_params& params, const tg_buf* b, char* err, size_t errLen) {
+    std::lock_guard<std::mutex> lock(dev->mutex);
+    return Guard(err, errLen, [&] {
+        const bool body = kernelId == TG_KERNEL_CRATER_BODY_DATUMS;
+        const size_t cells = size_t(params.width) * params.rows;
+        const size_t recordCount = Items(b[0], sizeof(tg_crater_record), "records");
+        const size_t dataCount = Items(b[1], sizeof(float), "data");
+        const auto* records = static_cast<const tg_crater_record*>(b[0].data);
+        const auto* shape = static_cast<const tg_crater_shape*>(kernel);  // the first member
+        const uint32_t noTiles[5] = {1u, 16u, 0u, 0u, 0u};
+        ValidateCraterBatch(*shape, records, recordCount, dataCount, noTiles, 5, nullptr, 0, 0);
+        const size_t lastRow = size_t(params.first_row) + params.rows;
+        const auto fail = [](const char* what) { throw HipError{std::string("tg_hip_run: ") + what}; };
+        const size_t entryCount = Items(b[5], sizeof(uint32_t), "entries");
+        const auto* entries = static_cast<const uint32_t*>(b[5].data);
+        for (size_t e = 0; e < entryCount; ++e) {
+            if (entries[e] >= recordCount) {
+                fail("an entry names a record past the end");
+            }
+        }
+        Check(hipSetDevice(dev->ordinal), "hipSetDevice");
+        DeviceArray<float> dHeights(cells);
+        dHeights.Upload(heights, cells * sizeof(float));
+        DeviceArray<tg_crater_record> dRecords(std::max<size_t>(recordCount, 1));
+        dRecords.Upload(records, recordCount * sizeof(tg_crater_record));
+        DeviceArray<float> dData(std::max<size_t>(dataCount, 1));
+        dData.Upload(b[1].data, dataCount * sizeof(float));
+        DeviceArray<uint32_t> dEntries(std::max<size_t>(entryCount, 1));
+        dEntries.Upload(entries, entryCount * sizeof(uint32_t));
+        // Rows along x, which takes any count; a row's points (at most a lattice) along y.
+        const dim3 grid(params.rows, (params.width + 63u) / 64u, 1);
+        if (!body) {
+            const auto& kd = *static_cast<const tg_crater_datum_kernel*>(kernel);
+            if (kd.lattice_side == 0) {
+                fail("the datum lattice needs a side");
+            }
+            const size_t bandFloats = Items(b[2], sizeof(float), "band floats");
+            const size_t bandCells = Items(b[3], sizeof(uint32_t), "band cells");
+            const size_t cellWords = Items(b[4], sizeof(uint32_t), "cells");
+            const size_t targetCount = Items(b[6], sizeof(uint32_t), "targets");
+            const auto* bc = static_cast<const uint32_t*>(b[3].data);
+            const auto* cl = static_cast<const uint32_t*>(b[4].data);
+            const auto* tg = static_cast<const uint32_t*>(b[6].data);
+            if (bandFloats < 3ull * kd.band_count || bandCells < 3ull * kd.band_count ||
+                cellWords % 2 != 0) {
+                fail("the crater index holds fewer bands than the kernel names");
+            }
+            for (uint32_t band = 0; band < kd.band_count; ++band) {
+                const uint64_t across = bc[3 * band], down = bc[3 * band + 1];
+                if (across == 0 || down == 0 || across * down > 0x7fffffffull ||
+                    bc[3 * band + 2] + across * down > cellWords / 2) {
+                    fail("a band's cells run past the cell list");
+                }
+            }
+            for (size_t c = 0; c < cellWords / 2; ++c) {
+                if (uint64_t(cl[2 * c]) + cl[2 * c + 1] > entryCount) {
+                    fail("a cell's entries run past the entry list");
+                }
+            }
+            if (targetCount < lastRow) {
+                fail("the targets hold fewer rows than the strip");
+            }
+            for (size_t r = params.first_row; r < lastRow; ++r) {
+                if (tg[r] >= recordCount) {
+                    fail("a target names a record past the end");
+                }
+            }
+            DeviceArray<tg_crater_datum_kernel> dKernel(1);
+            dKernel.Upload(&kd, sizeof kd);
+            DeviceArray<float> dBandFloats(std::max<size_t>(bandFloats, 1));
+            dBandFloats.Upload(b[2].data, bandFloats * sizeof(float));
+            DeviceArray<uint32_t> dBandCells(std::max<size_t>(bandCells, 1));
+            dBandCells.Upload(bc, bandCells * sizeof(uint32_t));
+            DeviceArray<uint32_t> dCells(std::max<size_t>(cellWords, 1));
+            dCells.Upload(cl, cellWords * sizeof(uint32_t));
+            DeviceArray<uint32_t> dTargets(std::max<size_t>(targetCount, 1));
+            dTargets.Upload(tg, targetCount * sizeof(uint32_t));
+            hipLaunchKernelGGL(CraterDatumStrip, grid, dim3(64), 0, nullptr, dHeights.ptr,
+                               dKernel.ptr, dRecords.ptr, dData.ptr, dBandFloats.ptr,
+                               dBandCells.ptr, dCells.ptr, dEntries.ptr, dTargets.ptr, params);
+        } else {
+            const size_t frameCount = Items(b[2], sizeof(tg_crater_frame), "frames");
+            const size_t boundCount = Items(b[3], sizeof(float), "bounds");
+            const size_t listWords = Items(b[4], sizeof(uint32_t), "lists");
+            const size_t positionCount = Items(b[6], sizeof(tg_split_point), "positions");
+            const auto* lists = static_cast<const uint32_t*>(b[4].data);
+            if (frameCount < recordCount || boundCount < recordCount) {
+                fail("a body datum batch needs a frame and a bound per record");
+            }
+            if (listWords < 2 * lastRow) {
+                fail("the lists hold fewer rows than the strip");
+            }
+            for (size_t r = params.first_row; r < lastRow; ++r) {
+                if (uint64_t(lists[2 * r]) + lists[2 * r + 1] > entryCount) {
+                    fail("a row's entries run past the entry list");
+                }
+            }
+            if (positionCount < cells) {
+                fail("the body datum kernel binds one position per point of the strip");
+            }
+            DeviceArray<tg_crater_body_datum_kernel> dKernel(1);
+            dKernel.Upload(kernel, sizeof(tg_crater_body_datum_kernel));
+            DeviceArray<tg_crater_frame> dFrames(std::max<size_t>(frameCount, 1));
+            dFrames.Upload(b[2].data, frameCount * sizeof(tg_crater_frame));
+            DeviceArray<float> dBounds(std::max<size_t>(boundCount, 1));
+            dBounds.Upload(b[3].data, boundCount * sizeof(float));
+            DeviceArray<uint32_t> dLists(std::max<size_t>(listWords, 1));
+            dLists.Upload(lists, listWords * sizeof(uint32_t));
+            DeviceArray<tg_split_point> dPositions(cells);
+            dPositions.Upload(b[6].data, cells * sizeof(tg_split_point));
+            hipLaunchKernelGGL(CraterBodyDatumStrip, grid, dim3(64), 0, nullptr, dHeights.ptr,
+                               dKernel.ptr, dRecords.ptr, dData.ptr, dFrames.ptr, dBounds.ptr,
+                               dLists.ptr, dEntries.ptr, dPositions.ptr, params);
+        }
+        Check(hipGetLastError(), "launching the datum kernel");
+        Check(hipDeviceSynchronize(), "running the datum kernel");
+        Check(hipMemcpy(heights, dHeights.ptr, cells * sizeof(float), hipMemcpyDeviceToHost),
+              "hipMemcpy to host");
+    });
+}
 
 }  // namespace
 
@@ -320,7 +580,9 @@ const char* tg_hip_device_name(const tg_dev* dev) {
 
 int tg_hip_can_run(const tg_dev* dev, int kernelId) {
     return dev != nullptr && (kernelId == TG_KERNEL_NOISE_LAYER || kernelId == TG_KERNEL_CRATERS ||
-                              kernelId == TG_KERNEL_BOULDERS || kernelId == TG_KERNEL_NOISE_BODY);
+                              kernelId == TG_KERNEL_BOULDERS || kernelId == TG_KERNEL_NOISE_BODY ||
+                              kernelId == TG_KERNEL_CRATER_DATUMS ||
+                              kernelId == TG_KERNEL_CRATER_BODY_DATUMS);
 }
 
 int tg_hip_close(tg_dev* dev) {
@@ -374,12 +636,31 @@ int tg_hip_run(tg_dev* dev, int kernelId, const void* kernel, size_t kernelBytes
                                 "and lows");
             return TG_ERR_ARGUMENT;
         }
+    } else if (kernelId == TG_KERNEL_CRATER_DATUMS || kernelId == TG_KERNEL_CRATER_BODY_DATUMS) {
+        const bool body = kernelId == TG_KERNEL_CRATER_BODY_DATUMS;
+        if (kernelBytes != (body ? sizeof(tg_crater_body_datum_kernel)
+                                 : sizeof(tg_crater_datum_kernel))) {
+            SetErr(err, errLen, body ? "tg_hip_run: a CraterBodyDatumKernel is 188 bytes"
+                                     : "tg_hip_run: a CraterDatumKernel is 200 bytes");
+            return TG_ERR_ARGUMENT;
+        }
+        if (nExtra != 7 || extra == nullptr) {
+            SetErr(err, errLen,
+                   body ? "tg_hip_run: the body datum kernel binds records, data, frames, bounds, "
+                          "lists, entries and positions"
+                        : "tg_hip_run: the datum kernel binds records, data, band floats, band "
+                          "cells, cells, entries and targets");
+            return TG_ERR_ARGUMENT;
+        }
     } else {
         SetErr(err, errLen, "tg_hip_run: unknown kernel id " + std::to_string(kernelId));
         return TG_ERR_ARGUMENT;
     }
     if (cells == 0) {
         return TG_OK;
+    }
+    if (kernelId == TG_KERNEL_CRATER_DATUMS || kernelId == TG_KERNEL_CRATER_BODY_DATUMS) {
+        return RunDatums(dev, kernelId, kernel, heights, *params, extra, err, errLen);
     }
     std::lock_guard<std::mutex> lock(dev->mutex);
     return Guard(err, errLen, [&] {

@@ -52,6 +52,21 @@ CraterShape = _struct("CraterShape", [("profile", CraterProfile)] + [
         ("noise_rim_outer", f32), ("wall_streaks", f32), ("wall_streak_width", f32),
         ("ejecta", f32), ("ejecta_lineation", f32), ("min_freshness", f32)]])
 
+
+class CraterDatumKernel(C.Structure):
+    """CraterDatumKernel (craters.h:1214-1221): shapes, fit footprint in radii, lattice side,
+    size bands of the crater index."""
+    _fields_ = [("shape", CraterShape), ("fit_radii", C.c_float), ("lattice_side", C.c_uint32),
+                ("band_count", C.c_uint32)]
+
+
+class CraterBodyDatumKernel(C.Structure):
+    """CraterBodyDatumKernel (craters.h:1308-1311)."""
+    _fields_ = [("shape", CraterShape)]
+
+
+assert C.sizeof(CraterDatumKernel) == 200 and C.sizeof(CraterBodyDatumKernel) == 188
+
 RECORD_FIELDS = [
     ("x", f32), ("y", f32), ("diameter", f32), ("resolution", f32), ("fresh_depth", f32),
     ("fresh_rim", f32), ("current_depth", f32), ("current_rim", f32), ("diffusion", f32),
@@ -86,6 +101,8 @@ TG_KERNEL_NOISE_LAYER = 1
 TG_KERNEL_CRATERS = 2
 TG_KERNEL_BOULDERS = 3
 TG_KERNEL_NOISE_BODY = 4
+TG_KERNEL_CRATER_DATUMS = 5
+TG_KERNEL_CRATER_BODY_DATUMS = 6
 NODATA_NONE, NODATA_NAN, NODATA_VALUE = 0, 1, 2
 
 # Every entry point include/tg_hip.h declares, with its ctypes signature.
