@@ -208,6 +208,19 @@ int tgx_index_bin(const tgx_index* index, const tgx_tile* tiles, size_t tile_cou
                   unsigned threads, tg_tile_desc** descs, tg_bin** bins, size_t* bin_count,
                   uint32_t** entries, size_t* entry_count, char* err, size_t err_len);
 
+/* One crater's cells in one tile (tg_window, tg_layouts.h), for binning on the device. */
+typedef tg_window tgx_window;
+
+/* The first half of tgx_index_bin, for binning on the device (tg_hip_batch_stage_windows): the
+ * tile descriptors for bins of `bin_side` cells (8, 16 or 32), and per tile the windows of its
+ * resolved candidates, in catalogue order; `tile_windows` holds (first, count) per tile. The bins
+ * the windows give (every window's crater in each bin its cells meet) are those tgx_index_bin
+ * gives for 16-cell bins. No counterpart in the reference, which bins one grid on the CPU. */
+int tgx_index_windows(const tgx_index* index, const tgx_tile* tiles, size_t tile_count,
+                      uint32_t bin_side, unsigned threads, tg_tile_desc** descs,
+                      uint32_t** tile_windows, tgx_window** windows, size_t* window_count,
+                      char* err, size_t err_len);
+
 /* Counts the (cell, crater) pairs of a binned batch (the output of tgx_batch_bin) whose FP64 cell
  * centre lies strictly inside the crater's footprint half-width `reach`: the pairs a crater kernel
  * evaluates in full. `listed` receives every (cell, listed crater) pair, `inside` those within

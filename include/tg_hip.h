@@ -231,6 +231,28 @@ int tg_hip_batch_stage_async(tg_batch* batch, const tg_tile_desc* tiles, size_t 
                              const tg_bin* bins, size_t bin_count, const uint32_t* entries,
                              size_t entry_count, char* err, size_t err_len);
 
+/* Stages a batch binned on the device (no counterpart in the reference, which bins one grid on
+ * the CPU): the tiles (descriptors for bins of `bin_side` cells, 8, 16 or 32, as
+ * tgx_index_windows makes them) and each tile's crater windows ((first, count) per tile in
+ * `tile_windows`, the windows in catalogue order). One wave per bin tests its tile's windows 64 at
+ * a time against the bin's cells and ballots the hits; the counts' exclusive sum places each bin's
+ * list and a second pass writes it, in window order. With 16-cell bins the bins and entries are
+ * those tgx_index_bin gives. Every window is checked on the host; the entry count is read back
+ * to check the batch's capacity and returned in `entry_count` (may be null). Synchronous. */
+int tg_hip_batch_stage_windows(tg_batch* batch, const tg_tile_desc* tiles, size_t tile_count,
+                               uint32_t bin_side, const uint32_t* tile_windows,
+                               const tg_window* windows, size_t window_count, size_t* entry_count,
+                               char* err, size_t err_len);
+/* The staged batch's bin and entry counts and, given host arrays at least that long, its bins and
+ * entries (no counterpart in the reference). */
+int tg_hip_batch_read_bins(tg_batch* batch, tg_bin* bins, size_t bin_capacity, uint32_t* entries,
+                           size_t entry_capacity, size_t* bin_count, size_t* entry_count,
+                           char* err, size_t err_len);
+/* Entries the culled crater walk tests per pass: 32, 64 (the default) or 128 (each lane testing
+ * two before the ballots). A launch choice for the autotuner, as tg_hip_batch_tune's: every depth
+ * gives the same heights. */
+int tg_hip_batch_cull_depth(tg_batch* batch, uint32_t depth, char* err, size_t err_len);
+
 /* Evaluates the staged batch into `out_dev` (at least `out_capacity` floats): each cell's height
  * is `elevation` plus the plan's noise (none when `plan` is null) at its centre, then the
  * catalogue's craters of its bin (none when `catalogue` is null). With `use_graph` the launch is

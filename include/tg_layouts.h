@@ -169,6 +169,15 @@ typedef struct tg_split_point {
     float x, x_low, y, y_low, z, z_low;
 } tg_split_point;
 
+/* One crater's cells in one tile, for binning on the device: the inclusive column and row ranges
+ * of the tile's cells its footprint box covers (its pixel window, src/layers/craters/shaping/
+ * apply.cpp:119-140; on a sphere the cells within reach with one to spare each side). */
+typedef struct tg_window {
+    uint32_t crater;
+    uint32_t col0, col1;
+    uint32_t row0, row1;
+} tg_window;
+
 /* CraterDatumKernel (include/terraingen/layers/kernels/craters.h:1214-1221): what fitting crater
  * datums on a plane takes: the craters' shapes, the fit footprint in crater radii, lattice points
  * per side and the size bands of the crater index. */
@@ -228,6 +237,7 @@ static_assert(sizeof(tg_crater_shape) == 188, "tg_crater_shape must be 188 bytes
 static_assert(sizeof(tg_crater_record) == 144, "tg_crater_record must be 144 bytes");
 static_assert(sizeof(tg_crater_frame) == 52, "tg_crater_frame must be 52 bytes");
 static_assert(sizeof(tg_split_point) == 24, "tg_split_point must be 24 bytes");
+static_assert(sizeof(tg_window) == 20, "tg_window is 20 bytes");
 static_assert(sizeof(tg_crater_datum_kernel) == 200, "tg_crater_datum_kernel must be 200 bytes");
 static_assert(sizeof(tg_crater_body_datum_kernel) == 188,
               "tg_crater_body_datum_kernel must be 188 bytes");

@@ -13,6 +13,7 @@
 #include "sphere_math.h"
 
 #include <hip/hip_runtime.h>
+#include <hipcub/hipcub.hpp>
 
 #include <cmath>
 #include <cstring>
@@ -71,6 +72,21 @@ struct tg_batch {
     // The tiles' surface (tg_hip_batch_surface): 0 for a plane, else a sphere of this radius.
     double radius = 0.0;
     double graphRadius = 0.0;
+    // Cells per bin side of the staged batch (16 from tg_hip_batch_stage; 8, 16 or 32 from
+    // tg_hip_batch_stage_windows), and entries a culled walk tests per pass (32, 64 or 128).
+    uint32_t binSide = 16;
+    uint32_t cullDepth = 64;
+    uint32_t graphBinSide = 0;
+    uint32_t graphCullDepth = 0;
+    size_t entryCount = 0;  // entries staged
+    // Device binning's buffers, grown as batches need them.
+    tg_window* windows = nullptr;
+    size_t windowCapacity = 0;
+    uint32_t* tileWindows = nullptr;  // (first, count) per tile, maxTiles pairs
+    uint32_t* binCounts = nullptr;    // per bin, then their exclusive sums
+    uint32_t* binFirsts = nullptr;
+    void* scanTemp = nullptr;
+    size_t scanBytes = 0;
 };
 
 namespace {
@@ -97,6 +113,76 @@ __global__ void StaticCounters(uint32_t* counters, uint32_t binCount) {
     }
     if (threadIdx.x == 0) {
         counters[kCountSlot] = binCount;
+    }
+}
+
+// ---- Binning on the device ---------------------------------------------------------------------
+// A batch's bins from its tiles' crater windows (tgx_index_windows): one wave per bin walks its
+// tile's windows 64 at a time, tests which meet the bin's cells, and ballots them: the count pass
+// adds up the hits, an exclusive sum over the bins places each bin's list, and the fill pass writes
+// the hits at their places in window order (a lane's place is the hits of the lanes below it), so
+// each bin lists its craters in catalogue order, as the host's binning does.
+
+// The tile whose bins hold bin b: the last tile with bin_first <= b (tiles in bin order).
+__device__ __forceinline__ uint32_t TileOfBin(const tg_tile_desc* tiles, uint32_t tileCount,
+                                              uint32_t b) {
+    uint32_t lo = 0u, hi = tileCount - 1u;
+    while (lo < hi) {
+        const uint32_t mid = (lo + hi + 1u) >> 1u;
+        if (tiles[mid].bin_first <= b) {
+            lo = mid;
+        } else {
+            hi = mid - 1u;
+        }
+    }
+    return lo;
+}
+
+template <bool kFill>
+__global__ void __launch_bounds__(256)
+    BinWindows(const tg_tile_desc* __restrict__ tiles, uint32_t tileCount,
+               const uint32_t* __restrict__ tileWindows, const tg_window* __restrict__ windows,
+               uint32_t side, uint32_t binCount, uint32_t* __restrict__ counts,
+               const uint32_t* __restrict__ firsts, tg_bin* __restrict__ bins,
+               uint32_t* __restrict__ entries) {
+    const uint32_t b = blockIdx.x * 4u + (threadIdx.x >> 6u);
+    const uint32_t lane = threadIdx.x & 63u;
+    if (b >= binCount) {
+        return;  // uniform across the wave
+    }
+    const uint32_t t = TileOfBin(tiles, tileCount, b);
+    const tg_tile_desc tile = tiles[t];
+    const uint32_t local = b - tile.bin_first;
+    const uint32_t c0 = (local % tile.bins_across) * side;
+    const uint32_t r0 = (local / tile.bins_across) * side;
+    const uint32_t c1 = min(c0 + side, tile.width) - 1u;
+    const uint32_t r1 = min(r0 + side, tile.height) - 1u;
+    const uint32_t first = tileWindows[2u * t];
+    const uint32_t end = first + tileWindows[2u * t + 1u];
+    uint32_t placed = kFill ? firsts[b] : 0u;
+    const uint32_t start = placed;
+    for (uint32_t base = first; base < end; base += 64u) {
+        const uint32_t e = base + lane;
+        bool hit = false;
+        uint32_t crater = 0u;
+        if (e < end) {
+            const tg_window w = windows[e];
+            crater = w.crater;
+            hit = w.col0 <= c1 && w.col1 >= c0 && w.row0 <= r1 && w.row1 >= r0;
+        }
+        const uint64_t mask = __ballot(hit);
+        if (kFill && hit) {
+            const uint64_t below = lane == 0u ? 0ull : (~0ull >> (64u - lane));
+            entries[placed + uint32_t(__builtin_popcountll(mask & below))] = crater;
+        }
+        placed += uint32_t(__builtin_popcountll(mask));
+    }
+    if (lane == 0u) {
+        if (kFill) {
+            bins[b] = tg_bin{t, start, placed - start, 0u};
+        } else {
+            counts[b] = placed;
+        }
     }
 }
 
@@ -259,6 +345,41 @@ __device__ __forceinline__ void DrainBins(uint32_t* counters, uint32_t chunk, F&
     }
 }
 
+// The culled walk over entries [first, end): `depth` entries (32, 64 or 128) are tested per pass,
+// one or two per lane (`test(e, n)` reads entry e's crater into n and says whether it can reach
+// the wave's cells), the passing ones gathered with a ballot and visited in list order. Every depth
+// visits the same craters in the same order, so it changes only the time.
+template <class Test, class Visit>
+__device__ __forceinline__ void CulledWalk(uint32_t first, uint32_t end, uint32_t depth,
+                                           uint32_t lane, const Test& test, const Visit& visit) {
+    const auto gather = [&](uint64_t candidates, uint32_t n) {
+        while (candidates != 0ull) {
+            const uint32_t j = uint32_t(__builtin_ctzll(candidates));
+            candidates &= candidates - 1ull;
+            visit(uint32_t(__builtin_amdgcn_readlane(int(n), int(j))));
+        }
+    };
+    if (depth == 128u) {
+        for (uint32_t base = first; base < end; base += 128u) {
+            // Both halves' loads and tests before either ballot, for more loads in flight.
+            uint32_t n0 = 0u, n1 = 0u;
+            const bool near0 = test(base + lane, n0);
+            const bool near1 = test(base + 64u + lane, n1);
+            const uint64_t c0 = __ballot(near0);
+            const uint64_t c1 = __ballot(near1);
+            gather(c0, n0);
+            gather(c1, n1);
+        }
+        return;
+    }
+    const uint32_t step = depth == 32u ? 32u : 64u;
+    for (uint32_t base = first; base < end; base += step) {
+        uint32_t n = 0u;
+        const bool near = lane < step && test(base + lane, n);
+        gather(__ballot(near), n);
+    }
+}
+
 // One bin per wave: lane 0 takes the next bin with one atomic add and the index is broadcast
 // with readfirstlane, so the bin loop is uniform across the wave and no wave waits for another
 // (the kernel has no block barrier). The wave covers the bin as four 8 x 8 squares, the shape the
@@ -277,17 +398,18 @@ __global__ void __launch_bounds__(kGroupThreads, kAllFast ? (kPrecise ? 6 : 8) :
                const float* __restrict__ harm, const float4* __restrict__ outline,
                const tg::HotCrater* __restrict__ hot, const float* __restrict__ hotHarm,
                const tg::PreciseTerms* __restrict__ precise, uint32_t hoisted, uint32_t chunk,
-               float* __restrict__ out) {
+               uint32_t binSide, uint32_t cullDepth, float* __restrict__ out) {
     const uint32_t lane = threadIdx.x & 63u;
+    const uint32_t squaresAcross = binSide / 8u;
     DrainBins(counters, chunk, [&](uint32_t b) {
         const tg_bin bin = bins[b];
         const tg_tile_desc tile = tiles[bin.tile];
         const uint32_t local = b - tile.bin_first;
-        const uint32_t col0 = (local % tile.bins_across) * kBinSide;
-        const uint32_t row0 = (local / tile.bins_across) * kBinSide;
-        for (uint32_t square = 0; square < 4u; ++square) {
-            const uint32_t squareCol = col0 + (square & 1u) * 8u;
-            const uint32_t squareRow = row0 + (square >> 1u) * 8u;
+        const uint32_t col0 = (local % tile.bins_across) * binSide;
+        const uint32_t row0 = (local / tile.bins_across) * binSide;
+        for (uint32_t square = 0; square < squaresAcross * squaresAcross; ++square) {
+            const uint32_t squareCol = col0 + (square % squaresAcross) * 8u;
+            const uint32_t squareRow = row0 + (square / squaresAcross) * 8u;
             if (squareCol >= tile.width || squareRow >= tile.height) {
                 continue;  // the square lies past the tile's edge (uniform across the wave)
             }
@@ -354,30 +476,23 @@ __global__ void __launch_bounds__(kGroupThreads, kAllFast ? (kPrecise ? 6 : 8) :
                     const double east = tile.origin_x + (double(lastCol) + 0.5) * tile.pixel + 0.0;
                     const double north = tile.origin_y + (double(squareRow) + 0.5) * -tile.pixel;
                     const double south = tile.origin_y + (double(lastRow) + 0.5) * -tile.pixel;
-                    for (uint32_t base = bin.first; base < end; base += 64u) {
-                        const uint32_t e = base + lane;
-                        uint32_t n = 0u;
-                        bool near = false;
-                        if (e < end) {
-                            n = entries[e];
-                            const tg::HotCrater& c = hot[n];
-                            // The centre's split sum is exact in FP64. A cell passing the FP32
-                            // test above lies within the cull radius to a relative 1e-6 or so,
-                            // and the box is no farther than any of its cells, so the margin
-                            // keeps every entry a cell of the square can take.
-                            const double px = double(c.x) + double(c.x_low);
-                            const double py = double(c.y) + double(c.y_low);
-                            const double ex = fmax(0.0, fmax(west - px, px - east));
-                            const double ey = fmax(0.0, fmax(south - py, py - north));
-                            near = ex * ex + ey * ey <= double(c.cull) * (1.0 + 1e-4) + 1e-6;
+                    // Entry e of the list near the square's box. The centre's split sum is exact
+                    // in FP64. A cell passing the FP32 test of the visit lies within the cull
+                    // radius to a relative 1e-6 or so, and the box is no farther than any of its
+                    // cells, so the margin keeps every entry a cell of the square can take.
+                    const auto test = [&](uint32_t e, uint32_t& n) {
+                        if (e >= end) {
+                            return false;
                         }
-                        uint64_t candidates = __ballot(near);
-                        while (candidates != 0ull) {
-                            const uint32_t j = uint32_t(__builtin_ctzll(candidates));
-                            candidates &= candidates - 1ull;
-                            visit(uint32_t(__builtin_amdgcn_readlane(int(n), int(j))));
-                        }
-                    }
+                        n = entries[e];
+                        const tg::HotCrater& c = hot[n];
+                        const double px = double(c.x) + double(c.x_low);
+                        const double py = double(c.y) + double(c.y_low);
+                        const double ex = fmax(0.0, fmax(west - px, px - east));
+                        const double ey = fmax(0.0, fmax(south - py, py - north));
+                        return ex * ex + ey * ey <= double(c.cull) * (1.0 + 1e-4) + 1e-6;
+                    };
+                    CulledWalk(bin.first, end, cullDepth, lane, test, visit);
                 }
             }
             if (inside) {
@@ -461,19 +576,20 @@ __global__ void __launch_bounds__(kGroupThreads, kAllFast ? (kPrecise ? 6 : 8) :
                    const tg::HotCrater* __restrict__ hot, const float* __restrict__ hotHarm,
                    const tg_crater_frame* __restrict__ frames,
                    const tg::PreciseTerms* __restrict__ precise, uint32_t hoisted, uint32_t chunk,
-                   float* __restrict__ out) {
+                   uint32_t binSide, uint32_t cullDepth, float* __restrict__ out) {
     const uint32_t lane = threadIdx.x & 63u;
     const tg::CubeSteps steps = tg::MakeCubeSteps(lane);
+    const uint32_t squaresAcross = binSide / 8u;
     DrainBins(counters, chunk, [&](uint32_t b) {
         const tg_bin bin = bins[b];
         const tg_tile_desc tile = tiles[bin.tile];
         const uint32_t local = b - tile.bin_first;
-        const uint32_t col0 = (local % tile.bins_across) * kBinSide;
-        const uint32_t row0 = (local / tile.bins_across) * kBinSide;
+        const uint32_t col0 = (local % tile.bins_across) * binSide;
+        const uint32_t row0 = (local / tile.bins_across) * binSide;
         const double spacing = tile.pixel * unitLength;
-        for (uint32_t square = 0; square < 4u; ++square) {
-            const uint32_t squareCol = col0 + (square & 1u) * 8u;
-            const uint32_t squareRow = row0 + (square >> 1u) * 8u;
+        for (uint32_t square = 0; square < squaresAcross * squaresAcross; ++square) {
+            const uint32_t squareCol = col0 + (square % squaresAcross) * 8u;
+            const uint32_t squareRow = row0 + (square / squaresAcross) * 8u;
             if (squareCol >= tile.width || squareRow >= tile.height) {
                 continue;  // the square lies past the tile's edge (uniform across the wave)
             }
@@ -531,30 +647,23 @@ __global__ void __launch_bounds__(kGroupThreads, kAllFast ? (kPrecise ? 6 : 8) :
                     const double reach = WaveMax(sqrt((at.x - ox) * (at.x - ox) +
                                                       (at.y - oy) * (at.y - oy) +
                                                       (at.z - oz) * (at.z - oz)));
-                    for (uint32_t base = bin.first; base < end; base += 64u) {
-                        const uint32_t e = base + lane;
-                        uint32_t n = 0u;
-                        bool near = false;
-                        if (e < end) {
-                            n = entries[e];
-                            const tg_crater_frame& f = frames[n];
-                            // A cell passing the FP32 test lies within the cull radius of the
-                            // centre along the sphere to a relative 1e-6 or so, and the chord is
-                            // no longer than the arc; the margins keep every entry a cell of the
-                            // square can take.
-                            const double qx = double(f.x) + double(f.x_low) - ox;
-                            const double qy = double(f.y) + double(f.y_low) - oy;
-                            const double qz = double(f.z) + double(f.z_low) - oz;
-                            const double d = sqrt(qx * qx + qy * qy + qz * qz);
-                            near = d <= sqrt(double(hot[n].cull)) * (1.0 + 1e-4) + 1e-3 + reach;
+                    // Entry e of the list near the ball. A cell passing the FP32 test lies within
+                    // the cull radius of the centre along the sphere to a relative 1e-6 or so,
+                    // and the chord is no longer than the arc; the margins keep every entry a
+                    // cell of the square can take.
+                    const auto test = [&](uint32_t e, uint32_t& n) {
+                        if (e >= end) {
+                            return false;
                         }
-                        uint64_t candidates = __ballot(near);
-                        while (candidates != 0ull) {
-                            const uint32_t j = uint32_t(__builtin_ctzll(candidates));
-                            candidates &= candidates - 1ull;
-                            visit(uint32_t(__builtin_amdgcn_readlane(int(n), int(j))));
-                        }
-                    }
+                        n = entries[e];
+                        const tg_crater_frame& f = frames[n];
+                        const double qx = double(f.x) + double(f.x_low) - ox;
+                        const double qy = double(f.y) + double(f.y_low) - oy;
+                        const double qz = double(f.z) + double(f.z_low) - oz;
+                        const double d = sqrt(qx * qx + qy * qy + qz * qz);
+                        return d <= sqrt(double(hot[n].cull)) * (1.0 + 1e-4) + 1e-3 + reach;
+                    };
+                    CulledWalk(bin.first, end, cullDepth, lane, test, visit);
                 }
             }
             if (inside) {
@@ -713,7 +822,7 @@ void Launch(tg_batch& b, const tg_noise_plan* plan, float elevation, const tg_ca
                            cat != nullptr ? cat->hotHarm : nullptr,
                            cat != nullptr ? cat->frames : nullptr,
                            cat != nullptr ? cat->precise : nullptr, b.hoisted ? 1u : 0u, chunk,
-                           out);
+                           b.binSide, b.cullDepth, out);
         Check(hipGetLastError(), "launching the batched body tile kernel");
         return;
     }
@@ -726,7 +835,8 @@ void Launch(tg_batch& b, const tg_noise_plan* plan, float elevation, const tg_ca
                        cat != nullptr ? cat->outline : nullptr,
                        cat != nullptr ? cat->hot : nullptr,
                        cat != nullptr ? cat->hotHarm : nullptr,
-                       cat != nullptr ? cat->precise : nullptr, b.hoisted ? 1u : 0u, chunk, out);
+                       cat != nullptr ? cat->precise : nullptr, b.hoisted ? 1u : 0u, chunk,
+                       b.binSide, b.cullDepth, out);
     Check(hipGetLastError(), "launching the batched tile kernel");
 }
 
@@ -858,7 +968,10 @@ int tg_hip_batch_free(tg_batch* b) {
     }
     DropGraph(*b);
     for (void* p : {static_cast<void*>(b->tiles), static_cast<void*>(b->bins),
-                    static_cast<void*>(b->entries), static_cast<void*>(b->counters)}) {
+                    static_cast<void*>(b->entries), static_cast<void*>(b->counters),
+                    static_cast<void*>(b->windows), static_cast<void*>(b->tileWindows),
+                    static_cast<void*>(b->binCounts), static_cast<void*>(b->binFirsts),
+                    b->scanTemp}) {
         if (p != nullptr) {
             (void)hipFree(p);
         }
@@ -885,25 +998,32 @@ struct Checked {
     bool any = false;
 };
 
-// Checks every index of a batch's tiles, bins and entries against the others.
-Checked CheckBatch(const tg_tile_desc* tiles, size_t tileCount, const tg_bin* bins,
-                   size_t binCount, const uint32_t* entries, size_t entryCount) {
+// Checks every tile against the bin list's length: its shape, its bins for bins of `side` cells,
+// and its output; the end of the output.
+uint64_t CheckTiles(const tg_tile_desc* tiles, size_t tileCount, size_t binCount, uint32_t side) {
     uint64_t outEnd = 0;
     for (size_t i = 0; i < tileCount; ++i) {
         const tg_tile_desc& t = tiles[i];
         const uint64_t cells = uint64_t(t.width) * t.height;
         if (t.width == 0 || t.height == 0 ||
-            t.bins_across != (t.width + kBinSide - 1) / kBinSide || !(t.pixel > 0.0) ||
+            t.bins_across != (t.width + side - 1) / side || !(t.pixel > 0.0) ||
             !std::isfinite(t.origin_x) || !std::isfinite(t.origin_y) ||
             t.out_offset > ~uint64_t(0) - cells) {
             throw HipError{"tile " + std::to_string(i) + " is malformed"};
         }
-        const uint64_t binsDown = (t.height + kBinSide - 1) / kBinSide;
+        const uint64_t binsDown = (t.height + side - 1) / side;
         if (uint64_t(t.bin_first) + uint64_t(t.bins_across) * binsDown > binCount) {
             throw HipError{"tile " + std::to_string(i) + "'s bins run past the bin list"};
         }
         outEnd = std::max<uint64_t>(outEnd, t.out_offset + cells);
     }
+    return outEnd;
+}
+
+// Checks every index of a batch's tiles, bins and entries against the others.
+Checked CheckBatch(const tg_tile_desc* tiles, size_t tileCount, const tg_bin* bins,
+                   size_t binCount, const uint32_t* entries, size_t entryCount) {
+    const uint64_t outEnd = CheckTiles(tiles, tileCount, binCount, kBinSide);
     uint64_t maxEntry = 0;
     bool any = false;
     for (size_t i = 0; i < binCount; ++i) {
@@ -986,6 +1106,8 @@ int Stage(tg_batch* b, const tg_tile_desc* tiles, size_t tileCount, const tg_bin
         b->maxEntry = checked.maxEntry;
         b->anyEntries = checked.any;
         b->outEnd = checked.outEnd;
+        b->binSide = kBinSide;
+        b->entryCount = entryCount;
     });
 }
 
@@ -1019,6 +1141,166 @@ int tg_hip_batch_stage_async(tg_batch* b, const tg_tile_desc* tiles, size_t tile
     return Stage(b, tiles, tileCount, bins, binCount, entries, entryCount, true, err, errLen);
 }
 
+int tg_hip_batch_stage_windows(tg_batch* b, const tg_tile_desc* tiles, size_t tileCount,
+                               uint32_t binSide, const uint32_t* tileWindows,
+                               const tg_window* windows, size_t windowCount, size_t* entryCount,
+                               char* err, size_t errLen) {
+    if (b == nullptr || (tileCount > 0 && (tiles == nullptr || tileWindows == nullptr)) ||
+        (windowCount > 0 && windows == nullptr)) {
+        SetErr(err, errLen, "tg_hip_batch_stage_windows: null argument");
+        return TG_ERR_ARGUMENT;
+    }
+    if (binSide != 8 && binSide != 16 && binSide != 32) {
+        SetErr(err, errLen, "tg_hip_batch_stage_windows: the bin side must be 8, 16 or 32");
+        return TG_ERR_ARGUMENT;
+    }
+    return Guard(err, errLen, [&] {
+        if (tileCount > b->maxTiles) {
+            throw HipError{"batch exceeds the capacities it was created with"};
+        }
+        // The tiles as CheckBatch checks them, their bins counted from the descriptors.
+        uint64_t binCount = 0;
+        for (size_t t = 0; t < tileCount; ++t) {
+            if (tiles[t].bin_first != binCount) {
+                throw HipError{"tile " + std::to_string(t) + "'s bins do not follow the last's"};
+            }
+            binCount += uint64_t(tiles[t].bins_across) *
+                        ((tiles[t].height + binSide - 1) / binSide);
+        }
+        if (binCount > b->maxBins || binCount > 0x7fffffffull) {
+            throw HipError{"batch exceeds the capacities it was created with"};
+        }
+        const uint64_t outEnd = CheckTiles(tiles, tileCount, binCount, binSide);
+        // The windows: each tile's within the list, inside the tile, craters ascending.
+        uint64_t maxEntry = 0;
+        for (size_t t = 0; t < tileCount; ++t) {
+            const uint64_t first = tileWindows[2 * t], count = tileWindows[2 * t + 1];
+            if (first + count > windowCount) {
+                throw HipError{"tile " + std::to_string(t) + "'s windows run past the list"};
+            }
+            for (uint64_t e = first; e < first + count; ++e) {
+                const tg_window& w = windows[e];
+                if (w.col0 > w.col1 || w.col1 >= tiles[t].width || w.row0 > w.row1 ||
+                    w.row1 >= tiles[t].height || (e > first && w.crater <= windows[e - 1].crater)) {
+                    throw HipError{"window " + std::to_string(e) + " is malformed"};
+                }
+                maxEntry = std::max<uint64_t>(maxEntry, w.crater);
+            }
+        }
+        Check(hipSetDevice(b->dev->ordinal), "hipSetDevice");
+        Check(hipStreamSynchronize(b->stream), "waiting for the batch's previous run");
+        if (windowCount > b->windowCapacity) {
+            if (b->windows != nullptr) {
+                Check(hipFree(b->windows), "hipFree");
+                b->windows = nullptr;
+                b->windowCapacity = 0;
+            }
+            b->windows = Alloc<tg_window>(windowCount + windowCount / 4);
+            b->windowCapacity = windowCount + windowCount / 4;
+        }
+        if (b->tileWindows == nullptr) {
+            b->tileWindows = Alloc<uint32_t>(2 * std::max<size_t>(b->maxTiles, 1));
+            b->binCounts = Alloc<uint32_t>(std::max<size_t>(b->maxBins, 1));
+            b->binFirsts = Alloc<uint32_t>(std::max<size_t>(b->maxBins, 1));
+            Check(hipcub::DeviceScan::ExclusiveSum(nullptr, b->scanBytes, b->binCounts,
+                                                   b->binFirsts, int(std::max<size_t>(b->maxBins, 1)),
+                                                   b->stream),
+                  "sizing the bin scan");
+            Check(hipMalloc(&b->scanTemp, std::max<size_t>(b->scanBytes, 1)), "hipMalloc");
+        }
+        const auto put = [&](void* dst, const void* src, size_t bytes, const char* what) {
+            if (bytes > 0) {
+                Check(hipMemcpy(dst, src, bytes, hipMemcpyHostToDevice), what);
+            }
+        };
+        put(b->tiles, tiles, tileCount * sizeof(tg_tile_desc), "staging tiles");
+        put(b->tileWindows, tileWindows, 2 * tileCount * sizeof(uint32_t), "staging windows");
+        put(b->windows, windows, windowCount * sizeof(tg_window), "staging windows");
+        size_t total = 0;
+        if (binCount > 0) {
+            const dim3 grid(uint32_t((binCount + 3) / 4));
+            hipLaunchKernelGGL(BinWindows<false>, grid, dim3(256), 0, b->stream, b->tiles,
+                               uint32_t(tileCount), b->tileWindows, b->windows, binSide,
+                               uint32_t(binCount), b->binCounts, nullptr, nullptr, nullptr);
+            Check(hipGetLastError(), "launching the bin count");
+            size_t bytes = b->scanBytes;
+            Check(hipcub::DeviceScan::ExclusiveSum(b->scanTemp, bytes, b->binCounts, b->binFirsts,
+                                                   int(binCount), b->stream),
+                  "summing the bin counts");
+            uint32_t tail[2] = {0u, 0u};
+            Check(hipMemcpyAsync(&tail[0], b->binFirsts + binCount - 1, sizeof(uint32_t),
+                                 hipMemcpyDeviceToHost, b->stream), "reading the entry count");
+            Check(hipMemcpyAsync(&tail[1], b->binCounts + binCount - 1, sizeof(uint32_t),
+                                 hipMemcpyDeviceToHost, b->stream), "reading the entry count");
+            Check(hipStreamSynchronize(b->stream), "binning the batch");
+            total = size_t(tail[0]) + tail[1];
+            if (total > b->maxEntries) {
+                throw HipError{"the batch's bins list " + std::to_string(total) +
+                               " entries, past the " + std::to_string(b->maxEntries) +
+                               " it was created with"};
+            }
+            hipLaunchKernelGGL(BinWindows<true>, grid, dim3(256), 0, b->stream, b->tiles,
+                               uint32_t(tileCount), b->tileWindows, b->windows, binSide,
+                               uint32_t(binCount), nullptr, b->binFirsts, b->bins, b->entries);
+            Check(hipGetLastError(), "launching the bin fill");
+        }
+        hipLaunchKernelGGL(StaticCounters, dim3(1), dim3(64), 0, b->stream, b->counters,
+                           uint32_t(binCount));
+        Check(hipGetLastError(), "setting the queue counters");
+        Check(hipStreamSynchronize(b->stream), "binning the batch");
+        b->tileCount = tileCount;
+        b->binCount = binCount;
+        b->maxEntry = maxEntry;
+        b->anyEntries = total > 0;
+        b->outEnd = outEnd;
+        b->binSide = binSide;
+        b->entryCount = total;
+        if (entryCount != nullptr) {
+            *entryCount = total;
+        }
+    });
+}
+
+int tg_hip_batch_read_bins(tg_batch* b, tg_bin* bins, size_t binCapacity, uint32_t* entries,
+                           size_t entryCapacity, size_t* binCount, size_t* entryCount, char* err,
+                           size_t errLen) {
+    if (b == nullptr || binCount == nullptr || entryCount == nullptr) {
+        SetErr(err, errLen, "tg_hip_batch_read_bins: null argument");
+        return TG_ERR_ARGUMENT;
+    }
+    return Guard(err, errLen, [&] {
+        *binCount = b->binCount;
+        *entryCount = b->entryCount;
+        if (bins == nullptr && entries == nullptr) {
+            return;  // the counts only
+        }
+        if (binCapacity < b->binCount || entryCapacity < b->entryCount || bins == nullptr ||
+            entries == nullptr) {
+            throw HipError{"tg_hip_batch_read_bins: the host arrays are too small"};
+        }
+        Check(hipSetDevice(b->dev->ordinal), "hipSetDevice");
+        Check(hipStreamSynchronize(b->stream), "waiting for the batch");
+        if (b->binCount > 0) {
+            Check(hipMemcpy(bins, b->bins, b->binCount * sizeof(tg_bin), hipMemcpyDeviceToHost),
+                  "reading the bins");
+        }
+        if (b->entryCount > 0) {
+            Check(hipMemcpy(entries, b->entries, b->entryCount * sizeof(uint32_t),
+                            hipMemcpyDeviceToHost),
+                  "reading the entries");
+        }
+    });
+}
+
+int tg_hip_batch_cull_depth(tg_batch* b, uint32_t depth, char* err, size_t errLen) {
+    if (b == nullptr || (depth != 32 && depth != 64 && depth != 128)) {
+        SetErr(err, errLen, "tg_hip_batch_cull_depth: a batch and a depth of 32, 64 or 128");
+        return TG_ERR_ARGUMENT;
+    }
+    b->cullDepth = depth;
+    return TG_OK;
+}
+
 int tg_hip_batch_run(tg_batch* b, const tg_noise_plan* plan, float elevation,
                      const tg_catalogue* cat, float* out, size_t outCapacity, int useGraph,
                      char* err, size_t errLen) {
@@ -1045,7 +1327,8 @@ int tg_hip_batch_run(tg_batch* b, const tg_noise_plan* plan, float elevation,
             b->graphCatalogue != catalogueGeneration ||
             b->graphOut != out || b->graphElevation != elevation || b->graphGroups != groups ||
             b->graphWalk != b->walk || b->graphHoisted != b->hoisted ||
-            b->graphRadius != b->radius ||
+            b->graphRadius != b->radius || b->graphBinSide != b->binSide ||
+            b->graphCullDepth != b->cullDepth ||
             b->graphChunk != ChunkFor(*b, groups)) {
             DropGraph(*b);
             hipGraph_t graph = nullptr;
@@ -1073,6 +1356,8 @@ int tg_hip_batch_run(tg_batch* b, const tg_noise_plan* plan, float elevation,
             b->graphHoisted = b->hoisted;
             b->graphChunk = ChunkFor(*b, groups);
             b->graphRadius = b->radius;
+            b->graphBinSide = b->binSide;
+            b->graphCullDepth = b->cullDepth;
         }
         Check(hipGraphLaunch(b->graph, b->stream), "hipGraphLaunch");
     });

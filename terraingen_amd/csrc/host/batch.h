@@ -76,6 +76,24 @@ void BinSphereBatch(const SphereCatalogue& catalogue, const tgx_tile* tiles,
 // The radius of the sphere an index lies on; 0 on a plane.
 double IndexRadius(const tgx_index& index);
 
+// The tile descriptors of `tiles` for bins of `side` cells, heights packed from offset 0.
+void TileDescs(const tgx_tile* tiles, std::size_t tileCount, std::uint32_t side,
+               std::vector<tg_tile_desc>& descs);
+// Appends to `out` the bins of bin rows [row0, row1) of `tile` for bins of `side` cells: each
+// window's crater in every bin its cells meet, in window order.
+void ListWindows(const std::vector<tgx_window>& windows, const tgx_tile& tile, std::uint32_t side,
+                 std::size_t row0, std::size_t row1, TileBins& out);
+// The sphere's form of the windows of one tile (sphere.cpp): cells within reach of each resolved
+// candidate, for the tile's rows [rowFirst, rowEnd).
+void SphereWindows(const SphereCatalogue& index, const tgx_tile& tile, std::size_t rowFirst,
+                   std::size_t rowEnd, std::vector<std::uint32_t>& candidates,
+                   std::vector<tgx_window>& windows);
+// tgx_index_windows: the descriptors for bins of `side` cells, each tile's (first, count) of
+// windows, and the windows, every tile's in catalogue order.
+void WindowBatch(const tgx_index& index, const tgx_tile* tiles, std::size_t tileCount,
+                 std::uint32_t side, unsigned threads, std::vector<tg_tile_desc>& descs,
+                 std::vector<std::uint32_t>& tileWindows, std::vector<tgx_window>& windows);
+
 // Bins `tiles` into 16 x 16 bins against the index (resolved craters only, in catalogue order),
 // packing the tiles' heights one after another from offset 0, over `threads` threads (0: all).
 void BinBatch(const tgx_index& index, const tgx_tile* tiles, std::size_t tileCount,

@@ -82,19 +82,40 @@ IndexPtr MakeIndex(const double* cx, const double* cy, const double* reach,
     return index;
 }
 
+void ListWindows(const std::vector<tgx_window>& windows, const tgx_tile& tile, std::uint32_t side,
+                 std::size_t row0, std::size_t row1, TileBins& out) {
+    const std::size_t binsX = (tile.width + side - 1) / side;
+    std::vector<std::vector<std::uint32_t>> lists(binsX * (row1 - row0));
+    for (const tgx_window& w : windows) {
+        const std::size_t by0 = std::max<std::size_t>(w.row0 / side, row0);
+        const std::size_t by1 = std::min<std::size_t>(w.row1 / side + 1, row1);
+        for (std::size_t by = by0; by < by1; ++by) {
+            for (std::size_t bx = w.col0 / side; bx <= w.col1 / side; ++bx) {
+                lists[(by - row0) * binsX + bx].push_back(w.crater);
+            }
+        }
+    }
+    out.counts.resize(lists.size());
+    out.entries.clear();
+    for (std::size_t b = 0; b < lists.size(); ++b) {
+        out.counts[b] = std::uint32_t(lists[b].size());
+        out.entries.insert(out.entries.end(), lists[b].begin(), lists[b].end());
+    }
+}
+
 namespace {
 
-// One tile's bins against the index: per-bin counts and the entries in bin order.
-// The bins of bin rows [row0, row1) of `tile`: each crater's pixel window is the whole tile's, so
-// a strip's bins are those of the whole tile; only its candidates are gathered from the strip.
-void BinTile(const tgx_index& index, const tgx_tile& tile, std::size_t row0, std::size_t row1,
-             std::vector<std::uint32_t>& candidates, TileBins& out) {
-    constexpr std::uint32_t kSide = 16;
+// The windows of one tile's candidates against the index: for each crater resolved at the tile's
+// pixel size whose footprint box meets the tile, in catalogue order, the cells its pixel window
+// covers (PixelWindowIn over the whole tile). Candidates are gathered for the tile's rows
+// [rowFirst, rowEnd) only, so a strip of a large tile finds the craters that can reach it.
+void PlaneWindows(const tgx_index& index, const tgx_tile& tile, std::size_t rowFirst,
+                  std::size_t rowEnd, std::vector<std::uint32_t>& candidates,
+                  std::vector<tgx_window>& windows) {
     const Grid g{tile.width, tile.height, tile.origin_x, tile.origin_y, tile.pixel};
     const double minX = tile.origin_x, maxX = tile.origin_x + tile.width * tile.pixel;
-    const double maxY = tile.origin_y - double(row0 * kSide) * tile.pixel;
-    const double minY =
-        tile.origin_y - double(std::min<std::size_t>(row1 * kSide, tile.height)) * tile.pixel;
+    const double maxY = tile.origin_y - double(rowFirst) * tile.pixel;
+    const double minY = tile.origin_y - double(std::min<std::size_t>(rowEnd, tile.height)) * tile.pixel;
     candidates.clear();
     for (const IndexLevel& level : index.levels) {
         if (level.cell == 0.0 || !Resolved(level.maxDiameter, tile.pixel)) {
@@ -122,8 +143,7 @@ void BinTile(const tgx_index& index, const tgx_tile& tile, std::size_t row0, std
     }
     std::sort(candidates.begin(), candidates.end());
     candidates.erase(std::unique(candidates.begin(), candidates.end()), candidates.end());
-    const std::size_t binsX = (tile.width + kSide - 1) / kSide;
-    std::vector<std::vector<std::uint32_t>> lists(binsX * (row1 - row0));
+    windows.clear();
     for (const std::uint32_t k : candidates) {
         if (!Resolved(index.diameter[k], tile.pixel)) {
             continue;
@@ -131,23 +151,23 @@ void BinTile(const tgx_index& index, const tgx_tile& tile, std::size_t row0, std
         const double r = index.reach[k];
         const PixelWindow w = PixelWindowIn(g, index.cx[k] - r, index.cy[k] - r,
                                             index.cx[k] + r, index.cy[k] + r);
-        if (w.Empty()) {
+        if (w.Empty() || w.rowEnd <= rowFirst || w.rowBegin >= rowEnd) {
             continue;
         }
-        const std::size_t by0 = std::max(w.rowBegin / kSide, row0);
-        const std::size_t by1 = std::min((w.rowEnd - 1) / kSide + 1, row1);
-        for (std::size_t by = by0; by < by1; ++by) {
-            for (std::size_t bx = w.colBegin / kSide; bx <= (w.colEnd - 1) / kSide; ++bx) {
-                lists[(by - row0) * binsX + bx].push_back(k);
-            }
-        }
+        windows.push_back({k, std::uint32_t(w.colBegin), std::uint32_t(w.colEnd - 1),
+                           std::uint32_t(w.rowBegin), std::uint32_t(w.rowEnd - 1)});
     }
-    out.counts.resize(lists.size());
-    out.entries.clear();
-    for (std::size_t b = 0; b < lists.size(); ++b) {
-        out.counts[b] = std::uint32_t(lists[b].size());
-        out.entries.insert(out.entries.end(), lists[b].begin(), lists[b].end());
-    }
+}
+
+// One tile's bins against the index: per-bin counts and the entries in bin order.
+// The bins of bin rows [row0, row1) of `tile`: each crater's pixel window is the whole tile's, so
+// a strip's bins are those of the whole tile; only its candidates are gathered from the strip.
+void BinTile(const tgx_index& index, const tgx_tile& tile, std::size_t row0, std::size_t row1,
+             std::vector<std::uint32_t>& candidates, TileBins& out) {
+    constexpr std::uint32_t kSide = 16;
+    std::vector<tgx_window> windows;
+    PlaneWindows(index, tile, row0 * kSide, row1 * kSide, candidates, windows);
+    ListWindows(windows, tile, kSide, row0, row1, out);
 }
 
 }  // namespace
@@ -199,9 +219,15 @@ void BinBatch(const tgx_index& index, const tgx_tile* tiles, std::size_t tileCou
         th.join();
     }
     // Concatenated in tile order.
-    descs.clear();
+    TileDescs(tiles, tileCount, kSide, descs);
     bins.clear();
     entries.clear();
+    AppendStrips(strips, per, bins, entries);
+}
+
+void TileDescs(const tgx_tile* tiles, std::size_t tileCount, std::uint32_t side,
+               std::vector<tg_tile_desc>& descs) {
+    descs.clear();
     std::uint64_t out = 0;
     std::uint64_t binCursor = 0;
     for (std::size_t t = 0; t < tileCount; ++t) {
@@ -214,15 +240,50 @@ void BinBatch(const tgx_index& index, const tgx_tile* tiles, std::size_t tileCou
         d.height = tile.height;
         d.out_offset = out;
         d.bin_first = std::uint32_t(binCursor);
-        d.bins_across = (tile.width + kSide - 1) / kSide;
+        d.bins_across = (tile.width + side - 1) / side;
         descs.push_back(d);
         out += std::uint64_t(tile.width) * tile.height;
-        binCursor += std::uint64_t(d.bins_across) * ((tile.height + kSide - 1) / kSide);
+        binCursor += std::uint64_t(d.bins_across) * ((tile.height + side - 1) / side);
         if (binCursor > 0xFFFFFFFFull) {
             throw std::invalid_argument("batch too large for 32-bit bin and entry indices");
         }
     }
-    AppendStrips(strips, per, bins, entries);
+}
+
+void WindowBatch(const tgx_index& index, const tgx_tile* tiles, std::size_t tileCount,
+                 std::uint32_t side, unsigned threads, std::vector<tg_tile_desc>& descs,
+                 std::vector<std::uint32_t>& tileWindows, std::vector<tgx_window>& windows) {
+    if (side != 8 && side != 16 && side != 32) {
+        throw std::invalid_argument("the bin side must be 8, 16 or 32 cells");
+    }
+    for (std::size_t t = 0; t < tileCount; ++t) {
+        CheckTile(tiles[t], t);
+        if (index.sphere && !(tiles[t].origin_y <= 90.0 &&
+                              tiles[t].origin_y - tiles[t].height * tiles[t].pixel >= -90.0)) {
+            throw std::invalid_argument("tile " + std::to_string(t) +
+                                        " reaches past a pole (its latitudes leave [-90, 90])");
+        }
+    }
+    TileDescs(tiles, tileCount, side, descs);
+    std::vector<std::vector<tgx_window>> per(tileCount);
+    ForEachIndex(tileCount, threads, [&](std::size_t t) {
+        std::vector<std::uint32_t> candidates;
+        if (index.sphere) {
+            SphereWindows(*index.sphere, tiles[t], 0, tiles[t].height, candidates, per[t]);
+        } else {
+            PlaneWindows(index, tiles[t], 0, tiles[t].height, candidates, per[t]);
+        }
+    });
+    tileWindows.assign(2 * tileCount, 0u);
+    windows.clear();
+    for (std::size_t t = 0; t < tileCount; ++t) {
+        tileWindows[2 * t] = std::uint32_t(windows.size());
+        tileWindows[2 * t + 1] = std::uint32_t(per[t].size());
+        windows.insert(windows.end(), per[t].begin(), per[t].end());
+        if (windows.size() > 0xFFFFFFFFull) {
+            throw std::invalid_argument("batch too large for 32-bit window indices");
+        }
+    }
 }
 
 std::vector<Strip> MakeStrips(const tgx_tile* tiles, std::size_t tileCount) {
@@ -393,6 +454,26 @@ void tgx_index_free(tgx_index* index) {
 
 size_t tgx_index_count(const tgx_index* index) {
     return index != nullptr ? index->cx.size() : 0;
+}
+
+int tgx_index_windows(const tgx_index* index, const tgx_tile* tiles, size_t tileCount,
+                      uint32_t binSide, unsigned threads, tg_tile_desc** descs,
+                      uint32_t** tileWindows, tgx_window** windows, size_t* windowCount,
+                      char* err, size_t errLen) {
+    return tgh::Guard(err, errLen, [&] {
+        if (index == nullptr || descs == nullptr || tileWindows == nullptr ||
+            windows == nullptr || windowCount == nullptr || (tileCount > 0 && tiles == nullptr)) {
+            throw std::invalid_argument("tgx_index_windows: null argument");
+        }
+        std::vector<tg_tile_desc> d;
+        std::vector<std::uint32_t> tw;
+        std::vector<tgx_window> w;
+        tgh::WindowBatch(*index, tiles, tileCount, binSide, threads, d, tw, w);
+        *descs = tgh::Export(d);
+        *tileWindows = tgh::Export(tw);
+        *windows = tgh::Export(w);
+        *windowCount = w.size();
+    });
 }
 
 int tgx_index_bin(const tgx_index* index, const tgx_tile* tiles, size_t tileCount,

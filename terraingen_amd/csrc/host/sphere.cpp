@@ -106,19 +106,25 @@ double LonHalfWidth(double lat, double arc) {
     return std::asin(s) / kDegree * (1.0 + 1e-12) + 1e-12;
 }
 
-// One tile's bins: per-bin counts and the entries in bin order.
-// Bin rows [row0, row1) of the tile (strips bin exactly as the whole tile: every bound below is
-// the tile's; only the candidates are gathered from the strip's latitudes).
-void BinSphereTile(const SphereCatalogue& index, const tgx_tile& tile, std::size_t row0,
-                   std::size_t row1, double unitLength, std::vector<std::uint32_t>& candidates,
-                   std::vector<std::uint32_t>& counts, std::vector<std::uint32_t>& entries) {
+}  // namespace
+
+// The windows of one tile: for each candidate resolved at the tile's spacing, in catalogue order,
+// the cells within reach of its footprint (a cell to spare each side), for the tile's rows
+// [rowFirst, rowEnd). Every bound is the whole tile's; only the candidates are gathered from the
+// rows' latitudes, so a strip of a large tile finds the craters that can reach it.
+void SphereWindows(const SphereCatalogue& index, const tgx_tile& tile, std::size_t rowFirst,
+                   std::size_t rowEnd, std::vector<std::uint32_t>& candidates,
+                   std::vector<tgx_window>& windows) {
     const double west = tile.origin_x, step = tile.pixel;
     const double north = tile.origin_y;
     const double span = tile.width * step;
-    const double stripNorth = north - double(row0 * kSide) * step;
+    // The rows' latitudes, two cells wider each way: a window reaches a cell and a half past the
+    // rows in reach of its crater, so every window meeting the rows is gathered, whichever rows
+    // the tile is cut into.
+    const double stripNorth = north - (double(rowFirst) - 2.0) * step;
     const double stripSouth =
-        north - double(std::min<std::size_t>(row1 * kSide, tile.height)) * step;
-    const double spacing = step * unitLength;
+        north - (double(std::min<std::size_t>(rowEnd, tile.height)) + 2.0) * step;
+    const double spacing = step * index.radius * kDegree;
     // Candidates: per diameter level whose largest crater the tile can resolve, the cells of the
     // tile's box grown by the level's largest footprint.
     candidates.clear();
@@ -161,8 +167,7 @@ void BinSphereTile(const SphereCatalogue& index, const tgx_tile& tile, std::size
     }
     std::sort(candidates.begin(), candidates.end());
     candidates.erase(std::unique(candidates.begin(), candidates.end()), candidates.end());
-    const std::size_t binsX = (tile.width + kSide - 1) / kSide;
-    std::vector<std::vector<std::uint32_t>> lists(binsX * (row1 - row0));
+    windows.clear();
     for (const std::uint32_t k : candidates) {
         if (!Resolved(index.diameter[k], spacing)) {
             continue;
@@ -189,27 +194,28 @@ void BinSphereTile(const SphereCatalogue& index, const tgx_tile& tile, std::size
         }
         // Cells from floor(a) - 1 to ceil(b) + 1 cover the band with a cell to spare each side.
         const auto first = [](double a, std::uint32_t n) {
-            return std::size_t(std::clamp(std::floor(a) - 1.0, 0.0, double(n - 1)));
+            return std::uint32_t(std::clamp(std::floor(a) - 1.0, 0.0, double(n - 1)));
         };
         const auto last = [](double b, std::uint32_t n) {
-            return std::size_t(std::clamp(std::ceil(b) + 1.0, 0.0, double(n - 1)));
+            return std::uint32_t(std::clamp(std::ceil(b) + 1.0, 0.0, double(n - 1)));
         };
-        const std::size_t c0 = first(colA, tile.width), c1 = last(colB, tile.width);
-        const std::size_t r0 = first(rowA, tile.height), r1 = last(rowB, tile.height);
-        const std::size_t by0 = std::max(r0 / kSide, row0);
-        const std::size_t by1 = std::min(r1 / kSide + 1, row1);
-        for (std::size_t by = by0; by < by1; ++by) {
-            for (std::size_t bx = c0 / kSide; bx <= c1 / kSide; ++bx) {
-                lists[(by - row0) * binsX + bx].push_back(k);
-            }
+        const tgx_window w{k, first(colA, tile.width), last(colB, tile.width),
+                           first(rowA, tile.height), last(rowB, tile.height)};
+        if (w.row1 < rowFirst || w.row0 >= rowEnd) {
+            continue;
         }
+        windows.push_back(w);
     }
-    counts.resize(lists.size());
-    entries.clear();
-    for (std::size_t b = 0; b < lists.size(); ++b) {
-        counts[b] = std::uint32_t(lists[b].size());
-        entries.insert(entries.end(), lists[b].begin(), lists[b].end());
-    }
+}
+
+namespace {
+
+// One tile's bins: per-bin counts and the entries in bin order, for bin rows [row0, row1).
+void BinSphereTile(const SphereCatalogue& index, const tgx_tile& tile, std::size_t row0,
+                   std::size_t row1, std::vector<std::uint32_t>& candidates, TileBins& out) {
+    std::vector<tgx_window> windows;
+    SphereWindows(index, tile, row0 * kSide, row1 * kSide, candidates, windows);
+    ListWindows(windows, tile, kSide, row0, row1, out);
 }
 
 // The (cell, listed crater) pairs of tiles [first, last) with the cell centre strictly within
@@ -338,7 +344,6 @@ void BinSphereBatch(const SphereCatalogue& catalogue, const tgx_tile* tiles,
                                         " reaches past a pole (its latitudes leave [-90, 90])");
         }
     }
-    const double unitLength = catalogue.radius * kDegree;
     const std::vector<Strip> strips = MakeStrips(tiles, tileCount);
     std::vector<TileBins> per(strips.size());
     std::atomic<std::size_t> next{0};
@@ -350,8 +355,7 @@ void BinSphereBatch(const SphereCatalogue& catalogue, const tgx_tile* tiles,
                 return;
             }
             const Strip& s = strips[n];
-            BinSphereTile(catalogue, tiles[s.tile], s.row0, s.row1, unitLength, candidates,
-                          per[n].counts, per[n].entries);
+            BinSphereTile(catalogue, tiles[s.tile], s.row0, s.row1, candidates, per[n]);
         }
     };
     if (threads == 0) {
@@ -366,29 +370,9 @@ void BinSphereBatch(const SphereCatalogue& catalogue, const tgx_tile* tiles,
     for (std::thread& th : pool) {
         th.join();
     }
-    d.clear();
+    TileDescs(tiles, tileCount, kSide, d);
     b.clear();
     e.clear();
-    std::uint64_t out = 0;
-    std::uint64_t binCursor = 0;
-    for (std::size_t t = 0; t < tileCount; ++t) {
-        const tgx_tile& tile = tiles[t];
-        tg_tile_desc desc{};
-        desc.origin_x = tile.origin_x;
-        desc.origin_y = tile.origin_y;
-        desc.pixel = tile.pixel;
-        desc.width = tile.width;
-        desc.height = tile.height;
-        desc.out_offset = out;
-        desc.bin_first = std::uint32_t(binCursor);
-        desc.bins_across = (tile.width + kSide - 1) / kSide;
-        d.push_back(desc);
-        out += std::uint64_t(tile.width) * tile.height;
-        binCursor += std::uint64_t(desc.bins_across) * ((tile.height + kSide - 1) / kSide);
-        if (binCursor > 0xFFFFFFFFull) {
-            throw std::invalid_argument("batch too large for 32-bit bin and entry indices");
-        }
-    }
     AppendStrips(strips, per, b, e);
 }
 

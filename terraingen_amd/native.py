@@ -154,6 +154,11 @@ HIP_ABI = {
     "tg_hip_memcpy_d2d_async": (C.c_int, [VP, VP, VP, SZ, VP, CP, SZ]),
     "tg_hip_batch_tune": (C.c_int, [VP, u32, u32, u32, u32, CP, SZ]),
     "tg_hip_batch_surface": (C.c_int, [VP, C.c_double, CP, SZ]),
+    "tg_hip_batch_stage_windows": (C.c_int, [VP, VP, SZ, u32, VP, VP, SZ, C.POINTER(SZ), CP,
+                                             SZ]),
+    "tg_hip_batch_read_bins": (C.c_int, [VP, VP, SZ, VP, SZ, C.POINTER(SZ), C.POINTER(SZ), CP,
+                                         SZ]),
+    "tg_hip_batch_cull_depth": (C.c_int, [VP, u32, CP, SZ]),
     "tg_hip_batch_run_static": (C.c_int, [VP, VP, SZ, VP, SZ, VP, SZ, VP, C.c_float, VP,
                                           C.c_double, VP, SZ, VP, CP, SZ]),
     "tg_hip_catalogue_frames": (C.c_int, [VP, VP, SZ, CP, SZ]),
@@ -673,6 +678,10 @@ HOST_ABI["tgx_sphere_index_create"] = (C.c_int, [VP, VP, VP, VP, SZ, C.c_double,
 HOST_ABI["tgx_index_count"] = (SZ, [VP])
 HOST_ABI["tgx_index_bin"] = (C.c_int, [VP, VP, SZ, C.c_uint, C.POINTER(VP), C.POINTER(VP),
                                        C.POINTER(SZ), C.POINTER(VP), C.POINTER(SZ), CP, SZ])
+HOST_ABI["tgx_index_windows"] = (C.c_int, [VP, VP, SZ, u32, C.c_uint, C.POINTER(VP),
+                                           C.POINTER(VP), C.POINTER(VP), C.POINTER(SZ), CP, SZ])
+WINDOW_DTYPE = np.dtype([("crater", np.uint32), ("col0", np.uint32), ("col1", np.uint32),
+                         ("row0", np.uint32), ("row1", np.uint32)])
 
 
 def _take_batch(lib, tiles_n, dp, bp, bn, ep, en):
@@ -806,6 +815,26 @@ class CatalogueIndex:
                                  C.byref(dp), C.byref(bp), C.byref(bn), C.byref(ep),
                                  C.byref(en), err, 512), err)
         return _take_batch(lib, tiles.size, dp, bp, bn, ep, en)
+
+    def windows(self, tiles: np.ndarray, bin_side: int = 16, threads: int = 0):
+        """``tgx_index_windows``: (descs for bins of `bin_side` cells, per-tile (first, count) of
+        windows, WINDOW_DTYPE windows), for binning on the device."""
+        lib = host_library()
+        tiles = np.ascontiguousarray(tiles, TILE_DTYPE)
+        dp, tp, wp = VP(), VP(), VP()
+        wn = SZ()
+        err = C.create_string_buffer(512)
+        _check(lib.tgx_index_windows(self.handle, tiles.ctypes.data, tiles.size, bin_side,
+                                     threads, C.byref(dp), C.byref(tp), C.byref(wp),
+                                     C.byref(wn), err, 512), err)
+        try:
+            descs = _owned_copy(dp, tiles.size * 48).view(TILE_DESC_DTYPE)
+            tile_windows = _owned_copy(tp, tiles.size * 8).view(np.uint32)
+            windows = _owned_copy(wp, wn.value * 20).view(WINDOW_DTYPE)
+        finally:
+            for p in (dp, tp, wp):
+                lib.tgx_free(p)
+        return descs, tile_windows, windows
 
     def __del__(self):
         if getattr(self, "handle", None):

@@ -267,27 +267,69 @@ class TileBatch:
         self.timings.update(generate_ms=(t1 - t0) * 1e3, prepare_ms=(t2 - t1) * 1e3,
                             catalogue_upload_ms=(t3 - t2) * 1e3)
 
-    def stage(self, tiles: np.ndarray, threads: int = 0) -> None:
+    def stage(self, tiles: np.ndarray, threads: int = 0, device_bins: bool = False,
+              bin_side: int = 16) -> None:
+        """Bins `tiles` against the catalogue and stages them on the batch: on the host into
+        16-cell bins (tgx_index_bin, tg_hip_batch_stage), or with `device_bins` the host finds
+        each tile's crater windows (tgx_index_windows) and the device bins them into bins of
+        `bin_side` cells (tg_hip_batch_stage_windows)."""
         t0 = time.perf_counter()
         self.tiles = np.ascontiguousarray(tiles, N.TILE_DTYPE)
-        self.descs, self.bins, self.entries = self.index.bin(self.tiles, threads)
+        if device_bins:
+            self.descs, tile_windows, windows = self.index.windows(self.tiles, bin_side, threads)
+            s = bin_side
+            w = self.tiles["width"].astype(np.int64)
+            h = self.tiles["height"].astype(np.int64)
+            bins = int(((w + s - 1) // s * ((h + s - 1) // s)).sum())
+            across = windows["col1"].astype(np.int64) // s - windows["col0"] // s + 1
+            down = windows["row1"].astype(np.int64) // s - windows["row0"] // s + 1
+            entries = int((across * down).sum())
+            self.bins = self.entries = None
+        else:
+            self.descs, self.bins, self.entries = self.index.bin(self.tiles, threads)
+            bins, entries, bin_side = self.bins.size, self.entries.size, 16
         t1 = time.perf_counter()
-        if self.batch is None or self._capacity < (self.tiles.size, self.bins.size,
-                                                   self.entries.size):
+        need = (self.tiles.size, bins, entries)
+        if self.batch is None or any(c < n for c, n in zip(self._capacity, need)):
             if self.batch is not None:
                 self.dev.lib.tg_hip_batch_free(self.batch)
-            cap = (int(self.tiles.size * 1.25) + 1, int(self.bins.size * 1.25) + 1,
-                   int(self.entries.size * 1.25) + 1)
+            cap = (int(self.tiles.size * 1.25) + 1, int(bins * 1.25) + 1,
+                   int(entries * 1.25) + 1)
             b = C.c_void_p()
             self.dev.call("tg_hip_batch_create", self.dev.handle, *cap, C.byref(b))
             self.batch, self._capacity = b, cap
-        self.dev.call("tg_hip_batch_stage", self.batch, self.descs.ctypes.data, self.descs.size,
-                      self.bins.ctypes.data, self.bins.size, self.entries.ctypes.data,
-                      self.entries.size)
+            if getattr(self, "radius", 0.0) > 0.0:
+                self.dev.call("tg_hip_batch_surface", self.batch, self.radius)
+        if device_bins:
+            staged = N.SZ()
+            self.dev.call("tg_hip_batch_stage_windows", self.batch, self.descs.ctypes.data,
+                          self.descs.size, bin_side, tile_windows.ctypes.data,
+                          windows.ctypes.data, windows.size, C.byref(staged))
+            if staged.value != entries:
+                raise RuntimeError(f"the device listed {staged.value} entries, the windows give "
+                                   f"{entries}")
+        else:
+            self.dev.call("tg_hip_batch_stage", self.batch, self.descs.ctypes.data,
+                          self.descs.size, self.bins.ctypes.data, self.bins.size,
+                          self.entries.ctypes.data, self.entries.size)
         t2 = time.perf_counter()
+        self.bin_side = bin_side
+        self.bin_count = bins
         self.cells = int((self.tiles["width"].astype(np.int64) *
                           self.tiles["height"].astype(np.int64)).sum())
         self.timings.update(bin_ms=(t1 - t0) * 1e3, stage_ms=(t2 - t1) * 1e3)
+
+    def read_bins(self) -> None:
+        """The staged bins and entries in host memory (tg_hip_batch_read_bins), for a batch the
+        device binned."""
+        nb, ne = N.SZ(), N.SZ()
+        self.dev.call("tg_hip_batch_read_bins", self.batch, None, 0, None, 0, C.byref(nb),
+                      C.byref(ne))
+        bins = np.zeros(nb.value, N.BIN_DTYPE)
+        entries = np.zeros(max(ne.value, 1), np.uint32)
+        self.dev.call("tg_hip_batch_read_bins", self.batch, bins.ctypes.data, bins.size,
+                      entries.ctypes.data, entries.size, C.byref(nb), C.byref(ne))
+        self.bins, self.entries = bins, entries[:ne.value]
 
     @property
     def stream(self):
@@ -338,8 +380,11 @@ class TileBatch:
     # ---- the plain per-tile path ---------------------------------------------------------
     def tile_entries(self, i: int) -> np.ndarray:
         """Catalogue rows (apply order) that tile i's bins list, each once, in order."""
+        if self.bins is None:
+            self.read_bins()
         d = self.descs[i]
-        nb = int(d["bins_across"]) * ((int(d["height"]) + 15) // 16)
+        side = getattr(self, "bin_side", 16)
+        nb = int(d["bins_across"]) * ((int(d["height"]) + side - 1) // side)
         first = self.bins[int(d["bin_first"])]
         last = self.bins[int(d["bin_first"]) + nb - 1]
         # A tile's bins list their entries one after another.
@@ -477,30 +522,6 @@ class SphereTileBatch(TileBatch):
             self.noise_kernel = kernel
         self.batch = None
         self.timings.update(prepare_ms=(t1 - t0) * 1e3, catalogue_upload_ms=(t2 - t1) * 1e3)
-
-    def stage(self, tiles: np.ndarray, threads: int = 0) -> None:
-        t0 = time.perf_counter()
-        self.tiles = np.ascontiguousarray(tiles, N.TILE_DTYPE)
-        self.descs, self.bins, self.entries = N.sphere_bin(self.foot, self.radius, self.tiles,
-                                                           threads)
-        t1 = time.perf_counter()
-        if self.batch is None or self._capacity < (self.tiles.size, self.bins.size,
-                                                   self.entries.size):
-            if self.batch is not None:
-                self.dev.lib.tg_hip_batch_free(self.batch)
-            cap = (int(self.tiles.size * 1.25) + 1, int(self.bins.size * 1.25) + 1,
-                   int(self.entries.size * 1.25) + 1)
-            b = C.c_void_p()
-            self.dev.call("tg_hip_batch_create", self.dev.handle, *cap, C.byref(b))
-            self.batch, self._capacity = b, cap
-            self.dev.call("tg_hip_batch_surface", self.batch, self.radius)
-        self.dev.call("tg_hip_batch_stage", self.batch, self.descs.ctypes.data, self.descs.size,
-                      self.bins.ctypes.data, self.bins.size, self.entries.ctypes.data,
-                      self.entries.size)
-        t2 = time.perf_counter()
-        self.cells = int((self.tiles["width"].astype(np.int64) *
-                          self.tiles["height"].astype(np.int64)).sum())
-        self.timings.update(bin_ms=(t1 - t0) * 1e3, stage_ms=(t2 - t1) * 1e3)
 
     def tile_spacing(self, i: int) -> float:
         """RasterGrid::Spacing of tile i: its pixel (degrees) times the unit length."""
