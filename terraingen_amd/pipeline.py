@@ -342,25 +342,48 @@ class TileBatch:
 
     def autotune(self, out: N.DeviceArray,
                  walks=(N.TG_WALK_TOGETHER, N.TG_WALK_PER_LANE, N.TG_WALK_CULLED),
-                 groups=(4, 8), chunks=(2, 8, 32), repeats: int = 2) -> dict:
-        """Times every (crater walk, workgroups per CU, bins per queue take) on the staged batch
-        with HIP events on its stream, keeps the fastest and returns {(walk, groups, chunk): ms}.
-        Every choice gives the same heights, so it changes only the time."""
+                 groups=(4, 8), chunks=(2, 8, 32), depths=(64,), bin_sides=None,
+                 repeats: int = 2) -> dict:
+        """Times the launch choices on the staged tiles with HIP events on the batch's stream,
+        keeps the fastest and returns {(bin side, walk, groups per CU, chunk, depth): ms}: the
+        crater walk, workgroups per CU and bins per queue take first (at the staged bin side and
+        a depth of 64), then, at the best of those, each culled-walk depth and, with
+        `bin_sides`, each bin side (restaging the tiles with device binning). Every choice gives
+        the same heights, so it changes only the time."""
         timings = {}
+
+        def time_one(key):
+            self.run(out, use_graph=False)  # warm: code objects, caches
+            a, b = N.Event(self.dev), N.Event(self.dev)
+            a.record(self.stream)
+            for _ in range(repeats):
+                self.run(out, use_graph=False)
+            b.record(self.stream)
+            self.wait()
+            timings[key] = a.elapsed_ms(b) / repeats
+
+        side = getattr(self, "bin_side", 16)
         for walk in walks:
             for g in groups:
                 for chunk in chunks:
                     self.tune(walk, g, True, chunk)
-                    self.run(out, use_graph=False)  # warm: code objects, caches
-                    a, b = N.Event(self.dev), N.Event(self.dev)
-                    a.record(self.stream)
-                    for _ in range(repeats):
-                        self.run(out, use_graph=False)
-                    b.record(self.stream)
-                    self.wait()
-                    timings[(walk, g, chunk)] = a.elapsed_ms(b) / repeats
+                    time_one((side, walk, g, chunk, 64))
         best = min(timings, key=timings.get)
-        self.tune(best[0], best[1], True, best[2])
+        _, walk, g, chunk, _ = best
+        for s in (bin_sides or (side,)):
+            if s != getattr(self, "bin_side", 16):
+                self.stage(self.tiles, device_bins=True, bin_side=s)
+            self.tune(walk, g, True, chunk)
+            for depth in depths:
+                if (s, walk, g, chunk, depth) in timings:
+                    continue
+                self.dev.call("tg_hip_batch_cull_depth", self.batch, depth)
+                time_one((s, walk, g, chunk, depth))
+        best = min(timings, key=timings.get)
+        if best[0] != getattr(self, "bin_side", 16):
+            self.stage(self.tiles, device_bins=True, bin_side=best[0])
+        self.tune(best[1], best[2], True, best[3])
+        self.dev.call("tg_hip_batch_cull_depth", self.batch, best[4])
         self.tuned = best
         return timings
 
