@@ -164,6 +164,43 @@ typedef struct tgx_boulder {
     uint32_t model; /* 1: placed as a 3D model, not drawn into the heights */
 } tgx_boulder;
 
+/* A boulder layer with generated boulders of power-law sizes, N(>=D) = sfd_coefficient
+ * D^-sfd_exponent per m^2 for D in [min_d, max_d] (BoulderSizes with a PowerLawSfd), seed `seed`,
+ * the pose ranges of BoulderPose (the reference's defaults: height 0.4-0.6 D, burial 0.1-0.4 of
+ * the height, lean up to 0.26 rad), and no density or size-slope map, patches, clusters, bedrock
+ * exposure, crater ejecta, fragmentation, rolling, settling or ventifacts; over the region
+ * [min_x, max_x) x [min_y, max_y) (meters on a plane; longitude and latitude in degrees on a
+ * sphere of `sphere_radius` meters when that is positive) for a grid of `spacing` meters (0: every
+ * size). Candidates are drawn on `threads` threads (0: all), which does not change the list. */
+typedef struct tgx_boulder_layer {
+    double min_x, min_y, max_x, max_y;
+    double sphere_radius;
+    double spacing;
+    double sfd_coefficient, sfd_exponent, min_d, max_d;
+    double min_height_ratio, max_height_ratio;
+    double min_burial, max_burial;
+    double max_lean_degrees;
+    uint32_t seed;
+    uint32_t threads;
+} tgx_boulder_layer;
+
+/* A placed boulder (the Boulder fields the layer draws, boulder.h:52-106): centre, diameter,
+ * height above its base, buried share of the height, yaw and lean (radians), wear and ID. */
+typedef struct tgx_placed_boulder {
+    double x, y, diameter, height, burial, yaw, lean, wear;
+    uint64_t id;
+    uint32_t model;
+    uint32_t pad;
+} tgx_placed_boulder;
+
+/* BoulderLayer::Boulders over the layer's region for its spacing (src/layers/boulders/
+ * boulder_layer.cpp:390-400): the candidates GenerateCraters draws with the layer's candidate
+ * sizes (ScatterBoulderCandidates, :263-294), each thinned by its Keep draw (Thin, :338-352) and
+ * posed by its Pose and Wear draws (DrawPose, boulder_pose.cpp:41-77), in candidate order,
+ * bit-identical to the reference's list. */
+int tgx_place_boulders(const tgx_boulder_layer* layer, tgx_placed_boulder** boulders,
+                       size_t* count, char* err, size_t err_len);
+
 /* BoulderLayer::PrepareRelief for one grid `spacing` (boulder_relief.cpp:373-423): the boulders
  * drawn into heights (not models, resolution weight above 0) as device records, in list order,
  * with the parameter block (`dome` 1: the dome averaged over a band one sample wide, the default

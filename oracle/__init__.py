@@ -179,6 +179,7 @@ def _declare(lib: C.CDLL) -> None:
         "ref_generate_sphere": (C.c_long, [C.POINTER(RefSphereGeneration), vp, vp, vp, vp, vp,
                                            C.c_long]),
         "ref_placement_error": (C.c_char_p, []),
+        "ref_place_boulders": (C.c_long, [vp, vp, vp, vp, C.c_long]),
         "ref_write_geotiff": (C.c_int, [C.c_char_p, vp, u, u, u, vp, C.c_int, f, vp, sz, vp, sz,
                                         C.c_char_p, C.c_int, u, C.c_int]),
         "ref_write_crater_csv": (C.c_int, [C.c_char_p, vp, vp, vp, vp, vp, vp, vp, sz, vp, vp]),
@@ -237,3 +238,23 @@ def ref_write_crater_csv(lib: C.CDLL, path, craters: np.ndarray, manual=None, re
                                   cx.ctypes.data if attributes is not None else None)
     if rc != 0:
         raise RuntimeError(lib.ref_output_error().decode())
+
+
+def reference_boulders(lib: C.CDLL, spec) -> np.ndarray:
+    """The reference's BoulderLayer::Boulders for a BoulderLayerSpec (ref_place_boulders), in
+    PLACED_BOULDER_DTYPE."""
+    from terraingen_amd import native as N
+
+    n = lib.ref_place_boulders(C.byref(spec), None, None, None, 0)
+    if n < 0:
+        raise RuntimeError(lib.ref_placement_error().decode())
+    values = np.zeros((max(n, 1), 8))
+    ids = np.zeros(max(n, 1), np.uint64)
+    models = np.zeros(max(n, 1), np.uint8)
+    lib.ref_place_boulders(C.byref(spec), values.ctypes.data, ids.ctypes.data,
+                           models.ctypes.data, n)
+    out = np.zeros(n, N.PLACED_BOULDER_DTYPE)
+    for k, name in enumerate(("x", "y", "diameter", "height", "burial", "yaw", "lean", "wear")):
+        out[name] = values[:n, k]
+    out["id"], out["model"] = ids[:n], models[:n]
+    return out

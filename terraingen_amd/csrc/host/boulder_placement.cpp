@@ -1,0 +1,121 @@
+// SPDX-License-Identifier: BSD-3-Clause
+//
+// Boulder placement (tgx_place_boulders): the boulders a BoulderLayer lists over a region for a
+// grid spacing, bit-identical to the reference's (BoulderLayer::Boulders and PlaceBoulders,
+// src/layers/boulders/boulder_layer.cpp:263-420), for a layer whose boulders are generated with
+// power-law sizes, without density or size-slope maps, patches, clusters, bedrock exposure,
+// ejecta, fragmentation, rolling, settling or ventifacts:
+//   candidates  GenerateCraters over the region with the layer's candidate sizes (the sizes
+//               weighted by the richest background abundance, 1 here: SummedSfd) and fresh ages,
+//               seeded SubstreamSeed(seed, kBoulderCandidateStream), candidate margins of one
+//               radius, the smallest resolved diameter the fade's start times the spacing
+//               (ScatterBoulderCandidates, :263-294);
+//   thinning    each candidate kept when a draw of its Keep stream is below its density over the
+//               candidates' (Thin, :338-352): with a uniform background the two are equal and
+//               every candidate is kept;
+//   pose        each kept candidate's height, burial, yaw and lean from its Pose stream and its
+//               wear from its Wear stream (DrawPose and Wear, boulder_pose.cpp:28-77).
+// Streams are Philox substreams keyed (seed, kBoulderAttributeStream, id low, id high, draw).
+
+#include "common.h"
+#include "placement.h"
+#include "random.h"
+#include "terraingen_amd.h"
+
+#include <cmath>
+#include <stdexcept>
+#include <vector>
+
+namespace tgh {
+namespace {
+
+constexpr std::uint32_t kBoulderCandidateStream = 0x524F434Bu;  // streams.h:31
+constexpr std::uint32_t kBoulderAttributeStream = 0x41545452u;  // streams.h:32
+// BoulderLayer::BoulderDraw (boulder_layer.h:181).
+enum class Draw : std::uint32_t { Keep, Pose, Wear };
+constexpr double kPi = 3.141592653589793238462643383279502884;
+constexpr double kDegree = kPi / 180.0;  // geometry.h:15
+
+Stream AttributeStream(std::uint32_t seed, std::uint64_t id, Draw draw) {
+    return Stream(seed, kBoulderAttributeStream, static_cast<std::uint32_t>(id),
+                  static_cast<std::uint32_t>(id >> 32), static_cast<std::uint32_t>(draw));
+}
+
+double Lerp(double lo, double hi, double u) {
+    return lo + (hi - lo) * u;  // ValueRange::Lerp (boulder_shape.h:25-27)
+}
+
+std::vector<tgx_placed_boulder> Place(const tgx_boulder_layer& o) {
+    if (!(o.sfd_coefficient > 0.0 && o.sfd_exponent > 0.0)) {
+        throw std::invalid_argument("boulder sizes need a positive coefficient and exponent");
+    }
+    if (!(o.spacing >= 0.0) || !std::isfinite(o.spacing)) {
+        throw std::invalid_argument("the spacing must be finite and not negative");
+    }
+    const SummedPowerLaws candidates({{1.0, PowerLawDensity(o.sfd_coefficient, o.sfd_exponent)}});
+    Ages fresh;  // FreshAges: every candidate of age 0, nothing drawn
+    GenerationOptions g;
+    g.minDiameter = o.min_d;
+    g.maxDiameter = o.max_d;
+    g.marginRadii = 1.0;  // OwnMarginRadii of the dome relief, no fillet; nothing breaks
+    g.cellSizeFactor = 16.0;
+    g.minResolvedDiameter = o.spacing > 0.0 ? 2.0 * o.spacing : 0.0;  // fade.startSamples 2
+    g.threads = o.threads;
+    const std::uint32_t seed = SubstreamSeed(o.seed, kBoulderCandidateStream);
+    const std::vector<Crater> drawn =
+        o.sphere_radius > 0.0
+            ? GenerateCratersOnSphere(o.sphere_radius, {{o.min_x, o.min_y, o.max_x, o.max_y}},
+                                      candidates, fresh, seed, g)
+            : GenerateCraters(o.min_x, o.min_y, o.max_x, o.max_y, candidates, fresh, seed, g);
+    std::vector<tgx_placed_boulder> out;
+    out.reserve(drawn.size());
+    for (const Crater& c : drawn) {
+        // Thin: the density over the candidates' at this diameter, with a background of 1.
+        const double density = 1.0 * PowerLawDensity(o.sfd_coefficient, o.sfd_exponent)
+                                         .Incremental(c.diameter);
+        Stream keep = AttributeStream(o.seed, c.id, Draw::Keep);
+        if (keep.NextDouble() >= density / candidates.Incremental(c.diameter)) {
+            continue;
+        }
+        Stream pose = AttributeStream(o.seed, c.id, Draw::Pose);
+        const double ratio = Lerp(o.min_height_ratio, o.max_height_ratio, pose.NextDouble());
+        Stream wearStream = AttributeStream(o.seed, c.id, Draw::Wear);
+        (void)wearStream.NextDouble();  // whether it was thrown from a crater: none here
+        const double wear = wearStream.NextDouble();
+        const double share = pose.NextDouble();
+        const double burial = Lerp(o.min_burial, o.max_burial, share);
+        const double yaw = 2.0 * kPi * pose.NextDouble();
+        const double lean = (o.max_lean_degrees * kDegree) * pose.NextDouble();
+        tgx_placed_boulder b{};
+        b.x = c.x;
+        b.y = c.y;
+        b.diameter = c.diameter;
+        b.height = ratio * c.diameter;
+        b.burial = burial;
+        b.yaw = yaw;
+        b.lean = lean;
+        b.wear = wear;
+        b.id = c.id;
+        out.push_back(b);
+    }
+    return out;
+}
+
+}  // namespace
+}  // namespace tgh
+
+extern "C" {
+
+int tgx_place_boulders(const tgx_boulder_layer* layer, tgx_placed_boulder** boulders,
+                       size_t* count, char* err, size_t errLen) {
+    return tgh::Guard(err, errLen, [&] {
+        if (layer == nullptr || boulders == nullptr || count == nullptr) {
+            throw std::invalid_argument("tgx_place_boulders: null argument");
+        }
+        const std::vector<tgx_placed_boulder> out = tgh::Place(*layer);
+        *boulders = tgh::Export(out);
+        *count = out.size();
+    });
+}
+
+}  // extern "C"

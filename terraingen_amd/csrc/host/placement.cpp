@@ -171,8 +171,8 @@ double LunarDensity::ExpectedCount(double area, double minD, double maxD) const 
     return area * (m_surface.DensityChecked(minD) - upper);
 }
 
-double LunarDensity::Invert(double target, double minD, double maxD) const {
-    const auto cumulative = [this](double d) { return m_surface.Density(d); };
+double InvertCumulative(const std::function<double(double)>& cumulative, double target,
+                        double minD, double maxD) {
     double low = std::log(minD);
     double high = std::isinf(maxD) ? low + 1.0 : std::log(maxD);
     while (std::isinf(maxD) && cumulative(std::exp(high)) > target) {
@@ -217,6 +217,60 @@ double LunarDensity::Invert(double target, double minD, double maxD) const {
         (!(cumulative(std::exp(middle)) > target) ? high : low) = middle;
     }
     return std::exp(0.5 * (low + high));
+}
+
+double LunarDensity::Invert(double target, double minD, double maxD) const {
+    return InvertCumulative([this](double d) { return m_surface.Density(d); }, target, minD, maxD);
+}
+
+double SummedPowerLaws::Cumulative(double d) const {
+    if (!(d > 0.0 && std::isfinite(d))) {
+        throw std::domain_error("Diameter must be positive and finite");
+    }
+    double sum = 0.0;
+    for (const Term& t : m_terms) {
+        sum += t.weight * (t.law.coefficient * std::pow(d, -t.law.exponent));
+    }
+    return sum;
+}
+
+double SummedPowerLaws::Incremental(double d) const {
+    if (!(d > 0.0 && std::isfinite(d))) {
+        throw std::domain_error("Diameter must be positive and finite");
+    }
+    double sum = 0.0;
+    for (const Term& t : m_terms) {
+        sum += t.weight * t.law.Incremental(d);
+    }
+    return sum;
+}
+
+double SummedPowerLaws::ExpectedCount(double area, double minD, double maxD) const {
+    if (!(minD > 0.0 && std::isfinite(minD) && maxD > minD) || std::isnan(maxD)) {
+        throw std::invalid_argument("Diameter interval must satisfy 0 < minDiameter < maxDiameter");
+    }
+    if (!(area >= 0.0 && std::isfinite(area))) {
+        throw std::invalid_argument("Area must be finite and not negative");
+    }
+    const double upper = std::isinf(maxD) ? 0.0 : Cumulative(maxD);
+    return area * (Cumulative(minD) - upper);
+}
+
+double SummedPowerLaws::SampleDiameter(Stream& random, double minD, double maxD) const {
+    const double count = ExpectedCount(1.0, minD, maxD);
+    if (!(count > 0.0)) {
+        throw std::invalid_argument("The distribution has no mass in the diameter interval");
+    }
+    const double top = Cumulative(minD);
+    const double target = top - random.NextDouble() * count;
+    const auto cumulative = [this](double d) {
+        double sum = 0.0;
+        for (const Term& t : m_terms) {
+            sum += t.weight * (t.law.coefficient * std::pow(d, -t.law.exponent));
+        }
+        return sum;
+    };
+    return std::clamp(InvertCumulative(cumulative, target, minD, maxD), minD, maxD);
 }
 
 double LunarDensity::SampleDiameter(Stream& random, double minD, double maxD) const {

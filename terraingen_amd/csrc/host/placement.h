@@ -10,6 +10,7 @@
 #include <algorithm>
 #include <cmath>
 #include <cstdint>
+#include <functional>
 #include <stdexcept>
 #include <vector>
 
@@ -42,9 +43,40 @@ struct PowerLawDensity final : Density {
     double Invert(double target) const {
         return std::pow(target / coefficient, -1.0 / exponent);
     }
+    // PowerLawSegment::Incremental (power_law.h:26-28): -dN/dD = c e D^(-e-1).
+    double Incremental(double d) const {
+        return coefficient * exponent * std::pow(d, -exponent - 1.0);
+    }
     double ExpectedCount(double area, double minD, double maxD) const override;
     double SampleDiameter(Stream& random, double minD, double maxD) const override;
 };
+
+// A weighted sum of power laws, N(>=D) = sum of w c D^-e (SummedSfd, src/distribution/
+// summed.cpp:35-49, over PowerLawSfd terms), sampled as any distribution without a closed-form
+// inverse is (SizeFrequencyDistribution::SampleDiameter and InvertCumulative,
+// size_frequency.cpp:95-140): the boulder layer's candidate sizes.
+class SummedPowerLaws final : public Density {
+  public:
+    struct Term {
+        double weight;
+        PowerLawDensity law;
+    };
+    explicit SummedPowerLaws(std::vector<Term> terms) : m_terms(std::move(terms)) {}
+    double Cumulative(double d) const;
+    // IncrementalAt: the sum of the terms' incremental densities, -dN/dD.
+    double Incremental(double d) const;
+    double ExpectedCount(double area, double minD, double maxD) const override;
+    double SampleDiameter(Stream& random, double minD, double maxD) const override;
+
+  private:
+    std::vector<Term> m_terms;
+};
+
+// SizeFrequencyDistribution::InvertCumulative (size_frequency.cpp:95-113): the diameter in
+// [minD, maxD] where `cumulative` falls to `target`, by Illinois steps on log N over log D
+// (InvertBySecant, :34-61), else by bisection.
+double InvertCumulative(const std::function<double(double)>& cumulative, double target,
+                        double minD, double maxD);
 
 // The density of the reference's default crater models (lunar_models.h).
 class LunarDensity final : public Density {

@@ -970,6 +970,50 @@ HOST_ABI["tgx_prepare_boulders"] = (C.c_int, [VP, SZ, C.c_double, C.c_int,
                                               C.POINTER(SZ), C.POINTER(VP), CP, SZ])
 
 
+BoulderLayerSpec = _struct("BoulderLayerSpec", [(n, C.c_double) for n in (
+    "min_x", "min_y", "max_x", "max_y", "sphere_radius", "spacing", "sfd_coefficient",
+    "sfd_exponent", "min_d", "max_d", "min_height_ratio", "max_height_ratio", "min_burial",
+    "max_burial", "max_lean_degrees")] + [("seed", u32), ("threads", u32)])
+PLACED_BOULDER_DTYPE = np.dtype([(n, np.float64) for n in (
+    "x", "y", "diameter", "height", "burial", "yaw", "lean", "wear")] +
+    [("id", np.uint64), ("model", np.uint32), ("pad", np.uint32)])
+assert PLACED_BOULDER_DTYPE.itemsize == 80
+HOST_ABI["tgx_place_boulders"] = (C.c_int, [C.POINTER(BoulderLayerSpec), C.POINTER(VP),
+                                            C.POINTER(SZ), CP, SZ])
+# BoulderPose's defaults (boulder_options.h:247-252): lean up to 0.26 rad.
+BOULDER_POSE_DEFAULTS = dict(min_height_ratio=0.4, max_height_ratio=0.6, min_burial=0.1,
+                             max_burial=0.4, max_lean_degrees=0.26 * 180.0 / np.pi)
+
+
+def place_boulders(region: tuple, spacing: float, sfd_coefficient: float, sfd_exponent: float,
+                   min_d: float, max_d: float, seed: int = 1, sphere_radius: float = 0.0,
+                   threads: int = 0, **pose) -> np.ndarray:
+    """``tgx_place_boulders``: the boulders a power-law BoulderLayer lists over `region`
+    (min_x, min_y, max_x, max_y; degrees on a sphere) for a grid of `spacing` meters, as the
+    reference lists them, in PLACED_BOULDER_DTYPE."""
+    spec = BoulderLayerSpec(*region, sphere_radius, spacing, sfd_coefficient, sfd_exponent,
+                            min_d, max_d, **(BOULDER_POSE_DEFAULTS | pose), seed=seed,
+                            threads=threads)
+    lib = host_library()
+    ptr, n = VP(), SZ()
+    err = C.create_string_buffer(512)
+    _check(lib.tgx_place_boulders(C.byref(spec), C.byref(ptr), C.byref(n), err, 512), err)
+    try:
+        return _owned_copy(ptr, n.value * 80).view(PLACED_BOULDER_DTYPE)
+    finally:
+        lib.tgx_free(ptr)
+
+
+def relief_boulders(placed: np.ndarray) -> np.ndarray:
+    """BOULDER_DTYPE of placed boulders for the dome relief: centre, diameter, exposed height
+    (Boulder::ExposedHeight, height times one less the burial), model flag, no facets."""
+    out = np.zeros(placed.size, BOULDER_DTYPE)
+    out["x"], out["y"], out["diameter"] = placed["x"], placed["y"], placed["diameter"]
+    out["exposed"] = placed["height"] * (1.0 - placed["burial"])
+    out["model"] = placed["model"]
+    return out
+
+
 def prepare_boulders(boulders: np.ndarray, spacing: float, dome: bool = True):
     """``tgx_prepare_boulders``: (kernel block, records, footprint[n, 4]: x, y, reach, D)."""
     lib = host_library()
