@@ -359,6 +359,14 @@ int tgx_server_create(tg_dev* dev, const tg_noise_plan* plan, float elevation,
                       const tg_catalogue* catalogue, const tgx_index* index,
                       const tgx_server_config* config, tgx_server** out, char* err,
                       size_t err_len);
+/* A boulder layer drawn after the craters in every served tile (no counterpart in the reference):
+ * the device's boulder catalogue (tg_hip_boulder_catalogue_create) and an index of the boulders'
+ * relief boxes in the same order (tgx_index_create, or tgx_sphere_index_create on the craters'
+ * sphere), each batch's boulder lists binned against it and staged with the craters'
+ * (tg_hip_batch_stage_boulders). Set once, before the first submit; both must outlive the
+ * server. */
+int tgx_server_boulders(tgx_server* server, const tg_boulder_catalogue* catalogue,
+                        const tgx_index* index, char* err, size_t err_len);
 /* Thread-safe. `dst` receives width x height floats, row-major from the north-west cell, and must
  * stay valid until the ticket is waited for. */
 int tgx_server_submit(tgx_server* server, const tgx_tile* tile, float* dst, uint64_t* ticket,

@@ -133,6 +133,7 @@ int tg_hip_noise_fill(tg_dev* dev, const tg_noise_plan* plan, float elevation, f
  * (src/layers/craters/shaping/apply.cpp:181-188). `lows` holds `width` column remainders then one
  * per grid row. Frames may be null on a plane. */
 typedef struct tg_crater_set tg_crater_set;
+typedef struct tg_boulder_set tg_boulder_set;
 
 int tg_hip_crater_set_create(tg_dev* dev, const tg_crater_shape* kernel,
                              const tg_crater_record* records, size_t record_count,
@@ -147,6 +148,15 @@ int tg_hip_crater_set_free(tg_crater_set* set);
 int tg_hip_crater_launch(tg_dev* dev, const tg_crater_set* set, float* heights_dev,
                          const float* xs_dev, const float* ys_dev, const tg_kernel_params* params,
                          void* stream, char* err, size_t err_len);
+/* tg_hip_crater_launch with a boulder set of the same grid drawn in the same pass (no counterpart
+ * in the reference, which bakes boulders on the CPU after its crater pass): each cell takes its
+ * craters, then the boulders of its 16 x 16 tile over the craters' height as tg_hip_boulder_launch
+ * draws them, read and written once. The same heights as the two launches one after the other. */
+int tg_hip_crater_boulder_launch(tg_dev* dev, const tg_crater_set* set,
+                                 const tg_boulder_set* boulders, float* heights_dev,
+                                 const float* xs_dev, const float* ys_dev,
+                                 const tg_kernel_params* params, void* stream, char* err,
+                                 size_t err_len);
 
 /* ---- Batched tiles ---------------------------------------------------------------------------
  * Many tiles of uneven size and pixel size evaluated in one persistent launch: the source fill
@@ -162,7 +172,6 @@ int tg_hip_crater_launch(tg_dev* dev, const tg_crater_set* set, float* heights_d
  * 16 x 16 tile lists (tgx_bin_plane over their relief boxes) and the grid's lows, as
  * tg_hip_run(TG_KERNEL_BOULDERS) binds them. tg_hip_boulder_launch bakes them into heights on the
  * device (each cell's height on entry is its base), on `stream`. */
-typedef struct tg_boulder_set tg_boulder_set;
 int tg_hip_boulder_set_create(tg_dev* dev, const tg_boulder_kernel* kernel,
                               const tg_boulder* boulders, size_t count, const uint32_t* tiles,
                               size_t tile_words, const uint32_t* entries, size_t entry_count,
@@ -281,6 +290,34 @@ int tg_hip_batch_tune(tg_batch* batch, uint32_t crater_walk, uint32_t groups_per
  * times radius times pi / 180 (RasterGrid::Spacing, src/raster/grid.cpp:28-33). The bins come
  * from tgx_sphere_bin. The crater walk is always the culled one. */
 int tg_hip_batch_surface(tg_batch* batch, double sphere_radius, char* err, size_t err_len);
+/* Boulders resident for batched tiles (no counterpart in the reference, which bakes boulders on
+ * the CPU one grid at a time): each boulder's record (tgx_prepare_boulders at the finest spacing
+ * of the tiles it will be drawn into; its centre split hi + lo, radius, exposed height and facets
+ * are read, its reach and weight are not), its FP64 diameter, and on a sphere its frame
+ * (tgx_sphere_frames of its centre; null on a plane). A tile draws a boulder as BoulderLayer::
+ * PrepareRelief (src/layers/boulders/boulder_relief.cpp:373-423) prepares it for that tile's
+ * grid: at the resolution weight of the tile's spacing (src/layers/fade.cpp:48-52), its dome
+ * averaged over a band one spacing wide and reaching half of (diameter + spacing). */
+typedef struct tg_boulder_catalogue tg_boulder_catalogue;
+int tg_hip_boulder_catalogue_create(tg_dev* dev, const tg_boulder* boulders, size_t count,
+                                    const double* diameters, const tg_crater_frame* frames,
+                                    tg_boulder_catalogue** out, char* err, size_t err_len);
+int tg_hip_boulder_catalogue_free(tg_boulder_catalogue* catalogue);
+/* The boulder layer of the staged batch: per bin of the staged tiles (16 x 16 cells, one per
+ * staged bin, in bin order, `tile` as the crater bins'), the catalogue's boulders
+ * entries[first .. first+count) that may reach it (tgx_index_bin over the boulders' relief boxes,
+ * for instance). The batch's next runs draw them after each cell's craters, as a BoulderLayer
+ * after a CraterLayer: every cell keeps its height after the craters as the base and takes the
+ * highest of that and base + weight x relief over its boulders (BoulderLayer::AddRelief without a
+ * fillet, boulder_relief.cpp:244-262), on a plane or, with frames, on the batch's sphere. Staging
+ * the tiles again drops the boulders. With `async` the copies are queued on the batch's stream
+ * after what it holds, as tg_hip_batch_stage_async queues them (the host arrays, pinned for the
+ * copies to be asynchronous, must stay unchanged until the next run completes; a list longer
+ * than any before waits for the stream, to replace its buffer); else they are waited for. Not
+ * for tg_hip_batch_run_static. */
+int tg_hip_batch_stage_boulders(tg_batch* batch, const tg_boulder_catalogue* catalogue,
+                                const tg_bin* bins, size_t bin_count, const uint32_t* entries,
+                                size_t entry_count, int async, char* err, size_t err_len);
 int tg_hip_batch_wait(tg_batch* batch, char* err, size_t err_len);
 /* The batch's stream, for events and copies ordered with its runs. */
 void* tg_hip_batch_stream(tg_batch* batch);

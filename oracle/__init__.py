@@ -76,6 +76,15 @@ def _ref_boulder_case_type():
 RefBoulderCase = _ref_boulder_case_type()
 
 
+class RefBoulderSphereCase(C.Structure):
+    """``ref_boulder_sphere_case`` of ref_boulders.cpp: a power-law boulder layer on a sphere
+    over a longitude-latitude raster, flat at 0 beneath."""
+    _fields_ = [("side", C.c_uint32), ("lon", C.c_double), ("lat", C.c_double),
+                ("step", C.c_double), ("radius", C.c_double), ("sfd_coefficient", C.c_double),
+                ("sfd_exponent", C.c_double), ("min_d", C.c_double), ("max_d", C.c_double),
+                ("ventifact_share", C.c_double), ("seed", C.c_uint32), ("threads", C.c_uint32)]
+
+
 class RefSphereCase(C.Structure):
     """``ref_sphere_case`` of ref_sphere.cpp: a crater layer on a sphere over a lon/lat raster."""
 
@@ -162,11 +171,14 @@ def _declare(lib: C.CDLL) -> None:
         "or_boulder_grid": (None, [vp, sz, vp, C.c_int, vp]),
         "or_boulder_grid_f64": (None, [vp, sz, vp, C.c_int, vp]),
         "or_boulder_relief_f64": (d, [vp, d, d, d]),
+        "or_boulder_body_f64": (None, [vp, sz, vp, d, C.c_int, vp]),
+        "or_sphere_offset": (None, [d, d, d, d, d, vp]),
         "ref_boulders_open": (vp, [C.POINTER(RefBoulderCase)]),
         "ref_boulders_close": (None, [vp]),
         "ref_boulders_evaluate": (C.c_int, [vp, vp]),
         "ref_boulders_list": (C.c_long, [vp, vp]),
         "ref_boulders_error": (C.c_char_p, []),
+        "ref_boulders_sphere": (C.c_long, [C.POINTER(RefBoulderSphereCase), vp, vp, C.c_long]),
         "ref_sphere_open": (vp, [C.POINTER(RefSphereCase)]),
         "ref_sphere_close": (None, [vp]),
         "ref_sphere_sizes": (C.c_long, [vp, C.POINTER(C.c_long)]),

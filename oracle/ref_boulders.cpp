@@ -8,7 +8,9 @@
 
 #include "terraingen/distribution/power_law.h"
 #include "terraingen/layers/boulders/boulder_layer.h"
+#include "terraingen/layers/layer.h"
 #include "terraingen/pipeline/terrain.h"
+#include "terraingen/raster/grid.h"
 #include "terraingen/sources/flat_source.h"
 #include "terraingen/sources/synthetic_source.h"
 
@@ -36,6 +38,17 @@ typedef struct ref_boulder_case {
     std::uint32_t seed;
     std::uint32_t threads;
 } ref_boulder_case;
+
+// A boulder layer on a sphere of `radius` meters over a longitude-latitude raster of `side` x
+// `side` cells of `step` degrees from its north-west corner (lon, lat), flat at 0 beneath.
+typedef struct ref_boulder_sphere_case {
+    std::uint32_t side;
+    double lon, lat, step, radius;
+    double sfd_coefficient, sfd_exponent, min_d, max_d;
+    double ventifact_share;
+    std::uint32_t seed;
+    std::uint32_t threads;
+} ref_boulder_sphere_case;
 
 // One boulder as the relief reads it (boulder_relief.cpp:244-275): centre, diameter, exposed
 // height, model flag and up to two facets (azimuth, angle, reach).
@@ -143,6 +156,68 @@ int ref_boulders_evaluate(void* handle, float* heights) {
     }
 }
 
+namespace {
+
+void Export(const BoulderList& list, ref_boulder* out) {
+    for (std::size_t n = 0; n < list.Size(); ++n) {
+        const Boulder b = list.At(n);
+        ref_boulder& r = out[n];
+        r.x = b.x;
+        r.y = b.y;
+        r.diameter = b.diameter;
+        r.exposed = b.ExposedHeight();
+        for (int k = 0; k < 2; ++k) {
+            r.facets[3 * k] = b.facets[k].azimuth;
+            r.facets[3 * k + 1] = b.facets[k].angle;
+            r.facets[3 * k + 2] = b.facets[k].reach;
+        }
+        r.facet_count = b.facetCount;
+        r.model = b.model ? 1u : 0u;
+    }
+}
+
+}  // namespace
+
+// The layer of a sphere case applied on the CPU to its raster of zeros: the FP32 heights into
+// `heights` (side x side, row by row) and, when `out` is not null, the boulders it draws there
+// (those of the raster's box grown by half a sample, at the raster's spacing, in the layer's
+// order; at most `capacity`). Returns the boulder count, or -1 on error.
+long ref_boulders_sphere(const ref_boulder_sphere_case* spec, float* heights, ref_boulder* out,
+                         long capacity) {
+    try {
+        const Surface surface = Surface::Sphere(spec->radius);
+        BoulderLayerOptions options;
+        options.seed = spec->seed;
+        options.sizes.distribution =
+            std::make_shared<PowerLawSfd>(spec->sfd_coefficient, spec->sfd_exponent);
+        options.sizes.minDiameter = spec->min_d;
+        options.sizes.maxDiameter = spec->max_d;
+        options.ventifacts.share = spec->ventifact_share;
+        const BoulderLayer layer(options, nullptr, TerrainBase{surface});
+        const RasterGrid grid{spec->side, spec->side,
+                              GeoTransform::NorthUp(spec->lon, spec->lat, spec->step)};
+        Raster<float> raster(grid, 0.0f);
+        LayerContext context;
+        context.threads = spec->threads;
+        layer.Apply(raster, context);
+        std::memcpy(heights, raster.Values().data(), raster.Size() * sizeof(float));
+        const double spacing = ResolvedSpacing(grid, 0.0, surface);
+        const BoulderList list = layer.Boulders(surface.Expand(grid.Bounds(), 0.5 * spacing),
+                                                spacing);
+        if (out != nullptr) {
+            if (long(list.Size()) > capacity) {
+                gBoulderError = "the boulder list is longer than the capacity given";
+                return -1;
+            }
+            Export(list, out);
+        }
+        return long(list.Size());
+    } catch (const std::exception& e) {
+        gBoulderError = e.what();
+        return -1;
+    }
+}
+
 // The boulders the layer draws into the case's grid: those of the grid's box grown by half a
 // sample, as BoulderLayer::ApplySamples gathers them for dome relief (boulder_relief.cpp:287-291,
 // 429-432), in the layer's order. Returns the count; fills `out` when it is not null.
@@ -153,21 +228,7 @@ long ref_boulders_list(void* handle, ref_boulder* out) {
         const BoulderList list =
             c.layer->Boulders(Surface{}.Expand(c.grid.Bounds(), 0.5 * spacing), spacing);
         if (out != nullptr) {
-            for (std::size_t n = 0; n < list.Size(); ++n) {
-                const Boulder b = list.At(n);
-                ref_boulder& r = out[n];
-                r.x = b.x;
-                r.y = b.y;
-                r.diameter = b.diameter;
-                r.exposed = b.ExposedHeight();
-                for (int k = 0; k < 2; ++k) {
-                    r.facets[3 * k] = b.facets[k].azimuth;
-                    r.facets[3 * k + 1] = b.facets[k].angle;
-                    r.facets[3 * k + 2] = b.facets[k].reach;
-                }
-                r.facet_count = b.facetCount;
-                r.model = b.model ? 1u : 0u;
-            }
+            Export(list, out);
         }
         return long(list.Size());
     } catch (const std::exception& e) {

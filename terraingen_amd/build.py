@@ -43,9 +43,24 @@ def _run(cmd: list[str]) -> None:
 
 
 def build_hip() -> Path:
+    """Each source compiled to an object on its own process (they share no device code), then
+    linked into the shared library."""
+    from concurrent.futures import ThreadPoolExecutor
+
     sources = [CSRC / "tg_hip.hip", CSRC / "batch.hip", CSRC / "boulders.hip", CSRC / "probe.hip"]
+    objdir = PKG / "_build"
+    objdir.mkdir(exist_ok=True)
+    compile_flags = [f for f in HIP_FLAGS if f != "-shared"]
+
+    def compile_one(src: Path) -> Path:
+        obj = objdir / (src.stem + ".o")
+        _run([HIPCC, *compile_flags, f"-I{INCLUDE}", f"-I{CSRC}", "-c", src, "-o", obj])
+        return obj
+
+    with ThreadPoolExecutor(max_workers=len(sources)) as pool:
+        objects = list(pool.map(compile_one, sources))
     tmp = HIP_SO.with_suffix(".so.tmp")
-    _run([HIPCC, *HIP_FLAGS, f"-I{INCLUDE}", f"-I{CSRC}", *sources, "-o", tmp])
+    _run([HIPCC, *HIP_FLAGS, *objects, "-o", tmp])
     os.replace(tmp, HIP_SO)
     return HIP_SO
 

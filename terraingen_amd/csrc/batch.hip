@@ -7,6 +7,7 @@
 // heights once. A wave keeps taking bins until the queue is empty, so tiles of any mix of sizes
 // fill every CU without a launch per tile, and nothing is read back or allocated between tiles.
 
+#include "boulder_device.h"
 #include "crater_device.h"
 #include "internal.h"
 #include "noise_device.h"
@@ -17,6 +18,7 @@
 
 #include <cmath>
 #include <cstring>
+#include <vector>
 
 using namespace tgi;
 
@@ -38,6 +40,15 @@ struct tg_catalogue {
     bool allFast = false;  // every crater takes the hoisted MPD path
     bool anyPrecise = false;  // some crater takes it in FP64 (tg::kFastPrecise)
     // Changes whenever what a launch reads of the catalogue changes (creation, frames).
+    uint64_t generation = tgi::NextGeneration();
+};
+
+struct tg_boulder_catalogue {
+    tg_dev* dev = nullptr;
+    tg_boulder* records = nullptr;    // geometry: split centre, radius, exposed height, facets
+    double* diameters = nullptr;      // FP64 diameters, for each tile's weight and reach
+    tg_crater_frame* frames = nullptr;  // per boulder on a sphere, else null
+    size_t count = 0;
     uint64_t generation = tgi::NextGeneration();
 };
 
@@ -87,6 +98,15 @@ struct tg_batch {
     uint32_t* binFirsts = nullptr;
     void* scanTemp = nullptr;
     size_t scanBytes = 0;
+    // Boulder lists of the staged bins (tg_hip_batch_stage_boulders), dropped by a restage.
+    tg_bin* boulderBins = nullptr;
+    uint32_t* boulderEntries = nullptr;
+    size_t boulderEntryCapacity = 0;
+    const tg_boulder_catalogue* boulders = nullptr;  // null: no boulders staged
+    uint64_t boulderGeneration = 0;  // of the lists' buffers and catalogue a graph reads
+    uint64_t graphBoulders = 0;
+    // Each staged tile's bins, for checking boulder lists against them.
+    std::vector<uint32_t> tileBinFirst, tileBinCount;
 };
 
 namespace {
@@ -314,6 +334,43 @@ __device__ __forceinline__ void VisitCrater(uint32_t n, bool inside, tg::Precise
     }
 }
 
+// The boulder layer over a cell after its craters (BoulderLayer::AddRelief without a fillet,
+// boulder_relief.cpp:244-262): over the bin's boulders, each drawn at the tile's `spacing` as
+// BoulderLayer::PrepareRelief (boulder_relief.cpp:373-423) prepares it for that grid: its
+// resolution weight at the spacing, the dome averaged over a band one spacing wide and reaching
+// half of (diameter + spacing). The reach, weight and band are narrowed from FP64 as the host's
+// preparation narrows them for the plain boulder kernel, whose cull test this repeats, so a tile
+// here is bit-equal to its craters then its boulders through the plain kernels. `offsetOf(n, b,
+// dx, dy)` gives the cell's offset east and north of boulder n. The maximum over the boulders does
+// not depend on their order.
+template <class OffsetOf>
+__device__ __forceinline__ float DrawBoulders(float base, const tg_bin& list,
+                                              const uint32_t* entries, const tg_boulder* boulders,
+                                              const double* diameters, double spacing,
+                                              const OffsetOf& offsetOf) {
+    float h = base;
+    const float width = float(spacing);
+    const uint32_t end = list.first + list.count;
+    for (uint32_t e = list.first; e < end; ++e) {
+        const uint32_t n = entries[e];
+        const double diameter = diameters[n];
+        const float weight = ResolutionAt(diameter, spacing);
+        if (weight == 0.0f) {
+            continue;  // unresolved at this spacing (uniform across the wave)
+        }
+        const float reach = float(0.5 * (diameter + spacing));
+        const double slack = double(reach) * (1.0 + 1e-4);
+        const float cull = float(slack * slack);
+        const tg_boulder& b = boulders[n];
+        float dx, dy;
+        offsetOf(n, b, dx, dy);
+        if (dx * dx + dy * dy < cull) {
+            h = tg::ApplyBoulder(b, width, weight, h, base, dx, dy);
+        }
+    }
+    return h;
+}
+
 // Takes bins off the queues until every range is empty, calling `bin(b)` for each bin b taken:
 // lane 0 takes `chunk` bins with one atomic add and the first is broadcast with readfirstlane, so
 // the calls are uniform across the wave and no wave waits for another.
@@ -388,7 +445,8 @@ __device__ __forceinline__ void CulledWalk(uint32_t first, uint32_t end, uint32_
 // not compiled in; the kernel then holds far fewer registers and keeps more waves resident.
 // kPrecise: some crater takes that path in FP64 (a catalogue without km-scale craters runs the
 // variant without the FP64 code, whose registers cost the other variants residency).
-template <uint32_t kWalk, bool kAllFast, bool kPrecise>
+// kBoulders: a boulder layer follows the craters (its code only in the variants that draw it).
+template <uint32_t kWalk, bool kAllFast, bool kPrecise, bool kBoulders = false>
 __global__ void __launch_bounds__(kGroupThreads, kAllFast ? (kPrecise ? 6 : 8) : 1)
     BatchTiles(const tg_tile_desc* __restrict__ tiles, const tg_bin* __restrict__ bins,
                const uint32_t* __restrict__ entries, uint32_t* __restrict__ counters,
@@ -397,7 +455,11 @@ __global__ void __launch_bounds__(kGroupThreads, kAllFast ? (kPrecise ? 6 : 8) :
                const tg_crater_record* __restrict__ records, const float* __restrict__ data,
                const float* __restrict__ harm, const float4* __restrict__ outline,
                const tg::HotCrater* __restrict__ hot, const float* __restrict__ hotHarm,
-               const tg::PreciseTerms* __restrict__ precise, uint32_t hoisted, uint32_t chunk,
+               const tg::PreciseTerms* __restrict__ precise,
+               const tg_bin* __restrict__ boulderBins,
+               const uint32_t* __restrict__ boulderEntries,
+               const tg_boulder* __restrict__ boulders,
+               const double* __restrict__ boulderDiameters, uint32_t hoisted, uint32_t chunk,
                uint32_t binSide, uint32_t cullDepth, float* __restrict__ out) {
     const uint32_t lane = threadIdx.x & 63u;
     const uint32_t squaresAcross = binSide / 8u;
@@ -495,8 +557,22 @@ __global__ void __launch_bounds__(kGroupThreads, kAllFast ? (kPrecise ? 6 : 8) :
                     CulledWalk(bin.first, end, cullDepth, lane, test, visit);
                 }
             }
+            float height = cell.Folded();
+            if constexpr (kBoulders) {
+                const tg_bin list = boulderBins[b];
+                if (list.count > 0u) {
+                    const float xLow = float(cx - double(x));
+                    const float yLow = float(cy - double(y));
+                    height = DrawBoulders(
+                        height, list, boulderEntries, boulders, boulderDiameters, tile.pixel,
+                        [&](uint32_t, const tg_boulder& r, float& dx, float& dy) {
+                            dx = (x - r.x) + (xLow - r.x_low);
+                            dy = (y - r.y) + (yLow - r.y_low);
+                        });
+                }
+            }
             if (inside) {
-                out[tile.out_offset + size_t(row) * tile.width + col] = cell.Folded();
+                out[tile.out_offset + size_t(row) * tile.width + col] = height;
             }
         }
     });
@@ -564,8 +640,8 @@ __device__ __forceinline__ void VisitBodyCrater(uint32_t n, bool inside, tg::Pre
 // host from the radius and the plan's highest frequency).
 // kGeneral: the plan holds warped fractals or bases other than Perlin (one variant, with the
 // general crater walk and 64-bit lattice indices, carries their code).
-// kPrecise: as for BatchTiles.
-template <bool kAllFast, bool kSmall, bool kGeneral, bool kPrecise>
+// kPrecise, kBoulders: as for BatchTiles.
+template <bool kAllFast, bool kSmall, bool kGeneral, bool kPrecise, bool kBoulders = false>
 __global__ void __launch_bounds__(kGroupThreads, kAllFast ? (kPrecise ? 6 : 8) : 1)
     BatchBodyTiles(const tg_tile_desc* __restrict__ tiles, const tg_bin* __restrict__ bins,
                    const uint32_t* __restrict__ entries, uint32_t* __restrict__ counters,
@@ -575,8 +651,14 @@ __global__ void __launch_bounds__(kGroupThreads, kAllFast ? (kPrecise ? 6 : 8) :
                    const float* __restrict__ harm, const float4* __restrict__ outline,
                    const tg::HotCrater* __restrict__ hot, const float* __restrict__ hotHarm,
                    const tg_crater_frame* __restrict__ frames,
-                   const tg::PreciseTerms* __restrict__ precise, uint32_t hoisted, uint32_t chunk,
-                   uint32_t binSide, uint32_t cullDepth, float* __restrict__ out) {
+                   const tg::PreciseTerms* __restrict__ precise,
+                   const tg_bin* __restrict__ boulderBins,
+                   const uint32_t* __restrict__ boulderEntries,
+                   const tg_boulder* __restrict__ boulders,
+                   const double* __restrict__ boulderDiameters,
+                   const tg_crater_frame* __restrict__ boulderFrames, uint32_t hoisted,
+                   uint32_t chunk, uint32_t binSide, uint32_t cullDepth,
+                   float* __restrict__ out) {
     const uint32_t lane = threadIdx.x & 63u;
     const tg::CubeSteps steps = tg::MakeCubeSteps(lane);
     const uint32_t squaresAcross = binSide / 8u;
@@ -620,7 +702,8 @@ __global__ void __launch_bounds__(kGroupThreads, kAllFast ? (kPrecise ? 6 : 8) :
                                     ? elevation + tg::EvaluateStack3<true, kSmall, kGeneral>(
                                                       *plan, at.x, at.y, at.z, lane, steps)
                                     : elevation};
-            if (kernel != nullptr && bin.count > 0u) {
+            // MakeSplitPoint of the cell's position (crater_kernel.cpp:303-310).
+            const auto split = [&] {
                 tg_split_point p;
                 p.x = float(at.x);
                 p.x_low = float(at.x - double(p.x));
@@ -628,6 +711,10 @@ __global__ void __launch_bounds__(kGroupThreads, kAllFast ? (kPrecise ? 6 : 8) :
                 p.y_low = float(at.y - double(p.y));
                 p.z = float(at.z);
                 p.z_low = float(at.z - double(p.z));
+                return p;
+            };
+            if (kernel != nullptr && bin.count > 0u) {
+                const tg_split_point p = split();
                 const uint32_t end = bin.first + bin.count;
                 const auto visit = [&](uint32_t n) {
                     VisitBodyCrater<kAllFast, kPrecise>(n, inside, cell, p, spacing, hoisted != 0u, hot,
@@ -666,8 +753,25 @@ __global__ void __launch_bounds__(kGroupThreads, kAllFast ? (kPrecise ? 6 : 8) :
                     CulledWalk(bin.first, end, cullDepth, lane, test, visit);
                 }
             }
+            float height = cell.Folded();
+            if constexpr (kBoulders) {
+                const tg_bin list = boulderBins[b];
+                if (list.count > 0u) {
+                    // The offset in the boulder's frame: its east and north at its centre, the
+                    // tangent offset rescaled to the arc (Surface::SphereOffset,
+                    // surface.cpp:214-229, to FP32 rounding at a boulder's few meters).
+                    const tg_split_point p = split();
+                    height = DrawBoulders(
+                        height, list, boulderEntries, boulders, boulderDiameters, spacing,
+                        [&](uint32_t n, const tg_boulder&, float& dx, float& dy) {
+                            const tg::Offset o = tg::BodySplitOffset(boulderFrames[n], p);
+                            dx = o.dx;
+                            dy = o.dy;
+                        });
+                }
+            }
             if (inside) {
-                out[tile.out_offset + size_t(row) * tile.width + col] = cell.Folded();
+                out[tile.out_offset + size_t(row) * tile.width + col] = height;
             }
         }
     });
@@ -772,14 +876,24 @@ PlaneKernelFn PlaneKernelOf(bool allFast, bool precise) {
                    : (precise ? BatchTiles<kWalk, false, true> : BatchTiles<kWalk, false, false>);
 }
 
-PlaneKernelFn PlaneKernel(uint32_t walk, bool allFast, bool precise) {
+// With boulders: the culled walk and the full crater evaluation (which takes the hoisted path
+// crater by crater), so that only two variants carry the boulder code.
+PlaneKernelFn PlaneKernel(uint32_t walk, bool allFast, bool precise, bool boulders) {
+    if (boulders) {
+        return precise ? BatchTiles<kWalkCulled, false, true, true>
+                       : BatchTiles<kWalkCulled, false, false, true>;
+    }
     return walk == TG_WALK_PER_LANE ? PlaneKernelOf<kWalkPerLane>(allFast, precise)
            : walk == TG_WALK_CULLED ? PlaneKernelOf<kWalkCulled>(allFast, precise)
                                     : PlaneKernelOf<kWalkTogether>(allFast, precise);
 }
 
 template <bool kPrecise>
-BodyKernelFn BodyKernelOf(bool allFast, bool small, bool general) {
+BodyKernelFn BodyKernelOf(bool allFast, bool small, bool general, bool boulders) {
+    if (boulders) {  // the full crater evaluation and 64-bit lattice indices, as kGeneral's
+        return general ? BatchBodyTiles<false, false, true, kPrecise, true>
+                       : BatchBodyTiles<false, false, false, kPrecise, true>;
+    }
     if (general) {  // other bases or warped fractals: one variant carries their code
         return BatchBodyTiles<false, false, true, kPrecise>;
     }
@@ -789,13 +903,18 @@ BodyKernelFn BodyKernelOf(bool allFast, bool small, bool general) {
                             : BatchBodyTiles<false, false, false, kPrecise>);
 }
 
-BodyKernelFn BodyKernel(bool allFast, bool small, bool general, bool precise) {
-    return precise ? BodyKernelOf<true>(allFast, small, general)
-                   : BodyKernelOf<false>(allFast, small, general);
+BodyKernelFn BodyKernel(bool allFast, bool small, bool general, bool precise, bool boulders) {
+    return precise ? BodyKernelOf<true>(allFast, small, general, boulders)
+                   : BodyKernelOf<false>(allFast, small, general, boulders);
 }
 
 void Launch(tg_batch& b, const tg_noise_plan* plan, float elevation, const tg_catalogue* cat,
             float* out) {
+    const tg_boulder_catalogue* rocks = b.boulders;
+    const tg_bin* boulderBins = rocks != nullptr ? b.boulderBins : nullptr;
+    const uint32_t* boulderEntries = rocks != nullptr ? b.boulderEntries : nullptr;
+    const tg_boulder* boulderRecords = rocks != nullptr ? rocks->records : nullptr;
+    const double* boulderDiameters = rocks != nullptr ? rocks->diameters : nullptr;
     const uint32_t groups = GroupsFor(b);
     const uint32_t chunk = ChunkFor(b, groups);
     hipLaunchKernelGGL(ResetQueue, dim3(1), dim3(64), 0, b.stream, b.counters);
@@ -809,7 +928,7 @@ void Launch(tg_batch& b, const tg_noise_plan* plan, float elevation, const tg_ca
             plan == nullptr || tg::BodyLatticeBound(plan->host, b.radius) < 2147483648.0;
         const bool precise = cat != nullptr && cat->anyPrecise;
         auto* body = BodyKernel(allFast, small, plan != nullptr && !plan->host.bodyPerlin,
-                                precise);
+                                precise, rocks != nullptr);
         hipLaunchKernelGGL(body, dim3(groups), dim3(kGroupThreads), 0, b.stream, b.tiles, b.bins,
                            b.entries, b.counters, plan != nullptr ? plan->device : nullptr,
                            elevation, b.radius, unitLength,
@@ -821,12 +940,14 @@ void Launch(tg_batch& b, const tg_noise_plan* plan, float elevation, const tg_ca
                            cat != nullptr ? cat->hot : nullptr,
                            cat != nullptr ? cat->hotHarm : nullptr,
                            cat != nullptr ? cat->frames : nullptr,
-                           cat != nullptr ? cat->precise : nullptr, b.hoisted ? 1u : 0u, chunk,
+                           cat != nullptr ? cat->precise : nullptr, boulderBins, boulderEntries,
+                           boulderRecords, boulderDiameters,
+                           rocks != nullptr ? rocks->frames : nullptr, b.hoisted ? 1u : 0u, chunk,
                            b.binSide, b.cullDepth, out);
         Check(hipGetLastError(), "launching the batched body tile kernel");
         return;
     }
-    auto* fn = PlaneKernel(b.walk, allFast, cat != nullptr && cat->anyPrecise);
+    auto* fn = PlaneKernel(b.walk, allFast, cat != nullptr && cat->anyPrecise, rocks != nullptr);
     hipLaunchKernelGGL(fn, dim3(groups), dim3(kGroupThreads), 0, b.stream, b.tiles, b.bins,
                        b.entries, b.counters, plan != nullptr ? plan->device : nullptr,
                        elevation, cat != nullptr ? cat->kernel : nullptr,
@@ -835,8 +956,9 @@ void Launch(tg_batch& b, const tg_noise_plan* plan, float elevation, const tg_ca
                        cat != nullptr ? cat->outline : nullptr,
                        cat != nullptr ? cat->hot : nullptr,
                        cat != nullptr ? cat->hotHarm : nullptr,
-                       cat != nullptr ? cat->precise : nullptr, b.hoisted ? 1u : 0u, chunk,
-                       b.binSide, b.cullDepth, out);
+                       cat != nullptr ? cat->precise : nullptr, boulderBins, boulderEntries,
+                       boulderRecords, boulderDiameters, b.hoisted ? 1u : 0u, chunk, b.binSide,
+                       b.cullDepth, out);
     Check(hipGetLastError(), "launching the batched tile kernel");
 }
 
@@ -971,7 +1093,8 @@ int tg_hip_batch_free(tg_batch* b) {
                     static_cast<void*>(b->entries), static_cast<void*>(b->counters),
                     static_cast<void*>(b->windows), static_cast<void*>(b->tileWindows),
                     static_cast<void*>(b->binCounts), static_cast<void*>(b->binFirsts),
-                    b->scanTemp}) {
+                    b->scanTemp, static_cast<void*>(b->boulderBins),
+                    static_cast<void*>(b->boulderEntries)}) {
         if (p != nullptr) {
             (void)hipFree(p);
         }
@@ -1044,6 +1167,18 @@ Checked CheckBatch(const tg_tile_desc* tiles, size_t tileCount, const tg_bin* bi
     return {outEnd, maxEntry, entryCount > 0 && any};
 }
 
+// Keeps each staged tile's bins on the host, for checking boulder lists against them, and drops
+// the boulder lists of the previous staging.
+void KeepTileBins(tg_batch& b, const tg_tile_desc* tiles, size_t tileCount) {
+    b.tileBinFirst.resize(tileCount);
+    b.tileBinCount.resize(tileCount);
+    for (size_t t = 0; t < tileCount; ++t) {
+        b.tileBinFirst[t] = tiles[t].bin_first;
+        b.tileBinCount[t] = tiles[t].bins_across * ((tiles[t].height + b.binSide - 1) / b.binSide);
+    }
+    b.boulders = nullptr;
+}
+
 // Validates a staging and copies it to the batch: synchronously from any host memory, or, with
 // `async`, queued on the batch's stream from memory the caller keeps unchanged until the batch's
 // next run has completed.
@@ -1108,6 +1243,7 @@ int Stage(tg_batch* b, const tg_tile_desc* tiles, size_t tileCount, const tg_bin
         b->outEnd = checked.outEnd;
         b->binSide = kBinSide;
         b->entryCount = entryCount;
+        KeepTileBins(*b, tiles, tileCount);
     });
 }
 
@@ -1123,6 +1259,17 @@ void CheckRun(const tg_batch& b, const tg_catalogue* cat, size_t outCapacity) {
         throw HipError{"tiles on a sphere need the catalogue's crater frames "
                        "(tg_hip_catalogue_frames)"};
     }
+    if (b.boulders != nullptr && (b.radius > 0.0) != (b.boulders->frames != nullptr)) {
+        throw HipError{b.radius > 0.0
+                           ? "boulders on a sphere need their frames"
+                           : "boulders with frames are for tiles on a sphere"};
+    }
+}
+
+// What a captured graph read of a batch's boulders: a generation taken at each boulder staging,
+// which is when their lists' buffers and catalogue can change.
+uint64_t BouldersKey(const tg_batch& b) {
+    return b.boulders != nullptr ? b.boulderGeneration : 0;
 }
 
 }  // namespace
@@ -1255,6 +1402,7 @@ int tg_hip_batch_stage_windows(tg_batch* b, const tg_tile_desc* tiles, size_t ti
         b->outEnd = outEnd;
         b->binSide = binSide;
         b->entryCount = total;
+        KeepTileBins(*b, tiles, tileCount);
         if (entryCount != nullptr) {
             *entryCount = total;
         }
@@ -1328,7 +1476,7 @@ int tg_hip_batch_run(tg_batch* b, const tg_noise_plan* plan, float elevation,
             b->graphOut != out || b->graphElevation != elevation || b->graphGroups != groups ||
             b->graphWalk != b->walk || b->graphHoisted != b->hoisted ||
             b->graphRadius != b->radius || b->graphBinSide != b->binSide ||
-            b->graphCullDepth != b->cullDepth ||
+            b->graphCullDepth != b->cullDepth || b->graphBoulders != BouldersKey(*b) ||
             b->graphChunk != ChunkFor(*b, groups)) {
             DropGraph(*b);
             hipGraph_t graph = nullptr;
@@ -1358,6 +1506,7 @@ int tg_hip_batch_run(tg_batch* b, const tg_noise_plan* plan, float elevation,
             b->graphRadius = b->radius;
             b->graphBinSide = b->binSide;
             b->graphCullDepth = b->cullDepth;
+            b->graphBoulders = BouldersKey(*b);
         }
         Check(hipGraphLaunch(b->graph, b->stream), "hipGraphLaunch");
     });
@@ -1497,6 +1646,136 @@ int tg_hip_catalogue_frames(tg_catalogue* cat, const tg_crater_frame* frames, si
         cat->frames = uploaded;
         cat->precise = precise;
         cat->generation = NextGeneration();  // graphs captured without the frames are stale
+    });
+}
+
+int tg_hip_boulder_catalogue_create(tg_dev* dev, const tg_boulder* boulders, size_t count,
+                                    const double* diameters, const tg_crater_frame* frames,
+                                    tg_boulder_catalogue** out, char* err, size_t errLen) {
+    if (dev == nullptr || out == nullptr ||
+        (count > 0 && (boulders == nullptr || diameters == nullptr))) {
+        SetErr(err, errLen, "tg_hip_boulder_catalogue_create: null argument");
+        return TG_ERR_ARGUMENT;
+    }
+    *out = nullptr;
+    auto cat = std::make_unique<tg_boulder_catalogue>();
+    const int code = Guard(err, errLen, [&] {
+        for (size_t i = 0; i < count; ++i) {
+            const tg_boulder& b = boulders[i];
+            // The record's radius is half its diameter, narrowed: the dome and the tile's reach
+            // must describe one rock.
+            if (!(diameters[i] > 0.0) || !std::isfinite(diameters[i]) ||
+                b.radius != float(0.5 * diameters[i]) || !std::isfinite(b.exposed) ||
+                !std::isfinite(b.x) || !std::isfinite(b.y) || b.facet_count > 2) {
+                throw HipError{"boulder " + std::to_string(i) + " is malformed"};
+            }
+            if (frames != nullptr &&
+                (!(frames[i].sphere_radius > 0.0f) || !std::isfinite(frames[i].x) ||
+                 !std::isfinite(frames[i].y) || !std::isfinite(frames[i].z))) {
+                throw HipError{"boulder frame " + std::to_string(i) + " is not a frame on a sphere"};
+            }
+        }
+        Check(hipSetDevice(dev->ordinal), "hipSetDevice");
+        cat->dev = dev;
+        cat->count = count;
+        cat->records = UploadArray(boulders, count);
+        cat->diameters = UploadArray(diameters, count);
+        if (frames != nullptr) {
+            cat->frames = UploadArray(frames, count);
+        }
+    });
+    if (code != TG_OK) {
+        tg_hip_boulder_catalogue_free(cat.release());
+        return code;
+    }
+    *out = cat.release();
+    return TG_OK;
+}
+
+int tg_hip_boulder_catalogue_free(tg_boulder_catalogue* cat) {
+    if (cat == nullptr) {
+        return TG_OK;
+    }
+    if (cat->dev != nullptr) {
+        (void)hipSetDevice(cat->dev->ordinal);
+    }
+    for (void* p : {static_cast<void*>(cat->records), static_cast<void*>(cat->diameters),
+                    static_cast<void*>(cat->frames)}) {
+        if (p != nullptr) {
+            (void)hipFree(p);
+        }
+    }
+    delete cat;
+    return TG_OK;
+}
+
+int tg_hip_batch_stage_boulders(tg_batch* b, const tg_boulder_catalogue* cat, const tg_bin* bins,
+                                size_t binCount, const uint32_t* entries, size_t entryCount,
+                                int async, char* err, size_t errLen) {
+    if (b == nullptr || cat == nullptr || (binCount > 0 && bins == nullptr) ||
+        (entryCount > 0 && entries == nullptr)) {
+        SetErr(err, errLen, "tg_hip_batch_stage_boulders: null argument");
+        return TG_ERR_ARGUMENT;
+    }
+    return Guard(err, errLen, [&] {
+        if (cat->dev != b->dev) {
+            throw HipError{"the boulders are on another device than the batch"};
+        }
+        if (binCount != b->binCount || b->binSide != kBinSide) {
+            throw HipError{"boulder lists need one bin per staged bin, of 16 x 16 cells"};
+        }
+        if (entryCount > 0xFFFFFFFFull) {
+            throw HipError{"boulder entries must fit 32-bit indices"};
+        }
+        // Every bin where the staged tiles put it, its list within the entries; every entry a
+        // boulder of the catalogue.
+        for (size_t t = 0; t < b->tileBinFirst.size(); ++t) {
+            for (uint64_t i = b->tileBinFirst[t];
+                 i < uint64_t(b->tileBinFirst[t]) + b->tileBinCount[t]; ++i) {
+                if (bins[i].tile != t || uint64_t(bins[i].first) + bins[i].count > entryCount) {
+                    throw HipError{"boulder bin " + std::to_string(i) + " is malformed"};
+                }
+            }
+        }
+        for (size_t e = 0; e < entryCount; ++e) {
+            if (entries[e] >= cat->count) {
+                throw HipError{"boulder entry " + std::to_string(e) +
+                               " names a boulder the catalogue does not hold"};
+            }
+        }
+        Check(hipSetDevice(b->dev->ordinal), "hipSetDevice");
+        // Copies waited for are not ordered with the batch's stream, so its work must be done;
+        // queued ones follow it on the stream.
+        if (!async) {
+            Check(hipStreamSynchronize(b->stream), "waiting for the batch's previous run");
+        }
+        if (b->boulderBins == nullptr) {
+            b->boulderBins = Alloc<tg_bin>(std::max<size_t>(b->maxBins, 1));
+        }
+        if (entryCount > b->boulderEntryCapacity) {
+            // The entries' buffer is replaced, so nothing queued may still read it.
+            Check(hipStreamSynchronize(b->stream), "waiting for the batch's previous run");
+            if (b->boulderEntries != nullptr) {
+                Check(hipFree(b->boulderEntries), "hipFree");
+                b->boulderEntries = nullptr;
+                b->boulderEntryCapacity = 0;
+            }
+            const size_t capacity = entryCount + entryCount / 4 + 1;
+            b->boulderEntries = Alloc<uint32_t>(capacity);
+            b->boulderEntryCapacity = capacity;
+        }
+        const auto put = [&](void* dst, const void* src, size_t bytes, const char* what) {
+            if (bytes == 0) {
+                return;
+            }
+            Check(async ? hipMemcpyAsync(dst, src, bytes, hipMemcpyHostToDevice, b->stream)
+                        : hipMemcpy(dst, src, bytes, hipMemcpyHostToDevice),
+                  what);
+        };
+        put(b->boulderBins, bins, binCount * sizeof(tg_bin), "staging boulder bins");
+        put(b->boulderEntries, entries, entryCount * sizeof(uint32_t), "staging boulder entries");
+        b->boulders = cat;
+        b->boulderGeneration = NextGeneration();
     });
 }
 

@@ -42,11 +42,17 @@ __device__ __forceinline__ float BoulderRelief(const tg_boulder& b, float width,
 }
 
 // BoulderLayer::AddRelief without a fillet (boulder_relief.cpp:244-262): the cell's height `h` and
-// its height `base` before the layer's boulders.
+// its height `base` before the layer's boulders, the relief faded by `weight`.
+__device__ __forceinline__ float ApplyBoulder(const tg_boulder& b, float width, float weight,
+                                              float h, float base, float dx, float dy) {
+    const float relief = BoulderRelief(b, width, dx, dy);
+    return relief > 0.0f ? fmaxf(h, base + weight * relief) : h;
+}
+
+// The same at the weight the record was prepared with.
 __device__ __forceinline__ float ApplyBoulder(const tg_boulder& b, float width, float h,
                                               float base, float dx, float dy) {
-    const float relief = BoulderRelief(b, width, dx, dy);
-    return relief > 0.0f ? fmaxf(h, base + b.weight * relief) : h;
+    return ApplyBoulder(b, width, b.weight, h, base, dx, dy);
 }
 
 }  // namespace tg

@@ -16,21 +16,6 @@
 
 using namespace tgi;
 
-struct tg_boulder_set {
-    tg_dev* dev = nullptr;
-    tg_boulder_kernel kernel{};
-    tg_boulder* boulders = nullptr;
-    uint32_t* tiles = nullptr;
-    uint32_t* entries = nullptr;
-    float* lows = nullptr;
-    float* cull = nullptr;  // per boulder: squared distance beyond which its relief is 0
-    size_t count = 0;
-    size_t lowCount = 0;
-    uint32_t tilesAcross = 0;
-    uint32_t tileRows = 0;
-    uint32_t tileSide = 16;
-};
-
 namespace {
 
 constexpr uint32_t kTile = 16;

@@ -61,6 +61,7 @@ const Loaded& Load() {
         bind(a.batch_stage, "tg_hip_batch_stage");
         bind(a.batch_stage_async, "tg_hip_batch_stage_async");
         bind(a.batch_surface, "tg_hip_batch_surface");
+        bind(a.batch_stage_boulders, "tg_hip_batch_stage_boulders");
         bind(a.batch_run, "tg_hip_batch_run");
         bind(a.batch_stream, "tg_hip_batch_stream");
         bind(a.malloc, "tg_hip_malloc");

@@ -23,6 +23,7 @@ struct HipApi {
     decltype(&tg_hip_batch_run) batch_run;
     decltype(&tg_hip_batch_stream) batch_stream;
     decltype(&tg_hip_batch_surface) batch_surface;
+    decltype(&tg_hip_batch_stage_boulders) batch_stage_boulders;
     decltype(&tg_hip_malloc) malloc;
     decltype(&tg_hip_free) free;
     decltype(&tg_hip_host_alloc) host_alloc;

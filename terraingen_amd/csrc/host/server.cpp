@@ -123,6 +123,10 @@ struct Slot {
     tg_bin* hostBins = nullptr;
     std::uint32_t* hostEntries = nullptr;
     std::size_t maxTiles = 0, maxBins = 0, maxEntries = 0;
+    // The same for the batch's boulder lists, when the server draws boulders.
+    tg_bin* hostBoulderBins = nullptr;
+    std::uint32_t* hostBoulderEntries = nullptr;
+    std::size_t maxBoulderBins = 0, maxBoulderEntries = 0;
     bool busy = false;
     bool copying = false;  // finished on the device, its heights with a copier
     std::vector<Request*> requests;  // in batch order
@@ -148,6 +152,9 @@ struct tgx_server {
     const tg_catalogue* catalogue = nullptr;
     const tgx_index* index = nullptr;
     tgh::IndexPtr emptyIndex;
+    // The boulder layer (tgx_server_boulders): its catalogue and the index of its relief boxes.
+    const tg_boulder_catalogue* boulders = nullptr;
+    const tgx_index* boulderIndex = nullptr;
     tgx_server_config config{};
     float* pool = nullptr;
     std::unique_ptr<tgh::RangePool> ranges;
@@ -188,7 +195,9 @@ void FreeSlot(tgx_server& s, Slot& slot) {
         slot.event = nullptr;
     }
     for (void* p : {static_cast<void*>(slot.staging), static_cast<void*>(slot.hostDescs),
-                    static_cast<void*>(slot.hostBins), static_cast<void*>(slot.hostEntries)}) {
+                    static_cast<void*>(slot.hostBins), static_cast<void*>(slot.hostEntries),
+                    static_cast<void*>(slot.hostBoulderBins),
+                    static_cast<void*>(slot.hostBoulderEntries)}) {
         if (p != nullptr) {
             h.host_free(s.dev, p);
         }
@@ -197,6 +206,8 @@ void FreeSlot(tgx_server& s, Slot& slot) {
     slot.hostDescs = nullptr;
     slot.hostBins = nullptr;
     slot.hostEntries = nullptr;
+    slot.hostBoulderBins = nullptr;
+    slot.hostBoulderEntries = nullptr;
 }
 
 template <class T> T* PinnedArray(tgx_server& s, std::size_t count) {
@@ -267,6 +278,35 @@ void EnsureCapacity(tgx_server& s, Slot& slot, std::size_t tiles, std::size_t bi
     slot.maxEntries = e;
 }
 
+// Gives the slot (idle) pinned boulder lists of at least the given capacities, as EnsureCapacity
+// does for its crater lists: what it outgrows is kept until the server is freed.
+void EnsureBoulderCapacity(tgx_server& s, Slot& slot, std::size_t bins, std::size_t entries) {
+    if (slot.hostBoulderBins != nullptr && bins <= slot.maxBoulderBins &&
+        entries <= slot.maxBoulderEntries) {
+        return;
+    }
+    const HipApi& h = *s.hip;
+    const std::size_t b = std::max(bins * 2, slot.maxBoulderBins);
+    const std::size_t e = std::max(entries * 2 + 1024, slot.maxBoulderEntries);
+    tg_bin* binBuf = PinnedArray<tg_bin>(s, b);
+    std::uint32_t* entryBuf = nullptr;
+    try {
+        entryBuf = PinnedArray<std::uint32_t>(s, e);
+    } catch (...) {
+        h.host_free(s.dev, binBuf);
+        throw;
+    }
+    if (slot.hostBoulderBins != nullptr) {
+        s.retiredPinned.insert(s.retiredPinned.end(),
+                               {static_cast<void*>(slot.hostBoulderBins),
+                                static_cast<void*>(slot.hostBoulderEntries)});
+    }
+    slot.hostBoulderBins = binBuf;
+    slot.hostBoulderEntries = entryBuf;
+    slot.maxBoulderBins = b;
+    slot.maxBoulderEntries = e;
+}
+
 // Bins, stages and launches one batch in an idle slot, then queues the copy back and the event.
 // Runs without the server's lock (the slot is the dispatcher's while it is being launched) and
 // returns the launch time, which the dispatcher records under the lock.
@@ -288,6 +328,22 @@ double LaunchBatch(tgx_server& s, Slot& slot) {
     Raise(h.batch_stage_async(slot.batch, slot.hostDescs, descs.size(), slot.hostBins,
                               bins.size(), slot.hostEntries, entries.size(), err, sizeof err),
           err);
+    if (s.boulders != nullptr) {
+        // The boulder layer's lists of the same tiles, binned against its own index.
+        std::vector<tg_tile_desc> boulderDescs;
+        std::vector<tg_bin> boulderBins;
+        std::vector<std::uint32_t> boulderEntries;
+        BinBatch(*s.boulderIndex, tiles.data(), tiles.size(), s.config.bin_threads, boulderDescs,
+                 boulderBins, boulderEntries);
+        EnsureBoulderCapacity(s, slot, boulderBins.size(), boulderEntries.size());
+        std::memcpy(slot.hostBoulderBins, boulderBins.data(), boulderBins.size() * sizeof(tg_bin));
+        std::memcpy(slot.hostBoulderEntries, boulderEntries.data(),
+                    boulderEntries.size() * sizeof(std::uint32_t));
+        Raise(h.batch_stage_boulders(slot.batch, s.boulders, slot.hostBoulderBins,
+                                     boulderBins.size(), slot.hostBoulderEntries,
+                                     boulderEntries.size(), 1, err, sizeof err),
+              err);
+    }
     std::uint64_t cells = 0;
     slot.offsets.resize(descs.size());
     for (std::size_t i = 0; i < descs.size(); ++i) {
@@ -561,6 +617,26 @@ int tgx_server_create(tg_dev* dev, const tg_noise_plan* plan, float elevation,
     }
     *out = s.release();
     return TG_OK;
+}
+
+int tgx_server_boulders(tgx_server* s, const tg_boulder_catalogue* catalogue,
+                        const tgx_index* index, char* err, size_t errLen) {
+    return tgh::Guard(err, errLen, [&] {
+        if (s == nullptr || catalogue == nullptr || index == nullptr) {
+            throw std::invalid_argument("tgx_server_boulders: null argument");
+        }
+        if (tgh::IndexRadius(*index) != tgh::IndexRadius(*s->index)) {
+            throw std::invalid_argument(
+                "tgx_server_boulders: the boulders' index is on another surface than the craters'");
+        }
+        std::lock_guard<std::mutex> lock(s->mutex);
+        if (s->nextTicket != 1 || s->boulders != nullptr) {
+            throw std::invalid_argument(
+                "tgx_server_boulders: set once, before the first request is submitted");
+        }
+        s->boulders = catalogue;
+        s->boulderIndex = index;
+    });
 }
 
 int tgx_server_submit(tgx_server* s, const tgx_tile* tile, float* dst, uint64_t* ticket,

@@ -33,6 +33,23 @@ struct tg_dev {
     std::string scratchError;
 };
 
+// A boulder set (boulders.hip): the boulders of one grid spacing on the device, binned into the
+// grid's 16 x 16-cell tiles.
+struct tg_boulder_set {
+    tg_dev* dev = nullptr;
+    tg_boulder_kernel kernel{};
+    tg_boulder* boulders = nullptr;
+    uint32_t* tiles = nullptr;
+    uint32_t* entries = nullptr;
+    float* lows = nullptr;
+    float* cull = nullptr;  // per boulder: squared distance beyond which its relief is 0
+    size_t count = 0;
+    size_t lowCount = 0;
+    uint32_t tilesAcross = 0;
+    uint32_t tileRows = 0;
+    uint32_t tileSide = 16;
+};
+
 namespace tgi {
 
 // The static scratch of tg_hip_batch_run_static, one per device: the queue counters, then the

@@ -4,6 +4,7 @@
 // Built with -ffp-contract=off, as the reference is (its CMakeLists.txt:32-38), so FP32 results
 // depend only on the order of operations written here.
 
+#include "boulder_device.h"
 #include "tg_hip.h"
 
 #include "crater_device.h"
@@ -120,8 +121,20 @@ void LaunchNoise(const tg::NoisePlan* plan, float* heights, const float* xs, con
 // craters.slang's main(): each cell runs the crater list of its tile.
 constexpr uint32_t kCraterTile = 16;
 
+// The boulder layer a crater pass may carry (a plane boulder set's arrays, null when none): drawn
+// over each cell after its craters, as the boulder kernel draws it over the craters' heights, so
+// the grid is read and written once for both layers.
+struct BoulderPass {
+    tg_boulder_kernel kernel;
+    const tg_boulder* __restrict__ boulders;
+    const uint32_t* __restrict__ tiles;
+    const uint32_t* __restrict__ entries;
+    const float* __restrict__ cull;
+};
+
 // kAllFast: every crater of the set takes the hoisted MPD path (see BatchTiles).
-template <bool kAllFast>
+// kBoulders: a boulder layer follows the craters (BoulderPass).
+template <bool kAllFast, bool kBoulders = false>
 __global__ void __launch_bounds__(kCraterTile * kCraterTile)
     CraterStrip(float* __restrict__ heights, const float* __restrict__ xs,
                 const float* __restrict__ ys, const tg_crater_shape* __restrict__ kernel,
@@ -131,7 +144,8 @@ __global__ void __launch_bounds__(kCraterTile * kCraterTile)
                 const tg_split_point* __restrict__ points, const float* __restrict__ lows,
                 const float* __restrict__ cull, const float* __restrict__ harm,
                 const float4* __restrict__ outline, const tg::MpdTerms* __restrict__ terms,
-                const tg::PreciseTerms* __restrict__ precise, tg_kernel_params params) {
+                const tg::PreciseTerms* __restrict__ precise, tg_kernel_params params,
+                BoulderPass rocks) {
     // Each 64-lane wave covers an 8 x 8 square of the 16 x 16 tile, so a disc-shaped reach
     // leaves fewer waves partly active than rows of the tile would.
     const uint32_t t = threadIdx.x;
@@ -214,7 +228,28 @@ __global__ void __launch_bounds__(kCraterTile * kCraterTile)
             }
         }
     }
-    heights[i] = cell.Folded();
+    float out = cell.Folded();
+    if constexpr (kBoulders) {
+        // BoulderStrip's loop over the cell's boulder tile, the craters' height as the base.
+        const uint32_t bside = rocks.tiles[1];
+        const uint32_t bhead = 3u + 2u * ((row / bside) * rocks.tiles[0] + col / bside);
+        const uint32_t bfirst = rocks.tiles[bhead];
+        const uint32_t bend = bfirst + rocks.tiles[bhead + 1];
+        const float base = out;
+        const float x = xs[col], xLow = lows[col];
+        const float y = ys[localRow], yLow = lows[params.width + row];
+        for (uint32_t e = bfirst; e < bend; ++e) {
+            const uint32_t n = rocks.entries[e];
+            const tg_boulder& b = rocks.boulders[n];
+            const float dx = (x - b.x) + (xLow - b.x_low);
+            const float dy = (y - b.y) + (yLow - b.y_low);
+            if (dx * dx + dy * dy >= rocks.cull[n]) {
+                continue;
+            }
+            out = tg::ApplyBoulder(b, rocks.kernel.width, out, base, dx, dy);
+        }
+    }
+    heights[i] = out;
 }
 
 // CraterRelief (craters.h:1225-1238): the relief crater `c` adds at (dx, dy) from its centre
@@ -343,19 +378,27 @@ __global__ __launch_bounds__(64) void CraterBodyDatumStrip(
 }
 
 void LaunchCraters(const tg_crater_set& set, float* heights, const float* xs, const float* ys,
-                   const tg_kernel_params& params, hipStream_t stream) {
+                   const tg_kernel_params& params, hipStream_t stream,
+                   const tg_boulder_set* rocks = nullptr) {
     if (params.width == 0 || params.rows == 0) {
         return;
     }
     const dim3 block(kCraterTile * kCraterTile, 1, 1);
     const dim3 grid((params.width + kCraterTile - 1) / kCraterTile,
                     (params.rows + kCraterTile - 1) / kCraterTile, 1);
+    BoulderPass pass{};
+    if (rocks != nullptr) {
+        pass = BoulderPass{rocks->kernel, rocks->boulders, rocks->tiles, rocks->entries,
+                           rocks->cull};
+    }
     // The compact kernel for a plane set whose craters all take the hoisted path.
-    auto* fn = set.allFast && !set.onBody ? CraterStrip<true> : CraterStrip<false>;
+    const bool fast = set.allFast && !set.onBody;
+    auto* fn = rocks != nullptr ? (fast ? CraterStrip<true, true> : CraterStrip<false, true>)
+                                : (fast ? CraterStrip<true> : CraterStrip<false>);
     hipLaunchKernelGGL(fn, grid, block, 0, stream, heights, xs, ys, set.kernel,
                        set.records, set.data, set.tiles, set.entries, set.frames, set.points,
                        set.lows, set.cull, set.harm, set.outline, set.terms, set.precise,
-                       params);
+                       params, pass);
     Check(hipGetLastError(), "launching the crater kernel");
 }
 
@@ -929,6 +972,34 @@ int tg_hip_crater_launch(tg_dev* dev, const tg_crater_set* set, float* heights, 
     return Guard(err, errLen, [&] {
         CheckLaunch(*set, *params);
         LaunchCraters(*set, heights, xs, ys, *params, static_cast<hipStream_t>(stream));
+    });
+}
+
+int tg_hip_crater_boulder_launch(tg_dev* dev, const tg_crater_set* set,
+                                 const tg_boulder_set* boulders, float* heights, const float* xs,
+                                 const float* ys, const tg_kernel_params* params, void* stream,
+                                 char* err, size_t errLen) {
+    if (dev == nullptr || set == nullptr || boulders == nullptr || params == nullptr ||
+        heights == nullptr || xs == nullptr || ys == nullptr) {
+        SetErr(err, errLen, "tg_hip_crater_boulder_launch: null argument");
+        return TG_ERR_ARGUMENT;
+    }
+    if (set->onBody) {
+        SetErr(err, errLen,
+               "tg_hip_crater_boulder_launch: the set is binned on a body; use the batched path");
+        return TG_ERR_ARGUMENT;
+    }
+    return Guard(err, errLen, [&] {
+        CheckLaunch(*set, *params);
+        // The boulder set's tiles and lows must cover the strip as its own launch requires.
+        if (uint64_t(boulders->tilesAcross) * boulders->tileSide < params->width ||
+            uint64_t(boulders->tileRows) * boulders->tileSide <
+                uint64_t(params->first_row) + params->rows ||
+            boulders->lowCount < size_t(params->width) + params->first_row + params->rows) {
+            throw HipError{"the boulder tiles or lows do not cover the grid"};
+        }
+        LaunchCraters(*set, heights, xs, ys, *params, static_cast<hipStream_t>(stream),
+                      boulders);
     });
 }
 

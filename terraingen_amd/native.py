@@ -162,10 +162,15 @@ HIP_ABI = {
     "tg_hip_batch_run_static": (C.c_int, [VP, VP, SZ, VP, SZ, VP, SZ, VP, C.c_float, VP,
                                           C.c_double, VP, SZ, VP, CP, SZ]),
     "tg_hip_catalogue_frames": (C.c_int, [VP, VP, SZ, CP, SZ]),
+    "tg_hip_boulder_catalogue_create": (C.c_int, [VP, VP, SZ, VP, VP, VP, CP, SZ]),
+    "tg_hip_boulder_catalogue_free": (C.c_int, [VP]),
+    "tg_hip_batch_stage_boulders": (C.c_int, [VP, VP, VP, SZ, VP, SZ, C.c_int, CP, SZ]),
     "tg_hip_fill": (C.c_int, [VP, VP, C.c_float, SZ, VP, CP, SZ]),
     "tg_hip_boulder_set_create": (C.c_int, [VP, VP, VP, SZ, VP, SZ, VP, SZ, VP, SZ,
                                             C.POINTER(VP), CP, SZ]),
     "tg_hip_boulder_set_free": (C.c_int, [VP]),
+    "tg_hip_crater_boulder_launch": (C.c_int, [VP, VP, VP, VP, VP, VP,
+                                               C.POINTER(KernelParams), VP, CP, SZ]),
     "tg_hip_boulder_launch": (C.c_int, [VP, VP, VP, VP, VP, C.POINTER(KernelParams), VP, CP,
                                         SZ]),
     "tg_hip_valu_probe": (C.c_int, [VP, C.c_int, u32, u32, C.POINTER(C.c_float),
@@ -862,6 +867,7 @@ TileDesc = _struct("TileDesc", [("origin_x", C.c_double), ("origin_y", C.c_doubl
 
 HOST_ABI["tgx_server_create"] = (C.c_int, [VP, VP, C.c_float, VP, VP, C.POINTER(ServerConfig),
                                            C.POINTER(VP), CP, SZ])
+HOST_ABI["tgx_server_boulders"] = (C.c_int, [VP, VP, VP, CP, SZ])
 HOST_ABI["tgx_server_submit"] = (C.c_int, [VP, C.POINTER(TileDesc), VP, C.POINTER(C.c_uint64),
                                            CP, SZ])
 HOST_ABI["tgx_server_wait"] = (C.c_int, [VP, C.c_uint64, C.c_double, C.POINTER(TicketInfo), CP,

@@ -182,8 +182,19 @@ class PlaneTerrain:
         self.dev.call("tg_hip_boulder_launch", self.dev.handle, self.boulder_set, heights.ptr,
                       self.d_xs.ptr, self.d_ys.ptr, C.byref(self.params), stream)
 
-    def launch(self, heights: N.DeviceArray, stream=None) -> None:
+    def launch_craters_and_boulders(self, heights: N.DeviceArray, stream=None) -> None:
+        """Both layers in one pass over the grid (tg_hip_crater_boulder_launch)."""
+        self.dev.call("tg_hip_crater_boulder_launch", self.dev.handle, self.crater_set,
+                      self.boulder_set, heights.ptr, self.d_xs.ptr, self.d_ys.ptr,
+                      C.byref(self.params), stream)
+
+    def launch(self, heights: N.DeviceArray, stream=None, fused: bool = True) -> None:
+        """The source fill, then the craters and the boulders: in one pass when the terrain has
+        both (and `fused`), else one launch per layer."""
         self.launch_noise(heights, stream)
+        if fused and self.crater_set is not None and self.boulder_set is not None:
+            self.launch_craters_and_boulders(heights, stream)
+            return
         self.launch_craters(heights, stream)
         self.launch_boulders(heights, stream)
 
@@ -225,6 +236,9 @@ class TileBatch:
     finest_pixel: float = 0.25
     elevation: float = 0.0
     timings: dict = field(default_factory=dict)
+    # A boulder layer after the craters: a list in BOULDER_DTYPE (x, y in meters on a plane,
+    # longitude and latitude on a sphere), drawn in each tile at the tile's spacing.
+    boulders: np.ndarray | None = None
 
     def build(self) -> None:
         spec = self.craters
@@ -266,13 +280,48 @@ class TileBatch:
         self.batch = None
         self.timings.update(generate_ms=(t1 - t0) * 1e3, prepare_ms=(t2 - t1) * 1e3,
                             catalogue_upload_ms=(t3 - t2) * 1e3)
+        self._build_boulders()
+
+    def _build_boulders(self) -> None:
+        """The boulder catalogue on the device (tg_hip_boulder_catalogue_create): every boulder
+        resolved at the finest spacing, prepared there for its geometry, with its diameter and,
+        on a sphere, its frame, and an index over relief boxes of 0.75 D (half of D plus the
+        widest band a tile that resolves the boulder draws it with, D / 2)."""
+        self.boulder_catalogue = None
+        if self.boulders is None:
+            return
+        t0 = time.perf_counter()
+        radius = getattr(self, "radius", 0.0)
+        _, records, foot = N.prepare_boulders(self.boulders, self.finest_pixel)
+        self.boulder_records, self.boulder_foot = records, foot
+        diam = np.ascontiguousarray(foot[:, 3])
+        frames = N.sphere_frames(foot[:, 0], foot[:, 1], radius) if radius > 0.0 else None
+        box = np.column_stack([foot[:, 0], foot[:, 1], 0.75 * foot[:, 3], foot[:, 3]])
+        self.boulder_index = N.CatalogueIndex(box, radius=radius)
+        out = C.c_void_p()
+        self.dev.call("tg_hip_boulder_catalogue_create", self.dev.handle, records.ctypes.data,
+                      records.size, diam.ctypes.data,
+                      frames.ctypes.data if frames is not None else None, C.byref(out))
+        self.boulder_catalogue = out
+        self.timings.update(boulder_prepare_ms=(time.perf_counter() - t0) * 1e3)
+
+    def _stage_boulders(self, threads: int) -> None:
+        """Bins the boulders against the staged tiles on the host (tgx_index_bin) and stages
+        their lists (tg_hip_batch_stage_boulders)."""
+        descs, bins, entries = self.boulder_index.bin(self.tiles, threads)
+        if not np.array_equal(descs["bin_first"], self.descs["bin_first"]):
+            raise RuntimeError("the boulder bins do not follow the crater bins")
+        self.boulder_bins, self.boulder_entries = bins, entries
+        self.dev.call("tg_hip_batch_stage_boulders", self.batch, self.boulder_catalogue,
+                      bins.ctypes.data, bins.size, entries.ctypes.data, entries.size, 0)
 
     def stage(self, tiles: np.ndarray, threads: int = 0, device_bins: bool = False,
-              bin_side: int = 16) -> None:
+              bin_side: int = 16, with_boulders: bool = True) -> None:
         """Bins `tiles` against the catalogue and stages them on the batch: on the host into
         16-cell bins (tgx_index_bin, tg_hip_batch_stage), or with `device_bins` the host finds
         each tile's crater windows (tgx_index_windows) and the device bins them into bins of
-        `bin_side` cells (tg_hip_batch_stage_windows)."""
+        `bin_side` cells (tg_hip_batch_stage_windows). The boulders, when the batch has them
+        (and `with_boulders`), are binned on the host and staged after the craters."""
         t0 = time.perf_counter()
         self.tiles = np.ascontiguousarray(tiles, N.TILE_DTYPE)
         if device_bins:
@@ -312,8 +361,12 @@ class TileBatch:
             self.dev.call("tg_hip_batch_stage", self.batch, self.descs.ctypes.data,
                           self.descs.size, self.bins.ctypes.data, self.bins.size,
                           self.entries.ctypes.data, self.entries.size)
-        t2 = time.perf_counter()
         self.bin_side = bin_side
+        if self.boulder_catalogue is not None and with_boulders:
+            if bin_side != 16:
+                raise ValueError("boulders are binned into 16-cell bins")
+            self._stage_boulders(threads)
+        t2 = time.perf_counter()
         self.bin_count = bins
         self.cells = int((self.tiles["width"].astype(np.int64) *
                           self.tiles["height"].astype(np.int64)).sum())
@@ -455,9 +508,10 @@ class TileBatch:
             inputs["foot"] = foot
         return inputs
 
-    def plain_run(self, inputs: dict) -> np.ndarray:
-        """The tile through tg_hip_run: the noise layer kernel then the crater kernel, each a
-        synchronous call with host buffers, as RunKernel drives a device per layer."""
+    def plain_run(self, inputs: dict, with_boulders: bool = True) -> np.ndarray:
+        """The tile through tg_hip_run: the noise layer kernel, the crater kernel, then the
+        boulder kernel when the batch has boulders (and `with_boulders`), each a synchronous call
+        with host buffers, as RunKernel drives a device per layer."""
         g = inputs["grid"]
         h = np.full((g.height, g.width), self.elevation, np.float32)
         params = N.KernelParams(g.width, g.height, N.NODATA_NONE, 0.0, 0)
@@ -471,10 +525,19 @@ class TileBatch:
                      (inputs["lows"], 0)]
             self.dev.run(N.TG_KERNEL_CRATERS, kernel, h, inputs["xs"], inputs["ys"], params,
                          extra)
+        if self.boulders is not None and with_boulders:
+            # The boulder layer at the tile's spacing, as the reference prepares it per grid.
+            g = inputs["grid"]
+            kernel, records, foot = N.prepare_boulders(self.boulders, g.pixel)
+            if records.size:
+                tiles, entries = N.bin_plane(g, foot[:, 0], foot[:, 1], foot[:, 2], 16)
+                self.dev.run(N.TG_KERNEL_BOULDERS, kernel, h, inputs["xs"], inputs["ys"], params,
+                             [(records, 0), (tiles, 0), (entries, 0), (inputs["lows"], 0)])
         return h
 
     def close(self) -> None:
         for name, fn in (("batch", "tg_hip_batch_free"), ("catalogue", "tg_hip_catalogue_free"),
+                         ("boulder_catalogue", "tg_hip_boulder_catalogue_free"),
                          ("plan", "tg_hip_noise_plan_free")):
             h = getattr(self, name, None)
             if h is not None:
@@ -545,6 +608,7 @@ class SphereTileBatch(TileBatch):
             self.noise_kernel = kernel
         self.batch = None
         self.timings.update(prepare_ms=(t1 - t0) * 1e3, catalogue_upload_ms=(t2 - t1) * 1e3)
+        self._build_boulders()
 
     def tile_spacing(self, i: int) -> float:
         """RasterGrid::Spacing of tile i: its pixel (degrees) times the unit length."""
@@ -718,6 +782,11 @@ class TileServer:
             raise N.HipError(err.value.decode())
         self.handle = h.value
         self._dst = {}
+        if getattr(batch, "boulder_catalogue", None) is not None:
+            # The batch's boulder layer, drawn after the craters in every served tile.
+            if self.lib.tgx_server_boulders(self.handle, batch.boulder_catalogue,
+                                            batch.boulder_index.handle, err, 512):
+                raise N.HipError(err.value.decode())
 
     def submit(self, origin_x: float, origin_y: float, pixel: float, width: int,
                height: int) -> int:

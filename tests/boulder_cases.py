@@ -89,3 +89,39 @@ def oracle_heights_f64(orc, case: BoulderCase) -> np.ndarray:
 def binding(case: BoulderCase) -> list[tuple[np.ndarray, int]]:
     """The boulder kernel's extra buffers in binding order: boulders, tiles, entries, lows."""
     return [(case.records, 0), (case.tiles, 0), (case.entries, 0), (case.lows, 0)]
+
+
+SPHERE_NAMES = ["midlat", "polar_fade"]
+
+
+@dataclass
+class SphereBoulderCase:
+    name: str
+    spec: object
+    grid: N.Grid         # longitude-latitude, degrees
+    boulders: np.ndarray
+    heights: np.ndarray  # the reference's FP32 heights over a raster of zeros
+
+    @property
+    def spacing(self) -> float:
+        """RasterGrid::Spacing on the sphere: the cell's height in degrees times the unit
+        length."""
+        return self.grid.pixel * self.spec.radius * np.pi / 180.0
+
+
+def load_sphere(name: str) -> SphereBoulderCase:
+    from oracle import RefBoulderSphereCase
+
+    z = np.load(GOLDEN / f"boulders_sphere_{name}.npz")
+    spec = RefBoulderSphereCase.from_buffer_copy(z["case"].tobytes())
+    grid = N.Grid(spec.side, spec.side, spec.lon, spec.lat, spec.step)
+    boulders = np.ascontiguousarray(z["boulders"]).view(N.BOULDER_DTYPE)
+    return SphereBoulderCase(name, spec, grid, boulders, np.ascontiguousarray(z["heights"]))
+
+
+def sphere_oracle_f64(orc, case: SphereBoulderCase, base: np.ndarray | None = None) -> np.ndarray:
+    """or_boulder_body_f64 over `base` (zeros by default), FP64."""
+    h = (np.zeros(case.heights.shape) if base is None else base).astype(np.float64)
+    orc.or_boulder_body_f64(case.boulders.ctypes.data, case.boulders.size,
+                            C.byref(tgx_grid(case.grid)), case.spec.radius, 1, h.ctypes.data)
+    return h

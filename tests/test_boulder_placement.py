@@ -19,7 +19,8 @@ FIELDS = ["x", "y", "diameter", "height", "burial", "yaw", "lean", "wear", "id",
 import sys  # noqa: E402
 
 sys.path.insert(0, str(GOLDEN))
-from make_boulder_fixtures import CASES, spec  # noqa: E402
+from make_boulder_fixtures import PLACEMENT_CASES as CASES  # noqa: E402
+from make_boulder_fixtures import placement_spec as spec  # noqa: E402
 
 
 def mine(case: dict, threads: int = 0, **pose) -> np.ndarray:

@@ -10,7 +10,8 @@ import pytest
 
 from terraingen_amd import native as N
 
-from boulder_cases import NAMES, load, oracle_heights, oracle_heights_f64
+from boulder_cases import (NAMES, SPHERE_NAMES, load, load_sphere, oracle_heights,
+                          oracle_heights_f64, sphere_oracle_f64)
 
 
 @pytest.fixture(scope="module", params=NAMES)
@@ -70,3 +71,42 @@ def test_dome_closed_forms(orc):
     assert relief_cut(0.2, 0.0) == 0.0
     assert relief_cut(-0.9, 0.0) == pytest.approx(min(0.8 * math.sqrt(1 - 0.81),
                                                       0.9 * math.tan(math.radians(55.0))))
+
+
+@pytest.fixture(scope="module", params=SPHERE_NAMES)
+def sphere_case(request):
+    return load_sphere(request.param)
+
+
+def test_body_oracle_tracks_the_reference_on_a_sphere(orc, sphere_case):
+    """The FP64 restatement on a sphere (SphereOffset, the dome at the raster's spacing) against
+    the reference's BoulderLayer on Surface::Sphere, to the FP32 rounding of its stored heights."""
+    got = sphere_oracle_f64(orc, sphere_case)
+    ref = sphere_case.heights
+    assert (ref > 0.0).sum() > 50  # the layer drew boulders into the raster
+    assert np.abs(got - ref).max() <= 1e-7
+
+
+def test_sphere_offset_is_the_azimuthal_equidistant_offset(orc):
+    """SphereOffset against the great-circle distance (haversine) and the initial bearing
+    (atan2(sin dlon cos lat2, cos lat1 sin lat2 - sin lat1 cos lat2 cos dlon)) of textbook
+    spherical trigonometry: the offset's length is the arc and its direction the bearing, east
+    of north."""
+    rng = np.random.default_rng(5)
+    radius = 1737400.0
+    out = np.zeros(2)
+    for _ in range(200):
+        lon1, lat1 = rng.uniform(-180, 180), rng.uniform(-89, 89)
+        scale = 10.0 ** rng.uniform(-6, 0)
+        lon2 = lon1 + rng.uniform(-1, 1) * scale
+        lat2 = float(np.clip(lat1 + rng.uniform(-1, 1) * scale, -89.9, 89.9))
+        orc.or_sphere_offset(radius, lon1, lat1, lon2, lat2, out.ctypes.data)
+        p1, p2, dl = map(math.radians, (lat1, lat2, lon2 - lon1))
+        hav = math.sin((p2 - p1) / 2) ** 2 + math.cos(p1) * math.cos(p2) * math.sin(dl / 2) ** 2
+        arc = 2 * radius * math.asin(math.sqrt(hav))
+        bearing = math.atan2(math.sin(dl) * math.cos(p2),
+                             math.cos(p1) * math.sin(p2) - math.sin(p1) * math.cos(p2) * math.cos(dl))
+        assert math.hypot(*out) == pytest.approx(arc, rel=1e-9, abs=1e-9)
+        if arc > 1e-3:
+            assert out[0] == pytest.approx(arc * math.sin(bearing), rel=1e-7, abs=1e-7 * arc)
+            assert out[1] == pytest.approx(arc * math.cos(bearing), rel=1e-7, abs=1e-7 * arc)
