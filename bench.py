@@ -716,7 +716,7 @@ def bench_sphere(args, N, dev) -> dict:
 
 def bench_serving(args, N, dev) -> dict:
     from terraingen_amd.pipeline import CraterLayerSpec, SphereTileBatch, TileBatch, TileServer
-    from terraingen_amd.serving import CameraStream, GreatCircleStream, serve
+    from terraingen_amd.serving import CameraStream, GreatCircleStream, serve, served_boulders
 
     sphere = args.workload == "sphere"
     stream = (GreatCircleStream(duration=args.serve_seconds, radius=MOON) if sphere
@@ -730,17 +730,22 @@ def bench_serving(args, N, dev) -> dict:
                            max_diameter=300.0, complex_=1, transition_diameter=50.0, peaks=1,
                            terraces=1, terrace_onset=150.0, terrace_full=300.0, seed=5)
     finest = stream.finest_side / (max(stream.edges) - 1)
+    # The boulder field of C3 (N(>D) = 3.8e-3 D^-3 per m^2, D 0.2-3 m, seed 5) as the
+    # reference's BoulderLayer lists it for each served tile.
+    t_b = time.perf_counter()
+    boulders, boulder_stats = served_boulders(stream, requests, 3.8e-3, 3.0, 0.2, 3.0, seed=5)
+    boulder_place_ms = (time.perf_counter() - t_b) * 1e3
     if sphere:
         craters = stream.placed_swath(1000.0, 0.0156, 2.2, 0.5, 300.0,
                                       margin_radii=N.margin_radii(spec.models(), spec.apply()),
                                       seed=5)
-        place_ms = (time.perf_counter() - t0) * 1e3
+        place_ms = (time.perf_counter() - t0) * 1e3 - boulder_place_ms
         swath = SphereTileBatch(dev, [C2_FRACTAL], spec, region=None, finest_pixel=finest,
-                                radius=MOON, craters_given=craters)
+                                radius=MOON, craters_given=craters, boulders=boulders)
     else:
         swath = TileBatch(dev, [C2_FRACTAL], spec,
                           region=(-2000.0, -1000.0, stream.path_length + 2000.0, 1000.0),
-                          finest_pixel=finest)
+                          finest_pixel=finest, boulders=boulders)
     swath.build()
     server = TileServer(swath, slots=4, max_batch_tiles=32, max_batch_cells=1 << 24,
                         pool_bytes=64 << 30, bin_threads=8)
@@ -751,10 +756,30 @@ def bench_serving(args, N, dev) -> dict:
         server.wait(server.submit(float(r["origin_x"]), float(r["origin_y"]),
                                   float(r["pixel"]), int(r["width"]), int(r["height"])))
     print(f"[bench] serving {requests.size} requests over {stream.duration} s, "
-          f"{swath.records.size} craters", file=sys.stderr, flush=True)
-    rec = serve(server, requests)
+          f"{swath.records.size} craters, {swath.boulder_records.size} boulders",
+          file=sys.stderr, flush=True)
+    # Every 97th served tile is kept and checked against the same tiles in one batched launch.
+    kept = {}
+
+    def keep(i, heights):
+        if i % 97 == 0:
+            kept[i] = heights.copy()
+
+    rec = serve(server, requests, keep=keep)
     stats = server.stats()
     server.close()
+    sample = np.array(sorted(kept))
+    check = np.zeros(sample.size, N.TILE_DTYPE)
+    for name in ("origin_x", "origin_y", "pixel", "width", "height"):
+        check[name] = requests[name][sample]
+    swath.stage(check)
+    out = N.DeviceArray(dev, swath.cells * 4)
+    swath.run(out, use_graph=False)
+    swath.wait()
+    batched = out.download(np.empty(swath.cells, np.float32))
+    served_diff = max((float(np.abs(kept[i] - swath.tile_heights(batched, k)).max())
+                       for k, i in enumerate(sample)), default=0.0)
+    del out
     swath.close()
     lat_ms = rec["latency_s"] * 1e3
     is513 = requests["width"] == 513
@@ -766,7 +791,8 @@ def bench_serving(args, N, dev) -> dict:
                   "; a 2 km swath of C3's craters (N(>D)=0.0156 D^-2.2, D 0.5-300 m, complex "
                   "from 50 m with peaks, terraces from 150 m" +
                   (", placed as the reference places them on the sphere)" if sphere else ")") +
-                  "; no boulders",
+                  "; C3's boulders (N(>D)=3.8e-3 D^-3, D 0.2-3 m) as the reference's "
+                  "BoulderLayer lists them for each tile at its spacing, drawn after the craters",
         "requests": int(requests.size), "rate_per_s": stream.rate,
         "duration_s": stream.duration,
         "latency_513_ms": {"p50": round(percentile(lat_ms[is513], 50), 3),
@@ -782,12 +808,69 @@ def bench_serving(args, N, dev) -> dict:
         "levels": sorted(int(v) for v in np.unique(requests["level"])),
         "craters": int(swath.records.size), "setup_ms": round(setup_ms, 1),
         "placement_ms": round(place_ms, 1) if sphere else None,
+        "boulders": int(swath.boulder_records.size), "boulder_placement_ms":
+            round(boulder_place_ms, 1), "boulders_by_spacing": boulder_stats,
+        "served_vs_batched": {"tiles": int(sample.size), "max_diff_m": served_diff,
+                              "note": "every 97th served tile against the same tiles in one "
+                                      "batched launch (which the tests hold bit-equal to the "
+                                      "plain per-tile kernels)"},
     }
+
+
+def grid_as_one_tile(args, N, dev, terrain, spec, strip_heights: np.ndarray) -> dict:
+    """The C3 grid through the batched kernel as one tile: the same crater list (generated over
+    the grid's box at its spacing) and boulder list as the strip passes, staged as one tile of
+    16 x 16 bins and run as a captured graph on the batch's stream; checked bit for bit against
+    the strip passes' heights."""
+    from terraingen_amd.pipeline import TileBatch
+
+    g = terrain.grid
+    region = (g.origin_x, g.origin_y - g.height * g.pixel, g.origin_x + g.width * g.pixel,
+              g.origin_y)
+    t0 = time.perf_counter()
+    batch = TileBatch(dev, [], spec, region=region, finest_pixel=g.pixel,
+                      elevation=terrain.elevation, boulders=terrain.boulders)
+    batch.build()
+    one = np.zeros(1, N.TILE_DTYPE)
+    one[0] = (g.origin_x, g.origin_y, g.pixel, g.width, g.height)
+    batch.stage(one)
+    setup_ms = (time.perf_counter() - t0) * 1e3
+    out = N.DeviceArray(dev, batch.cells * 4)
+    for _ in range(max(args.warmup, 1)):
+        batch.run(out)
+    batch.wait()
+    start, stop = N.Event(dev), N.Event(dev)
+    t1 = time.perf_counter()
+    start.record(batch.stream)
+    for _ in range(args.steps):
+        batch.run(out)
+    stop.record(batch.stream)
+    batch.wait()
+    wall = time.perf_counter() - t1
+    ms = start.elapsed_ms(stop) / args.steps
+    got = out.download(np.empty((g.height, g.width), np.float32))
+    same = bool(np.array_equal(got.view(np.uint32), strip_heights.view(np.uint32)))
+    diff = float(np.abs(got - strip_heights).max())
+    strip_ms = terrain.timings.get("_strip_ms")
+    result = {"kernel": "BatchTiles<culled, general, boulders>", "ms": round(ms, 4),
+              "bins": int(batch.bin_count), "craters": int(batch.records.size),
+              "boulders": int(batch.boulder_records.size), "setup_ms": round(setup_ms, 1),
+              "bit_equal_to_strip_passes": same, "max_diff_vs_strip_m": diff,
+              "strip": {"path": "noise fill, then CraterStrip with the boulders in the same "
+                                "pass (tg_hip_crater_boulder_launch), one thread per cell "
+                                "walking its 16 x 16 tile's lists",
+                        "ms": round(strip_ms, 4) if strip_ms else None,
+                        "value": round(g.width * g.height / (strip_ms * 1e-3) / 1e9, 3)
+                        if strip_ms else None, "unit": "Gcells/s"},
+              "_ms": ms, "_wall_s": wall}
+    del out
+    batch.close()
+    return result
 
 
 def bench_grid(args, N, dev) -> dict:
     from terraingen_amd.pipeline import (CraterLayerSpec, PlaneTerrain, crater_pair_cost,
-                                         plane_batch_view, scatter_boulders)
+                                         plane_batch_view)
 
     c3 = args.workload == "c3"
     n = args.size if args.size else (8193 if c3 else 4097)
@@ -804,11 +887,14 @@ def bench_grid(args, N, dev) -> dict:
     t0 = time.perf_counter()
     terrain.prepare()
     if c3:
-        # The boulder layer over the grid's box grown by the largest boulder: C3's sizes, 30 %
-        # around crater rims, drawn as dome relief at the grid spacing.
-        half = 0.5 * n * pixel + 3.0
-        terrain.boulders = scatter_boulders((-half, -half, half, half), 3.8e-3, 3.0, 0.2, 3.0,
-                                            seed=2, rim_share=0.3, craters=terrain.crater_list)
+        # The boulder layer as the reference's BoulderLayer lists it for the grid (its box grown
+        # by half a sample, at the grid's spacing): C3's sizes, uniform background abundance,
+        # drawn as dome relief at the grid spacing.
+        half = 0.5 * n * pixel + 0.5 * pixel
+        t_b = time.perf_counter()
+        placed = N.place_boulders((-half, -half, half, half), pixel, 3.8e-3, 3.0, 0.2, 3.0, seed=2)
+        terrain.timings["boulder_placement_ms"] = (time.perf_counter() - t_b) * 1e3
+        terrain.boulders = N.relief_boulders(placed)
         terrain.all_boulders = terrain.boulders.size
         terrain._prepare_boulders()
     host_ms = (time.perf_counter() - t0) * 1e3
@@ -828,6 +914,7 @@ def bench_grid(args, N, dev) -> dict:
     dev.synchronize(stream)
     wall = time.perf_counter() - t0
     step_ms = start.elapsed_ms(stop) / args.steps
+    terrain.timings["_strip_ms"] = step_ms
 
     def time_kernel(fn):
         a, b = N.Event(dev), N.Event(dev)
@@ -841,6 +928,8 @@ def bench_grid(args, N, dev) -> dict:
     noise_ms = time_kernel(lambda: terrain.launch_noise(heights, stream))
     crater_ms = time_kernel(lambda: terrain.launch_craters(heights, stream))
     boulder_ms = time_kernel(lambda: terrain.launch_boulders(heights, stream)) if c3 else 0.0
+    fused_ms = (time_kernel(lambda: terrain.launch_craters_and_boulders(heights, stream))
+                if c3 else 0.0)
     terrain.launch(heights, stream)
     dev.synchronize(stream)
     got = heights.download(np.empty((n, n), np.float32))
@@ -859,19 +948,30 @@ def bench_grid(args, N, dev) -> dict:
         bcost = np.full(terrain.boulder_count, float(OPS_BOULDER_PAIR))
         _, b_inside, boulder_ops = N.batch_pair_ops(bdescs, bbins, bentries, terrain.boulder_foot,
                                                     bcost)
-        kernels["BoulderStrip"] = (boulder_ms, boulder_ops)
+        # The step runs the craters and boulders in one pass over the grid; the separate
+        # passes are timed for comparison.
+        kernels = {"FillValue": (noise_ms, 0),
+                   "CraterStrip<boulders>": (fused_ms, crater_ops + boulder_ops)}
         config |= {"boulders": int(terrain.all_boulders),
                    "boulders_drawn": int(terrain.boulder_count),
-                   "boulder_pairs_in_reach": b_inside}
+                   "boulder_pairs_in_reach": b_inside,
+                   "separate_passes_ms": {"CraterStrip": round(crater_ms, 4),
+                                          "BoulderStrip": round(boulder_ms, 4)}}
         config["workload"] = (f"flat source {n}x{n} @0.25 m + power-law craters N(>D)=0.0156 "
                               "D^-2.2, D 0.5-300 m, complex from 50 m with central peaks, "
                               "terraces from 150 m (full at 300 m), seed 2 + boulders "
-                              "N(>D)=3.8e-3 D^-3, D 0.2-3 m, "
-                              "30 % around crater rims (synthetic field), dome relief")
+                              "N(>D)=3.8e-3 D^-3, D 0.2-3 m, seed 2, placed as the reference's "
+                              "BoulderLayer places them (uniform background, no clusters), "
+                              "dome relief, drawn in the crater pass")
     else:
         config["workload"] = (f"synthetic source {n}x{n} @0.5 m: 8-octave Perlin fBm (512 m, "
                               "lac 2, gain 0.5, 20 m, seed 1) + power-law craters "
                               "N(>D)=0.05 D^-2, D 1-200 m, seed 1")
+    batched = None
+    if c3:
+        batched = grid_as_one_tile(args, N, dev, terrain, spec, got)
+        step_ms, wall = batched.pop("_ms"), batched.pop("_wall_s")
+        kernels = {"BatchTiles<boulders>": (step_ms, crater_ops + boulder_ops)}
     dominant = max(kernels, key=lambda k: kernels[k][0])
     dom_ms, dom_ops = kernels[dominant]
     result = {
@@ -880,10 +980,13 @@ def bench_grid(args, N, dev) -> dict:
         "dtype": "f32+u32",
         "config": {"workload": config.pop("workload")} | config,
         "kernels_ms": {k: round(v[0], 4) for k, v in kernels.items()},
-        "host_ms": {k: round(v, 1) for k, v in terrain.timings.items()} | {
-            "total_prepare": round(host_ms, 1)},
+        "host_ms": {k: round(v, 1) for k, v in terrain.timings.items()
+                    if not k.startswith("_")} | {"total_prepare": round(host_ms, 1)},
         "roofline": roofline(dom_ops, dom_ms, cells * 4.0) | {"kernel": dominant},
     }
+    if batched is not None:
+        result["plain"] = batched.pop("strip")
+        result["batched"] = batched
     if not args.no_cpu_baseline:
         base = cpu_grid(terrain, 16, os.cpu_count() or 1)
         r0, fp32, fp64 = base.pop("_r0"), base.pop("_fp32"), base.pop("_fp64")

@@ -200,6 +200,14 @@ typedef struct tgx_placed_boulder {
  * bit-identical to the reference's list. */
 int tgx_place_boulders(const tgx_boulder_layer* layer, tgx_placed_boulder** boulders,
                        size_t* count, char* err, size_t err_len);
+/* tgx_place_boulders over the union of `box_count` regions (min_x, min_y, max_x, max_y each; the
+ * layer's own region is not read), as a server places the boulders of many tiles at once: on a
+ * sphere one generation over the union (each cube-sphere cell drawn once, the reference's order
+ * for a union of regions); on a plane one per region, each boulder listed once. Every boulder
+ * the layer lists over one of the regions is listed, with the same draws. */
+int tgx_place_boulders_boxes(const tgx_boulder_layer* layer, const double* boxes,
+                             size_t box_count, tgx_placed_boulder** boulders, size_t* count,
+                             char* err, size_t err_len);
 
 /* BoulderLayer::PrepareRelief for one grid `spacing` (boulder_relief.cpp:373-423): the boulders
  * drawn into heights (not models, resolution weight above 0) as device records, in list order,

@@ -114,6 +114,14 @@ namespace {
 constexpr uint32_t kBinSide = 16;
 constexpr uint32_t kGroupThreads = 256;  // four independent waves
 
+// Waves per SIMD the all-fast variants are built for: 8 without the FP64 path, and with it the
+// fastest of 4-8 measured on the C4 sphere batch (4: 52.7 ms, 5: 50.7, 6: 49.6, 7: 49.3,
+// 8: 50.7): fewer waves spill less of the FP64 terms but hide less latency.
+#ifndef TG_PRECISE_WAVES
+#define TG_PRECISE_WAVES 7
+#endif
+constexpr int kPreciseWaves = TG_PRECISE_WAVES;
+
 // The bins are split into kQueues contiguous ranges, one per XCD: workgroups are dispatched to the
 // XCDs in turn, so workgroup g starts on range g % kQueues, and neighbouring bins (which list the
 // same craters) are shaped by waves sharing one L2. Each range has its own counter in its own
@@ -447,7 +455,7 @@ __device__ __forceinline__ void CulledWalk(uint32_t first, uint32_t end, uint32_
 // variant without the FP64 code, whose registers cost the other variants residency).
 // kBoulders: a boulder layer follows the craters (its code only in the variants that draw it).
 template <uint32_t kWalk, bool kAllFast, bool kPrecise, bool kBoulders = false>
-__global__ void __launch_bounds__(kGroupThreads, kAllFast ? (kPrecise ? 6 : 8) : 1)
+__global__ void __launch_bounds__(kGroupThreads, kAllFast ? (kPrecise ? kPreciseWaves : 8) : 1)
     BatchTiles(const tg_tile_desc* __restrict__ tiles, const tg_bin* __restrict__ bins,
                const uint32_t* __restrict__ entries, uint32_t* __restrict__ counters,
                const tg::NoisePlan* __restrict__ plan, float elevation,
@@ -642,7 +650,7 @@ __device__ __forceinline__ void VisitBodyCrater(uint32_t n, bool inside, tg::Pre
 // general crater walk and 64-bit lattice indices, carries their code).
 // kPrecise, kBoulders: as for BatchTiles.
 template <bool kAllFast, bool kSmall, bool kGeneral, bool kPrecise, bool kBoulders = false>
-__global__ void __launch_bounds__(kGroupThreads, kAllFast ? (kPrecise ? 6 : 8) : 1)
+__global__ void __launch_bounds__(kGroupThreads, kAllFast ? (kPrecise ? kPreciseWaves : 8) : 1)
     BatchBodyTiles(const tg_tile_desc* __restrict__ tiles, const tg_bin* __restrict__ bins,
                    const uint32_t* __restrict__ entries, uint32_t* __restrict__ counters,
                    const tg::NoisePlan* __restrict__ plan, float elevation, double radius,

@@ -18,12 +18,14 @@
 // Streams are Philox substreams keyed (seed, kBoulderAttributeStream, id low, id high, draw).
 
 #include "common.h"
+#include "cube_cells.h"
 #include "placement.h"
 #include "random.h"
 #include "terraingen_amd.h"
 
 #include <cmath>
 #include <stdexcept>
+#include <unordered_set>
 #include <vector>
 
 namespace tgh {
@@ -45,7 +47,29 @@ double Lerp(double lo, double hi, double u) {
     return lo + (hi - lo) * u;  // ValueRange::Lerp (boulder_shape.h:25-27)
 }
 
-std::vector<tgx_placed_boulder> Place(const tgx_boulder_layer& o) {
+// The layer's candidates over `regions`: on a sphere one generation over their union (each cube-
+// sphere cell drawn once); on a plane one generation per region, each candidate kept once (its
+// draws depend on its cell and band, not on the region, so this is the union's set).
+std::vector<Crater> Candidates(const tgx_boulder_layer& o, const std::vector<Box>& regions,
+                               const Density& sizes, const Ages& ages, std::uint32_t seed,
+                               const GenerationOptions& g) {
+    if (o.sphere_radius > 0.0) {
+        return GenerateCratersOnSphere(o.sphere_radius, regions, sizes, ages, seed, g);
+    }
+    std::vector<Crater> out;
+    std::unordered_set<std::uint64_t> seen;
+    for (const Box& r : regions) {
+        for (const Crater& c : GenerateCraters(r.minX, r.minY, r.maxX, r.maxY, sizes, ages, seed,
+                                               g)) {
+            if (seen.insert(c.id).second) {
+                out.push_back(c);
+            }
+        }
+    }
+    return out;
+}
+
+std::vector<tgx_placed_boulder> Place(const tgx_boulder_layer& o, const std::vector<Box>& regions) {
     if (!(o.sfd_coefficient > 0.0 && o.sfd_exponent > 0.0)) {
         throw std::invalid_argument("boulder sizes need a positive coefficient and exponent");
     }
@@ -62,11 +86,7 @@ std::vector<tgx_placed_boulder> Place(const tgx_boulder_layer& o) {
     g.minResolvedDiameter = o.spacing > 0.0 ? 2.0 * o.spacing : 0.0;  // fade.startSamples 2
     g.threads = o.threads;
     const std::uint32_t seed = SubstreamSeed(o.seed, kBoulderCandidateStream);
-    const std::vector<Crater> drawn =
-        o.sphere_radius > 0.0
-            ? GenerateCratersOnSphere(o.sphere_radius, {{o.min_x, o.min_y, o.max_x, o.max_y}},
-                                      candidates, fresh, seed, g)
-            : GenerateCraters(o.min_x, o.min_y, o.max_x, o.max_y, candidates, fresh, seed, g);
+    const std::vector<Crater> drawn = Candidates(o, regions, candidates, fresh, seed, g);
     std::vector<tgx_placed_boulder> out;
     out.reserve(drawn.size());
     for (const Crater& c : drawn) {
@@ -112,7 +132,30 @@ int tgx_place_boulders(const tgx_boulder_layer* layer, tgx_placed_boulder** boul
         if (layer == nullptr || boulders == nullptr || count == nullptr) {
             throw std::invalid_argument("tgx_place_boulders: null argument");
         }
-        const std::vector<tgx_placed_boulder> out = tgh::Place(*layer);
+        const std::vector<tgx_placed_boulder> out =
+            tgh::Place(*layer, {{layer->min_x, layer->min_y, layer->max_x, layer->max_y}});
+        *boulders = tgh::Export(out);
+        *count = out.size();
+    });
+}
+
+int tgx_place_boulders_boxes(const tgx_boulder_layer* layer, const double* boxes,
+                             size_t boxCount, tgx_placed_boulder** boulders, size_t* count,
+                             char* err, size_t errLen) {
+    return tgh::Guard(err, errLen, [&] {
+        if (layer == nullptr || boulders == nullptr || count == nullptr ||
+            (boxCount > 0 && boxes == nullptr)) {
+            throw std::invalid_argument("tgx_place_boulders_boxes: null argument");
+        }
+        std::vector<tgh::Box> regions(boxCount);
+        for (size_t i = 0; i < boxCount; ++i) {
+            regions[i] = {boxes[4 * i], boxes[4 * i + 1], boxes[4 * i + 2], boxes[4 * i + 3]};
+            if (!(regions[i].maxX >= regions[i].minX && regions[i].maxY >= regions[i].minY)) {
+                throw std::invalid_argument("tgx_place_boulders_boxes: box " + std::to_string(i) +
+                                            " is not a box");
+            }
+        }
+        const std::vector<tgx_placed_boulder> out = tgh::Place(*layer, regions);
         *boulders = tgh::Export(out);
         *count = out.size();
     });

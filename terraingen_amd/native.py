@@ -1010,6 +1010,30 @@ def place_boulders(region: tuple, spacing: float, sfd_coefficient: float, sfd_ex
         lib.tgx_free(ptr)
 
 
+HOST_ABI["tgx_place_boulders_boxes"] = (C.c_int, [C.POINTER(BoulderLayerSpec), VP, SZ,
+                                                  C.POINTER(VP), C.POINTER(SZ), CP, SZ])
+
+
+def place_boulders_boxes(boxes: np.ndarray, spacing: float, sfd_coefficient: float,
+                         sfd_exponent: float, min_d: float, max_d: float, seed: int = 1,
+                         sphere_radius: float = 0.0, threads: int = 0, **pose) -> np.ndarray:
+    """``tgx_place_boulders_boxes``: the boulders a power-law BoulderLayer lists over any of
+    `boxes` ([n, 4]: min_x, min_y, max_x, max_y) for `spacing`, each once."""
+    spec = BoulderLayerSpec(0.0, 0.0, 0.0, 0.0, sphere_radius, spacing, sfd_coefficient,
+                            sfd_exponent, min_d, max_d, **(BOULDER_POSE_DEFAULTS | pose),
+                            seed=seed, threads=threads)
+    boxes = np.ascontiguousarray(boxes, np.float64).reshape(-1, 4)
+    lib = host_library()
+    ptr, n = VP(), SZ()
+    err = C.create_string_buffer(512)
+    _check(lib.tgx_place_boulders_boxes(C.byref(spec), boxes.ctypes.data, boxes.shape[0],
+                                        C.byref(ptr), C.byref(n), err, 512), err)
+    try:
+        return _owned_copy(ptr, n.value * 80).view(PLACED_BOULDER_DTYPE)
+    finally:
+        lib.tgx_free(ptr)
+
+
 def relief_boulders(placed: np.ndarray) -> np.ndarray:
     """BOULDER_DTYPE of placed boulders for the dome relief: centre, diameter, exposed height
     (Boulder::ExposedHeight, height times one less the burial), model flag, no facets."""

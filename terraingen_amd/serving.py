@@ -190,6 +190,49 @@ class GreatCircleStream(CameraStream):
         return craters[keep]
 
 
+def served_boulders(stream, requests: np.ndarray, coefficient: float, exponent: float,
+                    min_diameter: float, max_diameter: float, seed: int = 5,
+                    threads: int = 0) -> tuple[np.ndarray, dict]:
+    """The boulders the stream's tiles draw, as the reference's BoulderLayer lists them for each
+    tile (ReliefBoulders over the tile's box grown by half a sample, at its spacing,
+    boulder_relief.cpp:287-291): for each spacing that resolves a boulder (below half the largest
+    diameter), one placement over the union of that spacing's tile boxes (tgx_place_boulders_boxes,
+    each box grown by the spacing and 3 m, which only adds boulders that reach no tile), and the
+    union over spacings, each boulder once. A boulder resolved at a spacing is listed with the same
+    draws at every finer one, so the union holds every tile's own list. Returns the relief list
+    (BOULDER_DTYPE; longitude and latitude on a sphere) and the placement's figures."""
+    from . import native as N
+
+    sphere = isinstance(stream, GreatCircleStream)
+    unit = stream.unit_length() if sphere else 1.0
+    spacing = requests["pixel"] * unit
+    parts, stats = [], {}
+    for sp in np.unique(spacing):
+        if not sp < 0.5 * max_diameter:
+            continue
+        sel = requests[spacing == sp]
+        w, h = sel["width"] * sel["pixel"], sel["height"] * sel["pixel"]
+        boxes = np.unique(np.stack([sel["origin_x"], sel["origin_y"] - h, sel["origin_x"] + w,
+                                    sel["origin_y"]], axis=1), axis=0)
+        pad = np.full(len(boxes), (sp + 3.0) / unit)
+        if sphere:
+            lat = np.maximum(np.abs(boxes[:, 1]), np.abs(boxes[:, 3]))
+            pad_x = pad / np.cos(np.radians(np.minimum(lat + pad, 89.0)))
+        else:
+            pad_x = pad
+        boxes = boxes + np.stack([-pad_x, -pad, pad_x, pad], axis=1)
+        placed = N.place_boulders_boxes(boxes, float(sp), coefficient, exponent, min_diameter,
+                                        max_diameter, seed=seed,
+                                        sphere_radius=stream.radius if sphere else 0.0,
+                                        threads=threads)
+        stats[f"{sp:g} m"] = {"tiles": int(len(boxes)), "boulders": int(placed.size)}
+        parts.append(placed)
+    placed = np.concatenate(parts) if parts else np.zeros(0, N.PLACED_BOULDER_DTYPE)
+    _, first = np.unique(placed["id"], return_index=True)
+    placed = placed[np.sort(first)]
+    return N.relief_boulders(placed), stats
+
+
 def unit_vector(lon, lat):
     """The unit vector(s) at longitude and latitude in degrees, as (x, y, z)."""
     lam, phi = np.radians(lon), np.radians(lat)

@@ -222,3 +222,19 @@ def test_strip_pass_with_boulders_equals_two_passes(hip):
           f"{int((two > bare).sum())} cells raised", flush=True)
     assert (two > bare).sum() > 1000
     assert np.array_equal(one.view(np.uint32), two.view(np.uint32))
+    # The same grid through the batched kernel as one tile: the same crater list (generated
+    # over the grid's box at its spacing) and boulders, the same bits.
+    box = (grid.origin_x, grid.origin_y - grid.height * grid.pixel,
+           grid.origin_x + grid.width * grid.pixel, grid.origin_y)
+    batch = TileBatch(hip, [C2_FRACTAL], spec, region=box, finest_pixel=grid.pixel,
+                      boulders=N.relief_boulders(placed))
+    batch.build()
+    try:
+        tile = np.zeros(1, N.TILE_DTYPE)
+        tile[0] = (grid.origin_x, grid.origin_y, grid.pixel, grid.width, grid.height)
+        batch.stage(tile)
+        assert batch.records.size == terrain.crater_count
+        got = run(hip, batch).reshape(grid.height, grid.width)
+    finally:
+        batch.close()
+    assert np.array_equal(got.view(np.uint32), two.view(np.uint32))

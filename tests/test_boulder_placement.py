@@ -90,3 +90,39 @@ def test_sizes_and_poses_follow_their_laws():
     assert got["yaw"].min() >= 0.0 and got["yaw"].max() < 2.0 * np.pi
     assert got["lean"].max() < 0.26
     assert np.unique(got["id"]).size == got.size
+
+
+@pytest.mark.parametrize("radius, boxes", [
+    (1737400.0, [(20.0, 5.0, 20.004, 5.004), (20.003, 5.003, 20.008, 5.006),
+                 (-60.0, 40.0, -59.997, 40.002)]),
+    (0.0, [(0.0, 0.0, 40.0, 30.0), (35.0, 20.0, 70.0, 60.0), (-100.0, 5.0, -80.0, 9.0)]),
+])
+def test_boxes_list_the_union_of_their_regions(radius, boxes):
+    """tgx_place_boulders_boxes lists each boulder of every region once, with the draws it has
+    in that region's own list (a served batch's catalogue is the union of its tiles' lists)."""
+    args = (0.25, 3.8e-3, 3.0, 0.2, 3.0)
+    got = N.place_boulders_boxes(np.array(boxes), *args, seed=5, sphere_radius=radius)
+    assert np.unique(got["id"]).size == got.size
+    by_id = {int(b["id"]): b for b in got}
+    union = set()
+    for box in boxes:
+        alone = N.place_boulders(box, *args, seed=5, sphere_radius=radius)
+        assert alone.size > 0
+        for b in alone:
+            mine = by_id[int(b["id"])]
+            assert mine.tobytes() == b.tobytes()
+            union.add(int(b["id"]))
+    assert union == set(by_id)
+
+
+def test_coarser_spacing_lists_the_resolved_subset():
+    """A boulder a coarser spacing resolves (D above twice it) is listed at every finer spacing
+    with the same draws, so one catalogue at the finest spacing serves tiles of every spacing."""
+    box, radius = (20.0, 5.0, 20.01, 5.01), 1737400.0
+    fine = N.place_boulders(box, 0.125, 3.8e-3, 3.0, 0.2, 3.0, seed=5, sphere_radius=radius)
+    for spacing in (0.25, 0.5, 1.0):
+        coarse = N.place_boulders(box, spacing, 3.8e-3, 3.0, 0.2, 3.0, seed=5,
+                                  sphere_radius=radius)
+        a = fine[fine["diameter"] > 2.0 * spacing]
+        b = coarse[coarse["diameter"] > 2.0 * spacing]
+        assert a.size > 0 and a.tobytes() == b.tobytes()
