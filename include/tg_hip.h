@@ -158,6 +158,19 @@ int tg_hip_crater_boulder_launch(tg_dev* dev, const tg_crater_set* set,
                                  const tg_kernel_params* params, void* stream, char* err,
                                  size_t err_len);
 
+/* ---- Sources with posts ---------------------------------------------------------------------
+ * Upsampler::Fill of a KernelUpsampler (src/sources/upsampling.cpp:297-335), which the reference
+ * runs on the CPU for every source that holds posts (SyntheticSource with a pixel size,
+ * src/sources/synthetic_source.cpp:36-53,87-95; GeoTiffSource; RasterSource): fills `heights_dev`
+ * (spec->width x spec->height floats, row by row) from the posts in device memory
+ * (spec->window_width x spec->window_height floats, NaN where a post has no data), on `stream`.
+ * Each cell is the kernel's FP64 sum over its taps narrowed to FP32, NaN where a tap of non-zero
+ * weight has no data; with spec->at_post_spacing a cell centred on a post takes the post. The
+ * heights are the reference's bit for bit for the nearest, bilinear, cubic and B-spline kernels;
+ * Lanczos's weights take the device's sine, within an ulp of the C library's. */
+int tg_hip_upsample(tg_dev* dev, const tg_upsample* spec, const float* posts_dev,
+                    float* heights_dev, void* stream, char* err, size_t err_len);
+
 /* ---- Batched tiles ---------------------------------------------------------------------------
  * Many tiles of uneven size and pixel size evaluated in one persistent launch: the source fill
  * (SyntheticSource noise, or a constant) and crater shaping fused per 16 x 16 bin of cells, bins

@@ -217,6 +217,38 @@ typedef struct tg_boulder_kernel {
     uint32_t pad[2];
 } tg_boulder_kernel;
 
+/* ResampleMethod (include/terraingen/raster/resample.h:12-23), the kernel upsamplers' kernels. */
+enum {
+    TG_RESAMPLE_NEAREST = 0,
+    TG_RESAMPLE_BILINEAR = 1,
+    TG_RESAMPLE_CUBIC = 2,
+    TG_RESAMPLE_LANCZOS3 = 3,
+    TG_RESAMPLE_CUBIC_BSPLINE = 4
+};
+
+/* One KernelUpsampler fill of a north-up grid from a lattice of posts (the reference's
+ * Upsampler::Fill over a PostLattice, src/sources/upsampling.cpp:297-312; no GPU layout there,
+ * which upsamples on the CPU). The lattice is north-up from (lattice_x, lattice_y) with posts
+ * `post` apart: post (col, row) sits at the centre of that cell. The posts held are a window of
+ * window_width x window_height from lattice (col0, row0), row by row; reads beyond it repeat its
+ * nearest edge post (RasterPosts, detail::PostIndex), after wrapping columns into
+ * [0, lattice_columns) when `wrap_columns`. The grid is width x height cells of `pixel` from its
+ * north-west corner (origin_x, origin_y). With `at_post_spacing` (the sample spacing is no finer
+ * than the posts, AtPostSpacing) a cell centred on a post takes the post. */
+typedef struct tg_upsample {
+    double lattice_x, lattice_y;
+    double post;
+    int64_t col0, row0;
+    uint32_t window_width, window_height;
+    uint32_t wrap_columns;
+    uint32_t lattice_columns;
+    double origin_x, origin_y;
+    double pixel;
+    uint32_t width, height;
+    uint32_t method;
+    uint32_t at_post_spacing;
+} tg_upsample;
+
 /* No-data modes of tg_kernel_params.no_data (src/gpu/kernel.cpp:22-27). */
 enum { TG_NODATA_NONE = 0, TG_NODATA_NAN = 1, TG_NODATA_VALUE = 2 };
 
@@ -244,6 +276,7 @@ static_assert(sizeof(tg_crater_body_datum_kernel) == 188,
 static_assert(sizeof(tg_kernel_params) == 20, "tg_kernel_params must be 20 bytes");
 static_assert(sizeof(tg_boulder) == 80, "tg_boulder is 80 bytes");
 static_assert(sizeof(tg_boulder_kernel) == 16, "tg_boulder_kernel is 16 bytes");
+static_assert(sizeof(tg_upsample) == 96, "tg_upsample is 96 bytes");
 #endif
 
 #endif /* TG_LAYOUTS_H */

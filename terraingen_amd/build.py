@@ -47,7 +47,8 @@ def build_hip() -> Path:
     linked into the shared library."""
     from concurrent.futures import ThreadPoolExecutor
 
-    sources = [CSRC / "tg_hip.hip", CSRC / "batch.hip", CSRC / "boulders.hip", CSRC / "probe.hip"]
+    sources = [CSRC / "tg_hip.hip", CSRC / "batch.hip", CSRC / "boulders.hip", CSRC / "probe.hip",
+               CSRC / "upsample.hip"]
     objdir = PKG / "_build"
     objdir.mkdir(exist_ok=True)
     compile_flags = [f for f in HIP_FLAGS if f != "-shared"]

@@ -107,6 +107,18 @@ NODATA_NONE, NODATA_NAN, NODATA_VALUE = 0, 1, 2
 
 # Every entry point include/tg_hip.h declares, with its ctypes signature.
 VP, SZ, CP = C.c_void_p, C.c_size_t, C.c_char_p
+Upsample = _struct("Upsample", [
+    ("lattice_x", C.c_double), ("lattice_y", C.c_double), ("post", C.c_double),
+    ("col0", C.c_int64), ("row0", C.c_int64), ("window_width", C.c_uint32),
+    ("window_height", C.c_uint32), ("wrap_columns", C.c_uint32),
+    ("lattice_columns", C.c_uint32), ("origin_x", C.c_double), ("origin_y", C.c_double),
+    ("pixel", C.c_double), ("width", C.c_uint32), ("height", C.c_uint32),
+    ("method", C.c_uint32), ("at_post_spacing", C.c_uint32)])
+assert C.sizeof(Upsample) == 96
+# ResampleMethod (include/terraingen/raster/resample.h:12-23).
+(TG_RESAMPLE_NEAREST, TG_RESAMPLE_BILINEAR, TG_RESAMPLE_CUBIC, TG_RESAMPLE_LANCZOS3,
+ TG_RESAMPLE_CUBIC_BSPLINE) = range(5)
+
 HIP_ABI = {
     "tg_hip_open": (C.c_int, [C.c_int, C.POINTER(VP), CP, SZ]),
     "tg_hip_device_name": (CP, [VP]),
@@ -163,6 +175,7 @@ HIP_ABI = {
                                           C.c_double, VP, SZ, VP, CP, SZ]),
     "tg_hip_catalogue_frames": (C.c_int, [VP, VP, SZ, CP, SZ]),
     "tg_hip_boulder_catalogue_create": (C.c_int, [VP, VP, SZ, VP, VP, VP, CP, SZ]),
+    "tg_hip_upsample": (C.c_int, [VP, C.POINTER(Upsample), VP, VP, VP, CP, SZ]),
     "tg_hip_boulder_catalogue_free": (C.c_int, [VP]),
     "tg_hip_batch_stage_boulders": (C.c_int, [VP, VP, VP, SZ, VP, SZ, C.c_int, CP, SZ]),
     "tg_hip_fill": (C.c_int, [VP, VP, C.c_float, SZ, VP, CP, SZ]),
