@@ -320,6 +320,44 @@ int tgx_sphere_pair_ops(const tg_tile_desc* descs, size_t tile_count, const tg_b
                         uint64_t* listed, uint64_t* inside, double* ops, char* err,
                         size_t err_len);
 
+/* ---- Reading a DEM: one band of a GeoTIFF ----------------------------------------------------
+ * What GeoTiffSource reads of a file (src/sources/geotiff_source.cpp, src/raster/geotiff/
+ * geotiff_read.cpp): the image's layout and georeferencing, and windows of one band decoded to
+ * floats, for its posts on the device (tg_hip_upsample). TIFF and BigTIFF in either byte order,
+ * strips or tiles, interleaved or planar samples of 8-64 bits (unsigned, signed, float), no
+ * compression, LZW, PackBits or Deflate, horizontal and floating-point predictors. */
+typedef struct tgx_geotiff_file tgx_geotiff_file;
+
+typedef struct tgx_geotiff_info {
+    uint32_t width, height, bands;
+    uint32_t bits_per_sample;
+    uint32_t sample_format;  /* 1 unsigned, 2 signed, 3 float */
+    uint32_t compression;    /* 1 none, 5 LZW, 8 or 32946 Deflate, 32773 PackBits */
+    uint32_t predictor;      /* 1 none, 2 horizontal, 3 floating point */
+    uint32_t tiled, chunk_width, chunk_height;
+    uint32_t big_tiff, big_endian;
+    uint32_t georeferenced;  /* the model tags give a transform */
+    uint32_t geographic;     /* GTModelTypeGeoKey: longitude and latitude */
+    uint32_t pixel_is_point; /* GTRasterTypeGeoKey: PixelIsPoint (the transform is moved back) */
+    uint32_t has_no_data;
+    double no_data;          /* GDAL's no-data tag, when has_no_data */
+    double transform[6];     /* GDAL order: origin x, pixel width, row rotation, origin y, column
+                              * rotation, pixel height */
+} tgx_geotiff_info;
+
+int tgx_geotiff_open(const char* path, tgx_geotiff_file** file, char* err, size_t err_len);
+/* The file's info and band `band`'s scale and offset (GDAL metadata; 1 and 0 without). */
+int tgx_geotiff_info_get(const tgx_geotiff_file* file, uint32_t band, tgx_geotiff_info* info,
+                         double* scale, double* offset);
+/* Band `band`'s cells [col0, col0 + width) x [row0, row0 + height) into `out` row by row: NaN
+ * where a sample is NaN or the no-data value, else float(double(sample) * scale + offset) (the
+ * sample as is when scale is 1 and offset 0), as GeoTiffSource's posts decode them
+ * (geotiff_source.cpp:115-128). Decodes only the chunks the window meets. */
+int tgx_geotiff_read(const tgx_geotiff_file* file, uint32_t band, uint64_t col0, uint64_t row0,
+                     uint32_t width, uint32_t height, double scale, double offset, float* out,
+                     char* err, size_t err_len);
+void tgx_geotiff_close(tgx_geotiff_file* file);
+
 /* ---- The tile server: continuous batching onto the batched kernel ---------------------------
  * No counterpart in the reference, which evaluates one grid per call. Requests (a tile and the host
  * memory its heights go to) queue as they arrive; a dispatcher thread admits them, in arrival order,

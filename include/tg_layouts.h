@@ -228,16 +228,20 @@ enum {
 
 /* One KernelUpsampler fill of a north-up grid from a lattice of posts (the reference's
  * Upsampler::Fill over a PostLattice, src/sources/upsampling.cpp:297-312; no GPU layout there,
- * which upsamples on the CPU). The lattice is north-up from (lattice_x, lattice_y) with posts
- * `post` apart: post (col, row) sits at the centre of that cell. The posts held are a window of
+ * which upsamples on the CPU). The lattice is north-up from (lattice_x, lattice_y) with cells
+ * `post` wide and `post_height` high: post (col, row) sits at the centre of that cell. The posts held are a window of
  * window_width x window_height from lattice (col0, row0), row by row; reads beyond it repeat its
  * nearest edge post (RasterPosts, detail::PostIndex), after wrapping columns into
  * [0, lattice_columns) when `wrap_columns`. The grid is width x height cells of `pixel` from its
- * north-west corner (origin_x, origin_y). With `at_post_spacing` (the sample spacing is no finer
- * than the posts, AtPostSpacing) a cell centred on a post takes the post. */
+ * north-west corner (origin_x, origin_y), written `out_pitch` floats apart row to row (0: width).
+ * With `at_post_spacing` (the sample spacing is no finer than the posts, AtPostSpacing) a cell
+ * centred on a post takes the post. With `covered`, a cell whose centre lies outside the lattice
+ * cells [cover_col0, cover_col1) x [cover_row0, cover_row1) is NaN (RasterHeights::Covers,
+ * src/sources/raster_source.cpp:78-82). */
 typedef struct tg_upsample {
     double lattice_x, lattice_y;
-    double post;
+    double post;         /* the lattice's cell width */
+    double post_height;  /* and height (0: the width) */
     int64_t col0, row0;
     uint32_t window_width, window_height;
     uint32_t wrap_columns;
@@ -247,6 +251,9 @@ typedef struct tg_upsample {
     uint32_t width, height;
     uint32_t method;
     uint32_t at_post_spacing;
+    uint32_t out_pitch;
+    uint32_t covered;
+    int64_t cover_col0, cover_row0, cover_col1, cover_row1;
 } tg_upsample;
 
 /* No-data modes of tg_kernel_params.no_data (src/gpu/kernel.cpp:22-27). */
@@ -276,7 +283,7 @@ static_assert(sizeof(tg_crater_body_datum_kernel) == 188,
 static_assert(sizeof(tg_kernel_params) == 20, "tg_kernel_params must be 20 bytes");
 static_assert(sizeof(tg_boulder) == 80, "tg_boulder is 80 bytes");
 static_assert(sizeof(tg_boulder_kernel) == 16, "tg_boulder_kernel is 16 bytes");
-static_assert(sizeof(tg_upsample) == 96, "tg_upsample is 96 bytes");
+static_assert(sizeof(tg_upsample) == 144, "tg_upsample is 144 bytes");
 #endif
 
 #endif /* TG_LAYOUTS_H */

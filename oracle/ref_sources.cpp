@@ -8,6 +8,7 @@
 
 #include "terraingen/noise/fractals.h"
 #include "terraingen/pipeline/terrain.h"
+#include "terraingen/raster/geotiff/geotiff.h"
 #include "terraingen/sources/geotiff_source.h"
 #include "terraingen/sources/synthetic_source.h"
 #include "terraingen/sources/upsampling.h"
@@ -44,8 +45,10 @@ Fractal FromLayoutS(const tg_fractal& f) {
 }
 
 int Evaluate(std::shared_ptr<const Source> source, std::uint32_t width, std::uint32_t height,
-             double originX, double originY, double pixel, double spacing, float* heights) {
-    const Terrain terrain(Surface{}, std::move(source), std::vector<LayerPtr>{});
+             double originX, double originY, double pixel, double spacing, float* heights,
+             double radius = 0.0) {
+    const Terrain terrain(radius > 0.0 ? Surface::Sphere(radius) : Surface{}, std::move(source),
+                          std::vector<LayerPtr>{});
     const RasterGrid grid{width, height, GeoTransform::NorthUp(originX, originY, pixel)};
     EvaluationOptions options;
     options.On(Backend::Cpu).Threads(8);
@@ -87,10 +90,11 @@ int ref_synthetic_grid(const tg_fractal* fractals, std::uint32_t count, float el
 
 // A GeoTiffSource of the file at `path` (band 0), upsampled by KernelUpsampler(`method`), with an
 // optional crop (crop[4] = minX, minY, maxX, maxY; null for the whole file), evaluated on the grid
-// as ref_synthetic_grid evaluates.
+// as ref_synthetic_grid evaluates: on a plane, or on a sphere of `radius` meters when it is
+// positive (a file in longitude and latitude).
 int ref_geotiff_grid(const char* path, std::uint32_t method, const double* crop,
                      std::uint32_t width, std::uint32_t height, double originX, double originY,
-                     double pixel, double spacing, float* heights) {
+                     double pixel, double spacing, double radius, float* heights) {
     try {
         GeoTiffSourceOptions options;
         options.upsampler =
@@ -99,10 +103,40 @@ int ref_geotiff_grid(const char* path, std::uint32_t method, const double* crop,
             options.crop = Bounds2d{crop[0], crop[1], crop[2], crop[3]};
         }
         auto source = std::make_shared<GeoTiffSource>(path, options);
-        return Evaluate(source, width, height, originX, originY, pixel, spacing, heights);
+        return Evaluate(source, width, height, originX, originY, pixel, spacing, heights, radius);
     } catch (const std::exception& e) {
         gSourceError = e.what();
         return 1;
+    }
+}
+
+// The reference's ReadGeoTiff of band `band`: the raw samples as floats (band scaling not applied)
+// into `values` (width x height, when not null), the transform (GDAL order) and the no-data
+// value. Returns the cell count, or -1 on error.
+long ref_geotiff_read(const char* path, std::uint32_t band, float* values, double* transform,
+                      double* noData, int* hasNoData) {
+    try {
+        const GeoTiff tiff = ReadGeoTiff(path, band);
+        const Raster<float>& r = tiff.raster;
+        if (values != nullptr) {
+            std::memcpy(values, r.Values().data(), r.Size() * sizeof(float));
+        }
+        if (transform != nullptr) {
+            const GeoTransform& t = r.Transform();
+            const double g[6] = {t.originX, t.pixelWidth, t.rowRotation,
+                                 t.originY, t.colRotation, t.pixelHeight};
+            std::memcpy(transform, g, sizeof g);
+        }
+        if (hasNoData != nullptr) {
+            *hasNoData = r.NoData().has_value() ? 1 : 0;
+            if (noData != nullptr && r.NoData()) {
+                *noData = *r.NoData();
+            }
+        }
+        return long(r.Size());
+    } catch (const std::exception& e) {
+        gSourceError = e.what();
+        return -1;
     }
 }
 

@@ -8,6 +8,7 @@
 //   RasterPosts::Read / detail::PostIndex            src/sources/upsampling.cpp:267-283,
 //                                                    src/sources/lattice.h:126-134
 //   CenteredPost, AtPostSpacing, Upsampler::Fill     src/sources/upsampling.cpp:31-40,289-312
+//   RasterHeights::Covers                            src/sources/raster_source.cpp:78-82
 //   GeoTransform::PixelToWorld / WorldToPixel        include/terraingen/raster/geo_transform.h:
 //                                                    63-79
 // One cell at a time, each step as written there.
@@ -138,13 +139,19 @@ float Cell(const tg_upsample& u, const float* posts, std::size_t col, std::size_
     const double y = u.origin_y + (static_cast<double>(col) + 0.5) * rot +
                      (static_cast<double>(row) + 0.5) * ph;
     // WorldToPixel of the lattice's north-up transform.
-    const double lw = u.post, lh = -u.post;
+    const double lw = u.post, lh = -(u.post_height > 0.0 ? u.post_height : u.post);
     const double det = lw * lh - rot * rot;
     const double dx = x - u.lattice_x;
     const double dy = y - u.lattice_y;
     const double px = (dx * lh - dy * rot) / det;
     const double py = (dy * lw - dx * rot) / det;
     const double lu = px - 0.5, lv = py - 0.5;
+    constexpr float kNaN = std::numeric_limits<float>::quiet_NaN();
+    if (u.covered != 0 &&
+        !(px >= static_cast<double>(u.cover_col0) && py >= static_cast<double>(u.cover_row0) &&
+          px < static_cast<double>(u.cover_col1) && py < static_cast<double>(u.cover_row1))) {
+        return kNaN;  // RasterHeights::Covers
+    }
     if (u.at_post_spacing != 0) {
         const auto c = Whole(lu);
         const auto r = Whole(lv);
@@ -152,7 +159,6 @@ float Cell(const tg_upsample& u, const float* posts, std::size_t col, std::size_
             return Post(u, posts, *c, *r);
         }
     }
-    constexpr float kNaN = std::numeric_limits<float>::quiet_NaN();
     if (u.method == TG_RESAMPLE_NEAREST) {
         const float v = Post(u, posts, static_cast<std::int64_t>(std::round(lu)),
                              static_cast<std::int64_t>(std::round(lv)));
@@ -187,9 +193,10 @@ extern "C" {
 
 // The grid `spec` describes, filled from `posts` (the window it names) into `out`, row by row.
 void or_upsample_grid(const tg_upsample* spec, const float* posts, float* out) {
+    const std::size_t pitch = spec->out_pitch != 0 ? spec->out_pitch : spec->width;
     for (std::size_t row = 0; row < spec->height; ++row) {
         for (std::size_t col = 0; col < spec->width; ++col) {
-            out[row * spec->width + col] = Cell(*spec, posts, col, row);
+            out[row * pitch + col] = Cell(*spec, posts, col, row);
         }
     }
 }

@@ -141,13 +141,23 @@ __global__ void __launch_bounds__(kSide * kSide)
     const double zero = 0.0;
     const double x = u.origin_x + (double(col) + 0.5) * u.pixel + (double(row) + 0.5) * zero;
     const double y = u.origin_y + (double(col) + 0.5) * zero + (double(row) + 0.5) * -u.pixel;
-    const double pw = u.post, ph = -u.post;
+    const double pw = u.post, ph = -(u.post_height > 0.0 ? u.post_height : u.post);
     const double det = pw * ph - zero * zero;
     const double dx = x - u.lattice_x;
     const double dy = y - u.lattice_y;
-    const double lu = (dx * ph - dy * zero) / det - 0.5;
-    const double lv = (dy * pw - dx * zero) / det - 0.5;
-    float* cell = out + size_t(row) * u.width + col;
+    const double px = (dx * ph - dy * zero) / det;
+    const double py = (dy * pw - dx * zero) / det;
+    const double lu = px - 0.5;
+    const double lv = py - 0.5;
+    float* cell = out + size_t(row) * (u.out_pitch != 0u ? u.out_pitch : u.width) + col;
+    if (u.covered != 0u) {
+        // WorldToPixel of the centre against the covered cells.
+        if (!(px >= double(u.cover_col0) && py >= double(u.cover_row0) &&
+              px < double(u.cover_col1) && py < double(u.cover_row1))) {
+            *cell = __builtin_nanf("");
+            return;
+        }
+    }
     if (u.at_post_spacing != 0u) {
         int64_t pc, pr;
         if (NearWhole(lu, pc) && NearWhole(lv, pr)) {
@@ -189,13 +199,17 @@ __global__ void __launch_bounds__(kSide * kSide)
 namespace tgi {
 
 void ValidateUpsample(const tg_upsample& u) {
-    if (!(u.post > 0.0) || !std::isfinite(u.post) || !(u.pixel > 0.0) || !std::isfinite(u.pixel) ||
+    if (!(u.post > 0.0) || !std::isfinite(u.post) || !(u.post_height >= 0.0) ||
+        !std::isfinite(u.post_height) || !(u.pixel > 0.0) || !std::isfinite(u.pixel) ||
         !std::isfinite(u.lattice_x) || !std::isfinite(u.lattice_y) ||
         !std::isfinite(u.origin_x) || !std::isfinite(u.origin_y)) {
         throw HipError{"upsampling: the lattice and grid need finite origins and positive sizes"};
     }
     if (u.window_width == 0 || u.window_height == 0) {
         throw HipError{"upsampling: the posts window is empty"};
+    }
+    if (u.out_pitch != 0u && u.out_pitch < u.width) {
+        throw HipError{"upsampling: the output's row pitch is below the grid's width"};
     }
     if (u.method > TG_RESAMPLE_CUBIC_BSPLINE) {
         throw HipError{"upsampling: unknown resample method"};

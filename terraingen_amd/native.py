@@ -109,12 +109,14 @@ NODATA_NONE, NODATA_NAN, NODATA_VALUE = 0, 1, 2
 VP, SZ, CP = C.c_void_p, C.c_size_t, C.c_char_p
 Upsample = _struct("Upsample", [
     ("lattice_x", C.c_double), ("lattice_y", C.c_double), ("post", C.c_double),
-    ("col0", C.c_int64), ("row0", C.c_int64), ("window_width", C.c_uint32),
+    ("post_height", C.c_double), ("col0", C.c_int64), ("row0", C.c_int64), ("window_width", C.c_uint32),
     ("window_height", C.c_uint32), ("wrap_columns", C.c_uint32),
     ("lattice_columns", C.c_uint32), ("origin_x", C.c_double), ("origin_y", C.c_double),
     ("pixel", C.c_double), ("width", C.c_uint32), ("height", C.c_uint32),
-    ("method", C.c_uint32), ("at_post_spacing", C.c_uint32)])
-assert C.sizeof(Upsample) == 96
+    ("method", C.c_uint32), ("at_post_spacing", C.c_uint32), ("out_pitch", C.c_uint32),
+    ("covered", C.c_uint32), ("cover_col0", C.c_int64), ("cover_row0", C.c_int64),
+    ("cover_col1", C.c_int64), ("cover_row1", C.c_int64)])
+assert C.sizeof(Upsample) == 144
 # ResampleMethod (include/terraingen/raster/resample.h:12-23).
 (TG_RESAMPLE_NEAREST, TG_RESAMPLE_BILINEAR, TG_RESAMPLE_CUBIC, TG_RESAMPLE_LANCZOS3,
  TG_RESAMPLE_CUBIC_BSPLINE) = range(5)
@@ -1045,6 +1047,53 @@ def place_boulders_boxes(boxes: np.ndarray, spacing: float, sfd_coefficient: flo
         return _owned_copy(ptr, n.value * 80).view(PLACED_BOULDER_DTYPE)
     finally:
         lib.tgx_free(ptr)
+
+
+GeoTiffInfo = _struct("GeoTiffInfo", [(n, u32) for n in (
+    "width", "height", "bands", "bits_per_sample", "sample_format", "compression", "predictor",
+    "tiled", "chunk_width", "chunk_height", "big_tiff", "big_endian", "georeferenced",
+    "geographic", "pixel_is_point", "has_no_data")] + [("no_data", C.c_double),
+                                                       ("transform", C.c_double * 6)])
+assert C.sizeof(GeoTiffInfo) == 120
+HOST_ABI["tgx_geotiff_open"] = (C.c_int, [C.c_char_p, C.POINTER(VP), CP, SZ])
+HOST_ABI["tgx_geotiff_info_get"] = (C.c_int, [VP, u32, C.POINTER(GeoTiffInfo),
+                                              C.POINTER(C.c_double), C.POINTER(C.c_double)])
+HOST_ABI["tgx_geotiff_read"] = (C.c_int, [VP, u32, C.c_uint64, C.c_uint64, u32, u32, C.c_double,
+                                          C.c_double, VP, CP, SZ])
+HOST_ABI["tgx_geotiff_close"] = (None, [VP])
+
+
+class GeoTiffFile:
+    """``tgx_geotiff_*``: a GeoTIFF's layout and windows of one band as floats."""
+
+    def __init__(self, path):
+        lib = host_library()
+        h = VP()
+        err = C.create_string_buffer(512)
+        _check(lib.tgx_geotiff_open(str(path).encode(), C.byref(h), err, 512), err)
+        self.handle = h.value
+
+    def info(self, band: int = 0):
+        """(GeoTiffInfo, the band's scale, its offset)."""
+        info = GeoTiffInfo()
+        scale, offset = C.c_double(), C.c_double()
+        if host_library().tgx_geotiff_info_get(self.handle, band, C.byref(info), C.byref(scale),
+                                               C.byref(offset)):
+            raise ValueError(f"band {band} is missing")
+        return info, scale.value, offset.value
+
+    def read(self, band: int, col0: int, row0: int, width: int, height: int,
+             scale: float = 1.0, offset: float = 0.0) -> np.ndarray:
+        out = np.zeros((height, width), np.float32)
+        err = C.create_string_buffer(512)
+        _check(host_library().tgx_geotiff_read(self.handle, band, col0, row0, width, height,
+                                                scale, offset, out.ctypes.data, err, 512), err)
+        return out
+
+    def __del__(self):
+        if getattr(self, "handle", None):
+            host_library().tgx_geotiff_close(self.handle)
+            self.handle = None
 
 
 def relief_boulders(placed: np.ndarray) -> np.ndarray:

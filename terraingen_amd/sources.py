@@ -33,25 +33,28 @@ METHODS = {"nearest": N.TG_RESAMPLE_NEAREST, "bilinear": N.TG_RESAMPLE_BILINEAR,
 SPACING_TOLERANCE = 1e-9
 
 
-def lattice_positions(grid: N.Grid, lattice_x: float, lattice_y: float, post: float):
+def lattice_positions(grid: N.Grid, lattice_x: float, lattice_y: float, post: float,
+                      post_height: float = 0.0):
     """The lattice position (in posts, from post (0, 0)) of each column's and each row's cell
     centres: GeoTransform::WorldToPixel of the lattice less half a post, in the kernel's FP64
     operations."""
     zero = 0.0
+    ph = -(post_height if post_height > 0.0 else post)
     cols = np.arange(grid.width, dtype=np.float64)
     rows = np.arange(grid.height, dtype=np.float64)
     x = grid.origin_x + (cols + 0.5) * grid.pixel + 0.5 * zero
     y = grid.origin_y + 0.5 * zero + (rows + 0.5) * -grid.pixel
-    det = post * -post - zero * zero
-    u = ((x - lattice_x) * -post - zero) / det - 0.5
+    det = post * ph - zero * zero
+    u = ((x - lattice_x) * ph - zero) / det - 0.5
     v = ((y - lattice_y) * post - zero) / det - 0.5
     return u, v
 
 
-def tap_window(grid: N.Grid, lattice_x: float, lattice_y: float, post: float, method: int):
+def tap_window(grid: N.Grid, lattice_x: float, lattice_y: float, post: float, method: int,
+               post_height: float = 0.0):
     """The posts a fill of `grid` reads: (col0, row0, width, height) of the lattice, one post to
     spare each side of the kernel's taps."""
-    u, v = lattice_positions(grid, lattice_x, lattice_y, post)
+    u, v = lattice_positions(grid, lattice_x, lattice_y, post, post_height)
     r = RADIUS[method]
     c0 = int(np.floor(u.min())) - r - 1
     c1 = int(np.floor(u.max())) + r + 1
@@ -60,17 +63,24 @@ def tap_window(grid: N.Grid, lattice_x: float, lattice_y: float, post: float, me
     return c0, r0, c1 - c0 + 1, r1 - r0 + 1
 
 
-def at_post_spacing(post: float, spacing: float) -> bool:
+def at_post_spacing(post: float, spacing: float, post_height: float = 0.0) -> bool:
     """AtPostSpacing (upsampling.cpp:289-294) of a north-up lattice."""
-    return spacing >= abs(post * -post) ** 0.5 * (1.0 - SPACING_TOLERANCE)
+    ph = -(post_height if post_height > 0.0 else post)
+    return spacing >= abs(post * ph - 0.0 * 0.0) ** 0.5 * (1.0 - SPACING_TOLERANCE)
 
 
 def upsample_spec(grid: N.Grid, lattice_x: float, lattice_y: float, post: float, window,
-                  method: int, spacing: float, wrap_columns: int = 0) -> N.Upsample:
+                  method: int, spacing: float, wrap_columns: int = 0, post_height: float = 0.0,
+                  covered=None, out_pitch: int = 0) -> N.Upsample:
+    """The tg_upsample of a fill of `grid` from the posts `window` (col0, row0, width, height)
+    of a lattice; `covered` (col0, row0, col1, row1) marks the lattice cells with heights."""
     c0, r0, w, h = window
-    return N.Upsample(lattice_x, lattice_y, post, c0, r0, w, h, 1 if wrap_columns else 0,
-                      wrap_columns, grid.origin_x, grid.origin_y, grid.pixel, grid.width,
-                      grid.height, method, 1 if at_post_spacing(post, spacing) else 0)
+    cover = covered if covered is not None else (0, 0, 0, 0)
+    return N.Upsample(lattice_x, lattice_y, post, post_height, c0, r0, w, h,
+                      1 if wrap_columns else 0, wrap_columns, grid.origin_x, grid.origin_y,
+                      grid.pixel, grid.width, grid.height, method,
+                      1 if at_post_spacing(post, spacing, post_height) else 0, out_pitch,
+                      1 if covered is not None else 0, *cover)
 
 
 @dataclass
@@ -167,3 +177,106 @@ class RasterPosts:
                       out.ptr, stream)
         self.dev.synchronize(stream)
         return spec
+
+
+TURN_DEGREES = 360.0
+
+
+class GeoTiffPosts:
+    """A GeoTiffSource on the device (src/sources/geotiff_source.cpp): one band of a GeoTIFF
+    read on the host (tgx_geotiff_read: decoded, scaled, NaN where there is no data), the window
+    of it the source's cells need held on the device, and grids filled from it by
+    tg_hip_upsample as RasterHeights fills them (src/sources/raster_source.cpp:78-179): NaN
+    where a cell's centre lies outside the cells covered (the whole file, or those that meet
+    `crop`), a raster in longitude and latitude spanning 360 degrees wrapping in longitude."""
+
+    def __init__(self, dev: N.HipDevice | None, path, band: int = 0,
+                 method: int = N.TG_RESAMPLE_CUBIC, crop=None, scale=None, offset=None):
+        self.dev = dev
+        self.method = method
+        self.file = N.GeoTiffFile(path)
+        info, file_scale, file_offset = self.file.info(band)
+        self.info = info
+        t = list(info.transform)  # GDAL order: originX, pixelWidth, rowRot, originY, colRot, ph
+        if t[2] != 0.0 or t[4] != 0.0 or not t[1] > 0.0 or not t[5] < 0.0:
+            raise ValueError("the file's raster is not north-up")
+        self.origin_x, self.post, self.origin_y, self.post_height = t[0], t[1], t[3], -t[5]
+        self.scale = file_scale if scale is None else scale
+        self.offset = file_offset if offset is None else offset
+        self.no_data = info.no_data if info.has_no_data else None
+        w, h = info.width, info.height
+        self.cells = self.covered_cells(crop)
+        # NativeWindow (raster_source.cpp:58-66): the cells and the kernel's reach around them.
+        margin = RADIUS[method] + 1
+        c0, c1, r0, r1 = (max(0, self.cells[0] - margin), min(w, self.cells[2] + margin),
+                          max(0, self.cells[1] - margin), min(h, self.cells[3] + margin))
+        # WrapsLongitude (raster_source.cpp:45-50).
+        self.wraps = bool(info.geographic) and abs(w * self.post - TURN_DEGREES) <= \
+            1e-9 * TURN_DEGREES
+        if self.wraps:
+            c0, c1 = 0, w
+        self.window = (c0, r0, c1 - c0, r1 - r0)
+        self.values = self.file.read(band, c0, r0, c1 - c0, r1 - r0, self.scale, self.offset)
+        self.d_posts = N.DeviceArray.from_host(dev, self.values) if dev is not None else None
+
+    def covered_cells(self, crop):
+        """(col0, row0, col1, row1): the file's cells that meet `crop` (PixelWindowIn of the box's
+        corners, cell_loops.cpp:21-28), or all of them."""
+        w, h = self.info.width, self.info.height
+        if crop is None:
+            return (0, 0, w, h)
+        x0, y0, x1, y1 = crop
+        if not (x0 < x1 and y0 < y1) or not np.all(np.isfinite(crop)):
+            raise ValueError("the crop must be finite with min < max")
+        ph = -self.post_height
+        det = self.post * ph - 0.0 * 0.0
+        us = [((x - self.origin_x) * ph - 0.0) / det for x in (x0, x1)]
+        vs = [((y - self.origin_y) * self.post - 0.0) / det for y in (y0, y1)]
+        clamp = lambda v, n: int(min(max(v, 0.0), float(n)))  # noqa: E731
+        cells = (clamp(np.floor(min(us)), w), clamp(np.floor(min(vs)), h),
+                 clamp(np.ceil(max(us)), w), clamp(np.ceil(max(vs)), h))
+        if cells[0] >= cells[2] or cells[1] >= cells[3]:
+            raise ValueError("the crop misses the file")
+        return cells
+
+    def spec(self, grid: N.Grid, spacing: float, out_pitch: int = 0) -> N.Upsample:
+        return upsample_spec(grid, self.origin_x, self.origin_y, self.post, self.window,
+                             self.method, spacing, self.info.width if self.wraps else 0,
+                             self.post_height, self.cells, out_pitch)
+
+    def runs(self, grid: N.Grid):
+        """The grid's runs of columns a whole number of turns of longitude apart from the
+        lattice (RasterHeights::FillGrid, raster_source.cpp:154-179): (first, end, grid of the
+        run moved onto the lattice's longitudes)."""
+        if not self.wraps:
+            return [(0, grid.width, grid)]
+        left = min(self.origin_x, self.origin_x + self.info.width * self.post)
+        cx = grid.origin_x + (np.arange(grid.width) + 0.5) * grid.pixel + 0.5 * 0.0
+        turns = np.floor((cx - left) / TURN_DEGREES)
+        if turns[0] == 0.0 and turns[-1] == 0.0:
+            return [(0, grid.width, grid)]
+        out, begin = [], 0
+        while begin < grid.width:
+            end = begin + 1
+            while end < grid.width and turns[end] == turns[begin]:
+                end += 1
+            origin = grid.origin_x + begin * grid.pixel - turns[begin] * TURN_DEGREES
+            out.append((begin, end, N.Grid(end - begin, grid.height, origin, grid.origin_y,
+                                           grid.pixel)))
+            begin = end
+        return out
+
+    def specs(self, grid: N.Grid, spacing: float = 0.0):
+        """(first column, tg_upsample) of each run of the grid's columns: what a fill of `grid` at
+        `spacing` (in the lattice's units; 0: the grid's) launches."""
+        spacing = max(spacing, grid.pixel)
+        return [(begin, self.spec(part, spacing, grid.width))
+                for begin, end, part in self.runs(grid)]
+
+    def fill(self, grid: N.Grid, out: N.DeviceArray, spacing: float = 0.0, stream=None):
+        """Source::Fill of `grid` (width x height floats into `out`) at `spacing` (0: the
+        grid's): NaN where the source has no height."""
+        for begin, spec in self.specs(grid, spacing):
+            self.dev.call("tg_hip_upsample", self.dev.handle, C.byref(spec), self.d_posts.ptr,
+                          out.ptr + 4 * begin, stream)
+        self.dev.synchronize(stream)

@@ -81,3 +81,82 @@ def reference_heights(ref, name: str) -> np.ndarray:
 
 def golden_heights(name: str) -> np.ndarray:
     return np.load(GOLDEN / "sources_synthetic.npz")[name]
+
+
+# GeoTiffSource cases: (file of tiff_craft.cases(), method, crop or None, grid (width, height,
+# origin_x, origin_y, pixel), spacing). Grids finer than the cells, coarser, cropped, across the
+# file's edges, and across the antimeridian of a file spanning every longitude.
+GEOTIFF_CASES = {
+    "int16_cubic_finer": ("int16_be_strips_deflate_pred2", N.TG_RESAMPLE_CUBIC, None,
+                          (90, 70, 495.3, 803.1, 0.75), 0.0),
+    "int16_bilinear_cropped": ("int16_be_strips_deflate_pred2", N.TG_RESAMPLE_BILINEAR,
+                               (530.0, 760.0, 580.0, 790.0), (80, 60, 520.1, 795.7, 1.1), 0.0),
+    "uint8_nearest_coarser": ("uint8_tiles_packbits", N.TG_RESAMPLE_NEAREST, None,
+                              (30, 25, 503.0, 797.0, 3.3), 0.0),
+    "float64_bspline_point": ("float64_planar_bigtiff_point", N.TG_RESAMPLE_CUBIC_BSPLINE, None,
+                              (70, 50, -121.0, 45.3, 0.1), 0.0),
+    "float32_lanczos_on_lattice": ("float32_be_pred3_deflate", N.TG_RESAMPLE_LANCZOS3, None,
+                                   (40, 30, 510.0, 790.0, 2.0), 0.0),
+    "int32_cubic_antimeridian": ("int32_geographic_wrap", N.TG_RESAMPLE_CUBIC, None,
+                                 (100, 60, 160.3, 70.2, 0.55), 0.0),
+}
+
+
+# Files in longitude and latitude, evaluated on a sphere of the Moon's radius.
+GEOGRAPHIC = {"int32_geographic_wrap": True}
+MOON_RADIUS = 1737400.0
+
+
+def geotiff_path(tmp_dir, file_case: str):
+    """The case's file written into `tmp_dir` (the same bytes every time)."""
+    import tiff_craft
+
+    path = Path(tmp_dir) / f"{file_case}.tif"
+    if not path.exists():
+        tiff_craft.write(path, tiff_craft.cases()[file_case])
+    return path
+
+
+def reference_geotiff_heights(ref, name: str, tmp_dir) -> np.ndarray:
+    file_case, method, crop, (w, h, ox, oy, pixel), spacing = GEOTIFF_CASES[name]
+    path = geotiff_path(tmp_dir, file_case)
+    box = np.asarray(crop, np.float64) if crop is not None else None
+    out = np.zeros((h, w), np.float32)
+    radius = MOON_RADIUS if GEOGRAPHIC.get(file_case) else 0.0
+    code = ref.ref_geotiff_grid(str(path).encode(), method,
+                                box.ctypes.data if box is not None else None, w, h, ox, oy,
+                                pixel, spacing, radius, out.ctypes.data)
+    if code != 0:
+        raise RuntimeError(ref.ref_sources_error().decode())
+    return out
+
+
+def golden_geotiff_heights(name: str) -> np.ndarray:
+    return np.load(GOLDEN / "sources_geotiff.npz")[name]
+
+
+def marked(heights: np.ndarray, posts) -> np.ndarray:
+    """NaN cells as Terrain::Evaluate marks them (MarkMissing, src/sources/source.cpp:10-19): the
+    file's no-data value when its band is not scaled, NaN otherwise."""
+    if posts.no_data is not None and posts.scale == 1.0 and posts.offset == 0.0:
+        return np.where(np.isnan(heights), np.float32(posts.no_data), heights)
+    return heights
+
+
+def geotiff_posts(dev, name: str, tmp_dir):
+    from terraingen_amd.sources import GeoTiffPosts
+
+    file_case, method, crop, grid, spacing = GEOTIFF_CASES[name]
+    return GeoTiffPosts(dev, geotiff_path(tmp_dir, file_case), method=method, crop=crop)
+
+
+def oracle_geotiff_heights(orc, name: str, tmp_dir) -> np.ndarray:
+    """The case on the CPU: the posts this repository's reader reads, filled by the oracle's
+    upsampling run by run."""
+    posts = geotiff_posts(None, name, tmp_dir)
+    file_case, method, crop, (w, h, ox, oy, pixel), spacing = GEOTIFF_CASES[name]
+    out = np.zeros((h, w), np.float32)
+    for begin, spec in posts.specs(N.Grid(w, h, ox, oy, pixel), spacing):
+        orc.or_upsample_grid(C.byref(spec), posts.values.ctypes.data, out[:, begin:].ctypes.data
+                             if begin else out.ctypes.data)
+    return marked(out, posts)

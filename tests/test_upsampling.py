@@ -66,7 +66,7 @@ def test_nan_posts_spread_to_the_cells_that_weigh_them(orc):
     to no other (InterpolateLatticeWeights returns nothing on a missing tap)."""
     posts = np.arange(100, dtype=np.float32).reshape(10, 10)
     posts[5, 5] = np.nan
-    spec = N.Upsample(0.0, 10.0, 1.0, 0, 0, 10, 10, 0, 0, 0.0, 10.0, 0.25, 40, 40,
+    spec = N.Upsample(0.0, 10.0, 1.0, 0.0, 0, 0, 10, 10, 0, 0, 0.0, 10.0, 0.25, 40, 40,
                       N.TG_RESAMPLE_BILINEAR, 0)
     out = np.zeros((40, 40), np.float32)
     orc.or_upsample_grid(C.byref(spec), posts.ctypes.data, out.ctypes.data)
@@ -77,3 +77,63 @@ def test_nan_posts_spread_to_the_cells_that_weigh_them(orc):
     near = np.abs(u - 5.0) < 1.0
     want = np.argwhere(near[:, None] & near[None, :])
     assert np.array_equal(bad, want)
+
+
+# ---- GeoTIFF DEMs -------------------------------------------------------------------------------
+
+from source_cases import (GEOTIFF_CASES, golden_geotiff_heights,  # noqa: E402
+                          oracle_geotiff_heights, reference_geotiff_heights)
+
+
+def same_heights(a: np.ndarray, b: np.ndarray) -> bool:
+    return (np.array_equal(np.isnan(a), np.isnan(b)) and
+            np.array_equal(a[~np.isnan(a)].view(np.uint32), b[~np.isnan(b)].view(np.uint32)))
+
+
+@pytest.mark.parametrize("name", GEOTIFF_CASES)
+def test_geotiff_source_on_the_cpu_matches_the_reference(orc, name, tmp_path):
+    """This repository's reader and the oracle's upsampling against the reference's
+    GeoTiffSource (tests/golden/sources_geotiff.npz): same heights, same cells without data."""
+    got = oracle_geotiff_heights(orc, name, tmp_path)
+    want = golden_geotiff_heights(name)
+    assert same_heights(got, want)
+    assert np.isfinite(want).sum() > want.size // 5
+
+
+def test_geotiff_source_matches_the_live_reference(orc, ref, tmp_path):
+    for name in ("int16_bilinear_cropped", "int32_cubic_antimeridian"):
+        assert same_heights(oracle_geotiff_heights(orc, name, tmp_path),
+                            reference_geotiff_heights(ref, name, tmp_path))
+
+
+def test_reader_matches_the_reference_reader(ref, tmp_path):
+    """Every layout of tests/tiff_craft.py, and this repository's and the reference's own LZW
+    GeoTIFFs, read to the reference's ReadGeoTiff samples (no-data samples as NaN)."""
+    import tiff_craft
+
+    from source_cases import GOLDEN
+
+    paths = []
+    for name, spec in tiff_craft.cases().items():
+        path = tmp_path / f"{name}.tif"
+        tiff_craft.write(path, spec)
+        paths.append((path, spec.values.shape[0]))
+    paths.append((GOLDEN / "output_sphere_lzw.tif", 1))
+    for path, bands in paths:
+        for band in range(bands):
+            f = N.GeoTiffFile(path)
+            info, _, _ = f.info(band)
+            got = f.read(band, 0, 0, info.width, info.height)
+            want = np.zeros((info.height, info.width), np.float32)
+            transform = np.zeros(6)
+            no_data, has = C.c_double(), C.c_int()
+            assert ref.ref_geotiff_read(str(path).encode(), band, want.ctypes.data,
+                                        transform.ctypes.data, C.byref(no_data),
+                                        C.byref(has)) == want.size
+            if has.value:
+                want[want == np.float32(no_data.value)] = np.nan
+            assert same_heights(got, want), (path.name, band)
+            assert np.array_equal(np.asarray(info.transform), transform), path.name
+            # A window decodes only the chunks it meets, to the same samples.
+            win = f.read(band, 3, 5, info.width - 7, info.height - 9)
+            assert same_heights(win, got[5:info.height - 4, 3:info.width - 4])

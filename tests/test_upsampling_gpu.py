@@ -76,3 +76,25 @@ def test_raster_posts_with_holes_match_the_oracle(hip, orc, method, wrap):
     assert np.array_equal(np.isnan(got), np.isnan(want))
     assert np.isnan(got).sum() > 0
     assert np.array_equal(got.view(np.uint32), want.view(np.uint32))
+
+
+from source_cases import GEOTIFF_CASES, geotiff_posts, golden_geotiff_heights, marked  # noqa: E402
+
+
+@pytest.mark.parametrize("name", GEOTIFF_CASES)
+def test_geotiff_source_on_the_device_matches_the_reference(hip, name, tmp_path):
+    """A GeoTiffSource's heights on the device (the reader's posts, tg_hip_upsample run by run
+    across the antimeridian) against the reference's own (tests/golden/sources_geotiff.npz):
+    the same bits and the same cells without data, cropped or not, finer and coarser."""
+    posts = geotiff_posts(hip, name, tmp_path)
+    _, _, _, (w, h, ox, oy, pixel), spacing = GEOTIFF_CASES[name]
+    out = N.DeviceArray(hip, w * h * 4)
+    posts.fill(N.Grid(w, h, ox, oy, pixel), out, spacing)
+    got = marked(out.download(np.empty((h, w), np.float32)), posts)
+    want = golden_geotiff_heights(name)
+    assert np.array_equal(np.isnan(got), np.isnan(want))
+    ok = ~np.isnan(want)
+    if GEOTIFF_CASES[name][1] == N.TG_RESAMPLE_LANCZOS3:
+        assert ulps(got[ok], want[ok]) <= 1
+    else:
+        assert np.array_equal(got[ok].view(np.uint32), want[ok].view(np.uint32))
