@@ -33,12 +33,15 @@ the square.
   ``tiles`` it is a straight track on a plane. The crater catalogue (C3's population: N(>D) =
   0.0156 D^-2.2 per m^2, D 0.5-300 m, complex from 50 m with central peaks and wall terraces from
   150 m, seed 5) covers a 2 km swath along the track, placed on the sphere as the reference places
-it (one generation over the union of the track's boxes); boulders are not served. It reports the
+  it (one generation over the union of the track's boxes), with boulders placed as the
+  reference's BoulderLayer places them and drawn after the craters. It reports the
   p50/p99 request-to-host latency (submission to heights in the caller's memory) for 513 x 513
   tiles and for all tiles, and the served Gcells/s.
 
 ``--workload grid`` is one 4097 x 4097 grid at 0.5 m, with the same noise and craters (seed 1),
-evaluated by the strip kernels.
+evaluated by the strip kernels. ``--workload c3`` is one 8193 x 8193 grid at 0.25 m with complex
+craters and boulders. ``--workload sources`` is the 4097 x 4097 grid at 0.5 m filled by a
+SyntheticSource whose noise lies on posts 2 m apart, upsampled by the kernel upsamplers.
 
 Prints one JSON line.
 """
@@ -91,6 +94,20 @@ OPS_BODY_PAIR_EXTRA = 35
 VALU_PEAK_TFLOPS = 157.3
 ISSUE_CEILING = 256 * 64 * 2.4e9
 HBM_PEAK_GBS = 8000.0
+# MI355X's published FP64 vector peak (an FMA as 2 flops a lane).
+FP64_PEAK_TFLOPS = 78.6
+#   kernel upsampling per cell, all in FP64 (src/raster/kernels.h:31-78, resample.cpp:19-135,
+#   geo_transform.h:63-79): the cell centre and its lattice position 26 (two PixelToWorld rows,
+#   two differences, the 2x2 inverse, the half-post shift); floor and fraction 4; per tap and
+#   axis the kernel's weight (cubic and B-spline 9: offset, fabs, two compares, five-term
+#   polynomial; Lanczos 3 15: offset, fabs, three compares, floor, two sinc of mul, sin, div, the
+#   third, the product, the normalising division; bilinear 1 an axis); per tap of the square 12
+#   (the weight product and its zero test, two clamped post indices, the read's NaN test, the
+#   widening, the weighted product and the sum); the narrowing 1. Nearest: position 26, two
+#   rounds, one tap 12.
+OPS_UPSAMPLE_POSITION = 27
+OPS_UPSAMPLE_TAP = 12
+OPS_UPSAMPLE_WEIGHT = {"cubic": 9, "cubic_bspline": 9, "lanczos3": 15}
 METRIC = "Gcells/s batched vs plain HIP at <=1e-3 m err; p50/p99 tile latency"
 
 C2_FRACTAL = dict(seed=1, wavelength=512.0, amplitude=20.0, octaves=8, lacunarity=2.0, gain=0.5)
@@ -868,6 +885,143 @@ def grid_as_one_tile(args, N, dev, terrain, spec, strip_heights: np.ndarray) -> 
     return result
 
 
+def upsample_ops(method: str) -> int:
+    """Algorithmic FP64 operations of one upsampled cell (the counts above)."""
+    if method == "nearest":
+        return OPS_UPSAMPLE_POSITION + 2 + OPS_UPSAMPLE_TAP
+    if method == "bilinear":
+        return OPS_UPSAMPLE_POSITION + 4 + 2 + 4 * OPS_UPSAMPLE_TAP
+    taps = 6 if method == "lanczos3" else 4
+    return (OPS_UPSAMPLE_POSITION + 4 + 2 * taps * OPS_UPSAMPLE_WEIGHT[method]
+            + taps * taps * OPS_UPSAMPLE_TAP)
+
+
+def bench_sources(args, N, dev) -> dict:
+    """A SyntheticSource with a pixel size: C2's fractal on posts 2 m apart, upsampled onto the
+    4097 x 4097 grid at 0.5 m by the reference's default upsampler (cubic). One step is the noise
+    at the posts the grid needs, then UpsampleGrid over the grid, inputs and outputs in HBM. The
+    other kernels are timed on the same grid; the CPU baseline is the reference's own
+    SyntheticSource::Fill on a band of rows."""
+    from terraingen_amd.sources import METHODS, SyntheticPosts
+
+    n = args.size if args.size else 4097
+    pixel, post = 0.5, 2.0
+    grid = N.north_up_grid(n, n, pixel)
+    fractal = [C2_FRACTAL]
+    sv = C.c_void_p()
+    dev.call("tg_hip_stream_create", dev.handle, C.byref(sv))
+    stream = sv.value
+    out = N.DeviceArray(dev, n * n * 4)
+    names = ["cubic", "nearest", "bilinear", "cubic_bspline", "lanczos3"]
+    sources = {m: SyntheticPosts(dev, fractal, 0.0, post, METHODS[m]) for m in names}
+    fills = {m: sources[m].prepare(grid) for m in names}
+
+    def timed(fn, steps):
+        for _ in range(args.warmup):
+            fn()
+        dev.synchronize(stream)
+        a, b = N.Event(dev), N.Event(dev)
+        t0 = time.perf_counter()
+        a.record(stream)
+        for _ in range(steps):
+            fn()
+        b.record(stream)
+        dev.synchronize(stream)
+        return a.elapsed_ms(b) / steps, (time.perf_counter() - t0) / steps
+
+    main = fills["cubic"]
+    step_ms, wall_s = timed(lambda: main.launch(out, stream), args.steps)
+    posts_ms, _ = timed(lambda: main.launch_posts(stream), args.steps)
+    per_method = {}
+    for m in names:
+        fills[m].launch_posts(stream)
+        ms, _ = timed(lambda f=fills[m]: f.launch_upsample(out, stream), args.steps)
+        per_method[m] = ms
+    main.launch(out, stream)
+    dev.synchronize(stream)
+    got = out.download(np.empty((n, n), np.float32))
+    cells = n * n
+    _, _, pw, ph = main.window
+    ups_ms = per_method["cubic"]
+    ops = cells * upsample_ops("cubic")
+    achieved = ops / (ups_ms * 1e-3)
+    bytes_moved = cells * 4.0 + pw * ph * 4.0
+    result = {
+        "metric": METRIC, "value": round(cells / (step_ms * 1e-3) / 1e9, 3), "unit": "Gcells/s",
+        "ms_per_step": round(step_ms, 4), "wall_ms_per_step": round(wall_s * 1e3, 4),
+        "dtype": "f64 (f32 heights)",
+        "config": {"workload": f"synthetic source with 2 m posts on a {n}x{n} grid @0.5 m: "
+                               "8-octave Perlin fBm (512 m, lac 2, gain 0.5, 20 m, seed 1) at "
+                               "the posts, cubic upsampling (the reference's DefaultUpsampler)",
+                   "cells": cells, "posts": pw * ph, "post_spacing_m": post},
+        "kernels_ms": {"NoiseFillStrip (posts)": round(posts_ms, 4),
+                       "UpsampleGrid (cubic)": round(ups_ms, 4)},
+        "upsample_ms": {m: round(v, 4) for m, v in per_method.items()},
+        "upsample_gcells_per_s": {m: round(cells / (v * 1e-3) / 1e9, 2)
+                                  for m, v in per_method.items()},
+        "roofline": {"bound": "valu", "achieved": round(achieved / 1e12, 3),
+                     "peak": FP64_PEAK_TFLOPS, "unit": "TFLOP/s",
+                     "frac": round(achieved / (FP64_PEAK_TFLOPS * 1e12), 4),
+                     "peak_basis": "MI355X published FP64 vector peak (78.6 TFLOP/s, FMA as 2 "
+                                   "flops a lane); achieved counts one per algorithmic FP64 "
+                                   "operation",
+                     "traffic": None, "kernel": "UpsampleGrid",
+                     "ops_per_cell": upsample_ops("cubic"),
+                     "hbm_algorithmic_GBs": round(bytes_moved / (ups_ms * 1e-3) / 1e9, 1),
+                     "hbm_frac": round(bytes_moved / (ups_ms * 1e-3) / 1e9 / HBM_PEAK_GBS, 4)},
+    }
+    if not args.no_cpu_baseline:
+        result |= cpu_sources(grid, fractal, post, got, args.cpu_rows)
+    for src in sources.values():
+        src.close()
+    return result
+
+
+def cpu_sources(grid, fractal, post: float, got: np.ndarray, rows: int) -> dict:
+    """The reference's SyntheticSource::Fill with its KernelUpsampler(cubic) (oracle/_ref) on the
+    grid or a band of its rows, on this host's cores: one warm-up, best of 3. Its heights are
+    compared with the device's bit for bit. Reports no baseline where oracle/_ref was not built."""
+    from oracle import reference
+    from terraingen_amd import native as N
+
+    model, cores = host_cpu()
+    rows = rows if 0 < rows < grid.height else grid.height
+    r0 = (grid.height - rows) // 2
+    band = N.Grid(grid.width, rows, grid.origin_x, grid.origin_y - r0 * grid.pixel, grid.pixel)
+    want = got[r0:r0 + rows]
+    ref = reference()
+    if ref is None:
+        return {"cpu_baseline": {"value": None, "note": "oracle/_ref not built"}}
+    out = np.empty((rows, grid.width), np.float32)
+    kernel = N.noise_kernel(fractal)
+    ref.ref_sources_threads(cores)
+
+    def run():
+        code = ref.ref_synthetic_grid(C.cast(C.byref(kernel.fractals), C.c_void_p), kernel.count,
+                                      0.0, post, N.TG_RESAMPLE_CUBIC, band.width, band.height,
+                                      band.origin_x, band.origin_y, band.pixel, 0.0,
+                                      out.ctypes.data)
+        if code != 0:
+            raise RuntimeError(ref.ref_sources_error().decode())
+
+    run()
+    times = []
+    for _ in range(3):
+        t0 = time.perf_counter()
+        run()
+        times.append(time.perf_counter() - t0)
+    cells = rows * grid.width
+    same = bool(np.array_equal(out.view(np.uint32), want.view(np.uint32)))
+    return {"cpu_baseline": {"value": round(cells / min(times) / 1e9, 6), "unit": "Gcells/s",
+                             "cores": cores, "kind": "reference", "cpu_model": model,
+                             "sample": f"rows {r0}..{r0 + rows - 1} (x {grid.width} cells) of "
+                                       "the grid: the reference's SyntheticSource with 2 m "
+                                       "posts, KernelUpsampler(cubic); best of 3 after 1 "
+                                       "warm-up"},
+            "bit_equal_to_reference_band": same,
+            "max_diff_vs_reference_m": float(np.abs(out.astype(np.float64) - want).max())}
+
+
 def bench_grid(args, N, dev) -> dict:
     from terraingen_amd.pipeline import (CraterLayerSpec, PlaneTerrain, crater_pair_cost,
                                          plane_batch_view)
@@ -1003,7 +1157,8 @@ def main() -> int:
     ap.add_argument("--gpus", type=int, default=1)
     ap.add_argument("--steps", type=int, default=10)
     ap.add_argument("--warmup", type=int, default=2)
-    ap.add_argument("--workload", choices=("sphere", "tiles", "grid", "c3"), default="sphere")
+    ap.add_argument("--workload", choices=("sphere", "tiles", "grid", "c3", "sources"),
+                    default="sphere")
     ap.add_argument("--tiles", type=int, default=1024)
     ap.add_argument("--size", type=int, default=0,
                     help="grid workload edge (0: 4097 for grid, 8193 for c3)")
@@ -1011,6 +1166,8 @@ def main() -> int:
                     help="length of the served stream (0: skip it)")
     ap.add_argument("--cpu-cells", type=int, default=6_000_000,
                     help="cells of the tiles workload timed on the host (CPU baseline)")
+    ap.add_argument("--cpu-rows", type=int, default=0,
+                    help="rows of the sources workload timed on the host (0: the whole grid)")
     ap.add_argument("--no-cpu-baseline", action="store_true")
     ap.add_argument("--no-plain", action="store_true", help="skip the plain per-tile path")
     ap.add_argument("--no-autotune", action="store_true",
@@ -1031,6 +1188,8 @@ def main() -> int:
         body = bench_tiles(args, N, dev)
     elif args.workload == "sphere":
         body = bench_sphere(args, N, dev)
+    elif args.workload == "sources":
+        body = bench_sources(args, N, dev)
     else:
         body = bench_grid(args, N, dev)
     if args.workload in ("tiles", "sphere") and args.serve_seconds > 0:

@@ -178,6 +178,7 @@ def _declare(lib: C.CDLL) -> None:
         "ref_synthetic_grid": (C.c_int, [vp, u, f, d, u, u, u, d, d, d, d, vp]),
         "ref_geotiff_grid": (C.c_int, [C.c_char_p, u, vp, u, u, d, d, d, d, d, vp]),
         "ref_sources_error": (C.c_char_p, []),
+        "ref_sources_threads": (None, [u]),
         "ref_geotiff_read": (C.c_long, [C.c_char_p, u, vp, vp, vp, vp]),
         "ref_boulders_open": (vp, [C.POINTER(RefBoulderCase)]),
         "ref_boulders_close": (None, [vp]),

@@ -25,6 +25,7 @@ using namespace terraingen;
 namespace {
 
 thread_local std::string gSourceError;
+unsigned gThreads = 8;
 
 Fractal FromLayoutS(const tg_fractal& f) {
     Fractal out;
@@ -51,7 +52,7 @@ int Evaluate(std::shared_ptr<const Source> source, std::uint32_t width, std::uin
                           std::vector<LayerPtr>{});
     const RasterGrid grid{width, height, GeoTransform::NorthUp(originX, originY, pixel)};
     EvaluationOptions options;
-    options.On(Backend::Cpu).Threads(8);
+    options.On(Backend::Cpu).Threads(gThreads);
     options.spacing = spacing;
     const Raster<float> raster = terrain.Evaluate(grid, options);
     std::memcpy(heights, raster.Values().data(), raster.Size() * sizeof(float));
@@ -64,6 +65,13 @@ extern "C" {
 
 const char* ref_sources_error() {
     return gSourceError.c_str();
+}
+
+// The CPU threads the grids below are evaluated on (8 until set; 0 leaves it).
+void ref_sources_threads(std::uint32_t threads) {
+    if (threads != 0) {
+        gThreads = threads;
+    }
 }
 
 // A SyntheticSource of `count` fractals at `elevation` whose posts lie `post` apart, upsampled

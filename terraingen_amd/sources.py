@@ -117,34 +117,70 @@ class SyntheticPosts:
 
     def posts(self, window, stream=None) -> N.DeviceArray:
         """The noise at the window's posts on the device (tg_hip_noise_fill), row by row."""
+        fill = self.prepare_window(window)
+        fill.launch_posts(stream)
+        self.dev.synchronize(stream)
+        return fill.d_posts
+
+    def prepare_window(self, window) -> "PreparedFill":
         c0, r0, w, h = window
         xs, ys = self.post_coordinates(window)
-        out = N.DeviceArray(self.dev, w * h * 4)
-        d_xs = N.DeviceArray.from_host(self.dev, xs)
-        d_ys = N.DeviceArray.from_host(self.dev, ys)
-        params = N.KernelParams(w, h, N.NODATA_NONE, 0.0, 0)
-        self.dev.call("tg_hip_noise_fill", self.dev.handle, self.plan, self.elevation, out.ptr,
-                      d_xs.ptr, d_ys.ptr, C.byref(params), stream)
-        self.dev.synchronize(stream)
-        return out
+        return PreparedFill(self, window, N.DeviceArray(self.dev, w * h * 4),
+                            N.DeviceArray.from_host(self.dev, xs),
+                            N.DeviceArray.from_host(self.dev, ys), None)
 
-    def fill(self, grid: N.Grid, out: N.DeviceArray, spacing: float = 0.0, stream=None):
-        """SyntheticSource::FillSamples of `grid` at `spacing` (0: the grid's, as
-        ResolvedSpacing gives it) into `out`; returns the spec and the posts' window."""
+    def prepare(self, grid: N.Grid, spacing: float = 0.0) -> "PreparedFill":
+        """The device buffers and the spec of a fill of `grid` at `spacing` (0: the grid's, as
+        ResolvedSpacing gives it), so the fill can be launched again allocating nothing."""
         spacing = max(spacing, grid.pixel)
         window = tap_window(grid, self.lattice_x, self.lattice_y, self.post, self.method)
-        posts = self.posts(window, stream)
-        spec = upsample_spec(grid, self.lattice_x, self.lattice_y, self.post, window,
-                             self.method, spacing)
-        self.dev.call("tg_hip_upsample", self.dev.handle, C.byref(spec), posts.ptr, out.ptr,
-                      stream)
+        fill = self.prepare_window(window)
+        fill.spec = upsample_spec(grid, self.lattice_x, self.lattice_y, self.post, window,
+                                  self.method, spacing)
+        return fill
+
+    def fill(self, grid: N.Grid, out: N.DeviceArray, spacing: float = 0.0, stream=None):
+        """SyntheticSource::FillSamples of `grid` at `spacing` (0: the grid's) into `out`;
+        returns the spec and the posts."""
+        fill = self.prepare(grid, spacing)
+        fill.launch(out, stream)
         self.dev.synchronize(stream)
-        return spec, posts
+        return fill.spec, fill.d_posts
 
     def close(self):
         if getattr(self, "plan", None) is not None:
             self.dev.lib.tg_hip_noise_plan_free(self.plan)
             self.plan = None
+
+
+@dataclass
+class PreparedFill:
+    """A SyntheticPosts fill with its device buffers made: the posts' window, its FP32
+    coordinates, the posts and the upsampling spec. Launches queue on `stream` and do not wait."""
+    source: SyntheticPosts
+    window: tuple
+    d_posts: N.DeviceArray
+    d_xs: N.DeviceArray
+    d_ys: N.DeviceArray
+    spec: N.Upsample | None
+
+    def launch_posts(self, stream=None):
+        """The noise at the window's posts (tg_hip_noise_fill), row by row."""
+        src = self.source
+        _, _, w, h = self.window
+        params = N.KernelParams(w, h, N.NODATA_NONE, 0.0, 0)
+        src.dev.call("tg_hip_noise_fill", src.dev.handle, src.plan, src.elevation,
+                     self.d_posts.ptr, self.d_xs.ptr, self.d_ys.ptr, C.byref(params), stream)
+
+    def launch_upsample(self, out: N.DeviceArray, stream=None):
+        """The grid from the posts (tg_hip_upsample)."""
+        src = self.source
+        src.dev.call("tg_hip_upsample", src.dev.handle, C.byref(self.spec), self.d_posts.ptr,
+                     out.ptr, stream)
+
+    def launch(self, out: N.DeviceArray, stream=None):
+        self.launch_posts(stream)
+        self.launch_upsample(out, stream)
 
 
 @dataclass
