@@ -414,34 +414,61 @@ __device__ __forceinline__ void DrainBins(uint32_t* counters, uint32_t chunk, F&
 // one or two per lane (`test(e, n)` reads entry e's crater into n and says whether it can reach
 // the wave's cells), the passing ones gathered with a ballot and visited in list order. Every depth
 // visits the same craters in the same order, so it changes only the time.
-template <class Test, class Visit>
+// kOneSite: `visit` is called from one place, so a large visit (the full crater evaluation) is
+// inlined once rather than made a function call; a small one is inlined at each of the
+// depths' gathers, which keeps the depth-64 loop tighter.
+template <bool kOneSite, class Test, class Visit>
 __device__ __forceinline__ void CulledWalk(uint32_t first, uint32_t end, uint32_t depth,
                                            uint32_t lane, const Test& test, const Visit& visit) {
-    const auto gather = [&](uint64_t candidates, uint32_t n) {
-        while (candidates != 0ull) {
-            const uint32_t j = uint32_t(__builtin_ctzll(candidates));
-            candidates &= candidates - 1ull;
-            visit(uint32_t(__builtin_amdgcn_readlane(int(n), int(j))));
+    if constexpr (!kOneSite) {
+        const auto gather = [&](uint64_t candidates, uint32_t n) {
+            while (candidates != 0ull) {
+                const uint32_t j = uint32_t(__builtin_ctzll(candidates));
+                candidates &= candidates - 1ull;
+                visit(uint32_t(__builtin_amdgcn_readlane(int(n), int(j))));
+            }
+        };
+        if (depth == 128u) {
+            for (uint32_t base = first; base < end; base += 128u) {
+                // Both halves' loads and tests before either ballot, for more loads in flight.
+                uint32_t n0 = 0u, n1 = 0u;
+                const bool near0 = test(base + lane, n0);
+                const bool near1 = test(base + 64u + lane, n1);
+                const uint64_t c0 = __ballot(near0);
+                const uint64_t c1 = __ballot(near1);
+                gather(c0, n0);
+                gather(c1, n1);
+            }
+            return;
         }
-    };
-    if (depth == 128u) {
-        for (uint32_t base = first; base < end; base += 128u) {
-            // Both halves' loads and tests before either ballot, for more loads in flight.
-            uint32_t n0 = 0u, n1 = 0u;
-            const bool near0 = test(base + lane, n0);
-            const bool near1 = test(base + 64u + lane, n1);
-            const uint64_t c0 = __ballot(near0);
-            const uint64_t c1 = __ballot(near1);
-            gather(c0, n0);
-            gather(c1, n1);
+        const uint32_t step = depth == 32u ? 32u : 64u;
+        for (uint32_t base = first; base < end; base += step) {
+            uint32_t n = 0u;
+            const bool near = lane < step && test(base + lane, n);
+            gather(__ballot(near), n);
         }
         return;
     }
-    const uint32_t step = depth == 32u ? 32u : 64u;
+    const uint32_t step = depth == 128u ? 128u : (depth == 32u ? 32u : 64u);
     for (uint32_t base = first; base < end; base += step) {
-        uint32_t n = 0u;
-        const bool near = lane < step && test(base + lane, n);
-        gather(__ballot(near), n);
+        // Both halves' loads and tests before either ballot, for more loads in flight.
+        uint32_t n0 = 0u, n1 = 0u;
+        const bool near0 = lane < step && test(base + lane, n0);
+        const bool near1 = step == 128u && test(base + 64u + lane, n1);
+        uint64_t low = __ballot(near0);
+        uint64_t high = __ballot(near1);
+        while ((low | high) != 0ull) {
+            const bool fromLow = low != 0ull;
+            const uint32_t j = uint32_t(__builtin_ctzll(fromLow ? low : high));
+            const uint32_t n =
+                uint32_t(__builtin_amdgcn_readlane(int(fromLow ? n0 : n1), int(j)));
+            if (fromLow) {
+                low &= low - 1ull;
+            } else {
+                high &= high - 1ull;
+            }
+            visit(n);
+        }
     }
 }
 
@@ -471,7 +498,7 @@ __global__ void __launch_bounds__(kGroupThreads, kAllFast ? (kPrecise ? kPrecise
                uint32_t binSide, uint32_t cullDepth, float* __restrict__ out) {
     const uint32_t lane = threadIdx.x & 63u;
     const uint32_t squaresAcross = binSide / 8u;
-    DrainBins(counters, chunk, [&](uint32_t b) {
+    DrainBins(counters, chunk, [&](uint32_t b) __attribute__((always_inline)) {
         const tg_bin bin = bins[b];
         const tg_tile_desc tile = tiles[bin.tile];
         const uint32_t local = b - tile.bin_first;
@@ -505,7 +532,7 @@ __global__ void __launch_bounds__(kGroupThreads, kAllFast ? (kPrecise ? kPrecise
                 const float xLow = float(cx - double(x));
                 const float yLow = float(cy - double(y));
                 const uint32_t end = bin.first + bin.count;
-                const auto visit = [&](uint32_t n) {
+                const auto visit = [&](uint32_t n) __attribute__((always_inline)) {
                     VisitCrater<kAllFast, kPrecise>(n, inside, cell, x, xLow, y, yLow, tile.pixel,
                                           hoisted != 0u, hot, hotHarm, kernel, data, records, harm,
                                           outline, precise);
@@ -534,11 +561,19 @@ __global__ void __launch_bounds__(kGroupThreads, kAllFast ? (kPrecise ? kPrecise
                             visit(n);
                         }
                     }
-                } else if (kWalk == kWalkTogether || bin.count <= kShortList) {
+                } else if constexpr (kWalk == kWalkTogether) {
+                    for (uint32_t e = bin.first; e < end; ++e) {
+                        visit(entries[e]);
+                    }
+                } else if (kAllFast && bin.count <= kShortList) {
                     for (uint32_t e = bin.first; e < end; ++e) {
                         visit(entries[e]);
                     }
                 } else {
+                    // Without kAllFast a short list is visited whole through the same walk, every
+                    // entry passing, in the order the loop above visits (the visit's own test
+                    // decides what each cell takes), so the full evaluation is inlined once.
+                    const bool whole = bin.count <= kShortList;
                     // The box of the square's cell centres (the same FP64 centres as above).
                     const uint32_t lastCol = std::min(squareCol + 7u, tile.width - 1u);
                     const uint32_t lastRow = std::min(squareRow + 7u, tile.height - 1u);
@@ -555,6 +590,9 @@ __global__ void __launch_bounds__(kGroupThreads, kAllFast ? (kPrecise ? kPrecise
                             return false;
                         }
                         n = entries[e];
+                        if (whole) {
+                            return true;
+                        }
                         const tg::HotCrater& c = hot[n];
                         const double px = double(c.x) + double(c.x_low);
                         const double py = double(c.y) + double(c.y_low);
@@ -562,7 +600,8 @@ __global__ void __launch_bounds__(kGroupThreads, kAllFast ? (kPrecise ? kPrecise
                         const double ey = fmax(0.0, fmax(south - py, py - north));
                         return ex * ex + ey * ey <= double(c.cull) * (1.0 + 1e-4) + 1e-6;
                     };
-                    CulledWalk(bin.first, end, cullDepth, lane, test, visit);
+                    CulledWalk<!kAllFast>(bin.first, end, whole ? 64u : cullDepth, lane, test,
+                                          visit);
                 }
             }
             float height = cell.Folded();
@@ -670,7 +709,7 @@ __global__ void __launch_bounds__(kGroupThreads, kAllFast ? (kPrecise ? kPrecise
     const uint32_t lane = threadIdx.x & 63u;
     const tg::CubeSteps steps = tg::MakeCubeSteps(lane);
     const uint32_t squaresAcross = binSide / 8u;
-    DrainBins(counters, chunk, [&](uint32_t b) {
+    DrainBins(counters, chunk, [&](uint32_t b) __attribute__((always_inline)) {
         const tg_bin bin = bins[b];
         const tg_tile_desc tile = tiles[bin.tile];
         const uint32_t local = b - tile.bin_first;
@@ -724,24 +763,32 @@ __global__ void __launch_bounds__(kGroupThreads, kAllFast ? (kPrecise ? kPrecise
             if (kernel != nullptr && bin.count > 0u) {
                 const tg_split_point p = split();
                 const uint32_t end = bin.first + bin.count;
-                const auto visit = [&](uint32_t n) {
+                const auto visit = [&](uint32_t n) __attribute__((always_inline)) {
                     VisitBodyCrater<kAllFast, kPrecise>(n, inside, cell, p, spacing, hoisted != 0u, hot,
                                               hotHarm, frames, kernel, data, records, harm,
                                               outline, precise);
                 };
-                if (bin.count <= kShortList) {
+                if (kAllFast && bin.count <= kShortList) {
                     for (uint32_t e = bin.first; e < end; ++e) {
                         visit(entries[e]);
                     }
                 } else {
+                    // Without kAllFast a short list is visited whole through the same walk, every
+                    // entry passing, in the order the loop above visits (the visit's own test
+                    // decides what each cell takes), so the full evaluation is inlined once.
+                    const bool whole = bin.count <= kShortList;
                     // The ball around the square's cells: centred on lane 27's cell (row 3,
-                    // column 3), its radius the farthest of the wave's cells from it.
-                    const double ox = __shfl(at.x, 27, 64);
-                    const double oy = __shfl(at.y, 27, 64);
-                    const double oz = __shfl(at.z, 27, 64);
-                    const double reach = WaveMax(sqrt((at.x - ox) * (at.x - ox) +
-                                                      (at.y - oy) * (at.y - oy) +
-                                                      (at.z - oz) * (at.z - oz)));
+                    // column 3), its radius the farthest of the wave's cells from it (only for
+                    // a list the walk tests).
+                    double ox = 0.0, oy = 0.0, oz = 0.0, reach = 0.0;
+                    if (!whole) {
+                        ox = __shfl(at.x, 27, 64);
+                        oy = __shfl(at.y, 27, 64);
+                        oz = __shfl(at.z, 27, 64);
+                        reach = WaveMax(sqrt((at.x - ox) * (at.x - ox) +
+                                             (at.y - oy) * (at.y - oy) +
+                                             (at.z - oz) * (at.z - oz)));
+                    }
                     // Entry e of the list near the ball. A cell passing the FP32 test lies within
                     // the cull radius of the centre along the sphere to a relative 1e-6 or so,
                     // and the chord is no longer than the arc; the margins keep every entry a
@@ -751,6 +798,9 @@ __global__ void __launch_bounds__(kGroupThreads, kAllFast ? (kPrecise ? kPrecise
                             return false;
                         }
                         n = entries[e];
+                        if (whole) {
+                            return true;
+                        }
                         const tg_crater_frame& f = frames[n];
                         const double qx = double(f.x) + double(f.x_low) - ox;
                         const double qy = double(f.y) + double(f.y_low) - oy;
@@ -758,7 +808,8 @@ __global__ void __launch_bounds__(kGroupThreads, kAllFast ? (kPrecise ? kPrecise
                         const double d = sqrt(qx * qx + qy * qy + qz * qz);
                         return d <= sqrt(double(hot[n].cull)) * (1.0 + 1e-4) + 1e-3 + reach;
                     };
-                    CulledWalk(bin.first, end, cullDepth, lane, test, visit);
+                    CulledWalk<!kAllFast>(bin.first, end, whole ? 64u : cullDepth, lane, test,
+                                          visit);
                 }
             }
             float height = cell.Folded();

@@ -80,6 +80,7 @@ struct tg_crater_set {
     tg::MpdTerms* terms = nullptr;  // per record: the hoisted MPD path's constants
     tg::PreciseTerms* precise = nullptr;  // per record: FP64 terms of km-scale craters
     bool allFast = false;           // every record takes the hoisted path
+    bool anyPrecise = false;        // some record takes the hoisted path in FP64
     size_t pointCount = 0;
     size_t lowCount = 0;
     uint32_t tilesAcross = 0;

@@ -134,7 +134,9 @@ struct BoulderPass {
 
 // kAllFast: every crater of the set takes the hoisted MPD path (see BatchTiles).
 // kBoulders: a boulder layer follows the craters (BoulderPass).
-template <bool kAllFast, bool kBoulders = false>
+// kPrecise: some crater of the set takes that path in FP64 (a set without km-scale craters runs
+// a variant without the FP64 code, whose registers it would otherwise hold).
+template <bool kAllFast, bool kBoulders = false, bool kPrecise = false>
 __global__ void __launch_bounds__(kCraterTile * kCraterTile)
     CraterStrip(float* __restrict__ heights, const float* __restrict__ xs,
                 const float* __restrict__ ys, const tg_crater_shape* __restrict__ kernel,
@@ -179,14 +181,18 @@ __global__ void __launch_bounds__(kCraterTile * kCraterTile)
         for (uint32_t e = first; e < first + count; ++e) {
             const uint32_t n = entries[e];
             const tg_crater_record& c = records[n];
-            if (terms[n].fast == tg::kFastPrecise) {
-                const tg::Offset64 o = tg::PlaneOffset64(precise[n], p.x, p.x_low, p.y, p.y_low);
-                if (o.distance * o.distance < double(cull[n])) {
-                    cell.pending += double(c.resolution) *
-                                    tg::MpdPrecise(precise[n], c.ellipticity, c.first_harmonic,
-                                                   c.harmonic_count, harm + c.harmonic_offset, o);
+            if constexpr (kPrecise) {
+                if (terms[n].fast == tg::kFastPrecise) {
+                    const tg::Offset64 o =
+                        tg::PlaneOffset64(precise[n], p.x, p.x_low, p.y, p.y_low);
+                    if (o.distance * o.distance < double(cull[n])) {
+                        cell.pending += double(c.resolution) *
+                                        tg::MpdPrecise(precise[n], c.ellipticity,
+                                                       c.first_harmonic, c.harmonic_count,
+                                                       harm + c.harmonic_offset, o);
+                    }
+                    continue;
                 }
-                continue;
             }
             const float dx = tg::SplitDifference(p.x, p.x_low, c.x, c.x_low);
             const float dy = tg::SplitDifference(p.y, p.y_low, c.y, c.y_low);
@@ -206,14 +212,17 @@ __global__ void __launch_bounds__(kCraterTile * kCraterTile)
         for (uint32_t e = first; e < first + count; ++e) {
             const uint32_t n = entries[e];
             const tg_crater_record& c = records[n];
-            if (terms[n].fast == tg::kFastPrecise) {
-                const tg::Offset64 o = tg::BodyOffset64(precise[n], p);
-                if (o.distance * o.distance < double(cull[n])) {
-                    cell.pending += double(c.resolution) *
-                                    tg::MpdPrecise(precise[n], c.ellipticity, c.first_harmonic,
-                                                   c.harmonic_count, harm + c.harmonic_offset, o);
+            if constexpr (kPrecise) {
+                if (terms[n].fast == tg::kFastPrecise) {
+                    const tg::Offset64 o = tg::BodyOffset64(precise[n], p);
+                    if (o.distance * o.distance < double(cull[n])) {
+                        cell.pending += double(c.resolution) *
+                                        tg::MpdPrecise(precise[n], c.ellipticity,
+                                                       c.first_harmonic, c.harmonic_count,
+                                                       harm + c.harmonic_offset, o);
+                    }
+                    continue;
                 }
-                continue;
             }
             const tg::Offset at = tg::BodySplitOffset(frames[n], p);
             if (at.dx * at.dx + at.dy * at.dy >= cull[n]) {
@@ -391,10 +400,16 @@ void LaunchCraters(const tg_crater_set& set, float* heights, const float* xs, co
         pass = BoulderPass{rocks->kernel, rocks->boulders, rocks->tiles, rocks->entries,
                            rocks->cull};
     }
-    // The compact kernel for a plane set whose craters all take the hoisted path.
+    // The compact kernel for a plane set whose craters all take the hoisted path; the FP64
+    // path's code only for a set with km-scale craters.
     const bool fast = set.allFast && !set.onBody;
-    auto* fn = rocks != nullptr ? (fast ? CraterStrip<true, true> : CraterStrip<false, true>)
-                                : (fast ? CraterStrip<true> : CraterStrip<false>);
+    using Strip = decltype(&CraterStrip<true>);
+    const Strip variants[2][2][2] = {
+        {{CraterStrip<false, false, false>, CraterStrip<false, false, true>},
+         {CraterStrip<false, true, false>, CraterStrip<false, true, true>}},
+        {{CraterStrip<true, false, false>, CraterStrip<true, false, true>},
+         {CraterStrip<true, true, false>, CraterStrip<true, true, true>}}};
+    Strip fn = variants[fast ? 1 : 0][rocks != nullptr ? 1 : 0][set.anyPrecise ? 1 : 0];
     hipLaunchKernelGGL(fn, grid, block, 0, stream, heights, xs, ys, set.kernel,
                        set.records, set.data, set.tiles, set.entries, set.frames, set.points,
                        set.lows, set.cull, set.harm, set.outline, set.terms, set.precise,
@@ -892,6 +907,7 @@ int CreateCraterSet(tg_dev* dev, const tg_crater_shape* kernel, const tg_crater_
                                           recordCount);
         owned.push_back(set->precise);
         set->allFast = AllMpdFast(*kernel, records, recordCount);
+        set->anyPrecise = AnyPrecise(*kernel, records, recordCount);
         set->tileSide = tiles[1];
         set->tilesAcross = tiles[0];
         set->tileRows = uint32_t((tileWords - 3) / 2 / tiles[0]);

@@ -66,7 +66,7 @@ __device__ __forceinline__ double LanczosAt(double x) {
     return SincAt(x) * SincAt(x / 3.0);
 }
 
-__device__ __forceinline__ int RadiusOf(uint32_t method) {
+__host__ __device__ constexpr int RadiusOf(uint32_t method) {
     return method == TG_RESAMPLE_LANCZOS3 ? 3
            : (method == TG_RESAMPLE_CUBIC || method == TG_RESAMPLE_CUBIC_BSPLINE) ? 2
                                                                                   : 1;
@@ -100,21 +100,30 @@ __device__ __forceinline__ void AxisWeights(uint32_t method, double fraction, do
     }
 }
 
-// The post at lattice (col, row) from the window held: columns wrapped first when the lattice
-// wraps, then both clamped to the window, so reads beyond it repeat its edge.
-__device__ __forceinline__ float PostAt(const tg_upsample& u, const float* __restrict__ posts,
-                                        int64_t col, int64_t row) {
+// The column in the window of lattice column `col`: wrapped first when the lattice wraps, then
+// clamped to the window, so reads beyond it repeat its edge.
+__device__ __forceinline__ uint32_t WindowColumn(const tg_upsample& u, int64_t col) {
     if (u.wrap_columns != 0u) {
         const int64_t n = int64_t(u.lattice_columns);
         col = ((col % n) + n) % n;
     }
-    const int64_t c = col < u.col0 ? 0 : (col >= u.col0 + int64_t(u.window_width)
-                                              ? int64_t(u.window_width) - 1
-                                              : col - u.col0);
+    return uint32_t(col < u.col0 ? 0 : (col >= u.col0 + int64_t(u.window_width)
+                                            ? int64_t(u.window_width) - 1
+                                            : col - u.col0));
+}
+
+// The offset in the window of the first post of lattice row `row`, clamped to the window.
+__device__ __forceinline__ size_t WindowRow(const tg_upsample& u, int64_t row) {
     const int64_t r = row < u.row0 ? 0 : (row >= u.row0 + int64_t(u.window_height)
                                               ? int64_t(u.window_height) - 1
                                               : row - u.row0);
-    return posts[size_t(r) * u.window_width + size_t(c)];
+    return size_t(r) * u.window_width;
+}
+
+// The post at lattice (col, row) from the window held.
+__device__ __forceinline__ float PostAt(const tg_upsample& u, const float* __restrict__ posts,
+                                        int64_t col, int64_t row) {
+    return posts[WindowRow(u, row) + WindowColumn(u, col)];
 }
 
 // `v` as a whole number when it lies within 1e-6 of one (detail::NearestWhole, lattice.h:33-39).
@@ -129,6 +138,9 @@ __device__ __forceinline__ bool NearWhole(double v, int64_t& whole) {
 
 constexpr uint32_t kSide = 16;
 
+// One variant per kernel (kMethod), so the taps' loops unroll and their indices and weights stay
+// in registers. Each tap's window index is found once per axis rather than once per tap.
+template <uint32_t kMethod>
 __global__ void __launch_bounds__(kSide * kSide)
     UpsampleGrid(tg_upsample u, const float* __restrict__ posts, float* __restrict__ out) {
     const uint32_t col = blockIdx.x * kSide + (threadIdx.x % kSide);
@@ -165,33 +177,44 @@ __global__ void __launch_bounds__(kSide * kSide)
             return;
         }
     }
-    if (u.method == TG_RESAMPLE_NEAREST) {
+    if constexpr (kMethod == TG_RESAMPLE_NEAREST) {
         *cell = PostAt(u, posts, int64_t(round(lu)), int64_t(round(lv)));
-        return;
-    }
-    const double u0 = floor(lu);
-    const double v0 = floor(lv);
-    double wx[6], wy[6];
-    AxisWeights(u.method, lu - u0, wx);
-    AxisWeights(u.method, lv - v0, wy);
-    const int radius = RadiusOf(u.method);
-    double sum = 0.0;
-    for (int j = 1 - radius; j <= radius; ++j) {
-        const double ww = wy[j + radius - 1];
-        for (int i = 1 - radius; i <= radius; ++i) {
-            const double w = wx[i + radius - 1];
-            if (w * ww == 0.0) {
-                continue;  // a tap of no weight is not read
-            }
-            const float p = PostAt(u, posts, int64_t(u0) + i, int64_t(v0) + j);
-            if (isnan(p)) {
-                *cell = __builtin_nanf("");
-                return;
-            }
-            sum += w * ww * double(p);
+    } else {
+        constexpr int kRadius = RadiusOf(kMethod);
+        constexpr int kTaps = 2 * kRadius;
+        const double u0 = floor(lu);
+        const double v0 = floor(lv);
+        double wx[6], wy[6];
+        AxisWeights(kMethod, lu - u0, wx);
+        AxisWeights(kMethod, lv - v0, wy);
+        // Tap k of an axis lies k + 1 - radius posts from (u0, v0).
+        uint32_t cols[kTaps];
+        size_t rows[kTaps];
+#pragma unroll
+        for (int k = 0; k < kTaps; ++k) {
+            cols[k] = WindowColumn(u, int64_t(u0) + (k + 1 - kRadius));
+            rows[k] = WindowRow(u, int64_t(v0) + (k + 1 - kRadius));
         }
+        double sum = 0.0;
+#pragma unroll
+        for (int j = 0; j < kTaps; ++j) {
+            const double ww = wy[j];
+#pragma unroll
+            for (int i = 0; i < kTaps; ++i) {
+                const double w = wx[i];
+                if (w * ww == 0.0) {
+                    continue;  // a tap of no weight is not read
+                }
+                const float p = posts[rows[j] + cols[i]];
+                if (isnan(p)) {
+                    *cell = __builtin_nanf("");
+                    return;
+                }
+                sum += w * ww * double(p);
+            }
+        }
+        *cell = float(sum);
     }
-    *cell = float(sum);
 }
 
 }  // namespace
@@ -225,7 +248,11 @@ void LaunchUpsample(const tg_upsample& u, const float* posts, float* out, hipStr
         return;
     }
     const dim3 grid((u.width + kSide - 1) / kSide, (u.height + kSide - 1) / kSide, 1);
-    hipLaunchKernelGGL(UpsampleGrid, grid, dim3(kSide * kSide), 0, stream, u, posts, out);
+    using Kernel = decltype(&UpsampleGrid<TG_RESAMPLE_CUBIC>);
+    const Kernel kernels[] = {UpsampleGrid<TG_RESAMPLE_NEAREST>, UpsampleGrid<TG_RESAMPLE_BILINEAR>,
+                              UpsampleGrid<TG_RESAMPLE_CUBIC>, UpsampleGrid<TG_RESAMPLE_LANCZOS3>,
+                              UpsampleGrid<TG_RESAMPLE_CUBIC_BSPLINE>};
+    hipLaunchKernelGGL(kernels[u.method], grid, dim3(kSide * kSide), 0, stream, u, posts, out);
     Check(hipGetLastError(), "launching the upsampling kernel");
 }
 
