@@ -115,10 +115,11 @@ constexpr uint32_t kBinSide = 16;
 constexpr uint32_t kGroupThreads = 256;  // four independent waves
 
 // Waves per SIMD the all-fast variants are built for: 8 without the FP64 path, and with it the
-// fastest of 4-8 measured on the C4 sphere batch (4: 52.7 ms, 5: 50.7, 6: 49.6, 7: 49.3,
-// 8: 50.7): fewer waves spill less of the FP64 terms but hide less latency.
+// fastest of 6-8 measured on the C4 sphere batch at one launch configuration (6: 49.07 ms, 80
+// VGPRs and 20 B of scratch a lane; 7: 50.54 ms, 72 and 60 B; 8: 50.76 ms, 64 and 88 B): fewer
+// waves spill less of the FP64 terms but hide less latency.
 #ifndef TG_PRECISE_WAVES
-#define TG_PRECISE_WAVES 7
+#define TG_PRECISE_WAVES 6
 #endif
 constexpr int kPreciseWaves = TG_PRECISE_WAVES;
 

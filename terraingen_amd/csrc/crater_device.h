@@ -1312,14 +1312,27 @@ __device__ __forceinline__ Offset64 BodyOffset64(const PreciseTerms& t, const tg
     const double oy = qx * t.northX + qy * t.northY + qz * t.northZ;
     const double chord = Sqrt64(qx * qx + qy * qy + qz * qz);
     const double s = fmin(chord * t.halfInvSphere, 1.0);
+    // asin(s) / s by its series below 0.25. Above (an offset of more than half the radius), the
+    // angle is halved until its sine is below 0.25 (sin(a/2) = sin a / sqrt(2 (1 + cos a))),
+    // and asin(s) = 2^k asin(t_k): a short loop rather than atan2, whose polynomial's constants
+    // the compiler keeps in registers across every kernel that inlines this.
+    const auto series = [](double u) {
+        const double u2 = u * u;
+        return 1.0 + u2 * (1.0 / 6.0 +
+                           u2 * (3.0 / 40.0 + u2 * (5.0 / 112.0 + u2 * (35.0 / 1152.0 +
+                                                                       u2 * (63.0 / 2816.0)))));
+    };
     double ratio;
     if (s < 0.25) {
-        const double s2 = s * s;
-        ratio = 1.0 + s2 * (1.0 / 6.0 +
-                            s2 * (3.0 / 40.0 + s2 * (5.0 / 112.0 + s2 * (35.0 / 1152.0 +
-                                                                        s2 * (63.0 / 2816.0)))));
+        ratio = series(s);
     } else {
-        ratio = atan2(s, sqrt(1.0 - s * s)) / s;
+        double u = s, scale = 1.0;
+#pragma unroll 1
+        while (u >= 0.25) {
+            u *= InvSqrt64(2.0 * (1.0 + Sqrt64(1.0 - u * u)));
+            scale *= 2.0;
+        }
+        ratio = scale * u * series(u) / s;
     }
     Offset64 o{chord * ratio, 0.0, 1.0};
     const double across = Sqrt64(ox * ox + oy * oy);

@@ -25,7 +25,7 @@ def run_batch(hip, b, use_graph=True):
     return out.download(np.empty(b.cells, np.float32))
 
 
-def check(hip, orc, b, oracle_of, label):
+def check(hip, orc, b, oracle_of, label, closer_than_fp32=True):
     got = run_batch(hip, b)
     worst64 = worst_ref = 0.0
     for i in range(b.tiles.size):
@@ -38,7 +38,8 @@ def check(hip, orc, b, oracle_of, label):
     print(f"[{label}] bit-equal to the plain path; vs FP64 oracle max {worst64:.3g} m (the "
           f"reference's FP32 arithmetic: {worst_ref:.3g} m)", flush=True)
     assert worst64 <= TOL_FP64
-    assert worst64 < worst_ref
+    if closer_than_fp32:
+        assert worst64 < worst_ref
 
 
 def moon_craters():
@@ -92,5 +93,44 @@ def test_km_craters_on_a_plane(hip, orc):
     b.stage(c4_tiles(16, 60000.0, seed=6, min_k=5, max_k=7, pixel_range=(1.0, 8.0)))
     try:
         check(hip, orc, b, plane_tile, "km craters on a plane")
+    finally:
+        b.close()
+
+
+def test_craters_large_against_a_small_body(hip, orc):
+    """Craters of 4-7 km on a body of 10 km radius: cells up to 2.4 crater radii out lie more than
+    half the body's radius from the centre along the chord, where the FP64 offset's arcsine ratio
+    leaves its series and halves the angle instead. Bit-equal to the plain path and within the
+    bound of the FP64 oracle (at this size the FP32 arithmetic is as close, so only the bound is
+    asserted)."""
+    radius = 10000.0
+    rng = np.random.default_rng(21)
+    c = np.zeros(6, N.CRATER_DTYPE)
+    c["x"] = np.arange(6) * 55.0 + rng.uniform(0.0, 10.0, 6)
+    c["y"] = rng.uniform(-20.0, 20.0, 6)
+    c["diameter"] = rng.uniform(4000.0, 7000.0, 6)
+    c["age_gyr"] = rng.uniform(0.0, 1.0, 6)
+    c["id"] = np.arange(1, 7, dtype=np.uint64)
+    b = SphereTileBatch(hip, [C2_FRACTAL], CraterLayerSpec(seed=4), region=None, radius=radius,
+                        craters_given=c)
+    b.build()
+    unit = radius * np.pi / 180.0
+    tiles = []
+    for k in c:
+        r = 0.5 * float(k["diameter"])
+        for radii in (0.97, 1.6, 2.4):
+            ang = rng.uniform(0.0, 2.0 * np.pi)
+            n = 33
+            step = 4.0 / unit
+            lon = float(k["x"]) + radii * r * np.cos(ang) / unit / np.cos(np.radians(float(k["y"])))
+            lat = float(k["y"]) + radii * r * np.sin(ang) / unit
+            tiles.append((lon - 0.5 * n * step, lat + 0.5 * n * step, step, n, n))
+    arr = np.zeros(len(tiles), N.TILE_DTYPE)
+    for j, t in enumerate(tiles):
+        arr[j] = t
+    b.stage(arr)
+    try:
+        check(hip, orc, b, sphere_tile, "craters large against a 10 km body",
+              closer_than_fp32=False)
     finally:
         b.close()
