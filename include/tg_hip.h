@@ -390,6 +390,13 @@ int tg_hip_event_elapsed(tg_dev* dev, void* start, void* stop, float* ms, char* 
  * warm-up one), `lane_ops` the lane operations it issued. */
 int tg_hip_valu_probe(tg_dev* dev, int kind, uint32_t iters, uint32_t blocks, float* ms,
                       double* lane_ops, char* err, size_t err_len);
+/* The batched kernels' output stores alone, for reading the HBM bytes a store shape writes (no
+ * counterpart in the reference; a measurement aid): `tiles` tiles of width x width floats packed
+ * into `out` (`out_capacity` floats), 16 x 16 bins in runs of consecutive bins per wave, each bin
+ * written as four 8 x 8 squares (`shape` 0, the batched kernels' stores), four 16 x 4 strips (1)
+ * or 64-float rows four bins across (2). `ms` is the launch, synchronous. */
+int tg_hip_store_probe(tg_dev* dev, int shape, uint32_t width, uint32_t tiles, float* out,
+                       size_t out_capacity, float* ms, char* err, size_t err_len);
 
 /* Pinned (page-locked) host memory, for copies that overlap device work. */
 int tg_hip_host_alloc(tg_dev* dev, size_t bytes, void** out, char* err, size_t err_len);

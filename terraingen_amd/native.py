@@ -190,6 +190,8 @@ HIP_ABI = {
                                         SZ]),
     "tg_hip_valu_probe": (C.c_int, [VP, C.c_int, u32, u32, C.POINTER(C.c_float),
                                     C.POINTER(C.c_double), CP, SZ]),
+    "tg_hip_store_probe": (C.c_int, [VP, C.c_int, u32, u32, VP, SZ, C.POINTER(C.c_float), CP,
+                                     SZ]),
 }
 TG_WALK_TOGETHER, TG_WALK_PER_LANE, TG_WALK_CULLED = 0, 1, 2
 
