@@ -835,24 +835,32 @@ def bench_serving(args, N, dev) -> dict:
 
 
 def grid_as_one_tile(args, N, dev, terrain, spec, strip_heights: np.ndarray) -> dict:
-    """The C3 grid through the batched kernel as one tile: the same crater list (generated over
-    the grid's box at its spacing) and boulder list as the strip passes, staged as one tile of
-    16 x 16 bins and run as a captured graph on the batch's stream; checked bit for bit against
-    the strip passes' heights."""
+    """The grid through the batched kernel as one tile: the same source noise, crater list
+    (generated over the grid's box at its spacing) and boulder list as the strip passes, staged as
+    one tile of 16 x 16 bins and run as a captured graph on the batch's stream; checked bit for bit
+    against the strip passes' heights."""
     from terraingen_amd.pipeline import TileBatch
 
     g = terrain.grid
     region = (g.origin_x, g.origin_y - g.height * g.pixel, g.origin_x + g.width * g.pixel,
               g.origin_y)
     t0 = time.perf_counter()
-    batch = TileBatch(dev, [], spec, region=region, finest_pixel=g.pixel,
-                      elevation=terrain.elevation, boulders=terrain.boulders)
+    batch = TileBatch(dev, list(terrain.source_fractals or []), spec, region=region,
+                      finest_pixel=g.pixel, elevation=terrain.elevation,
+                      boulders=terrain.boulders)
     batch.build()
     one = np.zeros(1, N.TILE_DTYPE)
     one[0] = (g.origin_x, g.origin_y, g.pixel, g.width, g.height)
     batch.stage(one)
     setup_ms = (time.perf_counter() - t0) * 1e3
     out = N.DeviceArray(dev, batch.cells * 4)
+    tuned = None
+    if not args.no_autotune:
+        # Boulders are binned into 16-cell bins, so only a grid without them tries other sides.
+        batch.autotune(out, depths=(32, 64, 128),
+                       bin_sides=None if terrain.boulders is not None else (8, 16, 32))
+        tuned = dict(zip(("bin_side", "walk", "groups_per_cu", "chunk", "cull_depth"),
+                         batch.tuned))
     for _ in range(max(args.warmup, 1)):
         batch.run(out)
     batch.wait()
@@ -869,13 +877,20 @@ def grid_as_one_tile(args, N, dev, terrain, spec, strip_heights: np.ndarray) -> 
     same = bool(np.array_equal(got.view(np.uint32), strip_heights.view(np.uint32)))
     diff = float(np.abs(got - strip_heights).max())
     strip_ms = terrain.timings.get("_strip_ms")
-    result = {"kernel": "BatchTiles<culled, general, boulders>", "ms": round(ms, 4),
+    boulders = terrain.boulders is not None
+    # The full-evaluation variant for complex craters, peaks or terraces; else the all-fast one.
+    general = bool(spec.complex_ or spec.peaks or spec.terraces or spec.profile != 1)
+    kind = "culled, " + ("general" if general else "all-fast") + (
+        ", boulders" if boulders else "")
+    result = {"kernel": f"BatchTiles<{kind}>", "ms": round(ms, 4),
               "bins": int(batch.bin_count), "craters": int(batch.records.size),
-              "boulders": int(batch.boulder_records.size), "setup_ms": round(setup_ms, 1),
+              "boulders": int(batch.boulder_records.size) if boulders else 0,
+              "setup_ms": round(setup_ms, 1), "autotune": tuned,
               "bit_equal_to_strip_passes": same, "max_diff_vs_strip_m": diff,
-              "strip": {"path": "noise fill, then CraterStrip with the boulders in the same "
-                                "pass (tg_hip_crater_boulder_launch), one thread per cell "
-                                "walking its 16 x 16 tile's lists",
+              "strip": {"path": "noise fill, then CraterStrip" + (
+                            " with the boulders in the same pass (tg_hip_crater_boulder_launch)"
+                            if boulders else "") + ", one thread per cell walking its 16 x 16 "
+                        "tile's lists",
                         "ms": round(strip_ms, 4) if strip_ms else None,
                         "value": round(g.width * g.height / (strip_ms * 1e-3) / 1e9, 3)
                         if strip_ms else None, "unit": "Gcells/s"},
@@ -1121,11 +1136,13 @@ def bench_grid(args, N, dev) -> dict:
         config["workload"] = (f"synthetic source {n}x{n} @0.5 m: 8-octave Perlin fBm (512 m, "
                               "lac 2, gain 0.5, 20 m, seed 1) + power-law craters "
                               "N(>D)=0.05 D^-2, D 1-200 m, seed 1")
-    batched = None
-    if c3:
-        batched = grid_as_one_tile(args, N, dev, terrain, spec, got)
-        step_ms, wall = batched.pop("_ms"), batched.pop("_wall_s")
-        kernels = {"BatchTiles<boulders>": (step_ms, crater_ops + boulder_ops)}
+    # The value: the grid as one tile of the batched kernel, noise, craters and boulders in one
+    # launch, bit-equal to the strip passes.
+    batched = grid_as_one_tile(args, N, dev, terrain, spec if c3 else CraterLayerSpec(seed=1),
+                               got)
+    step_ms, wall = batched.pop("_ms"), batched.pop("_wall_s")
+    kernels = {batched["kernel"]: (step_ms, noise_ops + crater_ops +
+                                   (boulder_ops if c3 else 0))}
     dominant = max(kernels, key=lambda k: kernels[k][0])
     dom_ms, dom_ops = kernels[dominant]
     result = {
