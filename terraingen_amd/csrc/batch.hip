@@ -94,6 +94,7 @@ struct tg_batch {
     tg_window* windows = nullptr;
     size_t windowCapacity = 0;
     uint32_t* tileWindows = nullptr;  // (first, count) per tile, maxTiles pairs
+    uint32_t* rowFirsts = nullptr;    // per tile its first bin row in the batch, maxTiles
     uint32_t* binCounts = nullptr;    // per bin, then their exclusive sums
     uint32_t* binFirsts = nullptr;
     void* scanTemp = nullptr;
@@ -146,19 +147,17 @@ __global__ void StaticCounters(uint32_t* counters, uint32_t binCount) {
 }
 
 // ---- Binning on the device ---------------------------------------------------------------------
-// A batch's bins from its tiles' crater windows (tgx_index_windows): one wave per bin walks its
-// tile's windows 64 at a time, tests which meet the bin's cells, and ballots them: the count pass
-// adds up the hits, an exclusive sum over the bins places each bin's list, and the fill pass writes
-// the hits at their places in window order (a lane's place is the hits of the lanes below it), so
-// each bin lists its craters in catalogue order, as the host's binning does.
+// A batch's bins from its tiles' crater windows (tgx_index_windows): a count pass gives each bin
+// its number of craters, an exclusive sum over the bins places each bin's list, and a fill pass
+// writes the lists (BinWindowRows below).
 
-// The tile whose bins hold bin b: the last tile with bin_first <= b (tiles in bin order).
-__device__ __forceinline__ uint32_t TileOfBin(const tg_tile_desc* tiles, uint32_t tileCount,
-                                              uint32_t b) {
+// The tile of global bin row R: the last tile whose first row is at or before it.
+__device__ __forceinline__ uint32_t TileOfRow(const uint32_t* rowFirsts, uint32_t tileCount,
+                                              uint32_t R) {
     uint32_t lo = 0u, hi = tileCount - 1u;
     while (lo < hi) {
         const uint32_t mid = (lo + hi + 1u) >> 1u;
-        if (tiles[mid].bin_first <= b) {
+        if (rowFirsts[mid] <= R) {
             lo = mid;
         } else {
             hi = mid - 1u;
@@ -167,50 +166,78 @@ __device__ __forceinline__ uint32_t TileOfBin(const tg_tile_desc* tiles, uint32_
     return lo;
 }
 
+// Bins of up to this many across are one segment of a row; wider rows take several passes.
+constexpr uint32_t kRowSegment = 512;
+
+// One wave per row of bins of one tile: the tile's windows (its candidates, in catalogue order)
+// are tested 64 at a time against the row's cells, and each window that meets them joins every
+// bin of the row its columns cover, windows one after another in list order, the lanes taking the
+// window's bins. So each bin lists its craters in catalogue order, as tgx_index_bin does, and a
+// tile's windows are read once per row rather than once per bin. kFill false: each bin's count
+// into `counts`; true: the entries at the bin's exclusive sum `firsts` and the bin records.
 template <bool kFill>
 __global__ void __launch_bounds__(256)
-    BinWindows(const tg_tile_desc* __restrict__ tiles, uint32_t tileCount,
-               const uint32_t* __restrict__ tileWindows, const tg_window* __restrict__ windows,
-               uint32_t side, uint32_t binCount, uint32_t* __restrict__ counts,
-               const uint32_t* __restrict__ firsts, tg_bin* __restrict__ bins,
-               uint32_t* __restrict__ entries) {
-    const uint32_t b = blockIdx.x * 4u + (threadIdx.x >> 6u);
+    BinWindowRows(const tg_tile_desc* __restrict__ tiles, uint32_t tileCount,
+                  const uint32_t* __restrict__ rowFirsts, uint32_t rowCount,
+                  const uint32_t* __restrict__ tileWindows, const tg_window* __restrict__ windows,
+                  uint32_t side, uint32_t* __restrict__ counts,
+                  const uint32_t* __restrict__ firsts, tg_bin* __restrict__ bins,
+                  uint32_t* __restrict__ entries) {
+    __shared__ uint32_t cursors[4][kRowSegment];
+    const uint32_t R = blockIdx.x * 4u + (threadIdx.x >> 6u);
     const uint32_t lane = threadIdx.x & 63u;
-    if (b >= binCount) {
+    if (R >= rowCount) {
         return;  // uniform across the wave
     }
-    const uint32_t t = TileOfBin(tiles, tileCount, b);
+    uint32_t* cursor = cursors[threadIdx.x >> 6u];
+    const uint32_t t = TileOfRow(rowFirsts, tileCount, R);
     const tg_tile_desc tile = tiles[t];
-    const uint32_t local = b - tile.bin_first;
-    const uint32_t c0 = (local % tile.bins_across) * side;
-    const uint32_t r0 = (local / tile.bins_across) * side;
-    const uint32_t c1 = min(c0 + side, tile.width) - 1u;
+    const uint32_t row = R - rowFirsts[t];
+    const uint32_t r0 = row * side;
     const uint32_t r1 = min(r0 + side, tile.height) - 1u;
+    const uint32_t across = tile.bins_across;
+    const uint32_t rowBin = tile.bin_first + row * across;
     const uint32_t first = tileWindows[2u * t];
     const uint32_t end = first + tileWindows[2u * t + 1u];
-    uint32_t placed = kFill ? firsts[b] : 0u;
-    const uint32_t start = placed;
-    for (uint32_t base = first; base < end; base += 64u) {
-        const uint32_t e = base + lane;
-        bool hit = false;
-        uint32_t crater = 0u;
-        if (e < end) {
-            const tg_window w = windows[e];
-            crater = w.crater;
-            hit = w.col0 <= c1 && w.col1 >= c0 && w.row0 <= r1 && w.row1 >= r0;
+    for (uint32_t seg0 = 0; seg0 < across; seg0 += kRowSegment) {
+        const uint32_t seg1 = min(seg0 + kRowSegment, across);
+        for (uint32_t j = seg0 + lane; j < seg1; j += 64u) {
+            cursor[j - seg0] = kFill ? firsts[rowBin + j] : 0u;
         }
-        const uint64_t mask = __ballot(hit);
-        if (kFill && hit) {
-            const uint64_t below = lane == 0u ? 0ull : (~0ull >> (64u - lane));
-            entries[placed + uint32_t(__builtin_popcountll(mask & below))] = crater;
+        for (uint32_t base = first; base < end; base += 64u) {
+            const uint32_t e = base + lane;
+            bool hit = false;
+            uint32_t crater = 0u, a = 0u, z = 0u;
+            if (e < end) {
+                const tg_window w = windows[e];
+                crater = w.crater;
+                a = max(w.col0 / side, seg0);
+                z = min(w.col1 / side + 1u, seg1);
+                hit = w.row0 <= r1 && w.row1 >= r0 && a < z;
+            }
+            uint64_t mask = __ballot(hit);
+            while (mask != 0ull) {
+                const int i = int(__builtin_ctzll(mask));
+                mask &= mask - 1ull;
+                const uint32_t n = uint32_t(__builtin_amdgcn_readlane(int(crater), i));
+                const uint32_t from = uint32_t(__builtin_amdgcn_readlane(int(a), i));
+                const uint32_t to = uint32_t(__builtin_amdgcn_readlane(int(z), i));
+                for (uint32_t j = from + lane; j < to; j += 64u) {
+                    const uint32_t at = cursor[j - seg0];
+                    if (kFill) {
+                        entries[at] = n;
+                    }
+                    cursor[j - seg0] = at + 1u;
+                }
+            }
         }
-        placed += uint32_t(__builtin_popcountll(mask));
-    }
-    if (lane == 0u) {
-        if (kFill) {
-            bins[b] = tg_bin{t, start, placed - start, 0u};
-        } else {
-            counts[b] = placed;
+        for (uint32_t j = seg0 + lane; j < seg1; j += 64u) {
+            if (kFill) {
+                const uint32_t at = firsts[rowBin + j];
+                bins[rowBin + j] = tg_bin{t, at, cursor[j - seg0] - at, 0u};
+            } else {
+                counts[rowBin + j] = cursor[j - seg0];
+            }
         }
     }
 }
@@ -1152,6 +1179,7 @@ int tg_hip_batch_free(tg_batch* b) {
     for (void* p : {static_cast<void*>(b->tiles), static_cast<void*>(b->bins),
                     static_cast<void*>(b->entries), static_cast<void*>(b->counters),
                     static_cast<void*>(b->windows), static_cast<void*>(b->tileWindows),
+                    static_cast<void*>(b->rowFirsts),
                     static_cast<void*>(b->binCounts), static_cast<void*>(b->binFirsts),
                     b->scanTemp, static_cast<void*>(b->boulderBins),
                     static_cast<void*>(b->boulderEntries)}) {
@@ -1407,6 +1435,7 @@ int tg_hip_batch_stage_windows(tg_batch* b, const tg_tile_desc* tiles, size_t ti
         }
         if (b->tileWindows == nullptr) {
             b->tileWindows = Alloc<uint32_t>(2 * std::max<size_t>(b->maxTiles, 1));
+            b->rowFirsts = Alloc<uint32_t>(std::max<size_t>(b->maxTiles, 1));
             b->binCounts = Alloc<uint32_t>(std::max<size_t>(b->maxBins, 1));
             b->binFirsts = Alloc<uint32_t>(std::max<size_t>(b->maxBins, 1));
             Check(hipcub::DeviceScan::ExclusiveSum(nullptr, b->scanBytes, b->binCounts,
@@ -1422,13 +1451,22 @@ int tg_hip_batch_stage_windows(tg_batch* b, const tg_tile_desc* tiles, size_t ti
         };
         put(b->tiles, tiles, tileCount * sizeof(tg_tile_desc), "staging tiles");
         put(b->tileWindows, tileWindows, 2 * tileCount * sizeof(uint32_t), "staging windows");
+        // Each tile's first bin row, for the binning's one wave per row.
+        std::vector<uint32_t> rowFirsts(tileCount);
+        uint64_t rowCount = 0;
+        for (size_t t = 0; t < tileCount; ++t) {
+            rowFirsts[t] = uint32_t(rowCount);
+            rowCount += (tiles[t].height + binSide - 1) / binSide;
+        }
+        put(b->rowFirsts, rowFirsts.data(), tileCount * sizeof(uint32_t), "staging rows");
         put(b->windows, windows, windowCount * sizeof(tg_window), "staging windows");
         size_t total = 0;
         if (binCount > 0) {
-            const dim3 grid(uint32_t((binCount + 3) / 4));
-            hipLaunchKernelGGL(BinWindows<false>, grid, dim3(256), 0, b->stream, b->tiles,
-                               uint32_t(tileCount), b->tileWindows, b->windows, binSide,
-                               uint32_t(binCount), b->binCounts, nullptr, nullptr, nullptr);
+            const dim3 grid(uint32_t((rowCount + 3) / 4));
+            hipLaunchKernelGGL(BinWindowRows<false>, grid, dim3(256), 0, b->stream, b->tiles,
+                               uint32_t(tileCount), b->rowFirsts, uint32_t(rowCount),
+                               b->tileWindows, b->windows, binSide, b->binCounts, nullptr,
+                               nullptr, nullptr);
             Check(hipGetLastError(), "launching the bin count");
             size_t bytes = b->scanBytes;
             Check(hipcub::DeviceScan::ExclusiveSum(b->scanTemp, bytes, b->binCounts, b->binFirsts,
@@ -1446,9 +1484,10 @@ int tg_hip_batch_stage_windows(tg_batch* b, const tg_tile_desc* tiles, size_t ti
                                " entries, past the " + std::to_string(b->maxEntries) +
                                " it was created with"};
             }
-            hipLaunchKernelGGL(BinWindows<true>, grid, dim3(256), 0, b->stream, b->tiles,
-                               uint32_t(tileCount), b->tileWindows, b->windows, binSide,
-                               uint32_t(binCount), nullptr, b->binFirsts, b->bins, b->entries);
+            hipLaunchKernelGGL(BinWindowRows<true>, grid, dim3(256), 0, b->stream, b->tiles,
+                               uint32_t(tileCount), b->rowFirsts, uint32_t(rowCount),
+                               b->tileWindows, b->windows, binSide, nullptr, b->binFirsts,
+                               b->bins, b->entries);
             Check(hipGetLastError(), "launching the bin fill");
         }
         hipLaunchKernelGGL(StaticCounters, dim3(1), dim3(64), 0, b->stream, b->counters,

@@ -122,3 +122,29 @@ def test_device_bins_equal_the_host_bins_and_heights(hip, kind):
               f"equal for bin sides 8/16/32 and depths 32/64/128", flush=True)
     finally:
         b.close()
+
+
+@pytest.mark.gpu
+def test_device_bins_of_a_row_wider_than_one_pass(hip):
+    """A tile more than 512 bins across is binned in several passes along each row: its bins and
+    entries are still the host's, and its heights the same at bin sides 8 and 32."""
+    b = TileBatch(hip, [C2_FRACTAL], CraterLayerSpec(seed=4), region=(0.0, 0.0, 300.0, 300.0))
+    tiles = np.zeros(2, N.TILE_DTYPE)
+    tiles[0] = (-2200.0, 160.0, 0.25, 8300, 24)  # 519 bins of 16 across, craters past the 512th
+    tiles[1] = (40.0, 250.0, 0.25, 65, 65)
+    b.build()
+    try:
+        b.stage(tiles)
+        host_bins, host_entries = b.bins.copy(), b.entries.copy()
+        want = heights(hip, b)
+        b.stage(tiles, device_bins=True, bin_side=16)
+        b.read_bins()
+        assert np.array_equal(b.bins, host_bins)
+        assert np.array_equal(b.entries, host_entries)
+        assert host_entries.size > 100
+        for side in (8, 32):
+            b.stage(tiles, device_bins=True, bin_side=side)
+            got = heights(hip, b)
+            assert np.array_equal(got.view(np.uint32), want.view(np.uint32)), side
+    finally:
+        b.close()
