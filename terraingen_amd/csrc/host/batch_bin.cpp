@@ -116,7 +116,20 @@ void PlaneWindows(const tgx_index& index, const tgx_tile& tile, std::size_t rowF
     const double minX = tile.origin_x, maxX = tile.origin_x + tile.width * tile.pixel;
     const double maxY = tile.origin_y - double(rowFirst) * tile.pixel;
     const double minY = tile.origin_y - double(std::min<std::size_t>(rowEnd, tile.height)) * tile.pixel;
-    candidates.clear();
+    // The candidates are marked in a bitmap over the catalogue, which then lists them in
+    // catalogue order in one pass over the marked span, rather than sorting them.
+    thread_local std::vector<std::uint64_t> marks;
+    if (marks.size() < (index.cx.size() + 63) / 64) {
+        marks.assign((index.cx.size() + 63) / 64, 0ull);
+    }
+    std::size_t lowWord = marks.size(), highWord = 0;
+    const auto mark = [&](const std::vector<std::uint32_t>& list) {
+        for (const std::uint32_t k : list) {
+            marks[k >> 6u] |= 1ull << (k & 63u);
+            lowWord = std::min<std::size_t>(lowWord, k >> 6u);
+            highWord = std::max<std::size_t>(highWord, k >> 6u);
+        }
+    };
     for (const IndexLevel& level : index.levels) {
         if (level.cell == 0.0 || !Resolved(level.maxDiameter, tile.pixel)) {
             continue;
@@ -128,7 +141,7 @@ void PlaneWindows(const tgx_index& index, const tgx_tile& tile, std::size_t rowF
         const auto j1 = std::int64_t(std::floor((maxY + grow) / level.cell));
         if ((i1 - i0 + 1) * (j1 - j0 + 1) > std::int64_t(level.cells.size())) {
             for (const auto& [key, list] : level.cells) {
-                candidates.insert(candidates.end(), list.begin(), list.end());
+                mark(list);
             }
             continue;
         }
@@ -136,13 +149,20 @@ void PlaneWindows(const tgx_index& index, const tgx_tile& tile, std::size_t rowF
             for (std::int64_t i = i0; i <= i1; ++i) {
                 const auto it = level.cells.find(CellKey(i, j));
                 if (it != level.cells.end()) {
-                    candidates.insert(candidates.end(), it->second.begin(), it->second.end());
+                    mark(it->second);
                 }
             }
         }
     }
-    std::sort(candidates.begin(), candidates.end());
-    candidates.erase(std::unique(candidates.begin(), candidates.end()), candidates.end());
+    candidates.clear();
+    for (std::size_t w = lowWord; w <= highWord && w < marks.size(); ++w) {
+        std::uint64_t bits = marks[w];
+        marks[w] = 0ull;  // left clear for the next tile
+        while (bits != 0ull) {
+            candidates.push_back(std::uint32_t(w * 64u + std::uint32_t(__builtin_ctzll(bits))));
+            bits &= bits - 1ull;
+        }
+    }
     windows.clear();
     for (const std::uint32_t k : candidates) {
         if (!Resolved(index.diameter[k], tile.pixel)) {
