@@ -124,6 +124,13 @@ constexpr uint32_t kGroupThreads = 256;  // four independent waves
 #endif
 constexpr int kPreciseWaves = TG_PRECISE_WAVES;
 
+// Waves per SIMD the full-evaluation (general) variants are built for, the fastest on the C3 grid
+// as one tile: left to the compiler (130 VGPRs, 3 waves) 3.52 ms, 4 waves (126 VGPRs, no
+// scratch) 2.96 ms, 5 waves 3.03 ms.
+#ifndef TG_GENERAL_WAVES
+#define TG_GENERAL_WAVES 4
+#endif
+
 // The bins are split into kQueues contiguous ranges, one per XCD: workgroups are dispatched to the
 // XCDs in turn, so workgroup g starts on range g % kQueues, and neighbouring bins (which list the
 // same craters) are shaped by waves sharing one L2. Each range has its own counter in its own
@@ -510,7 +517,7 @@ __device__ __forceinline__ void CulledWalk(uint32_t first, uint32_t end, uint32_
 // variant without the FP64 code, whose registers cost the other variants residency).
 // kBoulders: a boulder layer follows the craters (its code only in the variants that draw it).
 template <uint32_t kWalk, bool kAllFast, bool kPrecise, bool kBoulders = false>
-__global__ void __launch_bounds__(kGroupThreads, kAllFast ? (kPrecise ? kPreciseWaves : 8) : 1)
+__global__ void __launch_bounds__(kGroupThreads, kAllFast ? (kPrecise ? kPreciseWaves : 8) : TG_GENERAL_WAVES)
     BatchTiles(const tg_tile_desc* __restrict__ tiles, const tg_bin* __restrict__ bins,
                const uint32_t* __restrict__ entries, uint32_t* __restrict__ counters,
                const tg::NoisePlan* __restrict__ plan, float elevation,
@@ -717,7 +724,7 @@ __device__ __forceinline__ void VisitBodyCrater(uint32_t n, bool inside, tg::Pre
 // general crater walk and 64-bit lattice indices, carries their code).
 // kPrecise, kBoulders: as for BatchTiles.
 template <bool kAllFast, bool kSmall, bool kGeneral, bool kPrecise, bool kBoulders = false>
-__global__ void __launch_bounds__(kGroupThreads, kAllFast ? (kPrecise ? kPreciseWaves : 8) : 1)
+__global__ void __launch_bounds__(kGroupThreads, kAllFast ? (kPrecise ? kPreciseWaves : 8) : TG_GENERAL_WAVES)
     BatchBodyTiles(const tg_tile_desc* __restrict__ tiles, const tg_bin* __restrict__ bins,
                    const uint32_t* __restrict__ entries, uint32_t* __restrict__ counters,
                    const tg::NoisePlan* __restrict__ plan, float elevation, double radius,
