@@ -56,3 +56,33 @@ def test_c1_quick_start_matches_reference_terrain(hip):
           f"{got.size} cells, {terrain.crater_count} craters", flush=True)
     # FP32 rounding of the fast outline only (9.5e-7 m measured).
     assert d.max() <= 1e-5
+
+
+def test_c1_quick_start_with_the_readme_seed(hip, ref):
+    """The quick-start terrain with the seed the reference's README and options default to
+    (CraterLayerOptions::seed = 1), against the reference's own CPU pipeline run here (oracle/_ref):
+    the same crater list and records byte for byte, the heights within the same bound."""
+    import sys
+
+    sys.path.insert(0, str(GOLDEN))
+    from make_crater_fixtures import CASES, run_case
+
+    z = run_case("c1_seed1", dict(CASES["c1_default"], seed=1))
+    grid = N.north_up_grid(257, 257, 1.0, (0.0, 0.0))
+    spec = CraterLayerSpec(model=N.MODELS_DEFAULT, seed=1, min_diameter=2.5, max_diameter=250.0,
+                           irregular=1, peaks=1, terraces=1)
+    terrain = PlaneTerrain(hip, grid, [], 0.0, spec)
+    terrain.prepare()
+    try:
+        for col, key in [("x", "x"), ("y", "y"), ("diameter", "d"), ("age_gyr", "age"),
+                         ("id", "id")]:
+            assert np.array_equal(terrain.crater_list[col], z[key]), col
+        assert terrain.records.view(np.uint8).tobytes() == z["records"].tobytes()
+        got = terrain.evaluate()
+    finally:
+        terrain.close()
+    d = np.abs(got - z["heights"])
+    print(f"[c1 seed 1] GPU terrain vs reference CPU terrain: max {d.max():.3g} m over "
+          f"{got.size} cells, {terrain.crater_count} craters", flush=True)
+    assert terrain.crater_count > 100
+    assert d.max() <= 1e-5
