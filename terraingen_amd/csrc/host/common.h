@@ -70,17 +70,14 @@ inline std::size_t ClampTo(double value, std::size_t limit) {
 // coordinates, floored and ceiled, clamped to the grid.
 inline PixelWindow PixelWindowIn(const Grid& g, double minX, double minY, double maxX,
                                  double maxY) {
-    double lo[2] = {INFINITY, INFINITY}, hi[2] = {-INFINITY, -INFINITY};
-    for (const double x : {minX, maxX}) {
-        for (const double y : {minY, maxY}) {
-            double c, r;
-            g.WorldToPixel(x, y, c, r);
-            lo[0] = std::fmin(lo[0], c);
-            lo[1] = std::fmin(lo[1], r);
-            hi[0] = std::fmax(hi[0], c);
-            hi[1] = std::fmax(hi[1], r);
-        }
-    }
+    // On a north-up grid a corner's column depends on its x alone and its row on its y alone (the
+    // rotation terms are products with zero, which change at most the sign of a zero), so two
+    // opposite corners give the four corners' columns and rows.
+    double c0, r0, c1, r1;
+    g.WorldToPixel(minX, minY, c0, r0);
+    g.WorldToPixel(maxX, maxY, c1, r1);
+    const double lo[2] = {std::fmin(c0, c1), std::fmin(r0, r1)};
+    const double hi[2] = {std::fmax(c0, c1), std::fmax(r0, r1)};
     return {ClampTo(std::floor(lo[0]), g.width), ClampTo(std::ceil(hi[0]), g.width),
             ClampTo(std::floor(lo[1]), g.height), ClampTo(std::ceil(hi[1]), g.height)};
 }
