@@ -7,7 +7,7 @@
 // copies queued by tg_hip_batch_stage_async read the caller's pinned arrays later, on another
 // thread, and events report completion only once the worker has passed them. A run writes a
 // height per cell that names the tile, the cell and the number of craters and boulders in its
-// bin (standin_height), which the driver recomputes to check each request got its own tile.
+// bin (standin_height), which server_driver.cpp recomputes to check each request got its own tile.
 
 #include "tg_hip.h"
 
@@ -108,7 +108,7 @@ struct tg_batch {
 extern "C" {
 
 // The height the stand-in's run writes at (col, row) of tile `t` whose bin lists `craters` and
-// `boulders` entries: the driver computes the same FP32 expression.
+// `boulders` entries: server_driver.cpp computes the same FP32 expression.
 float standin_height(const tg_tile_desc* t, uint32_t col, uint32_t row, uint32_t craters,
                      uint32_t boulders, float elevation) {
     return elevation + float(t->origin_x) * 0.001f + float(t->origin_y) * 0.002f + float(col) +
