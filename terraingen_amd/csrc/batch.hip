@@ -116,9 +116,9 @@ constexpr uint32_t kBinSide = 16;
 constexpr uint32_t kGroupThreads = 256;  // four independent waves
 
 // Waves per SIMD the all-fast variants are built for: 8 without the FP64 path, and with it the
-// fastest of 6-8 measured on the C4 sphere batch at one launch configuration (6: 49.07 ms, 80
-// VGPRs and 20 B of scratch a lane; 7: 50.54 ms, 72 and 60 B; 8: 50.76 ms, 64 and 88 B): fewer
-// waves spill less of the FP64 terms but hide less latency.
+// fastest of 5-8 measured on the C4 sphere batch at one launch configuration (5: 50.01 ms, 96
+// VGPRs; 6: 49.06 ms, 80 VGPRs and 20 B of scratch a lane; 7: 50.54 ms, 72 and 60 B; 8: 50.76 ms,
+// 64 and 88 B): fewer waves spill less of the FP64 terms but hide less latency.
 #ifndef TG_PRECISE_WAVES
 #define TG_PRECISE_WAVES 6
 #endif
