@@ -1335,9 +1335,10 @@ __device__ __forceinline__ Offset64 BodyOffset64(const PreciseTerms& t, const tg
         ratio = scale * u * series(u) / s;
     }
     Offset64 o{chord * ratio, 0.0, 1.0};
-    const double across = Sqrt64(ox * ox + oy * oy);
-    if (across > 0.0) {
-        const double inv = Reciprocal64(across);
+    const double across2 = ox * ox + oy * oy;
+    if (across2 > 0.0) {
+        // The direction cosines take 1 / |(ox, oy)| alone, so its inverse square root serves.
+        const double inv = InvSqrt64(across2);
         o.ct = ox * inv;
         o.st = oy * inv;
     }
