@@ -38,9 +38,10 @@ the square.
   p50/p99 request-to-host latency (submission to heights in the caller's memory) for 513 x 513
   tiles and for all tiles, and the served Gcells/s.
 
-``--workload grid`` is one 4097 x 4097 grid at 0.5 m, with the same noise and craters (seed 1),
-evaluated by the strip kernels. ``--workload c3`` is one 8193 x 8193 grid at 0.25 m with complex
-craters and boulders. ``--workload sources`` is the 4097 x 4097 grid at 0.5 m filled by a
+``--workload grid`` is one 4097 x 4097 grid at 0.5 m, with the same noise and craters (seed 1):
+``value`` is the grid as one tile of the batched kernel, and the strip kernels' passes (the plain
+path) are timed beside it and checked bit for bit against it. ``--workload c3`` is one 8193 x 8193
+grid at 0.25 m with complex craters and boulders, measured the same way. ``--workload sources`` is the 4097 x 4097 grid at 0.5 m filled by a
 SyntheticSource whose noise lies on posts 2 m apart, upsampled by the kernel upsamplers.
 
 Prints one JSON line.
